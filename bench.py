@@ -1,0 +1,254 @@
+#!/usr/bin/env python3
+"""Cone-visibility throughput on one MI355X.
+
+A step is one plain per-view launch over the 1M-triangle rock field: the
+camera pass (1280 x 720 shading points) and one cone query per shading point
+and light (3 lights: a 0.53 degree sun disk and two area lights), on inputs
+already resident in HBM. Prints one JSON line: queries/s over the timed
+steps, p50/p99 view latency, the cone kernel's roofline, the oracle's
+throughput on the host's cores for a bounded sample of the same queries, and
+a bit-for-bit comparison of that sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "cone-visibility queries/s; p50/p99 view latency, batched/served vs plain"
+PEAK_HBM_GBPS = 8000.0
+PROFILE_DIR = ROOT / "profiles"
+
+
+def algorithmic_bytes(work, queries: int, receivers: int, lights: int) -> dict:
+    """Bytes a launch must move, per unit of work the kernel's counters
+    report: 32 B per receiver read, 64 B per light read, 20 B of results
+    (irradiance f32 + two u64 masks) per query; 32 B per node tested (object
+    and triangle hierarchies, compact nodes), 36 B per triangle tested, 16 B
+    range + 4 B item per mesh entered, 108 B of coverage samples per splat."""
+    obj_nodes, blas_nodes, tris_tested, _drawn, meshes, splats = (int(x) for x in work)
+    per = {
+        "receivers": 32 * queries,
+        "lights": 64 * queries,
+        "results": 20 * queries,
+        "object_nodes": 32 * obj_nodes,
+        "blas_nodes": 32 * blas_nodes,
+        "triangles": 36 * tris_tested,
+        "meshes": 20 * meshes,
+        "splats": 108 * splats,
+    }
+    per["total"] = sum(per.values())
+    return per
+
+
+def read_traffic(workload: str):
+    """HBM bytes per cone launch measured by rocprofv3 --pmc (FETCH_SIZE and
+    WRITE_SIZE, calibrated on a copy of known size), from the committed
+    summary, if one exists for this workload."""
+    f = PROFILE_DIR / f"cone_query_{workload}_traffic.md"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--width", type=int, default=1280)
+    ap.add_argument("--height", type=int, default=720)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target length of the CPU baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print(f"bench.py measures one GPU: this path runs on a single MI355X and has no multi-GPU form "
+              f"(asked for {max(args.gpus, world)} GPUs).", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    from continuum_amd import _lib
+    from continuum_amd.kernel import Device, Flat, Scene
+    from continuum_amd.scenes import meshes_to_arrays, rock_field
+
+    t0 = time.perf_counter()
+    sc = rock_field(width=args.width, height=args.height)
+    tris9, counts = meshes_to_arrays(sc.meshes)
+    flat = Flat.build_arrays(tris9, counts)
+    t_build = time.perf_counter() - t0
+    dev = torch.device("cuda:0")
+    device = Device(0)
+    scene = Scene(device, flat)
+    opts = _lib.default_options(hierarchy=2, prefilter=sc.prefilter)
+
+    W, H = sc.width, sc.height
+    px = W * H
+    nl = len(sc.lights)
+    Q = px * nl
+    cam = torch.from_numpy(sc.camera).to(dev)
+    lights = torch.from_numpy(np.ascontiguousarray(sc.lights)).to(dev)
+    hits = torch.empty((px, 4), dtype=torch.int32, device=dev)
+    recv = torch.empty((px, 8), dtype=torch.float32, device=dev)
+    vis = torch.empty(Q, dtype=torch.float32, device=dev)
+    full = torch.empty(Q, dtype=torch.int64, device=dev)
+    touched = torch.empty(Q, dtype=torch.int64, device=dev)
+
+    # A copy of known size (256 MiB read, 256 MiB written), for calibrating
+    # HBM byte counters when this command runs under rocprofv3 --pmc.
+    ca = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+    cb = torch.empty_like(ca)
+    torch.neg(ca, out=cb)
+    del ca, cb
+
+    stream = torch.cuda.current_stream()
+
+    def step(e0=None, e1=None, e2=None):
+        if e0 is not None:
+            e0.record(stream)
+        scene.primary_device(cam, W, H, lights, opts, hits, recv, stream)
+        if e1 is not None:
+            e1.record(stream)
+        scene.query_device(recv, lights, opts, vis, full, touched, None, stream)
+        if e2 is not None:
+            e2.record(stream)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    scene.counters(reset=True)
+
+    K = args.steps
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(K)]
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for k in range(K):
+        step(*ev[k])
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+
+    lat = np.array([ev[k][0].elapsed_time(ev[k][2]) for k in range(K)])
+    cone_ms = np.array([ev[k][1].elapsed_time(ev[k][2]) for k in range(K)])
+    prim_ms = np.array([ev[k][0].elapsed_time(ev[k][1]) for k in range(K)])
+    counters, _ = scene.counters(reset=True)
+
+    # One more step with the work counters on (untimed), for the roofline's bytes.
+    scene.set_count_work(True)
+    step()
+    torch.cuda.synchronize()
+    _, work = scene.counters(reset=True)
+    scene.set_count_work(False)
+
+    h_recv = recv.cpu().numpy()
+    lit = int((h_recv[:, 7] >= 0).sum())
+    traced_q = lit * nl
+    algo = algorithmic_bytes(work, traced_q, lit, nl)
+    cone_avg_s = float(cone_ms.mean()) / 1e3
+    achieved = algo["total"] / cone_avg_s / 1e9
+    traffic = read_traffic(sc.name)
+
+    result = {
+        "metric": METRIC,
+        "value": K * Q / wall,
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": wall / K * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded rock field; geometry generated, no assets)",
+        "config": {
+            "workload": "rock field, plain per-view launch (camera pass + one cone per pixel and light)",
+            "triangles": int(sc.n_tris),
+            "objects": len(sc.meshes),
+            "shading_points": px,
+            "lights": nl,
+            "queries_per_step": Q,
+            "traced_queries_per_step": traced_q,
+            "hierarchy": "triangles",
+            "prefilter_bins": sc.prefilter,
+            "scene_hbm_bytes": scene.hbm_bytes,
+            "host_build_s": round(t_build, 3),
+        },
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                       "camera_pass_mean": float(prim_ms.mean()), "cone_pass_mean": float(cone_ms.mean())},
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "cone_query_kernel",
+            "achieved": achieved,
+            "peak": PEAK_HBM_GBPS,
+            "unit": "GB/s",
+            "frac": achieved / PEAK_HBM_GBPS,
+            "traffic": traffic,
+            "algorithmic_bytes_per_launch": algo["total"],
+            "kernel_ms": float(cone_ms.mean()),
+            "work": {"object_nodes": int(work[0]), "blas_nodes": int(work[1]), "triangles_tested": int(work[2]),
+                     "triangles_drawn": int(work[3]), "meshes_entered": int(work[4]), "splats": int(work[5])},
+        },
+        "diagnostics": {"capped_walks": int(counters[0]), "deeper_than_48": int(counters[1]),
+                        "truncated": int(counters[2])},
+    }
+
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+
+        threads = O.threads_default()
+        o_opts = O.opts_array(hierarchy=2, prefilter=sc.prefilter)
+        h_vis = vis.cpu().numpy().reshape(nl, px)
+        h_full = full.cpu().numpy().view(np.uint64).reshape(nl, px)
+        h_touched = touched.cpu().numpy().view(np.uint64).reshape(nl, px)
+        # calibrate on 1/1024 of the pixels, then a sample of about cpu_seconds
+        probe = h_recv[::1024]
+        t = time.perf_counter()
+        O.irradiance(flat, probe, sc.lights, o_opts, threads=threads)
+        rate = len(probe) * nl / max(time.perf_counter() - t, 1e-6)
+        stride = max(1, int(np.ceil(Q / max(rate * args.cpu_seconds, 1.0))))
+        sample = h_recv[::stride]
+        t = time.perf_counter()
+        rv, rf, rt, _, capped, _ = O.irradiance(flat, sample, sc.lights, o_opts, threads=threads)
+        t_cpu = time.perf_counter() - t
+        sq = len(sample) * nl
+        gv = h_vis[:, ::stride].reshape(-1)
+        gf = h_full[:, ::stride].reshape(-1)
+        gt = h_touched[:, ::stride].reshape(-1)
+        ok = capped == 0
+        result["cpu_baseline"] = {
+            "value": sq / t_cpu,
+            "unit": "queries/s",
+            "cores": threads,
+            "kind": "port",
+            "sample": f"every {stride}th shading point of the {W}x{H} view x {nl} lights = {sq} queries "
+                      f"({t_cpu:.1f} s on {threads} threads; cones only, receivers from the GPU camera pass)",
+        }
+        # camera pass: every pixel against the oracle
+        rh, _, _ = O.primary(flat, sc.camera, W, H, sc.lights, threads=threads)
+        result["parity"] = {
+            "sample_queries": sq,
+            "visible_bits_differ": int((gv.view(np.uint32)[ok] != rv.view(np.uint32)[ok]).sum()),
+            "full_masks_differ": int((gf[ok] != rf[ok]).sum()),
+            "touched_masks_differ": int((gt[ok] != rt[ok]).sum()),
+            "capped_skipped": int((~ok).sum()),
+            "camera_hits_differ": int((hits.cpu().numpy().view(np.uint32) != rh).any(axis=1).sum()),
+        }
+
+    print(json.dumps(result))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,105 @@
+"""ctypes binding of libcnq.so (the C ABI in include/cnq.h).
+
+The library is loaded from this package's directory; there is no fallback:
+without it every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libcnq.so"
+NONE = 0xFFFFFFFF
+
+OK, NO_DEVICE, INVALID_ARG, CAPACITY, OOM, HIP_ERROR = range(6)
+
+
+class CnqError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"cnq error {code}: {msg}")
+        self.code = code
+
+
+class Options(C.Structure):
+    _fields_ = [
+        ("hierarchy", C.c_uint32),
+        ("prefilter", C.c_float),
+        ("wide_extent", C.c_float),
+        ("light_cone_cap", C.c_uint32),
+        ("wide_cone_cap", C.c_uint32),
+        ("profile_stage", C.c_uint32),
+        ("object_cov", C.c_uint32),
+        ("nearest_first", C.c_uint32),
+    ]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [
+        ("tris", C.c_void_p), ("n_tris", C.c_uint64),
+        ("ranges", C.c_void_p), ("n_objects", C.c_uint32),
+        ("blas", C.c_void_p), ("n_blas", C.c_uint32),
+        ("blas_items", C.c_void_p), ("n_blas_items", C.c_uint64),
+        ("blas_cov", C.c_void_p),
+        ("nodes", C.c_void_p), ("n_nodes", C.c_uint32),
+        ("items", C.c_void_p), ("n_items", C.c_uint32),
+        ("xforms", C.c_void_p), ("n_xforms", C.c_uint32),
+    ]
+
+
+# Every symbol include/cnq.h declares, with its ctypes signature.
+_P = C.c_void_p
+SIGNATURES = {
+    "cnq_options_default": (None, [C.POINTER(Options)]),
+    "cnq_last_error": (C.c_char_p, []),
+    "cnq_device_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
+    "cnq_device_destroy": (None, [_P]),
+    "cnq_flat_build": (C.c_int, [_P, _P, C.c_uint32, C.POINTER(_P)]),
+    "cnq_flat_sizes": (C.c_int, [_P, _P]),
+    "cnq_flat_copy": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P]),
+    "cnq_flat_destroy": (None, [_P]),
+    "cnq_scene_create": (C.c_int, [_P, C.POINTER(SceneDesc), C.POINTER(_P)]),
+    "cnq_scene_destroy": (None, [_P]),
+    "cnq_scene_bytes": (C.c_uint64, [_P]),
+    "cnq_irradiance": (C.c_int, [_P, _P, C.c_uint64, _P, C.c_uint32, C.POINTER(Options), _P, _P, _P, _P]),
+    "cnq_irradiance_device": (
+        C.c_int, [_P, _P, C.c_uint64, _P, C.c_uint32, C.POINTER(Options), _P, _P, _P, _P, _P]),
+    "cnq_primary_device": (
+        C.c_int, [_P, _P, C.c_uint32, C.c_uint32, _P, C.c_uint32, C.POINTER(Options), _P, _P, _P]),
+    "cnq_render_view": (
+        C.c_int, [_P, _P, C.c_uint32, C.c_uint32, _P, C.c_uint32, C.POINTER(Options), _P, _P, _P, _P]),
+    "cnq_scene_refit_device": (C.c_int, [_P, _P, _P]),
+    "cnq_scene_refit": (C.c_int, [_P, _P]),
+    "cnq_scene_read_blas": (C.c_int, [_P, _P, _P]),
+    "cnq_scene_counters": (C.c_int, [_P, C.c_int, _P, _P]),
+    "cnq_scene_set_count_work": (C.c_int, [_P, C.c_int]),
+}
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """The loaded library (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m continuum_amd.build`")
+        L = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != OK:
+        raise CnqError(rc, lib().cnq_last_error().decode(errors="replace"))
+
+
+def default_options(**kw) -> Options:
+    o = Options()
+    lib().cnq_options_default(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o

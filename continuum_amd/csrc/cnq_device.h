@@ -1,0 +1,396 @@
+// Device-side geometry of a mask cone for gfx950: vector math, the cone's
+// frame and culling tests, and the exact-area accumulation rasterizer.
+//
+// Semantics follow the reference's GPU kernel,
+// crates/continuum-gpu/src/mask_cones.wgsl (cited as "WGSL :line"), with the
+// WGSL specification's definitions for normalize (e / length(e)), mix,
+// clamp and cross, dot products summed left to right, and no fused
+// multiply-adds (the library is built with -ffp-contract=off). Division and
+// square root are IEEE correctly rounded (hipcc's default for f32).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cnq {
+
+constexpr uint32_t NONE = 0xffffffffu;
+constexpr uint32_t GRID = 8;
+constexpr uint32_t BINS = 64;
+constexpr float NEAR = 1e-4f;  // WGSL :211
+// WGSL's abstract-precision constant 1.0 - 1e-5, rounded once to f32.
+constexpr float ONE_MINUS_1EM5 = (float)(1.0 - 1e-5);
+// The reference's traversal stack (WGSL :111); a walk deeper than this is
+// counted as an overflow.
+constexpr uint32_t REF_STACK = 48;
+
+struct f3 {
+  float x, y, z;
+};
+struct f2 {
+  float x, y;
+};
+
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ f3 operator-(f3 a) { return f3{-a.x, -a.y, -a.z}; }
+__device__ __forceinline__ f3 operator*(f3 a, float s) { return f3{a.x * s, a.y * s, a.z * s}; }
+__device__ __forceinline__ f3 operator/(f3 a, float s) { return f3{a.x / s, a.y / s, a.z / s}; }
+__device__ __forceinline__ f2 operator+(f2 a, f2 b) { return f2{a.x + b.x, a.y + b.y}; }
+__device__ __forceinline__ f2 operator-(f2 a, f2 b) { return f2{a.x - b.x, a.y - b.y}; }
+__device__ __forceinline__ f2 operator*(f2 a, float s) { return f2{a.x * s, a.y * s}; }
+
+__device__ __forceinline__ float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float dot2(f2 a, f2 b) { return a.x * b.x + a.y * b.y; }
+__device__ __forceinline__ f3 cross3(f3 a, f3 b) {
+  return f3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__device__ __forceinline__ float len3(f3 a) { return sqrtf(dot3(a, a)); }
+__device__ __forceinline__ f3 norm3(f3 a) { return a / len3(a); }
+__device__ __forceinline__ float mn(float a, float b) { return b < a ? b : a; }
+__device__ __forceinline__ float mx(float a, float b) { return b > a ? b : a; }
+__device__ __forceinline__ float clampf(float e, float lo, float hi) { return mn(mx(e, lo), hi); }
+__device__ __forceinline__ float mixf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+__device__ __forceinline__ f3 ld3(const float* p) { return f3{p[0], p[1], p[2]}; }
+
+// Arctangent after S. L. Moshier's Cephes atanf: reduction by tan(3pi/8) and
+// tan(pi/8), then a degree-9 odd minimax polynomial (relative error below
+// 2e-7); atan2 by Cephes atan2f's quadrant rule.
+constexpr float kPi = 3.14159265358979323846f;
+constexpr float kPio2 = 1.57079632679489661923f;
+constexpr float kPio4 = 0.78539816339744830962f;
+
+__device__ __forceinline__ float atan_c(float xin) {
+  float x = xin;
+  float sgn = 1.0f;
+  if (x < 0.0f) {
+    sgn = -1.0f;
+    x = -x;
+  }
+  float y;
+  if (x > 2.414213562373095f) {
+    y = kPio2;
+    x = -1.0f / x;
+  } else if (x > 0.4142135623730950f) {
+    y = kPio4;
+    x = (x - 1.0f) / (x + 1.0f);
+  } else {
+    y = 0.0f;
+  }
+  float z = x * x;
+  y = y + ((((8.05374449538e-2f * z - 1.38776856032e-1f) * z + 1.99777106478e-1f) * z - 3.33329491539e-1f) * z *
+               x +
+           x);
+  return sgn * y;
+}
+
+__device__ __forceinline__ float atan2_c(float y, float x) {
+  if (x == 0.0f) {
+    if (y > 0.0f) return kPio2;
+    if (y < 0.0f) return -kPio2;
+    return 0.0f;
+  }
+  if (y == 0.0f) return x > 0.0f ? 0.0f : kPi;
+  float w = 0.0f;
+  if (x < 0.0f) w = y < 0.0f ? -kPi : kPi;
+  return w + atan_c(y / x);
+}
+
+__device__ __forceinline__ float acos_c(float y) { return atan2_c(sqrtf((1.0f - y) * (1.0f + y)), y); }
+
+// Exponential after Cephes expf: x = n ln2 + r (ln2 in two parts), a
+// degree-6 polynomial for e^r, then scaling by 2^n.
+__device__ __forceinline__ float exp_c(float x) {
+  if (x > 88.72283905206835f) return __builtin_inff();
+  if (x < -103.278929903431851103f) return 0.0f;
+  float z = floorf(1.44269504088896341f * x + 0.5f);
+  x = x - z * 0.693359375f;
+  x = x - z * -2.12194440e-4f;
+  int n = (int)z;
+  float xx = x * x;
+  float p = ((((((1.9875691500e-4f * x + 1.3981999507e-3f) * x + 8.3334519073e-3f) * x + 4.1665795894e-2f) * x +
+               1.6666665459e-1f) *
+                  x +
+              5.0000001201e-1f) *
+                 xx +
+             x) +
+            1.0f;
+  return ldexpf(p, n);
+}
+
+// A light cone (WGSL :125-146). Its far plane's normal is always its
+// forward axis and a light cone has no horizon, so neither is stored.
+struct Cone {
+  f3 apex, right, up, forward;
+  float w0, w1, w2, w3;  // window u0, v0, u1, v1
+  float far_offset;
+  f3 baxis;  // bounding circular cone: unit axis, cos and sin of the half-angle
+  float bcos, bsin;
+};
+
+__device__ __forceinline__ f3 direction(const Cone& c, float s, float t) {
+  return c.forward + c.right * (c.w0 + s * (c.w2 - c.w0)) + c.up * (c.w1 + t * (c.w3 - c.w1));
+}
+
+__device__ __forceinline__ f3 to_local(const Cone& c, f3 p) {
+  f3 d = p - c.apex;
+  return f3{dot3(c.right, d), dot3(c.up, d), dot3(c.forward, d)};
+}
+
+// WGSL :159-168
+__device__ __forceinline__ bool in_bound(const Cone& c, f3 p, float r) {
+  if (c.bcos < -1.0f) return true;
+  f3 d = p - c.apex;
+  float len = len3(d);
+  if (len <= r) return true;
+  float s = r / len;
+  float limit = c.bcos * sqrtf(1.0f - s * s) - c.bsin * s;
+  return dot3(c.baxis, d) >= limit * len;
+}
+
+// WGSL :174-186
+__device__ __forceinline__ void bound_cone(Cone& c, f3 axis) {
+  float cos_half = 1.0f;
+#pragma unroll
+  for (uint32_t k = 0; k < 4; k++) {
+    f3 d = norm3(direction(c, (float)(k & 1u), (float)(k >> 1u)));
+    cos_half = mn(cos_half, dot3(axis, d));
+  }
+  float cos_b = clampf(cos_half - 1e-5f, -1.0f, 1.0f);
+  c.baxis = axis;
+  c.bcos = cos_b;
+  c.bsin = sqrtf(mx(1.0f - cos_b * cos_b, 0.0f));
+}
+
+// WGSL :193-201
+__device__ __forceinline__ float lambert_edge_unit(f3 n, f3 an, f3 bn) {
+  f3 g = cross3(an, bn);
+  float len = len3(g);
+  if (len > 1e-12f) return atan2_c(len, dot3(an, bn)) * dot3(n, g / len);
+  return 0.0f;
+}
+__device__ __forceinline__ float lambert_edge(f3 n, f3 a, f3 b) { return lambert_edge_unit(n, norm3(a), norm3(b)); }
+
+__device__ __forceinline__ f3 box_corner(f3 lo, f3 hi, uint32_t k) {
+  return f3{(k & 1u) ? hi.x : lo.x, (k & 2u) ? hi.y : lo.y, (k & 4u) ? hi.z : lo.z};
+}
+
+// WGSL :234-255 (no horizon: light cones have none)
+__device__ __forceinline__ bool may_touch_box(const Cone& c, f3 lo, f3 hi) {
+  bool far_out = true, near_out = true, out_u0 = true, out_u1 = true, out_v0 = true, out_v1 = true;
+#pragma unroll
+  for (uint32_t k = 0; k < 8; k++) {
+    f3 corner = box_corner(lo, hi, k);
+    if (c.far_offset - dot3(c.forward, corner) >= 0.0f) far_out = false;
+    f3 p = to_local(c, corner);
+    if (p.z >= NEAR) near_out = false;
+    if (p.x - c.w0 * p.z >= 0.0f) out_u0 = false;
+    if (c.w2 * p.z - p.x >= 0.0f) out_u1 = false;
+    if (p.y - c.w1 * p.z >= 0.0f) out_v0 = false;
+    if (c.w3 * p.z - p.y >= 0.0f) out_v1 = false;
+  }
+  if (far_out || near_out || out_u0 || out_u1 || out_v0 || out_v1) return false;
+  return in_bound(c, (lo + hi) * 0.5f, 0.5f * len3(hi - lo));
+}
+
+// WGSL :213-232
+__device__ __forceinline__ bool may_touch(const Cone& c, f3 v0, f3 v1, f3 v2) {
+  float f0 = c.far_offset - dot3(c.forward, v0);
+  float f1 = c.far_offset - dot3(c.forward, v1);
+  float f2 = c.far_offset - dot3(c.forward, v2);
+  if (f0 < 0.0f && f1 < 0.0f && f2 < 0.0f) return false;
+  f3 p0 = to_local(c, v0), p1 = to_local(c, v1), p2 = to_local(c, v2);
+  if (p0.z < NEAR && p1.z < NEAR && p2.z < NEAR) return false;
+  if (p0.x - c.w0 * p0.z < 0.0f && p1.x - c.w0 * p1.z < 0.0f && p2.x - c.w0 * p2.z < 0.0f) return false;
+  if (c.w2 * p0.z - p0.x < 0.0f && c.w2 * p1.z - p1.x < 0.0f && c.w2 * p2.z - p2.x < 0.0f) return false;
+  if (p0.y - c.w1 * p0.z < 0.0f && p1.y - c.w1 * p1.z < 0.0f && p2.y - c.w1 * p2.z < 0.0f) return false;
+  if (c.w3 * p0.z - p0.y < 0.0f && c.w3 * p1.z - p1.y < 0.0f && c.w3 * p2.z - p2.y < 0.0f) return false;
+  return true;
+}
+
+// ---- The accumulation buffer: 68 cells per query in LDS, lane-interleaved
+// (cell i of lane l at i * 64 + l), so every access is bank-conflict free.
+struct AccRef {
+  float* base;  // this lane's cell 0
+  uint32_t span_lo, span_hi;
+  __device__ __forceinline__ void add(uint32_t cell, float v) { base[cell * 64u] += v; }
+  __device__ __forceinline__ float get(uint32_t cell) const { return base[cell * 64u]; }
+  __device__ __forceinline__ void set(uint32_t cell, float v) { base[cell * 64u] = v; }
+};
+
+// WGSL :297-348
+__device__ __forceinline__ void draw_edge(AccRef& acc, f2 q0, f2 q1, float weight) {
+  if (fabsf(q0.y - q1.y) <= 1.1920929e-7f) return;
+  float dir = 1.0f;
+  f2 p0 = q0, p1 = q1;
+  if (q0.y > q1.y) {
+    dir = -1.0f;
+    p0 = q1;
+    p1 = q0;
+  }
+  float dxdy = (p1.x - p0.x) / (p1.y - p0.y);
+  float x = p0.x;
+  uint32_t yc = (uint32_t)ceilf(p1.y);
+  uint32_t y_end = yc < GRID ? yc : GRID;
+  for (uint32_t y = (uint32_t)p0.y; y < y_end; y++) {
+    uint32_t row = y * GRID;
+    acc.span_lo = acc.span_lo < row ? acc.span_lo : row;
+    acc.span_hi = acc.span_hi > row + GRID + 1u ? acc.span_hi : row + GRID + 1u;
+    float dy = mn((float)(y + 1u), p1.y) - mx((float)y, p0.y);
+    float x_next = clampf(x + dxdy * dy, 0.0f, (float)GRID);
+    float d = dy * dir * weight;
+    float x0 = mn(x, x_next);
+    float x1 = mx(x, x_next);
+    float x0_floor = floorf(x0);
+    uint32_t x0i = (uint32_t)x0_floor;
+    float x1_ceil = ceilf(x1);
+    uint32_t x1i = (uint32_t)x1_ceil;
+    if (x1i <= x0i + 1u) {
+      float xmf = 0.5f * (x + x_next) - x0_floor;
+      acc.add(row + x0i, d - d * xmf);
+      acc.add(row + x0i + 1u, d * xmf);
+    } else {
+      float s = 1.0f / (x1 - x0);
+      float x0f = x0 - x0_floor;
+      float a0 = 0.5f * s * (1.0f - x0f) * (1.0f - x0f);
+      float x1f = x1 - x1_ceil + 1.0f;
+      float am = 0.5f * s * x1f * x1f;
+      acc.add(row + x0i, d * a0);
+      if (x1i == x0i + 2u) {
+        acc.add(row + x0i + 1u, d * (1.0f - a0 - am));
+      } else {
+        float a1 = s * (1.5f - x0f);
+        acc.add(row + x0i + 1u, d * (a1 - a0));
+        for (uint32_t xi = x0i + 2u; xi < x1i - 1u; xi++) acc.add(row + xi, d * s);
+        float a2 = a1 + (float)(x1i - x0i - 3u) * s;
+        acc.add(row + x1i - 1u, d * (1.0f - a2 - am));
+      }
+      acc.add(row + x1i, d * am);
+    }
+    x = x_next;
+  }
+}
+
+struct Poly3 {
+  f3 v[10];
+  uint32_t n;
+};
+struct Poly2 {
+  f2 v[10];
+  uint32_t n;
+};
+
+// WGSL :264-278. A clip keeps at most 10 vertices (the reference's MAX_POLY,
+// WGSL :11); in exact arithmetic these clips never produce more.
+__device__ __forceinline__ void clip_plane3(const Poly3& p, f3 normal, float offset, Poly3& out) {
+  out.n = 0;
+  for (uint32_t i = 0; i < p.n; i++) {
+    f3 a = p.v[i];
+    f3 b = p.v[i + 1 == p.n ? 0 : i + 1];
+    float da = offset - dot3(normal, a);
+    float db = offset - dot3(normal, b);
+    if (da >= 0.0f && out.n < 10) out.v[out.n++] = a;
+    if (da >= 0.0f && db < 0.0f && out.n < 10) out.v[out.n++] = a + (b - a) * (da / (da - db));
+    if (da < 0.0f && db >= 0.0f && out.n < 10) out.v[out.n++] = b + (a - b) * (db / (db - da));
+  }
+}
+
+// WGSL :280-294
+__device__ __forceinline__ void clip_line2(const Poly2& p, f2 normal, float offset, Poly2& out) {
+  out.n = 0;
+  for (uint32_t i = 0; i < p.n; i++) {
+    f2 a = p.v[i];
+    f2 b = p.v[i + 1 == p.n ? 0 : i + 1];
+    float da = offset - dot2(normal, a);
+    float db = offset - dot2(normal, b);
+    if (da >= 0.0f && out.n < 10) out.v[out.n++] = a;
+    if (da >= 0.0f && db < 0.0f && out.n < 10) out.v[out.n++] = a + (b - a) * (da / (da - db));
+    if (da < 0.0f && db >= 0.0f && out.n < 10) out.v[out.n++] = b + (a - b) * (db / (db - da));
+  }
+}
+
+__device__ __forceinline__ void clip_to_grid(Poly2& q) {
+  const float g = (float)GRID;
+  Poly2 t;
+  clip_line2(q, f2{-1.0f, 0.0f}, 0.0f, t);
+  clip_line2(t, f2{1.0f, 0.0f}, g, q);
+  clip_line2(q, f2{0.0f, -1.0f}, 0.0f, t);
+  clip_line2(t, f2{0.0f, 1.0f}, g, q);
+}
+
+__device__ __forceinline__ void draw_poly(AccRef& acc, const Poly2& q, float weight) {
+  const float g = (float)GRID;
+  for (uint32_t i = 0; i < q.n; i++) {
+    f2 a = q.v[i];
+    f2 b = q.v[i + 1 == q.n ? 0 : i + 1];
+    draw_edge(acc, f2{clampf(a.x, 0.0f, g), clampf(a.y, 0.0f, g)}, f2{clampf(b.x, 0.0f, g), clampf(b.y, 0.0f, g)},
+              weight);
+  }
+}
+
+// WGSL :350-451
+__device__ __noinline__ bool rasterize(const Cone& c, AccRef& acc, f3 v0, f3 v1, f3 v2, float weight) {
+  Poly3 p, t3;
+  p.v[0] = v0;
+  p.v[1] = v1;
+  p.v[2] = v2;
+  p.n = 3;
+  float f0 = c.far_offset - dot3(c.forward, v0);
+  float f1 = c.far_offset - dot3(c.forward, v1);
+  float f2_ = c.far_offset - dot3(c.forward, v2);
+  if (f0 < 0.0f || f1 < 0.0f || f2_ < 0.0f) {
+    clip_plane3(p, c.forward, c.far_offset, t3);
+    p = t3;
+  }
+  bool all_near_ok = true, all_in_window = true;
+  for (uint32_t i = 0; i < p.n; i++) {
+    p.v[i] = to_local(c, p.v[i]);
+    f3 v = p.v[i];
+    if (v.z < NEAR) all_near_ok = false;
+    if (v.x < c.w0 * v.z || v.x > c.w2 * v.z || v.y < c.w1 * v.z || v.y > c.w3 * v.z) all_in_window = false;
+  }
+  if (!all_near_ok) {
+    clip_plane3(p, f3{0.0f, 0.0f, -1.0f}, -NEAR, t3);
+    p = t3;
+  }
+  if (p.n < 3) return false;
+  float su = (float)GRID / (c.w2 - c.w0);
+  float sv = (float)GRID / (c.w3 - c.w1);
+  const float g = (float)GRID;
+  Poly2 q, t2;
+  if (all_near_ok && all_in_window) {
+    q.n = p.n;
+    for (uint32_t i = 0; i < p.n; i++)
+      q.v[i] = f2{(p.v[i].x / p.v[i].z - c.w0) * su, (p.v[i].y / p.v[i].z - c.w1) * sv};
+  } else {
+    float d = dot3(to_local(c, v0), cross3(to_local(c, v1), to_local(c, v2)));
+    if (d == 0.0f) return false;
+    float side = d > 0.0f ? 1.0f : -1.0f;
+    q.n = 4;
+    q.v[0] = f2{0.0f, 0.0f};
+    q.v[2] = f2{g, g};
+    if (side > 0.0f) {
+      q.v[1] = f2{g, 0.0f};
+      q.v[3] = f2{0.0f, g};
+    } else {
+      q.v[1] = f2{0.0f, g};
+      q.v[3] = f2{g, 0.0f};
+    }
+    for (uint32_t i = 0; i < p.n; i++) {
+      f3 e = cross3(p.v[i], p.v[i + 1 == p.n ? 0 : i + 1]);
+      float len = len3(e);
+      if (len == 0.0f) continue;
+      f3 k = e * (side / len);
+      clip_line2(q, f2{-k.x / su, -k.y / sv}, k.x * c.w0 + k.y * c.w1 + k.z, t2);
+      q = t2;
+    }
+  }
+  bool inside = true;
+  for (uint32_t i = 0; i < q.n; i++)
+    if (q.v[i].x < 0.0f || q.v[i].y < 0.0f || q.v[i].x > g || q.v[i].y > g) inside = false;
+  if (!inside) clip_to_grid(q);
+  if (q.n < 3) return false;
+  draw_poly(acc, q, weight);
+  return true;
+}
+
+}  // namespace cnq

@@ -1,0 +1,99 @@
+// Device-resident scene layout and kernel arguments shared by the HIP
+// kernels and the host library.
+//
+// HBM layout (per scene, resident across calls):
+//   blas      NodeC[]   32 B per triangle-hierarchy node, the reference's node
+//                       numbering (hierarchy.rs `NodeGpu`, 48 B) compacted:
+//                       box, then (a, b) = (left child, 0) for an interior
+//                       node or (first item, count) for a leaf.
+//   tri_leaf  float[9]  36 B per triangle, in `blas_items` order, so a leaf's
+//                       triangles are contiguous and fetched without the
+//                       item indirection.
+//   tris      float[12] 48 B per triangle in the reference's order (vec4 per
+//                       vertex), for walks without triangle hierarchies and
+//                       for camera hits' triangle indices.
+//   nodes     NodeC[]   object hierarchy; b's top bit marks a node without
+//                       prefiltered coverage (`NodeGpu::info.z` = NO_SPLAT).
+//   items, ranges (4 u32 per object), xforms (16 f32 per transform),
+//   blas_cov (27 f32 per node), blas_items: as the reference lays them out.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "cnq_device.h"
+
+namespace cnq {
+
+constexpr uint32_t NODE_NO_SPLAT = 0x80000000u;
+
+struct NodeC {
+  float lo[3];
+  float hi[3];
+  uint32_t a;
+  uint32_t b;
+};
+static_assert(sizeof(NodeC) == 32, "NodeC is 32 bytes");
+
+struct LightRect {
+  float4 corner, edge_u, edge_v, radiance;
+};
+
+struct SceneArgs {
+  const float* __restrict__ tris;
+  const float* __restrict__ tri_leaf;
+  const uint32_t* __restrict__ blas_items;
+  const uint32_t* __restrict__ ranges;
+  const NodeC* __restrict__ blas;
+  const float* __restrict__ blas_cov;
+  const NodeC* __restrict__ nodes;
+  const uint32_t* __restrict__ items;
+  const float* __restrict__ xforms;
+  uint32_t n_objects;
+};
+
+struct QueryArgs {
+  SceneArgs s;
+  const float* __restrict__ receivers;  // 8 floats each (point, normal); normal.w < 0: no cone
+  const float* __restrict__ lights;     // 16 floats each (LightGpu)
+  uint64_t n;                           // receivers; query q = l * n + i
+  uint64_t q_begin, q_end;              // the queries this launch evaluates
+  uint32_t hierarchy, light_cone_cap, wide_cone_cap, profile_stage, object_cov;
+  float prefilter, wide_extent;
+  float* __restrict__ out_visible;
+  uint64_t* __restrict__ out_full;
+  uint64_t* __restrict__ out_touched;
+  float* __restrict__ out_cov;        // nullable: 64 per query
+  uint32_t* __restrict__ counters;    // nullable: [capped, deeper than 48, truncated]
+  uint64_t* __restrict__ work;        // nullable: [object nodes, blas nodes, triangles tested, drawn,
+                                      //  meshes entered, blas splats]
+};
+
+struct PrimaryArgs {
+  SceneArgs s;
+  const float* __restrict__ camera;  // eye, forward, right, up, params (20 floats)
+  const float* __restrict__ lights;
+  uint32_t n_lights;
+  uint32_t width, height;
+  uint32_t hierarchy, nearest_first;
+  uint32_t* __restrict__ hits;   // 4 per pixel
+  float* __restrict__ receivers; // 8 per pixel
+  uint32_t* __restrict__ counters;  // nullable: [stack overflow]
+};
+
+struct RefitArgs {
+  const float* __restrict__ tris;        // new pose, 12 floats per triangle, reference order
+  const uint32_t* __restrict__ blas_items;
+  NodeC* blas;
+  float* blas_cov;
+  float* tri_leaf;
+  const uint32_t* __restrict__ parent;   // per node: parent index (NONE at a root)
+  uint32_t* arrivals;                    // per node: children finished (zeroed before)
+  const uint32_t* __restrict__ leaves;   // leaf node indices to start from
+  uint32_t n_leaves;
+};
+
+hipError_t launch_cone_queries(const QueryArgs& a, bool count_work, hipStream_t stream);
+hipError_t launch_primary(const PrimaryArgs& a, hipStream_t stream);
+hipError_t launch_refit(const RefitArgs& a, hipStream_t stream);
+
+}  // namespace cnq

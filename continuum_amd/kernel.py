@@ -1,0 +1,303 @@
+"""Host-side mirror of the reference's `MaskConeKernel`
+(crates/continuum-gpu/src/lib.rs:65) over the C ABI of libcnq.so.
+
+Same method names and argument meanings as the reference:
+``irradiance`` (lib.rs:1706), ``irradiance_with`` (:1718), ``coverage``
+(:3289), the setters ``set_prefilter``/``set_wide_cones``/
+``set_nearest_first``/``set_profile_stage`` (:1501-1675), and the per-view
+launch ``render_view`` (the cone part of ``render_instanced_lights``,
+:1857). Receivers are (point, normal) pairs; meshes are lists of triangles.
+Timings are the fastest of ``runs`` executions, submit to completion,
+excluding uploads and readback, as the reference's ``run`` times them
+(lib.rs:3497-3512).
+
+Failures raise ``CnqError``; contract violations (bad shapes, too many
+lights) raise ``ValueError``, where the reference panics.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+from enum import IntEnum
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+from ._lib import NONE, CnqError, SceneDesc, check, default_options, lib
+from .scenes import meshes_to_arrays, rect_light
+
+MAX_LIGHTS = 16  # lib.rs:751; the device path has no such limit, the mirror keeps it
+
+
+class Hierarchy(IntEnum):
+    """hierarchy.rs:727-741"""
+
+    NONE = 0
+    MESHES = 1
+    TRIANGLES = 2
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data if a.size else 0
+
+
+class Flat:
+    """The reference's flattened scene (hierarchy.rs `flatten` + `mesh_hierarchy`),
+    built by the library's host builder."""
+
+    def __init__(self, tris, ranges, blas, blas_items, blas_cov, nodes, items, xforms=None):
+        self.tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 12)
+        self.ranges = np.ascontiguousarray(ranges, dtype=np.uint32).reshape(-1, 4)
+        self.blas = np.ascontiguousarray(blas, dtype=np.uint32).reshape(-1, 12)
+        self.blas_items = np.ascontiguousarray(blas_items, dtype=np.uint32).reshape(-1)
+        self.blas_cov = np.ascontiguousarray(blas_cov, dtype=np.float32).reshape(-1, 27)
+        self.nodes = np.ascontiguousarray(nodes, dtype=np.uint32).reshape(-1, 12)
+        self.items = np.ascontiguousarray(items, dtype=np.uint32).reshape(-1)
+        self.xforms = (np.zeros((0, 16), np.float32) if xforms is None else
+                       np.ascontiguousarray(xforms, dtype=np.float32).reshape(-1, 16))
+
+    @classmethod
+    def build(cls, meshes) -> "Flat":
+        tris9, counts = meshes_to_arrays(meshes)
+        return cls.build_arrays(tris9, counts)
+
+    @classmethod
+    def build_arrays(cls, tris9: np.ndarray, counts: np.ndarray) -> "Flat":
+        L = lib()
+        tris9 = np.ascontiguousarray(tris9, dtype=np.float32)
+        counts = np.ascontiguousarray(counts, dtype=np.uint32)
+        h = C.c_void_p()
+        check(L.cnq_flat_build(_ptr(tris9), _ptr(counts), len(counts), C.byref(h)))
+        try:
+            sizes = np.zeros(6, dtype=np.uint64)
+            check(L.cnq_flat_sizes(h, _ptr(sizes)))
+            nt, nr, nb, nbi, nn, ni = (int(x) for x in sizes)
+            out = dict(tris=np.zeros((nt, 12), np.float32), ranges=np.zeros((nr, 4), np.uint32),
+                       blas=np.zeros((nb, 12), np.uint32), blas_items=np.zeros(nbi, np.uint32),
+                       blas_cov=np.zeros((nb, 27), np.float32), nodes=np.zeros((nn, 12), np.uint32),
+                       items=np.zeros(ni, np.uint32))
+            check(L.cnq_flat_copy(h, *(_ptr(out[k]) for k in
+                                       ("tris", "ranges", "blas", "blas_items", "blas_cov", "nodes", "items"))))
+        finally:
+            L.cnq_flat_destroy(h)
+        return cls(**out)
+
+    def desc(self) -> SceneDesc:
+        d = SceneDesc()
+        d.tris, d.n_tris = _ptr(self.tris), len(self.tris)
+        d.ranges, d.n_objects = _ptr(self.ranges), len(self.ranges)
+        d.blas, d.n_blas = _ptr(self.blas), len(self.blas)
+        d.blas_items, d.n_blas_items = _ptr(self.blas_items), len(self.blas_items)
+        d.blas_cov = _ptr(self.blas_cov)
+        d.nodes, d.n_nodes = _ptr(self.nodes), len(self.nodes)
+        d.items, d.n_items = _ptr(self.items), len(self.items)
+        d.xforms, d.n_xforms = _ptr(self.xforms), len(self.xforms)
+        return d
+
+
+class Device:
+    def __init__(self, ordinal: int = 0):
+        h = C.c_void_p()
+        check(lib().cnq_device_create(ordinal, C.byref(h)))
+        self.handle = h
+        self.ordinal = ordinal
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            lib().cnq_device_destroy(self.handle)
+            self.handle = None
+
+
+class Scene:
+    """A flattened scene resident in HBM."""
+
+    def __init__(self, device: Device, flat: Flat):
+        self.device = device
+        self.flat = flat
+        h = C.c_void_p()
+        d = flat.desc()
+        check(lib().cnq_scene_create(device.handle, C.byref(d), C.byref(h)))
+        self.handle = h
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            lib().cnq_scene_destroy(self.handle)
+            self.handle = None
+
+    @property
+    def hbm_bytes(self) -> int:
+        return int(lib().cnq_scene_bytes(self.handle))
+
+    def query(self, receivers: np.ndarray, lights: np.ndarray, opts: _lib.Options, want_cov: bool = False):
+        """Host buffers in and out: (visible, full, touched, cov) per query
+        (query l * n + i)."""
+        receivers = np.ascontiguousarray(receivers, dtype=np.float32).reshape(-1, 8)
+        lights = np.ascontiguousarray(lights, dtype=np.float32).reshape(-1, 16)
+        q = len(receivers) * len(lights)
+        vis = np.zeros(q, np.float32)
+        full = np.zeros(q, np.uint64)
+        touched = np.zeros(q, np.uint64)
+        cov = np.zeros((q, 64), np.float32) if want_cov else None
+        check(lib().cnq_irradiance(self.handle, _ptr(receivers), len(receivers), _ptr(lights), len(lights),
+                                   C.byref(opts), _ptr(vis), _ptr(full), _ptr(touched),
+                                   _ptr(cov) if want_cov else 0))
+        return vis, full, touched, cov
+
+    def query_device(self, receivers, lights, opts, vis, full, touched, cov=None, stream=None) -> None:
+        """Device tensors (torch) in and out; enqueued on `stream` (a
+        torch.cuda.Stream, or the current stream)."""
+        import torch
+
+        if stream is None:
+            stream = torch.cuda.current_stream()
+        n = receivers.numel() // 8
+        nl = lights.numel() // 16
+        for t, m in ((vis, 1), (full, 1), (touched, 1)):
+            if t.numel() < n * nl * m:
+                raise ValueError("output tensor too small")
+        check(lib().cnq_irradiance_device(self.handle, receivers.data_ptr(), n, lights.data_ptr(), nl,
+                                          C.byref(opts), vis.data_ptr(), full.data_ptr(), touched.data_ptr(),
+                                          cov.data_ptr() if cov is not None else 0, stream.cuda_stream))
+
+    def primary_device(self, camera, width, height, lights, opts, hits, receivers, stream=None) -> None:
+        import torch
+
+        if stream is None:
+            stream = torch.cuda.current_stream()
+        nl = lights.numel() // 16
+        check(lib().cnq_primary_device(self.handle, camera.data_ptr(), width, height, lights.data_ptr(), nl,
+                                       C.byref(opts), hits.data_ptr(), receivers.data_ptr(), stream.cuda_stream))
+
+    def render_view(self, camera: np.ndarray, width: int, height: int, lights: np.ndarray, opts: _lib.Options):
+        camera = np.ascontiguousarray(camera, dtype=np.float32)
+        lights = np.ascontiguousarray(lights, dtype=np.float32).reshape(-1, 16)
+        px = width * height
+        q = px * len(lights)
+        hits = np.zeros((px, 4), np.uint32)
+        vis = np.zeros(q, np.float32)
+        full = np.zeros(q, np.uint64)
+        touched = np.zeros(q, np.uint64)
+        check(lib().cnq_render_view(self.handle, _ptr(camera), width, height, _ptr(lights), len(lights),
+                                    C.byref(opts), _ptr(hits), _ptr(vis), _ptr(full), _ptr(touched)))
+        return hits, vis, full, touched
+
+    def refit(self, tris12: np.ndarray) -> None:
+        tris12 = np.ascontiguousarray(tris12, dtype=np.float32)
+        if tris12.size != self.flat.tris.size:
+            raise ValueError("refit needs the same triangles in the same order")
+        check(lib().cnq_scene_refit(self.handle, _ptr(tris12)))
+
+    def read_blas(self):
+        blas = np.zeros_like(self.flat.blas)
+        cov = np.zeros_like(self.flat.blas_cov)
+        check(lib().cnq_scene_read_blas(self.handle, _ptr(blas), _ptr(cov)))
+        return blas, cov
+
+    def counters(self, reset: bool = True):
+        c = np.zeros(3, np.uint32)
+        w = np.zeros(6, np.uint64)
+        check(lib().cnq_scene_counters(self.handle, int(reset), _ptr(c), _ptr(w)))
+        return c, w
+
+    def set_count_work(self, on: bool) -> None:
+        check(lib().cnq_scene_set_count_work(self.handle, int(on)))
+
+
+def receivers_array(receivers: Sequence) -> np.ndarray:
+    """ReceiverGpu rows (lib.rs:3353-3359) from (point, normal) pairs or an
+    (n, 2, 3) / (n, 6) array."""
+    a = np.asarray(receivers, dtype=np.float32).reshape(-1, 6)
+    out = np.zeros((len(a), 8), np.float32)
+    out[:, 0:3] = a[:, 0:3]
+    out[:, 4:7] = a[:, 3:6]
+    return out
+
+
+class MaskConeKernel:
+    """The reference's kernel object, on one MI355X (lib.rs:65)."""
+
+    def __init__(self, device: Device):
+        self.device = device
+        self._prefilter = 0.0
+        self._wide = True
+        self._nearest_first = True
+        self._profile_stage = 0
+
+    @classmethod
+    def new(cls, ordinal: int = 0) -> Optional["MaskConeKernel"]:
+        """None without a GPU, as lib.rs:1155."""
+        try:
+            return cls(Device(ordinal))
+        except CnqError as e:
+            if e.code == _lib.NO_DEVICE:
+                return None
+            raise
+
+    # setters (lib.rs:1501-1675)
+    def set_wide_cones(self, on: bool) -> None:
+        self._wide = bool(on)
+
+    def set_prefilter(self, bins: float) -> None:
+        self._prefilter = max(float(bins), 0.0)
+
+    def prefilter(self) -> float:
+        return self._prefilter
+
+    def set_nearest_first(self, on: bool) -> None:
+        self._nearest_first = bool(on)
+
+    def set_profile_stage(self, stage: int) -> None:
+        self._profile_stage = int(stage)
+
+    def options(self, hierarchy: Hierarchy = Hierarchy.TRIANGLES) -> _lib.Options:
+        return default_options(
+            hierarchy=int(hierarchy), prefilter=self._prefilter,
+            wide_extent=8.0 if self._wide else 1e30,
+            light_cone_cap=4096 if self._wide else NONE, wide_cone_cap=512 if self._wide else NONE,
+            profile_stage=self._profile_stage, nearest_first=int(self._nearest_first))
+
+    def scene(self, meshes) -> Scene:
+        return Scene(self.device, Flat.build(meshes))
+
+    @staticmethod
+    def _light_row(light) -> np.ndarray:
+        if isinstance(light, np.ndarray):
+            return light.astype(np.float32).reshape(16)
+        corner, edge_u, edge_v, one_sided = light
+        return rect_light(corner, edge_u, edge_v, one_sided)
+
+    def _run(self, receivers, light, meshes, runs, hierarchy, want_cov=False):
+        scene = self.scene(meshes)
+        recv = receivers_array(receivers)
+        lrow = self._light_row(light)[None, :]
+        opts = self.options(hierarchy)
+        best = float("inf")
+        out = None
+        for _ in range(max(1, runs)):
+            t0 = time.perf_counter()
+            out = scene.query(recv, lrow, opts, want_cov)
+            best = min(best, time.perf_counter() - t0)
+        return out, best
+
+    def irradiance(self, receivers, light, meshes, runs: int = 1):
+        """Every receiver tests every mesh's triangles (lib.rs:1706)."""
+        return self.irradiance_with(receivers, light, meshes, Hierarchy.NONE, runs)
+
+    def irradiance_with(self, receivers, light, meshes, hierarchy=Hierarchy.TRIANGLES, runs: int = 1):
+        """(irradiance per unit radiance per receiver, best seconds) (lib.rs:1718)."""
+        if isinstance(hierarchy, bool):
+            hierarchy = Hierarchy.TRIANGLES if hierarchy else Hierarchy.NONE
+        (vis, _, _, _), best = self._run(receivers, light, meshes, runs, hierarchy)
+        return vis, best
+
+    def coverage(self, receivers, light, meshes) -> np.ndarray:
+        """Per receiver, the 64 bins' coverage (lib.rs:3289; every triangle tested)."""
+        (_, _, _, cov), _ = self._run(receivers, light, meshes, 1, Hierarchy.NONE, want_cov=True)
+        return cov
+
+    def render_view(self, camera, width, height, lights, meshes, hierarchy=Hierarchy.TRIANGLES):
+        lights = np.asarray(lights, dtype=np.float32).reshape(-1, 16)
+        if len(lights) > MAX_LIGHTS:
+            raise ValueError(f"at most {MAX_LIGHTS} lights (lib.rs:1888)")
+        return self.scene(meshes).render_view(camera, width, height, lights, self.options(hierarchy))

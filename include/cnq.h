@@ -1,0 +1,162 @@
+/* cnq.h: the C ABI of the MI355X cone-visibility library (libcnq.so).
+ *
+ * A drop-in for the GPU path behind the reference's `MaskConeKernel`
+ * (crates/continuum-gpu/src/lib.rs:65) and the hierarchy it walks
+ * (crates/continuum-gpu/src/hierarchy.rs). The reference has no FFI; these
+ * are the entry points a Rust backend crate beside crates/continuum-gpu
+ * binds, with the reference's data layouts:
+ *
+ *   receiver  8 x f32   ReceiverGpu (lib.rs:56-59): point.xyz, _, normal.xyz, w
+ *                       (w < 0: trace nothing; the reference packs w = 0)
+ *   light    16 x f32   LightGpu (lib.rs:712-748): corner (w: 0 two-sided,
+ *                       1 one-sided, 2 sphere), edge_u, edge_v, radiance
+ *   node     12 x u32   NodeGpu (hierarchy.rs:70-76): lo vec4, hi vec4, info uvec4
+ *   triangle 12 x f32   three vec4 vertices (hierarchy.rs:10-13)
+ *   range     4 x u32   first triangle, count, hierarchy root, transform (hierarchy.rs:14-16)
+ *   xform    16 x f32   Placement::rows (hierarchy.rs:274-286)
+ *   camera   20 x f32   CameraUniform (lib.rs:3538-3553): eye, forward, right, up,
+ *                       params (width, height, tan(fov/2), radiance)
+ *
+ * A query is one (receiver i, light l) pair at index l * n + i. Its results
+ * are the irradiance per unit radiance (f32, as `hit_irradiance_rect`
+ * returns it), the bins fully covered (`full`, bit b set where coverage[b]
+ * >= 1) and the bins touched (`touched`, coverage[b] > 0), and optionally
+ * the 64 coverage values.
+ *
+ * Conventions: every function returns a status (CNQ_OK = 0) and never
+ * throws; cnq_last_error() describes the last failure on the calling
+ * thread. Outputs are caller-allocated. A handle is used from one host
+ * thread at a time. `*_device` functions take device pointers and a
+ * hipStream_t (NULL: the null stream) and return once work is enqueued.
+ */
+#ifndef CNQ_H
+#define CNQ_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  CNQ_OK = 0,
+  CNQ_NO_DEVICE = 1,
+  CNQ_INVALID_ARG = 2,
+  CNQ_CAPACITY = 3,
+  CNQ_OOM = 4,
+  CNQ_HIP_ERROR = 5
+};
+
+#define CNQ_NONE 0xffffffffu
+
+typedef struct cnq_device cnq_device;
+typedef struct cnq_scene cnq_scene;
+typedef struct cnq_flat cnq_flat;
+
+/* Kernel options: the reference's light-uniform fields and pipeline
+ * overrides (lib.rs:763-801 `RectLight::uniform`; mask_cones.wgsl
+ * WIDE_EXTENT :1047, LIGHT_CONE_CAP :1051, WIDE_CONE_CAP :1055). */
+typedef struct cnq_options {
+  uint32_t hierarchy;      /* Hierarchy (hierarchy.rs:727-741): 0 None, 1 Meshes, 2 Triangles */
+  float prefilter;         /* set_prefilter (lib.rs:1507): bins, 0 = exact */
+  float wide_extent;       /* set_wide_cones (lib.rs:1501): 8, or 1e30 when off */
+  uint32_t light_cone_cap; /* 4096, or CNQ_NONE when off */
+  uint32_t wide_cone_cap;  /* 512, or CNQ_NONE when off */
+  uint32_t profile_stage;  /* set_profile_stage (lib.rs:1675): 0 off */
+  uint32_t object_cov;     /* light.objects.x: first object node in blas_cov, CNQ_NONE: none */
+  uint32_t nearest_first;  /* set_nearest_first (lib.rs:1640): camera rays' child order */
+} cnq_options;
+
+/* The reference's defaults (lib.rs:1215-1240): Triangles, exact, wide cones on. */
+void cnq_options_default(cnq_options* opts);
+
+const char* cnq_last_error(void);
+
+/* MaskConeKernel::new (lib.rs:1155): CNQ_NO_DEVICE where the reference
+ * returns None. */
+int cnq_device_create(int ordinal, cnq_device** out);
+void cnq_device_destroy(cnq_device* dev);
+
+/* ---- Host hierarchy builder: `flatten` (hierarchy.rs:29-66) plus
+ * `mesh_hierarchy` (hierarchy.rs:183-196) over meshes given as 9 floats per
+ * triangle, counts[m] triangles each. */
+int cnq_flat_build(const float* tris9, const uint32_t* counts, uint32_t n_meshes, cnq_flat** out);
+/* sizes: n_tris, n_ranges, n_blas, n_blas_items, n_nodes, n_items */
+int cnq_flat_sizes(const cnq_flat* flat, uint64_t sizes[6]);
+int cnq_flat_copy(const cnq_flat* flat, float* tris, uint32_t* ranges, uint32_t* blas, uint32_t* blas_items,
+                  float* blas_cov, uint32_t* nodes, uint32_t* items);
+void cnq_flat_destroy(cnq_flat* flat);
+
+/* ---- A scene resident in HBM, from the reference's flattened arrays
+ * (lib.rs:3361-3440 `run`'s buffers; lib.rs:3002 `render_buffers_with`).
+ * Every index is checked on the host: children follow their parents,
+ * leaves and ranges stay inside their arrays. */
+typedef struct cnq_scene_desc {
+  const float* tris;          uint64_t n_tris;       /* 12 f32 per triangle */
+  const uint32_t* ranges;     uint32_t n_objects;    /* 4 u32 per object */
+  const uint32_t* blas;       uint32_t n_blas;       /* NodeGpu */
+  const uint32_t* blas_items; uint64_t n_blas_items;
+  const float* blas_cov;                             /* 27 per blas node (NULL: zeros) */
+  const uint32_t* nodes;      uint32_t n_nodes;      /* object hierarchy, NodeGpu */
+  const uint32_t* items;      uint32_t n_items;
+  const float* xforms;        uint32_t n_xforms;     /* 16 f32 per transform */
+} cnq_scene_desc;
+
+int cnq_scene_create(cnq_device* dev, const cnq_scene_desc* desc, cnq_scene** out);
+void cnq_scene_destroy(cnq_scene* scene);
+/* Bytes of HBM the scene holds. */
+uint64_t cnq_scene_bytes(const cnq_scene* scene);
+
+/* ---- Queries. MaskConeKernel::irradiance_with (lib.rs:1718) and
+ * `coverage` (lib.rs:3289): host buffers in and out, synchronous. out_cov
+ * (64 per query) may be NULL. */
+int cnq_irradiance(cnq_scene* scene, const float* receivers, uint64_t n, const float* lights, uint32_t n_lights,
+                   const cnq_options* opts, float* out_visible, uint64_t* out_full, uint64_t* out_touched,
+                   float* out_cov);
+
+/* The same on device buffers, enqueued on `stream` (the WGSL `main` entry,
+ * mask_cones.wgsl:1513, dispatched as lib.rs:3497-3512 does). */
+int cnq_irradiance_device(cnq_scene* scene, const float* receivers, uint64_t n, const float* lights,
+                          uint32_t n_lights, const cnq_options* opts, float* out_visible, uint64_t* out_full,
+                          uint64_t* out_touched, float* out_cov, void* stream);
+
+/* Camera rays for a width x height view (the WGSL `visibility` entry,
+ * mask_cones.wgsl:2227, with `camera_hit` :2222): per pixel, hits (mesh,
+ * triangle, distance bits, seen light + 1) and receivers (the shading point
+ * offset by `surface_offset`, the face normal toward the camera; w = -1
+ * where the pixel sees the sky or a light). Device buffers. */
+int cnq_primary_device(cnq_scene* scene, const float* camera, uint32_t width, uint32_t height, const float* lights,
+                       uint32_t n_lights, const cnq_options* opts, uint32_t* hits, float* receivers, void* stream);
+
+/* The plain per-view launch (MaskConeKernel::render_instanced_lights,
+ * lib.rs:1857, without shading): camera rays, then one query per pixel and
+ * light. Host buffers: hits 4 per pixel, the rest one per query. */
+int cnq_render_view(cnq_scene* scene, const float* camera, uint32_t width, uint32_t height, const float* lights,
+                    uint32_t n_lights, const cnq_options* opts, uint32_t* out_hits, float* out_visible,
+                    uint64_t* out_full, uint64_t* out_touched);
+
+/* ---- Refit (hierarchy.rs:579 `refit_triangle_hierarchy`, applied to
+ * every mesh with a hierarchy; instanced.rs:1154 `apply_poses`): new
+ * triangle positions (12 f32 per triangle, same count and order), on the
+ * device in one bottom-up pass. */
+int cnq_scene_refit_device(cnq_scene* scene, const float* tris, void* stream);
+int cnq_scene_refit(cnq_scene* scene, const float* tris);
+/* The scene's triangle hierarchy as it is now (NodeGpu layout) and its
+ * coverage samples (27 per node), for reading back a refit. */
+int cnq_scene_read_blas(cnq_scene* scene, uint32_t* blas, float* blas_cov);
+
+/* ---- Diagnostics: counters accumulated by queries since the last reset:
+ * [0] capped walks, [1] walks deeper than the reference's 48-entry stack,
+ * [2] walks this build truncated (cones, or camera rays deeper than 48);
+ * work, summed over queries when count_work is on (KernelOptions::
+ * count_work, lib.rs:523-537; WGSL :655-665): object nodes tested,
+ * triangle-hierarchy nodes tested, triangles tested, triangles that
+ * reached the rasterizer, meshes entered, triangle-hierarchy nodes
+ * splatted. */
+int cnq_scene_counters(cnq_scene* scene, int reset, uint32_t counters[3], uint64_t work[6]);
+int cnq_scene_set_count_work(cnq_scene* scene, int on);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

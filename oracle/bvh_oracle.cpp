@@ -1,0 +1,321 @@
+// TEST INFRASTRUCTURE ONLY (see oracle_math.h): a CPU restatement of the
+// reference's GPU hierarchy builder, crates/continuum-gpu/src/hierarchy.rs
+// (`split` :128-179, `triangle_hierarchy` :527-571, `object_hierarchy`
+// :198-253, `flatten` :29-66, `refit_triangle_hierarchy` :579-619,
+// `padded` :762, `mesh_bounds` :751, `coverage_directions` :736).
+//
+// The reference splits each node with Rust's `select_nth_unstable_by`, whose
+// permutation of the two halves is an implementation detail of the Rust
+// toolchain. Here a node's items are ordered by the total order (key, item
+// index) and split at the median; each side keeps that order. The tree's
+// shape and numbering depend only on the item count, exactly as in the
+// reference; which items land in a leaf and in what order follows the
+// (key, index) rule.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "oracle_math.h"
+
+namespace orc {
+
+constexpr uint32_t BNONE = 0xffffffffu;
+constexpr float FMAX = 3.40282347e38f;
+
+struct NodeGpu {
+  float lo[4];
+  float hi[4];
+  uint32_t info[4];
+};
+
+struct Box {
+  V3 lo, hi;
+};
+
+// glam's f32::max/min ignore NaN; none occur here.
+static inline float gmax(float a, float b) { return a > b ? a : (b > a ? b : a); }
+static inline float gmin(float a, float b) { return a < b ? a : (b < a ? b : a); }
+static inline V3 gvmin(V3 a, V3 b) { return V3{gmin(a.x, b.x), gmin(a.y, b.y), gmin(a.z, b.z)}; }
+static inline V3 gvmax(V3 a, V3 b) { return V3{gmax(a.x, b.x), gmax(a.y, b.y), gmax(a.z, b.z)}; }
+
+// hierarchy.rs:762-765
+static Box padded(Box b) {
+  V3 m = gvmax(vabs(b.lo), vabs(b.hi));
+  float e = gmax(gmax(m.x, m.y), m.z);
+  float pad = 1e-5f * gmax(e, 1.0f);
+  return Box{V3{b.lo.x - pad, b.lo.y - pad, b.lo.z - pad}, V3{b.hi.x + pad, b.hi.y + pad, b.hi.z + pad}};
+}
+
+// hierarchy.rs:751-756 over one triangle (9 floats) or a run of them.
+static Box tri_bounds(const float* t, size_t n_tris) {
+  Box b{V3{FMAX, FMAX, FMAX}, V3{-FMAX, -FMAX, -FMAX}};
+  for (size_t i = 0; i < 3 * n_tris; i++) {
+    V3 p{t[3 * i], t[3 * i + 1], t[3 * i + 2]};
+    b.lo = gvmin(b.lo, p);
+    b.hi = gvmax(b.hi, p);
+  }
+  return b;
+}
+
+// Rust's f32::total_cmp as an integer key.
+static inline int32_t total_key(float f) {
+  int32_t i;
+  std::memcpy(&i, &f, 4);
+  return i ^ (int32_t)(((uint32_t)(i >> 31)) >> 1);
+}
+
+struct Builder {
+  std::vector<NodeGpu>* nodes;
+  const std::vector<Box>* bounds;
+  size_t leaf;
+  uint32_t node_base, item_base;
+
+  // hierarchy.rs:128-179
+  void split(size_t node, uint32_t* items, size_t count, size_t start) {
+    const std::vector<Box>& bd = *bounds;
+    V3 lo{FMAX, FMAX, FMAX}, hi{-FMAX, -FMAX, -FMAX};
+    V3 clo{FMAX, FMAX, FMAX}, chi{-FMAX, -FMAX, -FMAX};
+    for (size_t i = 0; i < count; i++) {
+      const Box& b = bd[items[i]];
+      lo = gvmin(lo, b.lo);
+      hi = gvmax(hi, b.hi);
+      V3 c = (b.lo + b.hi) * 0.5f;
+      clo = gvmin(clo, c);
+      chi = gvmax(chi, c);
+    }
+    NodeGpu& nd = (*nodes)[node];
+    nd.lo[0] = lo.x, nd.lo[1] = lo.y, nd.lo[2] = lo.z, nd.lo[3] = 0.0f;
+    nd.hi[0] = hi.x, nd.hi[1] = hi.y, nd.hi[2] = hi.z, nd.hi[3] = 0.0f;
+    if (count <= leaf) {
+      nd.info[0] = item_base + (uint32_t)start;
+      nd.info[1] = (uint32_t)count;
+      nd.info[2] = nd.info[3] = 0;
+      return;
+    }
+    V3 e = chi - clo;
+    int axis = (e.x >= e.y && e.x >= e.z) ? 0 : (e.y >= e.z ? 1 : 2);
+    size_t mid = count / 2;
+    auto key = [&](uint32_t m) {
+      const Box& b = bd[m];
+      float a = axis == 0 ? b.lo.x : (axis == 1 ? b.lo.y : b.lo.z);
+      float c = axis == 0 ? b.hi.x : (axis == 1 ? b.hi.y : b.hi.z);
+      return total_key(a + c);
+    };
+    std::sort(items, items + count, [&](uint32_t a, uint32_t b) {
+      int32_t ka = key(a), kb = key(b);
+      return ka != kb ? ka < kb : a < b;
+    });
+    size_t left = nodes->size();
+    nodes->push_back(NodeGpu{});
+    nodes->push_back(NodeGpu{});
+    NodeGpu& nd2 = (*nodes)[node];
+    nd2.info[0] = node_base + (uint32_t)left;
+    nd2.info[1] = nd2.info[2] = nd2.info[3] = 0;
+    split(left, items, mid, start);
+    split(left + 1, items + mid, count - mid, start + mid);
+  }
+};
+
+static void coverage_directions(V3 dirs[27]) {
+  for (int n = 0; n < 27; n++) {
+    V3 d{(float)(n / 9) - 1.0f, (float)(n / 3 % 3) - 1.0f, (float)(n % 3) - 1.0f};
+    float len = std::sqrt(dot(d, d));
+    float rcp = 1.0f / len;
+    dirs[n] = (std::isfinite(rcp) && rcp > 0.0f) ? d * rcp : V3{0.0f, 0.0f, 0.0f};
+  }
+}
+
+static void tri_cov_add(const float* t, const V3 dirs[27], float* out) {
+  V3 a{t[0], t[1], t[2]}, b{t[3], t[4], t[5]}, c{t[6], t[7], t[8]};
+  V3 an = cross(b - a, c - a) * 0.5f;
+  for (int k = 0; k < 27; k++) out[k] += gmax(dot(an, dirs[k]), 0.0f);
+}
+
+// hierarchy.rs:527-571. `mesh` holds n triangles as 9 floats each.
+static void triangle_hierarchy(const float* mesh, size_t n, uint32_t root, uint32_t first,
+                               std::vector<NodeGpu>& nodes, std::vector<uint32_t>& items_out,
+                               std::vector<float>& cov) {
+  std::vector<Box> bounds(n);
+  for (size_t i = 0; i < n; i++) bounds[i] = padded(tri_bounds(mesh + 9 * i, 1));
+  std::vector<uint32_t> items(n);
+  for (size_t i = 0; i < n; i++) items[i] = (uint32_t)i;
+  nodes.assign(1, NodeGpu{});
+  Builder b{&nodes, &bounds, 8, root, first};
+  b.split(0, items.data(), n, 0);
+  V3 dirs[27];
+  coverage_directions(dirs);
+  cov.assign(nodes.size() * 27, 0.0f);
+  for (size_t nn = nodes.size(); nn-- > 0;) {
+    uint32_t x = nodes[nn].info[0], count = nodes[nn].info[1];
+    float* out = cov.data() + nn * 27;
+    if (count > 0) {
+      size_t st = x - first;
+      for (size_t k = st; k < st + count; k++) tri_cov_add(mesh + 9 * (size_t)items[k], dirs, out);
+    } else {
+      size_t left = x - root;
+      for (int k = 0; k < 27; k++) out[k] = cov[left * 27 + k] + cov[(left + 1) * 27 + k];
+    }
+  }
+  items_out.resize(n);
+  for (size_t i = 0; i < n; i++) items_out[i] = first + items[i];
+}
+
+// hierarchy.rs:198-253 (every object kept).
+static void object_hierarchy(const std::vector<Box>& bounds, std::vector<NodeGpu>& nodes,
+                             std::vector<uint32_t>& items) {
+  items.resize(bounds.size());
+  for (size_t i = 0; i < bounds.size(); i++) items[i] = (uint32_t)i;
+  nodes.assign(1, NodeGpu{});
+  if (items.empty()) {
+    items.push_back(0);
+    nodes[0].info[0] = 0;
+    nodes[0].info[1] = 1;
+    return;
+  }
+  auto diagonal = [&](uint32_t m) { return length(bounds[m].hi - bounds[m].lo); };
+  std::vector<float> sizes(items.size());
+  for (size_t i = 0; i < items.size(); i++) sizes[i] = diagonal(items[i]);
+  size_t mid = sizes.size() / 2;
+  std::vector<float> sorted = sizes;
+  std::sort(sorted.begin(), sorted.end(), [](float a, float b) { return total_key(a) < total_key(b); });
+  float median = sorted[mid];
+  std::vector<uint32_t> big, small;
+  for (uint32_t m : items) (diagonal(m) > 64.0f * median ? big : small).push_back(m);
+  if (big.empty() || small.empty()) {
+    Builder b{&nodes, &bounds, 2, 0, 0};
+    b.split(0, items.data(), items.size(), 0);
+    return;
+  }
+  size_t big_count = big.size();
+  items = big;
+  items.insert(items.end(), small.begin(), small.end());
+  nodes.push_back(NodeGpu{});
+  nodes.push_back(NodeGpu{});
+  Builder b{&nodes, &bounds, 2, 0, 0};
+  b.split(1, items.data(), big_count, 0);
+  b.split(2, items.data() + big_count, items.size() - big_count, big_count);
+  const NodeGpu &a = nodes[1], &c = nodes[2];
+  for (int k = 0; k < 3; k++) {
+    nodes[0].lo[k] = gmin(a.lo[k], c.lo[k]);
+    nodes[0].hi[k] = gmax(a.hi[k], c.hi[k]);
+  }
+  nodes[0].lo[3] = nodes[0].hi[3] = 0.0f;
+  nodes[0].info[0] = 1;
+  nodes[0].info[1] = nodes[0].info[2] = nodes[0].info[3] = 0;
+}
+
+}  // namespace orc
+
+using namespace orc;
+
+extern "C" {
+
+// flatten (hierarchy.rs:29-66) + mesh_hierarchy (:183-196) for meshes given
+// as 9 floats per triangle, `counts[m]` triangles each, in order. Two calls:
+// with null outputs it returns the sizes in `sizes` = [n_tris, n_ranges,
+// n_blas, n_blas_items, n_nodes, n_items]; then it fills them.
+int oracle_flatten(const float* tris, const uint32_t* counts, uint32_t n_meshes, uint64_t* sizes, float* out_tris,
+                   uint32_t* out_ranges, float* out_blas, uint32_t* out_blas_items, float* out_blas_cov,
+                   float* out_nodes, uint32_t* out_items) {
+  std::vector<float> ftris;
+  std::vector<uint32_t> ranges;
+  std::vector<NodeGpu> blas;
+  std::vector<uint32_t> blas_items;
+  std::vector<float> blas_cov;
+  std::vector<Box> mesh_boxes;
+  size_t off = 0;
+  for (uint32_t m = 0; m < n_meshes; m++) {
+    const float* mesh = tris + 9 * off;
+    size_t n = counts[m];
+    uint32_t first = (uint32_t)(ftris.size() / 12);
+    uint32_t root = BNONE;
+    if (n > 0) {
+      root = (uint32_t)blas.size();
+      std::vector<NodeGpu> nodes;
+      std::vector<uint32_t> items;
+      std::vector<float> cov;
+      triangle_hierarchy(mesh, n, root, first, nodes, items, cov);
+      blas.insert(blas.end(), nodes.begin(), nodes.end());
+      blas_items.insert(blas_items.end(), items.begin(), items.end());
+      blas_cov.insert(blas_cov.end(), cov.begin(), cov.end());
+    }
+    ranges.insert(ranges.end(), {first, (uint32_t)n, root, BNONE});
+    for (size_t i = 0; i < 3 * n; i++) ftris.insert(ftris.end(), {mesh[3 * i], mesh[3 * i + 1], mesh[3 * i + 2], 0.0f});
+    mesh_boxes.push_back(padded(tri_bounds(mesh, n)));
+    off += n;
+  }
+  if (ftris.empty()) ftris.assign(12, 0.0f);
+  if (ranges.empty()) ranges.insert(ranges.end(), {0, 0, BNONE, BNONE});
+  if (blas.empty()) {
+    blas.push_back(NodeGpu{});
+    blas_items.push_back(0);
+    blas_cov.assign(27, 0.0f);
+  }
+  std::vector<NodeGpu> nodes;
+  std::vector<uint32_t> items;
+  object_hierarchy(mesh_boxes, nodes, items);
+  if (!out_tris) {
+    sizes[0] = ftris.size() / 12;
+    sizes[1] = ranges.size() / 4;
+    sizes[2] = blas.size();
+    sizes[3] = blas_items.size();
+    sizes[4] = nodes.size();
+    sizes[5] = items.size();
+    return 0;
+  }
+  std::memcpy(out_tris, ftris.data(), ftris.size() * 4);
+  std::memcpy(out_ranges, ranges.data(), ranges.size() * 4);
+  std::memcpy(out_blas, blas.data(), blas.size() * sizeof(NodeGpu));
+  std::memcpy(out_blas_items, blas_items.data(), blas_items.size() * 4);
+  std::memcpy(out_blas_cov, blas_cov.data(), blas_cov.size() * 4);
+  std::memcpy(out_nodes, nodes.data(), nodes.size() * sizeof(NodeGpu));
+  std::memcpy(out_items, items.data(), items.size() * 4);
+  return 0;
+}
+
+// refit_triangle_hierarchy (hierarchy.rs:579-619) over a whole flattened
+// scene: for every mesh with a hierarchy, recompute its nodes' boxes and
+// coverage from `tris` (3 vec4 per triangle, the new pose) in place.
+void oracle_refit(const float* tris, const uint32_t* ranges, uint32_t n_ranges, float* blas_f,
+                  const uint32_t* blas_items, float* blas_cov, uint32_t n_blas) {
+  NodeGpu* blas = reinterpret_cast<NodeGpu*>(blas_f);
+  V3 dirs[27];
+  coverage_directions(dirs);
+  // Each mesh's nodes run from its root to the next root (or the end).
+  std::vector<uint32_t> roots;
+  for (uint32_t r = 0; r < n_ranges; r++)
+    if (ranges[4 * r + 2] != BNONE) roots.push_back(ranges[4 * r + 2]);
+  std::sort(roots.begin(), roots.end());
+  roots.erase(std::unique(roots.begin(), roots.end()), roots.end());
+  for (size_t ri = 0; ri < roots.size(); ri++) {
+    uint32_t root = roots[ri];
+    uint32_t end = ri + 1 < roots.size() ? roots[ri + 1] : n_blas;
+    for (uint32_t nn = end; nn-- > root;) {
+      uint32_t x = blas[nn].info[0], count = blas[nn].info[1];
+      V3 lo{FMAX, FMAX, FMAX}, hi{-FMAX, -FMAX, -FMAX};
+      float* out = blas_cov + (size_t)nn * 27;
+      for (int k = 0; k < 27; k++) out[k] = 0.0f;
+      if (count > 0) {
+        for (uint32_t k = x; k < x + count; k++) {
+          uint32_t g = blas_items[k];
+          const float* t = tris + 12 * (size_t)g;
+          float t9[9] = {t[0], t[1], t[2], t[4], t[5], t[6], t[8], t[9], t[10]};
+          Box b = padded(tri_bounds(t9, 1));
+          lo = gvmin(lo, b.lo);
+          hi = gvmax(hi, b.hi);
+          tri_cov_add(t9, dirs, out);
+        }
+      } else {
+        for (uint32_t c = x; c <= x + 1; c++) {
+          lo = gvmin(lo, V3{blas[c].lo[0], blas[c].lo[1], blas[c].lo[2]});
+          hi = gvmax(hi, V3{blas[c].hi[0], blas[c].hi[1], blas[c].hi[2]});
+        }
+        for (int k = 0; k < 27; k++) out[k] = blas_cov[(size_t)x * 27 + k] + blas_cov[(size_t)(x + 1) * 27 + k];
+      }
+      blas[nn].lo[0] = lo.x, blas[nn].lo[1] = lo.y, blas[nn].lo[2] = lo.z, blas[nn].lo[3] = 0.0f;
+      blas[nn].hi[0] = hi.x, blas[nn].hi[1] = hi.y, blas[nn].hi[2] = hi.z, blas[nn].hi[3] = 0.0f;
+    }
+  }
+}
+
+}  // extern "C"

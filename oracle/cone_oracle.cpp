@@ -1,0 +1,1466 @@
+// TEST INFRASTRUCTURE ONLY (see oracle_math.h): a CPU restatement of the
+// reference's cone-visibility path, used as the checker by tests/, smoke()
+// and bench.py's cpu_baseline leg. Never linked by the product.
+//
+// Every function follows the reference's GPU semantics, the WGSL in
+// crates/continuum-gpu/src/mask_cones.wgsl (cited as "WGSL :line"), which
+// differ in documented ways from the CPU model in crates/continuum/src/cone3.rs
+// (restated separately in cone3_oracle.cpp). One query walks alone (no
+// packets), exactly in the order the WGSL walks.
+#include <algorithm>
+#include <atomic>
+#include <cstring>
+#include <thread>
+#include <vector>
+
+#include "oracle_math.h"
+
+namespace orc {
+
+constexpr uint32_t NONE = 0xffffffffu;
+constexpr uint32_t GRID = 8;
+constexpr uint32_t BINS = 64;
+constexpr uint32_t STACK = 48;  // WGSL :111
+constexpr float NEAR = 1e-4f;   // WGSL :211
+
+// WGSL constant expressions are evaluated in abstract (f64) precision and
+// rounded to f32 once.
+constexpr float ONE_MINUS_1EM5 = (float)(1.0 - 1e-5);
+
+struct Node {
+  float lo[4];
+  float hi[4];
+  uint32_t info[4];
+};
+
+// The scene as the reference's `flatten`/`mesh_hierarchy` lay it out
+// (crates/continuum-gpu/src/hierarchy.rs:10-66, lib.rs:3361-3440).
+struct Scene {
+  const float* tris;  // 3 vec4 per triangle
+  const uint32_t* ranges;  // per object: first, count, blas root, xform
+  uint32_t n_objects;
+  const Node* blas;
+  const uint32_t* blas_items;
+  const float* blas_cov;  // 27 per blas node
+  const Node* nodes;      // object hierarchy
+  const uint32_t* items;
+  const float* xforms;  // 16 per transform
+};
+
+struct Options {
+  uint32_t hierarchy;       // 0 every triangle, 1 object BVH, 2 + triangle BVHs
+  float prefilter;          // bins (0: exact)
+  float wide_extent;        // WGSL :1047 WIDE_EXTENT
+  uint32_t light_cone_cap;  // WGSL :1051
+  uint32_t wide_cone_cap;   // WGSL :1055
+  uint32_t profile_stage;   // WGSL :21-26
+  uint32_t object_cov;      // light.objects.x: first object node's coverage in blas_cov (NONE: none)
+};
+
+struct Light {
+  float corner[4];
+  float edge_u[4];
+  float edge_v[4];
+  float radiance[4];
+};
+
+struct Cone {
+  V3 apex, right, up, forward;
+  float window[4];  // u0, v0, u1, v1
+  V3 far_normal;
+  float far_offset;
+  V3 horizon;
+  float bound[4];
+  float bound_sin;
+};
+
+// The per-invocation private state the WGSL keeps in module-scope `var<private>`s.
+struct Query {
+  float prefilter_bins = 0.0f;
+  float tri_weight = 1.0f;
+  uint32_t cone_work = 0;
+  uint32_t cone_cap = NONE;
+  bool wide_clipped = false;
+  bool wide_window = false;
+  bool stack_overflow = false;
+  uint32_t stack[STACK];
+  const Scene* s;
+  const Options* o;
+};
+
+static inline V3 xyz(const float* p) { return V3{p[0], p[1], p[2]}; }
+
+// ---- transforms (WGSL :58-81)
+static V3 rotate_to_object(const Scene& s, uint32_t x, V3 d) {
+  const float* r = s.xforms + 16 * x;
+  return xyz(r) * d.x + xyz(r + 4) * d.y + xyz(r + 8) * d.z;
+}
+static V3 translation(const Scene& s, uint32_t x) {
+  const float* r = s.xforms + 16 * x;
+  return V3{r[3], r[7], r[11]};
+}
+static V3 point_to_object(const Scene& s, uint32_t m, V3 p) {
+  uint32_t x = s.ranges[4 * m + 3];
+  if (x == NONE) return p;
+  return rotate_to_object(s, x, p - translation(s, x)) * s.xforms[16 * x + 13];
+}
+
+// ---- cone geometry (WGSL :159-255)
+static V3 direction(const Cone& c, float s, float t) {
+  return c.forward + c.right * (c.window[0] + s * (c.window[2] - c.window[0])) +
+         c.up * (c.window[1] + t * (c.window[3] - c.window[1]));
+}
+
+static V3 to_local(const Cone& c, V3 p) {
+  V3 d = p - c.apex;
+  return V3{dot(c.right, d), dot(c.up, d), dot(c.forward, d)};
+}
+
+// WGSL :159-168
+static bool in_bound(const Cone& c, V3 p, float r) {
+  if (c.bound[3] < -1.0f) return true;
+  V3 d = p - c.apex;
+  float len = length(d);
+  if (len <= r) return true;
+  float s = r / len;
+  float limit = c.bound[3] * std::sqrt(1.0f - s * s) - c.bound_sin * s;
+  return dot(xyz(c.bound), d) >= limit * len;
+}
+
+// WGSL :174-186
+static Cone bound_cone(Cone c, V3 axis) {
+  float cos_half = 1.0f;
+  for (uint32_t k = 0; k < 4; k++) {
+    V3 d = normalize(direction(c, (float)(k & 1u), (float)(k >> 1u)));
+    cos_half = fmin_(cos_half, dot(axis, d));
+  }
+  float cos_b = clampf(cos_half - 1e-5f, -1.0f, 1.0f);
+  c.bound[0] = axis.x;
+  c.bound[1] = axis.y;
+  c.bound[2] = axis.z;
+  c.bound[3] = cos_b;
+  c.bound_sin = std::sqrt(fmax_(1.0f - cos_b * cos_b, 0.0f));
+  return c;
+}
+
+// WGSL :193-201
+static float lambert_edge_unit(V3 n, V3 an, V3 bn) {
+  V3 g = cross(an, bn);
+  float len = length(g);
+  if (len > 1e-12f) return atan2_c(len, dot(an, bn)) * dot(n, g / len);
+  return 0.0f;
+}
+static float lambert_edge(V3 n, V3 a, V3 b) { return lambert_edge_unit(n, normalize(a), normalize(b)); }
+
+// WGSL :213-232
+static bool may_touch(const Cone& c, V3 v0, V3 v1, V3 v2) {
+  float f0 = c.far_offset - dot(c.far_normal, v0);
+  float f1 = c.far_offset - dot(c.far_normal, v1);
+  float f2 = c.far_offset - dot(c.far_normal, v2);
+  if (f0 < 0.0f && f1 < 0.0f && f2 < 0.0f) return false;
+  if (dot(c.horizon, v0 - c.apex) < 0.0f && dot(c.horizon, v1 - c.apex) < 0.0f &&
+      dot(c.horizon, v2 - c.apex) < 0.0f)
+    return false;
+  V3 p0 = to_local(c, v0), p1 = to_local(c, v1), p2 = to_local(c, v2);
+  if (p0.z < NEAR && p1.z < NEAR && p2.z < NEAR) return false;
+  const float* w = c.window;
+  if (p0.x - w[0] * p0.z < 0.0f && p1.x - w[0] * p1.z < 0.0f && p2.x - w[0] * p2.z < 0.0f) return false;
+  if (w[2] * p0.z - p0.x < 0.0f && w[2] * p1.z - p1.x < 0.0f && w[2] * p2.z - p2.x < 0.0f) return false;
+  if (p0.y - w[1] * p0.z < 0.0f && p1.y - w[1] * p1.z < 0.0f && p2.y - w[1] * p2.z < 0.0f) return false;
+  if (w[3] * p0.z - p0.y < 0.0f && w[3] * p1.z - p1.y < 0.0f && w[3] * p2.z - p2.y < 0.0f) return false;
+  return true;
+}
+
+static V3 box_corner(V3 lo, V3 hi, uint32_t k) {
+  return V3{(k & 1u) ? hi.x : lo.x, (k & 2u) ? hi.y : lo.y, (k & 4u) ? hi.z : lo.z};
+}
+
+// WGSL :234-255
+static bool may_touch_box(const Cone& c, V3 lo, V3 hi) {
+  bool far_out = true, near_out = true, out_u0 = true, out_u1 = true, out_v0 = true, out_v1 = true,
+       below = true;
+  for (uint32_t k = 0; k < 8; k++) {
+    V3 corner = box_corner(lo, hi, k);
+    if (c.far_offset - dot(c.far_normal, corner) >= 0.0f) far_out = false;
+    if (dot(c.horizon, corner - c.apex) >= 0.0f) below = false;
+    V3 p = to_local(c, corner);
+    if (p.z >= NEAR) near_out = false;
+    if (p.x - c.window[0] * p.z >= 0.0f) out_u0 = false;
+    if (c.window[2] * p.z - p.x >= 0.0f) out_u1 = false;
+    if (p.y - c.window[1] * p.z >= 0.0f) out_v0 = false;
+    if (c.window[3] * p.z - p.y >= 0.0f) out_v1 = false;
+  }
+  if (far_out || near_out || out_u0 || out_u1 || out_v0 || out_v1 || below) return false;
+  return in_bound(c, (lo + hi) * 0.5f, 0.5f * length(hi - lo));
+}
+
+// ---- polygons and the accumulation rasterizer (WGSL :257-451)
+struct Poly3 {
+  V3 v[10];
+  uint32_t n;
+};
+struct Poly2 {
+  V2 v[10];
+  uint32_t n;
+};
+
+// A clip keeps at most 10 vertices (the reference's MAX_POLY, WGSL :11); in
+// exact arithmetic these clips never produce more.
+static Poly3 clip_plane3(const Poly3& p, V3 normal, float offset) {
+  Poly3 out;
+  out.n = 0;
+  for (uint32_t i = 0; i < p.n; i++) {
+    V3 a = p.v[i];
+    V3 b = p.v[(i + 1) % p.n];
+    float da = offset - dot(normal, a);
+    float db = offset - dot(normal, b);
+    if (da >= 0.0f && out.n < 10) out.v[out.n++] = a;
+    if (da >= 0.0f && db < 0.0f && out.n < 10) out.v[out.n++] = a + (b - a) * (da / (da - db));
+    if (da < 0.0f && db >= 0.0f && out.n < 10) out.v[out.n++] = b + (a - b) * (db / (db - da));
+  }
+  return out;
+}
+
+static Poly2 clip_line2(const Poly2& p, V2 normal, float offset) {
+  Poly2 out;
+  out.n = 0;
+  for (uint32_t i = 0; i < p.n; i++) {
+    V2 a = p.v[i];
+    V2 b = p.v[(i + 1) % p.n];
+    float da = offset - dot2(normal, a);
+    float db = offset - dot2(normal, b);
+    if (da >= 0.0f && out.n < 10) out.v[out.n++] = a;
+    if (da >= 0.0f && db < 0.0f && out.n < 10) out.v[out.n++] = a + (b - a) * (da / (da - db));
+    if (da < 0.0f && db >= 0.0f && out.n < 10) out.v[out.n++] = b + (a - b) * (db / (db - da));
+  }
+  return out;
+}
+
+struct Acc {
+  float a[68];
+  uint32_t span_lo, span_hi;
+};
+
+// WGSL :297-348
+static void draw_edge(Acc& acc, V2 q0, V2 q1, float weight) {
+  if (std::fabs(q0.y - q1.y) <= 1.1920929e-7f) return;
+  float dir = 1.0f;
+  V2 p0 = q0, p1 = q1;
+  if (q0.y > q1.y) {
+    dir = -1.0f;
+    p0 = q1;
+    p1 = q0;
+  }
+  float dxdy = (p1.x - p0.x) / (p1.y - p0.y);
+  float x = p0.x;
+  uint32_t y_end = std::min((uint32_t)std::ceil(p1.y), GRID);
+  for (uint32_t y = (uint32_t)p0.y; y < y_end; y++) {
+    uint32_t row = y * GRID;
+    acc.span_lo = std::min(acc.span_lo, row);
+    acc.span_hi = std::max(acc.span_hi, row + GRID + 1u);
+    float dy = fmin_((float)(y + 1u), p1.y) - fmax_((float)y, p0.y);
+    float x_next = clampf(x + dxdy * dy, 0.0f, (float)GRID);
+    float d = dy * dir * weight;
+    float x0 = fmin_(x, x_next);
+    float x1 = fmax_(x, x_next);
+    float x0_floor = std::floor(x0);
+    uint32_t x0i = (uint32_t)x0_floor;
+    float x1_ceil = std::ceil(x1);
+    uint32_t x1i = (uint32_t)x1_ceil;
+    if (x1i <= x0i + 1u) {
+      float xmf = 0.5f * (x + x_next) - x0_floor;
+      acc.a[row + x0i] += d - d * xmf;
+      acc.a[row + x0i + 1u] += d * xmf;
+    } else {
+      float s = 1.0f / (x1 - x0);
+      float x0f = x0 - x0_floor;
+      float a0 = 0.5f * s * (1.0f - x0f) * (1.0f - x0f);
+      float x1f = x1 - x1_ceil + 1.0f;
+      float am = 0.5f * s * x1f * x1f;
+      acc.a[row + x0i] += d * a0;
+      if (x1i == x0i + 2u) {
+        acc.a[row + x0i + 1u] += d * (1.0f - a0 - am);
+      } else {
+        float a1 = s * (1.5f - x0f);
+        acc.a[row + x0i + 1u] += d * (a1 - a0);
+        for (uint32_t xi = x0i + 2u; xi < x1i - 1u; xi++) acc.a[row + xi] += d * s;
+        float a2 = a1 + (float)(x1i - x0i - 3u) * s;
+        acc.a[row + x1i - 1u] += d * (1.0f - a2 - am);
+      }
+      acc.a[row + x1i] += d * am;
+    }
+    x = x_next;
+  }
+}
+
+static V2 clamp2(V2 v, float g) { return V2{clampf(v.x, 0.0f, g), clampf(v.y, 0.0f, g)}; }
+
+static void draw_poly(Acc& acc, const Poly2& q, float weight) {
+  const float g = (float)GRID;
+  for (uint32_t i = 0; i < q.n; i++) {
+    V2 a = clamp2(q.v[i], g);
+    V2 b = clamp2(q.v[(i + 1) % q.n], g);
+    draw_edge(acc, a, b, weight);
+  }
+}
+
+// WGSL :350-451
+static bool rasterize(const Cone& c, Query& qs, Acc& acc, V3 v0, V3 v1, V3 v2) {
+  Poly3 p;
+  p.v[0] = v0;
+  p.v[1] = v1;
+  p.v[2] = v2;
+  p.n = 3;
+  float f0 = c.far_offset - dot(c.far_normal, v0);
+  float f1 = c.far_offset - dot(c.far_normal, v1);
+  float f2 = c.far_offset - dot(c.far_normal, v2);
+  if (f0 < 0.0f || f1 < 0.0f || f2 < 0.0f) p = clip_plane3(p, c.far_normal, c.far_offset);
+  bool all_near_ok = true, all_in_window = true;
+  const float* w = c.window;
+  for (uint32_t i = 0; i < p.n; i++) {
+    p.v[i] = to_local(c, p.v[i]);
+    V3 v = p.v[i];
+    if (v.z < NEAR) all_near_ok = false;
+    if (v.x < w[0] * v.z || v.x > w[2] * v.z || v.y < w[1] * v.z || v.y > w[3] * v.z) all_in_window = false;
+  }
+  if (!all_near_ok) p = clip_plane3(p, V3{0.0f, 0.0f, -1.0f}, -NEAR);
+  if (p.n < 3) return false;
+  float su = (float)GRID / (w[2] - w[0]);
+  float sv = (float)GRID / (w[3] - w[1]);
+  const float g = (float)GRID;
+  Poly2 q;
+  if (all_near_ok && all_in_window) {
+    q.n = p.n;
+    for (uint32_t i = 0; i < p.n; i++)
+      q.v[i] = V2{(p.v[i].x / p.v[i].z - w[0]) * su, (p.v[i].y / p.v[i].z - w[1]) * sv};
+  } else {
+    float d = dot(to_local(c, v0), cross(to_local(c, v1), to_local(c, v2)));
+    if (d == 0.0f) return false;
+    float side = d > 0.0f ? 1.0f : -1.0f;
+    q.n = 4;
+    q.v[0] = V2{0.0f, 0.0f};
+    q.v[2] = V2{g, g};
+    if (side > 0.0f) {
+      q.v[1] = V2{g, 0.0f};
+      q.v[3] = V2{0.0f, g};
+    } else {
+      q.v[1] = V2{0.0f, g};
+      q.v[3] = V2{g, 0.0f};
+    }
+    for (uint32_t i = 0; i < p.n; i++) {
+      V3 e = cross(p.v[i], p.v[(i + 1) % p.n]);
+      float len = length(e);
+      if (len == 0.0f) continue;
+      V3 k = e * (side / len);
+      q = clip_line2(q, V2{-k.x / su, -k.y / sv}, k.x * w[0] + k.y * w[1] + k.z);
+    }
+  }
+  bool inside = true;
+  for (uint32_t i = 0; i < q.n; i++)
+    if (q.v[i].x < 0.0f || q.v[i].y < 0.0f || q.v[i].x > g || q.v[i].y > g) inside = false;
+  if (!inside) {
+    q = clip_line2(q, V2{-1.0f, 0.0f}, 0.0f);
+    q = clip_line2(q, V2{1.0f, 0.0f}, g);
+    q = clip_line2(q, V2{0.0f, -1.0f}, 0.0f);
+    q = clip_line2(q, V2{0.0f, 1.0f}, g);
+  }
+  if (q.n < 3) return false;
+  draw_poly(acc, q, qs.tri_weight);
+  return true;
+}
+
+// ---- prefiltered splats (WGSL :455-574)
+static float cov_sample(const Scene& s, uint32_t n, int i, int j, int k) {
+  return s.blas_cov[n * 27u + (uint32_t)(9 * (i + 1) + 3 * (j + 1) + (k + 1))];
+}
+
+// WGSL :462-486
+static float node_front_area(const Scene& s, uint32_t n, V3 v) {
+  V3 a = vabs(v);
+  uint32_t axis = 0;
+  if (a.y > a.x && a.y >= a.z)
+    axis = 1;
+  else if (a.z > a.x && a.z > a.y)
+    axis = 2;
+  float mainv, sv, tv;
+  if (axis == 0) {
+    mainv = v.x;
+    sv = v.y / a.x;
+    tv = v.z / a.x;
+  } else if (axis == 1) {
+    mainv = v.y;
+    sv = v.x / a.y;
+    tv = v.z / a.y;
+  } else {
+    mainv = v.z;
+    sv = v.x / a.z;
+    tv = v.y / a.z;
+  }
+  int m = mainv >= 0.0f ? 1 : -1;
+  int s0 = sv < 0.0f ? -1 : 0;
+  int t0 = tv < 0.0f ? -1 : 0;
+  float fs = sv - (float)s0;
+  float ft = tv - (float)t0;
+  float f[4];
+  for (int c = 0; c < 4; c++) {
+    int gs = s0 + (c & 1);
+    int gt = t0 + (c >> 1);
+    if (axis == 0)
+      f[c] = cov_sample(s, n, m, gs, gt);
+    else if (axis == 1)
+      f[c] = cov_sample(s, n, gs, m, gt);
+    else
+      f[c] = cov_sample(s, n, gs, gt, m);
+  }
+  return mixf(mixf(f[0], f[1], fs), mixf(f[2], f[3], fs), ft);
+}
+
+// WGSL :490-512
+static Poly2 hull8(const V2 pts[8]) {
+  uint32_t start = 0;
+  for (uint32_t k = 1; k < 8; k++)
+    if (pts[k].x < pts[start].x || (pts[k].x == pts[start].x && pts[k].y < pts[start].y)) start = k;
+  Poly2 out;
+  out.n = 0;
+  uint32_t cur = start;
+  for (;;) {
+    out.v[out.n++] = pts[cur];
+    uint32_t next = (cur + 1) % 8;
+    for (uint32_t k = 0; k < 8; k++) {
+      V2 e = pts[next] - pts[cur];
+      V2 f = pts[k] - pts[cur];
+      float cr = e.x * f.y - e.y * f.x;
+      if (cr > 0.0f || (cr == 0.0f && dot2(f, f) > dot2(e, e))) next = k;
+    }
+    cur = next;
+    if (cur == start || out.n >= 8) break;
+  }
+  return out;
+}
+
+static Poly2 clip_to_grid(Poly2 q) {
+  const float g = (float)GRID;
+  q = clip_line2(q, V2{-1.0f, 0.0f}, 0.0f);
+  q = clip_line2(q, V2{1.0f, 0.0f}, g);
+  q = clip_line2(q, V2{0.0f, -1.0f}, 0.0f);
+  q = clip_line2(q, V2{0.0f, 1.0f}, g);
+  return q;
+}
+
+// WGSL :519-574: 0 culled, 1 splatted with nothing drawn, 2 splatted and drawn, 3 descend.
+static uint32_t node_cone_test(const Cone& c, Query& qs, Acc& acc, uint32_t n, V3 lo, V3 hi) {
+  const float g = (float)GRID;
+  V2 pts[8];
+  V2 mn{3.4e38f, 3.4e38f}, mx{-3.4e38f, -3.4e38f};
+  bool far_out = true, near_out = true, out_u0 = true, out_u1 = true, out_v0 = true, out_v1 = true,
+       below = true;
+  bool inside = qs.prefilter_bins > 0.0f;
+  for (uint32_t k = 0; k < 8; k++) {
+    V3 corner = box_corner(lo, hi, k);
+    if (dot(c.horizon, corner - c.apex) >= 0.0f) below = false;
+    bool in_far = c.far_offset - dot(c.far_normal, corner) >= 0.0f;
+    if (in_far)
+      far_out = false;
+    else
+      inside = false;
+    V3 p = to_local(c, corner);
+    if (p.z >= NEAR)
+      near_out = false;
+    else
+      inside = false;
+    if (p.x - c.window[0] * p.z >= 0.0f) out_u0 = false;
+    if (c.window[2] * p.z - p.x >= 0.0f) out_u1 = false;
+    if (p.y - c.window[1] * p.z >= 0.0f) out_v0 = false;
+    if (c.window[3] * p.z - p.y >= 0.0f) out_v1 = false;
+    if (inside) {
+      V2 q{(p.x / p.z - c.window[0]) / (c.window[2] - c.window[0]) * g,
+           (p.y / p.z - c.window[1]) / (c.window[3] - c.window[1]) * g};
+      pts[k] = q;
+      mn = V2{fmin_(mn.x, q.x), fmin_(mn.y, q.y)};
+      mx = V2{fmax_(mx.x, q.x), fmax_(mx.y, q.y)};
+    }
+  }
+  if (far_out || near_out || out_u0 || out_u1 || out_v0 || out_v1 || below) return 0;
+  if (!in_bound(c, (lo + hi) * 0.5f, 0.5f * length(hi - lo))) return 0;
+  if (!inside || fmax_(mx.x - mn.x, mx.y - mn.y) > qs.prefilter_bins) return 3;
+  if (mx.x <= 0.0f || mx.y <= 0.0f || mn.x >= g || mn.y >= g) return 1;
+  V3 e = hi - lo;
+  V3 v = normalize(c.apex - (lo + hi) * 0.5f);
+  float silhouette = std::fabs(v.x) * e.y * e.z + std::fabs(v.y) * e.x * e.z + std::fabs(v.z) * e.x * e.y;
+  if (silhouette <= 1e-30f) return 1;
+  float alpha = node_front_area(*qs.s, n, v) / silhouette;
+  if (alpha <= 0.0f) return 1;
+  Poly2 q = clip_to_grid(hull8(pts));
+  if (q.n < 3) return 1;
+  draw_poly(acc, q, alpha);
+  return 2;
+}
+
+// WGSL :594-616 (opaque triangles: no transmission)
+static bool cover_triangle(const Cone& c, Query& qs, Acc& acc, uint32_t t) {
+  if (qs.o->profile_stage == 4) return false;
+  const float* tr = qs.s->tris + 12u * t;
+  V3 v0 = xyz(tr), v1 = xyz(tr + 4), v2 = xyz(tr + 8);
+  V3 n = cross(v1 - v0, v2 - v0);
+  if (dot(n, c.apex - v0) <= 0.0f) return false;
+  if (dot(n, direction(c, 0.0f, 0.0f)) >= 0.0f && dot(n, direction(c, 1.0f, 0.0f)) >= 0.0f &&
+      dot(n, direction(c, 0.0f, 1.0f)) >= 0.0f && dot(n, direction(c, 1.0f, 1.0f)) >= 0.0f)
+    return false;
+  if (!may_touch(c, v0, v1, v2)) return false;
+  return rasterize(c, qs, acc, v0, v1, v2);
+}
+
+// WGSL :623-638
+static Cone cone_to_object(const Scene& s, const Cone& c, uint32_t m) {
+  uint32_t x = s.ranges[4 * m + 3];
+  if (x == NONE) return c;
+  Cone o;
+  o.apex = point_to_object(s, m, c.apex);
+  o.right = rotate_to_object(s, x, c.right);
+  o.up = rotate_to_object(s, x, c.up);
+  o.forward = rotate_to_object(s, x, c.forward);
+  for (int k = 0; k < 4; k++) o.window[k] = c.window[k];
+  o.far_normal = rotate_to_object(s, x, c.far_normal);
+  o.far_offset = (c.far_offset - dot(c.far_normal, translation(s, x))) * s.xforms[16 * x + 13];
+  o.horizon = rotate_to_object(s, x, c.horizon);
+  V3 b = rotate_to_object(s, x, xyz(c.bound));
+  o.bound[0] = b.x;
+  o.bound[1] = b.y;
+  o.bound[2] = b.z;
+  o.bound[3] = c.bound[3];
+  o.bound_sin = c.bound_sin;
+  return o;
+}
+
+// Combines one mesh's accumulation buffer into `cov` (WGSL :809-828).
+static void combine_mesh(const Acc& acc, float* cov, uint32_t& full, bool snap) {
+  float running = 0.0f;
+  uint32_t end = std::min(acc.span_hi, BINS);
+  for (uint32_t b = acc.span_lo; b < end; b++) {
+    running += acc.a[b];
+    float covered = std::fabs(running);
+    if (snap) {
+      if (covered >= ONE_MINUS_1EM5) covered = 1.0f;
+    } else {
+      covered = fmin_(covered, 1.0f);
+    }
+    if (covered > 1e-6f) {
+      float before = cov[b];
+      cov[b] = fmax_(before, fmin_(covered, 1.0f));
+      if (before < 1.0f && cov[b] >= 1.0f) full++;
+    }
+  }
+}
+
+// WGSL :719-829 (one invocation alone: `on` is its own packet)
+static void add_mesh_coverage(const Cone& world_cone, Query& qs, uint32_t m, float* cov, uint32_t& full,
+                              uint32_t base) {
+  if (qs.o->profile_stage == 3) return;
+  const Scene& s = *qs.s;
+  Cone c = cone_to_object(s, world_cone, m);
+  const uint32_t* range = s.ranges + 4 * m;
+  qs.tri_weight = 1.0f;
+  if (range[3] != NONE) {
+    float clear = s.xforms[16 * range[3] + 14];
+    qs.tri_weight = 1.0f - fmax_(clear, 0.0f);
+  }
+  Acc acc;
+  for (int b = 0; b < 68; b++) acc.a[b] = 0.0f;
+  acc.span_lo = 68;
+  acc.span_hi = 0;
+  bool touched = false;
+  if (qs.o->hierarchy == 2) {
+    if (range[2] != NONE) {
+      uint32_t sp = base + 1;
+      if (base >= STACK) {
+        qs.stack_overflow = true;
+        return;
+      }
+      qs.stack[base] = range[2];
+      while (sp > base) {
+        sp--;
+        uint32_t n = qs.stack[sp];
+        bool mine = qs.cone_work < qs.cone_cap;
+        if (mine) qs.cone_work++;
+        if (!mine) continue;
+        const Node& node = s.blas[n];
+        uint32_t test = node_cone_test(c, qs, acc, n, xyz(node.lo), xyz(node.hi));
+        if (test == 2) touched = true;
+        if (test != 3) continue;
+        if (node.info[1] > 0) {
+          qs.cone_work += node.info[1];
+          for (uint32_t k = node.info[0]; k < node.info[0] + node.info[1]; k++)
+            if (cover_triangle(c, qs, acc, s.blas_items[k])) touched = true;
+        } else {
+          if (sp + 2 > STACK) {
+            qs.stack_overflow = true;
+            return;
+          }
+          qs.stack[sp] = node.info[0];
+          qs.stack[sp + 1] = node.info[0] + 1;
+          sp += 2;
+        }
+      }
+    }
+  } else {
+    for (uint32_t t = range[0]; t < range[0] + range[1]; t++)
+      if (cover_triangle(c, qs, acc, t)) touched = true;
+  }
+  if (touched) combine_mesh(acc, cov, full, true);
+}
+
+// WGSL :1096-1150 (object-level splat; light cones combine by the maximum)
+static bool splat_object_node(const Cone& c, Query& qs, uint32_t n, V3 lo, V3 hi, float* cov, uint32_t& full);
+
+// WGSL :1160-1227
+static void scene_coverage(const Cone& c, Query& qs, float* cov, uint32_t& full) {
+  const Scene& s = *qs.s;
+  qs.cone_work = 0;
+  if (qs.o->hierarchy >= 1) {
+    uint32_t sp = 1;
+    qs.stack[0] = 0;
+    while (sp > 0) {
+      sp--;
+      uint32_t n = qs.stack[sp];
+      bool mine = full < BINS && qs.cone_work < qs.cone_cap;
+      if (mine) qs.cone_work++;
+      if (!mine) continue;
+      const Node& node = s.nodes[n];
+      bool descend = false;
+      if (may_touch_box(c, xyz(node.lo), xyz(node.hi))) {
+        if (node.info[2] == 0 && splat_object_node(c, qs, n, xyz(node.lo), xyz(node.hi), cov, full)) {
+        } else {
+          descend = true;
+        }
+      }
+      if (!descend) continue;
+      if (node.info[1] > 0) {
+        for (uint32_t k = node.info[0]; k < node.info[0] + node.info[1]; k++)
+          add_mesh_coverage(c, qs, s.items[k], cov, full, sp);
+      } else {
+        uint32_t nearc = node.info[0], farc = node.info[0] + 1;
+        const Node& a = s.nodes[nearc];
+        const Node& b = s.nodes[farc];
+        bool use_bound = c.bound[3] >= -1.0f && (c.bound[0] != 0.0f || c.bound[1] != 0.0f || c.bound[2] != 0.0f);
+        V3 axis = use_bound ? xyz(c.bound) : c.forward;
+        float da = dot(axis, (xyz(a.lo) + xyz(a.hi)) * 0.5f - c.apex);
+        float db = dot(axis, (xyz(b.lo) + xyz(b.hi)) * 0.5f - c.apex);
+        if (db < da) {
+          nearc = node.info[0] + 1;
+          farc = node.info[0];
+        }
+        if (sp + 2 > STACK) {
+          qs.stack_overflow = true;
+          return;
+        }
+        qs.stack[sp] = farc;
+        qs.stack[sp + 1] = nearc;
+        sp += 2;
+      }
+    }
+  } else {
+    for (uint32_t m = 0; m < s.n_objects; m++)
+      if (full < BINS) add_mesh_coverage(c, qs, m, cov, full, 0);
+  }
+}
+
+// Cephes expf (S. L. Moshier): x = n ln2 + r with ln2 split in two parts,
+// then a degree-6 polynomial for e^r.
+static float exp_c(float x) {
+  if (x > 88.72283905206835f) return INFINITY;
+  if (x < -103.278929903431851103f) return 0.0f;
+  float z = std::floor(1.44269504088896341f * x + 0.5f);
+  x = x - z * 0.693359375f;
+  x = x - z * -2.12194440e-4f;
+  int n = (int)z;
+  float xx = x * x;
+  float p = ((((((1.9875691500e-4f * x + 1.3981999507e-3f) * x + 8.3334519073e-3f) * x + 4.1665795894e-2f) * x +
+               1.6666665459e-1f) *
+                  x +
+              5.0000001201e-1f) *
+                 xx +
+             x) +
+            1.0f;
+  return std::ldexp(p, n);
+}
+
+static bool splat_object_node(const Cone& c, Query& qs, uint32_t n, V3 lo, V3 hi, float* cov, uint32_t& full) {
+  if (qs.o->object_cov == NONE || qs.prefilter_bins <= 0.0f) return false;
+  const float g = (float)GRID;
+  V2 pts[8];
+  V2 mn{3.4e38f, 3.4e38f}, mx{-3.4e38f, -3.4e38f};
+  for (uint32_t k = 0; k < 8; k++) {
+    V3 corner = box_corner(lo, hi, k);
+    if (c.far_offset - dot(c.far_normal, corner) < 0.0f) return false;
+    V3 p = to_local(c, corner);
+    if (p.z < NEAR) return false;
+    V2 q{(p.x / p.z - c.window[0]) / (c.window[2] - c.window[0]) * g,
+         (p.y / p.z - c.window[1]) / (c.window[3] - c.window[1]) * g};
+    pts[k] = q;
+    mn = V2{fmin_(mn.x, q.x), fmin_(mn.y, q.y)};
+    mx = V2{fmax_(mx.x, q.x), fmax_(mx.y, q.y)};
+  }
+  if (fmax_(mx.x - mn.x, mx.y - mn.y) > qs.prefilter_bins) return false;
+  if (mx.x <= 0.0f || mx.y <= 0.0f || mn.x >= g || mn.y >= g) return true;
+  V3 e = hi - lo;
+  V3 v = normalize(c.apex - (lo + hi) * 0.5f);
+  float silhouette = std::fabs(v.x) * e.y * e.z + std::fabs(v.y) * e.x * e.z + std::fabs(v.z) * e.x * e.y;
+  if (silhouette <= 1e-30f) return true;
+  float alpha = 1.0f - exp_c(-node_front_area(*qs.s, qs.o->object_cov + n, v) / silhouette);
+  if (alpha <= 0.0f) return true;
+  Poly2 q = clip_to_grid(hull8(pts));
+  if (q.n < 3) return true;
+  Acc acc;
+  for (int b = 0; b < 68; b++) acc.a[b] = 0.0f;
+  acc.span_lo = 68;
+  acc.span_hi = 0;
+  draw_poly(acc, q, alpha);
+  combine_mesh(acc, cov, full, false);
+  return true;
+}
+
+// ---- weights (WGSL :1232-1272, :1331-1410)
+static float bin_weight(const Cone& c, V3 normal, uint32_t b) {
+  float col = (float)(b % GRID), row = (float)(b / GRID), g = (float)GRID;
+  V3 d0 = direction(c, col / g, row / g);
+  V3 d1 = direction(c, (col + 1.0f) / g, row / g);
+  V3 d2 = direction(c, (col + 1.0f) / g, (row + 1.0f) / g);
+  V3 d3 = direction(c, col / g, (row + 1.0f) / g);
+  return std::fabs(0.5f * (lambert_edge(normal, d0, d1) + lambert_edge(normal, d1, d2) +
+                           lambert_edge(normal, d2, d3) + lambert_edge(normal, d3, d0)));
+}
+
+static float clipped_quad_weight(V3 normal, const V3 d[4], bool allow_full) {
+  V3 poly[8];
+  uint32_t n = 0;
+  for (uint32_t k = 0; k < 4; k++) {
+    V3 a = d[k], e = d[(k + 1) % 4];
+    float sa = dot(normal, a), se = dot(normal, e);
+    if (sa >= 0.0f) poly[n++] = a;
+    if ((sa >= 0.0f) != (se >= 0.0f)) poly[n++] = a + (e - a) * (sa / (sa - se));
+  }
+  if (!allow_full && n < 3) return 0.0f;
+  float sum = 0.0f;
+  for (uint32_t k = 0; k < n; k++) sum += lambert_edge(normal, poly[k], poly[(k + 1) % n]);
+  return std::fabs(0.5f * sum);
+}
+
+static float clipped_bin_weight(const Cone& c, V3 normal, uint32_t b) {
+  float col = (float)(b % GRID), row = (float)(b / GRID), g = (float)GRID;
+  V3 d[4];
+  d[0] = direction(c, col / g, row / g);
+  d[1] = direction(c, (col + 1.0f) / g, row / g);
+  d[2] = direction(c, (col + 1.0f) / g, (row + 1.0f) / g);
+  d[3] = direction(c, col / g, (row + 1.0f) / g);
+  uint32_t above = 0;
+  for (uint32_t k = 0; k < 4; k++)
+    if (dot(normal, d[k]) > 0.0f) above++;
+  if (above == 4) return bin_weight(c, normal, b);
+  if (above == 0) return 0.0f;
+  return clipped_quad_weight(normal, d, true);
+}
+
+static float clipped_window_weight(const Cone& c, V3 normal) {
+  V3 d[4];
+  d[0] = direction(c, 0.0f, 0.0f);
+  d[1] = direction(c, 1.0f, 0.0f);
+  d[2] = direction(c, 1.0f, 1.0f);
+  d[3] = direction(c, 0.0f, 1.0f);
+  return clipped_quad_weight(normal, d, false);
+}
+
+static float cone_visible(const Cone& c, V3 normal, const float* cov) {
+  V3 d00 = direction(c, 0.0f, 0.0f), d10 = direction(c, 1.0f, 0.0f);
+  V3 d11 = direction(c, 1.0f, 1.0f), d01 = direction(c, 0.0f, 1.0f);
+  if (dot(normal, d00) > 0.0f && dot(normal, d10) > 0.0f && dot(normal, d11) > 0.0f && dot(normal, d01) > 0.0f) {
+    float blocked = 0.0f;
+    for (uint32_t b = 0; b < BINS; b++) {
+      float covered = cov[b];
+      if (covered > 0.0f) blocked += covered * bin_weight(c, normal, b);
+    }
+    float total = std::fabs(0.5f * (lambert_edge(normal, d00, d10) + lambert_edge(normal, d10, d11) +
+                                    lambert_edge(normal, d11, d01) + lambert_edge(normal, d01, d00)));
+    return fmax_(total - blocked, 0.0f);
+  }
+  float blocked = 0.0f;
+  for (uint32_t b = 0; b < BINS; b++) {
+    float covered = cov[b];
+    if (covered > 0.0f) blocked += covered * clipped_bin_weight(c, normal, b);
+  }
+  return fmax_(clipped_window_weight(c, normal) - blocked, 0.0f);
+}
+
+static float window_open(const Cone& c, V3 normal) {
+  V3 d00 = direction(c, 0.0f, 0.0f), d10 = direction(c, 1.0f, 0.0f);
+  V3 d11 = direction(c, 1.0f, 1.0f), d01 = direction(c, 0.0f, 1.0f);
+  if (dot(normal, d00) > 0.0f && dot(normal, d10) > 0.0f && dot(normal, d11) > 0.0f && dot(normal, d01) > 0.0f)
+    return std::fabs(0.5f * (lambert_edge(normal, d00, d10) + lambert_edge(normal, d10, d11) +
+                             lambert_edge(normal, d11, d01) + lambert_edge(normal, d01, d00)));
+  return clipped_window_weight(c, normal);
+}
+
+// ---- lights (WGSL :850-1042, :2125-2190)
+static bool is_sphere(const Light& r) { return r.corner[3] > 1.5f; }
+
+static V3 light_center(const Light& r) {
+  if (is_sphere(r)) return xyz(r.corner);
+  return xyz(r.corner) + (xyz(r.edge_u) + xyz(r.edge_v)) * 0.5f;
+}
+
+static float spot_falloff(const Light& r, V3 w) {
+  if (!is_sphere(r) || r.edge_v[3] < 0.5f) return 1.0f;
+  float a = clampf(dot(xyz(r.edge_v), w) * r.edge_u[1] + r.edge_u[2], 0.0f, 1.0f);
+  return a * a;
+}
+
+static bool faces_light(const Light& r, V3 point, V3 normal) {
+  V3 to_light = light_center(r) - point;
+  if (is_sphere(r)) return dot(normal, to_light) > -r.edge_u[0];
+  return dot(normal, to_light) > 0.0f;
+}
+
+// Share of each bin of a sphere light's window inside the inscribed disk:
+// the exact area of the circle of radius 4 (bin units) about the window's
+// centre within each bin, rounded to 6 decimals (WGSL :888-897).
+static const float DISK[64] = {
+    0.000000f, 0.160355f, 0.694958f, 0.957934f, 0.957934f, 0.694958f, 0.160355f, 0.000000f,
+    0.160355f, 0.939877f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 0.939877f, 0.160355f,
+    0.694958f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 0.694958f,
+    0.957934f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 0.957934f,
+    0.957934f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 0.957934f,
+    0.694958f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 0.694958f,
+    0.160355f, 0.939877f, 1.000000f, 1.000000f, 1.000000f, 1.000000f, 0.939877f, 0.160355f,
+    0.000000f, 0.160355f, 0.694958f, 0.957934f, 0.957934f, 0.694958f, 0.160355f, 0.000000f,
+};
+
+struct LightCone {
+  Cone c;
+  bool ok;
+};
+
+static Cone zero_cone() {
+  Cone c;
+  std::memset(&c, 0, sizeof(c));
+  return c;
+}
+
+// WGSL :902-931
+static LightCone light_cone_sphere(Query& qs, V3 apex, const Light& r) {
+  LightCone out;
+  out.c = zero_cone();
+  out.ok = false;
+  qs.prefilter_bins = qs.o->prefilter;
+  V3 to_center = xyz(r.corner) - apex;
+  float d = length(to_center);
+  float radius = r.edge_u[0];
+  if (d <= radius * 1.0001f) return out;
+  V3 forward = to_center / d;
+  float s = radius / d;
+  float t = s / std::sqrt(fmax_(1.0f - s * s, 1e-12f));
+  qs.wide_clipped = 2.0f * t > qs.o->wide_extent;
+  qs.wide_window = qs.wide_clipped;
+  t = fmin_(t, 0.5f * qs.o->wide_extent);
+  V3 helper = std::fabs(forward.x) > 0.9f ? V3{0.0f, 1.0f, 0.0f} : V3{1.0f, 0.0f, 0.0f};
+  V3 right = normalize(cross(helper, forward));
+  out.c.apex = apex;
+  out.c.right = right;
+  out.c.up = cross(forward, right);
+  out.c.forward = forward;
+  out.c.window[0] = -t;
+  out.c.window[1] = -t;
+  out.c.window[2] = t;
+  out.c.window[3] = t;
+  out.c.far_normal = forward;
+  out.c.far_offset = dot(forward, apex) + (d - radius * s) * ONE_MINUS_1EM5;
+  out.c = bound_cone(out.c, forward);
+  out.ok = true;
+  return out;
+}
+
+// WGSL :937-955
+static float sphere_unshadowed(V3 point, V3 normal, const Light& r) {
+  V3 to_center = xyz(r.corner) - point;
+  float d2 = dot(to_center, to_center);
+  float radius = r.edge_u[0];
+  if (d2 <= radius * radius) return 0.0f;
+  float sin2 = radius * radius / d2;
+  float cos_t = clampf(dot(normal, to_center) * (1.0f / std::sqrt(d2)), -1.0f, 1.0f);
+  if (cos_t * cos_t > sin2) return 3.14159265f * sin2 * fmax_(cos_t, 0.0f);
+  float sin_t = std::sqrt(1.0f - cos_t * cos_t);
+  float x = std::sqrt(1.0f / sin2 - 1.0f);
+  float y = clampf(-x * cos_t / sin_t, -1.0f, 1.0f);
+  float q = sin_t * std::sqrt(1.0f - y * y);
+  return fmax_((cos_t * acos_c(y) - x * q) * sin2 + atan_c(q / x), 0.0f);
+}
+
+// WGSL :960-972
+static float sphere_open_share(const Query& qs, const Cone& c, V3 normal, const float* cov) {
+  float total = 0.0f, open = 0.0f;
+  const float g = (float)GRID;
+  for (uint32_t b = 0; b < BINS; b++) {
+    float m = qs.wide_clipped ? 1.0f : DISK[b];
+    if (m <= 0.0f) continue;
+    V3 d = direction(c, ((float)(b % GRID) + 0.5f) / g, ((float)(b / GRID) + 0.5f) / g);
+    float l2 = dot(d, d);
+    float w = m * fmax_(dot(normal, d), 0.0f) / (l2 * l2);
+    total += w;
+    open += w * (1.0f - cov[b]);
+  }
+  if (total <= 0.0f) return 1.0f;
+  return open / total;
+}
+
+// WGSL :974-1042
+static LightCone light_cone_rect(Query& qs, V3 apex, const Light& r) {
+  if (is_sphere(r)) return light_cone_sphere(qs, apex, r);
+  LightCone out;
+  out.c = zero_cone();
+  out.ok = false;
+  qs.prefilter_bins = qs.o->prefilter;
+  V3 corner0 = xyz(r.corner), edge_u = xyz(r.edge_u), edge_v = xyz(r.edge_v);
+  V3 forward = normalize(cross(edge_u, edge_v));
+  if (r.corner[3] > 0.5f && dot(forward, apex - corner0) <= 0.0f) return out;
+  if (dot(forward, corner0 - apex) < 0.0f) forward = -forward;
+  float dist = dot(forward, corner0 - apex);
+  if (dist <= 1e-6f) return out;
+  V3 right = normalize(edge_u);
+  V3 up = cross(forward, right);
+  V3 rel = corner0 - apex;
+  V2 lo{3.4e38f, 3.4e38f}, hi{-3.4e38f, -3.4e38f};
+  for (uint32_t k = 0; k < 4; k++) {
+    V3 corner = rel + edge_u * (float)(k & 1u) + edge_v * (float)((k >> 1u) & 1u);
+    V2 uv{dot(right, corner) / dist, dot(up, corner) / dist};
+    lo = V2{fmin_(lo.x, uv.x), fmin_(lo.y, uv.y)};
+    hi = V2{fmax_(hi.x, uv.x), fmax_(hi.y, uv.y)};
+  }
+  V2 extent = hi - lo;
+  V2 foot{fmin_(fmax_(0.0f, lo.x), hi.x), fmin_(fmax_(0.0f, lo.y), hi.y)};
+  V3 mid = normalize(V3{0.5f * (lo.x + hi.x), 0.5f * (lo.y + hi.y), 1.0f});
+  float cos_wide = 1.0f;
+  for (uint32_t k = 0; k < 4; k++) {
+    V3 q{(k & 1u) ? hi.x : lo.x, (k & 2u) ? hi.y : lo.y, 1.0f};
+    cos_wide = fmin_(cos_wide, dot(mid, normalize(q)));
+  }
+  float size = qs.o->wide_extent;
+  qs.wide_window = extent.x > size || extent.y > size;
+  qs.wide_clipped = qs.wide_window && cos_wide < 0.8660254f;
+  if (qs.wide_clipped) {
+    bool wx = extent.x > size, wy = extent.y > size;
+    V2 keep{clampf(foot.x - 0.5f * size, lo.x, hi.x - size), clampf(foot.y - 0.5f * size, lo.y, hi.y - size)};
+    V2 nlo{wx ? keep.x : lo.x, wy ? keep.y : lo.y};
+    V2 nhi{wx ? keep.x + size : hi.x, wy ? keep.y + size : hi.y};
+    lo = nlo;
+    hi = nhi;
+  }
+  out.c.apex = apex;
+  out.c.right = right;
+  out.c.up = up;
+  out.c.forward = forward;
+  out.c.window[0] = lo.x;
+  out.c.window[1] = lo.y;
+  out.c.window[2] = hi.x;
+  out.c.window[3] = hi.y;
+  out.c.far_normal = forward;
+  out.c.far_offset = dot(forward, apex) + dist * ONE_MINUS_1EM5;
+  V3 axis = normalize(rel + (edge_u + edge_v) * 0.5f);
+  if (qs.wide_clipped) axis = normalize(direction(out.c, 0.5f, 0.5f));
+  out.c = bound_cone(out.c, axis);
+  out.ok = true;
+  return out;
+}
+
+// WGSL :2148-2190
+static float rect_open(V3 point, V3 normal, const Light& r) {
+  V3 u = xyz(r.edge_u), v = xyz(r.edge_v);
+  V3 c0 = xyz(r.corner) - point;
+  float facing = dot(normalize(cross(u, v)), c0);
+  if ((r.corner[3] > 0.5f && facing >= 0.0f) || std::fabs(facing) <= 1e-6f) return 0.0f;
+  V3 d1 = c0 + u;
+  V3 d2 = c0 + u + v;
+  V3 d3 = c0 + v;
+  if (dot(normal, c0) >= 0.0f && dot(normal, d1) >= 0.0f && dot(normal, d2) >= 0.0f && dot(normal, d3) >= 0.0f) {
+    V3 a0 = normalize(c0), a1 = normalize(d1), a2 = normalize(d2), a3 = normalize(d3);
+    float sum = 0.0f;
+    sum += lambert_edge_unit(normal, a0, a1);
+    sum += lambert_edge_unit(normal, a1, a2);
+    sum += lambert_edge_unit(normal, a2, a3);
+    sum += lambert_edge_unit(normal, a3, a0);
+    return std::fabs(0.5f * sum);
+  }
+  V3 d[4] = {c0, c0 + u, c0 + u + v, c0 + v};
+  return clipped_quad_weight(normal, d, false);
+}
+
+// WGSL :1070-1076
+static float wide_visible(const Query& qs, const Cone& c, V3 apex, V3 normal, const Light& r, float visible) {
+  if (!qs.wide_clipped) return visible;
+  float kept = window_open(c, normal);
+  return (kept > 1e-9f ? visible / kept : 1.0f) * rect_open(apex, normal, r);
+}
+
+struct Result {
+  float visible;
+  uint64_t full, touched;
+  bool capped;
+  bool overflow;
+};
+
+// WGSL :1286-1321 and :2125-2128 (hit_irradiance_rect).
+static Result hit_irradiance_rect(const Scene& s, const Options& o, V3 point, V3 normal, const Light& r,
+                                  float* cov_out) {
+  Query qs;
+  qs.s = &s;
+  qs.o = &o;
+  Result res{0.0f, 0, 0, false, false};
+  float spot = spot_falloff(r, normalize(point - light_center(r)));
+  bool on = spot > 0.0f && faces_light(r, point, normal);
+  float cov[64];
+  for (int b = 0; b < 64; b++) cov[b] = 0.0f;
+  float value = 0.0f;
+  if (o.profile_stage != 1) {
+    LightCone lc = light_cone_rect(qs, point, r);
+    bool traced = on && lc.ok;
+    uint32_t full = 0;
+    qs.cone_cap = qs.wide_window ? o.wide_cone_cap : o.light_cone_cap;
+    qs.cone_work = 0;
+    if (o.profile_stage != 2 && traced) scene_coverage(lc.c, qs, cov, full);
+    if (qs.cone_work >= qs.cone_cap && qs.wide_window && !qs.wide_clipped) {
+      for (int b = 0; b < 64; b++) cov[b] = 1.0f;
+      full = BINS;
+    }
+    res.capped = qs.cone_work >= qs.cone_cap;
+    if (traced) {
+      if (is_sphere(r))
+        value = sphere_unshadowed(point, normal, r) * sphere_open_share(qs, lc.c, normal, cov);
+      else
+        value = wide_visible(qs, lc.c, point, normal, r, cone_visible(lc.c, normal, cov));
+    }
+  }
+  res.visible = spot * value;
+  for (int b = 0; b < 64; b++) {
+    if (cov[b] >= 1.0f) res.full |= (uint64_t)1 << b;
+    if (cov[b] > 0.0f) res.touched |= (uint64_t)1 << b;
+  }
+  res.overflow = qs.stack_overflow;
+  if (cov_out)
+    for (int b = 0; b < 64; b++) cov_out[b] = cov[b];
+  return res;
+}
+
+}  // namespace orc
+
+using namespace orc;
+
+static Scene make_scene(const float* tris, const uint32_t* ranges, uint32_t n_objects, const float* blas,
+                        const uint32_t* blas_items, const float* blas_cov, const float* nodes, const uint32_t* items,
+                        const float* xforms) {
+  Scene s;
+  s.tris = tris;
+  s.ranges = ranges;
+  s.n_objects = n_objects;
+  s.blas = reinterpret_cast<const Node*>(blas);
+  s.blas_items = blas_items;
+  s.blas_cov = blas_cov;
+  s.nodes = reinterpret_cast<const Node*>(nodes);
+  s.items = items;
+  s.xforms = xforms;
+  return s;
+}
+
+extern "C" {
+
+// One query per (receiver i, light l), stored at l * n + i. Receivers are
+// (point vec4, normal vec4), the reference's ReceiverGpu; one whose normal.w
+// is negative traces nothing (a pixel that sees the sky or a light). Lights
+// are the reference's LightGpu (64 B).
+// opts: [hierarchy, prefilter(f32 bits), wide_extent(f32 bits),
+// light_cone_cap, wide_cone_cap, profile_stage, object_cov]. Returns the
+// number of queries whose stack would overflow the reference's 48 entries.
+int oracle_irradiance(const float* tris, const uint32_t* ranges, uint32_t n_objects, const float* blas,
+                      const uint32_t* blas_items, const float* blas_cov, const float* nodes, const uint32_t* items,
+                      const float* xforms, const float* receivers, uint64_t n, const float* lights, uint32_t n_lights,
+                      const uint32_t* opts, float* out_visible, uint64_t* out_full, uint64_t* out_touched,
+                      float* out_cov, uint8_t* out_capped, int threads) {
+  Scene s = make_scene(tris, ranges, n_objects, blas, blas_items, blas_cov, nodes, items, xforms);
+  Options o;
+  o.hierarchy = opts[0];
+  std::memcpy(&o.prefilter, &opts[1], 4);
+  std::memcpy(&o.wide_extent, &opts[2], 4);
+  o.light_cone_cap = opts[3];
+  o.wide_cone_cap = opts[4];
+  o.profile_stage = opts[5];
+  o.object_cov = opts[6];
+  const uint64_t total = n * (uint64_t)n_lights;
+  std::atomic<uint64_t> next{0};
+  std::atomic<int> overflows{0};
+  auto work = [&]() {
+    for (;;) {
+      uint64_t q0 = next.fetch_add(256);
+      if (q0 >= total) break;
+      uint64_t q1 = std::min(total, q0 + 256);
+      for (uint64_t q = q0; q < q1; q++) {
+        uint64_t i = q % n;
+        uint32_t l = (uint32_t)(q / n);
+        const float* rc = receivers + 8 * i;
+        const Light& light = reinterpret_cast<const Light*>(lights)[l];
+        Result r{0.0f, 0, 0, false, false};
+        if (rc[7] >= 0.0f) {
+          r = hit_irradiance_rect(s, o, xyz(rc), xyz(rc + 4), light, out_cov ? out_cov + 64 * q : nullptr);
+        } else if (out_cov) {
+          for (int b = 0; b < 64; b++) out_cov[64 * q + b] = 0.0f;
+        }
+        out_visible[q] = r.visible;
+        out_full[q] = r.full;
+        out_touched[q] = r.touched;
+        if (out_capped) out_capped[q] = r.capped ? 1 : 0;
+        if (r.overflow) overflows++;
+      }
+    }
+  };
+  if (threads <= 1) {
+    work();
+  } else {
+    std::vector<std::thread> pool;
+    for (int t = 0; t < threads; t++) pool.emplace_back(work);
+    for (auto& t : pool) t.join();
+  }
+  return overflows.load();
+}
+
+// Single building blocks, for the tests that pin them.
+float oracle_atan2(float y, float x) { return atan2_c(y, x); }
+float oracle_acos(float y) { return acos_c(y); }
+float oracle_exp(float x) { return exp_c(x); }
+
+// The accumulation rasterizer on one polygon (points in bin units), for
+// closed-form checks: writes 68 accumulator cells and the coverage of the
+// 64 bins after the running sum (no snap).
+void oracle_draw_polygon(const float* pts, uint32_t n, float weight, float* acc_out, float* cov_out) {
+  Acc acc;
+  for (int b = 0; b < 68; b++) acc.a[b] = 0.0f;
+  acc.span_lo = 68;
+  acc.span_hi = 0;
+  Poly2 q;
+  q.n = n;
+  for (uint32_t i = 0; i < n; i++) q.v[i] = V2{pts[2 * i], pts[2 * i + 1]};
+  draw_poly(acc, q, weight);
+  for (int b = 0; b < 68; b++) acc_out[b] = acc.a[b];
+  float running = 0.0f;
+  for (int b = 0; b < 64; b++) {
+    running += acc.a[b];
+    cov_out[b] = running;
+  }
+}
+
+// The light cone's frame and window for one apex (for closed-form checks):
+// out = apex3 right3 up3 forward3 window4 far_normal3 far_offset bound4 bound_sin, ok.
+int oracle_light_cone(const float* apex, const float* light, const uint32_t* opts, float* out) {
+  Options o{};
+  o.hierarchy = opts[0];
+  std::memcpy(&o.prefilter, &opts[1], 4);
+  std::memcpy(&o.wide_extent, &opts[2], 4);
+  Query qs;
+  qs.o = &o;
+  LightCone lc = light_cone_rect(qs, xyz(apex), *reinterpret_cast<const Light*>(light));
+  const Cone& c = lc.c;
+  float v[] = {c.apex.x,      c.apex.y,      c.apex.z,      c.right.x,       c.right.y,       c.right.z,
+               c.up.x,        c.up.y,        c.up.z,        c.forward.x,     c.forward.y,     c.forward.z,
+               c.window[0],   c.window[1],   c.window[2],   c.window[3],     c.far_normal.x,  c.far_normal.y,
+               c.far_normal.z, c.far_offset, c.bound[0],    c.bound[1],      c.bound[2],      c.bound[3],
+               c.bound_sin};
+  std::memcpy(out, v, sizeof(v));
+  return lc.ok ? (qs.wide_clipped ? 3 : (qs.wide_window ? 2 : 1)) : 0;
+}
+
+// Bin weights of a light cone for a receiver normal (64 values) and the
+// window's total weight; returns 0 if no cone.
+int oracle_bin_weights(const float* apex, const float* normal, const float* light, float wide_extent,
+                       float* out64, float* out_total) {
+  Options o{};
+  o.wide_extent = wide_extent;
+  Query qs;
+  qs.o = &o;
+  LightCone lc = light_cone_rect(qs, xyz(apex), *reinterpret_cast<const Light*>(light));
+  if (!lc.ok) return 0;
+  for (uint32_t b = 0; b < 64; b++) out64[b] = clipped_bin_weight(lc.c, xyz(normal), b);
+  *out_total = window_open(lc.c, xyz(normal));
+  return 1;
+}
+
+float oracle_rect_open(const float* point, const float* normal, const float* light) {
+  return rect_open(xyz(point), xyz(normal), *reinterpret_cast<const Light*>(light));
+}
+
+float oracle_sphere_unshadowed(const float* point, const float* normal, const float* light) {
+  return sphere_unshadowed(xyz(point), xyz(normal), *reinterpret_cast<const Light*>(light));
+}
+
+}  // extern "C"
+
+// ---- Camera rays: shading points for the plain per-view launch
+// (WGSL :1524-1760 `camera_dir`, `ray_triangle`, `hit_triangle`,
+// `ray_box_entry`, `push_children`, `nearest_in_mesh`, `primary_hit`;
+// :2062-2115 `surface_offset`, `rect_distance`, `seen_light`). Camera rays
+// cull back faces (`camera_hit` :2222).
+namespace orc {
+
+struct Hit {
+  float distance;
+  uint32_t mesh, tri;
+  V3 normal;
+};
+
+constexpr float EDGE_TOLERANCE = 1e-5f;
+
+static V3 rotate_to_world(const Scene& s, uint32_t x, V3 d) {
+  const float* r = s.xforms + 16 * x;
+  return V3{dot(xyz(r), d), dot(xyz(r + 4), d), dot(xyz(r + 8), d)};
+}
+
+static float ray_triangle(V3 o, V3 d, V3 v0, V3 v1, V3 v2) {
+  V3 e1 = v1 - v0, e2 = v2 - v0;
+  V3 p = cross(d, e2);
+  float det = dot(e1, p);
+  if (std::fabs(det) < 1e-12f) return -1.0f;
+  float inv = 1.0f / det;
+  V3 s = o - v0;
+  float u = dot(s, p) * inv;
+  if (u < -EDGE_TOLERANCE || u > 1.0f + EDGE_TOLERANCE) return -1.0f;
+  V3 q = cross(s, e1);
+  float v = dot(d, q) * inv;
+  if (v < -EDGE_TOLERANCE || u + v > 1.0f + EDGE_TOLERANCE) return -1.0f;
+  return dot(e2, q) * inv;
+}
+
+static Hit hit_triangle(const Scene& s, V3 eye, V3 dir, uint32_t m, uint32_t t, Hit best) {
+  const float* tr = s.tris + 12u * t;
+  V3 v0 = xyz(tr), v1 = xyz(tr + 4), v2 = xyz(tr + 8);
+  V3 n = cross(v1 - v0, v2 - v0);
+  if (dot(n, dir) > 0.0f) return best;  // cull_back
+  float d = ray_triangle(eye, dir, v0, v1, v2);
+  if (d > 0.0f && m == best.mesh && std::fabs(d - best.distance) <= 2e-6f * d) {
+    bool facing = dot(n, dir) < 0.0f;
+    if (facing != (dot(best.normal, dir) < 0.0f)) {
+      if (facing) return Hit{d, m, t, normalize(n)};
+      return best;
+    }
+  }
+  if (d > 0.0f &&
+      (d < best.distance || (d == best.distance && (m < best.mesh || (m == best.mesh && t < best.tri)))))
+    return Hit{d, m, t, normalize(cross(v1 - v0, v2 - v0))};
+  return best;
+}
+
+static float ray_box_entry(V3 eye, V3 inv, V3 lo, V3 hi, float limit) {
+  V3 t0{(lo.x - eye.x) * inv.x, (lo.y - eye.y) * inv.y, (lo.z - eye.z) * inv.z};
+  V3 t1{(hi.x - eye.x) * inv.x, (hi.y - eye.y) * inv.y, (hi.z - eye.z) * inv.z};
+  float nearv = fmax_(fmax_(fmin_(t0.x, t1.x), fmin_(t0.y, t1.y)), fmin_(t0.z, t1.z));
+  float farv = fmin_(fmin_(fmax_(t0.x, t1.x), fmax_(t0.y, t1.y)), fmax_(t0.z, t1.z));
+  if (farv < fmax_(nearv, 0.0f) || nearv > limit) return -1.0f;
+  return fmax_(nearv, 0.0f);
+}
+
+struct RayWalk {
+  uint32_t stack[STACK];
+  bool overflow = false;
+  bool nearest_first = true;
+};
+
+static void push_children(RayWalk& w, V3 eye, V3 inv, uint32_t left, const Node& l, const Node& r, float limit,
+                          uint32_t& sp) {
+  if (sp + 2 > STACK) {
+    w.overflow = true;
+    return;
+  }
+  if (!w.nearest_first) {
+    w.stack[sp] = left;
+    w.stack[sp + 1] = left + 1;
+    sp += 2;
+    return;
+  }
+  float tl = ray_box_entry(eye, inv, xyz(l.lo), xyz(l.hi), limit);
+  float tr = ray_box_entry(eye, inv, xyz(r.lo), xyz(r.hi), limit);
+  if (tl >= 0.0f && tr >= 0.0f) {
+    if (tl <= tr) {
+      w.stack[sp] = left + 1;
+      w.stack[sp + 1] = left;
+    } else {
+      w.stack[sp] = left;
+      w.stack[sp + 1] = left + 1;
+    }
+    sp += 2;
+  } else if (tl >= 0.0f) {
+    w.stack[sp++] = left;
+  } else if (tr >= 0.0f) {
+    w.stack[sp++] = left + 1;
+  }
+}
+
+static V3 recip3(V3 d) { return V3{1.0f / d.x, 1.0f / d.y, 1.0f / d.z}; }
+
+static Hit nearest_in_mesh(const Scene& s, uint32_t hierarchy, RayWalk& w, V3 world_eye, V3 world_dir, uint32_t m,
+                           Hit best, uint32_t base) {
+  Hit hit = best;
+  const uint32_t* range = s.ranges + 4 * m;
+  V3 eye = world_eye, dir = world_dir;
+  if (range[3] != NONE) {
+    eye = point_to_object(s, m, world_eye);
+    dir = rotate_to_object(s, range[3], world_dir) * s.xforms[16 * range[3] + 13];
+  }
+  if (hierarchy == 2) {
+    if (range[2] == NONE) return hit;
+    V3 inv = recip3(dir);
+    if (base >= STACK) {
+      w.overflow = true;
+      return hit;
+    }
+    uint32_t sp = base + 1;
+    w.stack[base] = range[2];
+    while (sp > base && !w.overflow) {
+      sp--;
+      const Node& node = s.blas[w.stack[sp]];
+      if (ray_box_entry(eye, inv, xyz(node.lo), xyz(node.hi), hit.distance) < 0.0f) continue;
+      if (node.info[1] > 0) {
+        for (uint32_t k = node.info[0]; k < node.info[0] + node.info[1]; k++)
+          hit = hit_triangle(s, eye, dir, m, s.blas_items[k], hit);
+      } else {
+        push_children(w, eye, inv, node.info[0], s.blas[node.info[0]], s.blas[node.info[0] + 1], hit.distance, sp);
+      }
+    }
+  } else {
+    for (uint32_t t = range[0]; t < range[0] + range[1]; t++) hit = hit_triangle(s, eye, dir, m, t, hit);
+  }
+  if (range[3] != NONE && hit.mesh == m && best.mesh != m) hit.normal = normalize(rotate_to_world(s, range[3], hit.normal));
+  return hit;
+}
+
+static Hit primary_hit(const Scene& s, uint32_t hierarchy, RayWalk& w, V3 eye, V3 dir) {
+  Hit hit{3.4e38f, NONE, NONE, V3{0.0f, 0.0f, 0.0f}};
+  if (hierarchy >= 1) {
+    V3 inv = recip3(dir);
+    uint32_t sp = 1;
+    w.stack[0] = 0;
+    while (sp > 0 && !w.overflow) {
+      sp--;
+      const Node& node = s.nodes[w.stack[sp]];
+      if (ray_box_entry(eye, inv, xyz(node.lo), xyz(node.hi), hit.distance) < 0.0f) continue;
+      if (node.info[1] > 0) {
+        for (uint32_t k = node.info[0]; k < node.info[0] + node.info[1]; k++)
+          hit = nearest_in_mesh(s, hierarchy, w, eye, dir, s.items[k], hit, sp);
+      } else {
+        push_children(w, eye, inv, node.info[0], s.nodes[node.info[0]], s.nodes[node.info[0] + 1], hit.distance, sp);
+      }
+    }
+  } else {
+    for (uint32_t m = 0; m < s.n_objects; m++) hit = nearest_in_mesh(s, hierarchy, w, eye, dir, m, hit, 0);
+  }
+  if (dot(hit.normal, dir) > 0.0f) hit.normal = -hit.normal;
+  return hit;
+}
+
+static float rect_distance(const Light& l, V3 eye, V3 dir) {
+  if (is_sphere(l)) {
+    V3 oc = eye - xyz(l.corner);
+    float b = dot(oc, dir);
+    float c = dot(oc, oc) - l.edge_u[0] * l.edge_u[0];
+    float disc = b * b - c;
+    if (c <= 0.0f || disc < 0.0f) return 3.4e38f;
+    float t = -b - std::sqrt(disc);
+    if (t <= 0.0f) return 3.4e38f;
+    return t;
+  }
+  if (l.corner[3] < 0.5f) return 3.4e38f;
+  V3 u = xyz(l.edge_u), v = xyz(l.edge_v);
+  V3 n = cross(u, v);
+  float facing = dot(dir, n);
+  if (facing >= 0.0f) return 3.4e38f;
+  float t = dot(xyz(l.corner) - eye, n) / facing;
+  if (t <= 0.0f) return 3.4e38f;
+  V3 q = eye + dir * t - xyz(l.corner);
+  float ss = dot(q, u) / dot(u, u);
+  float rr = dot(q, v) / dot(v, v);
+  if (ss < 0.0f || ss > 1.0f || rr < 0.0f || rr > 1.0f) return 3.4e38f;
+  return t;
+}
+
+// camera_dir (WGSL :1769-1774). cam: eye4 forward4 right4 up4 params4.
+static V3 camera_dir(const float* cam, uint32_t x, uint32_t y) {
+  const float* p = cam + 16;
+  float aspect = p[0] / p[1];
+  float sx = (((float)x + 0.5f + cam[11]) / p[0] * 2.0f - 1.0f) * p[2] * aspect;
+  float sy = (1.0f - ((float)y + 0.5f + cam[15]) / p[1] * 2.0f) * p[2];
+  return normalize(xyz(cam + 4) + xyz(cam + 8) * sx + xyz(cam + 12) * sy);
+}
+
+}  // namespace orc
+
+extern "C" {
+
+// Per pixel of a w x h view: the camera ray's nearest front-facing hit and
+// the shading point a cone starts from. Writes hits[4*p] = (mesh, tri,
+// distance bits, seen light index + 1) and receivers[8*p] = (point, 0,
+// normal, 0 where a surface is shaded, -1 where the pixel sees the sky or a
+// light and no cone is traced). The shading normal is the hit
+// triangle's face normal toward the camera (flat shading). Returns the
+// number of rays whose stack would overflow 48 entries.
+int oracle_primary(const float* tris, const uint32_t* ranges, uint32_t n_objects, const float* blas,
+                   const uint32_t* blas_items, const float* nodes, const uint32_t* items, const float* xforms,
+                   uint32_t hierarchy, uint32_t nearest_first, const float* cam, uint32_t w, uint32_t h,
+                   const float* lights, uint32_t n_lights, uint32_t* hits, float* receivers, int threads) {
+  Scene s = make_scene(tris, ranges, n_objects, blas, blas_items, nullptr, nodes, items, xforms);
+  const Light* ls = reinterpret_cast<const Light*>(lights);
+  std::atomic<uint64_t> next{0};
+  std::atomic<int> overflows{0};
+  const uint64_t total = (uint64_t)w * h;
+  auto work = [&]() {
+    for (;;) {
+      uint64_t p0 = next.fetch_add(64);
+      if (p0 >= total) break;
+      uint64_t p1 = std::min(total, p0 + 64);
+      for (uint64_t px = p0; px < p1; px++) {
+        uint32_t x = (uint32_t)(px % w), y = (uint32_t)(px / w);
+        V3 dir = camera_dir(cam, x, y);
+        V3 eye = xyz(cam);
+        RayWalk rw;
+        rw.nearest_first = nearest_first != 0;
+        Hit hit = primary_hit(s, hierarchy, rw, eye, dir);
+        if (rw.overflow) overflows++;
+        uint32_t seen = 0;
+        float nearest = hit.distance;
+        for (uint32_t l = 0; l < n_lights; l++) {
+          float t = rect_distance(ls[l], eye, dir);
+          if (t < nearest) {
+            nearest = t;
+            seen = l + 1;
+          }
+        }
+        uint32_t dbits;
+        std::memcpy(&dbits, &hit.distance, 4);
+        hits[4 * px] = hit.mesh;
+        hits[4 * px + 1] = hit.tri;
+        hits[4 * px + 2] = dbits;
+        hits[4 * px + 3] = seen;
+        float* r = receivers + 8 * px;
+        bool lit = seen == 0 && hit.mesh != NONE;
+        V3 point{0.0f, 0.0f, 0.0f}, normal{0.0f, 0.0f, 0.0f};
+        if (lit) {
+          V3 surface = eye + dir * hit.distance;
+          point = surface + hit.normal * fmax_(1e-4f, 2e-6f * hit.distance);
+          normal = hit.normal;
+        }
+        r[0] = point.x, r[1] = point.y, r[2] = point.z, r[3] = 0.0f;
+        r[4] = normal.x, r[5] = normal.y, r[6] = normal.z, r[7] = lit ? 0.0f : -1.0f;
+      }
+    }
+  };
+  if (threads <= 1) {
+    work();
+  } else {
+    std::vector<std::thread> pool;
+    for (int t = 0; t < threads; t++) pool.emplace_back(work);
+    for (auto& t : pool) t.join();
+  }
+  return overflows.load();
+}
+
+}  // extern "C"

@@ -1,0 +1,162 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes wrapper of the oracle (liboracle.so), the
+CPU restatement of the reference's cone-visibility path. Only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg may import this
+module, and only to check or time the reference's algorithm; the product
+(continuum_amd) never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import struct
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "liboracle.so"
+NONE = 0xFFFFFFFF
+_P = C.c_void_p
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m continuum_amd.build`")
+        L = C.CDLL(str(LIB_PATH))
+        sig = {
+            "oracle_irradiance": (C.c_int, [_P] * 9 + [_P, C.c_uint64, _P, C.c_uint32, _P, _P, _P, _P, _P, _P, C.c_int]),
+            "oracle_primary": (C.c_int, [_P, _P, C.c_uint32, _P, _P, _P, _P, _P, C.c_uint32, C.c_uint32, _P,
+                                         C.c_uint32, C.c_uint32, _P, C.c_uint32, _P, _P, C.c_int]),
+            "oracle_flatten": (C.c_int, [_P, _P, C.c_uint32, _P, _P, _P, _P, _P, _P, _P, _P]),
+            "oracle_refit": (None, [_P, _P, C.c_uint32, _P, _P, _P, C.c_uint32]),
+            "oracle_atan2": (C.c_float, [C.c_float, C.c_float]),
+            "oracle_acos": (C.c_float, [C.c_float]),
+            "oracle_exp": (C.c_float, [C.c_float]),
+            "oracle_draw_polygon": (None, [_P, C.c_uint32, C.c_float, _P, _P]),
+            "oracle_light_cone": (C.c_int, [_P, _P, _P, _P]),
+            "oracle_bin_weights": (C.c_int, [_P, _P, _P, C.c_float, _P, _P]),
+            "oracle_rect_open": (C.c_float, [_P, _P, _P]),
+            "oracle_sphere_unshadowed": (C.c_float, [_P, _P, _P]),
+            "oracle_mask_range": (C.c_uint64, [C.c_uint32, C.c_uint32]),
+            "oracle_mask_from_interval": (C.c_uint64, [C.c_float, C.c_float]),
+            "oracle_coverage_visible": (C.c_float, [_P, C.c_uint32]),
+            "oracle_coverage_open_length": (C.c_float, [_P, C.c_uint32, C.c_float, C.c_float]),
+            "oracle_cone3_visible": (None, [_P, C.c_uint32, _P, _P, _P, C.c_uint32, C.c_int, _P, _P]),
+            "oracle_sampled_visible": (C.c_double, [_P, _P, _P, C.c_uint64, C.c_uint32]),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data if a is not None and a.size else 0
+
+
+def _f32bits(x: float) -> int:
+    return struct.unpack("<I", struct.pack("<f", x))[0]
+
+
+def opts_array(hierarchy=2, prefilter=0.0, wide_extent=8.0, light_cone_cap=4096, wide_cone_cap=512,
+               profile_stage=0, object_cov=NONE) -> np.ndarray:
+    return np.array([hierarchy, _f32bits(prefilter), _f32bits(wide_extent), light_cone_cap, wide_cone_cap,
+                     profile_stage, object_cov], dtype=np.uint32)
+
+
+def threads_default() -> int:
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+class OracleFlat:
+    """The oracle builder's flattened scene (same fields as the product's Flat)."""
+
+    def __init__(self, tris9: np.ndarray, counts: np.ndarray):
+        tris9 = np.ascontiguousarray(tris9, dtype=np.float32)
+        counts = np.ascontiguousarray(counts, dtype=np.uint32)
+        sizes = np.zeros(6, np.uint64)
+        L = lib()
+        L.oracle_flatten(_p(tris9), _p(counts), len(counts), _p(sizes), 0, 0, 0, 0, 0, 0, 0)
+        nt, nr, nb, nbi, nn, ni = (int(x) for x in sizes)
+        self.tris = np.zeros((nt, 12), np.float32)
+        self.ranges = np.zeros((nr, 4), np.uint32)
+        self.blas = np.zeros((nb, 12), np.uint32)
+        self.blas_items = np.zeros(nbi, np.uint32)
+        self.blas_cov = np.zeros((nb, 27), np.float32)
+        self.nodes = np.zeros((nn, 12), np.uint32)
+        self.items = np.zeros(ni, np.uint32)
+        self.xforms = np.zeros((1, 16), np.float32)
+        L.oracle_flatten(_p(tris9), _p(counts), len(counts), _p(sizes), _p(self.tris), _p(self.ranges),
+                         _p(self.blas), _p(self.blas_items), _p(self.blas_cov), _p(self.nodes), _p(self.items))
+
+
+def scene_args(flat):
+    xf = flat.xforms if len(flat.xforms) else np.zeros((1, 16), np.float32)
+    return (_p(flat.tris), _p(flat.ranges), len(flat.ranges), _p(flat.blas), _p(flat.blas_items), _p(flat.blas_cov),
+            _p(flat.nodes), _p(flat.items), _p(xf))
+
+
+def irradiance(flat, receivers: np.ndarray, lights: np.ndarray, opts: np.ndarray, want_cov=False, threads=None):
+    """(visible, full, touched, cov, capped, overflows) for queries l * n + i."""
+    receivers = np.ascontiguousarray(receivers, dtype=np.float32).reshape(-1, 8)
+    lights = np.ascontiguousarray(lights, dtype=np.float32).reshape(-1, 16)
+    n, nl = len(receivers), len(lights)
+    q = n * nl
+    vis = np.zeros(q, np.float32)
+    full = np.zeros(q, np.uint64)
+    touched = np.zeros(q, np.uint64)
+    capped = np.zeros(q, np.uint8)
+    cov = np.zeros((q, 64), np.float32) if want_cov else None
+    of = lib().oracle_irradiance(*scene_args(flat), _p(receivers), n, _p(lights), nl, _p(opts), _p(vis), _p(full),
+                                 _p(touched), _p(cov), _p(capped), threads or threads_default())
+    return vis, full, touched, cov, capped, of
+
+
+def primary(flat, camera: np.ndarray, width: int, height: int, lights: np.ndarray, hierarchy=2, nearest_first=1,
+            threads=None):
+    """(hits (px, 4) u32, receivers (px, 8) f32, overflows)."""
+    camera = np.ascontiguousarray(camera, dtype=np.float32)
+    lights = np.ascontiguousarray(lights, dtype=np.float32).reshape(-1, 16)
+    px = width * height
+    hits = np.zeros((px, 4), np.uint32)
+    recv = np.zeros((px, 8), np.float32)
+    xf = flat.xforms if len(flat.xforms) else np.zeros((1, 16), np.float32)
+    of = lib().oracle_primary(_p(flat.tris), _p(flat.ranges), len(flat.ranges), _p(flat.blas), _p(flat.blas_items),
+                              _p(flat.nodes), _p(flat.items), _p(xf), hierarchy, nearest_first, _p(camera), width,
+                              height, _p(lights), len(lights), _p(hits), _p(recv), threads or threads_default())
+    return hits, recv, of
+
+
+def refit(flat, new_tris12: np.ndarray):
+    """(blas, blas_cov) after refitting every mesh's hierarchy to new_tris12."""
+    blas = flat.blas.copy()
+    cov = flat.blas_cov.copy()
+    t = np.ascontiguousarray(new_tris12, dtype=np.float32)
+    lib().oracle_refit(_p(t), _p(flat.ranges), len(flat.ranges), _p(blas), _p(flat.blas_items), _p(cov), len(blas))
+    return blas, cov
+
+
+def cone3_visible(receivers6: np.ndarray, light: np.ndarray, meshes, cull=True):
+    from continuum_amd.scenes import meshes_to_arrays
+
+    r = np.ascontiguousarray(receivers6, dtype=np.float32).reshape(-1, 6)
+    tris9, counts = meshes_to_arrays(meshes)
+    out = np.zeros(len(r), np.float32)
+    total = np.zeros(len(r), np.float32)
+    lt = np.ascontiguousarray(light, dtype=np.float32)
+    lib().oracle_cone3_visible(_p(r), len(r), _p(lt), _p(tris9), _p(counts), len(counts), int(cull), _p(out),
+                               _p(total))
+    return out, total
+
+
+def sampled_visible(point, light: np.ndarray, meshes, k: int) -> float:
+    from continuum_amd.scenes import meshes_to_arrays
+
+    tris9, _ = meshes_to_arrays(meshes)
+    p = np.ascontiguousarray(point, dtype=np.float32)
+    lt = np.ascontiguousarray(light, dtype=np.float32)
+    return float(lib().oracle_sampled_visible(_p(p), _p(lt), _p(tris9), len(tris9), k))

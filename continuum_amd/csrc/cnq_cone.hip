@@ -150,7 +150,7 @@ __device__ __forceinline__ LightCone light_cone_rect(f3 apex, const LightRect& r
 }
 
 // ---- weights (WGSL :1232-1272, :1331-1410, :2148-2190)
-__device__ __noinline__ float bin_weight(const Cone& c, f3 normal, uint32_t b) {
+__device__ __forceinline__ float bin_weight(const Cone& c, f3 normal, uint32_t b) {
   float col = (float)(b % GRID), row = (float)(b / GRID), g = (float)GRID;
   f3 d0 = direction(c, col / g, row / g);
   f3 d1 = direction(c, (col + 1.0f) / g, row / g);
@@ -160,23 +160,34 @@ __device__ __noinline__ float bin_weight(const Cone& c, f3 normal, uint32_t b) {
                        lambert_edge(normal, d3, d0)));
 }
 
-__device__ __noinline__ float clipped_quad_weight(f3 normal, f3 d0, f3 d1, f3 d2, f3 d3, bool allow_small) {
-  f3 d[4] = {d0, d1, d2, d3};
-  f3 poly[8];
+// A quad of directions clipped to the horizon of `normal`, then Lambert's
+// edge integrals; its polygon (up to 8 vertices) at P3A.
+__device__ __forceinline__ float clipped_quad_weight(Lane& L, f3 normal, f3 d0, f3 d1, f3 d2, f3 d3,
+                                                     bool allow_small) {
   uint32_t n = 0;
+  f3 d[4] = {d0, d1, d2, d3};
+#pragma unroll
   for (uint32_t k = 0; k < 4; k++) {
-    f3 a = d[k], e = d[(k + 1) & 3u];
-    float sa = dot3(normal, a), se = dot3(normal, e);
-    if (sa >= 0.0f) poly[n++] = a;
-    if ((sa >= 0.0f) != (se >= 0.0f)) poly[n++] = a + (e - a) * (sa / (sa - se));
+    const f3 a = d[k], e = d[(k + 1) & 3u];
+    const float sa = dot3(normal, a), se = dot3(normal, e);
+    if (sa >= 0.0f) L.put3(P3A, n++, a);
+    if ((sa >= 0.0f) != (se >= 0.0f)) L.put3(P3A, n++, a + (e - a) * (sa / (sa - se)));
   }
   if (!allow_small && n < 3) return 0.0f;
   float sum = 0.0f;
-  for (uint32_t k = 0; k < n; k++) sum += lambert_edge(normal, poly[k], poly[k + 1 == n ? 0 : k + 1]);
+  if (n > 0) {
+    const f3 first = L.p3(P3A, 0);
+    f3 a = first;
+    for (uint32_t k = 0; k < n; k++) {
+      const f3 b = k + 1 < n ? L.p3(P3A, k + 1) : first;
+      sum += lambert_edge(normal, a, b);
+      a = b;
+    }
+  }
   return fabsf(0.5f * sum);
 }
 
-__device__ __forceinline__ float clipped_bin_weight(const Cone& c, f3 normal, uint32_t b) {
+__device__ __forceinline__ float clipped_bin_weight(Lane& L, const Cone& c, f3 normal, uint32_t b) {
   float col = (float)(b % GRID), row = (float)(b / GRID), g = (float)GRID;
   f3 d0 = direction(c, col / g, row / g);
   f3 d1 = direction(c, (col + 1.0f) / g, row / g);
@@ -186,10 +197,12 @@ __device__ __forceinline__ float clipped_bin_weight(const Cone& c, f3 normal, ui
                    (dot3(normal, d3) > 0.0f);
   if (above == 4) return bin_weight(c, normal, b);
   if (above == 0) return 0.0f;
-  return clipped_quad_weight(normal, d0, d1, d2, d3, true);
+  return clipped_quad_weight(L, normal, d0, d1, d2, d3, true);
 }
 
-__device__ __forceinline__ float window_weight4(const Cone& c, f3 normal, bool& all_above) {
+// The window's weight: one quad when wholly above the horizon, else clipped
+// (`cone_visible` :1249-1262, `window_open` :1078, `clipped_window_weight` :1375).
+__device__ __forceinline__ float window_weight4(Lane& L, const Cone& c, f3 normal, bool& all_above) {
   f3 d00 = direction(c, 0.0f, 0.0f), d10 = direction(c, 1.0f, 0.0f);
   f3 d11 = direction(c, 1.0f, 1.0f), d01 = direction(c, 0.0f, 1.0f);
   all_above = dot3(normal, d00) > 0.0f && dot3(normal, d10) > 0.0f && dot3(normal, d11) > 0.0f &&
@@ -197,25 +210,25 @@ __device__ __forceinline__ float window_weight4(const Cone& c, f3 normal, bool& 
   if (all_above)
     return fabsf(0.5f * (lambert_edge(normal, d00, d10) + lambert_edge(normal, d10, d11) +
                          lambert_edge(normal, d11, d01) + lambert_edge(normal, d01, d00)));
-  return clipped_quad_weight(normal, d00, d10, d11, d01, false);
+  return clipped_quad_weight(L, normal, d00, d10, d11, d01, false);
 }
 
-// `cone_visible` (WGSL :1244-1272) over the bins whose coverage is set in
-// `touched` (a bin with zero coverage adds nothing), read from `covs`.
-__device__ __forceinline__ float cone_visible(const Cone& c, f3 normal, uint64_t touched, const float* covs) {
+// `cone_visible` (WGSL :1244-1272) over the bins set in `touched` (a bin
+// with zero coverage adds nothing), coverage read from the accumulator cells.
+__device__ __forceinline__ float cone_visible(Lane& L, const Cone& c, f3 normal, uint64_t touched) {
   bool all_above;
-  float total = window_weight4(c, normal, all_above);
+  const float total = window_weight4(L, c, normal, all_above);
   float blocked = 0.0f;
   while (touched) {
-    uint32_t b = (uint32_t)__builtin_ctzll(touched);
+    const uint32_t b = (uint32_t)__builtin_ctzll(touched);
     touched &= touched - 1;
-    float covered = covs[b * 64u];
-    blocked += covered * (all_above ? bin_weight(c, normal, b) : clipped_bin_weight(c, normal, b));
+    const float covered = L.get(b);
+    blocked += covered * (all_above ? bin_weight(c, normal, b) : clipped_bin_weight(L, c, normal, b));
   }
   return mx(total - blocked, 0.0f);
 }
 
-__device__ __forceinline__ float rect_open(f3 point, f3 normal, const LightRect& r) {
+__device__ __forceinline__ float rect_open(Lane& L, f3 point, f3 normal, const LightRect& r) {
   f3 u{r.edge_u.x, r.edge_u.y, r.edge_u.z}, v{r.edge_v.x, r.edge_v.y, r.edge_v.z};
   f3 c0 = f3{r.corner.x, r.corner.y, r.corner.z} - point;
   float facing = dot3(norm3(cross3(u, v)), c0);
@@ -232,7 +245,7 @@ __device__ __forceinline__ float rect_open(f3 point, f3 normal, const LightRect&
     sum += lambert_edge_unit(normal, a3, a0);
     return fabsf(0.5f * sum);
   }
-  return clipped_quad_weight(normal, c0, c0 + u, c0 + u + v, c0 + v, false);
+  return clipped_quad_weight(L, normal, c0, c0 + u, c0 + u + v, c0 + v, false);
 }
 
 // WGSL :937-955
@@ -265,8 +278,8 @@ __constant__ float DISK[64] = {
     0.000000f, 0.160355f, 0.694958f, 0.957934f, 0.957934f, 0.694958f, 0.160355f, 0.000000f,
 };
 
-// WGSL :960-972 (coverage read from `covs`)
-__device__ __forceinline__ float sphere_open_share(const Cone& c, f3 normal, bool wide_clipped, const float* covs) {
+// WGSL :960-972 (coverage read from the accumulator cells)
+__device__ __forceinline__ float sphere_open_share(Lane& L, const Cone& c, f3 normal, bool wide_clipped) {
   float total = 0.0f, open = 0.0f;
   const float g = (float)GRID;
   for (uint32_t b = 0; b < BINS; b++) {
@@ -276,7 +289,7 @@ __device__ __forceinline__ float sphere_open_share(const Cone& c, f3 normal, boo
     float l2 = dot3(d, d);
     float w = m * mx(dot3(normal, d), 0.0f) / (l2 * l2);
     total += w;
-    open += w * (1.0f - covs[b * 64u]);
+    open += w * (1.0f - L.get(b));
   }
   if (total <= 0.0f) return 1.0f;
   return open / total;
@@ -310,6 +323,19 @@ struct Stack {
 
 constexpr uint32_t STACK_LIMIT = 64;
 
+// The scene and launch parameters, from the kernel's own (noalias) arguments.
+struct Ctx {
+  const NodeC* __restrict__ blas;
+  const NodeC* __restrict__ nodes;
+  const float* __restrict__ tri_leaf;
+  const float* __restrict__ tris;
+  const uint32_t* __restrict__ ranges;
+  const uint32_t* __restrict__ items;
+  const float* __restrict__ blas_cov;
+  const float* __restrict__ xforms;
+  uint32_t n_objects, hierarchy, profile_stage, object_cov;
+};
+
 // Per-lane walk state.
 struct Walk {
   float cov[64];
@@ -323,9 +349,7 @@ struct Walk {
                      // meshes entered, blas nodes splatted
 };
 
-__device__ __forceinline__ NodeC load_node(const NodeC* __restrict__ p, uint32_t n) { return p[n]; }
-
-// WGSL :623-638 (the cone in object `m`'s space; x is its transform)
+// WGSL :623-638 (the cone in an object's space, for transform `xf`)
 __device__ __forceinline__ Cone cone_to_object(const Cone& c, const float* __restrict__ xf) {
   f3 r0 = ld3(xf), r1 = ld3(xf + 4), r2 = ld3(xf + 8);
   f3 t{xf[3], xf[7], xf[11]};
@@ -393,32 +417,56 @@ __device__ __forceinline__ float node_front_area(const float* __restrict__ cov27
   return mixf(mixf(f[0], f[1], fs), mixf(f[2], f[3], fs), ft);
 }
 
-// WGSL :490-512
-__device__ __forceinline__ void hull8(const f2* pts, Poly2& out) {
+// WGSL :490-512: clockwise hull of the 8 points at P3A (2D, slots 0-15),
+// written at Q2A; returns its vertex count.
+__device__ __forceinline__ uint32_t hull8(Lane& L) {
   uint32_t start = 0;
-  for (uint32_t k = 1; k < 8; k++)
-    if (pts[k].x < pts[start].x || (pts[k].x == pts[start].x && pts[k].y < pts[start].y)) start = k;
-  out.n = 0;
+  f2 ps = L.p2(P3A, 0);
+  for (uint32_t k = 1; k < 8; k++) {
+    const f2 pk = L.p2(P3A, k);
+    if (pk.x < ps.x || (pk.x == ps.x && pk.y < ps.y)) {
+      start = k;
+      ps = pk;
+    }
+  }
+  uint32_t n = 0;
   uint32_t cur = start;
   for (;;) {
-    out.v[out.n++] = pts[cur];
+    const f2 pc = L.p2(P3A, cur);
+    L.put2(Q2A, n++, pc);
     uint32_t next = (cur + 1) & 7u;
+    f2 pn = L.p2(P3A, next);
     for (uint32_t k = 0; k < 8; k++) {
-      f2 e = pts[next] - pts[cur];
-      f2 f = pts[k] - pts[cur];
-      float cr = e.x * f.y - e.y * f.x;
-      if (cr > 0.0f || (cr == 0.0f && dot2(f, f) > dot2(e, e))) next = k;
+      const f2 pk = L.p2(P3A, k);
+      const f2 e = pn - pc;
+      const f2 f = pk - pc;
+      const float cr = e.x * f.y - e.y * f.x;
+      if (cr > 0.0f || (cr == 0.0f && dot2(f, f) > dot2(e, e))) {
+        next = k;
+        pn = pk;
+      }
     }
     cur = next;
-    if (cur == start || out.n >= 8) break;
+    if (cur == start || n >= 8) break;
   }
+  return n;
+}
+
+// The splat of a node's 8 projected corners (at P3A) with weight `alpha`:
+// their hull clipped to the grid, drawn into the accumulator.
+__device__ __forceinline__ bool draw_hull(Lane& L, float alpha) {
+  uint32_t n = hull8(L);
+  uint32_t at = Q2A;
+  n = clip_to_grid(L, at, n);
+  if (n < 3) return false;
+  draw_poly(L, at, n, alpha);
+  return true;
 }
 
 // WGSL :519-574: 0 culled, 1 splatted with nothing drawn, 2 splatted and drawn, 3 descend.
-__device__ __noinline__ uint32_t node_cone_test(const Cone& c, AccRef& acc, float prefilter_bins, f3 lo, f3 hi,
-                                                const float* __restrict__ cov27) {
+__device__ __forceinline__ uint32_t node_cone_test(const Cone& c, Lane& L, float prefilter_bins, f3 lo, f3 hi,
+                                                   const float* __restrict__ cov27) {
   const float g = (float)GRID;
-  f2 pts[8];
   f2 mnv{3.4e38f, 3.4e38f}, mxv{-3.4e38f, -3.4e38f};
   bool far_out = true, near_out = true, out_u0 = true, out_u1 = true, out_v0 = true, out_v1 = true;
   bool inside = prefilter_bins > 0.0f;
@@ -441,7 +489,7 @@ __device__ __noinline__ uint32_t node_cone_test(const Cone& c, AccRef& acc, floa
     if (c.w3 * p.z - p.y >= 0.0f) out_v1 = false;
     if (inside) {
       f2 q{(p.x / p.z - c.w0) / (c.w2 - c.w0) * g, (p.y / p.z - c.w1) / (c.w3 - c.w1) * g};
-      pts[k] = q;
+      L.put2(P3A, k, q);
       mnv = f2{mn(mnv.x, q.x), mn(mnv.y, q.y)};
       mxv = f2{mx(mxv.x, q.x), mx(mxv.y, q.y)};
     }
@@ -456,16 +504,11 @@ __device__ __noinline__ uint32_t node_cone_test(const Cone& c, AccRef& acc, floa
   if (silhouette <= 1e-30f) return 1;
   float alpha = node_front_area(cov27, v) / silhouette;
   if (alpha <= 0.0f) return 1;
-  Poly2 q;
-  hull8(pts, q);
-  clip_to_grid(q);
-  if (q.n < 3) return 1;
-  draw_poly(acc, q, alpha);
-  return 2;
+  return draw_hull(L, alpha) ? 2 : 1;
 }
 
 // WGSL :594-616 (opaque triangles)
-__device__ __forceinline__ bool cover_triangle(const Cone& c, AccRef& acc, f3 v0, f3 v1, f3 v2, float weight,
+__device__ __forceinline__ bool cover_triangle(const Cone& c, Lane& L, f3 v0, f3 v1, f3 v2, float weight,
                                                uint32_t stage, uint32_t& drawn_count) {
   if (stage == 4) return false;
   f3 n = cross3(v1 - v0, v2 - v0);
@@ -475,17 +518,17 @@ __device__ __forceinline__ bool cover_triangle(const Cone& c, AccRef& acc, f3 v0
     return false;
   if (!may_touch(c, v0, v1, v2)) return false;
   drawn_count++;
-  return rasterize(c, acc, v0, v1, v2, weight);
+  return rasterize(c, L, v0, v1, v2, weight);
 }
 
 // Adds one mesh's accumulation buffer to the coverage (WGSL :809-828).
-__device__ __forceinline__ void combine(Walk& w, AccRef& acc, bool snap) {
+__device__ __forceinline__ void combine(Walk& w, Lane& L, bool snap) {
   float running = 0.0f;
-  uint32_t end = acc.span_hi < BINS ? acc.span_hi : BINS;
+  const uint32_t end = L.span_hi < BINS ? L.span_hi : BINS;
 #pragma unroll
   for (uint32_t b = 0; b < BINS; b++) {
-    if (b >= acc.span_lo && b < end) {
-      running += acc.get(b);
+    if (b >= L.span_lo && b < end) {
+      running += L.get(b);
       float covered = fabsf(running);
       if (snap) {
         if (covered >= ONE_MINUS_1EM5) covered = 1.0f;
@@ -493,7 +536,7 @@ __device__ __forceinline__ void combine(Walk& w, AccRef& acc, bool snap) {
         covered = mn(covered, 1.0f);
       }
       if (covered > 1e-6f) {
-        float before = w.cov[b];
+        const float before = w.cov[b];
         w.cov[b] = mx(before, mn(covered, 1.0f));
         if (before < 1.0f && w.cov[b] >= 1.0f) w.full++;
       }
@@ -501,32 +544,31 @@ __device__ __forceinline__ void combine(Walk& w, AccRef& acc, bool snap) {
   }
 }
 
-__device__ __forceinline__ void clear_acc(AccRef& acc) {
+__device__ __forceinline__ void clear_acc(Lane& L) {
 #pragma unroll
-  for (uint32_t b = 0; b < 68; b++) acc.set(b, 0.0f);
-  acc.span_lo = 68;
-  acc.span_hi = 0;
+  for (uint32_t b = 0; b < ACC_CELLS; b++) L.set(b, 0.0f);
+  L.span_lo = ACC_CELLS;
+  L.span_hi = 0;
 }
 
 // WGSL :719-829 for mesh `m` (uniform), for the lanes with `on`.
 template <bool COUNT>
-__device__ __forceinline__ void add_mesh_coverage(const QueryArgs& a, const Cone& wc, uint32_t m, bool on, Walk& w,
-                                                  Stack& st, uint32_t base, AccRef& acc) {
-  if (a.profile_stage == 3) return;
+__device__ __forceinline__ void add_mesh_coverage(const Ctx& x, const Cone& wc, uint32_t m, bool on, Walk& w,
+                                                  Stack& st, uint32_t base, Lane& L) {
+  if (x.profile_stage == 3) return;
   if (!ballot(on)) return;
   if (COUNT && on) w.work[4]++;
-  const SceneArgs& s = a.s;
-  const uint32_t first = s.ranges[4 * m], count = s.ranges[4 * m + 1], root = s.ranges[4 * m + 2],
-                 xform = s.ranges[4 * m + 3];
+  const uint32_t first = x.ranges[4 * m], count = x.ranges[4 * m + 1], root = x.ranges[4 * m + 2],
+                 xform = x.ranges[4 * m + 3];
   Cone c = wc;
   float weight = 1.0f;
   if (xform != NONE) {
-    c = cone_to_object(wc, s.xforms + 16 * (size_t)xform);
-    weight = 1.0f - mx(s.xforms[16 * (size_t)xform + 14], 0.0f);
+    c = cone_to_object(wc, x.xforms + 16 * (size_t)xform);
+    weight = 1.0f - mx(x.xforms[16 * (size_t)xform + 14], 0.0f);
   }
-  clear_acc(acc);
+  clear_acc(L);
   bool touched = false;
-  if (a.hierarchy == 2) {
+  if (x.hierarchy == 2) {
     if (root != NONE) {
       if (base + 1 > STACK_LIMIT) {
         w.truncated = true;
@@ -541,12 +583,12 @@ __device__ __forceinline__ void add_mesh_coverage(const QueryArgs& a, const Cone
         bool mine = ((need >> lane_id()) & 1u) && w.cone_work < w.cone_cap;
         if (mine) w.cone_work++;
         if (!ballot(mine)) continue;
-        const NodeC node = load_node(s.blas, n);
+        const NodeC node = x.blas[n];
         bool descend = false;
         if (mine) {
           if (COUNT) w.work[1]++;
-          uint32_t test = node_cone_test(c, acc, w.prefilter_bins, f3{node.lo[0], node.lo[1], node.lo[2]},
-                                         f3{node.hi[0], node.hi[1], node.hi[2]}, s.blas_cov + 27 * (size_t)n);
+          const uint32_t test = node_cone_test(c, L, w.prefilter_bins, f3{node.lo[0], node.lo[1], node.lo[2]},
+                                               f3{node.hi[0], node.hi[1], node.hi[2]}, x.blas_cov + 27 * (size_t)n);
           if (test == 2) touched = true;
           if (COUNT && (test == 1 || test == 2)) w.work[5]++;
           descend = test == 3;
@@ -557,11 +599,11 @@ __device__ __forceinline__ void add_mesh_coverage(const QueryArgs& a, const Cone
           const uint32_t k0 = node.a, k1 = node.a + node.b;
           if (descend) w.cone_work += node.b;
           for (uint32_t k = k0; k < k1; k++) {
-            const float* t = s.tri_leaf + 9 * (size_t)k;
-            f3 v0{t[0], t[1], t[2]}, v1{t[3], t[4], t[5]}, v2{t[6], t[7], t[8]};
+            const float* t = x.tri_leaf + 9 * (size_t)k;
+            const f3 v0{t[0], t[1], t[2]}, v1{t[3], t[4], t[5]}, v2{t[6], t[7], t[8]};
             if (descend) {
               if (COUNT) w.work[2]++;
-              if (cover_triangle(c, acc, v0, v1, v2, weight, a.profile_stage, w.work[3])) touched = true;
+              if (cover_triangle(c, L, v0, v1, v2, weight, x.profile_stage, w.work[3])) touched = true;
             }
           }
         } else {
@@ -578,22 +620,20 @@ __device__ __forceinline__ void add_mesh_coverage(const QueryArgs& a, const Cone
     }
   } else if (on) {
     for (uint32_t t = first; t < first + count; t++) {
-      const float* tr = s.tris + 12 * (size_t)t;
+      const float* tr = x.tris + 12 * (size_t)t;
       if (COUNT) w.work[2]++;
-      if (cover_triangle(c, acc, ld3(tr), ld3(tr + 4), ld3(tr + 8), weight, a.profile_stage, w.work[3]))
-        touched = true;
+      if (cover_triangle(c, L, ld3(tr), ld3(tr + 4), ld3(tr + 8), weight, x.profile_stage, w.work[3])) touched = true;
     }
   }
-  if (touched) combine(w, acc, true);
+  if (touched) combine(w, L, true);
 }
 
-// WGSL :1096-1150 (object-level splat)
-__device__ __noinline__ bool splat_object_node(const QueryArgs& a, const Cone& c, float prefilter_bins, uint32_t n,
-                                               f3 lo, f3 hi, AccRef& acc, float& alpha_out, bool& drawn) {
-  drawn = false;
-  if (a.object_cov == NONE || prefilter_bins <= 0.0f) return false;
+// WGSL :1096-1150 (object-level splat): true when the node was splatted
+// (coverage combined if anything was drawn); false to descend.
+__device__ __forceinline__ bool splat_object_node(const Ctx& x, const Cone& c, Walk& w, uint32_t n, f3 lo, f3 hi,
+                                                  Lane& L) {
+  if (x.object_cov == NONE || w.prefilter_bins <= 0.0f) return false;
   const float g = (float)GRID;
-  f2 pts[8];
   f2 mnv{3.4e38f, 3.4e38f}, mxv{-3.4e38f, -3.4e38f};
   for (uint32_t k = 0; k < 8; k++) {
     f3 corner = box_corner(lo, hi, k);
@@ -601,36 +641,29 @@ __device__ __noinline__ bool splat_object_node(const QueryArgs& a, const Cone& c
     f3 p = to_local(c, corner);
     if (p.z < NEAR) return false;
     f2 q{(p.x / p.z - c.w0) / (c.w2 - c.w0) * g, (p.y / p.z - c.w1) / (c.w3 - c.w1) * g};
-    pts[k] = q;
+    L.put2(P3A, k, q);
     mnv = f2{mn(mnv.x, q.x), mn(mnv.y, q.y)};
     mxv = f2{mx(mxv.x, q.x), mx(mxv.y, q.y)};
   }
-  if (mx(mxv.x - mnv.x, mxv.y - mnv.y) > prefilter_bins) return false;
+  if (mx(mxv.x - mnv.x, mxv.y - mnv.y) > w.prefilter_bins) return false;
   if (mxv.x <= 0.0f || mxv.y <= 0.0f || mnv.x >= g || mnv.y >= g) return true;
   f3 e = hi - lo;
   f3 v = norm3(c.apex - (lo + hi) * 0.5f);
   float silhouette = fabsf(v.x) * e.y * e.z + fabsf(v.y) * e.x * e.z + fabsf(v.z) * e.x * e.y;
   if (silhouette <= 1e-30f) return true;
-  float alpha = 1.0f - exp_c(-node_front_area(a.s.blas_cov + 27 * (size_t)(a.object_cov + n), v) / silhouette);
+  float alpha = 1.0f - exp_c(-node_front_area(x.blas_cov + 27 * (size_t)(x.object_cov + n), v) / silhouette);
   if (alpha <= 0.0f) return true;
-  Poly2 q;
-  hull8(pts, q);
-  clip_to_grid(q);
-  if (q.n < 3) return true;
-  clear_acc(acc);
-  draw_poly(acc, q, alpha);
-  alpha_out = alpha;
-  drawn = true;
+  clear_acc(L);
+  if (draw_hull(L, alpha)) combine(w, L, false);
   return true;
 }
 
 // WGSL :1160-1227 for the lanes with `traced`.
 template <bool COUNT>
-__device__ __forceinline__ void scene_coverage(const QueryArgs& a, const Cone& c, bool traced, Walk& w, AccRef& acc) {
-  const SceneArgs& s = a.s;
+__device__ __forceinline__ void scene_coverage(const Ctx& x, const Cone& c, bool traced, Walk& w, Lane& L) {
   w.cone_work = 0;
   Stack st{0u, 0u, 0u};
-  if (a.hierarchy >= 1) {
+  if (x.hierarchy >= 1) {
     st.put(0, 0, ballot(traced));
     uint32_t sp = 1;
     while (sp > 0) {
@@ -640,19 +673,13 @@ __device__ __forceinline__ void scene_coverage(const QueryArgs& a, const Cone& c
       bool mine = ((need >> lane_id()) & 1u) && w.full < BINS && w.cone_work < w.cone_cap;
       if (mine) w.cone_work++;
       if (!ballot(mine)) continue;
-      const NodeC node = load_node(s.nodes, n);
+      const NodeC node = x.nodes[n];
       const f3 lo{node.lo[0], node.lo[1], node.lo[2]}, hi{node.hi[0], node.hi[1], node.hi[2]};
       bool descend = false;
       if (mine) {
         if (COUNT) w.work[0]++;
         if (may_touch_box(c, lo, hi)) {
-          bool splatted = false;
-          if ((node.b & NODE_NO_SPLAT) == 0 && a.object_cov != NONE) {
-            float alpha;
-            bool drawn;
-            splatted = splat_object_node(a, c, w.prefilter_bins, n, lo, hi, acc, alpha, drawn);
-            if (drawn) combine(w, acc, false);
-          }
+          const bool splatted = (node.b & NODE_NO_SPLAT) == 0 && splat_object_node(x, c, w, n, lo, hi, L);
           descend = !splatted;
         }
       }
@@ -660,21 +687,19 @@ __device__ __forceinline__ void scene_coverage(const QueryArgs& a, const Cone& c
       if (!dm) continue;
       const uint32_t count = node.b & ~NODE_NO_SPLAT;
       if (count > 0) {
-        for (uint32_t k = node.a; k < node.a + count; k++) {
-          const uint32_t m = s.items[k];
-          add_mesh_coverage<COUNT>(a, c, m, descend, w, st, sp, acc);
-        }
+        for (uint32_t k = node.a; k < node.a + count; k++) add_mesh_coverage<COUNT>(x, c, x.items[k], descend, w, st, sp, L);
       } else {
         uint32_t nearc = node.a, farc = node.a + 1;
-        const NodeC na = load_node(s.nodes, nearc);
-        const NodeC nb = load_node(s.nodes, farc);
-        bool use_bound = c.bcos >= -1.0f && (c.baxis.x != 0.0f || c.baxis.y != 0.0f || c.baxis.z != 0.0f);
-        f3 axis = use_bound ? c.baxis : c.forward;
-        float da = dot3(axis, (f3{na.lo[0], na.lo[1], na.lo[2]} + f3{na.hi[0], na.hi[1], na.hi[2]}) * 0.5f - c.apex);
-        float db = dot3(axis, (f3{nb.lo[0], nb.lo[1], nb.lo[2]} + f3{nb.hi[0], nb.hi[1], nb.hi[2]}) * 0.5f - c.apex);
+        const NodeC na = x.nodes[nearc];
+        const NodeC nb = x.nodes[farc];
+        const bool use_bound = c.bcos >= -1.0f && (c.baxis.x != 0.0f || c.baxis.y != 0.0f || c.baxis.z != 0.0f);
+        // (componentwise selects keep the cone in registers)
+        const f3 axis{use_bound ? c.baxis.x : c.forward.x, use_bound ? c.baxis.y : c.forward.y,
+                      use_bound ? c.baxis.z : c.forward.z};
+        const float da = dot3(axis, (f3{na.lo[0], na.lo[1], na.lo[2]} + f3{na.hi[0], na.hi[1], na.hi[2]}) * 0.5f - c.apex);
+        const float db = dot3(axis, (f3{nb.lo[0], nb.lo[1], nb.lo[2]} + f3{nb.hi[0], nb.hi[1], nb.hi[2]}) * 0.5f - c.apex);
         // packet_first: the first lane's order for the whole packet.
-        const uint32_t swap = uni(db < da ? 1u : 0u);
-        if (swap) {
+        if (uni(db < da ? 1u : 0u)) {
           nearc = node.a + 1;
           farc = node.a;
         }
@@ -689,21 +714,30 @@ __device__ __forceinline__ void scene_coverage(const QueryArgs& a, const Cone& c
       }
     }
   } else {
-    for (uint32_t m = 0; m < s.n_objects; m++) add_mesh_coverage<COUNT>(a, c, m, traced && w.full < BINS, w, st, 0, acc);
+    for (uint32_t m = 0; m < x.n_objects; m++) add_mesh_coverage<COUNT>(x, c, m, traced && w.full < BINS, w, st, 0, L);
   }
 }
 
 template <bool COUNT>
-__global__ __launch_bounds__(64) void cone_query_kernel(QueryArgs a) {
-  __shared__ float s_acc[68 * 64];
+__global__ __launch_bounds__(64) void cone_query_kernel(
+    const NodeC* __restrict__ blas, const NodeC* __restrict__ nodes, const float* __restrict__ tri_leaf,
+    const float* __restrict__ tris, const uint32_t* __restrict__ ranges, const uint32_t* __restrict__ items,
+    const float* __restrict__ blas_cov, const float* __restrict__ xforms, const float* __restrict__ receivers,
+    const float* __restrict__ lights, float* __restrict__ out_visible, uint64_t* __restrict__ out_full,
+    uint64_t* __restrict__ out_touched, float* __restrict__ out_cov, uint32_t* __restrict__ counters,
+    uint64_t* __restrict__ work_out, QueryParams p) {
+  __shared__ float s_acc[ACC_CELLS * 64];
+  __shared__ float s_poly[POLY_SLOTS * 64];
   const uint32_t lane = lane_id();
-  AccRef acc{s_acc + lane, 68u, 0u};
-  const uint64_t q = (a.q_begin + (uint64_t)blockIdx.x * 64u) + lane;
-  const bool valid = q < a.q_end;
-  const uint64_t i = valid ? q % a.n : 0;
-  const uint32_t l = valid ? (uint32_t)(q / a.n) : 0;
-  const float* rc = a.receivers + 8 * i;
-  const LightRect r = reinterpret_cast<const LightRect*>(a.lights)[l];
+  Lane L{s_acc + lane, s_poly + lane, ACC_CELLS, 0u};
+  const Ctx x{blas, nodes, tri_leaf, tris, ranges, items, blas_cov, xforms,
+              p.n_objects, p.hierarchy, p.profile_stage, p.object_cov};
+  const uint64_t q = (p.q_begin + (uint64_t)blockIdx.x * 64u) + lane;
+  const bool valid = q < p.q_end;
+  const uint64_t i = valid ? q % p.n : 0;
+  const uint32_t l = valid ? (uint32_t)(q / p.n) : 0;
+  const float* rc = receivers + 8 * i;
+  const LightRect r = reinterpret_cast<const LightRect*>(lights)[l];
   const f3 point = ld3(rc);
   const f3 normal = ld3(rc + 4);
   const bool active = valid && rc[7] >= 0.0f;
@@ -715,7 +749,7 @@ __global__ __launch_bounds__(64) void cone_query_kernel(QueryArgs a) {
   w.cone_work = 0;
   w.overflow = false;
   w.truncated = false;
-  w.prefilter_bins = a.prefilter;
+  w.prefilter_bins = p.prefilter;
 #pragma unroll
   for (int k = 0; k < 6; k++) w.work[k] = 0;
 
@@ -726,11 +760,12 @@ __global__ __launch_bounds__(64) void cone_query_kernel(QueryArgs a) {
   LightCone lc;
   lc.ok = false;
   lc.wide_clipped = false;
-  if (a.profile_stage != 1) {
-    lc = light_cone_rect(point, r, a.wide_extent);
+  lc.wide_window = false;
+  if (p.profile_stage != 1) {
+    lc = light_cone_rect(point, r, p.wide_extent);
     const bool traced = on && lc.ok;
-    w.cone_cap = lc.wide_window ? a.wide_cone_cap : a.light_cone_cap;
-    if (a.profile_stage != 2) scene_coverage<COUNT>(a, lc.c, traced, w, acc);
+    w.cone_cap = lc.wide_window ? p.wide_cone_cap : p.light_cone_cap;
+    if (p.profile_stage != 2) scene_coverage<COUNT>(x, lc.c, traced, w, L);
     capped = traced && w.cone_work >= w.cone_cap;
     if (w.cone_work >= w.cone_cap && lc.wide_window && !lc.wide_clipped) {
 #pragma unroll
@@ -747,26 +782,26 @@ __global__ __launch_bounds__(64) void cone_query_kernel(QueryArgs a) {
   for (uint32_t b = 0; b < BINS; b++) {
     full_mask |= (uint64_t)(w.cov[b] >= 1.0f) << b;
     touched_mask |= (uint64_t)(w.cov[b] > 0.0f) << b;
-    acc.set(b, w.cov[b]);  // the coverage, for the weights' loop (the accumulator is free now)
+    L.set(b, w.cov[b]);  // the coverage, for the weights' loop (the accumulator is free now)
   }
-  if (a.profile_stage != 1 && on && lc.ok) {
+  if (p.profile_stage != 1 && on && lc.ok) {
     if (is_sphere(r)) {
-      value = sphere_unshadowed(point, normal, r) * sphere_open_share(lc.c, normal, lc.wide_clipped, acc.base);
+      value = sphere_unshadowed(point, normal, r) * sphere_open_share(L, lc.c, normal, lc.wide_clipped);
     } else {
-      value = cone_visible(lc.c, normal, touched_mask, acc.base);
+      value = cone_visible(L, lc.c, normal, touched_mask);
       if (lc.wide_clipped) {
         bool all_above;
-        float kept = window_weight4(lc.c, normal, all_above);
-        value = (kept > 1e-9f ? value / kept : 1.0f) * rect_open(point, normal, r);
+        const float kept = window_weight4(L, lc.c, normal, all_above);
+        value = (kept > 1e-9f ? value / kept : 1.0f) * rect_open(L, point, normal, r);
       }
     }
   }
   if (valid) {
-    a.out_visible[q] = spot * value;
-    a.out_full[q] = full_mask;
-    a.out_touched[q] = touched_mask;
-    if (a.out_cov) {
-      float* oc = a.out_cov + 64 * q;
+    out_visible[q] = spot * value;
+    out_full[q] = full_mask;
+    out_touched[q] = touched_mask;
+    if (out_cov) {
+      float* oc = out_cov + 64 * q;
 #pragma unroll
       for (uint32_t b = 0; b < BINS; b++) oc[b] = w.cov[b];
     }
@@ -774,39 +809,58 @@ __global__ __launch_bounds__(64) void cone_query_kernel(QueryArgs a) {
   // Wave-aggregated diagnostics.
   const uint64_t cap_m = ballot(valid && capped), of_m = ballot(valid && w.overflow),
                  tr_m = ballot(valid && w.truncated);
-  if (lane == 0 && a.counters) {
-    if (cap_m) atomicAdd(&a.counters[0], (uint32_t)__builtin_popcountll(cap_m));
-    if (of_m) atomicAdd(&a.counters[1], (uint32_t)__builtin_popcountll(of_m));
-    if (tr_m) atomicAdd(&a.counters[2], (uint32_t)__builtin_popcountll(tr_m));
+  if (lane == 0 && counters) {
+    if (cap_m) atomicAdd(&counters[0], (uint32_t)__builtin_popcountll(cap_m));
+    if (of_m) atomicAdd(&counters[1], (uint32_t)__builtin_popcountll(of_m));
+    if (tr_m) atomicAdd(&counters[2], (uint32_t)__builtin_popcountll(tr_m));
   }
-  if (COUNT && a.work) {
-    // Per-wave sums of the work counters (lane 0 adds them).
+  if (COUNT && work_out) {
 #pragma unroll
     for (int k = 0; k < 6; k++) {
       uint32_t v = w.work[k];
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) v += (uint32_t)__shfl_xor((int)v, off, 64);
-      if (lane == 0) atomicAdd(reinterpret_cast<unsigned long long*>(&a.work[k]), (unsigned long long)v);
+      if (lane == 0) atomicAdd(reinterpret_cast<unsigned long long*>(&work_out[k]), (unsigned long long)v);
     }
   }
 }
 
-template __global__ void cone_query_kernel<false>(QueryArgs);
-template __global__ void cone_query_kernel<true>(QueryArgs);
+template __global__ void cone_query_kernel<false>(const NodeC*, const NodeC*, const float*, const float*,
+                                                  const uint32_t*, const uint32_t*, const float*, const float*,
+                                                  const float*, const float*, float*, uint64_t*, uint64_t*, float*,
+                                                  uint32_t*, uint64_t*, QueryParams);
+template __global__ void cone_query_kernel<true>(const NodeC*, const NodeC*, const float*, const float*,
+                                                 const uint32_t*, const uint32_t*, const float*, const float*,
+                                                 const float*, const float*, float*, uint64_t*, uint64_t*, float*,
+                                                 uint32_t*, uint64_t*, QueryParams);
 
-}  // namespace cnq
-
-// Launch helpers used by the C ABI.
-namespace cnq {
 hipError_t launch_cone_queries(const QueryArgs& a, bool count_work, hipStream_t stream) {
   if (a.q_end <= a.q_begin) return hipSuccess;
   const uint64_t waves = (a.q_end - a.q_begin + 63) / 64;
   if (waves > 0x7fffffffull) return hipErrorInvalidValue;
+  QueryParams p;
+  p.n = a.n;
+  p.q_begin = a.q_begin;
+  p.q_end = a.q_end;
+  p.n_objects = a.s.n_objects;
+  p.hierarchy = a.hierarchy;
+  p.light_cone_cap = a.light_cone_cap;
+  p.wide_cone_cap = a.wide_cone_cap;
+  p.profile_stage = a.profile_stage;
+  p.object_cov = a.object_cov;
+  p.prefilter = a.prefilter;
+  p.wide_extent = a.wide_extent;
   dim3 grid((uint32_t)waves), block(64);
+  const SceneArgs& s = a.s;
   if (count_work)
-    hipLaunchKernelGGL(cone_query_kernel<true>, grid, block, 0, stream, a);
+    hipLaunchKernelGGL(cone_query_kernel<true>, grid, block, 0, stream, s.blas, s.nodes, s.tri_leaf, s.tris, s.ranges,
+                       s.items, s.blas_cov, s.xforms, a.receivers, a.lights, a.out_visible, a.out_full, a.out_touched,
+                       a.out_cov, a.counters, a.work, p);
   else
-    hipLaunchKernelGGL(cone_query_kernel<false>, grid, block, 0, stream, a);
+    hipLaunchKernelGGL(cone_query_kernel<false>, grid, block, 0, stream, s.blas, s.nodes, s.tri_leaf, s.tris, s.ranges,
+                       s.items, s.blas_cov, s.xforms, a.receivers, a.lights, a.out_visible, a.out_full, a.out_touched,
+                       a.out_cov, a.counters, a.work, p);
   return hipGetLastError();
 }
+
 }  // namespace cnq

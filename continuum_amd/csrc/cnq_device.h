@@ -207,18 +207,47 @@ __device__ __forceinline__ bool may_touch(const Cone& c, f3 v0, f3 v1, f3 v2) {
   return true;
 }
 
-// ---- The accumulation buffer: 68 cells per query in LDS, lane-interleaved
-// (cell i of lane l at i * 64 + l), so every access is bank-conflict free.
-struct AccRef {
-  float* base;  // this lane's cell 0
+// ---- Per-query working memory in LDS, lane-interleaved (slot s of lane l
+// at s * 64 + l), so every access is bank-conflict free whatever slot each
+// lane touches: the 68-cell accumulation buffer, and 70 slots for clip
+// polygons (a 3D polygon of up to 10 vertices at slots 0-29, its clip
+// target at 30-59; 2D polygons at 30-49 and 50-69 once the 3D clip is done),
+// the hull's 8 points, and the horizon-clipped quads of the weights.
+constexpr uint32_t ACC_CELLS = 68;
+constexpr uint32_t POLY_SLOTS = 70;
+constexpr uint32_t P3A = 0, P3B = 30, Q2A = 30, Q2B = 50;
+constexpr uint32_t MAX_POLY = 10;  // WGSL :11
+
+struct Lane {
+  float* acc;   // this lane's accumulation cell 0
+  float* poly;  // this lane's polygon slot 0
   uint32_t span_lo, span_hi;
-  __device__ __forceinline__ void add(uint32_t cell, float v) { base[cell * 64u] += v; }
-  __device__ __forceinline__ float get(uint32_t cell) const { return base[cell * 64u]; }
-  __device__ __forceinline__ void set(uint32_t cell, float v) { base[cell * 64u] = v; }
+  __device__ __forceinline__ void add(uint32_t cell, float v) { acc[cell * 64u] += v; }
+  __device__ __forceinline__ float get(uint32_t cell) const { return acc[cell * 64u]; }
+  __device__ __forceinline__ void set(uint32_t cell, float v) { acc[cell * 64u] = v; }
+  __device__ __forceinline__ f3 p3(uint32_t base, uint32_t i) const {
+    const float* p = poly + (base + 3u * i) * 64u;
+    return f3{p[0], p[64], p[128]};
+  }
+  __device__ __forceinline__ void put3(uint32_t base, uint32_t i, f3 v) {
+    float* p = poly + (base + 3u * i) * 64u;
+    p[0] = v.x;
+    p[64] = v.y;
+    p[128] = v.z;
+  }
+  __device__ __forceinline__ f2 p2(uint32_t base, uint32_t i) const {
+    const float* p = poly + (base + 2u * i) * 64u;
+    return f2{p[0], p[64]};
+  }
+  __device__ __forceinline__ void put2(uint32_t base, uint32_t i, f2 v) {
+    float* p = poly + (base + 2u * i) * 64u;
+    p[0] = v.x;
+    p[64] = v.y;
+  }
 };
 
 // WGSL :297-348
-__device__ __forceinline__ void draw_edge(AccRef& acc, f2 q0, f2 q1, float weight) {
+__device__ __forceinline__ void draw_edge(Lane& acc, f2 q0, f2 q1, float weight) {
   if (fabsf(q0.y - q1.y) <= 1.1920929e-7f) return;
   float dir = 1.0f;
   f2 p0 = q0, p1 = q1;
@@ -270,126 +299,176 @@ __device__ __forceinline__ void draw_edge(AccRef& acc, f2 q0, f2 q1, float weigh
   }
 }
 
-struct Poly3 {
-  f3 v[10];
-  uint32_t n;
-};
-struct Poly2 {
-  f2 v[10];
-  uint32_t n;
-};
-
-// WGSL :264-278. A clip keeps at most 10 vertices (the reference's MAX_POLY,
-// WGSL :11); in exact arithmetic these clips never produce more.
-__device__ __forceinline__ void clip_plane3(const Poly3& p, f3 normal, float offset, Poly3& out) {
-  out.n = 0;
-  for (uint32_t i = 0; i < p.n; i++) {
-    f3 a = p.v[i];
-    f3 b = p.v[i + 1 == p.n ? 0 : i + 1];
-    float da = offset - dot3(normal, a);
-    float db = offset - dot3(normal, b);
-    if (da >= 0.0f && out.n < 10) out.v[out.n++] = a;
-    if (da >= 0.0f && db < 0.0f && out.n < 10) out.v[out.n++] = a + (b - a) * (da / (da - db));
-    if (da < 0.0f && db >= 0.0f && out.n < 10) out.v[out.n++] = b + (a - b) * (db / (db - da));
-  }
-}
-
-// WGSL :280-294
-__device__ __forceinline__ void clip_line2(const Poly2& p, f2 normal, float offset, Poly2& out) {
-  out.n = 0;
-  for (uint32_t i = 0; i < p.n; i++) {
-    f2 a = p.v[i];
-    f2 b = p.v[i + 1 == p.n ? 0 : i + 1];
-    float da = offset - dot2(normal, a);
-    float db = offset - dot2(normal, b);
-    if (da >= 0.0f && out.n < 10) out.v[out.n++] = a;
-    if (da >= 0.0f && db < 0.0f && out.n < 10) out.v[out.n++] = a + (b - a) * (da / (da - db));
-    if (da < 0.0f && db >= 0.0f && out.n < 10) out.v[out.n++] = b + (a - b) * (db / (db - da));
-  }
-}
-
-__device__ __forceinline__ void clip_to_grid(Poly2& q) {
+__device__ __forceinline__ f2 clamp_grid(f2 v) {
   const float g = (float)GRID;
-  Poly2 t;
-  clip_line2(q, f2{-1.0f, 0.0f}, 0.0f, t);
-  clip_line2(t, f2{1.0f, 0.0f}, g, q);
-  clip_line2(q, f2{0.0f, -1.0f}, 0.0f, t);
-  clip_line2(t, f2{0.0f, 1.0f}, g, q);
+  return f2{clampf(v.x, 0.0f, g), clampf(v.y, 0.0f, g)};
 }
 
-__device__ __forceinline__ void draw_poly(AccRef& acc, const Poly2& q, float weight) {
+// WGSL :264-278: keeps dot(normal, x) <= offset; crossings from the kept end.
+// Polygon `src` (n vertices) to `dst`; returns the new count (at most 10).
+__device__ __forceinline__ uint32_t clip_plane3(Lane& L, uint32_t src, uint32_t n, f3 normal, float offset,
+                                                uint32_t dst) {
+  uint32_t m = 0;
+  if (n == 0) return 0;
+  const f3 first = L.p3(src, 0);
+  f3 a = first;
+  float da = offset - dot3(normal, a);
+  for (uint32_t i = 0; i < n; i++) {
+    const f3 b = i + 1 < n ? L.p3(src, i + 1) : first;
+    const float db = offset - dot3(normal, b);
+    if (da >= 0.0f && m < MAX_POLY) L.put3(dst, m++, a);
+    if (da >= 0.0f && db < 0.0f && m < MAX_POLY) L.put3(dst, m++, a + (b - a) * (da / (da - db)));
+    if (da < 0.0f && db >= 0.0f && m < MAX_POLY) L.put3(dst, m++, b + (a - b) * (db / (db - da)));
+    a = b;
+    da = db;
+  }
+  return m;
+}
+
+// WGSL :280-294, in the projected plane.
+__device__ __forceinline__ uint32_t clip_line2(Lane& L, uint32_t src, uint32_t n, f2 normal, float offset,
+                                               uint32_t dst) {
+  uint32_t m = 0;
+  if (n == 0) return 0;
+  const f2 first = L.p2(src, 0);
+  f2 a = first;
+  float da = offset - dot2(normal, a);
+  for (uint32_t i = 0; i < n; i++) {
+    const f2 b = i + 1 < n ? L.p2(src, i + 1) : first;
+    const float db = offset - dot2(normal, b);
+    if (da >= 0.0f && m < MAX_POLY) L.put2(dst, m++, a);
+    if (da >= 0.0f && db < 0.0f && m < MAX_POLY) L.put2(dst, m++, a + (b - a) * (da / (da - db)));
+    if (da < 0.0f && db >= 0.0f && m < MAX_POLY) L.put2(dst, m++, b + (a - b) * (db / (db - da)));
+    a = b;
+    da = db;
+  }
+  return m;
+}
+
+// The four clips to [0, 8]^2 (WGSL :432-437), ping-ponging between the two
+// 2D slots; `at` is where the polygon is, and is where it ends up.
+__device__ __forceinline__ uint32_t clip_to_grid(Lane& L, uint32_t& at, uint32_t n) {
   const float g = (float)GRID;
-  for (uint32_t i = 0; i < q.n; i++) {
-    f2 a = q.v[i];
-    f2 b = q.v[i + 1 == q.n ? 0 : i + 1];
-    draw_edge(acc, f2{clampf(a.x, 0.0f, g), clampf(a.y, 0.0f, g)}, f2{clampf(b.x, 0.0f, g), clampf(b.y, 0.0f, g)},
-              weight);
+  const uint32_t other = at == Q2A ? Q2B : Q2A;
+  n = clip_line2(L, at, n, f2{-1.0f, 0.0f}, 0.0f, other);
+  n = clip_line2(L, other, n, f2{1.0f, 0.0f}, g, at);
+  n = clip_line2(L, at, n, f2{0.0f, -1.0f}, 0.0f, other);
+  n = clip_line2(L, other, n, f2{0.0f, 1.0f}, g, at);
+  return n;
+}
+
+__device__ __forceinline__ void draw_poly(Lane& L, uint32_t at, uint32_t n, float weight) {
+  const f2 first = clamp_grid(L.p2(at, 0));
+  f2 a = first;
+  for (uint32_t i = 0; i < n; i++) {
+    const f2 b = i + 1 < n ? clamp_grid(L.p2(at, i + 1)) : first;
+    draw_edge(L, a, b, weight);
+    a = b;
   }
 }
 
-// WGSL :350-451
-__device__ __noinline__ bool rasterize(const Cone& c, AccRef& acc, f3 v0, f3 v1, f3 v2, float weight) {
-  Poly3 p, t3;
-  p.v[0] = v0;
-  p.v[1] = v1;
-  p.v[2] = v2;
-  p.n = 3;
-  float f0 = c.far_offset - dot3(c.forward, v0);
-  float f1 = c.far_offset - dot3(c.forward, v1);
-  float f2_ = c.far_offset - dot3(c.forward, v2);
-  if (f0 < 0.0f || f1 < 0.0f || f2_ < 0.0f) {
-    clip_plane3(p, c.forward, c.far_offset, t3);
-    p = t3;
+__device__ __forceinline__ bool in_window(const Cone& c, f3 v) {
+  return !(v.x < c.w0 * v.z || v.x > c.w2 * v.z || v.y < c.w1 * v.z || v.y > c.w3 * v.z);
+}
+
+__device__ __forceinline__ f2 project(const Cone& c, f3 v, float su, float sv) {
+  return f2{(v.x / v.z - c.w0) * su, (v.y / v.z - c.w1) * sv};
+}
+
+__device__ __forceinline__ bool in_grid(f2 q) {
+  const float g = (float)GRID;
+  return !(q.x < 0.0f || q.y < 0.0f || q.x > g || q.y > g);
+}
+
+// WGSL :350-451. A triangle wholly in front of the near plane, before the far
+// plane and inside the window projects to a triangle inside the grid, drawn
+// straight from registers; any other case builds its polygons in LDS.
+__device__ __forceinline__ bool rasterize(const Cone& c, Lane& L, f3 v0, f3 v1, f3 v2, float weight) {
+  const float f0 = c.far_offset - dot3(c.forward, v0);
+  const float f1 = c.far_offset - dot3(c.forward, v1);
+  const float f2_ = c.far_offset - dot3(c.forward, v2);
+  const bool far_clip = f0 < 0.0f || f1 < 0.0f || f2_ < 0.0f;
+  const float su = (float)GRID / (c.w2 - c.w0);
+  const float sv = (float)GRID / (c.w3 - c.w1);
+  const float g = (float)GRID;
+  if (!far_clip) {
+    const f3 l0 = to_local(c, v0), l1 = to_local(c, v1), l2 = to_local(c, v2);
+    if (l0.z >= NEAR && l1.z >= NEAR && l2.z >= NEAR && in_window(c, l0) && in_window(c, l1) && in_window(c, l2)) {
+      const f2 q0 = project(c, l0, su, sv), q1 = project(c, l1, su, sv), q2 = project(c, l2, su, sv);
+      if (in_grid(q0) && in_grid(q1) && in_grid(q2)) {
+        draw_edge(L, q0, q1, weight);
+        draw_edge(L, q1, q2, weight);
+        draw_edge(L, q2, q0, weight);
+        return true;
+      }
+      // Rounding put a corner a hair outside the grid: clip as the general path does.
+      L.put2(Q2A, 0, q0);
+      L.put2(Q2A, 1, q1);
+      L.put2(Q2A, 2, q2);
+      uint32_t at = Q2A;
+      const uint32_t n = clip_to_grid(L, at, 3);
+      if (n < 3) return false;
+      draw_poly(L, at, n, weight);
+      return true;
+    }
+  }
+  // General path: the triangle as a polygon at P3A.
+  L.put3(P3A, 0, v0);
+  L.put3(P3A, 1, v1);
+  L.put3(P3A, 2, v2);
+  uint32_t n = 3;
+  if (far_clip) {
+    n = clip_plane3(L, P3A, n, c.forward, c.far_offset, P3B);
+    for (uint32_t i = 0; i < n; i++) L.put3(P3A, i, L.p3(P3B, i));
   }
   bool all_near_ok = true, all_in_window = true;
-  for (uint32_t i = 0; i < p.n; i++) {
-    p.v[i] = to_local(c, p.v[i]);
-    f3 v = p.v[i];
+  for (uint32_t i = 0; i < n; i++) {
+    const f3 v = to_local(c, L.p3(P3A, i));
+    L.put3(P3A, i, v);
     if (v.z < NEAR) all_near_ok = false;
-    if (v.x < c.w0 * v.z || v.x > c.w2 * v.z || v.y < c.w1 * v.z || v.y > c.w3 * v.z) all_in_window = false;
+    if (!in_window(c, v)) all_in_window = false;
   }
   if (!all_near_ok) {
-    clip_plane3(p, f3{0.0f, 0.0f, -1.0f}, -NEAR, t3);
-    p = t3;
+    n = clip_plane3(L, P3A, n, f3{0.0f, 0.0f, -1.0f}, -NEAR, P3B);
+    for (uint32_t i = 0; i < n; i++) L.put3(P3A, i, L.p3(P3B, i));
   }
-  if (p.n < 3) return false;
-  float su = (float)GRID / (c.w2 - c.w0);
-  float sv = (float)GRID / (c.w3 - c.w1);
-  const float g = (float)GRID;
-  Poly2 q, t2;
+  if (n < 3) return false;
+  uint32_t at = Q2A, qn;
   if (all_near_ok && all_in_window) {
-    q.n = p.n;
-    for (uint32_t i = 0; i < p.n; i++)
-      q.v[i] = f2{(p.v[i].x / p.v[i].z - c.w0) * su, (p.v[i].y / p.v[i].z - c.w1) * sv};
+    qn = n;
+    for (uint32_t i = 0; i < n; i++) L.put2(Q2A, i, project(c, L.p3(P3A, i), su, sv));
   } else {
-    float d = dot3(to_local(c, v0), cross3(to_local(c, v1), to_local(c, v2)));
+    // The window's square clipped by the planes through the apex and each
+    // edge, with the whole triangle's winding (WGSL :376-425).
+    const float d = dot3(to_local(c, v0), cross3(to_local(c, v1), to_local(c, v2)));
     if (d == 0.0f) return false;
-    float side = d > 0.0f ? 1.0f : -1.0f;
-    q.n = 4;
-    q.v[0] = f2{0.0f, 0.0f};
-    q.v[2] = f2{g, g};
+    const float side = d > 0.0f ? 1.0f : -1.0f;
+    qn = 4;
+    L.put2(Q2A, 0, f2{0.0f, 0.0f});
+    L.put2(Q2A, 2, f2{g, g});
     if (side > 0.0f) {
-      q.v[1] = f2{g, 0.0f};
-      q.v[3] = f2{0.0f, g};
+      L.put2(Q2A, 1, f2{g, 0.0f});
+      L.put2(Q2A, 3, f2{0.0f, g});
     } else {
-      q.v[1] = f2{0.0f, g};
-      q.v[3] = f2{g, 0.0f};
+      L.put2(Q2A, 1, f2{0.0f, g});
+      L.put2(Q2A, 3, f2{g, 0.0f});
     }
-    for (uint32_t i = 0; i < p.n; i++) {
-      f3 e = cross3(p.v[i], p.v[i + 1 == p.n ? 0 : i + 1]);
-      float len = len3(e);
+    for (uint32_t i = 0; i < n; i++) {
+      const f3 e = cross3(L.p3(P3A, i), L.p3(P3A, i + 1 == n ? 0 : i + 1));
+      const float len = len3(e);
       if (len == 0.0f) continue;
-      f3 k = e * (side / len);
-      clip_line2(q, f2{-k.x / su, -k.y / sv}, k.x * c.w0 + k.y * c.w1 + k.z, t2);
-      q = t2;
+      const f3 k = e * (side / len);
+      const uint32_t to = at == Q2A ? Q2B : Q2A;
+      qn = clip_line2(L, at, qn, f2{-k.x / su, -k.y / sv}, k.x * c.w0 + k.y * c.w1 + k.z, to);
+      at = to;
     }
   }
   bool inside = true;
-  for (uint32_t i = 0; i < q.n; i++)
-    if (q.v[i].x < 0.0f || q.v[i].y < 0.0f || q.v[i].x > g || q.v[i].y > g) inside = false;
-  if (!inside) clip_to_grid(q);
-  if (q.n < 3) return false;
-  draw_poly(acc, q, weight);
+  for (uint32_t i = 0; i < qn; i++)
+    if (!in_grid(L.p2(at, i))) inside = false;
+  if (!inside) qn = clip_to_grid(L, at, qn);
+  if (qn < 3) return false;
+  draw_poly(L, at, qn, weight);
   return true;
 }
 

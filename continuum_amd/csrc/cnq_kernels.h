@@ -68,6 +68,14 @@ struct QueryArgs {
                                       //  meshes entered, blas splats]
 };
 
+// The cone kernel's scalar parameters (its arrays are separate, noalias
+// kernel arguments).
+struct QueryParams {
+  uint64_t n, q_begin, q_end;
+  uint32_t n_objects, hierarchy, light_cone_cap, wide_cone_cap, profile_stage, object_cov;
+  float prefilter, wide_extent;
+};
+
 struct PrimaryArgs {
   SceneArgs s;
   const float* __restrict__ camera;  // eye, forward, right, up, params (20 floats)

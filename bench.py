@@ -67,6 +67,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--width", type=int, default=1280)
     ap.add_argument("--height", type=int, default=720)
+    ap.add_argument("--tile-w", type=int, default=8, help="pixels across a wave's tile (tile_w x 64/tile_w)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
@@ -121,7 +122,7 @@ def main() -> int:
         scene.primary_device(cam, W, H, lights, opts, hits, recv, stream)
         if e1 is not None:
             e1.record(stream)
-        scene.query_device(recv, lights, opts, vis, full, touched, None, stream)
+        scene.view_query_device(recv, W, H, lights, opts, vis, full, touched, tile_w=args.tile_w, stream=stream)
         if e2 is not None:
             e2.record(stream)
 

@@ -46,6 +46,15 @@ class SceneDesc(C.Structure):
     ]
 
 
+class View(C.Structure):
+    _fields_ = [("camera", C.c_float * 20), ("width", C.c_uint32), ("height", C.c_uint32),
+                ("light_first", C.c_uint32), ("light_count", C.c_uint32)]
+
+
+class Tuning(C.Structure):
+    _fields_ = [("tile_w", C.c_uint32), ("persistent_blocks", C.c_uint32)]
+
+
 # Every symbol include/cnq.h declares, with its ctypes signature.
 _P = C.c_void_p
 SIGNATURES = {
@@ -63,6 +72,18 @@ SIGNATURES = {
     "cnq_irradiance": (C.c_int, [_P, _P, C.c_uint64, _P, C.c_uint32, C.POINTER(Options), _P, _P, _P, _P]),
     "cnq_irradiance_device": (
         C.c_int, [_P, _P, C.c_uint64, _P, C.c_uint32, C.POINTER(Options), _P, _P, _P, _P, _P]),
+    "cnq_view_irradiance_device": (
+        C.c_int, [_P, _P, C.c_uint32, C.c_uint32, _P, C.c_uint32, C.POINTER(Options), C.c_uint32, _P, _P, _P, _P]),
+    "cnq_tuning_default": (None, [C.POINTER(Tuning)]),
+    "cnq_batch_create": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_uint32, C.POINTER(_P)]),
+    "cnq_batch_destroy": (None, [_P]),
+    "cnq_batch_queries": (C.c_uint64, [_P]),
+    "cnq_batch_run": (C.c_int, [_P, C.POINTER(Options), C.POINTER(Tuning), _P]),
+    "cnq_batch_capture": (C.c_int, [_P, C.POINTER(Options), C.POINTER(Tuning)]),
+    "cnq_batch_replay": (C.c_int, [_P, _P]),
+    "cnq_batch_autotune": (C.c_int, [_P, C.POINTER(Options), _P, C.POINTER(Tuning), C.POINTER(C.c_float)]),
+    "cnq_batch_read": (C.c_int, [_P, C.c_uint32, _P, _P, _P, _P]),
+    "cnq_batch_set_lights": (C.c_int, [_P, _P, C.c_uint32]),
     "cnq_primary_device": (
         C.c_int, [_P, _P, C.c_uint32, C.c_uint32, _P, C.c_uint32, C.POINTER(Options), _P, _P, _P]),
     "cnq_render_view": (

@@ -217,29 +217,67 @@ int validate(const cnq_scene_desc* d) {
   return CNQ_OK;
 }
 
-cnq::QueryArgs query_args(cnq_scene* s, const float* receivers, uint64_t n, const float* lights, uint32_t n_lights,
-                          const cnq_options* o, float* vis, uint64_t* full, uint64_t* touched, float* cov) {
-  cnq::QueryArgs a;
+uint32_t log2u(uint32_t v) {
+  uint32_t l = 0;
+  while ((1u << l) < v) l++;
+  return l;
+}
+
+// A view's work items for a cone tile tile_w x (64 / tile_w), laid out from
+// item `item_off`; returns the items it takes.
+uint64_t layout_view(cnq::ViewDesc& v, uint32_t tile_w, uint64_t item_off) {
+  const uint32_t tw = tile_w, th = 64u / tile_w;
+  v.tiles_x = (v.w + tw - 1) / tw;
+  const uint64_t tiles = (uint64_t)v.tiles_x * ((v.h + th - 1) / th);
+  v.tiles = (uint32_t)tiles;
+  v.item_off = (uint32_t)item_off;
+  return tiles * v.n_lights;
+}
+
+cnq::QueryParams query_params(cnq_scene* s, const cnq_options* o, uint32_t tile_w) {
+  cnq::QueryParams p;
+  p.n_objects = s->n_objects;
+  p.hierarchy = o->hierarchy;
+  p.light_cone_cap = o->light_cone_cap;
+  p.wide_cone_cap = o->wide_cone_cap;
+  p.profile_stage = o->profile_stage;
+  p.object_cov = o->object_cov;
+  p.prefilter = o->prefilter;
+  p.wide_extent = o->wide_extent;
+  p.n_views = 1;
+  p.n_items = 0;
+  p.item_begin = 0;
+  p.tile_w_log2 = log2u(tile_w);
+  return p;
+}
+
+cnq::ItemArgs item_args(cnq_scene* s, const float* receivers, const float* lights, float* vis, uint64_t* full,
+                        uint64_t* touched, float* cov) {
+  cnq::ItemArgs a;
   a.s = s->args();
   a.receivers = receivers;
   a.lights = lights;
-  a.n = n;
-  a.q_begin = 0;
-  a.q_end = n * (uint64_t)n_lights;
-  a.hierarchy = o->hierarchy;
-  a.light_cone_cap = o->light_cone_cap;
-  a.wide_cone_cap = o->wide_cone_cap;
-  a.profile_stage = o->profile_stage;
-  a.object_cov = o->object_cov;
-  a.prefilter = o->prefilter;
-  a.wide_extent = o->wide_extent;
+  a.views = nullptr;
   a.out_visible = vis;
   a.out_full = full;
   a.out_touched = touched;
   a.out_cov = cov;
   a.counters = s->counters.p;
   a.work = s->count_work ? s->work.p : nullptr;
+  a.work_counter = nullptr;
   return a;
+}
+
+// Launches the items of `a` in direct mode, at most 2^31 - 1 blocks a launch.
+hipError_t launch_items_direct(cnq::ItemArgs a, uint64_t n_items, bool count_work, hipStream_t stream) {
+  const uint64_t per = 0x7fffffffull;
+  for (uint64_t i0 = 0; i0 < n_items; i0 += per) {
+    a.p.item_begin = (uint32_t)i0;
+    a.p.n_items = (uint32_t)(i0 + per < n_items ? i0 + per : n_items);
+    hipError_t e = cnq::launch_cone_items(a, count_work, 0, stream);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 int check_object_cov(cnq_scene* s, const cnq_options* o) {
@@ -411,16 +449,47 @@ int cnq_irradiance_device(cnq_scene* s, const float* receivers, uint64_t n, cons
   if ((rc = check_object_cov(s, o))) return rc;
   if (n == 0 || n_lights == 0) return CNQ_OK;
   if (!receivers || !lights || !vis || !full || !touched) return fail(CNQ_INVALID_ARG, "null buffer");
-  if (n > (1ull << 40) / n_lights) return fail(CNQ_CAPACITY, "too many queries");
-  cnq::QueryArgs a = query_args(s, receivers, n, lights, n_lights, o, vis, full, touched, cov);
-  // At most 2^31 - 1 waves per launch.
-  const uint64_t per_launch = 64ull * 0x7fffffffull;
-  for (uint64_t q0 = 0; q0 < a.q_end; q0 += per_launch) {
-    cnq::QueryArgs b = a;
-    b.q_begin = q0;
-    b.q_end = q0 + per_launch < a.q_end ? q0 + per_launch : a.q_end;
-    HIP_TRY(cnq::launch_cone_queries(b, s->count_work, (hipStream_t)stream), "cone query launch");
-  }
+  if (n > 0xffffffffull || (n + 63) / 64 * n_lights > 0xffffffffull)
+    return fail(CNQ_CAPACITY, "too many queries for one call");
+  // The receivers as an n x 1 view, tiles of 64 consecutive receivers.
+  cnq::ItemArgs a = item_args(s, receivers, lights, vis, full, touched, cov);
+  a.p = query_params(s, o, 64);
+  cnq::ViewDesc& v = a.v0;
+  v.recv_off = 0;
+  v.out_off = 0;
+  v.w = (uint32_t)n;
+  v.h = 1;
+  v.light_off = 0;
+  v.n_lights = n_lights;
+  v.ptile_off = 0;
+  const uint64_t items = layout_view(v, 64, 0);
+  HIP_TRY(launch_items_direct(a, items, s->count_work, (hipStream_t)stream), "cone query launch");
+  return CNQ_OK;
+}
+
+int cnq_view_irradiance_device(cnq_scene* s, const float* receivers, uint32_t width, uint32_t height,
+                               const float* lights, uint32_t n_lights, const cnq_options* o, uint32_t tile_w,
+                               float* vis, uint64_t* full, uint64_t* touched, void* stream) {
+  if (!s) return fail(CNQ_INVALID_ARG, "scene is null");
+  int rc = check_options(o);
+  if (rc) return rc;
+  if ((rc = check_object_cov(s, o))) return rc;
+  if (tile_w == 0 || tile_w > 64 || (tile_w & (tile_w - 1))) return fail(CNQ_INVALID_ARG, "tile_w must be 1..64, a power of two");
+  if ((uint64_t)width * height == 0 || n_lights == 0) return CNQ_OK;
+  if (!receivers || !lights || !vis || !full || !touched) return fail(CNQ_INVALID_ARG, "null buffer");
+  cnq::ItemArgs a = item_args(s, receivers, lights, vis, full, touched, nullptr);
+  a.p = query_params(s, o, tile_w);
+  cnq::ViewDesc& v = a.v0;
+  v.recv_off = 0;
+  v.out_off = 0;
+  v.w = width;
+  v.h = height;
+  v.light_off = 0;
+  v.n_lights = n_lights;
+  v.ptile_off = 0;
+  const uint64_t items = layout_view(v, tile_w, 0);
+  if (items > 0xffffffffull) return fail(CNQ_CAPACITY, "too many queries for one call");
+  HIP_TRY(launch_items_direct(a, items, s->count_work, (hipStream_t)stream), "cone query launch");
   return CNQ_OK;
 }
 
@@ -469,11 +538,20 @@ int cnq_primary_device(cnq_scene* s, const float* camera, uint32_t width, uint32
   if (!camera || !hits || !receivers || (n_lights && !lights)) return fail(CNQ_INVALID_ARG, "null buffer");
   cnq::PrimaryArgs a;
   a.s = s->args();
-  a.camera = camera;
+  a.cameras = camera;
   a.lights = lights;
-  a.n_lights = n_lights;
-  a.width = width;
-  a.height = height;
+  a.views = nullptr;
+  cnq::ViewDesc& v = a.v0;
+  v = cnq::ViewDesc{};
+  v.w = width;
+  v.h = height;
+  v.light_off = 0;
+  v.n_lights = n_lights;
+  v.ptile_off = 0;
+  const uint64_t tiles = (uint64_t)((width + 7) / 8) * ((height + 7) / 8);
+  if (tiles > 0x7fffffffull) return fail(CNQ_CAPACITY, "view too large");
+  a.n_views = 1;
+  a.n_tiles = (uint32_t)tiles;
   a.hierarchy = o->hierarchy;
   a.nearest_first = o->nearest_first;
   a.hits = hits;
@@ -509,7 +587,8 @@ int cnq_render_view(cnq_scene* s, const float* camera, uint32_t width, uint32_t 
     return hip_fail(e, "view buffers");
   }
   rc = cnq_primary_device(s, cam.p, width, height, l.p, n_lights, o, h.p, recv.p, nullptr);
-  if (rc == CNQ_OK && n_lights) rc = cnq_irradiance_device(s, recv.p, px, l.p, n_lights, o, v.p, f.p, t.p, nullptr, nullptr);
+  if (rc == CNQ_OK && n_lights)
+    rc = cnq_view_irradiance_device(s, recv.p, width, height, l.p, n_lights, o, 8, v.p, f.p, t.p, nullptr);
   if (rc == CNQ_OK) {
     e = hipDeviceSynchronize();
     if (e == hipSuccess && out_hits) e = hipMemcpy(out_hits, h.p, 16 * px, hipMemcpyDeviceToHost);
@@ -630,6 +709,280 @@ int cnq_scene_counters(cnq_scene* s, int reset, uint32_t counters[3], uint64_t w
 int cnq_scene_set_count_work(cnq_scene* s, int on) {
   if (!s) return fail(CNQ_INVALID_ARG, "scene is null");
   s->count_work = on != 0;
+  return CNQ_OK;
+}
+
+}  // extern "C"
+
+// ---- Batches: many views in one camera pass and one cone launch.
+struct cnq_batch {
+  cnq_scene* scene = nullptr;
+  uint32_t n_views = 0, n_lights = 0, n_ptiles = 0;
+  uint64_t px_total = 0, q_total = 0, n_items = 0;
+  std::vector<cnq::ViewDesc> h_views;
+  uint32_t layout_tile_w = 0;
+  DevBuf<cnq::ViewDesc> views;
+  DevBuf<float> cameras, lights, receivers, visible;
+  DevBuf<uint32_t> hits, work_counter;
+  DevBuf<uint64_t> full, touched;
+  hipStream_t own_stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  cnq_tuning graph_tuning{};
+
+  void release() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (own_stream) (void)hipStreamDestroy(own_stream);
+    exec = nullptr, graph = nullptr, own_stream = nullptr;
+    views.release(), cameras.release(), lights.release(), receivers.release(), visible.release();
+    hits.release(), work_counter.release(), full.release(), touched.release();
+  }
+};
+
+namespace {
+
+int check_tuning(const cnq_tuning* t) {
+  if (!t) return fail(CNQ_INVALID_ARG, "tuning is null");
+  if (t->tile_w == 0 || t->tile_w > 64 || (t->tile_w & (t->tile_w - 1)))
+    return fail(CNQ_INVALID_ARG, "tile_w must be 1..64, a power of two");
+  return CNQ_OK;
+}
+
+// Lays the batch's work items out for a cone tile width and uploads the
+// view table (synchronously: never during a capture).
+int batch_layout(cnq_batch* b, uint32_t tile_w) {
+  if (b->layout_tile_w == tile_w) return CNQ_OK;
+  uint64_t item = 0;
+  for (auto& v : b->h_views) {
+    item += layout_view(v, tile_w, item);
+    if (item > 0xffffffffull) return fail(CNQ_CAPACITY, "a batch holds at most 2^32 - 1 work items");
+  }
+  b->n_items = item;
+  HIP_TRY(hipMemcpy(b->views.p, b->h_views.data(), b->h_views.size() * sizeof(cnq::ViewDesc), hipMemcpyHostToDevice),
+          "view table");
+  b->layout_tile_w = tile_w;
+  return CNQ_OK;
+}
+
+// Enqueues the camera pass and the cone pass of a laid-out batch.
+int batch_enqueue(cnq_batch* b, const cnq_options* o, const cnq_tuning* t, hipStream_t st) {
+  cnq_scene* s = b->scene;
+  cnq::PrimaryArgs pa;
+  pa.s = s->args();
+  pa.cameras = b->cameras.p;
+  pa.lights = b->lights.p;
+  pa.views = b->views.p;
+  pa.v0 = b->h_views[0];
+  pa.n_views = b->n_views;
+  pa.n_tiles = b->n_ptiles;
+  pa.hierarchy = o->hierarchy;
+  pa.nearest_first = o->nearest_first;
+  pa.hits = b->hits.p;
+  pa.receivers = b->receivers.p;
+  pa.counters = s->counters.p + 3;
+  HIP_TRY(cnq::launch_primary(pa, st), "batch camera pass");
+  cnq::ItemArgs a = item_args(s, b->receivers.p, b->lights.p, b->visible.p, b->full.p, b->touched.p, nullptr);
+  a.p = query_params(s, o, t->tile_w);
+  a.p.n_views = b->n_views;
+  a.views = b->views.p;
+  a.v0 = b->h_views[0];
+  a.work_counter = b->work_counter.p;
+  if (t->persistent_blocks) {
+    HIP_TRY(fill_u32(b->work_counter.p, 0, 1, st), "work counter");
+    a.p.item_begin = 0;
+    a.p.n_items = (uint32_t)b->n_items;
+    HIP_TRY(cnq::launch_cone_items(a, s->count_work, t->persistent_blocks, st), "batch cone pass");
+  } else {
+    HIP_TRY(launch_items_direct(a, b->n_items, s->count_work, st), "batch cone pass");
+  }
+  return CNQ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void cnq_tuning_default(cnq_tuning* t) {
+  t->tile_w = 8;
+  t->persistent_blocks = 0;
+}
+
+int cnq_batch_create(cnq_scene* s, const cnq_view* views, uint32_t n_views, const float* lights, uint32_t n_lights,
+                     cnq_batch** out) {
+  if (!s || !views || !out || n_views == 0) return fail(CNQ_INVALID_ARG, "null argument or no views");
+  if (n_lights && !lights) return fail(CNQ_INVALID_ARG, "lights is null");
+  HIP_TRY(hipSetDevice(s->dev->ordinal), "hipSetDevice");
+  cnq_batch* b = new (std::nothrow) cnq_batch;
+  if (!b) return fail(CNQ_OOM, "out of host memory");
+  b->scene = s;
+  b->n_views = n_views;
+  b->n_lights = n_lights;
+  b->h_views.resize(n_views);
+  uint64_t px = 0, q = 0, ptiles = 0;
+  std::vector<float> cams(20 * (size_t)n_views);
+  for (uint32_t i = 0; i < n_views; i++) {
+    const cnq_view& cv = views[i];
+    if (cv.width == 0 || cv.height == 0 || (uint64_t)cv.light_first + cv.light_count > n_lights) {
+      delete b;
+      return fail(CNQ_INVALID_ARG, "view " + std::to_string(i) + ": empty, or lights outside the batch's");
+    }
+    cnq::ViewDesc& v = b->h_views[i];
+    v = cnq::ViewDesc{};
+    v.recv_off = px;
+    v.out_off = q;
+    v.w = cv.width;
+    v.h = cv.height;
+    v.light_off = cv.light_first;
+    v.n_lights = cv.light_count;
+    v.ptile_off = (uint32_t)ptiles;
+    px += (uint64_t)cv.width * cv.height;
+    q += (uint64_t)cv.width * cv.height * cv.light_count;
+    ptiles += (uint64_t)((cv.width + 7) / 8) * ((cv.height + 7) / 8);
+    std::memcpy(&cams[20 * (size_t)i], cv.camera, 80);
+  }
+  if (ptiles > 0x7fffffffull) {
+    delete b;
+    return fail(CNQ_CAPACITY, "too many pixels in one batch");
+  }
+  b->px_total = px;
+  b->q_total = q;
+  b->n_ptiles = (uint32_t)ptiles;
+  hipError_t e = b->views.alloc(n_views);
+  if (e == hipSuccess) e = b->cameras.upload(cams.data(), cams.size());
+  if (e == hipSuccess) e = b->lights.upload(lights, 16 * (size_t)n_lights);
+  if (e == hipSuccess) e = b->receivers.alloc(8 * px);
+  if (e == hipSuccess) e = b->hits.alloc(4 * px);
+  if (e == hipSuccess) e = b->visible.alloc(q);
+  if (e == hipSuccess) e = b->full.alloc(q);
+  if (e == hipSuccess) e = b->touched.alloc(q);
+  if (e == hipSuccess) e = b->work_counter.alloc(1);
+  if (e != hipSuccess) {
+    b->release();
+    delete b;
+    return hip_fail(e, "batch buffers");
+  }
+  int rc = batch_layout(b, 8);
+  if (rc) {
+    b->release();
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return CNQ_OK;
+}
+
+void cnq_batch_destroy(cnq_batch* b) {
+  if (!b) return;
+  (void)hipSetDevice(b->scene->dev->ordinal);
+  b->release();
+  delete b;
+}
+
+uint64_t cnq_batch_queries(const cnq_batch* b) { return b ? b->q_total : 0; }
+
+int cnq_batch_run(cnq_batch* b, const cnq_options* o, const cnq_tuning* t, void* stream) {
+  if (!b) return fail(CNQ_INVALID_ARG, "batch is null");
+  int rc = check_options(o);
+  if (rc || (rc = check_tuning(t)) || (rc = check_object_cov(b->scene, o))) return rc;
+  if ((rc = batch_layout(b, t->tile_w))) return rc;
+  return batch_enqueue(b, o, t, (hipStream_t)stream);
+}
+
+int cnq_batch_capture(cnq_batch* b, const cnq_options* o, const cnq_tuning* t) {
+  if (!b) return fail(CNQ_INVALID_ARG, "batch is null");
+  int rc = check_options(o);
+  if (rc || (rc = check_tuning(t)) || (rc = check_object_cov(b->scene, o))) return rc;
+  if ((rc = batch_layout(b, t->tile_w))) return rc;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  if (!b->own_stream) HIP_TRY(hipStreamCreateWithFlags(&b->own_stream, hipStreamNonBlocking), "capture stream");
+  HIP_TRY(hipStreamBeginCapture(b->own_stream, hipStreamCaptureModeThreadLocal), "begin capture");
+  rc = batch_enqueue(b, o, t, b->own_stream);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(b->own_stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  if (e != hipSuccess) return hip_fail(e, "end capture");
+  b->graph = g;
+  HIP_TRY(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0), "graph instantiate");
+  b->graph_tuning = *t;
+  return CNQ_OK;
+}
+
+int cnq_batch_replay(cnq_batch* b, void* stream) {
+  if (!b || !b->exec) return fail(CNQ_INVALID_ARG, "no captured graph");
+  if (b->layout_tile_w != b->graph_tuning.tile_w) return fail(CNQ_INVALID_ARG, "the batch was laid out again after capture");
+  HIP_TRY(hipGraphLaunch(b->exec, (hipStream_t)stream), "graph launch");
+  return CNQ_OK;
+}
+
+int cnq_batch_autotune(cnq_batch* b, const cnq_options* o, void* stream, cnq_tuning* best, float* best_ms) {
+  if (!b || !best) return fail(CNQ_INVALID_ARG, "null argument");
+  int rc = check_options(o);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  int cus = 256;
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->scene->dev->ordinal);
+  const uint32_t tiles[] = {8, 16, 4, 64};
+  const uint32_t per_cu[] = {0, 4, 8};
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0), "event");
+  HIP_TRY(hipEventCreate(&e1), "event");
+  float best_t = 3.4e38f;
+  cnq_tuning best_c{};
+  for (uint32_t tw : tiles) {
+    for (uint32_t k : per_cu) {
+      cnq_tuning c{tw, k * (uint32_t)cus};
+      if ((rc = cnq_batch_run(b, o, &c, st))) break;  // warm
+      float total = 0.0f;
+      for (int r = 0; r < 2 && !rc; r++) {
+        (void)hipEventRecord(e0, st);
+        rc = batch_enqueue(b, o, &c, st);
+        (void)hipEventRecord(e1, st);
+        (void)hipEventSynchronize(e1);
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        total += ms;
+      }
+      if (rc) break;
+      if (total < best_t) {
+        best_t = total;
+        best_c = c;
+      }
+    }
+    if (rc) break;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  *best = best_c;
+  if (best_ms) *best_ms = best_t / 2.0f;
+  return batch_layout(b, best_c.tile_w);
+}
+
+int cnq_batch_read(cnq_batch* b, uint32_t view, uint32_t* hits, float* visible, uint64_t* full, uint64_t* touched) {
+  if (!b || view >= b->n_views) return fail(CNQ_INVALID_ARG, "no such view");
+  HIP_TRY(hipSetDevice(b->scene->dev->ordinal), "hipSetDevice");
+  HIP_TRY(hipDeviceSynchronize(), "sync");
+  const cnq::ViewDesc& v = b->h_views[view];
+  const uint64_t px = (uint64_t)v.w * v.h, q = px * v.n_lights;
+  if (hits) HIP_TRY(hipMemcpy(hits, b->hits.p + 4 * v.recv_off, 16 * px, hipMemcpyDeviceToHost), "read hits");
+  if (q && visible) HIP_TRY(hipMemcpy(visible, b->visible.p + v.out_off, 4 * q, hipMemcpyDeviceToHost), "read visible");
+  if (q && full) HIP_TRY(hipMemcpy(full, b->full.p + v.out_off, 8 * q, hipMemcpyDeviceToHost), "read full");
+  if (q && touched) HIP_TRY(hipMemcpy(touched, b->touched.p + v.out_off, 8 * q, hipMemcpyDeviceToHost), "read touched");
+  return CNQ_OK;
+}
+
+int cnq_batch_set_lights(cnq_batch* b, const float* lights, uint32_t n_lights) {
+  if (!b || (n_lights && !lights)) return fail(CNQ_INVALID_ARG, "null argument");
+  if (n_lights != b->n_lights) return fail(CNQ_INVALID_ARG, "a batch keeps its light count");
+  HIP_TRY(hipDeviceSynchronize(), "sync");
+  HIP_TRY(hipMemcpy(b->lights.p, lights, 64 * (size_t)n_lights, hipMemcpyHostToDevice), "lights");
   return CNQ_OK;
 }
 

@@ -487,16 +487,22 @@ __device__ __forceinline__ uint32_t node_cone_test(const Cone& c, Lane& L, float
     if (c.w2 * p.z - p.x >= 0.0f) out_u1 = false;
     if (p.y - c.w1 * p.z >= 0.0f) out_v0 = false;
     if (c.w3 * p.z - p.y >= 0.0f) out_v1 = false;
-    if (inside) {
-      f2 q{(p.x / p.z - c.w0) / (c.w2 - c.w0) * g, (p.y / p.z - c.w1) / (c.w3 - c.w1) * g};
-      L.put2(P3A, k, q);
-      mnv = f2{mn(mnv.x, q.x), mn(mnv.y, q.y)};
-      mxv = f2{mx(mxv.x, q.x), mx(mxv.y, q.y)};
-    }
   }
   if (far_out || near_out || out_u0 || out_u1 || out_v0 || out_v1) return 0;
   if (!in_bound(c, (lo + hi) * 0.5f, 0.5f * len3(hi - lo))) return 0;
-  if (!inside || mx(mxv.x - mnv.x, mxv.y - mnv.y) > prefilter_bins) return 3;
+  if (!inside) return 3;
+  // Every corner between the near and far planes: their projections (the
+  // same arithmetic the WGSL does in its corner loop, done only for nodes
+  // that survive the culls).
+#pragma unroll
+  for (uint32_t k = 0; k < 8; k++) {
+    const f3 p = to_local(c, box_corner(lo, hi, k));
+    const f2 q{(p.x / p.z - c.w0) / (c.w2 - c.w0) * g, (p.y / p.z - c.w1) / (c.w3 - c.w1) * g};
+    L.put2(P3A, k, q);
+    mnv = f2{mn(mnv.x, q.x), mn(mnv.y, q.y)};
+    mxv = f2{mx(mxv.x, q.x), mx(mxv.y, q.y)};
+  }
+  if (mx(mxv.x - mnv.x, mxv.y - mnv.y) > prefilter_bins) return 3;
   if (mxv.x <= 0.0f || mxv.y <= 0.0f || mnv.x >= g || mnv.y >= g) return 1;
   f3 e = hi - lo;
   f3 v = norm3(c.apex - (lo + hi) * 0.5f);
@@ -718,30 +724,21 @@ __device__ __forceinline__ void scene_coverage(const Ctx& x, const Cone& c, bool
   }
 }
 
+// One query, `valid` or not (every lane of the wave takes part in the walk):
+// WGSL :2125-2128 `hit_irradiance_rect` around `cone_irradiance_rect` :1286.
+struct QueryOut {
+  float visible;
+  uint64_t full, touched;
+  bool capped, overflow, truncated;
+};
+
 template <bool COUNT>
-__global__ __launch_bounds__(64) void cone_query_kernel(
-    const NodeC* __restrict__ blas, const NodeC* __restrict__ nodes, const float* __restrict__ tri_leaf,
-    const float* __restrict__ tris, const uint32_t* __restrict__ ranges, const uint32_t* __restrict__ items,
-    const float* __restrict__ blas_cov, const float* __restrict__ xforms, const float* __restrict__ receivers,
-    const float* __restrict__ lights, float* __restrict__ out_visible, uint64_t* __restrict__ out_full,
-    uint64_t* __restrict__ out_touched, float* __restrict__ out_cov, uint32_t* __restrict__ counters,
-    uint64_t* __restrict__ work_out, QueryParams p) {
-  __shared__ float s_acc[ACC_CELLS * 64];
-  __shared__ float s_poly[POLY_SLOTS * 64];
-  const uint32_t lane = lane_id();
-  Lane L{s_acc + lane, s_poly + lane, ACC_CELLS, 0u};
-  const Ctx x{blas, nodes, tri_leaf, tris, ranges, items, blas_cov, xforms,
-              p.n_objects, p.hierarchy, p.profile_stage, p.object_cov};
-  const uint64_t q = (p.q_begin + (uint64_t)blockIdx.x * 64u) + lane;
-  const bool valid = q < p.q_end;
-  const uint64_t i = valid ? q % p.n : 0;
-  const uint32_t l = valid ? (uint32_t)(q / p.n) : 0;
-  const float* rc = receivers + 8 * i;
-  const LightRect r = reinterpret_cast<const LightRect*>(lights)[l];
+__device__ __forceinline__ QueryOut evaluate_query(const Ctx& x, const QueryParams& p, Lane& L, bool valid,
+                                                   const float* __restrict__ rc, const LightRect& r,
+                                                   float* __restrict__ cov_out, uint64_t* __restrict__ work_out) {
   const f3 point = ld3(rc);
   const f3 normal = ld3(rc + 4);
   const bool active = valid && rc[7] >= 0.0f;
-
   Walk w;
 #pragma unroll
   for (uint32_t b = 0; b < BINS; b++) w.cov[b] = 0.0f;
@@ -777,18 +774,20 @@ __global__ __launch_bounds__(64) void cone_query_kernel(
       for (uint32_t b = 0; b < BINS; b++) w.cov[b] = 0.0f;
     }
   }
-  uint64_t full_mask = 0, touched_mask = 0;
+  QueryOut o;
+  o.full = 0;
+  o.touched = 0;
 #pragma unroll
   for (uint32_t b = 0; b < BINS; b++) {
-    full_mask |= (uint64_t)(w.cov[b] >= 1.0f) << b;
-    touched_mask |= (uint64_t)(w.cov[b] > 0.0f) << b;
+    o.full |= (uint64_t)(w.cov[b] >= 1.0f) << b;
+    o.touched |= (uint64_t)(w.cov[b] > 0.0f) << b;
     L.set(b, w.cov[b]);  // the coverage, for the weights' loop (the accumulator is free now)
   }
   if (p.profile_stage != 1 && on && lc.ok) {
     if (is_sphere(r)) {
       value = sphere_unshadowed(point, normal, r) * sphere_open_share(L, lc.c, normal, lc.wide_clipped);
     } else {
-      value = cone_visible(L, lc.c, normal, touched_mask);
+      value = cone_visible(L, lc.c, normal, o.touched);
       if (lc.wide_clipped) {
         bool all_above;
         const float kept = window_weight4(L, lc.c, normal, all_above);
@@ -796,70 +795,124 @@ __global__ __launch_bounds__(64) void cone_query_kernel(
       }
     }
   }
-  if (valid) {
-    out_visible[q] = spot * value;
-    out_full[q] = full_mask;
-    out_touched[q] = touched_mask;
-    if (out_cov) {
-      float* oc = out_cov + 64 * q;
+  o.visible = spot * value;
+  o.capped = capped;
+  o.overflow = w.overflow;
+  o.truncated = w.truncated;
+  if (valid && cov_out) {
 #pragma unroll
-      for (uint32_t b = 0; b < BINS; b++) oc[b] = w.cov[b];
-    }
-  }
-  // Wave-aggregated diagnostics.
-  const uint64_t cap_m = ballot(valid && capped), of_m = ballot(valid && w.overflow),
-                 tr_m = ballot(valid && w.truncated);
-  if (lane == 0 && counters) {
-    if (cap_m) atomicAdd(&counters[0], (uint32_t)__builtin_popcountll(cap_m));
-    if (of_m) atomicAdd(&counters[1], (uint32_t)__builtin_popcountll(of_m));
-    if (tr_m) atomicAdd(&counters[2], (uint32_t)__builtin_popcountll(tr_m));
+    for (uint32_t b = 0; b < BINS; b++) cov_out[b] = w.cov[b];
   }
   if (COUNT && work_out) {
+    const uint32_t lane = lane_id();
 #pragma unroll
     for (int k = 0; k < 6; k++) {
-      uint32_t v = w.work[k];
+      uint32_t v = valid ? w.work[k] : 0u;
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) v += (uint32_t)__shfl_xor((int)v, off, 64);
       if (lane == 0) atomicAdd(reinterpret_cast<unsigned long long*>(&work_out[k]), (unsigned long long)v);
     }
   }
+  return o;
 }
 
-template __global__ void cone_query_kernel<false>(const NodeC*, const NodeC*, const float*, const float*,
-                                                  const uint32_t*, const uint32_t*, const float*, const float*,
-                                                  const float*, const float*, float*, uint64_t*, uint64_t*, float*,
-                                                  uint32_t*, uint64_t*, QueryParams);
-template __global__ void cone_query_kernel<true>(const NodeC*, const NodeC*, const float*, const float*,
-                                                 const uint32_t*, const uint32_t*, const float*, const float*,
-                                                 const float*, const float*, float*, uint64_t*, uint64_t*, float*,
-                                                 uint32_t*, uint64_t*, QueryParams);
+// The view holding work item `item` (uniform): binary search over the
+// views' first items.
+__device__ __forceinline__ uint32_t find_view(const ViewDesc* __restrict__ views, uint32_t n_views, uint32_t item) {
+  uint32_t lo = 0, hi = n_views;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (views[mid].item_off <= item)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
 
-hipError_t launch_cone_queries(const QueryArgs& a, bool count_work, hipStream_t stream) {
-  if (a.q_end <= a.q_begin) return hipSuccess;
-  const uint64_t waves = (a.q_end - a.q_begin + 63) / 64;
-  if (waves > 0x7fffffffull) return hipErrorInvalidValue;
-  QueryParams p;
-  p.n = a.n;
-  p.q_begin = a.q_begin;
-  p.q_end = a.q_end;
-  p.n_objects = a.s.n_objects;
-  p.hierarchy = a.hierarchy;
-  p.light_cone_cap = a.light_cone_cap;
-  p.wide_cone_cap = a.wide_cone_cap;
-  p.profile_stage = a.profile_stage;
-  p.object_cov = a.object_cov;
-  p.prefilter = a.prefilter;
-  p.wide_extent = a.wide_extent;
-  dim3 grid((uint32_t)waves), block(64);
+// Work items: (view, light, tile of 64 receivers). A view is a w x h grid
+// of receivers (a camera view, or a list as n x 1); a tile is tile_w x
+// tile_h of them, so a wave's packet covers a compact patch of pixels. The
+// query of (view, light l, pixel y * w + x) is stored at
+// view.out_off + l * w * h + y * w + x. Direct mode: one item per block;
+// persistent mode: a fixed grid takes items from `work_counter` in order.
+template <bool COUNT, bool PERSISTENT>
+__global__ __launch_bounds__(64) void cone_items_kernel(
+    const NodeC* __restrict__ blas, const NodeC* __restrict__ nodes, const float* __restrict__ tri_leaf,
+    const float* __restrict__ tris, const uint32_t* __restrict__ ranges, const uint32_t* __restrict__ items,
+    const float* __restrict__ blas_cov, const float* __restrict__ xforms, const float* __restrict__ receivers,
+    const float* __restrict__ lights, const ViewDesc* __restrict__ views, float* __restrict__ out_visible,
+    uint64_t* __restrict__ out_full, uint64_t* __restrict__ out_touched, float* __restrict__ out_cov,
+    uint32_t* __restrict__ counters, uint64_t* __restrict__ work_out, uint32_t* __restrict__ work_counter,
+    QueryParams p, ViewDesc v0) {
+  __shared__ float s_acc[ACC_CELLS * 64];
+  __shared__ float s_poly[POLY_SLOTS * 64];
+  const uint32_t lane = lane_id();
+  Lane L{s_acc + lane, s_poly + lane, ACC_CELLS, 0u};
+  const Ctx x{blas, nodes, tri_leaf, tris, ranges, items, blas_cov, xforms,
+              p.n_objects, p.hierarchy, p.profile_stage, p.object_cov};
+  const uint32_t tw_log2 = p.tile_w_log2;
+  for (uint32_t round = 0;; round++) {
+    uint32_t item;
+    if (PERSISTENT) {
+      uint32_t got = 0;
+      if (lane == 0) got = atomicAdd(work_counter, 1u);
+      item = uni(got);
+    } else {
+      item = p.item_begin + blockIdx.x;
+    }
+    if (item >= p.n_items) break;
+    const ViewDesc v = p.n_views == 1 ? v0 : views[find_view(views, p.n_views, item)];
+    const uint32_t local = item - v.item_off;
+    const uint32_t light = local / v.tiles;
+    const uint32_t tile = local - light * v.tiles;
+    const uint32_t tx = tile % v.tiles_x, ty = tile / v.tiles_x;
+    const uint32_t xpix = (tx << tw_log2) + (lane & ((1u << tw_log2) - 1u));
+    const uint32_t ypix = (ty << (6u - tw_log2)) + (lane >> tw_log2);
+    const bool valid = xpix < v.w && ypix < v.h;
+    const uint64_t pixel = valid ? (uint64_t)ypix * v.w + xpix : 0;
+    const float* rc = receivers + 8 * (v.recv_off + pixel);
+    const LightRect r = reinterpret_cast<const LightRect*>(lights)[v.light_off + light];
+    const uint64_t q = v.out_off + (uint64_t)light * ((uint64_t)v.w * v.h) + pixel;
+    const QueryOut o = evaluate_query<COUNT>(x, p, L, valid, rc, r, out_cov ? out_cov + 64 * q : nullptr, work_out);
+    if (valid) {
+      out_visible[q] = o.visible;
+      out_full[q] = o.full;
+      out_touched[q] = o.touched;
+    }
+    const uint64_t cap_m = ballot(valid && o.capped), of_m = ballot(valid && o.overflow),
+                   tr_m = ballot(valid && o.truncated);
+    if (lane == 0 && counters) {
+      if (cap_m) atomicAdd(&counters[0], (uint32_t)__builtin_popcountll(cap_m));
+      if (of_m) atomicAdd(&counters[1], (uint32_t)__builtin_popcountll(of_m));
+      if (tr_m) atomicAdd(&counters[2], (uint32_t)__builtin_popcountll(tr_m));
+    }
+    if (!PERSISTENT) break;
+  }
+}
+
+#define CNQ_ITEMS_ARGS                                                                                         \
+  s.blas, s.nodes, s.tri_leaf, s.tris, s.ranges, s.items, s.blas_cov, s.xforms, a.receivers, a.lights, a.views, \
+      a.out_visible, a.out_full, a.out_touched, a.out_cov, a.counters, a.work, a.work_counter, a.p, a.v0
+
+hipError_t launch_cone_items(const ItemArgs& a, bool count_work, uint32_t persistent_blocks, hipStream_t stream) {
+  if (a.p.n_items <= a.p.item_begin) return hipSuccess;
   const SceneArgs& s = a.s;
-  if (count_work)
-    hipLaunchKernelGGL(cone_query_kernel<true>, grid, block, 0, stream, s.blas, s.nodes, s.tri_leaf, s.tris, s.ranges,
-                       s.items, s.blas_cov, s.xforms, a.receivers, a.lights, a.out_visible, a.out_full, a.out_touched,
-                       a.out_cov, a.counters, a.work, p);
-  else
-    hipLaunchKernelGGL(cone_query_kernel<false>, grid, block, 0, stream, s.blas, s.nodes, s.tri_leaf, s.tris, s.ranges,
-                       s.items, s.blas_cov, s.xforms, a.receivers, a.lights, a.out_visible, a.out_full, a.out_touched,
-                       a.out_cov, a.counters, a.work, p);
+  dim3 block(64);
+  if (persistent_blocks) {
+    const uint32_t n = a.p.n_items - a.p.item_begin;
+    dim3 grid(persistent_blocks < n ? persistent_blocks : n);
+    if (count_work)
+      hipLaunchKernelGGL((cone_items_kernel<true, true>), grid, block, 0, stream, CNQ_ITEMS_ARGS);
+    else
+      hipLaunchKernelGGL((cone_items_kernel<false, true>), grid, block, 0, stream, CNQ_ITEMS_ARGS);
+  } else {
+    dim3 grid(a.p.n_items - a.p.item_begin);
+    if (count_work)
+      hipLaunchKernelGGL((cone_items_kernel<true, false>), grid, block, 0, stream, CNQ_ITEMS_ARGS);
+    else
+      hipLaunchKernelGGL((cone_items_kernel<false, false>), grid, block, 0, stream, CNQ_ITEMS_ARGS);
+  }
   return hipGetLastError();
 }
 

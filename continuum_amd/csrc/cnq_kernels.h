@@ -51,37 +51,55 @@ struct SceneArgs {
   uint32_t n_objects;
 };
 
-struct QueryArgs {
-  SceneArgs s;
-  const float* __restrict__ receivers;  // 8 floats each (point, normal); normal.w < 0: no cone
-  const float* __restrict__ lights;     // 16 floats each (LightGpu)
-  uint64_t n;                           // receivers; query q = l * n + i
-  uint64_t q_begin, q_end;              // the queries this launch evaluates
-  uint32_t hierarchy, light_cone_cap, wide_cone_cap, profile_stage, object_cov;
-  float prefilter, wide_extent;
-  float* __restrict__ out_visible;
-  uint64_t* __restrict__ out_full;
-  uint64_t* __restrict__ out_touched;
-  float* __restrict__ out_cov;        // nullable: 64 per query
-  uint32_t* __restrict__ counters;    // nullable: [capped, deeper than 48, truncated]
-  uint64_t* __restrict__ work;        // nullable: [object nodes, blas nodes, triangles tested, drawn,
-                                      //  meshes entered, blas splats]
-};
-
 // The cone kernel's scalar parameters (its arrays are separate, noalias
 // kernel arguments).
 struct QueryParams {
-  uint64_t n, q_begin, q_end;
   uint32_t n_objects, hierarchy, light_cone_cap, wide_cone_cap, profile_stage, object_cov;
   float prefilter, wide_extent;
+  uint32_t n_views, n_items, item_begin, tile_w_log2;
 };
 
+// One view of a batch: a w x h grid of receivers (a camera view, or a list
+// as n x 1) lit by lights [light_off, light_off + n_lights). Its work items
+// are (light, tile) pairs from item_off, tiles_x tiles per row and `tiles`
+// per light; its query (light l, pixel p) is stored at out_off + l * w * h + p.
+// Its camera pass's 8 x 8 tiles are numbered from ptile_off.
+struct ViewDesc {
+  uint64_t recv_off;
+  uint64_t out_off;
+  uint32_t w, h;
+  uint32_t light_off, n_lights;
+  uint32_t tiles_x, tiles;
+  uint32_t item_off, ptile_off;
+};
+
+struct ItemArgs {
+  SceneArgs s;
+  const float* __restrict__ receivers;
+  const float* __restrict__ lights;
+  const ViewDesc* __restrict__ views;  // device copy (used when p.n_views > 1)
+  ViewDesc v0;                         // the view when p.n_views == 1
+  float* __restrict__ out_visible;
+  uint64_t* __restrict__ out_full;
+  uint64_t* __restrict__ out_touched;
+  float* __restrict__ out_cov;
+  uint32_t* __restrict__ counters;
+  uint64_t* __restrict__ work;
+  uint32_t* __restrict__ work_counter;  // persistent mode: the next item (zeroed before launch)
+  QueryParams p;
+};
+
+// The camera pass over one or many views: view v's pixels are the receivers
+// (and hits) from views[v].recv_off, its camera is 20 floats at
+// cameras + 20 v, and the lights it may see are its own range. One block per
+// 8 x 8 tile; views[v].item_off numbers the tiles (tiles_x per row).
 struct PrimaryArgs {
   SceneArgs s;
-  const float* __restrict__ camera;  // eye, forward, right, up, params (20 floats)
+  const float* __restrict__ cameras;  // eye, forward, right, up, params (20 floats per view)
   const float* __restrict__ lights;
-  uint32_t n_lights;
-  uint32_t width, height;
+  const ViewDesc* __restrict__ views;  // device copy (used when n_views > 1)
+  ViewDesc v0;
+  uint32_t n_views, n_tiles;
   uint32_t hierarchy, nearest_first;
   uint32_t* __restrict__ hits;   // 4 per pixel
   float* __restrict__ receivers; // 8 per pixel
@@ -100,7 +118,7 @@ struct RefitArgs {
   uint32_t n_leaves;
 };
 
-hipError_t launch_cone_queries(const QueryArgs& a, bool count_work, hipStream_t stream);
+hipError_t launch_cone_items(const ItemArgs& a, bool count_work, uint32_t persistent_blocks, hipStream_t stream);
 hipError_t launch_primary(const PrimaryArgs& a, hipStream_t stream);
 hipError_t launch_refit(const RefitArgs& a, hipStream_t stream);
 

@@ -105,7 +105,7 @@ __device__ __forceinline__ void push_children(RayStack& st, f3 eye, f3 inv, uint
 
 __device__ __forceinline__ f3 recip3(f3 d) { return f3{1.0f / d.x, 1.0f / d.y, 1.0f / d.z}; }
 
-__device__ Hit nearest_in_mesh(const PrimaryArgs& a, RayStack& st, f3 world_eye, f3 world_dir, uint32_t m, Hit best,
+__device__ __forceinline__ Hit nearest_in_mesh(const PrimaryArgs& a, RayStack& st, f3 world_eye, f3 world_dir, uint32_t m, Hit best,
                                uint32_t base) {
   const SceneArgs& s = a.s;
   Hit hit = best;
@@ -183,12 +183,27 @@ __global__ __launch_bounds__(64) void primary_kernel(PrimaryArgs a) {
   __shared__ uint32_t s_stack[RSTACK * 64];
   const uint32_t lane = threadIdx.x & 63u;
   // 8x8 pixel tiles: one wave per tile, so a wave's rays are neighbours.
-  const uint32_t tiles_x = (a.width + 7) / 8;
-  const uint32_t tx = blockIdx.x % tiles_x, ty = blockIdx.x / tiles_x;
+  const uint32_t tile = blockIdx.x;
+  uint32_t vi = 0;
+  if (a.n_views > 1) {
+    uint32_t lo = 0, hi = a.n_views;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (a.views[mid].ptile_off <= tile)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    vi = lo;
+  }
+  const ViewDesc v = a.n_views > 1 ? a.views[vi] : a.v0;
+  const uint32_t local = tile - v.ptile_off;
+  const uint32_t tiles_x = (v.w + 7) / 8;
+  const uint32_t tx = local % tiles_x, ty = local / tiles_x;
   const uint32_t x = tx * 8 + (lane & 7u), y = ty * 8 + (lane >> 3);
-  if (x >= a.width || y >= a.height) return;
-  const uint64_t px = (uint64_t)y * a.width + x;
-  const float* cam = a.camera;
+  if (x >= v.w || y >= v.h) return;
+  const uint64_t px = v.recv_off + (uint64_t)y * v.w + x;
+  const float* cam = a.cameras + 20 * (size_t)vi;
   const float* p = cam + 16;
   float aspect = p[0] / p[1];
   float sx = (((float)x + 0.5f + cam[11]) / p[0] * 2.0f - 1.0f) * p[2] * aspect;
@@ -219,8 +234,8 @@ __global__ __launch_bounds__(64) void primary_kernel(PrimaryArgs a) {
   if (dot3(hit.normal, dir) > 0.0f) hit.normal = -hit.normal;
   uint32_t seen = 0;
   float nearest = hit.distance;
-  const LightRect* ls = reinterpret_cast<const LightRect*>(a.lights);
-  for (uint32_t l = 0; l < a.n_lights; l++) {
+  const LightRect* ls = reinterpret_cast<const LightRect*>(a.lights) + v.light_off;
+  for (uint32_t l = 0; l < v.n_lights; l++) {
     float t = rect_distance(ls[l], eye, dir);
     if (t < nearest) {
       nearest = t;
@@ -246,10 +261,8 @@ __global__ __launch_bounds__(64) void primary_kernel(PrimaryArgs a) {
 }
 
 hipError_t launch_primary(const PrimaryArgs& a, hipStream_t stream) {
-  const uint64_t tiles = (uint64_t)((a.width + 7) / 8) * ((a.height + 7) / 8);
-  if (tiles == 0) return hipSuccess;
-  if (tiles > 0x7fffffffull) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(primary_kernel, dim3((uint32_t)tiles), dim3(64), 0, stream, a);
+  if (a.n_tiles == 0) return hipSuccess;
+  hipLaunchKernelGGL(primary_kernel, dim3(a.n_tiles), dim3(64), 0, stream, a);
   return hipGetLastError();
 }
 

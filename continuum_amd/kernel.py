@@ -160,6 +160,21 @@ class Scene:
                                           C.byref(opts), vis.data_ptr(), full.data_ptr(), touched.data_ptr(),
                                           cov.data_ptr() if cov is not None else 0, stream.cuda_stream))
 
+    def view_query_device(self, receivers, width, height, lights, opts, vis, full, touched, tile_w=8,
+                          stream=None) -> None:
+        """Cone queries over a view's receivers (device tensors), each wave
+        taking a tile_w x 64/tile_w patch of pixels."""
+        import torch
+
+        if stream is None:
+            stream = torch.cuda.current_stream()
+        nl = lights.numel() // 16
+        if receivers.numel() < 8 * width * height or vis.numel() < width * height * nl:
+            raise ValueError("tensor too small")
+        check(lib().cnq_view_irradiance_device(self.handle, receivers.data_ptr(), width, height, lights.data_ptr(),
+                                               nl, C.byref(opts), tile_w, vis.data_ptr(), full.data_ptr(),
+                                               touched.data_ptr(), stream.cuda_stream))
+
     def primary_device(self, camera, width, height, lights, opts, hits, receivers, stream=None) -> None:
         import torch
 
@@ -202,6 +217,76 @@ class Scene:
 
     def set_count_work(self, on: bool) -> None:
         check(lib().cnq_scene_set_count_work(self.handle, int(on)))
+
+
+class Batch:
+    """Many views of one scene, each with its own camera, size and lights, in
+    one camera pass and one cone launch (optionally a replayed hipGraph)."""
+
+    def __init__(self, scene: Scene, views: Sequence[tuple], lights: np.ndarray):
+        """`views`: (camera20, width, height, light_first, light_count) each;
+        `lights`: the batch's LightGpu rows."""
+        self.scene = scene
+        arr = (_lib.View * len(views))()
+        self.shapes = []
+        for k, (cam, w, h, lf, lc) in enumerate(views):
+            arr[k].camera[:] = [float(x) for x in np.asarray(cam, np.float32).reshape(20)]
+            arr[k].width, arr[k].height, arr[k].light_first, arr[k].light_count = int(w), int(h), int(lf), int(lc)
+            self.shapes.append((int(w), int(h), int(lc)))
+        self._lights = np.ascontiguousarray(lights, dtype=np.float32).reshape(-1, 16)
+        h_ = C.c_void_p()
+        check(lib().cnq_batch_create(scene.handle, C.cast(arr, C.c_void_p), len(views), _ptr(self._lights),
+                                     len(self._lights), C.byref(h_)))
+        self.handle = h_
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            lib().cnq_batch_destroy(self.handle)
+            self.handle = None
+
+    @property
+    def queries(self) -> int:
+        return int(lib().cnq_batch_queries(self.handle))
+
+    @staticmethod
+    def tuning(tile_w: int = 8, persistent_blocks: int = 0) -> _lib.Tuning:
+        return _lib.Tuning(tile_w, persistent_blocks)
+
+    def run(self, opts: _lib.Options, tuning: _lib.Tuning, stream=None) -> None:
+        check(lib().cnq_batch_run(self.handle, C.byref(opts), C.byref(tuning), _stream_ptr(stream)))
+
+    def capture(self, opts: _lib.Options, tuning: _lib.Tuning) -> None:
+        check(lib().cnq_batch_capture(self.handle, C.byref(opts), C.byref(tuning)))
+
+    def replay(self, stream=None) -> None:
+        check(lib().cnq_batch_replay(self.handle, _stream_ptr(stream)))
+
+    def autotune(self, opts: _lib.Options, stream=None):
+        best = _lib.Tuning()
+        ms = C.c_float()
+        check(lib().cnq_batch_autotune(self.handle, C.byref(opts), _stream_ptr(stream), C.byref(best), C.byref(ms)))
+        return best, float(ms.value)
+
+    def read(self, view: int):
+        """(hits (px, 4), visible, full, touched) of one view, queries light-major."""
+        w, h, nl = self.shapes[view]
+        px, q = w * h, w * h * nl
+        hits = np.zeros((px, 4), np.uint32)
+        vis = np.zeros(q, np.float32)
+        full = np.zeros(q, np.uint64)
+        touched = np.zeros(q, np.uint64)
+        check(lib().cnq_batch_read(self.handle, view, _ptr(hits), _ptr(vis), _ptr(full), _ptr(touched)))
+        return hits, vis, full, touched
+
+    def set_lights(self, lights: np.ndarray) -> None:
+        self._lights = np.ascontiguousarray(lights, dtype=np.float32).reshape(-1, 16)
+        check(lib().cnq_batch_set_lights(self.handle, _ptr(self._lights), len(self._lights)))
+
+
+def _stream_ptr(stream) -> int:
+    if stream is None:
+        return 0
+    return int(getattr(stream, "cuda_stream", stream))
 
 
 def receivers_array(receivers: Sequence) -> np.ndarray:

@@ -238,6 +238,139 @@ def rock_field(seed: int = 0xC0_0002, width: int = 1280, height: int = 720, rock
                      notes={"eye": (0.0, -5.0, 19.0), "target": (0.0, 1.0, 0.0), "fov": 60.0})
 
 
+def _cylinder(center, radius: float, z0: float, z1: float, seg: int, rings: int) -> np.ndarray:
+    """An outward cylinder side (no caps), seg x rings quads."""
+    th = np.linspace(0.0, 2.0 * np.pi, seg + 1)
+    zs = np.linspace(z0, z1, rings + 1)
+    T, Zz = np.meshgrid(th, zs, indexing="ij")
+    P = np.stack([center[0] + radius * np.cos(T), center[1] + radius * np.sin(T), Zz], -1)
+    a, b, c, d = P[:-1, :-1], P[1:, :-1], P[1:, 1:], P[:-1, 1:]
+    return np.concatenate([np.stack([a, b, c], -2).reshape(-1, 3, 3),
+                           np.stack([a, c, d], -2).reshape(-1, 3, 3)]).astype(F32)
+
+
+def _vault(x0: float, x1: float, y0: float, y1: float, z_spring: float, seg: int, rows: int, inward: bool) -> np.ndarray:
+    """A half-cylinder arch over [x0, x1] (axis along y from y0 to y1),
+    springing at z_spring; its inside faces down (toward the passage)."""
+    r = 0.5 * (x1 - x0)
+    cx = 0.5 * (x0 + x1)
+    th = np.linspace(np.pi, 0.0, seg + 1)
+    ys = np.linspace(y0, y1, rows + 1)
+    T, Y = np.meshgrid(th, ys, indexing="ij")
+    P = np.stack([cx + r * np.cos(T), Y, z_spring + r * np.sin(T)], -1)
+    a, b, c, d = P[:-1, :-1], P[1:, :-1], P[1:, 1:], P[:-1, 1:]
+    t1, t2 = np.stack([a, b, c], -2), np.stack([a, c, d], -2)
+    if not inward:
+        t1, t2 = t1[..., ::-1, :], t2[..., ::-1, :]
+    return np.concatenate([t1.reshape(-1, 3, 3), t2.reshape(-1, 3, 3)]).astype(F32)
+
+
+def _curtain(x0: float, x1: float, y: float, z_top: float, z_bot: float, nu: int, nv: int, sag: float,
+             phase: float) -> np.ndarray:
+    """A hanging curtain grid in the plane y (both faces: two sheets a
+    hair apart), sagging toward +y by up to `sag`."""
+    u = np.linspace(0.0, 1.0, nu + 1)
+    v = np.linspace(0.0, 1.0, nv + 1)
+    U, V = np.meshgrid(u, v, indexing="ij")
+    X = x0 + (x1 - x0) * U
+    Z = z_top + (z_bot - z_top) * V
+    S = sag * np.sin(np.pi * U) * V * (0.8 + 0.2 * np.sin(6.0 * U + phase))
+    out = []
+    for side in (1.0, -1.0):
+        P = np.stack([X, y + S + side * 0.002, Z], -1)
+        a, b, c, d = P[:-1, :-1], P[1:, :-1], P[1:, 1:], P[:-1, 1:]
+        t1, t2 = np.stack([a, b, c], -2), np.stack([a, c, d], -2)
+        if side < 0:
+            t1, t2 = t1[..., ::-1, :], t2[..., ::-1, :]
+        out += [t1.reshape(-1, 3, 3), t2.reshape(-1, 3, 3)]
+    return np.concatenate(out).astype(F32)
+
+
+def courtyard(seed: int = 0xC0_0003, width: int = 1280, height: int = 720, detail: float = 1.0) -> SceneSpec:
+    """The 3M-triangle courtyard: an open atrium 30 x 15 x 20 m; on each long
+    side two floors of 12 arches (48 in all) on columns, a gallery slab
+    between the floors; hanging curtains (about 40% of the triangles) under
+    the upper arches. `detail` scales every tessellation (1.0: about 3M
+    triangles). ``notes['curtains']`` lists the curtain meshes, which
+    `courtyard_frame` displaces each frame."""
+    rng = Pcg32(seed, 13)
+    L, W, H = 30.0, 15.0, 20.0
+    k = math.sqrt(detail)
+
+    def n(x):
+        return max(2, int(round(x * k)))
+
+    meshes = []
+    # ground and outer walls (inward)
+    meshes.append(grid_quad([0, 0, 0], [L, 0, 0], [0, W, 0], n(450), n(225)))
+    meshes.append(grid_quad([0, 0, 0], [0, W, 0], [0, 0, H], n(225), n(300)))
+    meshes.append(grid_quad([L, 0, 0], [0, 0, H], [0, W, 0], n(300), n(225)))
+    meshes.append(grid_quad([0, 0, 0], [0, 0, H], [L, 0, 0], n(300), n(450)))
+    meshes.append(grid_quad([0, W, 0], [L, 0, 0], [0, 0, H], n(450), n(300)))
+    bay = L / 12.0
+    depth = 3.0
+    curtains = []
+    for floor, (z0, z1) in enumerate(((0.0, 10.0), (10.0, 20.0))):
+        spring = z0 + 6.0
+        for side, (ya, yb) in enumerate(((0.0, depth), (W - depth, W))):
+            for i in range(12):
+                xa, xb = i * bay + 0.35, (i + 1) * bay - 0.35
+                meshes.append(_vault(xa, xb, ya, yb, spring, n(72), n(36), inward=True))
+                meshes.append(_cylinder([i * bay + 0.175 + 0.0, ya + (depth if side == 0 else 0.0), 0], 0.3, z0,
+                                        spring, n(60), n(45)))
+            # the gallery slab over this floor's arcade (downward and upward faces)
+            if floor == 0:
+                y_in = depth if side == 0 else W - depth
+                slab_lo = grid_quad([0, ya, 9.6], [0, yb - ya, 0], [L, 0, 0], n(90), n(450))
+                slab_hi = grid_quad([0, ya, 10.0], [L, 0, 0], [0, yb - ya, 0], n(450), n(90))
+                meshes += [slab_lo, slab_hi]
+                del y_in
+    # curtains under the upper arches, toward the courtyard
+    for side, y in enumerate((depth + 0.4, W - depth - 0.4)):
+        for i in range(12):
+            xa, xb = i * bay + 0.45, (i + 1) * bay - 0.45
+            sag = 0.2 + 0.6 * rng.next_f32()
+            phase = 2.0 * math.pi * rng.next_f32()
+            c = _curtain(xa, xb, y, 15.8, 10.6, n(112), n(112), sag if side == 0 else -sag, phase)
+            curtains.append(len(meshes))
+            meshes.append(c)
+    # lights: a sun disk and three area lights of 0.5-8 m^2 with random colours
+    sun_dir = np.array([0.35, 0.25, 0.9])
+    sun_dir /= np.linalg.norm(sun_dir)
+    lights = [sphere_light(sun_dir * 1000.0 + np.array([L / 2, W / 2, 0]), 1000.0 * math.tan(math.radians(0.265)),
+                           radiance=(3.0, 2.9, 2.7))]
+    for _ in range(3):
+        area = 0.5 + 7.5 * rng.next_f32()
+        s = math.sqrt(area)
+        cx = 4.0 + 22.0 * rng.next_f32()
+        cy = 4.5 + 6.0 * rng.next_f32()
+        z = 8.0 + 9.0 * rng.next_f32()
+        col = (0.3 + 0.7 * rng.next_f32(), 0.3 + 0.7 * rng.next_f32(), 0.3 + 0.7 * rng.next_f32())
+        lights.append(rect_light([cx - s / 2, cy + s / 2, z], [s, 0, 0], [0, -s, 0], one_sided=True,
+                                 radiance=tuple(10.0 * c for c in col)))
+    eye, target = (2.5, 7.5, 3.0), (22.0, 6.0, 7.0)
+    cam = camera_uniform(eye, target, 60.0, width, height)
+    return SceneSpec("courtyard", meshes, np.stack(lights), cam, width, height, prefilter=1.0,
+                     notes={"curtains": curtains, "eye": eye, "target": target, "fov": 60.0, "seed": seed})
+
+
+def courtyard_frame(spec: SceneSpec, frame: int, tris12: np.ndarray, ranges: np.ndarray) -> np.ndarray:
+    """The courtyard's triangles (12 f32 each, flattened order) at `frame`:
+    curtain vertices displaced across the curtain by a seeded sine field of
+    time. Other geometry is unchanged."""
+    out = tris12.copy()
+    t = 0.08 * frame
+    seed = spec.notes.get("seed", 0)
+    for m in spec.notes["curtains"]:
+        first, count = int(ranges[m][0]), int(ranges[m][1])
+        blk = out[first:first + count].reshape(count, 3, 4)
+        x, z = blk[..., 0], blk[..., 2]
+        amp = 0.15 * np.clip((15.8 - z) / 5.2, 0.0, 1.0)
+        blk[..., 1] += (amp * np.sin(1.3 * x + 0.9 * z + t + 0.001 * seed)).astype(F32)
+        out[first:first + count] = blk.reshape(count, 12)
+    return out
+
+
 def meshes_to_arrays(meshes) -> tuple[np.ndarray, np.ndarray]:
     """Concatenated triangles (n, 9) f32 and per-mesh counts (u32)."""
     counts = np.array([len(m) for m in meshes], dtype=np.uint32)

@@ -120,6 +120,15 @@ int cnq_irradiance_device(cnq_scene* scene, const float* receivers, uint64_t n, 
                           uint32_t n_lights, const cnq_options* opts, float* out_visible, uint64_t* out_full,
                           uint64_t* out_touched, float* out_cov, void* stream);
 
+/* The same over a width x height view's receivers (pixel y * width + x),
+ * each wave taking a tile_w x (64 / tile_w) patch of pixels (tile_w a power
+ * of two up to 64): the cone part of the WGSL `render` entry
+ * (mask_cones.wgsl:2294, 8 x 8 workgroups, lib.rs:3126-3134). Query
+ * (light l, pixel p) is stored at l * width * height + p. */
+int cnq_view_irradiance_device(cnq_scene* scene, const float* receivers, uint32_t width, uint32_t height,
+                               const float* lights, uint32_t n_lights, const cnq_options* opts, uint32_t tile_w,
+                               float* out_visible, uint64_t* out_full, uint64_t* out_touched, void* stream);
+
 /* Camera rays for a width x height view (the WGSL `visibility` entry,
  * mask_cones.wgsl:2227, with `camera_hit` :2222): per pixel, hits (mesh,
  * triangle, distance bits, seen light + 1) and receivers (the shading point
@@ -155,6 +164,47 @@ int cnq_scene_read_blas(cnq_scene* scene, uint32_t* blas, float* blas_cov);
  * splatted. */
 int cnq_scene_counters(cnq_scene* scene, int reset, uint32_t counters[3], uint64_t work[6]);
 int cnq_scene_set_count_work(cnq_scene* scene, int on);
+
+/* ---- Batches: many views of uneven size and light count in one camera
+ * pass and one cone launch (no counterpart in the reference, which renders
+ * one view per `render_instanced_lights` call, lib.rs:1857). Each view has
+ * its camera, size and a range of the batch's lights; results are read per
+ * view in the layout of cnq_render_view. */
+typedef struct cnq_view {
+  float camera[20];
+  uint32_t width, height;
+  uint32_t light_first, light_count;
+} cnq_view;
+
+/* How the cone launch is shaped: the pixel tile a wave takes (tile_w x
+ * 64 / tile_w) and, if non-zero, a persistent grid of that many one-wave
+ * blocks that take work items from a counter (else one block per item). */
+typedef struct cnq_tuning {
+  uint32_t tile_w;
+  uint32_t persistent_blocks;
+} cnq_tuning;
+
+typedef struct cnq_batch cnq_batch;
+
+void cnq_tuning_default(cnq_tuning* t);
+int cnq_batch_create(cnq_scene* scene, const cnq_view* views, uint32_t n_views, const float* lights,
+                     uint32_t n_lights, cnq_batch** out);
+void cnq_batch_destroy(cnq_batch* batch);
+uint64_t cnq_batch_queries(const cnq_batch* batch);
+/* Enqueue the batch's camera and cone passes on `stream`. */
+int cnq_batch_run(cnq_batch* batch, const cnq_options* opts, const cnq_tuning* tuning, void* stream);
+/* Record both passes as a hipGraph (the counter reset is a kernel, not a
+ * memset), then replay it on any stream. */
+int cnq_batch_capture(cnq_batch* batch, const cnq_options* opts, const cnq_tuning* tuning);
+int cnq_batch_replay(cnq_batch* batch, void* stream);
+/* Time every candidate tuning on this batch (tile shapes x persistent grid
+ * sizes) and return the fastest; results do not depend on the tuning. */
+int cnq_batch_autotune(cnq_batch* batch, const cnq_options* opts, void* stream, cnq_tuning* best, float* best_ms);
+/* Synchronize and copy one view's hits and query results. */
+int cnq_batch_read(cnq_batch* batch, uint32_t view, uint32_t* hits, float* out_visible, uint64_t* out_full,
+                   uint64_t* out_touched);
+/* Replace the batch's lights (same count): a light edit. */
+int cnq_batch_set_lights(cnq_batch* batch, const float* lights, uint32_t n_lights);
 
 #ifdef __cplusplus
 }

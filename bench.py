@@ -60,6 +60,114 @@ def read_traffic(workload: str):
     return None
 
 
+def courtyard_forms(device, steps: int) -> dict:
+    """The 3M-triangle courtyard: (1) the plain per-view frame with its
+    per-frame refit (curtains displaced by a sine field of time), camera pass
+    and cones for 1280x720 x 4 lights; (2) 64 views of uneven size and 1-16
+    lights as 64 sequential plain launches; (3) the same views as one batch
+    (autotuned on a subset, captured as a hipGraph and replayed)."""
+    import numpy as np
+    import torch
+
+    from continuum_amd import _lib
+    from continuum_amd.kernel import Batch, Flat, Scene
+    from continuum_amd.scenes import courtyard, courtyard_frame, meshes_to_arrays
+    from continuum_amd.workloads import courtyard_views, view_queries
+
+    out = {}
+    sc = courtyard()
+    tris9, counts = meshes_to_arrays(sc.meshes)
+    flat = Flat.build_arrays(tris9, counts)
+    scene = Scene(device, flat)
+    opts = _lib.default_options(hierarchy=2, prefilter=sc.prefilter)
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    W, H, nl = sc.width, sc.height, len(sc.lights)
+    cam = torch.from_numpy(sc.camera).to(dev)
+    lights = torch.from_numpy(np.ascontiguousarray(sc.lights)).to(dev)
+    hits = torch.empty((W * H, 4), dtype=torch.int32, device=dev)
+    recv = torch.empty((W * H, 8), dtype=torch.float32, device=dev)
+    vis = torch.empty(W * H * nl, dtype=torch.float32, device=dev)
+    full = torch.empty(W * H * nl, dtype=torch.int64, device=dev)
+    touched = torch.empty(W * H * nl, dtype=torch.int64, device=dev)
+    poses = [torch.from_numpy(courtyard_frame(sc, f, flat.tris, flat.ranges)).to(dev) for f in range(steps + 1)]
+    torch.cuda.synchronize()
+
+    def frame(f):
+        scene.refit_device(poses[f], stream)
+        scene.primary_device(cam, W, H, lights, opts, hits, recv, stream)
+        scene.view_query_device(recv, W, H, lights, opts, vis, full, touched, tile_w=8, stream=stream)
+
+    frame(0)
+    torch.cuda.synchronize()
+    lat = []
+    t0 = time.perf_counter()
+    for f in range(1, steps + 1):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        frame(f)
+        e1.record(stream)
+        lat.append((e0, e1))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = np.array([a.elapsed_time(b) for a, b in lat])
+    q = W * H * nl
+    out["courtyard_plain"] = {
+        "workload": "courtyard 3.05M triangles, 1280x720 x 4 lights, per-frame GPU refit of the curtains",
+        "queries_per_s": steps * q / wall, "ms_per_frame": wall / steps * 1e3,
+        "p50_ms": float(np.percentile(ms, 50)), "p99_ms": float(np.percentile(ms, 99)),
+        "triangles": int(sc.n_tris)}
+
+    views, vlights = courtyard_views()
+    total_q = view_queries(views)
+    # 64 sequential plain launches (camera pass + cone pass per view).
+    maxpx = max(w * h for _, w, h, _, _ in views)
+    maxq = max(w * h * n for _, w, h, _, n in views)
+    s_hits = torch.empty((maxpx, 4), dtype=torch.int32, device=dev)
+    s_recv = torch.empty((maxpx, 8), dtype=torch.float32, device=dev)
+    s_vis = torch.empty(maxq, dtype=torch.float32, device=dev)
+    s_full = torch.empty(maxq, dtype=torch.int64, device=dev)
+    s_touched = torch.empty(maxq, dtype=torch.int64, device=dev)
+    v_cams = [torch.from_numpy(c).to(dev) for c, *_ in views]
+    v_lights = [torch.from_numpy(np.ascontiguousarray(vlights[lf:lf + n])).to(dev) for _, _, _, lf, n in views]
+
+    def sequential():
+        for k, (_, w, h, lf, n) in enumerate(views):
+            scene.primary_device(v_cams[k], w, h, v_lights[k], opts, s_hits, s_recv, stream)
+            scene.view_query_device(s_recv, w, h, v_lights[k], opts, s_vis, s_full, s_touched, tile_w=8,
+                                    stream=stream)
+
+    sequential()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sequential()
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+
+    probe = Batch(scene, views[::8], vlights)
+    best, _ = probe.autotune(opts)
+    del probe
+    batch = Batch(scene, views, vlights)
+    batch.capture(opts, best)
+    batch.replay()
+    torch.cuda.synchronize()
+    reps = 2
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        batch.replay()
+    torch.cuda.synchronize()
+    t_batch = (time.perf_counter() - t0) / reps
+    out["courtyard_64_views"] = {
+        "workload": "64 views over the courtyard, sizes from {256^2, 640x360, 1280x720, 1920x1080}, 1-16 lights each",
+        "queries": total_q,
+        "sequential_plain": {"queries_per_s": total_q / t_seq, "s": t_seq},
+        "batched_graph": {"queries_per_s": total_q / t_batch, "s": t_batch,
+                          "tuning": {"tile_w": best.tile_w, "persistent_blocks": best.persistent_blocks}},
+        "batched_over_sequential": t_seq / t_batch,
+    }
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +178,7 @@ def main() -> int:
     ap.add_argument("--tile-w", type=int, default=8, help="pixels across a wave's tile (tile_w x 64/tile_w)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-forms", action="store_true", help="skip the courtyard's plain/batched forms")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -190,7 +299,7 @@ def main() -> int:
                        "camera_pass_mean": float(prim_ms.mean()), "cone_pass_mean": float(cone_ms.mean())},
         "roofline": {
             "bound": "hbm",
-            "kernel": "cone_query_kernel",
+            "kernel": "cone_items_kernel",
             "achieved": achieved,
             "peak": PEAK_HBM_GBPS,
             "unit": "GB/s",
@@ -246,6 +355,10 @@ def main() -> int:
             "capped_skipped": int((~ok).sum()),
             "camera_hits_differ": int((hits.cpu().numpy().view(np.uint32) != rh).any(axis=1).sum()),
         }
+
+    if not args.no_forms:
+        del scene
+        result["forms"] = courtyard_forms(device, steps=5)
 
     print(json.dumps(result))
     return 0

@@ -102,10 +102,13 @@ class Device:
         check(lib().cnq_device_create(ordinal, C.byref(h)))
         self.handle = h
         self.ordinal = ordinal
+        # Bound now: module globals may already be cleared when a destructor
+        # runs at interpreter exit.
+        self._destroy = lib().cnq_device_destroy
 
     def __del__(self):
         if getattr(self, "handle", None):
-            lib().cnq_device_destroy(self.handle)
+            self._destroy(self.handle)
             self.handle = None
 
 
@@ -119,10 +122,11 @@ class Scene:
         d = flat.desc()
         check(lib().cnq_scene_create(device.handle, C.byref(d), C.byref(h)))
         self.handle = h
+        self._destroy = lib().cnq_scene_destroy
 
     def __del__(self):
         if getattr(self, "handle", None):
-            lib().cnq_scene_destroy(self.handle)
+            self._destroy(self.handle)
             self.handle = None
 
     @property
@@ -203,6 +207,13 @@ class Scene:
             raise ValueError("refit needs the same triangles in the same order")
         check(lib().cnq_scene_refit(self.handle, _ptr(tris12)))
 
+    def refit_device(self, tris12, stream=None) -> None:
+        """Refit to a new pose given as a device tensor (12 f32 per triangle,
+        the flattened order); returns once the object hierarchy is rebuilt."""
+        if tris12.numel() != self.flat.tris.size:
+            raise ValueError("refit needs the same triangles in the same order")
+        check(lib().cnq_scene_refit_device(self.handle, tris12.data_ptr(), _stream_ptr(stream)))
+
     def read_blas(self):
         blas = np.zeros_like(self.flat.blas)
         cov = np.zeros_like(self.flat.blas_cov)
@@ -238,10 +249,11 @@ class Batch:
         check(lib().cnq_batch_create(scene.handle, C.cast(arr, C.c_void_p), len(views), _ptr(self._lights),
                                      len(self._lights), C.byref(h_)))
         self.handle = h_
+        self._destroy = lib().cnq_batch_destroy
 
     def __del__(self):
         if getattr(self, "handle", None):
-            lib().cnq_batch_destroy(self.handle)
+            self._destroy(self.handle)
             self.handle = None
 
     @property

@@ -93,6 +93,11 @@ SIGNATURES = {
     "cnq_scene_read_blas": (C.c_int, [_P, _P, _P]),
     "cnq_scene_counters": (C.c_int, [_P, C.c_int, _P, _P]),
     "cnq_scene_set_count_work": (C.c_int, [_P, C.c_int]),
+    "cnq_block_planes": (C.c_int, [_P, _P, C.c_uint32, _P, _P]),
+    "cnq_cache_create": (C.c_int, [_P, _P, _P, C.c_uint32, _P, _P, _P, C.POINTER(Options), C.POINTER(_P)]),
+    "cnq_cache_destroy": (None, [_P]),
+    "cnq_cache_update": (C.c_int, [_P, _P, C.c_uint32, _P, _P]),
+    "cnq_cache_read": (C.c_int, [_P, _P, _P, _P]),
 }
 
 _lib = None

@@ -203,9 +203,11 @@ void triangle_hierarchy(const float* mesh, size_t n, uint32_t root, uint32_t fir
   for (size_t i = 0; i < n; i++) items_out[i] = first + items[i];
 }
 
-void object_hierarchy(const std::vector<Box>& bounds, std::vector<NodeGpu>& nodes, std::vector<uint32_t>& items) {
-  items.resize(bounds.size());
-  for (size_t i = 0; i < bounds.size(); i++) items[i] = (uint32_t)i;
+void object_hierarchy(const std::vector<Box>& bounds, std::vector<NodeGpu>& nodes, std::vector<uint32_t>& items,
+                      const uint8_t* keep) {
+  items.clear();
+  for (size_t i = 0; i < bounds.size(); i++)
+    if (!keep || keep[i]) items.push_back((uint32_t)i);
   nodes.assign(1, NodeGpu{});
   if (items.empty()) {
     items.push_back(0);

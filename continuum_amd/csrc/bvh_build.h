@@ -40,7 +40,9 @@ void split_tree(std::vector<NodeGpu>& nodes, size_t root_node, const std::vector
                 size_t count, size_t leaf, uint32_t node_base, uint32_t item_base);
 void triangle_hierarchy(const float* mesh9, size_t n, uint32_t root, uint32_t first, std::vector<NodeGpu>& nodes,
                         std::vector<uint32_t>& items, std::vector<float>& cov);
-void object_hierarchy(const std::vector<Box>& bounds, std::vector<NodeGpu>& nodes, std::vector<uint32_t>& items);
+// Over the objects keep[i] != 0 selects (null: every object); indices stay global.
+void object_hierarchy(const std::vector<Box>& bounds, std::vector<NodeGpu>& nodes, std::vector<uint32_t>& items,
+                      const uint8_t* keep = nullptr);
 void flatten(const float* tris9, const uint32_t* counts, uint32_t n_meshes, Flat& out);
 
 }  // namespace host

@@ -3,6 +3,7 @@
 // cnq_primary.hip and cnq_refit.hip.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <new>
 #include <string>
@@ -10,6 +11,7 @@
 
 #include "../../include/cnq.h"
 #include "bvh_build.h"
+#include "cache_host.h"
 #include "cnq_kernels.h"
 
 using cnq::NodeC;
@@ -983,6 +985,346 @@ int cnq_batch_set_lights(cnq_batch* b, const float* lights, uint32_t n_lights) {
   if (n_lights != b->n_lights) return fail(CNQ_INVALID_ARG, "a batch keeps its light count");
   HIP_TRY(hipDeviceSynchronize(), "sync");
   HIP_TRY(hipMemcpy(b->lights.p, lights, 64 * (size_t)n_lights, hipMemcpyHostToDevice), "lights");
+  return CNQ_OK;
+}
+
+}  // extern "C"
+
+// ---- The lighting cache
+
+struct cnq_cache {
+  cnq_scene* scene;
+  uint32_t n_blocks, n_slots;
+  cnq_options opts;
+  cnq::LightRect light;
+  std::vector<uint8_t> dynamic;
+  std::vector<cnq::host::Box> bounds;          // every mesh's current bounds (unpadded)
+  std::vector<std::vector<float>> mesh_tris;   // dynamic meshes' current triangles (9 floats each)
+  std::vector<uint32_t> node_span;             // per mesh: its triangle hierarchy's node count
+  DevBuf<float> texels, planes, static_cov, irradiance;
+  DevBuf<uint32_t> block_slot, slot_block, dirty, n_dirty, static_items;
+  DevBuf<NodeC> static_nodes;
+  DevBuf<uint64_t> bake_full, bake_touched;
+  uint32_t rebake_grid = 0;
+
+  void release() {
+    texels.release(), planes.release(), static_cov.release(), irradiance.release();
+    block_slot.release(), slot_block.release(), dirty.release(), n_dirty.release(), static_items.release();
+    static_nodes.release(), bake_full.release(), bake_touched.release();
+  }
+
+  cnq::CacheArgs args() const {
+    cnq::CacheArgs a;
+    a.s = scene->args();
+    a.static_nodes = static_nodes.p;
+    a.static_items = static_items.p;
+    a.texels = texels.p;
+    a.planes = reinterpret_cast<const float4*>(planes.p);
+    a.block_slot = block_slot.p;
+    a.slot_block = slot_block.p;
+    a.static_cov = static_cov.p;
+    a.dirty = dirty.p;
+    a.n_dirty = n_dirty.p;
+    a.irradiance = irradiance.p;
+    a.light = light;
+    a.p = query_params(scene, &opts, 64);
+    a.n_blocks = n_blocks;
+    a.n_slots = n_slots;
+    return a;
+  }
+};
+
+namespace {
+
+cnq::host::Box tri9_bounds(const float* t, size_t n) {
+  cnq::host::Box b;
+  for (int k = 0; k < 3; k++) {
+    b.lo[k] = cnq::host::FMAX;
+    b.hi[k] = -cnq::host::FMAX;
+  }
+  for (size_t v = 0; v < 3 * n; v++)
+    for (int k = 0; k < 3; k++) {
+      const float x = t[3 * v + k];
+      b.lo[k] = x < b.lo[k] ? x : b.lo[k];
+      b.hi[k] = x > b.hi[k] ? x : b.hi[k];
+    }
+  return b;
+}
+
+// The object hierarchy over the padded bounds of the meshes `keep` selects
+// (`mesh_hierarchy_of`, hierarchy.rs:188-196), compacted.
+void mesh_hierarchy_of(const std::vector<cnq::host::Box>& bounds, const uint8_t* keep, std::vector<NodeC>& out,
+                       std::vector<uint32_t>& items) {
+  std::vector<cnq::host::Box> padded(bounds.size());
+  for (size_t m = 0; m < bounds.size(); m++) padded[m] = cnq::host::padded(bounds[m]);
+  std::vector<cnq::host::NodeGpu> nodes;
+  cnq::host::object_hierarchy(padded, nodes, items, keep);
+  out.resize(nodes.size());
+  for (size_t n = 0; n < nodes.size(); n++) out[n] = compact(reinterpret_cast<const uint32_t*>(&nodes[n]), true);
+}
+
+}  // namespace
+
+extern "C" {
+
+int cnq_block_planes(const float* texels, const float* blocks, uint32_t n_blocks, const float* light, float* planes) {
+  if (n_blocks && (!texels || !blocks || !planes)) return fail(CNQ_INVALID_ARG, "null buffer");
+  if (!light) return fail(CNQ_INVALID_ARG, "light is null");
+  if (light[3] > 1.5f) return fail(CNQ_INVALID_ARG, "the lighting cache takes a rectangular light");
+  cnq::host::block_planes(texels, blocks, n_blocks, light, planes);
+  return CNQ_OK;
+}
+
+int cnq_cache_create(cnq_scene* s, const float* texels, const float* blocks, uint32_t n_blocks, const float* light,
+                     const uint8_t* dynamic, const float* irradiance, const cnq_options* o, cnq_cache** out) {
+  if (!s || !out || !light || !dynamic) return fail(CNQ_INVALID_ARG, "null argument");
+  if (n_blocks && (!texels || !blocks)) return fail(CNQ_INVALID_ARG, "null buffer");
+  int rc = check_options(o);
+  if (rc) return rc;
+  if (o->hierarchy != 2) return fail(CNQ_INVALID_ARG, "the lighting cache walks triangle hierarchies (hierarchy 2)");
+  if (s->instanced) return fail(CNQ_INVALID_ARG, "the lighting cache takes a scene without transforms");
+  if (light[3] > 1.5f) return fail(CNQ_INVALID_ARG, "the lighting cache takes a rectangular light");
+  if (n_blocks > 0x3ffffffu) return fail(CNQ_CAPACITY, "too many lightmap blocks");
+  HIP_TRY(hipSetDevice(s->dev->ordinal), "hipSetDevice");
+  const uint32_t n_obj = s->n_objects;
+  uint32_t n_dynamic = 0;
+  for (uint32_t m = 0; m < n_obj; m++)
+    if (dynamic[m] && s->h_ranges[4 * m + 1] > 0) n_dynamic++;
+  if (n_dynamic > cnq::MAX_MOVING) return fail(CNQ_CAPACITY, "at most 16 dynamic meshes");
+
+  // Meshes' triangles and hierarchy layout, from the device copy.
+  std::vector<float> tris(12 * s->n_tris);
+  std::vector<uint32_t> blas_items(s->n_blas_items);
+  HIP_TRY(hipDeviceSynchronize(), "sync");
+  if (s->n_tris) HIP_TRY(hipMemcpy(tris.data(), s->tris.p, 48 * s->n_tris, hipMemcpyDeviceToHost), "read tris");
+  if (s->n_blas_items)
+    HIP_TRY(hipMemcpy(blas_items.data(), s->blas_items.p, 4 * s->n_blas_items, hipMemcpyDeviceToHost), "read items");
+  cnq_cache* c = new (std::nothrow) cnq_cache;
+  if (!c) return fail(CNQ_OOM, "out of host memory");
+  c->scene = s;
+  c->n_blocks = n_blocks;
+  c->opts = *o;
+  std::memcpy(&c->light, light, 64);
+  c->light.radiance = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+  c->dynamic.assign(dynamic, dynamic + n_obj);
+  c->bounds.resize(n_obj);
+  c->mesh_tris.resize(n_obj);
+  c->node_span.assign(n_obj, 0);
+  std::vector<uint32_t> roots;
+  for (uint32_t m = 0; m < n_obj; m++)
+    if (s->h_ranges[4 * m + 2] != NONE) roots.push_back(s->h_ranges[4 * m + 2]);
+  std::sort(roots.begin(), roots.end());
+  for (uint32_t m = 0; m < n_obj; m++) {
+    const uint32_t first = s->h_ranges[4 * m], count = s->h_ranges[4 * m + 1], root = s->h_ranges[4 * m + 2];
+    std::vector<float> t9(9 * (size_t)count);
+    for (size_t v = 0; v < 3 * (size_t)count; v++)
+      for (int k = 0; k < 3; k++) t9[3 * v + k] = tris[12 * (size_t)first + 4 * v + k];
+    c->bounds[m] = tri9_bounds(t9.data(), count);
+    if (!c->dynamic[m]) continue;
+    if (root != NONE) {
+      // A dynamic mesh's hierarchy is rebuilt in place: its items must be its
+      // own triangles at its own offsets, as `flatten` lays them out.
+      for (uint32_t k = first; k < first + count; k++)
+        if (k >= blas_items.size() || blas_items[k] < first || blas_items[k] >= first + count) {
+          delete c;
+          return fail(CNQ_INVALID_ARG, "mesh " + std::to_string(m) + ": not laid out by flatten");
+        }
+      auto next = std::upper_bound(roots.begin(), roots.end(), root);
+      c->node_span[m] = (next == roots.end() ? s->n_blas : *next) - root;
+    }
+    c->mesh_tris[m] = std::move(t9);
+  }
+  tris.clear();
+  tris.shrink_to_fit();
+
+  // Planes, slots, the static meshes' hierarchy.
+  std::vector<float> planes(64 * (size_t)n_blocks), soa(64 * (size_t)n_blocks);
+  cnq::host::block_planes(texels, blocks, n_blocks, light, planes.data());
+  for (uint32_t b = 0; b < n_blocks; b++)
+    for (uint32_t p = 0; p < cnq::BLOCK_PLANES; p++)
+      std::memcpy(&soa[4 * ((size_t)p * n_blocks + b)], &planes[64 * (size_t)b + 4 * p], 16);
+  std::vector<uint32_t> block_slot(n_blocks), slot_block;
+  for (uint32_t b = 0; b < n_blocks; b++) {
+    const float* p0 = &planes[64 * (size_t)b];
+    const bool never = p0[0] == 0.0f && p0[1] == 0.0f && p0[2] == 0.0f && p0[3] == -1.0f;
+    block_slot[b] = never ? NONE : (uint32_t)slot_block.size();
+    if (!never) slot_block.push_back(b);
+  }
+  c->n_slots = (uint32_t)slot_block.size();
+  std::vector<uint8_t> keep(n_obj);
+  for (uint32_t m = 0; m < n_obj; m++) keep[m] = c->dynamic[m] ? 0 : 1;
+  std::vector<NodeC> snodes;
+  std::vector<uint32_t> sitems;
+  mesh_hierarchy_of(c->bounds, keep.data(), snodes, sitems);
+
+  hipError_t e = c->texels.upload(texels, 512 * (size_t)n_blocks);
+  if (e == hipSuccess) e = c->planes.upload(soa.data(), soa.size());
+  if (e == hipSuccess) e = c->block_slot.upload(block_slot.data(), block_slot.size());
+  if (e == hipSuccess) e = c->slot_block.upload(slot_block.data(), slot_block.size());
+  if (e == hipSuccess) e = c->static_cov.alloc((size_t)c->n_slots * 64 * 64);
+  if (e == hipSuccess) e = c->irradiance.alloc(64 * (size_t)n_blocks);
+  if (e == hipSuccess) e = c->dirty.alloc(n_blocks);
+  if (e == hipSuccess) e = c->n_dirty.alloc(1);
+  if (e == hipSuccess) e = c->static_nodes.upload(snodes.data(), snodes.size());
+  if (e == hipSuccess) e = c->static_items.upload(sitems.data(), sitems.size());
+  if (e == hipSuccess) e = hipMemset(c->n_dirty.p, 0, 4);
+  if (e != hipSuccess) {
+    c->release();
+    delete c;
+    return hip_fail(e, e == hipErrorOutOfMemory ? "cache buffers (out of device memory)" : "cache buffers");
+  }
+  int dev_cus = 0;
+  (void)hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, s->dev->ordinal);
+  c->rebake_grid = (uint32_t)std::max(1, dev_cus) * 8;
+  e = cnq::launch_bake_static(c->args(), nullptr);
+  if (e == hipSuccess && irradiance)
+    e = hipMemcpy(c->irradiance.p, irradiance, 256 * (size_t)n_blocks, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    c->release();
+    delete c;
+    return hip_fail(e, "static coverage bake");
+  }
+  if (!irradiance && n_blocks) {
+    // MaskConeKernel::bake: one query per texel through the whole scene.
+    e = c->bake_full.alloc(64 * (size_t)n_blocks);
+    if (e == hipSuccess) e = c->bake_touched.alloc(64 * (size_t)n_blocks);
+    float l16[16];
+    std::memcpy(l16, &c->light, 64);
+    DevBuf<float> l;
+    if (e == hipSuccess) e = l.upload(l16, 16);
+    if (e != hipSuccess) {
+      l.release();
+      c->release();
+      delete c;
+      return hip_fail(e, "lightmap bake buffers");
+    }
+    rc = cnq_irradiance_device(s, c->texels.p, 64 * (uint64_t)n_blocks, l.p, 1, o, c->irradiance.p, c->bake_full.p,
+                               c->bake_touched.p, nullptr, nullptr);
+    e = hipDeviceSynchronize();
+    l.release();
+    c->bake_full.release();
+    c->bake_touched.release();
+    if (rc || e != hipSuccess) {
+      c->release();
+      delete c;
+      return rc ? rc : hip_fail(e, "lightmap bake");
+    }
+  }
+  HIP_TRY(hipDeviceSynchronize(), "cache set-up");
+  *out = c;
+  return CNQ_OK;
+}
+
+void cnq_cache_destroy(cnq_cache* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->scene->dev->ordinal);
+  (void)hipDeviceSynchronize();
+  c->release();
+  delete c;
+}
+
+int cnq_cache_update(cnq_cache* c, const uint32_t* meshes, uint32_t n, const float* tris, void* stream) {
+  if (!c || (n && (!meshes || !tris))) return fail(CNQ_INVALID_ARG, "null argument");
+  cnq_scene* s = c->scene;
+  HIP_TRY(hipSetDevice(s->dev->ordinal), "hipSetDevice");
+  hipStream_t st = (hipStream_t)stream;
+  // Validate every change before touching anything.
+  std::vector<size_t> offset(n);
+  size_t off = 0;
+  uint32_t changed = 0;
+  for (uint32_t i = 0; i < n; i++) {
+    const uint32_t m = meshes[i];
+    if (m >= s->n_objects) return fail(CNQ_INVALID_ARG, "no mesh " + std::to_string(m));
+    if (!c->dynamic[m]) return fail(CNQ_INVALID_ARG, "mesh " + std::to_string(m) + " has lightmap texels and cannot move");
+    offset[i] = off;
+    const size_t count = s->h_ranges[4 * m + 1];
+    if (std::memcmp(tris + off, c->mesh_tris[m].data(), 36 * count) != 0) changed++;
+    off += 9 * count;
+  }
+  if (changed > cnq::MAX_SWEPT) return fail(CNQ_CAPACITY, "at most 16 meshes per update");
+  cnq::SweptBoxes sw{};
+  sw.n = 0;
+  std::vector<NodeC> cn;
+  std::vector<float> leaf, t12;
+  for (uint32_t i = 0; i < n; i++) {
+    const uint32_t m = meshes[i];
+    const uint32_t first = s->h_ranges[4 * m], count = s->h_ranges[4 * m + 1], root = s->h_ranges[4 * m + 2];
+    const float* t9 = tris + offset[i];
+    if (std::memcmp(t9, c->mesh_tris[m].data(), 36 * (size_t)count) == 0) continue;
+    const cnq::host::Box old = c->bounds[m], now = tri9_bounds(t9, count);
+    for (int k = 0; k < 3; k++) {
+      (&sw.lo[sw.n].x)[k] = old.lo[k] < now.lo[k] ? old.lo[k] : now.lo[k];
+      (&sw.hi[sw.n].x)[k] = old.hi[k] > now.hi[k] ? old.hi[k] : now.hi[k];
+    }
+    sw.lo[sw.n].w = sw.hi[sw.n].w = 0.0f;
+    sw.n++;
+    t12.assign(12 * (size_t)count, 0.0f);
+    for (size_t v = 0; v < 3 * (size_t)count; v++)
+      for (int k = 0; k < 3; k++) t12[4 * v + k] = t9[3 * v + k];
+    HIP_TRY(hipMemcpyAsync(s->tris.p + 12 * (size_t)first, t12.data(), t12.size() * 4, hipMemcpyHostToDevice, st),
+            "mesh triangles");
+    if (root != NONE) {
+      // Same count, so the same shape and offsets (live.rs:858-873).
+      std::vector<cnq::host::NodeGpu> nodes;
+      std::vector<uint32_t> items;
+      std::vector<float> cov;
+      cnq::host::triangle_hierarchy(t9, count, root, first, nodes, items, cov);
+      if (nodes.size() != c->node_span[m]) return fail(CNQ_INVALID_ARG, "mesh hierarchy changed shape");
+      cn.resize(nodes.size());
+      for (size_t k = 0; k < nodes.size(); k++) cn[k] = compact(reinterpret_cast<const uint32_t*>(&nodes[k]), false);
+      leaf.resize(9 * (size_t)count);
+      for (size_t k = 0; k < count; k++) std::memcpy(&leaf[9 * k], t9 + 9 * (size_t)(items[k] - first), 36);
+      HIP_TRY(hipMemcpyAsync(s->blas.p + root, cn.data(), cn.size() * sizeof(NodeC), hipMemcpyHostToDevice, st),
+              "mesh nodes");
+      HIP_TRY(hipMemcpyAsync(s->blas_cov.p + 27 * (size_t)root, cov.data(), cov.size() * 4, hipMemcpyHostToDevice, st),
+              "mesh coverage");
+      HIP_TRY(hipMemcpyAsync(s->blas_items.p + first, items.data(), items.size() * 4, hipMemcpyHostToDevice, st),
+              "mesh items");
+      HIP_TRY(hipMemcpyAsync(s->tri_leaf.p + 9 * (size_t)first, leaf.data(), leaf.size() * 4, hipMemcpyHostToDevice, st),
+              "mesh leaves");
+    }
+    HIP_TRY(hipStreamSynchronize(st), "mesh upload");
+    c->mesh_tris[m].assign(t9, t9 + 9 * (size_t)count);
+    c->bounds[m] = now;
+  }
+  if (sw.n == 0) return CNQ_OK;
+  // Median splits depend only on the mesh count: the layout is unchanged.
+  std::vector<NodeC> nodes;
+  std::vector<uint32_t> items;
+  mesh_hierarchy_of(c->bounds, nullptr, nodes, items);
+  if (nodes.size() != s->n_nodes || items.size() != s->n_items)
+    return fail(CNQ_INVALID_ARG, "the scene's object hierarchy was not built by mesh_hierarchy");
+  HIP_TRY(hipMemcpyAsync(s->nodes.p, nodes.data(), nodes.size() * sizeof(NodeC), hipMemcpyHostToDevice, st), "nodes");
+  HIP_TRY(hipMemcpyAsync(s->items.p, items.data(), items.size() * 4, hipMemcpyHostToDevice, st), "items");
+  HIP_TRY(hipStreamSynchronize(st), "object hierarchy upload");
+  // Every dynamic mesh with triangles, by its padded bounds (live.rs:487-508).
+  cnq::MovingMeshes mv{};
+  mv.n = 0;
+  for (uint32_t m = 0; m < s->n_objects; m++) {
+    if (!c->dynamic[m] || s->h_ranges[4 * m + 1] == 0) continue;
+    const cnq::host::Box p = cnq::host::padded(c->bounds[m]);
+    mv.lo[mv.n] = make_float4(p.lo[0], p.lo[1], p.lo[2], 0.0f);
+    mv.hi[mv.n] = make_float4(p.hi[0], p.hi[1], p.hi[2], 0.0f);
+    mv.mesh[mv.n] = m;
+    mv.n++;
+  }
+  const cnq::CacheArgs a = c->args();
+  HIP_TRY(fill_u32(c->n_dirty.p, 0, 1, st), "clear marks");
+  HIP_TRY(cnq::launch_mark_blocks(a, sw, st), "mark_blocks launch");
+  HIP_TRY(cnq::launch_rebake(a, mv, std::min(c->rebake_grid, c->n_blocks), st), "rebake launch");
+  return CNQ_OK;
+}
+
+int cnq_cache_read(cnq_cache* c, float* irradiance, uint32_t* blocks, uint32_t* n_blocks) {
+  if (!c) return fail(CNQ_INVALID_ARG, "cache is null");
+  HIP_TRY(hipSetDevice(c->scene->dev->ordinal), "hipSetDevice");
+  HIP_TRY(hipDeviceSynchronize(), "sync");
+  uint32_t nd = 0;
+  HIP_TRY(hipMemcpy(&nd, c->n_dirty.p, 4, hipMemcpyDeviceToHost), "read marks");
+  if (n_blocks) *n_blocks = nd;
+  if (blocks && nd) HIP_TRY(hipMemcpy(blocks, c->dirty.p, 4 * (size_t)nd, hipMemcpyDeviceToHost), "read blocks");
+  if (irradiance && c->n_blocks)
+    HIP_TRY(hipMemcpy(irradiance, c->irradiance.p, 256 * (size_t)c->n_blocks, hipMemcpyDeviceToHost), "read texels");
   return CNQ_OK;
 }
 

@@ -118,8 +118,51 @@ struct RefitArgs {
   uint32_t n_leaves;
 };
 
+// The lighting cache: a lightmap's texels in blocks of 8 x 8 (64
+// consecutive receivers), re-baked when meshes move.
+//   planes      float4[16][n_blocks]  plane-major, so a wave of consecutive
+//                                     blocks reads each plane coalesced
+//   block_slot  u32 per block: its slot in static_cov, or NONE
+//   slot_block  u32 per slot: the block
+//   static_cov  f32[slot][bin][texel]: the static meshes' coverage, bin-major
+//               within a block so a wave's 64 texels read each bin coalesced
+constexpr uint32_t BLOCK_PLANES = 16;  // lightmap.rs:83
+constexpr uint32_t MAX_SWEPT = 16;     // live.rs:220
+constexpr uint32_t MAX_MOVING = 16;    // live.rs:175
+
+struct SweptBoxes {
+  float4 lo[MAX_SWEPT], hi[MAX_SWEPT];
+  uint32_t n;
+};
+
+struct MovingMeshes {
+  float4 lo[MAX_MOVING], hi[MAX_MOVING];  // padded bounds
+  uint32_t mesh[MAX_MOVING];
+  uint32_t n;
+};
+
+struct CacheArgs {
+  SceneArgs s;                     // the whole scene (walks of moving meshes)
+  const NodeC* __restrict__ static_nodes;  // the object hierarchy of the static meshes
+  const uint32_t* __restrict__ static_items;
+  const float* __restrict__ texels;        // 8 floats per texel (point, normal)
+  const float4* __restrict__ planes;
+  const uint32_t* __restrict__ block_slot;
+  const uint32_t* __restrict__ slot_block;
+  float* __restrict__ static_cov;
+  uint32_t* __restrict__ dirty;            // marked blocks
+  uint32_t* __restrict__ n_dirty;          // their count (zeroed before marking)
+  float* __restrict__ irradiance;          // per texel
+  LightRect light;
+  QueryParams p;
+  uint32_t n_blocks, n_slots;
+};
+
 hipError_t launch_cone_items(const ItemArgs& a, bool count_work, uint32_t persistent_blocks, hipStream_t stream);
 hipError_t launch_primary(const PrimaryArgs& a, hipStream_t stream);
 hipError_t launch_refit(const RefitArgs& a, hipStream_t stream);
+hipError_t launch_bake_static(const CacheArgs& a, hipStream_t stream);
+hipError_t launch_mark_blocks(const CacheArgs& a, const SweptBoxes& boxes, hipStream_t stream);
+hipError_t launch_rebake(const CacheArgs& a, const MovingMeshes& moving, uint32_t grid, hipStream_t stream);
 
 }  // namespace cnq

@@ -295,6 +295,91 @@ class Batch:
         check(lib().cnq_batch_set_lights(self.handle, _ptr(self._lights), len(self._lights)))
 
 
+def block_planes(texels: np.ndarray, blocks: np.ndarray, light: np.ndarray) -> np.ndarray:
+    """Lightmap::block_planes (lightmap.rs:256-333): (n_blocks, 16, 4) planes
+    (n, w; inside n.p <= w) per block of 64 texels toward a rect light."""
+    texels = np.ascontiguousarray(texels, dtype=np.float32).reshape(-1, 64, 8)
+    blocks = np.ascontiguousarray(blocks, dtype=np.float32).reshape(-1, 6)
+    light = np.ascontiguousarray(light, dtype=np.float32).reshape(16)
+    if len(texels) != len(blocks):
+        raise ValueError("one box per block of 64 texels")
+    out = np.zeros((len(blocks), 16, 4), np.float32)
+    check(lib().cnq_block_planes(_ptr(texels), _ptr(blocks), len(blocks), _ptr(light), _ptr(out)))
+    return out
+
+
+class LiveLightmap:
+    """A lightmap's direct irradiance from one rect light, kept current as
+    dynamic meshes move: the lighting cache of the reference's
+    `LiveRenderer` (live.rs:255-1090). Texels are 64 receivers per block
+    (`Lightmap::receivers`) with each block's bounds (`Lightmap::blocks`);
+    `irradiance` is the baked lightmap (`Lightmap::irradiance`), or None to
+    bake it here (`MaskConeKernel::bake`, lib.rs:2961)."""
+
+    def __init__(self, scene: Scene, texels: np.ndarray, blocks: np.ndarray, light: np.ndarray,
+                 dynamic: Sequence[bool], irradiance: Optional[np.ndarray] = None,
+                 opts: Optional[_lib.Options] = None):
+        self.scene = scene
+        self._texels = np.ascontiguousarray(texels, dtype=np.float32).reshape(-1, 8)
+        self._blocks = np.ascontiguousarray(blocks, dtype=np.float32).reshape(-1, 6)
+        if len(self._texels) != 64 * len(self._blocks):
+            raise ValueError("64 texels per block")
+        self._light = np.ascontiguousarray(light, dtype=np.float32).reshape(16)
+        self._dynamic = np.ascontiguousarray([1 if d else 0 for d in dynamic], dtype=np.uint8)
+        self.counts = [int(c) for c in scene.flat.ranges[:, 1]]
+        if len(self._dynamic) != len(self.counts):
+            raise ValueError("one dynamic flag per mesh")
+        self.opts = opts or default_options(hierarchy=2)
+        irr = None if irradiance is None else np.ascontiguousarray(irradiance, dtype=np.float32)
+        h = C.c_void_p()
+        check(lib().cnq_cache_create(scene.handle, _ptr(self._texels), _ptr(self._blocks), len(self._blocks),
+                                     _ptr(self._light), _ptr(self._dynamic), 0 if irr is None else _ptr(irr),
+                                     C.byref(self.opts), C.byref(h)))
+        self.handle = h
+        self._destroy = lib().cnq_cache_destroy
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            self._destroy(self.handle)
+            self.handle = None
+
+    @property
+    def blocks(self) -> int:
+        """Total lightmap blocks (live.rs:1088)."""
+        return len(self._blocks)
+
+    def update_meshes(self, changes: Sequence[tuple], stream=None) -> None:
+        """LiveRenderer::update_meshes (live.rs:795): (mesh, triangles) pairs,
+        each mesh dynamic and keeping its triangle count."""
+        if not changes:
+            return
+        ids = np.ascontiguousarray([int(m) for m, _ in changes], dtype=np.uint32)
+        parts = []
+        for m, tris in changes:
+            t = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 9)
+            if len(t) != self.counts[int(m)]:
+                raise ValueError(f"mesh {m} keeps its {self.counts[int(m)]} triangles")
+            parts.append(t)
+        tris = np.ascontiguousarray(np.concatenate(parts), dtype=np.float32)
+        check(lib().cnq_cache_update(self.handle, _ptr(ids), len(ids), _ptr(tris), _stream_ptr(stream)))
+
+    def read(self):
+        """(texel irradiance, the blocks the last update re-baked, sorted)."""
+        irr = np.zeros(64 * self.blocks, np.float32)
+        marked = np.zeros(max(self.blocks, 1), np.uint32)
+        n = C.c_uint32()
+        check(lib().cnq_cache_read(self.handle, _ptr(irr), _ptr(marked), C.byref(n)))
+        return irr, np.sort(marked[: n.value])
+
+    def read_texels(self) -> np.ndarray:
+        """LiveRenderer::read_texels (live.rs:1078)."""
+        return self.read()[0]
+
+    def rebaked_blocks(self) -> int:
+        """LiveRenderer::rebaked_blocks (live.rs:1083)."""
+        return len(self.read()[1])
+
+
 def _stream_ptr(stream) -> int:
     if stream is None:
         return 0

@@ -168,6 +168,44 @@ def room_scene(seed: int = 0xC0_0001, width: int = 256, height: int = 256) -> Sc
     return SceneSpec("room", faces + boxes, light[None, :], cam, width, height, prefilter=0.0)
 
 
+def quad_pair(a, du, dv) -> np.ndarray:
+    """One parallelogram a + [0,1]du + [0,1]dv as the two triangles (a, b, c),
+    (a, c, d) that `Surface::from_triangles` takes as one chart."""
+    a, du, dv = (np.asarray(x, dtype=F32) for x in (a, du, dv))
+    b, c, d = a + du, a + du + dv, a + dv
+    return np.array([[a, b, c], [a, c, d]], dtype=F32)
+
+
+def lit_room(seed: int = 0xC0_0005, boxes: int = 2, size: float = 1.0) -> SceneSpec:
+    """A room of large quads (floor, ceiling and four walls, each one chart)
+    with a static pillar and `boxes` dynamic boxes on the floor, lit by a
+    one-sided ceiling rect: the lighting cache's workload. Meshes: 6 faces,
+    the pillar, then the boxes; `notes['dynamic']` flags the boxes."""
+    X, Y, Z = 4.0 * size, 4.0 * size, 3.0 * size
+    faces = [
+        quad_pair([0, 0, 0], [X, 0, 0], [0, Y, 0]),
+        quad_pair([0, 0, Z], [0, Y, 0], [X, 0, 0]),
+        quad_pair([0, 0, 0], [0, Y, 0], [0, 0, Z]),
+        quad_pair([X, 0, 0], [0, 0, Z], [0, Y, 0]),
+        quad_pair([0, 0, 0], [0, 0, Z], [X, 0, 0]),
+        quad_pair([0, Y, 0], [X, 0, 0], [0, 0, Z]),
+    ]
+    pillar = box_mesh([0.6 * size, 2.8 * size, 0.0], [1.0 * size, 3.2 * size, 2.2 * size])
+    rng = Pcg32(seed, 11)
+    movers = []
+    for _ in range(boxes):
+        s = [(0.25 + 0.35 * rng.next_f32()) * size for _ in range(3)]
+        x = (0.4 + rng.next_f32() * 2.8) * size
+        y = (0.4 + rng.next_f32() * 2.2) * size
+        movers.append(box_mesh([x, y, 0.0], [x + s[0], y + s[1], s[2]]))
+    light = rect_light([1.6 * size, 2.4 * size, Z - 0.001], [0.8 * size, 0, 0], [0, -0.8 * size, 0], one_sided=True)
+    cam = camera_uniform([0.4 * size, 0.4 * size, 1.6 * size], [3.6 * size, 3.6 * size, 0.8 * size], 60.0, 256, 256)
+    meshes = faces + [pillar] + movers
+    spec = SceneSpec("lit room", meshes, light[None, :], cam, 256, 256, prefilter=0.0)
+    spec.notes["dynamic"] = [False] * 7 + [True] * boxes
+    return spec
+
+
 def _icosphere_grid(freq: int) -> np.ndarray:
     """Unit geodesic sphere: each icosahedron face split into freq^2
     triangles, vertices projected to the sphere; (20 freq^2, 3, 3)."""

@@ -206,6 +206,47 @@ int cnq_batch_read(cnq_batch* batch, uint32_t view, uint32_t* hits, float* out_v
 /* Replace the batch's lights (same count): a light edit. */
 int cnq_batch_set_lights(cnq_batch* batch, const float* lights, uint32_t n_lights);
 
+/* ---- The lighting cache: a lightmap's direct irradiance from one rect
+ * light, kept current as meshes move (the reference's `LiveRenderer`,
+ * crates/continuum-gpu/src/live.rs, over a `Lightmap`, lightmap.rs).
+ * Texels come in blocks of 8 x 8: 64 consecutive receivers per block
+ * (Lightmap::receivers; texels on no triangle have a zero normal), with
+ * each block's bounds (Lightmap::blocks, 6 floats: lo, hi). Meshes flagged
+ * dynamic may move; the others hold the texels. */
+#define CNQ_BLOCK_PLANES 16 /* lightmap.rs:83 BLOCK_PLANES */
+#define CNQ_MAX_SWEPT 16    /* live.rs:220 MAX_SWEPT: changed meshes per update */
+#define CNQ_MAX_MOVING 16   /* live.rs:175 MAX_MOVING: dynamic meshes */
+
+typedef struct cnq_cache cnq_cache;
+
+/* Lightmap::block_planes (lightmap.rs:256-333): 16 planes (n.xyz, w;
+ * inside n.p <= w) per block enclosing its texels' cones toward the rect
+ * `light`. Host only. */
+int cnq_block_planes(const float* texels, const float* blocks, uint32_t n_blocks, const float* light,
+                     float* planes);
+
+/* LiveRenderer::new's lightmap set-up (live.rs:515-720): block planes,
+ * static-coverage slots, the static meshes' hierarchy (`mesh_hierarchy_of`,
+ * hierarchy.rs:188) and the `bake_static` pass (WGSL :2468). `irradiance`
+ * (64 per block) is the baked lightmap (Lightmap::irradiance); NULL bakes
+ * it here as MaskConeKernel::bake does (lib.rs:2961). The scene must be
+ * flattened by the reference's `flatten` (no transforms), `opts.hierarchy`
+ * Triangles; the cache keeps `scene`, which must outlive it. */
+int cnq_cache_create(cnq_scene* scene, const float* texels, const float* blocks, uint32_t n_blocks,
+                     const float* light, const uint8_t* dynamic, const float* irradiance, const cnq_options* opts,
+                     cnq_cache** out);
+void cnq_cache_destroy(cnq_cache* cache);
+/* LiveRenderer::update_meshes (live.rs:795-900): `n` dynamic meshes take
+ * new triangles (9 floats each, the mesh's own count, concatenated in the
+ * order of `meshes`). Each changed mesh's hierarchy is rebuilt in place and
+ * the object hierarchy over all meshes rebuilt; then `mark_blocks` (WGSL
+ * :2430) marks the blocks its swept bounds can reach and `rebake` (WGSL
+ * :2514) recomputes their texels, enqueued on `stream`. */
+int cnq_cache_update(cnq_cache* cache, const uint32_t* meshes, uint32_t n, const float* tris, void* stream);
+/* Synchronize and copy the texels' irradiance (64 per block, nullable), the
+ * blocks the last update re-baked (nullable; any order) and their count. */
+int cnq_cache_read(cnq_cache* cache, float* irradiance, uint32_t* blocks, uint32_t* n_blocks);
+
 #ifdef __cplusplus
 }
 #endif

@@ -160,11 +160,13 @@ static void triangle_hierarchy(const float* mesh, size_t n, uint32_t root, uint3
   for (size_t i = 0; i < n; i++) items_out[i] = first + items[i];
 }
 
-// hierarchy.rs:198-253 (every object kept).
+// hierarchy.rs:198-253, over the objects `keep` selects (null: all);
+// indices stay global.
 static void object_hierarchy(const std::vector<Box>& bounds, std::vector<NodeGpu>& nodes,
-                             std::vector<uint32_t>& items) {
-  items.resize(bounds.size());
-  for (size_t i = 0; i < bounds.size(); i++) items[i] = (uint32_t)i;
+                             std::vector<uint32_t>& items, const uint8_t* keep = nullptr) {
+  items.clear();
+  for (size_t i = 0; i < bounds.size(); i++)
+    if (!keep || keep[i]) items.push_back((uint32_t)i);
   nodes.assign(1, NodeGpu{});
   if (items.empty()) {
     items.push_back(0);
@@ -316,6 +318,31 @@ void oracle_refit(const float* tris, const uint32_t* ranges, uint32_t n_ranges, 
       blas[nn].hi[0] = hi.x, blas[nn].hi[1] = hi.y, blas[nn].hi[2] = hi.z, blas[nn].hi[3] = 0.0f;
     }
   }
+}
+
+// mesh_hierarchy_of (hierarchy.rs:188-196): the object hierarchy over the
+// padded bounds of the meshes `keep` selects (null: all), mesh indices
+// global. Meshes as in oracle_flatten. Two calls: with null outputs it
+// returns [n_nodes, n_items] in `sizes`; then it fills them.
+int oracle_mesh_hierarchy(const float* tris, const uint32_t* counts, uint32_t n_meshes, const uint8_t* keep,
+                          uint64_t* sizes, float* out_nodes, uint32_t* out_items) {
+  std::vector<Box> boxes;
+  size_t off = 0;
+  for (uint32_t m = 0; m < n_meshes; m++) {
+    boxes.push_back(padded(tri_bounds(tris + 9 * off, counts[m])));
+    off += counts[m];
+  }
+  std::vector<NodeGpu> nodes;
+  std::vector<uint32_t> items;
+  object_hierarchy(boxes, nodes, items, keep);
+  if (!out_nodes) {
+    sizes[0] = nodes.size();
+    sizes[1] = items.size();
+    return 0;
+  }
+  std::memcpy(out_nodes, nodes.data(), nodes.size() * sizeof(NodeGpu));
+  std::memcpy(out_items, items.data(), items.size() * 4);
+  return 0;
 }
 
 }  // extern "C"

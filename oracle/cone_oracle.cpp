@@ -8,6 +8,7 @@
 // (restated separately in cone3_oracle.cpp). One query walks alone (no
 // packets), exactly in the order the WGSL walks.
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cstring>
 #include <thread>
@@ -1047,6 +1048,135 @@ static Result hit_irradiance_rect(const Scene& s, const Options& o, V3 point, V3
   return res;
 }
 
+// ---- The lighting cache: lightmap blocks of 8 x 8 texels re-baked when
+// meshes move (lightmap.rs:256-333; WGSL :2408-2520; live.rs:515-720, 815-900).
+
+constexpr uint32_t BLOCK_PLANES = 16;  // lightmap.rs:83
+
+// glam's Vec3::normalize (v * (1 / length)), for host-side planes.
+static V3 glam_normalize(V3 a) { return a * (1.0f / length(a)); }
+
+// Lightmap::block_planes (lightmap.rs:256-333) for one block: the hull of
+// the block's box and the light's corners, pushed out by 1e-3; or one plane
+// that excludes everything when no texel can receive the light.
+static void block_planes_one(const float* texels, const float* box, const Light& light, float* out) {
+  const float MARGIN = 1e-3f;
+  const V3 corner = xyz(light.corner), eu = xyz(light.edge_u), ev = xyz(light.edge_v);
+  const bool one_sided = light.corner[3] > 0.5f;
+  const V3 center = corner + (eu + ev) * 0.5f;
+  const V3 emits = glam_normalize(cross(eu, ev));
+  const V3 light_corners[4] = {corner, corner + eu, corner + ev, corner + eu + ev};
+  std::vector<std::array<float, 4>> planes;
+  bool lit = false;
+  for (int t = 0; t < 64 && !lit; t++) {
+    V3 p = xyz(texels + 8 * t), n = xyz(texels + 8 * t + 4);
+    bool nonzero = n.x != 0.0f || n.y != 0.0f || n.z != 0.0f;
+    lit = nonzero && dot(n, center - p) > -1e-3f && (!one_sided || dot(emits, p - corner) > -1e-3f);
+  }
+  if (!lit) {
+    planes.push_back({0.0f, 0.0f, 0.0f, -1.0f});
+  } else {
+    const V3 lo = xyz(box), hi = xyz(box + 3);
+    std::vector<V3> pts;
+    for (int k = 0; k < 8; k++) {
+      V3 p{(k & 1) ? hi.x : lo.x, (k & 2) ? hi.y : lo.y, (k & 4) ? hi.z : lo.z};
+      bool seen = false;
+      for (const V3& q : pts) seen = seen || (q.x == p.x && q.y == p.y && q.z == p.z);
+      if (!seen) pts.push_back(p);
+    }
+    for (const V3& c : light_corners) pts.push_back(c);
+    for (size_t i = 0; i < pts.size(); i++)
+      for (size_t j = i + 1; j < pts.size(); j++)
+        for (size_t k = j + 1; k < pts.size(); k++) {
+          V3 n = cross(pts[j] - pts[i], pts[k] - pts[i]);
+          if (dot(n, n) < 1e-14f) continue;
+          n = glam_normalize(n);
+          float d = dot(n, pts[i]);
+          float below = 0.0f, above = 0.0f;
+          for (const V3& p : pts) {
+            below = std::fmin(below, dot(n, p) - d);
+            above = std::fmax(above, dot(n, p) - d);
+          }
+          if (above <= 1e-6f) {
+          } else if (below >= -1e-6f) {
+            n = -n;
+            d = -d;
+          } else {
+            continue;
+          }
+          bool dup = false;
+          for (const auto& q : planes)
+            dup = dup || (dot(V3{q[0], q[1], q[2]}, n) > 1.0f - 1e-6f && std::fabs(q[3] - MARGIN - d) < 1e-6f);
+          if (dup) continue;
+          planes.push_back({n.x, n.y, n.z, d + MARGIN});
+        }
+  }
+  planes.resize(BLOCK_PLANES, {0.0f, 0.0f, 0.0f, 0.0f});
+  for (uint32_t p = 0; p < BLOCK_PLANES; p++)
+    for (int c = 0; c < 4; c++) out[4 * p + c] = planes[p][c];
+}
+
+// WGSL :2430-2452 mark_blocks for one block: true unless a plane separates
+// every swept box from it.
+static bool block_marked(const float* planes, const float* boxes, uint32_t n_boxes) {
+  for (uint32_t k = 0; k < n_boxes; k++) {
+    const V3 lo = xyz(boxes + 8 * k), hi = xyz(boxes + 8 * k + 4);
+    const V3 center = (lo + hi) * 0.5f, extent = (hi - lo) * 0.5f;
+    bool separated = false;
+    for (uint32_t p = 0; p < BLOCK_PLANES && !separated; p++) {
+      const float* q = planes + 4 * p;
+      const V3 n = xyz(q);
+      separated = dot(n, center) - dot(vabs(n), extent) > q[3];
+    }
+    if (!separated) return true;
+  }
+  return false;
+}
+
+// WGSL :2468-2487 bake_static for texel `t` of a block with a slot: the
+// coverage of the meshes in the scene's (static) object hierarchy, uncapped.
+static void static_coverage(const Scene& s, const Options& o, const float* texel, const Light& light, float* cov) {
+  Query qs;
+  qs.s = &s;
+  qs.o = &o;
+  for (int b = 0; b < 64; b++) cov[b] = 0.0f;
+  const V3 point = xyz(texel), normal = xyz(texel + 4);
+  const V3 to_light = xyz(light.corner) + (xyz(light.edge_u) + xyz(light.edge_v)) * 0.5f - point;
+  if (!(dot(normal, to_light) > 0.0f)) return;
+  LightCone lc = light_cone_rect(qs, point, light);
+  if (!lc.ok) return;
+  uint32_t full = 0;
+  scene_coverage(lc.c, qs, cov, full);
+}
+
+// WGSL :2491-2510 cached_irradiance: the static coverage plus every moving
+// mesh (in order) whose padded bounds the cone may touch, then the window's
+// visible weight.
+static float cached_irradiance(const Scene& s, const Options& o, const float* texel, const Light& light,
+                               const float* static_cov, const uint32_t* moving, const float* moving_boxes,
+                               uint32_t n_moving) {
+  Query qs;
+  qs.s = &s;
+  qs.o = &o;
+  const V3 apex = xyz(texel), normal = xyz(texel + 4);
+  const V3 to_light = xyz(light.corner) + (xyz(light.edge_u) + xyz(light.edge_v)) * 0.5f - apex;
+  if (dot(normal, to_light) <= 0.0f) return 0.0f;
+  LightCone lc = light_cone_rect(qs, apex, light);
+  if (!lc.ok) return 0.0f;
+  float cov[64];
+  uint32_t full = 0;
+  for (int b = 0; b < 64; b++) {
+    cov[b] = static_cov[b];
+    if (cov[b] >= 1.0f) full++;
+  }
+  for (uint32_t k = 0; k < n_moving; k++) {
+    if (full == BINS) break;
+    if (!may_touch_box(lc.c, xyz(moving_boxes + 8 * k), xyz(moving_boxes + 8 * k + 4))) continue;
+    add_mesh_coverage(lc.c, qs, moving[k], cov, full, 0);
+  }
+  return wide_visible(qs, lc.c, apex, normal, light, cone_visible(lc.c, normal, cov));
+}
+
 }  // namespace orc
 
 using namespace orc;
@@ -1065,6 +1195,38 @@ static Scene make_scene(const float* tris, const uint32_t* ranges, uint32_t n_ob
   s.items = items;
   s.xforms = xforms;
   return s;
+}
+
+static Options make_options(const uint32_t* opts) {
+  Options o;
+  o.hierarchy = opts[0];
+  std::memcpy(&o.prefilter, &opts[1], 4);
+  std::memcpy(&o.wide_extent, &opts[2], 4);
+  o.light_cone_cap = opts[3];
+  o.wide_cone_cap = opts[4];
+  o.profile_stage = opts[5];
+  o.object_cov = opts[6];
+  return o;
+}
+
+// fn(i) for i in [0, n), over `threads` threads in chunks of 256.
+template <class F>
+static void parallel_for(uint64_t n, int threads, F fn) {
+  std::atomic<uint64_t> next{0};
+  auto work = [&]() {
+    for (;;) {
+      uint64_t i0 = next.fetch_add(256);
+      if (i0 >= n) break;
+      for (uint64_t i = i0; i < std::min(n, i0 + 256); i++) fn(i);
+    }
+  };
+  if (threads <= 1) {
+    work();
+    return;
+  }
+  std::vector<std::thread> pool;
+  for (int t = 0; t < threads; t++) pool.emplace_back(work);
+  for (auto& t : pool) t.join();
 }
 
 extern "C" {
@@ -1193,6 +1355,63 @@ float oracle_rect_open(const float* point, const float* normal, const float* lig
 
 float oracle_sphere_unshadowed(const float* point, const float* normal, const float* light) {
   return sphere_unshadowed(xyz(point), xyz(normal), *reinterpret_cast<const Light*>(light));
+}
+
+// Lightmap::block_planes: texels are 64 receivers (point vec4, normal vec4)
+// per block, `boxes` 6 floats (lo, hi) per block, the light a LightGpu rect;
+// 16 planes (n, w) per block out.
+void oracle_block_planes(const float* texels, const float* boxes, uint32_t n_blocks, const float* light,
+                         float* planes) {
+  const Light& l = *reinterpret_cast<const Light*>(light);
+  for (uint32_t b = 0; b < n_blocks; b++)
+    block_planes_one(texels + 512 * (size_t)b, boxes + 6 * (size_t)b, l, planes + 64 * (size_t)b);
+}
+
+// mark_blocks: marked[b] = 1 when no plane of block b separates some swept
+// box (8 floats each: lo vec4, hi vec4) from it.
+void oracle_mark_blocks(const float* planes, uint32_t n_blocks, const float* boxes, uint32_t n_boxes,
+                        uint8_t* marked) {
+  for (uint32_t b = 0; b < n_blocks; b++) marked[b] = block_marked(planes + 64 * (size_t)b, boxes, n_boxes) ? 1 : 0;
+}
+
+// bake_static over every block with a slot (block_slot[b] != NONE): the
+// static scene's coverage per texel, 64 bins each, at
+// cov[(slot * 64 + texel) * 64 + bin] (the reference's layout). The scene's
+// object hierarchy (nodes/items) holds the static meshes only. opts as
+// oracle_irradiance's.
+void oracle_bake_static(const float* tris, const uint32_t* ranges, uint32_t n_objects, const float* blas,
+                        const uint32_t* blas_items, const float* blas_cov, const float* nodes, const uint32_t* items,
+                        const float* xforms, const float* texels, const uint32_t* block_slot, uint32_t n_blocks,
+                        const float* light, const uint32_t* opts, float* cov, int threads) {
+  Scene s = make_scene(tris, ranges, n_objects, blas, blas_items, blas_cov, nodes, items, xforms);
+  Options o = make_options(opts);
+  const Light& l = *reinterpret_cast<const Light*>(light);
+  parallel_for(n_blocks * 64ull, threads, [&](uint64_t i) {
+    uint32_t slot = block_slot[i / 64];
+    if (slot == NONE) return;
+    static_coverage(s, o, texels + 8 * i, l, cov + 64 * (64 * (size_t)slot + i % 64));
+  });
+}
+
+// rebake over `blocks`: every texel of each block from its cached static
+// coverage plus the moving meshes (`moving` mesh indices with padded bounds,
+// 8 floats each), into irradiance[block * 64 + texel].
+void oracle_rebake(const float* tris, const uint32_t* ranges, uint32_t n_objects, const float* blas,
+                   const uint32_t* blas_items, const float* blas_cov, const float* xforms, const float* texels,
+                   const uint32_t* block_slot, const float* static_cov, const uint32_t* blocks, uint32_t n_list,
+                   const float* light, const uint32_t* moving, const float* moving_boxes, uint32_t n_moving,
+                   const uint32_t* opts, float* irradiance, int threads) {
+  Scene s = make_scene(tris, ranges, n_objects, blas, blas_items, blas_cov, nullptr, nullptr, xforms);
+  Options o = make_options(opts);
+  const Light& l = *reinterpret_cast<const Light*>(light);
+  parallel_for(n_list * 64ull, threads, [&](uint64_t j) {
+    uint32_t block = blocks[j / 64], t = (uint32_t)(j % 64);
+    uint32_t slot = block_slot[block];
+    uint64_t i = 64ull * block + t;
+    irradiance[i] = slot == NONE ? 0.0f
+                                 : cached_irradiance(s, o, texels + 8 * i, l, static_cov + 64 * (64 * (size_t)slot + t),
+                                                     moving, moving_boxes, n_moving);
+  });
 }
 
 }  // extern "C"

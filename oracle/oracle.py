@@ -45,6 +45,12 @@ def lib() -> C.CDLL:
             "oracle_coverage_open_length": (C.c_float, [_P, C.c_uint32, C.c_float, C.c_float]),
             "oracle_cone3_visible": (None, [_P, C.c_uint32, _P, _P, _P, C.c_uint32, C.c_int, _P, _P]),
             "oracle_sampled_visible": (C.c_double, [_P, _P, _P, C.c_uint64, C.c_uint32]),
+            "oracle_mesh_hierarchy": (C.c_int, [_P, _P, C.c_uint32, _P, _P, _P, _P]),
+            "oracle_block_planes": (None, [_P, _P, C.c_uint32, _P, _P]),
+            "oracle_mark_blocks": (None, [_P, C.c_uint32, _P, C.c_uint32, _P]),
+            "oracle_bake_static": (None, [_P] * 9 + [_P, _P, C.c_uint32, _P, _P, _P, C.c_int]),
+            "oracle_rebake": (None, [_P] * 7 + [_P, _P, _P, _P, C.c_uint32, _P, _P, _P, C.c_uint32, _P, _P,
+                                                C.c_int]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -160,3 +166,89 @@ def sampled_visible(point, light: np.ndarray, meshes, k: int) -> float:
     p = np.ascontiguousarray(point, dtype=np.float32)
     lt = np.ascontiguousarray(light, dtype=np.float32)
     return float(lib().oracle_sampled_visible(_p(p), _p(lt), _p(tris9), len(tris9), k))
+
+
+# ---- The lighting cache (lightmap.rs:256-333; WGSL :2408-2520)
+
+def mesh_hierarchy(tris9: np.ndarray, counts, keep=None):
+    """mesh_hierarchy_of (hierarchy.rs:188): (nodes (n, 12) u32, items) over
+    the padded bounds of the meshes `keep` selects (None: all)."""
+    tris9 = np.ascontiguousarray(tris9, dtype=np.float32)
+    counts = np.ascontiguousarray(counts, dtype=np.uint32)
+    k = None if keep is None else np.ascontiguousarray([1 if x else 0 for x in keep], dtype=np.uint8)
+    sizes = np.zeros(2, np.uint64)
+    L = lib()
+    L.oracle_mesh_hierarchy(_p(tris9), _p(counts), len(counts), _p(k), _p(sizes), 0, 0)
+    nodes = np.zeros((int(sizes[0]), 12), np.uint32)
+    items = np.zeros(int(sizes[1]), np.uint32)
+    L.oracle_mesh_hierarchy(_p(tris9), _p(counts), len(counts), _p(k), _p(sizes), _p(nodes), _p(items))
+    return nodes, items
+
+
+def block_planes(texels: np.ndarray, blocks: np.ndarray, light: np.ndarray) -> np.ndarray:
+    texels = np.ascontiguousarray(texels, dtype=np.float32).reshape(-1, 64, 8)
+    blocks = np.ascontiguousarray(blocks, dtype=np.float32).reshape(-1, 6)
+    light = np.ascontiguousarray(light, dtype=np.float32).reshape(16)
+    out = np.zeros((len(blocks), 16, 4), np.float32)
+    lib().oracle_block_planes(_p(texels), _p(blocks), len(blocks), _p(light), _p(out))
+    return out
+
+
+def mark_blocks(planes: np.ndarray, swept: np.ndarray) -> np.ndarray:
+    """Indices of the blocks no plane separates from some swept box
+    ((k, 2, 3) lo/hi)."""
+    planes = np.ascontiguousarray(planes, dtype=np.float32).reshape(-1, 16, 4)
+    sw = np.asarray(swept, dtype=np.float32).reshape(-1, 2, 3)
+    boxes = np.zeros((len(sw), 8), np.float32)
+    boxes[:, 0:3] = sw[:, 0]
+    boxes[:, 4:7] = sw[:, 1]
+    marked = np.zeros(len(planes), np.uint8)
+    lib().oracle_mark_blocks(_p(planes), len(planes), _p(boxes), len(boxes), _p(marked))
+    return np.nonzero(marked)[0].astype(np.uint32)
+
+
+def block_slots(planes: np.ndarray) -> np.ndarray:
+    """live.rs:576-589: a slot per block that can receive light."""
+    planes = np.asarray(planes, dtype=np.float32).reshape(-1, 16, 4)
+    never = np.all(planes[:, 0] == np.array([0, 0, 0, -1], np.float32), axis=1)
+    slots = np.full(len(planes), NONE, np.uint32)
+    slots[~never] = np.arange(int((~never).sum()), dtype=np.uint32)
+    return slots
+
+
+def bake_static(flat, static_nodes, static_items, texels, slots, light, opts, threads=None) -> np.ndarray:
+    """(n_slots, 64, 64) static coverage, the scene walked through the static
+    meshes' hierarchy."""
+    texels = np.ascontiguousarray(texels, dtype=np.float32).reshape(-1, 8)
+    slots = np.ascontiguousarray(slots, dtype=np.uint32)
+    n_slots = int((slots != NONE).sum())
+    cov = np.zeros((max(n_slots, 1), 64, 64), np.float32)
+    lt = np.ascontiguousarray(light, dtype=np.float32).reshape(16)
+    xf = flat.xforms if len(flat.xforms) else np.zeros((1, 16), np.float32)
+    sn = np.ascontiguousarray(static_nodes, dtype=np.uint32)
+    si = np.ascontiguousarray(static_items, dtype=np.uint32)
+    lib().oracle_bake_static(_p(flat.tris), _p(flat.ranges), len(flat.ranges), _p(flat.blas), _p(flat.blas_items),
+                             _p(flat.blas_cov), _p(sn), _p(si), _p(xf), _p(texels), _p(slots), len(slots), _p(lt),
+                             _p(opts), _p(cov), threads or threads_default())
+    return cov
+
+
+def rebake(flat, texels, slots, static_cov, blocks, light, moving, opts, threads=None) -> np.ndarray:
+    """Irradiance of every texel of `blocks` (others 0) from the static
+    coverage plus `moving` = [(mesh, lo, hi)] (padded bounds)."""
+    texels = np.ascontiguousarray(texels, dtype=np.float32).reshape(-1, 8)
+    slots = np.ascontiguousarray(slots, dtype=np.uint32)
+    blocks = np.ascontiguousarray(blocks, dtype=np.uint32)
+    mv = np.ascontiguousarray([m for m, _, _ in moving], dtype=np.uint32)
+    boxes = np.zeros((max(len(moving), 1), 8), np.float32)
+    for k, (_, lo, hi) in enumerate(moving):
+        boxes[k, 0:3] = lo
+        boxes[k, 4:7] = hi
+    out = np.zeros(len(texels), np.float32)
+    lt = np.ascontiguousarray(light, dtype=np.float32).reshape(16)
+    xf = flat.xforms if len(flat.xforms) else np.zeros((1, 16), np.float32)
+    sc = np.ascontiguousarray(static_cov, dtype=np.float32)
+    lib().oracle_rebake(_p(flat.tris), _p(flat.ranges), len(flat.ranges), _p(flat.blas), _p(flat.blas_items),
+                        _p(flat.blas_cov), _p(xf), _p(texels), _p(slots), _p(sc), _p(blocks), len(blocks), _p(lt),
+                        _p(mv), _p(boxes), len(moving), _p(opts), _p(out), threads or threads_default())
+    return out

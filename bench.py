@@ -51,7 +51,7 @@ def read_traffic(workload: str):
     """HBM bytes per cone launch measured by rocprofv3 --pmc (FETCH_SIZE and
     WRITE_SIZE, calibrated on a copy of known size), from the committed
     summary, if one exists for this workload."""
-    f = PROFILE_DIR / f"cone_query_{workload}_traffic.md"
+    f = PROFILE_DIR / f"cone_items_{workload}_traffic.md"
     if not f.exists():
         return None
     for line in f.read_text().splitlines():

@@ -473,6 +473,17 @@ class MaskConeKernel:
         (vis, _, _, _), best = self._run(receivers, light, meshes, runs, hierarchy)
         return vis, best
 
+    def bake(self, lightmap, light, meshes) -> float:
+        """One cone per lightmap texel toward `light` through the meshes'
+        hierarchy, into `lightmap.irradiance`; returns the time (lib.rs:2961)."""
+        scene = self.scene(meshes)
+        best = float("inf")
+        t0 = time.perf_counter()
+        vis, _, _, _ = scene.query(lightmap.texels, self._light_row(light)[None, :], self.options(Hierarchy.TRIANGLES))
+        best = time.perf_counter() - t0
+        lightmap.irradiance = vis
+        return best
+
     def coverage(self, receivers, light, meshes) -> np.ndarray:
         """Per receiver, the 64 bins' coverage (lib.rs:3289; every triangle tested)."""
         (_, _, _, cov), _ = self._run(receivers, light, meshes, 1, Hierarchy.NONE, want_cov=True)

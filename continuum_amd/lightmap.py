@@ -12,7 +12,7 @@ from __future__ import annotations
 
 import math
 from dataclasses import dataclass
-from typing import Callable, List, Sequence
+from typing import Callable, List, Optional, Sequence
 
 import numpy as np
 
@@ -132,6 +132,7 @@ class Lightmap:
     texels: np.ndarray  # (n_blocks * 64, 8)
     blocks: np.ndarray  # (n_blocks, 6): lo, hi
     charts: List[tuple]  # (mesh, first texel, nu, nv, blocks per row)
+    irradiance: Optional[np.ndarray] = None  # per texel, once baked
 
     @property
     def n_blocks(self) -> int:

@@ -55,6 +55,21 @@ class Tuning(C.Structure):
     _fields_ = [("tile_w", C.c_uint32), ("persistent_blocks", C.c_uint32)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("pixel_capacity", C.c_uint64), ("query_capacity", C.c_uint64), ("max_lights", C.c_uint32),
+                ("max_view_lights", C.c_uint32), ("max_views_per_launch", C.c_uint32), ("slots", C.c_uint32),
+                ("tuning", Tuning)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("state", C.c_int32), ("width", C.c_uint32), ("height", C.c_uint32), ("n_lights", C.c_uint32),
+                ("launch", C.c_uint64), ("queued_ms", C.c_double), ("latency_ms", C.c_double),
+                ("admitted_at_ms", C.c_double), ("done_at_ms", C.c_double)]
+
+
+REQ_QUEUED, REQ_ADMITTED, REQ_DONE, REQ_RELEASED = range(4)
+
+
 # Every symbol include/cnq.h declares, with its ctypes signature.
 _P = C.c_void_p
 SIGNATURES = {
@@ -93,6 +108,16 @@ SIGNATURES = {
     "cnq_scene_read_blas": (C.c_int, [_P, _P, _P]),
     "cnq_scene_counters": (C.c_int, [_P, C.c_int, _P, _P]),
     "cnq_scene_set_count_work": (C.c_int, [_P, C.c_int]),
+    "cnq_server_config_default": (None, [C.POINTER(ServerConfig)]),
+    "cnq_server_create": (C.c_int, [_P, C.POINTER(ServerConfig), C.POINTER(Options), _P, C.c_uint32, C.POINTER(_P)]),
+    "cnq_server_destroy": (None, [_P]),
+    "cnq_server_submit_view": (C.c_int, [_P, _P, C.c_uint32, C.c_uint32, _P, C.c_uint32, C.POINTER(C.c_uint64)]),
+    "cnq_server_submit_light": (C.c_int, [_P, C.c_uint32, _P, C.POINTER(C.c_uint64)]),
+    "cnq_server_wait": (C.c_int, [_P, C.c_uint64, C.c_double, C.POINTER(C.c_int)]),
+    "cnq_server_info": (C.c_int, [_P, C.c_uint64, C.POINTER(RequestInfo)]),
+    "cnq_server_read": (C.c_int, [_P, C.c_uint64, _P, _P, _P, _P]),
+    "cnq_server_release": (C.c_int, [_P, C.c_uint64]),
+    "cnq_server_counts": (C.c_int, [_P, _P]),
     "cnq_block_planes": (C.c_int, [_P, _P, C.c_uint32, _P, _P]),
     "cnq_cache_create": (C.c_int, [_P, _P, _P, C.c_uint32, _P, _P, _P, C.POINTER(Options), C.POINTER(_P)]),
     "cnq_cache_destroy": (None, [_P]),

@@ -12,12 +12,13 @@
 #include "../../include/cnq.h"
 #include "bvh_build.h"
 #include "cache_host.h"
-#include "cnq_kernels.h"
+#include "cnq_internal.h"
 
 using cnq::NodeC;
 using cnq::NONE;
 
-namespace {
+namespace cnq {
+namespace detail {
 
 thread_local std::string g_last_error;
 
@@ -30,33 +31,7 @@ int hip_fail(hipError_t e, const char* what) {
   return fail(CNQ_HIP_ERROR, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-#define HIP_TRY(expr, what)              \
-  do {                                   \
-    hipError_t _e = (expr);              \
-    if (_e != hipSuccess) return hip_fail(_e, what); \
-  } while (0)
 
-template <class T>
-struct DevBuf {
-  T* p = nullptr;
-  size_t n = 0;
-  hipError_t alloc(size_t count) {
-    n = count;
-    return hipMalloc((void**)&p, (count ? count : 1) * sizeof(T));
-  }
-  hipError_t upload(const T* src, size_t count) {
-    hipError_t e = alloc(count);
-    if (e != hipSuccess) return e;
-    if (count) e = hipMemcpy(p, src, count * sizeof(T), hipMemcpyHostToDevice);
-    return e;
-  }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    n = 0;
-  }
-  size_t bytes() const { return (n ? n : 1) * sizeof(T); }
-};
 
 __global__ void fill_u32_kernel(uint32_t* p, uint32_t v, uint64_t n) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -108,56 +83,14 @@ __global__ __launch_bounds__(256) void mesh_bounds_kernel(const float* __restric
     for (int k = 0; k < 6; k++) out[6 * m + k] = s[k][0];
 }
 
-}  // namespace
+}  // namespace detail
+}  // namespace cnq
 
-struct cnq_device {
-  int ordinal;
-};
+using namespace cnq::detail;
 
-struct cnq_scene {
-  cnq_device* dev;
-  uint64_t n_tris;
-  uint32_t n_objects, n_blas, n_nodes, n_items, n_xforms;
-  uint64_t n_blas_items;
-  bool count_work = false;
-  DevBuf<float> tris, tri_leaf, blas_cov, xforms, mesh_box;
-  DevBuf<uint32_t> blas_items, ranges, items, parent, leaves, arrivals, counters;
-  DevBuf<uint64_t> work;
-  DevBuf<NodeC> blas, nodes;
-  std::vector<uint32_t> h_ranges;  // for host-side checks and object rebuilds
-  bool instanced = false;
 
-  cnq::SceneArgs args() const {
-    cnq::SceneArgs s;
-    s.tris = tris.p;
-    s.tri_leaf = tri_leaf.p;
-    s.blas_items = blas_items.p;
-    s.ranges = ranges.p;
-    s.blas = blas.p;
-    s.blas_cov = blas_cov.p;
-    s.nodes = nodes.p;
-    s.items = items.p;
-    s.xforms = xforms.p;
-    s.n_objects = n_objects;
-    return s;
-  }
-  void release() {
-    tris.release(), tri_leaf.release(), blas_cov.release(), xforms.release(), mesh_box.release();
-    blas_items.release(), ranges.release(), items.release(), parent.release(), leaves.release();
-    arrivals.release(), counters.release(), work.release(), blas.release(), nodes.release();
-  }
-  uint64_t bytes() const {
-    return tris.bytes() + tri_leaf.bytes() + blas_cov.bytes() + xforms.bytes() + mesh_box.bytes() +
-           blas_items.bytes() + ranges.bytes() + items.bytes() + parent.bytes() + leaves.bytes() + arrivals.bytes() +
-           counters.bytes() + work.bytes() + blas.bytes() + nodes.bytes();
-  }
-};
-
-struct cnq_flat {
-  cnq::host::Flat f;
-};
-
-namespace {
+namespace cnq {
+namespace detail {
 
 NodeC compact(const uint32_t* g, bool object_level) {
   NodeC c;
@@ -288,7 +221,8 @@ int check_object_cov(cnq_scene* s, const cnq_options* o) {
   return CNQ_OK;
 }
 
-}  // namespace
+}  // namespace detail
+}  // namespace cnq
 
 extern "C" {
 

@@ -1,0 +1,112 @@
+// Internal to libcnq.so: the handles' definitions and the helpers the
+// API's translation units share (status reporting, device buffers, launch
+// set-up). Not part of the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/cnq.h"
+#include "bvh_build.h"
+#include "cnq_kernels.h"
+
+namespace cnq {
+namespace detail {
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  hipError_t alloc(size_t count) {
+    n = count;
+    return hipMalloc((void**)&p, (count ? count : 1) * sizeof(T));
+  }
+  hipError_t upload(const T* src, size_t count) {
+    hipError_t e = alloc(count);
+    if (e != hipSuccess) return e;
+    if (count) e = hipMemcpy(p, src, count * sizeof(T), hipMemcpyHostToDevice);
+    return e;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  size_t bytes() const { return (n ? n : 1) * sizeof(T); }
+};
+
+// Records `msg` as the calling thread's last error; returns `code`.
+int fail(int code, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+#define HIP_TRY(expr, what)              \
+  do {                                   \
+    hipError_t _e = (expr);              \
+    if (_e != hipSuccess) return hip_fail(_e, what); \
+  } while (0)
+
+hipError_t fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s);
+int check_options(const cnq_options* o);
+uint64_t layout_view(ViewDesc& v, uint32_t tile_w, uint64_t item_off);
+
+}  // namespace detail
+}  // namespace cnq
+
+struct cnq_device {
+  int ordinal;
+};
+
+struct cnq_scene {
+  cnq_device* dev;
+  uint64_t n_tris;
+  uint32_t n_objects, n_blas, n_nodes, n_items, n_xforms;
+  uint64_t n_blas_items;
+  bool count_work = false;
+  cnq::detail::DevBuf<float> tris, tri_leaf, blas_cov, xforms, mesh_box;
+  cnq::detail::DevBuf<uint32_t> blas_items, ranges, items, parent, leaves, arrivals, counters;
+  cnq::detail::DevBuf<uint64_t> work;
+  cnq::detail::DevBuf<cnq::NodeC> blas, nodes;
+  std::vector<uint32_t> h_ranges;  // for host-side checks and object rebuilds
+  bool instanced = false;
+
+  cnq::SceneArgs args() const {
+    cnq::SceneArgs s;
+    s.tris = tris.p;
+    s.tri_leaf = tri_leaf.p;
+    s.blas_items = blas_items.p;
+    s.ranges = ranges.p;
+    s.blas = blas.p;
+    s.blas_cov = blas_cov.p;
+    s.nodes = nodes.p;
+    s.items = items.p;
+    s.xforms = xforms.p;
+    s.n_objects = n_objects;
+    return s;
+  }
+  void release() {
+    tris.release(), tri_leaf.release(), blas_cov.release(), xforms.release(), mesh_box.release();
+    blas_items.release(), ranges.release(), items.release(), parent.release(), leaves.release();
+    arrivals.release(), counters.release(), work.release(), blas.release(), nodes.release();
+  }
+  uint64_t bytes() const {
+    return tris.bytes() + tri_leaf.bytes() + blas_cov.bytes() + xforms.bytes() + mesh_box.bytes() +
+           blas_items.bytes() + ranges.bytes() + items.bytes() + parent.bytes() + leaves.bytes() + arrivals.bytes() +
+           counters.bytes() + work.bytes() + blas.bytes() + nodes.bytes();
+  }
+};
+
+struct cnq_flat {
+  cnq::host::Flat f;
+};
+
+namespace cnq {
+namespace detail {
+
+QueryParams query_params(cnq_scene* s, const cnq_options* o, uint32_t tile_w);
+ItemArgs item_args(cnq_scene* s, const float* receivers, const float* lights, float* vis, uint64_t* full,
+                   uint64_t* touched, float* cov);
+hipError_t launch_items_direct(ItemArgs a, uint64_t n_items, bool count_work, hipStream_t stream);
+
+}  // namespace detail
+}  // namespace cnq

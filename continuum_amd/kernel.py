@@ -295,6 +295,83 @@ class Batch:
         check(lib().cnq_batch_set_lights(self.handle, _ptr(self._lights), len(self._lights)))
 
 
+class Server:
+    """The served form over one resident scene: view requests and light
+    edits admitted in arrival order, batched per launch, launches in flight
+    on several streams; see `cnq_server_*` in include/cnq.h."""
+
+    def __init__(self, scene: Scene, lights: np.ndarray, opts: Optional[_lib.Options] = None, **config):
+        self.scene = scene
+        cfg = _lib.ServerConfig()
+        lib().cnq_server_config_default(C.byref(cfg))
+        for k, v in config.items():
+            if k in ("tile_w", "persistent_blocks"):
+                setattr(cfg.tuning, k, int(v))
+            else:
+                setattr(cfg, k, int(v))
+        self.config = cfg
+        self.opts = opts or default_options(hierarchy=2)
+        lt = np.ascontiguousarray(lights, dtype=np.float32).reshape(-1, 16)
+        h = C.c_void_p()
+        check(lib().cnq_server_create(scene.handle, C.byref(cfg), C.byref(self.opts), _ptr(lt), len(lt), C.byref(h)))
+        self.handle = h
+        self._destroy = lib().cnq_server_destroy
+        self._shapes = {}
+
+    def close(self) -> None:
+        if getattr(self, "handle", None):
+            self._destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()
+
+    def submit_view(self, camera, width: int, height: int, lights: Sequence[int]) -> int:
+        cam = np.ascontiguousarray(camera, dtype=np.float32).reshape(20)
+        idx = np.ascontiguousarray(list(lights), dtype=np.uint32)
+        rid = C.c_uint64()
+        check(lib().cnq_server_submit_view(self.handle, _ptr(cam), int(width), int(height), _ptr(idx), len(idx),
+                                           C.byref(rid)))
+        self._shapes[rid.value] = (int(width), int(height), len(idx))
+        return rid.value
+
+    def submit_light(self, index: int, light: np.ndarray) -> int:
+        lt = np.ascontiguousarray(light, dtype=np.float32).reshape(16)
+        rid = C.c_uint64()
+        check(lib().cnq_server_submit_light(self.handle, int(index), _ptr(lt), C.byref(rid)))
+        return rid.value
+
+    def wait(self, rid: int, timeout_ms: float = 60_000.0) -> bool:
+        done = C.c_int()
+        check(lib().cnq_server_wait(self.handle, int(rid), float(timeout_ms), C.byref(done)))
+        return bool(done.value)
+
+    def info(self, rid: int) -> _lib.RequestInfo:
+        out = _lib.RequestInfo()
+        check(lib().cnq_server_info(self.handle, int(rid), C.byref(out)))
+        return out
+
+    def read(self, rid: int):
+        """(hits (px, 4), visible, full, touched) of a done view, queries light-major."""
+        w, h, nl = self._shapes[rid]
+        px, q = w * h, w * h * nl
+        hits = np.zeros((px, 4), np.uint32)
+        vis = np.zeros(q, np.float32)
+        full = np.zeros(q, np.uint64)
+        touched = np.zeros(q, np.uint64)
+        check(lib().cnq_server_read(self.handle, int(rid), _ptr(hits), _ptr(vis), _ptr(full), _ptr(touched)))
+        return hits, vis, full, touched
+
+    def release(self, rid: int) -> None:
+        check(lib().cnq_server_release(self.handle, int(rid)))
+        self._shapes.pop(rid, None)
+
+    def counts(self) -> dict:
+        c = np.zeros(6, np.uint64)
+        check(lib().cnq_server_counts(self.handle, _ptr(c)))
+        return dict(zip(("views", "edits", "launches", "queries", "pixels_held", "queries_held"), (int(x) for x in c)))
+
+
 def block_planes(texels: np.ndarray, blocks: np.ndarray, light: np.ndarray) -> np.ndarray:
     """Lightmap::block_planes (lightmap.rs:256-333): (n_blocks, 16, 4) planes
     (n, w; inside n.p <= w) per block of 64 texels toward a rect light."""

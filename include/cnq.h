@@ -206,6 +206,62 @@ int cnq_batch_read(cnq_batch* batch, uint32_t view, uint32_t* hits, float* out_v
 /* Replace the batch's lights (same count): a light edit. */
 int cnq_batch_set_lights(cnq_batch* batch, const float* lights, uint32_t n_lights);
 
+/* ---- Serving: a request loop over one resident scene (no counterpart in
+ * the reference, which renders on the caller's thread; this is the
+ * continuous-batching server its Renderer would sit behind). View requests
+ * (a camera, a size, and 1..max_view_lights indices into the server's light
+ * table) and light edits are admitted in arrival order: a view is admitted
+ * when its results fit in the free HBM (48 B per pixel for shading points
+ * and hits, 20 B per query) and a launch slot is free; admitted views go out
+ * together as one camera pass and one cone launch on the slot's stream, and
+ * launches on different slots overlap on the device. A light edit applies
+ * to the views admitted after it. A view's memory stays held until
+ * cnq_server_release. Submit, wait, read and release may be called from any
+ * host thread. */
+typedef struct cnq_server cnq_server;
+
+typedef struct cnq_server_config {
+  uint64_t pixel_capacity;       /* shading points of admitted views held at once */
+  uint64_t query_capacity;       /* queries of admitted views held at once */
+  uint32_t max_lights;           /* size of the light table */
+  uint32_t max_view_lights;      /* lights per view */
+  uint32_t max_views_per_launch; /* views batched into one launch */
+  uint32_t slots;                /* launches in flight (one stream each) */
+  cnq_tuning tuning;             /* cone launch shape */
+} cnq_server_config;
+
+enum { CNQ_REQ_QUEUED = 0, CNQ_REQ_ADMITTED = 1, CNQ_REQ_DONE = 2, CNQ_REQ_RELEASED = 3 };
+
+typedef struct cnq_request_info {
+  int32_t state;
+  uint32_t width, height, n_lights;
+  uint64_t launch;       /* the launch that carried it */
+  double queued_ms;      /* submit to admission */
+  double latency_ms;     /* submit to results readable */
+  double admitted_at_ms; /* admission and completion, from the server's creation */
+  double done_at_ms;
+} cnq_request_info;
+
+void cnq_server_config_default(cnq_server_config* cfg);
+int cnq_server_create(cnq_scene* scene, const cnq_server_config* cfg, const cnq_options* opts, const float* lights,
+                      uint32_t n_lights, cnq_server** out);
+/* Waits for in-flight launches, drops queued requests, frees everything. */
+void cnq_server_destroy(cnq_server* server);
+int cnq_server_submit_view(cnq_server* server, const float* camera, uint32_t width, uint32_t height,
+                           const uint32_t* lights, uint32_t n_lights, uint64_t* id);
+int cnq_server_submit_light(cnq_server* server, uint32_t index, const float* light, uint64_t* id);
+/* Block until request `id` is done or `timeout_ms` passes (*done = 0). */
+int cnq_server_wait(cnq_server* server, uint64_t id, double timeout_ms, int* done);
+int cnq_server_info(cnq_server* server, uint64_t id, cnq_request_info* info);
+/* A done view's hits (4 u32 per pixel) and queries (light-major, as
+ * cnq_render_view); any pointer may be NULL. */
+int cnq_server_read(cnq_server* server, uint64_t id, uint32_t* hits, float* out_visible, uint64_t* out_full,
+                    uint64_t* out_touched);
+/* Free a done request's memory (admits more); its results are gone. */
+int cnq_server_release(cnq_server* server, uint64_t id);
+/* [views done, edits done, launches, queries done, pixels held, queries held] */
+int cnq_server_counts(cnq_server* server, uint64_t counts[6]);
+
 /* ---- The lighting cache: a lightmap's direct irradiance from one rect
  * light, kept current as meshes move (the reference's `LiveRenderer`,
  * crates/continuum-gpu/src/live.rs, over a `Lightmap`, lightmap.rs).

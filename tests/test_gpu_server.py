@@ -1,0 +1,141 @@
+"""GPU: the served form. Each view's results are the same alone and in a mix
+of concurrent requests (bit for bit, against its own plain launch);
+light edits apply to the views that arrive after them; admission waits for
+HBM that released views free; launches on different streams overlap."""
+import threading
+
+import numpy as np
+import pytest
+
+from continuum_amd import _lib
+from continuum_amd._lib import CnqError
+from continuum_amd.kernel import Flat, Scene, Server
+from continuum_amd.scenes import camera_uniform, meshes_to_arrays, rect_light, room_scene, sphere_light
+
+pytestmark = pytest.mark.gpu
+
+
+def _room(device):
+    sc = room_scene()
+    tris9, counts = meshes_to_arrays(sc.meshes)
+    scene = Scene(device, Flat.build_arrays(tris9, counts))
+    lights = np.stack([sc.lights[0], rect_light([0.5, 0.5, 2.5], [0.6, 0, 0], [0, 0.4, 0], one_sided=False),
+                       sphere_light([3.2, 1.0, 2.4], 0.15), rect_light([3.0, 3.0, 2.99], [0.3, 0, 0], [0, -0.3, 0], True)])
+    return sc, scene, lights
+
+
+def _views(n, seed=5):
+    rng = np.random.default_rng(seed)
+    sizes = [(64, 64), (40, 23), (96, 54), (8, 8), (17, 90), (130, 7), (33, 33)]
+    out = []
+    for k in range(n):
+        w, h = sizes[k % len(sizes)]
+        eye = rng.uniform([0.3, 0.3, 0.5], [3.7, 3.7, 2.5])
+        tgt = rng.uniform([0.3, 0.3, 0.0], [3.7, 3.7, 2.0])
+        idx = sorted(rng.choice(4, size=1 + k % 4, replace=False).tolist())
+        out.append((camera_uniform(eye, tgt, 55.0, w, h), w, h, idx))
+    return out
+
+
+def _same(a, b):
+    for x, y in zip(a, b):
+        assert np.array_equal(np.ascontiguousarray(x).view(np.uint32), np.ascontiguousarray(y).view(np.uint32))
+
+
+def test_alone_equals_in_the_mix_with_light_edits(device):
+    sc, scene, lights = _room(device)
+    opts = _lib.default_options(hierarchy=2)
+    views = _views(24)
+    srv = Server(scene, lights, opts, max_views_per_launch=5, slots=2)
+    # alone: one view, waited for before the next
+    alone = []
+    for cam, w, h, idx in views[:3]:
+        rid = srv.submit_view(cam, w, h, idx)
+        assert srv.wait(rid)
+        alone.append(srv.read(rid))
+        srv.release(rid)
+    for (cam, w, h, idx), got in zip(views[:3], alone):
+        _same(got, scene.render_view(cam, w, h, lights[idx], opts))
+    # the mix: two client threads submit at once, views and light edits
+    # interleaved; a view sees the light table as of its submission (request
+    # ids number submissions), whatever launch carries it
+    edits = [(1, rect_light([0.7, 0.4, 2.6], [0.5, 0, 0], [0, 0.5, 0], one_sided=False)),
+             (3, rect_light([2.8, 2.9, 2.99], [0.4, 0, 0], [0, -0.4, 0], True)),
+             (1, lights[1])]
+    parts = [[("view", v) for v in views[0::2]] + [("edit", *edits[0]), ("edit", *edits[2])],
+             [("view", v) for v in views[1::2]] + [("edit", *edits[1])]]
+    for p in parts:
+        np.random.default_rng(len(p)).shuffle(p)
+    ids = []
+    lock = threading.Lock()
+
+    def client(part):
+        for item in part:
+            rid = srv.submit_light(item[1], item[2]) if item[0] == "edit" else srv.submit_view(*item[1])
+            with lock:
+                ids.append((rid, item))
+
+    threads = [threading.Thread(target=client, args=(p,)) for p in parts]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    for rid, _ in ids:
+        assert srv.wait(rid)
+    table = lights.copy()
+    launches = set()
+    for rid, item in sorted(ids, key=lambda x: x[0]):
+        if item[0] == "edit":
+            table = table.copy()
+            table[item[1]] = item[2]
+            continue
+        cam, w, h, idx = item[1]
+        _same(srv.read(rid), scene.render_view(cam, w, h, table[idx], opts))
+        launches.add(srv.info(rid).launch)
+        srv.release(rid)
+    assert len(launches) > 1
+    c = srv.counts()
+    assert c["views"] == 24 + 3 and c["edits"] == 3 and c["pixels_held"] == 0 and c["queries_held"] == 0
+    srv.close()
+
+
+def test_admission_waits_for_memory(device):
+    sc, scene, lights = _room(device)
+    opts = _lib.default_options(hierarchy=2)
+    cam = camera_uniform([0.4, 0.4, 1.6], [3.6, 3.6, 0.8], 60.0, 64, 64)
+    # room for two 64 x 64 views with two lights, not three
+    srv = Server(scene, lights, opts, pixel_capacity=2 * 64 * 64 + 100, query_capacity=4 * 64 * 64 + 100,
+                 max_views_per_launch=8, slots=2)
+    with pytest.raises(CnqError):
+        srv.submit_view(cam, 128, 128, [0])      # can never fit
+    ids = [srv.submit_view(cam, 64, 64, [0, 1]) for _ in range(3)]
+    assert srv.wait(ids[0]) and srv.wait(ids[1])
+    assert not srv.wait(ids[2], timeout_ms=300)
+    assert srv.info(ids[2]).state == _lib.REQ_QUEUED
+    first = srv.read(ids[0])
+    srv.release(ids[0])
+    assert srv.wait(ids[2])
+    _same(srv.read(ids[2]), first)
+    _same(first, scene.render_view(cam, 64, 64, lights[[0, 1]], opts))
+    srv.close()
+
+
+def test_launches_overlap_on_the_device(device):
+    sc, scene, lights = _room(device)
+    opts = _lib.default_options(hierarchy=2)
+    cam = camera_uniform([0.4, 0.4, 1.6], [3.6, 3.6, 0.8], 60.0, 256, 256)
+    srv = Server(scene, lights, opts, max_views_per_launch=1, slots=3)
+    ids = [srv.submit_view(cam, 256, 256, [0, 1, 2, 3]) for _ in range(9)]
+    for r in ids:
+        assert srv.wait(r)
+    info = [srv.info(r) for r in ids]
+    assert len({i.launch for i in info}) == 9
+    # some launch was admitted while an earlier one was still on the device
+    overlapped = any(info[j].admitted_at_ms < info[i].done_at_ms
+                     for i in range(len(info)) for j in range(i + 1, len(info)))
+    assert overlapped
+    ref = scene.render_view(cam, 256, 256, lights, opts)
+    for r in ids:
+        _same(srv.read(r), ref)
+        srv.release(r)
+    srv.close()

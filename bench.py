@@ -168,6 +168,127 @@ def courtyard_forms(device, steps: int) -> dict:
     return out
 
 
+def served_form(device, n_requests: int, seconds: float) -> dict:
+    """The served form over the 1M-instance boulder field (10 boulder shapes,
+    a 32-light table): a seeded Poisson stream of view requests (a random
+    size from 256^2 to 1920x1080, 1-32 lights) and light edits goes through
+    the continuous-batching server. Run 1 offers the stream faster than the
+    GPU drains it (throughput); run 2 offers it at half that throughput
+    (latency under load). Views of run 2 are also rendered one by one as
+    plain per-view launches, for the plain rate on the same work."""
+    import threading
+
+    import numpy as np
+    import torch
+
+    from continuum_amd import _lib
+    from continuum_amd.kernel import Flat, Scene, Server
+    from continuum_amd.scenes import boulder_field
+    from continuum_amd.workloads import serving_requests
+
+    t_build = time.perf_counter()
+    tris9, counts, pl, lights, _, _ = boulder_field()
+    flat = Flat.build_instanced(tris9, counts, pl, object_prefilter=True)
+    scene = Scene(device, flat)
+    t_build = time.perf_counter() - t_build
+    opts = _lib.default_options(hierarchy=2, prefilter=1.0, object_cov=flat.object_cov)
+
+    def run(reqs):
+        srv = Server(scene, lights, opts, pixel_capacity=40 << 20, query_capacity=400 << 20,
+                     max_lights=len(lights), max_views_per_launch=16, slots=2)
+        first_view = next(p for _, k, p in reqs if k == "view")
+        warm = srv.submit_view(first_view[0], 64, 64, [0])
+        srv.wait(warm)
+        srv.release(warm)
+        ids, done = [], []
+        lock = threading.Condition()
+
+        def collector():
+            k = 0
+            while True:
+                with lock:
+                    while k >= len(ids):
+                        lock.wait()
+                    rid, kind = ids[k]
+                if rid is None:
+                    return
+                k += 1
+                if kind != "view":
+                    continue
+                srv.wait(rid)
+                info = srv.info(rid)
+                done.append((info.latency_ms, info.queued_ms, info.width * info.height * info.n_lights,
+                             time.perf_counter()))
+                srv.release(rid)
+
+        th = threading.Thread(target=collector)
+        th.start()
+        t0 = time.perf_counter()
+        for t_arr, kind, payload in reqs:
+            delay = t0 + t_arr - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            rid = srv.submit_view(*payload) if kind == "view" else srv.submit_light(*payload)
+            with lock:
+                ids.append((rid, kind))
+                lock.notify()
+        with lock:
+            ids.append((None, None))
+            lock.notify()
+        th.join()
+        wall = max(d[3] for d in done) - t0
+        counts_ = srv.counts()
+        srv.close()
+        lat = np.array([d[0] for d in done])
+        q = sum(d[2] for d in done)
+        return {"requests": len(reqs), "views": len(done), "edits": counts_["edits"], "launches": counts_["launches"],
+                "queries": int(q), "s": wall, "queries_per_s": q / wall,
+                "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                "queued_p50_ms": float(np.percentile([d[1] for d in done], 50))}
+
+    # Run 1: everything offered within the first 10 ms.
+    burst = serving_requests(n_requests, n_requests / 0.01, lights)
+    r1 = run(burst)
+    views_per_s = r1["views"] / r1["s"]
+    # Run 2: Poisson at half the drained rate, for about `seconds`.
+    rate = 0.5 * views_per_s / 0.8
+    n2 = max(20, int(rate * seconds))
+    steady = serving_requests(n2, rate, lights, seed=0xC0_0008)
+    r2 = run(steady)
+
+    # The same views of run 2 as plain launches, one after another.
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    views = [p for _, k, p in steady if k == "view"]
+    table = np.array(lights, np.float32, copy=True)
+    maxpx = max(w * h for _, w, h, _ in views)
+    maxq = max(w * h * len(i) for _, w, h, i in views)
+    hits = torch.empty((maxpx, 4), dtype=torch.int32, device=dev)
+    recv = torch.empty((maxpx, 8), dtype=torch.float32, device=dev)
+    vis = torch.empty(maxq, dtype=torch.float32, device=dev)
+    full = torch.empty(maxq, dtype=torch.int64, device=dev)
+    touched = torch.empty(maxq, dtype=torch.int64, device=dev)
+    cams = [torch.from_numpy(np.asarray(c, np.float32)).to(dev) for c, _, _, _ in views]
+    lts = [torch.from_numpy(np.ascontiguousarray(table[i])).to(dev) for _, _, _, i in views]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    q = 0
+    for k, (_, w, h, i) in enumerate(views):
+        scene.primary_device(cams[k], w, h, lts[k], opts, hits, recv, stream)
+        scene.view_query_device(recv, w, h, lts[k], opts, vis, full, touched, tile_w=8, stream=stream)
+        q += w * h * len(i)
+    torch.cuda.synchronize()
+    t_plain = time.perf_counter() - t0
+    return {
+        "workload": "boulder field: 10 boulder shapes x 999,987 instances + ground (100,002 triangles, 1.05M object "
+                    "nodes), 32-light table; views 256^2..1920x1080 x 1-32 lights, 20% light edits",
+        "instances": int(len(pl)), "host_build_s": round(t_build, 2), "scene_hbm_bytes": scene.hbm_bytes,
+        "saturated": r1, "steady": dict(r2, offered_views_per_s=0.8 * rate),
+        "plain_same_views": {"queries_per_s": q / t_plain, "s": t_plain},
+        "served_over_plain": r1["queries_per_s"] / (q / t_plain),
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -179,6 +300,9 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-forms", action="store_true", help="skip the courtyard's plain/batched forms")
+    ap.add_argument("--no-served", action="store_true", help="skip the served form over the boulder field")
+    ap.add_argument("--served-requests", type=int, default=64, help="requests in the saturated served run")
+    ap.add_argument("--served-seconds", type=float, default=10.0, help="length of the steady served run")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -359,6 +483,9 @@ def main() -> int:
     if not args.no_forms:
         del scene
         result["forms"] = courtyard_forms(device, steps=5)
+    if not args.no_served:
+        result.setdefault("forms", {})["served"] = served_form(device, n_requests=args.served_requests,
+                                                               seconds=args.served_seconds)
 
     print(json.dumps(result))
     return 0

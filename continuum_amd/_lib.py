@@ -43,7 +43,13 @@ class SceneDesc(C.Structure):
         ("nodes", C.c_void_p), ("n_nodes", C.c_uint32),
         ("items", C.c_void_p), ("n_items", C.c_uint32),
         ("xforms", C.c_void_p), ("n_xforms", C.c_uint32),
+        ("n_blas_cov", C.c_uint32),
     ]
+
+
+class Placement(C.Structure):
+    _fields_ = [("geometry", C.c_uint32), ("rotation", C.c_float * 9), ("scale", C.c_float),
+                ("translation", C.c_float * 3), ("clear", C.c_float)]
 
 
 class View(C.Structure):
@@ -81,6 +87,9 @@ SIGNATURES = {
     "cnq_flat_sizes": (C.c_int, [_P, _P]),
     "cnq_flat_copy": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P]),
     "cnq_flat_destroy": (None, [_P]),
+    "cnq_flat_build_instanced": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_uint64, C.c_int, C.POINTER(_P)]),
+    "cnq_flat_instanced_sizes": (C.c_int, [_P, _P]),
+    "cnq_flat_copy_instanced": (C.c_int, [_P, _P, _P]),
     "cnq_scene_create": (C.c_int, [_P, C.POINTER(SceneDesc), C.POINTER(_P)]),
     "cnq_scene_destroy": (None, [_P]),
     "cnq_scene_bytes": (C.c_uint64, [_P]),

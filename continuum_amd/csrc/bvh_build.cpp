@@ -292,5 +292,157 @@ void flatten(const float* tris, const uint32_t* counts, uint32_t n_meshes, Flat&
   object_hierarchy(mesh_boxes, f.nodes, f.items);
 }
 
+namespace {
+
+// M * p with M row-major, summed as glam's Mat3 * Vec3 (column by column).
+void mat_vec(const float* m, const float* p, float* out) {
+  for (int i = 0; i < 3; i++) out[i] = m[3 * i] * p[0] + m[3 * i + 1] * p[1] + m[3 * i + 2] * p[2];
+}
+
+// M^T * p, summed the same way.
+void mat_t_vec(const float* m, const float* p, float* out) {
+  for (int j = 0; j < 3; j++) out[j] = m[j] * p[0] + m[3 + j] * p[1] + m[6 + j] * p[2];
+}
+
+// hierarchy.rs:620-653 `front_area`: the host's bilinear lookup (a + (b - a) t).
+float front_area(const float* cov, const float* v) {
+  const float a[3] = {std::fabs(v[0]), std::fabs(v[1]), std::fabs(v[2])};
+  int axis = 0;
+  if (a[1] > a[0] && a[1] >= a[2])
+    axis = 1;
+  else if (a[2] > a[0] && a[2] > a[1])
+    axis = 2;
+  float mainv, s, t;
+  if (axis == 0) {
+    mainv = v[0], s = v[1] / a[0], t = v[2] / a[0];
+  } else if (axis == 1) {
+    mainv = v[1], s = v[0] / a[1], t = v[2] / a[1];
+  } else {
+    mainv = v[2], s = v[0] / a[2], t = v[1] / a[2];
+  }
+  const int m = mainv >= 0.0f ? 1 : -1;
+  const int s0 = s < 0.0f ? -1 : 0, t0 = t < 0.0f ? -1 : 0;
+  const float fs = s - (float)s0, ft = t - (float)t0;
+  float f[4];
+  for (int c = 0; c < 4; c++) {
+    const int gs = s0 + (c & 1), gt = t0 + (c >> 1);
+    int i, j, k;
+    if (axis == 0)
+      i = m, j = gs, k = gt;
+    else if (axis == 1)
+      i = gs, j = m, k = gt;
+    else
+      i = gs, j = gt, k = m;
+    f[c] = cov[9 * (i + 1) + 3 * (j + 1) + (k + 1)];
+  }
+  auto lerp = [](float x, float y, float w) { return x + (y - x) * w; };
+  return lerp(lerp(f[0], f[1], fs), lerp(f[2], f[3], fs), ft);
+}
+
+}  // namespace
+
+void flatten_instanced(const float* tris9, const uint32_t* counts, uint32_t n_geometries, const Placement* objects,
+                       size_t n_objects, bool object_prefilter, Flat& f, std::vector<float>& xforms,
+                       uint32_t& object_cov) {
+  flatten(tris9, counts, n_geometries, f);
+  const size_t n_blas = f.blas.size();
+  // A blended geometry's coverage scales by its first object's opacity.
+  std::vector<std::pair<uint32_t, uint32_t>> roots;  // (root, geometry)
+  for (uint32_t g = 0; g < n_geometries; g++)
+    if (f.ranges[4 * g + 2] != NONE_U32) roots.push_back({f.ranges[4 * g + 2], g});
+  std::sort(roots.begin(), roots.end());
+  for (size_t k = 0; k < roots.size(); k++) {
+    const uint32_t root = roots[k].first, g = roots[k].second;
+    const Placement* o = nullptr;
+    for (size_t i = 0; i < n_objects && !o; i++)
+      if (objects[i].geometry == g) o = &objects[i];
+    if (!o || o->clear == 0.0f) continue;
+    const uint32_t end = k + 1 < roots.size() ? roots[k + 1].first : (uint32_t)n_blas;
+    float opacity;
+    if (o->clear > 0.0f) {
+      opacity = 1.0f - o->clear;
+    } else if (o->clear <= -2.0f) {
+      const float x = 0.96f * (-2.0f - o->clear);
+      opacity = 1.0f - x * x;
+    } else {
+      opacity = 1.0f - 0.96f * -o->clear;
+    }
+    for (size_t c = 27 * (size_t)root; c < 27 * (size_t)end; c++) f.blas_cov[c] *= opacity;
+  }
+  // Per object: its geometry's range with a transform, and its world bounds.
+  const std::vector<uint32_t> geometry_ranges = f.ranges;
+  std::vector<Box> local(n_geometries);
+  size_t off = 0;
+  for (uint32_t g = 0; g < n_geometries; g++) {
+    local[g] = tri_run_bounds(tris9 + 9 * off, counts[g]);
+    off += counts[g];
+  }
+  f.ranges.clear();
+  xforms.clear();
+  std::vector<Box> bounds(n_objects);
+  for (size_t k = 0; k < n_objects; k++) {
+    const Placement& o = objects[k];
+    const uint32_t* gr = &geometry_ranges[4 * (size_t)o.geometry];
+    f.ranges.insert(f.ranges.end(), {gr[0], gr[1], gr[2], (uint32_t)k});
+    const float* r = o.rotation;
+    xforms.insert(xforms.end(), {r[0], r[1], r[2], o.translation[0], r[3], r[4], r[5], o.translation[1], r[6], r[7],
+                                 r[8], o.translation[2], o.scale, 1.0f / o.scale, o.clear, 0.0f});
+    Box w;
+    for (int a = 0; a < 3; a++) {
+      w.lo[a] = FMAX;
+      w.hi[a] = -FMAX;
+    }
+    const Box& lb = local[o.geometry];
+    for (int c = 0; c < 8; c++) {
+      const float p[3] = {(c & 1) ? lb.hi[0] : lb.lo[0], (c & 2) ? lb.hi[1] : lb.lo[1], (c & 4) ? lb.hi[2] : lb.lo[2]};
+      float q[3];
+      mat_vec(r, p, q);
+      for (int a = 0; a < 3; a++) {
+        const float x = q[a] * o.scale + o.translation[a];
+        w.lo[a] = gmin(w.lo[a], x);
+        w.hi[a] = gmax(w.hi[a], x);
+      }
+    }
+    bounds[k] = padded(w);
+  }
+  if (f.ranges.empty()) f.ranges.insert(f.ranges.end(), {0, 0, NONE_U32, NONE_U32});
+  if (xforms.empty()) xforms.assign(16, 0.0f);
+  object_hierarchy(bounds, f.nodes, f.items);
+  object_cov = NONE_U32;
+  if (!object_prefilter) return;
+  // Each object's 27 samples from its geometry's root coverage, seen from
+  // each sample direction turned into the object, times scale squared.
+  float dirs[27][3];
+  coverage_directions(dirs);
+  std::vector<float> per_object(27 * n_objects, 0.0f);
+  for (size_t k = 0; k < n_objects; k++) {
+    const uint32_t root = f.ranges[4 * k + 2];
+    if (root == NONE_U32) continue;
+    const float* cov = f.blas_cov.data() + 27 * (size_t)root;
+    const Placement& o = objects[k];
+    for (int n = 0; n < 27; n++) {
+      if (dirs[n][0] == 0.0f && dirs[n][1] == 0.0f && dirs[n][2] == 0.0f) continue;
+      float v[3];
+      mat_t_vec(o.rotation, dirs[n], v);
+      per_object[27 * k + n] = o.scale * o.scale * front_area(cov, v);
+    }
+  }
+  // Children follow their parents: a reverse sweep sees them first.
+  const size_t nn = f.nodes.size();
+  std::vector<float> ocov(27 * nn, 0.0f);
+  for (size_t n = nn; n-- > 0;) {
+    const uint32_t x = f.nodes[n].info[0], count = f.nodes[n].info[1];
+    float* out = ocov.data() + 27 * n;
+    if (count > 0) {
+      for (uint32_t i = x; i < x + count; i++)
+        for (int c = 0; c < 27; c++) out[c] += per_object[27 * (size_t)f.items[i] + c];
+    } else {
+      for (int c = 0; c < 27; c++) out[c] = ocov[27 * (size_t)x + c] + ocov[27 * (size_t)(x + 1) + c];
+    }
+  }
+  object_cov = (uint32_t)(f.blas_cov.size() / 27);
+  f.blas_cov.insert(f.blas_cov.end(), ocov.begin(), ocov.end());
+}
+
 }  // namespace host
 }  // namespace cnq

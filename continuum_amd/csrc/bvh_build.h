@@ -45,5 +45,26 @@ void object_hierarchy(const std::vector<Box>& bounds, std::vector<NodeGpu>& node
                       const uint8_t* keep = nullptr);
 void flatten(const float* tris9, const uint32_t* counts, uint32_t n_meshes, Flat& out);
 
+// A copy of a geometry placed in the world (hierarchy.rs:262-301
+// `Placement`): world = scale * rotation * p + translation, rotation
+// row-major and orthonormal; `clear` is one less the opacity of a blended
+// object (0: opaque).
+struct Placement {
+  uint32_t geometry;
+  float rotation[9];
+  float scale;
+  float translation[3];
+  float clear;
+};
+
+// `flatten_instanced` (hierarchy.rs:464-522) and the object hierarchy over
+// the objects' padded world bounds (lib.rs:1984); with `object_prefilter`,
+// the object hierarchy's coverage samples (`object_coverage`,
+// hierarchy.rs:661-700) appended to blas_cov from row `object_cov` (else
+// object_cov = NONE). xforms: 16 floats per object (`Placement::rows`).
+void flatten_instanced(const float* tris9, const uint32_t* counts, uint32_t n_geometries, const Placement* objects,
+                       size_t n_objects, bool object_prefilter, Flat& out, std::vector<float>& xforms,
+                       uint32_t& object_cov);
+
 }  // namespace host
 }  // namespace cnq

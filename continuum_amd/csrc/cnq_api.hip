@@ -149,6 +149,7 @@ int validate(const cnq_scene_desc* d) {
   for (uint32_t k = 0; k < d->n_items; k++)
     if (d->items[k] >= d->n_objects) return fail(CNQ_INVALID_ARG, "object item out of ranges");
   if (d->n_xforms && !d->xforms) return fail(CNQ_INVALID_ARG, "xforms is null");
+  if (d->n_blas_cov && d->n_blas_cov < d->n_blas) return fail(CNQ_INVALID_ARG, "fewer coverage rows than blas nodes");
   return CNQ_OK;
 }
 
@@ -216,7 +217,7 @@ hipError_t launch_items_direct(cnq::ItemArgs a, uint64_t n_items, bool count_wor
 }
 
 int check_object_cov(cnq_scene* s, const cnq_options* o) {
-  if (o->object_cov != NONE && (uint64_t)o->object_cov + s->n_nodes > s->n_blas)
+  if (o->object_cov != NONE && (uint64_t)o->object_cov + s->n_nodes > s->n_cov_rows)
     return fail(CNQ_INVALID_ARG, "object_cov + n_nodes exceeds the coverage samples");
   return CNQ_OK;
 }
@@ -290,13 +291,53 @@ int cnq_flat_copy(const cnq_flat* flat, float* tris, uint32_t* ranges, uint32_t*
   if (ranges) std::memcpy(ranges, f.ranges.data(), f.ranges.size() * 4);
   if (blas) std::memcpy(blas, f.blas.data(), f.blas.size() * 48);
   if (blas_items) std::memcpy(blas_items, f.blas_items.data(), f.blas_items.size() * 4);
-  if (blas_cov) std::memcpy(blas_cov, f.blas_cov.data(), f.blas_cov.size() * 4);
+  if (blas_cov) std::memcpy(blas_cov, f.blas_cov.data(), f.blas.size() * 27 * 4);
   if (nodes) std::memcpy(nodes, f.nodes.data(), f.nodes.size() * 48);
   if (items) std::memcpy(items, f.items.data(), f.items.size() * 4);
   return CNQ_OK;
 }
 
 void cnq_flat_destroy(cnq_flat* flat) { delete flat; }
+
+int cnq_flat_build_instanced(const float* tris9, const uint32_t* counts, uint32_t n_geometries,
+                             const cnq_placement* objects, uint64_t n_objects, int object_prefilter, cnq_flat** out) {
+  if (!out || (n_geometries && !counts) || (n_objects && !objects)) return fail(CNQ_INVALID_ARG, "null argument");
+  uint64_t total = 0;
+  for (uint32_t g = 0; g < n_geometries; g++) total += counts[g];
+  if (total && !tris9) return fail(CNQ_INVALID_ARG, "tris is null");
+  if (total > 0x3fffffffull || n_objects > 0x3fffffffull) return fail(CNQ_CAPACITY, "too many triangles or objects");
+  for (uint64_t k = 0; k < n_objects; k++) {
+    if (objects[k].geometry >= n_geometries) return fail(CNQ_INVALID_ARG, "object " + std::to_string(k) + ": no such geometry");
+    if (!(objects[k].scale > 0.0f)) return fail(CNQ_INVALID_ARG, "object " + std::to_string(k) + ": scale must be > 0");
+  }
+  static_assert(sizeof(cnq_placement) == sizeof(cnq::host::Placement), "placement layouts agree");
+  cnq_flat* f = new (std::nothrow) cnq_flat;
+  if (!f) return fail(CNQ_OOM, "out of host memory");
+  try {
+    cnq::host::flatten_instanced(tris9, counts, n_geometries, reinterpret_cast<const cnq::host::Placement*>(objects),
+                                 n_objects, object_prefilter != 0, f->f, f->xforms, f->object_cov);
+  } catch (const std::bad_alloc&) {
+    delete f;
+    return fail(CNQ_OOM, "out of host memory");
+  }
+  *out = f;
+  return CNQ_OK;
+}
+
+int cnq_flat_instanced_sizes(const cnq_flat* flat, uint64_t sizes[3]) {
+  if (!flat || !sizes) return fail(CNQ_INVALID_ARG, "null argument");
+  sizes[0] = flat->xforms.size() / 16;
+  sizes[1] = flat->f.blas_cov.size() / 27;
+  sizes[2] = flat->object_cov;
+  return CNQ_OK;
+}
+
+int cnq_flat_copy_instanced(const cnq_flat* flat, float* xforms, float* blas_cov) {
+  if (!flat) return fail(CNQ_INVALID_ARG, "null argument");
+  if (xforms) std::memcpy(xforms, flat->xforms.data(), flat->xforms.size() * 4);
+  if (blas_cov) std::memcpy(blas_cov, flat->f.blas_cov.data(), flat->f.blas_cov.size() * 4);
+  return CNQ_OK;
+}
 
 int cnq_scene_create(cnq_device* dev, const cnq_scene_desc* d, cnq_scene** out) {
   if (!dev || !out) return fail(CNQ_INVALID_ARG, "null argument");
@@ -334,10 +375,11 @@ int cnq_scene_create(cnq_device* dev, const cnq_scene_desc* d, cnq_scene** out) 
     o[0] = t[0], o[1] = t[1], o[2] = t[2], o[3] = t[4], o[4] = t[5], o[5] = t[6], o[6] = t[8], o[7] = t[9],
     o[8] = t[10];
   }
+  s->n_cov_rows = d->n_blas_cov ? d->n_blas_cov : d->n_blas;
   std::vector<float> zero_cov;
   const float* cov = d->blas_cov;
   if (!cov) {
-    zero_cov.assign(27 * (size_t)d->n_blas, 0.0f);
+    zero_cov.assign(27 * (size_t)s->n_cov_rows, 0.0f);
     cov = zero_cov.data();
   }
   hipError_t e = hipSuccess;
@@ -346,7 +388,7 @@ int cnq_scene_create(cnq_device* dev, const cnq_scene_desc* d, cnq_scene** out) 
   if (e == hipSuccess) e = s->blas_items.upload(d->blas_items, d->n_blas_items);
   if (e == hipSuccess) e = s->ranges.upload(d->ranges, 4 * (size_t)d->n_objects);
   if (e == hipSuccess) e = s->blas.upload(blas.data(), blas.size());
-  if (e == hipSuccess) e = s->blas_cov.upload(cov, 27 * (size_t)d->n_blas);
+  if (e == hipSuccess) e = s->blas_cov.upload(cov, 27 * (size_t)s->n_cov_rows);
   if (e == hipSuccess) e = s->nodes.upload(nodes.data(), nodes.size());
   if (e == hipSuccess) e = s->items.upload(d->items, d->n_items);
   if (e == hipSuccess) e = s->xforms.upload(d->xforms, 16 * (size_t)d->n_xforms);

@@ -61,6 +61,7 @@ struct cnq_scene {
   cnq_device* dev;
   uint64_t n_tris;
   uint32_t n_objects, n_blas, n_nodes, n_items, n_xforms;
+  uint32_t n_cov_rows = 0;  // rows of 27 coverage samples (>= n_blas)
   uint64_t n_blas_items;
   bool count_work = false;
   cnq::detail::DevBuf<float> tris, tri_leaf, blas_cov, xforms, mesh_box;
@@ -98,6 +99,8 @@ struct cnq_scene {
 
 struct cnq_flat {
   cnq::host::Flat f;
+  std::vector<float> xforms;      // instanced worlds: 16 per object
+  uint32_t object_cov = 0xffffffffu;
 };
 
 namespace cnq {
@@ -107,6 +110,7 @@ QueryParams query_params(cnq_scene* s, const cnq_options* o, uint32_t tile_w);
 ItemArgs item_args(cnq_scene* s, const float* receivers, const float* lights, float* vis, uint64_t* full,
                    uint64_t* touched, float* cov);
 hipError_t launch_items_direct(ItemArgs a, uint64_t n_items, bool count_work, hipStream_t stream);
+int check_object_cov(cnq_scene* s, const cnq_options* o);
 
 }  // namespace detail
 }  // namespace cnq

@@ -345,7 +345,7 @@ int cnq_server_create(cnq_scene* scene, const cnq_server_config* cfg, const cnq_
                       uint32_t n_lights, cnq_server** out) {
   if (!scene || !cfg || !opts || !out) return fail(CNQ_INVALID_ARG, "null argument");
   int rc = check_options(opts);
-  if (rc) return rc;
+  if (rc || (rc = check_object_cov(scene, opts))) return rc;
   if (cfg->slots == 0 || cfg->slots > 8) return fail(CNQ_INVALID_ARG, "slots must be 1..8");
   if (cfg->max_views_per_launch == 0 || cfg->max_view_lights == 0 || cfg->max_lights == 0)
     return fail(CNQ_INVALID_ARG, "zero-sized server limits");

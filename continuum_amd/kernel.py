@@ -42,11 +42,16 @@ def _ptr(a: np.ndarray) -> int:
     return a.ctypes.data if a.size else 0
 
 
+# `cnq_placement` as a numpy record (60 bytes).
+PLACEMENT_DTYPE = np.dtype([("geometry", "<u4"), ("rotation", "<f4", (9,)), ("scale", "<f4"),
+                            ("translation", "<f4", (3,)), ("clear", "<f4")])
+
+
 class Flat:
     """The reference's flattened scene (hierarchy.rs `flatten` + `mesh_hierarchy`),
     built by the library's host builder."""
 
-    def __init__(self, tris, ranges, blas, blas_items, blas_cov, nodes, items, xforms=None):
+    def __init__(self, tris, ranges, blas, blas_items, blas_cov, nodes, items, xforms=None, object_cov=NONE):
         self.tris = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1, 12)
         self.ranges = np.ascontiguousarray(ranges, dtype=np.uint32).reshape(-1, 4)
         self.blas = np.ascontiguousarray(blas, dtype=np.uint32).reshape(-1, 12)
@@ -56,6 +61,8 @@ class Flat:
         self.items = np.ascontiguousarray(items, dtype=np.uint32).reshape(-1)
         self.xforms = (np.zeros((0, 16), np.float32) if xforms is None else
                        np.ascontiguousarray(xforms, dtype=np.float32).reshape(-1, 16))
+        # First row of the object hierarchy's coverage in blas_cov (NONE: none).
+        self.object_cov = int(object_cov)
 
     @classmethod
     def build(cls, meshes) -> "Flat":
@@ -83,6 +90,52 @@ class Flat:
             L.cnq_flat_destroy(h)
         return cls(**out)
 
+    @classmethod
+    def build_instanced(cls, tris9: np.ndarray, counts, placements, object_prefilter: bool = True) -> "Flat":
+        """The instanced world (`flatten_instanced`, hierarchy.rs:464): geometries
+        as 9 floats per triangle, `counts[g]` each; placements as
+        (geometry, rotation 3x3 row-major, scale, translation, clear) tuples
+        or a `Placement` array."""
+        L = lib()
+        tris9 = np.ascontiguousarray(tris9, dtype=np.float32)
+        counts = np.ascontiguousarray(counts, dtype=np.uint32)
+        if isinstance(placements, np.ndarray):
+            arr = np.ascontiguousarray(placements, dtype=PLACEMENT_DTYPE)
+            ptr, n = _ptr(arr), len(arr)
+        elif isinstance(placements, C.Array):
+            arr = placements
+            ptr, n = C.cast(arr, C.c_void_p), len(arr)
+        else:
+            arr = (_lib.Placement * len(placements))()
+            for k, (g, rot, scale, t, clear) in enumerate(placements):
+                arr[k].geometry = int(g)
+                arr[k].rotation[:] = [float(x) for x in np.asarray(rot, np.float32).reshape(9)]
+                arr[k].scale = float(scale)
+                arr[k].translation[:] = [float(x) for x in np.asarray(t, np.float32).reshape(3)]
+                arr[k].clear = float(clear)
+            ptr, n = C.cast(arr, C.c_void_p), len(arr)
+        h = C.c_void_p()
+        check(L.cnq_flat_build_instanced(_ptr(tris9), _ptr(counts), len(counts), ptr, n, int(object_prefilter),
+                                         C.byref(h)))
+        try:
+            sizes = np.zeros(6, dtype=np.uint64)
+            check(L.cnq_flat_sizes(h, _ptr(sizes)))
+            nt, nr, nb, nbi, nn, ni = (int(x) for x in sizes)
+            extra = np.zeros(3, dtype=np.uint64)
+            check(L.cnq_flat_instanced_sizes(h, _ptr(extra)))
+            nx, ncov, ocov = (int(x) for x in extra)
+            out = dict(tris=np.zeros((nt, 12), np.float32), ranges=np.zeros((nr, 4), np.uint32),
+                       blas=np.zeros((nb, 12), np.uint32), blas_items=np.zeros(nbi, np.uint32),
+                       nodes=np.zeros((nn, 12), np.uint32), items=np.zeros(ni, np.uint32))
+            check(L.cnq_flat_copy(h, _ptr(out["tris"]), _ptr(out["ranges"]), _ptr(out["blas"]),
+                                  _ptr(out["blas_items"]), 0, _ptr(out["nodes"]), _ptr(out["items"])))
+            xforms = np.zeros((nx, 16), np.float32)
+            cov = np.zeros((ncov, 27), np.float32)
+            check(L.cnq_flat_copy_instanced(h, _ptr(xforms), _ptr(cov)))
+        finally:
+            L.cnq_flat_destroy(h)
+        return cls(blas_cov=cov, xforms=xforms, object_cov=ocov, **out)
+
     def desc(self) -> SceneDesc:
         d = SceneDesc()
         d.tris, d.n_tris = _ptr(self.tris), len(self.tris)
@@ -93,6 +146,7 @@ class Flat:
         d.nodes, d.n_nodes = _ptr(self.nodes), len(self.nodes)
         d.items, d.n_items = _ptr(self.items), len(self.items)
         d.xforms, d.n_xforms = _ptr(self.xforms), len(self.xforms)
+        d.n_blas_cov = len(self.blas_cov)
         return d
 
 

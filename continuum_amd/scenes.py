@@ -392,6 +392,97 @@ def courtyard(seed: int = 0xC0_0003, width: int = 1280, height: int = 720, detai
                      notes={"curtains": curtains, "eye": eye, "target": target, "fov": 60.0, "seed": seed})
 
 
+def boulder(coef, nu: int = 100, nv: int = 50) -> np.ndarray:
+    """A lumpy boulder about 1 m across, as experiments/room3d/src/field.rs:30-58
+    builds it (a sphere with low-frequency bumps, squashed vertically, on a
+    `Surface::parametric` nu x nv grid, degenerate pole triangles dropped,
+    wound outward), with the three bump amplitudes `coef`."""
+    a1, a2, a3 = coef
+    u = np.arange(nu + 1, dtype=np.float64) / nu
+    v = np.arange(nv + 1, dtype=np.float64) / nv
+    th, ph = np.meshgrid(2 * np.pi * u, np.pi * v, indexing="ij")
+    r = (1.0 + a1 * np.sin(3 * th + 1) * np.sin(2 * ph) + a2 * np.cos(5 * th) * np.sin(3 * ph + 0.5)
+         + a3 * np.sin(9 * th + 2 * ph))
+    grid = np.stack([np.sin(ph) * np.cos(th) * r * 0.5, np.sin(ph) * np.sin(th) * r * 0.5, -np.cos(ph) * r * 0.33],
+                    axis=-1).astype(F32)
+    a, b, c, d = grid[:-1, :-1], grid[1:, :-1], grid[1:, 1:], grid[:-1, 1:]
+    tris = np.stack([np.stack([a, b, c], axis=2), np.stack([a, c, d], axis=2)], axis=2).reshape(-1, 3, 3)
+    e1, e2 = tris[:, 1] - tris[:, 0], tris[:, 2] - tris[:, 0]
+    scale = np.maximum(np.linalg.norm(e1, axis=1), np.linalg.norm(e2, axis=1))
+    keep = np.linalg.norm(np.cross(e1, e2), axis=1) > 1e-6 * scale * scale
+    tris = tris[keep]
+    t = tris[0]
+    if np.dot(np.cross(t[1] - t[0], t[2] - t[0]), t[0]) < 0:
+        tris = tris[:, [0, 2, 1]]
+    return np.ascontiguousarray(tris, dtype=F32)
+
+
+def boulder_field(seed: int = 0xC0_0005, count: int = 1_000_000, kinds: int = 10, half: int = 500,
+                  spacing: float = 2.0, lights: int = 32):
+    """The instanced serving world: a ground quad about 2.1 km across
+    (geometry 0) and `kinds` seeded boulder shapes (geometries 1..kinds,
+    about 9,800 triangles each), placed `count` times on a jittered 2 m grid
+    (the `count` nearest the camera, as field.rs:60-124 does), each turned
+    about z and scaled 0.6-1.4. Lights: a 0.5-degree sun panel 3 km away, then
+    rectangles of 1-4 m^2 scattered over the field 6-12 m up.
+
+    Returns (geometries as one (n, 3, 3) array, triangle counts, placements
+    as a PLACEMENT_DTYPE-shaped record array, lights, eye, target). Random
+    numbers come from numpy's PCG64 stream `seed`."""
+    rng = np.random.default_rng(seed)
+    h = 1050.0
+    ground = quad_pair([-h, -h, 0.0], [2 * h, 0, 0], [0, 2 * h, 0])
+    shapes = [boulder(rng.uniform([0.12, 0.06, 0.02], [0.30, 0.18, 0.10])) for _ in range(kinds)]
+    geoms = [ground] + shapes
+    eye = np.array([-990.0, -990.0, 1.7])
+    target = np.array([0.0, 0.0, 1.0])
+    n = 2 * half
+    j, i = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    jit = rng.uniform(-0.6, 0.6, (n, n, 2))
+    x = (i - half + 0.5) * spacing + jit[..., 0]
+    y = (j - half + 0.5) * spacing + jit[..., 1]
+    angle = rng.uniform(0, 2 * np.pi, (n, n))
+    scale = rng.uniform(0.6, 1.4, (n, n))
+    kind = rng.integers(1, kinds + 1, (n, n))
+    pos = np.stack([x, y, 0.2 * scale], axis=-1).reshape(-1, 3)
+    d2 = ((pos[:, :2] - eye[:2]) ** 2).sum(1)
+    ok = np.sqrt(d2) >= 4.0
+    order = np.argsort(np.where(ok, d2, np.inf), kind="stable")[: min(count, int(ok.sum()))]
+    m = len(order) + 1
+    from .kernel import PLACEMENT_DTYPE  # the library's record layout
+
+    pl = np.zeros(m, dtype=PLACEMENT_DTYPE)
+    pl["rotation"][0] = [1, 0, 0, 0, 1, 0, 0, 0, 1]
+    pl["scale"][0] = 1.0
+    ang, sc = angle.reshape(-1)[order], scale.reshape(-1)[order]
+    c, s = np.cos(ang).astype(F32), np.sin(ang).astype(F32)
+    z, o = np.zeros_like(c), np.ones_like(c)
+    pl["geometry"][1:] = kind.reshape(-1)[order]
+    pl["rotation"][1:] = np.stack([c, -s, z, s, c, z, z, z, o], axis=1)
+    pl["scale"][1:] = sc
+    pl["translation"][1:] = pos[order]
+    # the sun: 30 degrees up, azimuth 200 degrees, 3 km out, 0.5 degrees across
+    el, az = math.radians(30.0), math.radians(200.0)
+    toward = np.array([math.cos(el) * math.cos(az), math.cos(el) * math.sin(az), math.sin(el)])
+    dist = 3000.0
+    size = 2.0 * dist * math.tan(math.radians(0.25))
+    w = -toward
+    a = np.cross([0.0, 0.0, 1.0], w)
+    a /= np.linalg.norm(a)
+    b = np.cross(w, a)
+    center = toward * dist
+    out_lights = [rect_light(center - (a + b) * size / 2, a * size, b * size, one_sided=True)]
+    for _ in range(lights - 1):
+        area = rng.uniform(1.0, 4.0)
+        s_ = math.sqrt(area)
+        cx, cy = rng.uniform(-990, 990, 2)
+        cz = rng.uniform(6.0, 12.0)
+        out_lights.append(rect_light([cx - s_ / 2, cy + s_ / 2, cz], [s_, 0, 0], [0, -s_, 0], one_sided=True,
+                                     radiance=tuple(rng.uniform(2.0, 10.0, 3))))
+    tris9, counts = meshes_to_arrays(geoms)
+    return tris9, counts, pl, np.stack(out_lights), eye, target
+
+
 def courtyard_frame(spec: SceneSpec, frame: int, tris12: np.ndarray, ranges: np.ndarray) -> np.ndarray:
     """The courtyard's triangles (12 f32 each, flattened order) at `frame`:
     curtain vertices displaced across the curtain by a seeded sine field of

@@ -52,3 +52,44 @@ def courtyard_views(n_views: int = 64, seed: int = 0xC0_0004, max_lights: int = 
 
 def view_queries(views) -> int:
     return int(sum(w * h * nl for _, w, h, _, nl in views))
+
+
+def serving_requests(n: int, rate: float, lights: np.ndarray, seed: int = 0xC0_0007, view_share: float = 0.8,
+                     max_view_lights: int = 32, resolutions=RESOLUTIONS, extent: float = 900.0):
+    """A seeded request stream over the boulder field: Poisson arrivals at
+    `rate` per second; a `view_share` of them views (a resolution from
+    `resolutions`, an eye 1.5-20 m up anywhere in the field looking 20-200 m
+    away, 1-`max_view_lights` distinct lights of the table), the rest light
+    edits (move, resize or recolour one rectangle of the table). Returns
+    [(arrival_s, "view", (camera20, w, h, light indices))
+     | (arrival_s, "edit", (index, LightGpu))]."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, n))
+    table = np.array(lights, np.float32, copy=True)
+    n_lights = len(table)
+    out = []
+    for k in range(n):
+        if rng.uniform() < view_share:
+            w, h = resolutions[int(rng.integers(len(resolutions)))]
+            eye = np.array([rng.uniform(-extent, extent), rng.uniform(-extent, extent), rng.uniform(1.5, 20.0)])
+            ang = rng.uniform(0, 2 * np.pi)
+            dist = rng.uniform(20.0, 200.0)
+            tgt = eye + np.array([np.cos(ang) * dist, np.sin(ang) * dist, -eye[2] + rng.uniform(0.0, 2.0)])
+            nl = int(rng.integers(1, min(max_view_lights, n_lights) + 1))
+            idx = sorted(rng.choice(n_lights, size=nl, replace=False).tolist())
+            out.append((float(t[k]), "view", (camera_uniform(eye, tgt, 60.0, w, h), w, h, idx)))
+        else:
+            i = int(rng.integers(1, n_lights))
+            row = table[i].copy()
+            what = int(rng.integers(3))
+            if what == 0:
+                row[0:2] += rng.uniform(-5.0, 5.0, 2).astype(np.float32)
+            elif what == 1:
+                s = np.float32(rng.uniform(0.7, 1.4))
+                row[4:7] *= s
+                row[8:11] *= s
+            else:
+                row[12:15] = rng.uniform(2.0, 10.0, 3).astype(np.float32)
+            table[i] = row
+            out.append((float(t[k]), "edit", (i, row)))
+    return out

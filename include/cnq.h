@@ -87,6 +87,29 @@ int cnq_flat_copy(const cnq_flat* flat, float* tris, uint32_t* ranges, uint32_t*
                   float* blas_cov, uint32_t* nodes, uint32_t* items);
 void cnq_flat_destroy(cnq_flat* flat);
 
+/* An object placing one geometry (hierarchy.rs:262-301 `Placement`):
+ * world = scale * rotation * p + translation. */
+typedef struct cnq_placement {
+  uint32_t geometry;
+  float rotation[9];    /* row-major, orthonormal */
+  float scale;
+  float translation[3]; /* relative to the render origin */
+  float clear;          /* one less the opacity of a blended object; 0: opaque */
+} cnq_placement;
+
+/* The instanced world: `flatten_instanced` (hierarchy.rs:464-522), the
+ * object hierarchy over the objects' padded world bounds (lib.rs:1984)
+ * and, with object_prefilter, the object hierarchy's coverage samples
+ * (`object_coverage`, hierarchy.rs:661-700) after the triangle
+ * hierarchies' in blas_cov (lib.rs:2271-2278). cnq_flat_sizes/copy give the
+ * arrays as for flatten (blas_cov: the triangle hierarchies' rows); then: */
+int cnq_flat_build_instanced(const float* tris9, const uint32_t* counts, uint32_t n_geometries,
+                             const cnq_placement* objects, uint64_t n_objects, int object_prefilter, cnq_flat** out);
+/* sizes: n_xforms, coverage rows in blas_cov (with the object hierarchy's),
+ * object_cov (the object hierarchy's first row; CNQ_NONE: none). */
+int cnq_flat_instanced_sizes(const cnq_flat* flat, uint64_t sizes[3]);
+int cnq_flat_copy_instanced(const cnq_flat* flat, float* xforms, float* blas_cov);
+
 /* ---- A scene resident in HBM, from the reference's flattened arrays
  * (lib.rs:3361-3440 `run`'s buffers; lib.rs:3002 `render_buffers_with`).
  * Every index is checked on the host: children follow their parents,
@@ -100,6 +123,7 @@ typedef struct cnq_scene_desc {
   const uint32_t* nodes;      uint32_t n_nodes;      /* object hierarchy, NodeGpu */
   const uint32_t* items;      uint32_t n_items;
   const float* xforms;        uint32_t n_xforms;     /* 16 f32 per transform */
+  uint32_t n_blas_cov;                               /* rows in blas_cov (0: n_blas) */
 } cnq_scene_desc;
 
 int cnq_scene_create(cnq_device* dev, const cnq_scene_desc* desc, cnq_scene** out);

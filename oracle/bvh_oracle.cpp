@@ -12,6 +12,7 @@
 // reference; which items land in a leaf and in what order follows the
 // (key, index) rule.
 #include <algorithm>
+#include <array>
 #include <cstring>
 #include <vector>
 
@@ -206,6 +207,162 @@ static void object_hierarchy(const std::vector<Box>& bounds, std::vector<NodeGpu
   nodes[0].info[1] = nodes[0].info[2] = nodes[0].info[3] = 0;
 }
 
+// The reference's `Flat` with the object hierarchy.
+struct OFlat {
+  std::vector<float> tris;
+  std::vector<uint32_t> ranges;
+  std::vector<NodeGpu> blas;
+  std::vector<uint32_t> blas_items;
+  std::vector<float> blas_cov;
+  std::vector<NodeGpu> nodes;
+  std::vector<uint32_t> items;
+};
+
+// hierarchy.rs:29-66 `flatten` + :183 `mesh_hierarchy`.
+static void flatten(const float* tris, const uint32_t* counts, uint32_t n_meshes, OFlat& f) {
+  std::vector<Box> mesh_boxes;
+  size_t off = 0;
+  for (uint32_t m = 0; m < n_meshes; m++) {
+    const float* mesh = tris + 9 * off;
+    size_t n = counts[m];
+    uint32_t first = (uint32_t)(f.tris.size() / 12);
+    uint32_t root = BNONE;
+    if (n > 0) {
+      root = (uint32_t)f.blas.size();
+      std::vector<NodeGpu> nodes;
+      std::vector<uint32_t> items;
+      std::vector<float> cov;
+      triangle_hierarchy(mesh, n, root, first, nodes, items, cov);
+      f.blas.insert(f.blas.end(), nodes.begin(), nodes.end());
+      f.blas_items.insert(f.blas_items.end(), items.begin(), items.end());
+      f.blas_cov.insert(f.blas_cov.end(), cov.begin(), cov.end());
+    }
+    f.ranges.insert(f.ranges.end(), {first, (uint32_t)n, root, BNONE});
+    for (size_t i = 0; i < 3 * n; i++) f.tris.insert(f.tris.end(), {mesh[3 * i], mesh[3 * i + 1], mesh[3 * i + 2], 0.0f});
+    mesh_boxes.push_back(padded(tri_bounds(mesh, n)));
+    off += n;
+  }
+  if (f.tris.empty()) f.tris.assign(12, 0.0f);
+  if (f.ranges.empty()) f.ranges.insert(f.ranges.end(), {0, 0, BNONE, BNONE});
+  if (f.blas.empty()) {
+    f.blas.push_back(NodeGpu{});
+    f.blas_items.push_back(0);
+    f.blas_cov.assign(27, 0.0f);
+  }
+  object_hierarchy(mesh_boxes, f.nodes, f.items);
+}
+
+// hierarchy.rs:262-301 `Placement`: world = scale * R p + t, R by rows.
+struct Placed {
+  uint32_t geometry;
+  V3 row[3];
+  float scale;
+  V3 t;
+  float clear;
+  // glam's Mat3 * Vec3: the columns scaled by p's components, summed in order.
+  V3 apply(V3 p) const {
+    V3 c0{row[0].x, row[1].x, row[2].x}, c1{row[0].y, row[1].y, row[2].y}, c2{row[0].z, row[1].z, row[2].z};
+    return c0 * p.x + c1 * p.y + c2 * p.z;
+  }
+  // rotation.transpose() * d: the rows scaled by d's components.
+  V3 apply_t(V3 d) const { return row[0] * d.x + row[1] * d.y + row[2] * d.z; }
+};
+
+// hierarchy.rs:624-653 `front_area` (host lerp: a + (b - a) t).
+static float host_front_area(const float* cov, V3 v) {
+  V3 a = vabs(v);
+  int axis = (a.y > a.x && a.y >= a.z) ? 1 : ((a.z > a.x && a.z > a.y) ? 2 : 0);
+  float mainv = axis == 0 ? v.x : (axis == 1 ? v.y : v.z);
+  float s = axis == 0 ? v.y / a.x : (axis == 1 ? v.x / a.y : v.x / a.z);
+  float t = axis == 0 ? v.z / a.x : (axis == 1 ? v.z / a.y : v.y / a.z);
+  int m = mainv >= 0.0f ? 1 : -1;
+  int s0 = s < 0.0f ? -1 : 0, t0 = t < 0.0f ? -1 : 0;
+  float fs = s - (float)s0, ft = t - (float)t0;
+  auto sample = [&](int i, int j, int k) { return cov[9 * (i + 1) + 3 * (j + 1) + (k + 1)]; };
+  float f[4];
+  for (int c = 0; c < 4; c++) {
+    int gs = s0 + (c & 1), gt = t0 + (c >> 1);
+    f[c] = axis == 0 ? sample(m, gs, gt) : (axis == 1 ? sample(gs, m, gt) : sample(gs, gt, m));
+  }
+  auto lerp = [](float x, float y, float w) { return x + (y - x) * w; };
+  return lerp(lerp(f[0], f[1], fs), lerp(f[2], f[3], fs), ft);
+}
+
+static void flatten_instanced(const float* tris, const uint32_t* counts, uint32_t n_geometries,
+                              const std::vector<Placed>& objects, bool object_prefilter, OFlat& f,
+                              std::vector<float>& xforms, uint32_t& object_cov) {
+  flatten(tris, counts, n_geometries, f);
+  // Blended geometries: coverage times the first object's opacity.
+  std::vector<std::pair<uint32_t, uint32_t>> roots;
+  for (uint32_t g = 0; g < n_geometries; g++)
+    if (f.ranges[4 * g + 2] != BNONE) roots.push_back({f.ranges[4 * g + 2], g});
+  std::sort(roots.begin(), roots.end());
+  for (size_t k = 0; k < roots.size(); k++) {
+    auto it = std::find_if(objects.begin(), objects.end(), [&](const Placed& o) { return o.geometry == roots[k].second; });
+    if (it == objects.end() || it->clear == 0.0f) continue;
+    uint32_t end = k + 1 < roots.size() ? roots[k + 1].first : (uint32_t)f.blas.size();
+    float opacity = it->clear > 0.0f ? 1.0f - it->clear
+                    : it->clear <= -2.0f ? 1.0f - (0.96f * (-2.0f - it->clear)) * (0.96f * (-2.0f - it->clear))
+                                         : 1.0f - 0.96f * -it->clear;
+    for (size_t c = 27 * (size_t)roots[k].first; c < 27 * (size_t)end; c++) f.blas_cov[c] *= opacity;
+  }
+  std::vector<uint32_t> granges = f.ranges;
+  std::vector<Box> local;
+  size_t off = 0;
+  for (uint32_t g = 0; g < n_geometries; g++) {
+    local.push_back(tri_bounds(tris + 9 * off, counts[g]));
+    off += counts[g];
+  }
+  f.ranges.clear();
+  std::vector<Box> bounds;
+  for (size_t k = 0; k < objects.size(); k++) {
+    const Placed& o = objects[k];
+    f.ranges.insert(f.ranges.end(), {granges[4 * o.geometry], granges[4 * o.geometry + 1], granges[4 * o.geometry + 2],
+                                     (uint32_t)k});
+    xforms.insert(xforms.end(), {o.row[0].x, o.row[0].y, o.row[0].z, o.t.x, o.row[1].x, o.row[1].y, o.row[1].z, o.t.y,
+                                 o.row[2].x, o.row[2].y, o.row[2].z, o.t.z, o.scale, 1.0f / o.scale, o.clear, 0.0f});
+    Box w{V3{FMAX, FMAX, FMAX}, V3{-FMAX, -FMAX, -FMAX}};
+    const Box& lb = local[o.geometry];
+    for (int c = 0; c < 8; c++) {
+      V3 p{(c & 1) ? lb.hi.x : lb.lo.x, (c & 2) ? lb.hi.y : lb.lo.y, (c & 4) ? lb.hi.z : lb.lo.z};
+      V3 q = o.apply(p) * o.scale + o.t;
+      w.lo = gvmin(w.lo, q);
+      w.hi = gvmax(w.hi, q);
+    }
+    bounds.push_back(padded(w));
+  }
+  if (f.ranges.empty()) f.ranges.insert(f.ranges.end(), {0, 0, BNONE, BNONE});
+  if (xforms.empty()) xforms.assign(16, 0.0f);
+  object_hierarchy(bounds, f.nodes, f.items);
+  object_cov = BNONE;
+  if (!object_prefilter) return;
+  V3 dirs[27];
+  coverage_directions(dirs);
+  std::vector<std::array<float, 27>> per(objects.size());
+  for (size_t k = 0; k < objects.size(); k++) {
+    per[k].fill(0.0f);
+    uint32_t root = f.ranges[4 * k + 2];
+    if (root == BNONE) continue;
+    for (int n = 0; n < 27; n++) {
+      if (dirs[n].x == 0.0f && dirs[n].y == 0.0f && dirs[n].z == 0.0f) continue;
+      per[k][n] = objects[k].scale * objects[k].scale *
+                  host_front_area(f.blas_cov.data() + 27 * (size_t)root, objects[k].apply_t(dirs[n]));
+    }
+  }
+  std::vector<float> cov(27 * f.nodes.size(), 0.0f);
+  for (size_t n = f.nodes.size(); n-- > 0;) {
+    uint32_t x = f.nodes[n].info[0], count = f.nodes[n].info[1];
+    if (count > 0) {
+      for (uint32_t i = x; i < x + count; i++)
+        for (int c = 0; c < 27; c++) cov[27 * n + c] += per[f.items[i]][c];
+    } else {
+      for (int c = 0; c < 27; c++) cov[27 * n + c] = cov[27 * (size_t)x + c] + cov[27 * (size_t)(x + 1) + c];
+    }
+  }
+  object_cov = (uint32_t)(f.blas_cov.size() / 27);
+  f.blas_cov.insert(f.blas_cov.end(), cov.begin(), cov.end());
+}
+
 }  // namespace orc
 
 using namespace orc;
@@ -219,59 +376,73 @@ extern "C" {
 int oracle_flatten(const float* tris, const uint32_t* counts, uint32_t n_meshes, uint64_t* sizes, float* out_tris,
                    uint32_t* out_ranges, float* out_blas, uint32_t* out_blas_items, float* out_blas_cov,
                    float* out_nodes, uint32_t* out_items) {
-  std::vector<float> ftris;
-  std::vector<uint32_t> ranges;
-  std::vector<NodeGpu> blas;
-  std::vector<uint32_t> blas_items;
-  std::vector<float> blas_cov;
-  std::vector<Box> mesh_boxes;
-  size_t off = 0;
-  for (uint32_t m = 0; m < n_meshes; m++) {
-    const float* mesh = tris + 9 * off;
-    size_t n = counts[m];
-    uint32_t first = (uint32_t)(ftris.size() / 12);
-    uint32_t root = BNONE;
-    if (n > 0) {
-      root = (uint32_t)blas.size();
-      std::vector<NodeGpu> nodes;
-      std::vector<uint32_t> items;
-      std::vector<float> cov;
-      triangle_hierarchy(mesh, n, root, first, nodes, items, cov);
-      blas.insert(blas.end(), nodes.begin(), nodes.end());
-      blas_items.insert(blas_items.end(), items.begin(), items.end());
-      blas_cov.insert(blas_cov.end(), cov.begin(), cov.end());
-    }
-    ranges.insert(ranges.end(), {first, (uint32_t)n, root, BNONE});
-    for (size_t i = 0; i < 3 * n; i++) ftris.insert(ftris.end(), {mesh[3 * i], mesh[3 * i + 1], mesh[3 * i + 2], 0.0f});
-    mesh_boxes.push_back(padded(tri_bounds(mesh, n)));
-    off += n;
-  }
-  if (ftris.empty()) ftris.assign(12, 0.0f);
-  if (ranges.empty()) ranges.insert(ranges.end(), {0, 0, BNONE, BNONE});
-  if (blas.empty()) {
-    blas.push_back(NodeGpu{});
-    blas_items.push_back(0);
-    blas_cov.assign(27, 0.0f);
-  }
-  std::vector<NodeGpu> nodes;
-  std::vector<uint32_t> items;
-  object_hierarchy(mesh_boxes, nodes, items);
+  OFlat f;
+  flatten(tris, counts, n_meshes, f);
   if (!out_tris) {
-    sizes[0] = ftris.size() / 12;
-    sizes[1] = ranges.size() / 4;
-    sizes[2] = blas.size();
-    sizes[3] = blas_items.size();
-    sizes[4] = nodes.size();
-    sizes[5] = items.size();
+    sizes[0] = f.tris.size() / 12;
+    sizes[1] = f.ranges.size() / 4;
+    sizes[2] = f.blas.size();
+    sizes[3] = f.blas_items.size();
+    sizes[4] = f.nodes.size();
+    sizes[5] = f.items.size();
     return 0;
   }
-  std::memcpy(out_tris, ftris.data(), ftris.size() * 4);
-  std::memcpy(out_ranges, ranges.data(), ranges.size() * 4);
-  std::memcpy(out_blas, blas.data(), blas.size() * sizeof(NodeGpu));
-  std::memcpy(out_blas_items, blas_items.data(), blas_items.size() * 4);
-  std::memcpy(out_blas_cov, blas_cov.data(), blas_cov.size() * 4);
-  std::memcpy(out_nodes, nodes.data(), nodes.size() * sizeof(NodeGpu));
-  std::memcpy(out_items, items.data(), items.size() * 4);
+  std::memcpy(out_tris, f.tris.data(), f.tris.size() * 4);
+  std::memcpy(out_ranges, f.ranges.data(), f.ranges.size() * 4);
+  std::memcpy(out_blas, f.blas.data(), f.blas.size() * sizeof(NodeGpu));
+  std::memcpy(out_blas_items, f.blas_items.data(), f.blas_items.size() * 4);
+  std::memcpy(out_blas_cov, f.blas_cov.data(), f.blas_cov.size() * 4);
+  std::memcpy(out_nodes, f.nodes.data(), f.nodes.size() * sizeof(NodeGpu));
+  std::memcpy(out_items, f.items.data(), f.items.size() * 4);
+  return 0;
+}
+
+// flatten_instanced (hierarchy.rs:464-522) + object_hierarchy over the
+// objects' world bounds (lib.rs:1984) + object_coverage (hierarchy.rs:661)
+// when object_prefilter. Placements: 15 words each (geometry u32, rotation
+// row-major 9 f32, scale, translation 3, clear). Two calls: with null
+// outputs, sizes = [n_tris, n_ranges, n_blas, n_blas_items, n_nodes,
+// n_items, n_cov_rows, object_cov]; then it fills them (xforms: 16 per object).
+int oracle_flatten_instanced(const float* tris, const uint32_t* counts, uint32_t n_geometries, const uint32_t* placements,
+                             uint64_t n_objects, int object_prefilter, uint64_t* sizes, float* out_tris,
+                             uint32_t* out_ranges, float* out_blas, uint32_t* out_blas_items, float* out_blas_cov,
+                             float* out_nodes, uint32_t* out_items, float* out_xforms) {
+  std::vector<Placed> objs(n_objects);
+  for (uint64_t k = 0; k < n_objects; k++) {
+    const uint32_t* w = placements + 15 * k;
+    objs[k].geometry = w[0];
+    float fw[14];
+    std::memcpy(fw, w + 1, 56);
+    objs[k].row[0] = V3{fw[0], fw[1], fw[2]};
+    objs[k].row[1] = V3{fw[3], fw[4], fw[5]};
+    objs[k].row[2] = V3{fw[6], fw[7], fw[8]};
+    objs[k].scale = fw[9];
+    objs[k].t = V3{fw[10], fw[11], fw[12]};
+    objs[k].clear = fw[13];
+  }
+  OFlat f;
+  std::vector<float> xforms;
+  uint32_t object_cov = BNONE;
+  flatten_instanced(tris, counts, n_geometries, objs, object_prefilter != 0, f, xforms, object_cov);
+  if (!out_tris) {
+    sizes[0] = f.tris.size() / 12;
+    sizes[1] = f.ranges.size() / 4;
+    sizes[2] = f.blas.size();
+    sizes[3] = f.blas_items.size();
+    sizes[4] = f.nodes.size();
+    sizes[5] = f.items.size();
+    sizes[6] = f.blas_cov.size() / 27;
+    sizes[7] = object_cov;
+    return 0;
+  }
+  std::memcpy(out_tris, f.tris.data(), f.tris.size() * 4);
+  std::memcpy(out_ranges, f.ranges.data(), f.ranges.size() * 4);
+  std::memcpy(out_blas, f.blas.data(), f.blas.size() * sizeof(NodeGpu));
+  std::memcpy(out_blas_items, f.blas_items.data(), f.blas_items.size() * 4);
+  std::memcpy(out_blas_cov, f.blas_cov.data(), f.blas_cov.size() * 4);
+  std::memcpy(out_nodes, f.nodes.data(), f.nodes.size() * sizeof(NodeGpu));
+  std::memcpy(out_items, f.items.data(), f.items.size() * 4);
+  std::memcpy(out_xforms, xforms.data(), xforms.size() * 4);
   return 0;
 }
 

@@ -46,6 +46,7 @@ def lib() -> C.CDLL:
             "oracle_cone3_visible": (None, [_P, C.c_uint32, _P, _P, _P, C.c_uint32, C.c_int, _P, _P]),
             "oracle_sampled_visible": (C.c_double, [_P, _P, _P, C.c_uint64, C.c_uint32]),
             "oracle_mesh_hierarchy": (C.c_int, [_P, _P, C.c_uint32, _P, _P, _P, _P]),
+            "oracle_flatten_instanced": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_uint64, C.c_int] + [_P] * 9),
             "oracle_block_planes": (None, [_P, _P, C.c_uint32, _P, _P]),
             "oracle_mark_blocks": (None, [_P, C.c_uint32, _P, C.c_uint32, _P]),
             "oracle_bake_static": (None, [_P] * 9 + [_P, _P, C.c_uint32, _P, _P, _P, C.c_int]),
@@ -98,6 +99,35 @@ class OracleFlat:
         self.xforms = np.zeros((1, 16), np.float32)
         L.oracle_flatten(_p(tris9), _p(counts), len(counts), _p(sizes), _p(self.tris), _p(self.ranges),
                          _p(self.blas), _p(self.blas_items), _p(self.blas_cov), _p(self.nodes), _p(self.items))
+
+
+class OracleInstancedFlat:
+    """The oracle's instanced world (`flatten_instanced` + object hierarchy
+    + object coverage); `placements` is a numpy record array of the
+    library's placement layout (geometry u32, rotation 9 f32 row-major,
+    scale, translation 3, clear)."""
+
+    def __init__(self, tris9, counts, placements, object_prefilter=True):
+        tris9 = np.ascontiguousarray(tris9, dtype=np.float32)
+        counts = np.ascontiguousarray(counts, dtype=np.uint32)
+        pl = np.ascontiguousarray(placements)
+        assert pl.dtype.itemsize == 60
+        sizes = np.zeros(8, np.uint64)
+        L = lib()
+        args = (_p(tris9), _p(counts), len(counts), _p(pl), len(pl), int(object_prefilter), _p(sizes))
+        L.oracle_flatten_instanced(*args, 0, 0, 0, 0, 0, 0, 0, 0)
+        nt, nr, nb, nbi, nn, ni, ncov, ocov = (int(x) for x in sizes)
+        self.tris = np.zeros((nt, 12), np.float32)
+        self.ranges = np.zeros((nr, 4), np.uint32)
+        self.blas = np.zeros((nb, 12), np.uint32)
+        self.blas_items = np.zeros(nbi, np.uint32)
+        self.blas_cov = np.zeros((ncov, 27), np.float32)
+        self.nodes = np.zeros((nn, 12), np.uint32)
+        self.items = np.zeros(ni, np.uint32)
+        self.xforms = np.zeros((max(len(pl), 1), 16), np.float32)
+        self.object_cov = ocov
+        L.oracle_flatten_instanced(*args, _p(self.tris), _p(self.ranges), _p(self.blas), _p(self.blas_items),
+                                   _p(self.blas_cov), _p(self.nodes), _p(self.items), _p(self.xforms))
 
 
 def scene_args(flat):

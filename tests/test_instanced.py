@@ -1,0 +1,145 @@
+"""CPU: the instanced world's host builder (`flatten_instanced`,
+hierarchy.rs:464-522; object hierarchy over world bounds, lib.rs:1984;
+`object_coverage`, hierarchy.rs:661-700).
+
+Pins: the library's builder equals the oracle's bit for bit (with blended
+objects); transforms are `Placement::rows`; world bounds contain every
+transformed vertex and are padded as `padded`; a lone identity instance
+gives its geometry's own flatten; the object hierarchy's coverage equals,
+for quarter-turn rotations where the bilinear lookup lands on samples, the
+projected front area of the placed triangles summed in f64, and every
+interior node is the sum of its children.
+"""
+import numpy as np
+import pytest
+
+from continuum_amd.kernel import PLACEMENT_DTYPE, Flat
+from continuum_amd.scenes import box_mesh, boulder, boulder_field, meshes_to_arrays
+from oracle import oracle as O
+
+
+def _world(n=150, seed=1, quarter=False, blend=True):
+    rng = np.random.default_rng(seed)
+    geoms = [box_mesh([-0.5, -0.3, 0.0], [0.5, 0.3, 0.4]), boulder((0.2, 0.1, 0.05), 24, 12),
+             boulder((0.1, 0.15, 0.08), 16, 8)]
+    tris9, counts = meshes_to_arrays(geoms)
+    pl = np.zeros(n, dtype=PLACEMENT_DTYPE)
+    for k in range(n):
+        if quarter:
+            q = int(rng.integers(0, 4))
+            c, s = [1.0, 0.0, -1.0, 0.0][q], [0.0, 1.0, 0.0, -1.0][q]
+            rot = np.array([[c, -s, 0], [s, c, 0], [0, 0, 1]], np.float32)
+        else:
+            a = rng.normal(size=3)
+            a /= np.linalg.norm(a)
+            ang = rng.uniform(0, 2 * np.pi)
+            K = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+            rot = (np.eye(3) + np.sin(ang) * K + (1 - np.cos(ang)) * K @ K).astype(np.float32)
+        pl[k]["geometry"] = k % 3
+        pl[k]["rotation"] = rot.reshape(9)
+        pl[k]["scale"] = rng.uniform(0.5, 2.0)
+        pl[k]["translation"] = rng.uniform([-20, -20, 0], [20, 20, 3])
+        pl[k]["clear"] = (0.3 if (blend and k == 4) else 0.0)
+    return geoms, tris9, counts, pl
+
+
+def _same_bits(a, b):
+    a, b = np.ascontiguousarray(a), np.ascontiguousarray(b)
+    assert a.shape == b.shape and np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+@pytest.mark.parametrize("prefilter", [True, False])
+def test_product_builder_equals_oracle(prefilter):
+    geoms, tris9, counts, pl = _world()
+    f = Flat.build_instanced(tris9, counts, pl, object_prefilter=prefilter)
+    o = O.OracleInstancedFlat(tris9, counts, pl, object_prefilter=prefilter)
+    for name in ("tris", "ranges", "blas", "blas_items", "blas_cov", "nodes", "items", "xforms"):
+        _same_bits(getattr(f, name), getattr(o, name))
+    assert f.object_cov == o.object_cov
+    assert (f.object_cov == len(f.blas)) if prefilter else (f.object_cov == O.NONE)
+
+
+def test_rows_bounds_and_blending():
+    geoms, tris9, counts, pl = _world()
+    f = Flat.build_instanced(tris9, counts, pl)
+    assert len(f.ranges) == len(pl) and len(f.xforms) == len(pl)
+    plain = Flat.build_arrays(tris9, counts)
+    for k, p in enumerate(pl):
+        g = int(p["geometry"])
+        assert list(f.ranges[k][:3]) == list(plain.ranges[g][:3]) and f.ranges[k][3] == k
+        r = p["rotation"].reshape(3, 3)
+        want = np.concatenate([np.concatenate([r, p["translation"][:, None]], axis=1).reshape(12),
+                               [p["scale"], np.float32(1.0) / p["scale"], p["clear"], 0]]).astype(np.float32)
+        _same_bits(f.xforms[k], want)
+    # the blended object's geometry carries its opacity in every node's samples
+    g4 = int(pl[4]["geometry"])
+    first_with_g4 = int(np.nonzero(pl["geometry"] == g4)[0][0])
+    root = plain.ranges[g4][2]
+    end = plain.ranges[g4 + 1][2] if g4 + 1 < len(counts) else len(plain.blas)
+    scale = np.float32(1.0) - pl[first_with_g4]["clear"] if first_with_g4 == 4 else np.float32(1.0)
+    _same_bits(f.blas_cov[root:end], (plain.blas_cov[root:end] * scale).astype(np.float32))
+    # leaves of the object hierarchy hold objects whose world bounds contain their vertices
+    off = np.concatenate([[0], np.cumsum(counts)]).astype(int)
+    node_of = {}
+    for n, node in enumerate(f.nodes):
+        if node[9] > 0:
+            for i in range(node[8], node[8] + node[9]):
+                node_of[int(f.items[i])] = n
+    assert sorted(node_of) == list(range(len(pl)))
+    for k, p in enumerate(pl):
+        g = int(p["geometry"])
+        v = tris9[off[g]:off[g + 1]].reshape(-1, 3).astype(np.float64)
+        w = v @ p["rotation"].reshape(3, 3).astype(np.float64).T * p["scale"] + p["translation"]
+        n = f.nodes[node_of[k]]
+        lo, hi = n[0:3].view(np.float32), n[4:7].view(np.float32)
+        assert np.all(w >= lo - 1e-4) and np.all(w <= hi + 1e-4)
+
+
+def test_lone_identity_instance_is_its_geometry():
+    geoms = [boulder((0.2, 0.1, 0.05), 24, 12)]
+    tris9, counts = meshes_to_arrays(geoms)
+    pl = np.zeros(1, dtype=PLACEMENT_DTYPE)
+    pl[0]["rotation"] = [1, 0, 0, 0, 1, 0, 0, 0, 1]
+    pl[0]["scale"] = 1.0
+    f = Flat.build_instanced(tris9, counts, pl, object_prefilter=True)
+    plain = Flat.build_arrays(tris9, counts)
+    for name in ("tris", "blas", "blas_items", "nodes", "items"):
+        _same_bits(getattr(f, name), getattr(plain, name))
+    _same_bits(f.blas_cov[: len(plain.blas)], plain.blas_cov)
+    # one object node: its coverage is the geometry root's, exactly at the samples
+    _same_bits(f.blas_cov[f.object_cov], plain.blas_cov[0])
+
+
+def test_object_coverage_is_the_placed_front_area():
+    geoms, tris9, counts, pl = _world(n=40, seed=3, quarter=True, blend=False)
+    f = Flat.build_instanced(tris9, counts, pl, object_prefilter=True)
+    cov = f.blas_cov[f.object_cov:]
+    assert len(cov) == len(f.nodes)
+    # interior nodes sum their children (children follow their parents)
+    for n, node in enumerate(f.nodes):
+        if node[9] == 0:
+            x = node[8]
+            assert np.allclose(cov[n], cov[x] + cov[x + 1], rtol=1e-6, atol=1e-6)
+    # the root: sum over objects of the placed triangles' front areas (f64)
+    dirs = np.array([[(n // 9) - 1, (n // 3 % 3) - 1, (n % 3) - 1] for n in range(27)], np.float64)
+    norms = np.linalg.norm(dirs, axis=1)
+    dirs[norms > 0] /= norms[norms > 0, None]
+    off = np.concatenate([[0], np.cumsum(counts)]).astype(int)
+    want = np.zeros(27)
+    for p in pl:
+        g = int(p["geometry"])
+        t = tris9[off[g]:off[g + 1]].reshape(-1, 3, 3).astype(np.float64)
+        r = p["rotation"].reshape(3, 3).astype(np.float64)
+        w = t @ r.T * float(p["scale"])
+        an = 0.5 * np.cross(w[:, 1] - w[:, 0], w[:, 2] - w[:, 0])
+        want += np.maximum(an @ dirs.T, 0.0).sum(0)
+    assert np.allclose(cov[0], want, rtol=2e-4, atol=1e-5), np.abs(cov[0] - want).max()
+
+
+def test_boulder_field_shape():
+    tris9, counts, pl, lights, eye, tgt = boulder_field(count=2000, half=40)
+    assert counts[0] == 2 and all(c == 10000 for c in counts[1:])
+    assert len(pl) == 2001 and pl[0]["geometry"] == 0 and np.all(pl["geometry"][1:] >= 1)
+    d = np.linalg.norm(pl["translation"][1:, :2] - eye[:2], axis=1)
+    assert np.all(d >= 4.0) and np.all(np.diff(d) >= -1e-3)   # nearest first
+    assert len(lights) == 32 and np.all(lights[:, 3] == 1.0)

@@ -143,6 +143,14 @@ def lib() -> C.CDLL:
     if _lib is None:
         if not LIB_PATH.exists():
             raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m continuum_amd.build`")
+        # PyTorch ships its own HIP runtime under the same soname. Loading it
+        # first makes the process hold one runtime, shared with the library,
+        # so torch's device buffers and streams are valid here (loaded the
+        # other way round, torch then finds no device).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(str(LIB_PATH))
         for name, (res, args) in SIGNATURES.items():
             f = getattr(L, name)

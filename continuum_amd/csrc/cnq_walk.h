@@ -496,19 +496,50 @@ __device__ __forceinline__ uint32_t node_cone_test(const Cone& c, Lane& L, float
   if (far_out || near_out || out_u0 || out_u1 || out_v0 || out_v1) return 0;
   if (!in_bound(c, (lo + hi) * 0.5f, 0.5f * len3(hi - lo))) return 0;
   if (!inside) return 3;
+  // Most nodes that get here project far wider than the prefilter's bins.
+  // A bound from approximate reciprocals settles those without the exact
+  // divisions, and only where it proves the exact extent exceeds the
+  // threshold: with a = px * rcp(pz) (within 3e-7 of px / pz relative) and
+  // G = GRID * rcp(w2 - w0), every exact projection lies within
+  // 4.4e-7 * G * (U + |w0|) of G * (a - w0), U bounding |a|; so an
+  // approximate extent above the threshold by the margins below is one the
+  // exact arithmetic also finds above it, and the result is the same.
+  {
+    const float du = c.w2 - c.w0, dv = c.w3 - c.w1;
+    if (du > 0.0f && dv > 0.0f) {
+      float ulo = 3.4e38f, uhi = -3.4e38f, vlo = 3.4e38f, vhi = -3.4e38f, umag = 0.0f, vmag = 0.0f;
+#pragma unroll
+      for (uint32_t k = 0; k < 8; k++) {
+        const f3 p = to_local(c, box_corner(lo, hi, k));
+        const float r = __builtin_amdgcn_rcpf(p.z);
+        const float u = p.x * r, v = p.y * r;
+        ulo = mn(ulo, u), uhi = mx(uhi, u), vlo = mn(vlo, v), vhi = mx(vhi, v);
+        umag = mx(umag, fabsf(u)), vmag = mx(vmag, fabsf(v));
+      }
+      const float gu = g * __builtin_amdgcn_rcpf(du), gv = g * __builtin_amdgcn_rcpf(dv);
+      const float eu = (uhi - ulo) * gu * (1.0f - 1e-6f) - 2e-6f * gu * (umag * 1.000001f + fabsf(c.w0));
+      const float ev = (vhi - vlo) * gv * (1.0f - 1e-6f) - 2e-6f * gv * (vmag * 1.000001f + fabsf(c.w1));
+      if (eu > prefilter_bins * 1.000001f || ev > prefilter_bins * 1.000001f) return 3;
+    }
+  }
   // Every corner between the near and far planes: their projections (the
   // same arithmetic the WGSL does in its corner loop, done only for nodes
-  // that survive the culls).
+  // that survive the culls). Most such nodes are too large to splat, so the
+  // extent is found first and the points are stored only for a splat.
 #pragma unroll
   for (uint32_t k = 0; k < 8; k++) {
     const f3 p = to_local(c, box_corner(lo, hi, k));
     const f2 q{(p.x / p.z - c.w0) / (c.w2 - c.w0) * g, (p.y / p.z - c.w1) / (c.w3 - c.w1) * g};
-    L.put2(P3A, k, q);
     mnv = f2{mn(mnv.x, q.x), mn(mnv.y, q.y)};
     mxv = f2{mx(mxv.x, q.x), mx(mxv.y, q.y)};
   }
   if (mx(mxv.x - mnv.x, mxv.y - mnv.y) > prefilter_bins) return 3;
   if (mxv.x <= 0.0f || mxv.y <= 0.0f || mnv.x >= g || mnv.y >= g) return 1;
+#pragma unroll
+  for (uint32_t k = 0; k < 8; k++) {
+    const f3 p = to_local(c, box_corner(lo, hi, k));
+    L.put2(P3A, k, f2{(p.x / p.z - c.w0) / (c.w2 - c.w0) * g, (p.y / p.z - c.w1) / (c.w3 - c.w1) * g});
+  }
   f3 e = hi - lo;
   f3 v = norm3(c.apex - (lo + hi) * 0.5f);
   float silhouette = fabsf(v.x) * e.y * e.z + fabsf(v.y) * e.x * e.z + fabsf(v.z) * e.x * e.y;
@@ -652,12 +683,16 @@ __device__ __forceinline__ bool splat_object_node(const Ctx& x, const Cone& c, W
     f3 p = to_local(c, corner);
     if (p.z < NEAR) return false;
     f2 q{(p.x / p.z - c.w0) / (c.w2 - c.w0) * g, (p.y / p.z - c.w1) / (c.w3 - c.w1) * g};
-    L.put2(P3A, k, q);
     mnv = f2{mn(mnv.x, q.x), mn(mnv.y, q.y)};
     mxv = f2{mx(mxv.x, q.x), mx(mxv.y, q.y)};
   }
   if (mx(mxv.x - mnv.x, mxv.y - mnv.y) > w.prefilter_bins) return false;
   if (mxv.x <= 0.0f || mxv.y <= 0.0f || mnv.x >= g || mnv.y >= g) return true;
+  // A splat: the hull's points (as computed above) into the workspace.
+  for (uint32_t k = 0; k < 8; k++) {
+    const f3 p = to_local(c, box_corner(lo, hi, k));
+    L.put2(P3A, k, f2{(p.x / p.z - c.w0) / (c.w2 - c.w0) * g, (p.y / p.z - c.w1) / (c.w3 - c.w1) * g});
+  }
   f3 e = hi - lo;
   f3 v = norm3(c.apex - (lo + hi) * 0.5f);
   float silhouette = fabsf(v.x) * e.y * e.z + fabsf(v.y) * e.x * e.z + fabsf(v.z) * e.x * e.y;

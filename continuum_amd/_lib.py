@@ -67,6 +67,11 @@ class ServerConfig(C.Structure):
                 ("tuning", Tuning)]
 
 
+class MoverTree(C.Structure):
+    _fields_ = [("fixed_root", C.c_uint32 * 12), ("first_node", C.c_uint32), ("nodes", C.c_uint32),
+                ("first_item", C.c_uint32), ("n_objects", C.c_uint32), ("objects", C.c_void_p)]
+
+
 class RequestInfo(C.Structure):
     _fields_ = [("state", C.c_int32), ("width", C.c_uint32), ("height", C.c_uint32), ("n_lights", C.c_uint32),
                 ("launch", C.c_uint64), ("queued_ms", C.c_double), ("latency_ms", C.c_double),
@@ -88,6 +93,12 @@ SIGNATURES = {
     "cnq_flat_copy": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P]),
     "cnq_flat_destroy": (None, [_P]),
     "cnq_flat_build_instanced": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_uint64, C.c_int, C.POINTER(_P)]),
+    "cnq_flat_build_instanced_movers": (
+        C.c_int, [_P, _P, C.c_uint32, _P, C.c_uint64, _P, C.c_uint32, C.c_int, C.POINTER(_P)]),
+    "cnq_flat_mover_tree": (C.c_int, [_P, C.POINTER(MoverTree)]),
+    "cnq_scene_set_movers": (C.c_int, [_P, C.POINTER(MoverTree), _P, C.c_uint32, _P]),
+    "cnq_scene_set_transform": (C.c_int, [_P, C.c_uint32, _P, _P]),
+    "cnq_scene_apply_moves": (C.c_int, [_P, _P]),
     "cnq_flat_instanced_sizes": (C.c_int, [_P, _P]),
     "cnq_flat_copy_instanced": (C.c_int, [_P, _P, _P]),
     "cnq_scene_create": (C.c_int, [_P, C.POINTER(SceneDesc), C.POINTER(_P)]),

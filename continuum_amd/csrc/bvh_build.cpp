@@ -341,9 +341,83 @@ float front_area(const float* cov, const float* v) {
 
 }  // namespace
 
+Box world_bounds(const Placement& o, const Box& lb) {
+  Box w;
+  for (int a = 0; a < 3; a++) {
+    w.lo[a] = FMAX;
+    w.hi[a] = -FMAX;
+  }
+  for (int c = 0; c < 8; c++) {
+    const float p[3] = {(c & 1) ? lb.hi[0] : lb.lo[0], (c & 2) ? lb.hi[1] : lb.lo[1], (c & 4) ? lb.hi[2] : lb.lo[2]};
+    float q[3];
+    mat_vec(o.rotation, p, q);
+    for (int a = 0; a < 3; a++) {
+      const float x = q[a] * o.scale + o.translation[a];
+      w.lo[a] = gmin(w.lo[a], x);
+      w.hi[a] = gmax(w.hi[a], x);
+    }
+  }
+  return padded(w);
+}
+
+NodeGpu union_root(const NodeGpu& a, const NodeGpu& b) {
+  NodeGpu r{};
+  for (int k = 0; k < 3; k++) {
+    r.lo[k] = gmin(a.lo[k], b.lo[k]);
+    r.hi[k] = gmax(a.hi[k], b.hi[k]);
+  }
+  r.info[0] = 1;
+  r.info[2] = 1;  // NO_SPLAT
+  return r;
+}
+
+void mover_subtree(const MoverTree& tree, const std::vector<Box>& bounds, NodeGpu& root, std::vector<NodeGpu>& region,
+                   std::vector<uint32_t>& items) {
+  std::vector<NodeGpu> local(1, NodeGpu{});
+  std::vector<uint32_t> order(bounds.size());
+  for (size_t k = 0; k < order.size(); k++) order[k] = (uint32_t)k;
+  split_tree(local, 0, bounds, order.data(), order.size(), 2, tree.first_node - 1, tree.first_item);
+  for (NodeGpu& n : local) n.info[2] = 1;  // NO_SPLAT: their boxes change
+  items.resize(order.size());
+  for (size_t k = 0; k < order.size(); k++) items[k] = tree.objects[order[k]];
+  root = local[0];
+  region.assign(local.begin() + 1, local.end());
+}
+
+void object_hierarchy_with_movers(const std::vector<Box>& bounds, const std::vector<uint32_t>& movable,
+                                  std::vector<NodeGpu>& nodes, std::vector<uint32_t>& items, MoverTree& tree) {
+  std::vector<uint8_t> keep(bounds.size(), 1);
+  for (uint32_t m : movable) keep[m] = 0;
+  std::vector<NodeGpu> fixed;
+  object_hierarchy(bounds, fixed, items, keep.data());
+  auto place = [](uint32_t i) { return i == 0 ? 1u : i + 2; };
+  tree = MoverTree{};
+  tree.first_node = (uint32_t)fixed.size() + 2;
+  tree.first_item = (uint32_t)items.size();
+  tree.objects = movable;
+  std::vector<Box> mover_bounds;
+  for (uint32_t m : movable) mover_bounds.push_back(bounds[m]);
+  NodeGpu root;
+  std::vector<NodeGpu> region;
+  std::vector<uint32_t> mover_items;
+  mover_subtree(tree, mover_bounds, root, region, mover_items);
+  tree.nodes = (uint32_t)region.size();
+  nodes.assign(tree.first_node, NodeGpu{});
+  for (size_t i = 0; i < fixed.size(); i++) {
+    NodeGpu n = fixed[i];
+    if (n.info[1] == 0) n.info[0] = place(n.info[0]);
+    nodes[place((uint32_t)i)] = n;
+  }
+  nodes[2] = root;
+  nodes.insert(nodes.end(), region.begin(), region.end());
+  nodes[0] = union_root(nodes[1], nodes[2]);
+  tree.fixed_root = nodes[1];
+  items.insert(items.end(), mover_items.begin(), mover_items.end());
+}
+
 void flatten_instanced(const float* tris9, const uint32_t* counts, uint32_t n_geometries, const Placement* objects,
                        size_t n_objects, bool object_prefilter, Flat& f, std::vector<float>& xforms,
-                       uint32_t& object_cov) {
+                       uint32_t& object_cov, const uint32_t* movable, size_t n_movable, MoverTree* tree) {
   flatten(tris9, counts, n_geometries, f);
   const size_t n_blas = f.blas.size();
   // A blended geometry's coverage scales by its first object's opacity.
@@ -387,27 +461,15 @@ void flatten_instanced(const float* tris9, const uint32_t* counts, uint32_t n_ge
     const float* r = o.rotation;
     xforms.insert(xforms.end(), {r[0], r[1], r[2], o.translation[0], r[3], r[4], r[5], o.translation[1], r[6], r[7],
                                  r[8], o.translation[2], o.scale, 1.0f / o.scale, o.clear, 0.0f});
-    Box w;
-    for (int a = 0; a < 3; a++) {
-      w.lo[a] = FMAX;
-      w.hi[a] = -FMAX;
-    }
-    const Box& lb = local[o.geometry];
-    for (int c = 0; c < 8; c++) {
-      const float p[3] = {(c & 1) ? lb.hi[0] : lb.lo[0], (c & 2) ? lb.hi[1] : lb.lo[1], (c & 4) ? lb.hi[2] : lb.lo[2]};
-      float q[3];
-      mat_vec(r, p, q);
-      for (int a = 0; a < 3; a++) {
-        const float x = q[a] * o.scale + o.translation[a];
-        w.lo[a] = gmin(w.lo[a], x);
-        w.hi[a] = gmax(w.hi[a], x);
-      }
-    }
-    bounds[k] = padded(w);
+    bounds[k] = world_bounds(o, local[o.geometry]);
   }
   if (f.ranges.empty()) f.ranges.insert(f.ranges.end(), {0, 0, NONE_U32, NONE_U32});
   if (xforms.empty()) xforms.assign(16, 0.0f);
-  object_hierarchy(bounds, f.nodes, f.items);
+  if (n_movable && tree) {
+    object_hierarchy_with_movers(bounds, std::vector<uint32_t>(movable, movable + n_movable), f.nodes, f.items, *tree);
+  } else {
+    object_hierarchy(bounds, f.nodes, f.items);
+  }
   object_cov = NONE_U32;
   if (!object_prefilter) return;
   // Each object's 27 samples from its geometry's root coverage, seen from

@@ -62,9 +62,31 @@ struct Placement {
 // the object hierarchy's coverage samples (`object_coverage`,
 // hierarchy.rs:661-700) appended to blas_cov from row `object_cov` (else
 // object_cov = NONE). xforms: 16 floats per object (`Placement::rows`).
+// With movable objects (lib.rs:1986-1993), the object hierarchy is
+// `object_hierarchy_with_movers` and `tree` describes the movers' subtree.
 void flatten_instanced(const float* tris9, const uint32_t* counts, uint32_t n_geometries, const Placement* objects,
                        size_t n_objects, bool object_prefilter, Flat& out, std::vector<float>& xforms,
-                       uint32_t& object_cov);
+                       uint32_t& object_cov, const uint32_t* movable = nullptr, size_t n_movable = 0,
+                       struct MoverTree* tree = nullptr);
+
+// Where the movable objects' subtree lives (hierarchy.rs:308-325
+// `MoverTree`): its root is node 2, its other nodes `nodes` from
+// `first_node`, its items from `first_item`; node 1 roots the static objects.
+struct MoverTree {
+  NodeGpu fixed_root;
+  uint32_t first_node = 0, nodes = 0, first_item = 0;
+  std::vector<uint32_t> objects;
+};
+
+// `object_hierarchy_with_movers` (hierarchy.rs:331-372), `union_root`
+// (:374-384) and `mover_subtree` (:388-411).
+void object_hierarchy_with_movers(const std::vector<Box>& bounds, const std::vector<uint32_t>& movable,
+                                  std::vector<NodeGpu>& nodes, std::vector<uint32_t>& items, MoverTree& tree);
+NodeGpu union_root(const NodeGpu& a, const NodeGpu& b);
+void mover_subtree(const MoverTree& tree, const std::vector<Box>& mover_bounds, NodeGpu& root,
+                   std::vector<NodeGpu>& region, std::vector<uint32_t>& items);
+// A placement's padded world bounds of an object-space box (hierarchy.rs:288-301).
+Box world_bounds(const Placement& p, const Box& local);
 
 }  // namespace host
 }  // namespace cnq

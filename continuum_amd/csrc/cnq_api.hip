@@ -324,6 +324,53 @@ int cnq_flat_build_instanced(const float* tris9, const uint32_t* counts, uint32_
   return CNQ_OK;
 }
 
+int cnq_flat_build_instanced_movers(const float* tris9, const uint32_t* counts, uint32_t n_geometries,
+                                    const cnq_placement* objects, uint64_t n_objects, const uint32_t* movable,
+                                    uint32_t n_movable, int object_prefilter, cnq_flat** out) {
+  if (!out || (n_geometries && !counts) || (n_objects && !objects) || (n_movable && !movable))
+    return fail(CNQ_INVALID_ARG, "null argument");
+  if (n_movable == 0) return cnq_flat_build_instanced(tris9, counts, n_geometries, objects, n_objects, object_prefilter, out);
+  std::vector<uint8_t> seen(n_objects, 0);
+  for (uint32_t k = 0; k < n_movable; k++) {
+    if (movable[k] >= n_objects || seen[movable[k]]) return fail(CNQ_INVALID_ARG, "movable objects must be distinct objects");
+    seen[movable[k]] = 1;
+  }
+  if (n_movable >= n_objects) return fail(CNQ_INVALID_ARG, "a scene with movers needs a static object");
+  uint64_t total = 0;
+  for (uint32_t g = 0; g < n_geometries; g++) total += counts[g];
+  if (total && !tris9) return fail(CNQ_INVALID_ARG, "tris is null");
+  if (total > 0x3fffffffull || n_objects > 0x3fffffffull) return fail(CNQ_CAPACITY, "too many triangles or objects");
+  for (uint64_t k = 0; k < n_objects; k++) {
+    if (objects[k].geometry >= n_geometries) return fail(CNQ_INVALID_ARG, "object " + std::to_string(k) + ": no such geometry");
+    if (!(objects[k].scale > 0.0f)) return fail(CNQ_INVALID_ARG, "object " + std::to_string(k) + ": scale must be > 0");
+  }
+  cnq_flat* f = new (std::nothrow) cnq_flat;
+  if (!f) return fail(CNQ_OOM, "out of host memory");
+  try {
+    cnq::host::flatten_instanced(tris9, counts, n_geometries, reinterpret_cast<const cnq::host::Placement*>(objects),
+                                 n_objects, object_prefilter != 0, f->f, f->xforms, f->object_cov, movable, n_movable,
+                                 &f->movers);
+  } catch (const std::bad_alloc&) {
+    delete f;
+    return fail(CNQ_OOM, "out of host memory");
+  }
+  f->has_movers = true;
+  *out = f;
+  return CNQ_OK;
+}
+
+int cnq_flat_mover_tree(const cnq_flat* flat, cnq_mover_tree* out) {
+  if (!flat || !out) return fail(CNQ_INVALID_ARG, "null argument");
+  if (!flat->has_movers) return fail(CNQ_INVALID_ARG, "the flat has no movers");
+  std::memcpy(out->fixed_root, &flat->movers.fixed_root, 48);
+  out->first_node = flat->movers.first_node;
+  out->nodes = flat->movers.nodes;
+  out->first_item = flat->movers.first_item;
+  out->n_objects = (uint32_t)flat->movers.objects.size();
+  out->objects = flat->movers.objects.data();
+  return CNQ_OK;
+}
+
 int cnq_flat_instanced_sizes(const cnq_flat* flat, uint64_t sizes[3]) {
   if (!flat || !sizes) return fail(CNQ_INVALID_ARG, "null argument");
   sizes[0] = flat->xforms.size() / 16;
@@ -577,6 +624,102 @@ int cnq_render_view(cnq_scene* s, const float* camera, uint32_t width, uint32_t 
   }
   cleanup();
   return rc;
+}
+
+int cnq_scene_set_movers(cnq_scene* s, const cnq_mover_tree* t, const float* geometry_bounds, uint32_t n_geometries,
+                         const cnq_placement* placements) {
+  if (!s || !t || !geometry_bounds || (t->n_objects && (!t->objects || !placements)))
+    return fail(CNQ_INVALID_ARG, "null argument");
+  if (!s->instanced) return fail(CNQ_INVALID_ARG, "movers need an instanced scene");
+  if (t->n_objects == 0) return fail(CNQ_INVALID_ARG, "no movers");
+  if ((uint64_t)t->first_node + t->nodes > s->n_nodes || (uint64_t)t->first_item + t->n_objects > s->n_items ||
+      s->n_nodes < 3)
+    return fail(CNQ_INVALID_ARG, "mover tree outside the object hierarchy");
+  if (t->first_node < 3) return fail(CNQ_INVALID_ARG, "the movers' nodes follow nodes 0-2");
+  std::vector<uint32_t> mover_of(s->n_objects, NONE);
+  for (uint32_t k = 0; k < t->n_objects; k++) {
+    const uint32_t m = t->objects[k];
+    if (m >= s->n_objects || mover_of[m] != NONE) return fail(CNQ_INVALID_ARG, "movers must be distinct objects");
+    if (s->h_ranges[4 * m + 3] == NONE) return fail(CNQ_INVALID_ARG, "a mover needs a transform");
+    if (placements[k].geometry >= n_geometries || !(placements[k].scale > 0.0f))
+      return fail(CNQ_INVALID_ARG, "mover " + std::to_string(k) + ": bad placement");
+    mover_of[m] = k;
+  }
+  s->movers = cnq::host::MoverTree{};
+  std::memcpy(&s->movers.fixed_root, t->fixed_root, 48);
+  s->movers.first_node = t->first_node;
+  s->movers.nodes = t->nodes;
+  s->movers.first_item = t->first_item;
+  s->movers.objects.assign(t->objects, t->objects + t->n_objects);
+  s->geometry_bounds.resize(n_geometries);
+  for (uint32_t g = 0; g < n_geometries; g++) {
+    std::memcpy(s->geometry_bounds[g].lo, geometry_bounds + 6 * (size_t)g, 12);
+    std::memcpy(s->geometry_bounds[g].hi, geometry_bounds + 6 * (size_t)g + 3, 12);
+  }
+  s->mover_placements.assign(reinterpret_cast<const cnq::host::Placement*>(placements),
+                             reinterpret_cast<const cnq::host::Placement*>(placements) + t->n_objects);
+  s->mover_dirty.assign(t->n_objects, 0);
+  s->mover_of = std::move(mover_of);
+  s->has_movers = true;
+  return CNQ_OK;
+}
+
+int cnq_scene_set_transform(cnq_scene* s, uint32_t object, const float* rotation, const float* translation) {
+  if (!s || !rotation || !translation) return fail(CNQ_INVALID_ARG, "null argument");
+  if (!s->has_movers) return fail(CNQ_INVALID_ARG, "the scene has no movers");
+  if (object >= s->n_objects || s->mover_of[object] == NONE)
+    return fail(CNQ_INVALID_ARG, "object " + std::to_string(object) + " is not movable");
+  cnq::host::Placement& p = s->mover_placements[s->mover_of[object]];
+  std::memcpy(p.rotation, rotation, 36);
+  std::memcpy(p.translation, translation, 12);
+  s->mover_dirty[s->mover_of[object]] = 1;
+  return CNQ_OK;
+}
+
+int cnq_scene_apply_moves(cnq_scene* s, void* stream) {
+  if (!s) return fail(CNQ_INVALID_ARG, "scene is null");
+  if (!s->has_movers) return fail(CNQ_INVALID_ARG, "the scene has no movers");
+  bool any = false;
+  for (uint8_t d : s->mover_dirty) any = any || d;
+  if (!any) return CNQ_OK;
+  HIP_TRY(hipSetDevice(s->dev->ordinal), "hipSetDevice");
+  hipStream_t st = (hipStream_t)stream;
+  const cnq::host::MoverTree& t = s->movers;
+  std::vector<cnq::host::Box> bounds(t.objects.size());
+  for (size_t k = 0; k < t.objects.size(); k++) {
+    const cnq::host::Placement& p = s->mover_placements[k];
+    bounds[k] = cnq::host::world_bounds(p, s->geometry_bounds[p.geometry]);
+  }
+  for (size_t k = 0; k < t.objects.size(); k++) {
+    if (!s->mover_dirty[k]) continue;
+    s->mover_dirty[k] = 0;
+    const cnq::host::Placement& p = s->mover_placements[k];
+    const float* r = p.rotation;
+    const float rows[16] = {r[0], r[1], r[2], p.translation[0], r[3], r[4], r[5], p.translation[1],
+                            r[6], r[7], r[8], p.translation[2], p.scale, 1.0f / p.scale, p.clear, 0.0f};
+    const uint32_t x = s->h_ranges[4 * t.objects[k] + 3];
+    HIP_TRY(hipMemcpyAsync(s->xforms.p + 16 * (size_t)x, rows, 64, hipMemcpyHostToDevice, st), "mover transform");
+    HIP_TRY(hipStreamSynchronize(st), "mover transform");
+  }
+  cnq::host::NodeGpu root;
+  std::vector<cnq::host::NodeGpu> region;
+  std::vector<uint32_t> items;
+  cnq::host::mover_subtree(t, bounds, root, region, items);
+  if (region.size() != t.nodes) return fail(CNQ_INVALID_ARG, "movers' subtree changed shape");
+  const cnq::host::NodeGpu top = cnq::host::union_root(t.fixed_root, root);
+  const NodeC ctop = compact(reinterpret_cast<const uint32_t*>(&top), true);
+  const NodeC croot = compact(reinterpret_cast<const uint32_t*>(&root), true);
+  std::vector<NodeC> cregion(region.size());
+  for (size_t n = 0; n < region.size(); n++) cregion[n] = compact(reinterpret_cast<const uint32_t*>(&region[n]), true);
+  HIP_TRY(hipMemcpyAsync(s->nodes.p, &ctop, sizeof(NodeC), hipMemcpyHostToDevice, st), "root node");
+  HIP_TRY(hipMemcpyAsync(s->nodes.p + 2, &croot, sizeof(NodeC), hipMemcpyHostToDevice, st), "movers' root");
+  if (!cregion.empty())
+    HIP_TRY(hipMemcpyAsync(s->nodes.p + t.first_node, cregion.data(), cregion.size() * sizeof(NodeC),
+                           hipMemcpyHostToDevice, st), "movers' nodes");
+  HIP_TRY(hipMemcpyAsync(s->items.p + t.first_item, items.data(), items.size() * 4, hipMemcpyHostToDevice, st),
+          "movers' items");
+  HIP_TRY(hipStreamSynchronize(st), "apply moves");
+  return CNQ_OK;
 }
 
 int cnq_scene_refit_device(cnq_scene* s, const float* tris, void* stream) {

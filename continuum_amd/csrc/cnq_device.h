@@ -147,6 +147,33 @@ __device__ __forceinline__ bool in_bound(const Cone& c, f3 p, float r) {
   return dot3(c.baxis, d) >= limit * len;
 }
 
+// in_bound for a box's bounding sphere, settled by approximate square roots
+// and reciprocals where they prove the exact answer: 1 (true), 0 (false),
+// 2 (too close to call: use in_bound). v_sqrt_f32 and v_rcp_f32 are within
+// 1 ulp, so `la`, `ra` are within 2.4e-7 of the exact len and r relative
+// (the margins below allow 4e-6), and with 1 - s^2 >= 1e-3 the
+// approximate limit * len is within 2.5e-5 (|cos| + |sin|) len of the exact
+// one (the margin allows 1e-4).
+__device__ __forceinline__ int in_bound_quick(const Cone& c, f3 lo, f3 hi) {
+  if (c.bcos < -1.0f) return 1;
+  const f3 e = hi - lo;
+  const f3 d = (lo + hi) * 0.5f - c.apex;
+  const float r2 = dot3(e, e), l2 = dot3(d, d);
+  if (!(r2 > 1e-30f && l2 > 1e-30f && r2 < 1e30f && l2 < 1e30f)) return 2;
+  const float ra = 0.5f * __builtin_amdgcn_sqrtf(r2), la = __builtin_amdgcn_sqrtf(l2);
+  if (la <= ra * (1.0f - 4e-6f)) return 1;
+  if (!(la > ra * (1.0f + 4e-6f))) return 2;
+  const float s = ra * __builtin_amdgcn_rcpf(la);
+  const float om = 1.0f - s * s;
+  if (!(om >= 1e-3f)) return 2;
+  const float rhs = (c.bcos * __builtin_amdgcn_sqrtf(om) - c.bsin * s) * la;
+  const float margin = 1e-4f * (fabsf(c.bcos) + fabsf(c.bsin)) * la;
+  const float lhs = dot3(c.baxis, d);
+  if (lhs >= rhs + margin) return 1;
+  if (lhs < rhs - margin) return 0;
+  return 2;
+}
+
 // WGSL :174-186
 __device__ __forceinline__ void bound_cone(Cone& c, f3 axis) {
   float cos_half = 1.0f;
@@ -176,6 +203,10 @@ __device__ __forceinline__ f3 box_corner(f3 lo, f3 hi, uint32_t k) {
 
 // WGSL :234-255 (no horizon: light cones have none)
 __device__ __forceinline__ bool may_touch_box(const Cone& c, f3 lo, f3 hi) {
+  // The bounding-cone test first (a cheap bound settles most boxes; both
+  // tests must pass, so their order does not change the answer).
+  const int quick = in_bound_quick(c, lo, hi);
+  if (quick == 0) return false;
   bool far_out = true, near_out = true, out_u0 = true, out_u1 = true, out_v0 = true, out_v1 = true;
 #pragma unroll
   for (uint32_t k = 0; k < 8; k++) {
@@ -189,7 +220,7 @@ __device__ __forceinline__ bool may_touch_box(const Cone& c, f3 lo, f3 hi) {
     if (c.w3 * p.z - p.y >= 0.0f) out_v1 = false;
   }
   if (far_out || near_out || out_u0 || out_u1 || out_v0 || out_v1) return false;
-  return in_bound(c, (lo + hi) * 0.5f, 0.5f * len3(hi - lo));
+  return quick == 1 || in_bound(c, (lo + hi) * 0.5f, 0.5f * len3(hi - lo));
 }
 
 // WGSL :213-232

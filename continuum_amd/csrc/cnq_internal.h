@@ -69,6 +69,15 @@ struct cnq_scene {
   cnq::detail::DevBuf<uint64_t> work;
   cnq::detail::DevBuf<cnq::NodeC> blas, nodes;
   std::vector<uint32_t> h_ranges;  // for host-side checks and object rebuilds
+  // Moving objects (instanced scenes): the subtree's layout, each geometry's
+  // object-space bounds, the movers' placements and which moved since the
+  // last apply.
+  bool has_movers = false;
+  cnq::host::MoverTree movers;
+  std::vector<cnq::host::Box> geometry_bounds;
+  std::vector<cnq::host::Placement> mover_placements;
+  std::vector<uint8_t> mover_dirty;
+  std::vector<uint32_t> mover_of;  // per object: its index among the movers, or NONE
   bool instanced = false;
 
   cnq::SceneArgs args() const {
@@ -101,6 +110,8 @@ struct cnq_flat {
   cnq::host::Flat f;
   std::vector<float> xforms;      // instanced worlds: 16 per object
   uint32_t object_cov = 0xffffffffu;
+  bool has_movers = false;
+  cnq::host::MoverTree movers;
 };
 
 namespace cnq {

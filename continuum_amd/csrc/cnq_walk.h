@@ -475,6 +475,9 @@ __device__ __forceinline__ uint32_t node_cone_test(const Cone& c, Lane& L, float
   f2 mnv{3.4e38f, 3.4e38f}, mxv{-3.4e38f, -3.4e38f};
   bool far_out = true, near_out = true, out_u0 = true, out_u1 = true, out_v0 = true, out_v1 = true;
   bool inside = prefilter_bins > 0.0f;
+  // The bounding-cone test first (see may_touch_box).
+  const int quick = in_bound_quick(c, lo, hi);
+  if (quick == 0) return 0;
 #pragma unroll
   for (uint32_t k = 0; k < 8; k++) {
     f3 corner = box_corner(lo, hi, k);
@@ -494,7 +497,7 @@ __device__ __forceinline__ uint32_t node_cone_test(const Cone& c, Lane& L, float
     if (c.w3 * p.z - p.y >= 0.0f) out_v1 = false;
   }
   if (far_out || near_out || out_u0 || out_u1 || out_v0 || out_v1) return 0;
-  if (!in_bound(c, (lo + hi) * 0.5f, 0.5f * len3(hi - lo))) return 0;
+  if (quick == 2 && !in_bound(c, (lo + hi) * 0.5f, 0.5f * len3(hi - lo))) return 0;
   if (!inside) return 3;
   // Most nodes that get here project far wider than the prefilter's bins.
   // A bound from approximate reciprocals settles those without the exact

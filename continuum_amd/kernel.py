@@ -63,6 +63,7 @@ class Flat:
                        np.ascontiguousarray(xforms, dtype=np.float32).reshape(-1, 16))
         # First row of the object hierarchy's coverage in blas_cov (NONE: none).
         self.object_cov = int(object_cov)
+        self.movers = None  # the movers' subtree layout, for instanced worlds with movers
 
     @classmethod
     def build(cls, meshes) -> "Flat":
@@ -91,11 +92,13 @@ class Flat:
         return cls(**out)
 
     @classmethod
-    def build_instanced(cls, tris9: np.ndarray, counts, placements, object_prefilter: bool = True) -> "Flat":
+    def build_instanced(cls, tris9: np.ndarray, counts, placements, object_prefilter: bool = True,
+                        movable: Optional[Sequence[int]] = None) -> "Flat":
         """The instanced world (`flatten_instanced`, hierarchy.rs:464): geometries
         as 9 floats per triangle, `counts[g]` each; placements as
         (geometry, rotation 3x3 row-major, scale, translation, clear) tuples
-        or a `Placement` array."""
+        or a `Placement` array; `movable` objects get their own subtree
+        (`object_hierarchy_with_movers`, hierarchy.rs:331)."""
         L = lib()
         tris9 = np.ascontiguousarray(tris9, dtype=np.float32)
         counts = np.ascontiguousarray(counts, dtype=np.uint32)
@@ -115,8 +118,10 @@ class Flat:
                 arr[k].clear = float(clear)
             ptr, n = C.cast(arr, C.c_void_p), len(arr)
         h = C.c_void_p()
-        check(L.cnq_flat_build_instanced(_ptr(tris9), _ptr(counts), len(counts), ptr, n, int(object_prefilter),
-                                         C.byref(h)))
+        mv = np.ascontiguousarray(list(movable or []), dtype=np.uint32)
+        check(L.cnq_flat_build_instanced_movers(_ptr(tris9), _ptr(counts), len(counts), ptr, n, _ptr(mv), len(mv),
+                                                int(object_prefilter), C.byref(h)))
+        tree = None
         try:
             sizes = np.zeros(6, dtype=np.uint64)
             check(L.cnq_flat_sizes(h, _ptr(sizes)))
@@ -132,9 +137,22 @@ class Flat:
             xforms = np.zeros((nx, 16), np.float32)
             cov = np.zeros((ncov, 27), np.float32)
             check(L.cnq_flat_copy_instanced(h, _ptr(xforms), _ptr(cov)))
+            if len(mv):
+                t = _lib.MoverTree()
+                check(L.cnq_flat_mover_tree(h, C.byref(t)))
+                objs = np.ctypeslib.as_array((C.c_uint32 * t.n_objects).from_address(t.objects)).copy()
+                tree = dict(fixed_root=np.array(t.fixed_root, np.uint32), first_node=t.first_node, nodes=t.nodes,
+                            first_item=t.first_item, objects=objs)
         finally:
             L.cnq_flat_destroy(h)
-        return cls(blas_cov=cov, xforms=xforms, object_cov=ocov, **out)
+        f = cls(blas_cov=cov, xforms=xforms, object_cov=ocov, **out)
+        f.movers = tree
+        if tree is not None:
+            pl = placements if isinstance(placements, np.ndarray) else None
+            f.mover_placements = None if pl is None else np.ascontiguousarray(pl[tree["objects"]], PLACEMENT_DTYPE)
+            f.geometry_bounds = np.stack([np.concatenate([t.reshape(-1, 3).min(0), t.reshape(-1, 3).max(0)])
+                                          for t in np.split(tris9, np.cumsum(counts)[:-1])]).astype(np.float32)
+        return f
 
     def desc(self) -> SceneDesc:
         d = SceneDesc()
@@ -177,6 +195,27 @@ class Scene:
         check(lib().cnq_scene_create(device.handle, C.byref(d), C.byref(h)))
         self.handle = h
         self._destroy = lib().cnq_scene_destroy
+        if getattr(flat, "movers", None) is not None and getattr(flat, "mover_placements", None) is not None:
+            t = flat.movers
+            mt = _lib.MoverTree()
+            mt.fixed_root[:] = [int(x) for x in t["fixed_root"]]
+            mt.first_node, mt.nodes, mt.first_item = t["first_node"], t["nodes"], t["first_item"]
+            self._mover_objects = np.ascontiguousarray(t["objects"], np.uint32)
+            mt.n_objects, mt.objects = len(self._mover_objects), _ptr(self._mover_objects)
+            gb = np.ascontiguousarray(flat.geometry_bounds, np.float32)
+            check(lib().cnq_scene_set_movers(self.handle, C.byref(mt), _ptr(gb), len(gb),
+                                             _ptr(flat.mover_placements)))
+
+    def set_transform(self, obj: int, rotation, translation) -> None:
+        """Renderer::set_transform (renderer.rs:442) for a movable object."""
+        r = np.ascontiguousarray(rotation, np.float32).reshape(9)
+        t = np.ascontiguousarray(translation, np.float32).reshape(3)
+        check(lib().cnq_scene_set_transform(self.handle, int(obj), _ptr(r), _ptr(t)))
+
+    def apply_moves(self, stream=None) -> None:
+        """Write moved objects' transforms and rebuild the movers' subtree
+        (instanced.rs:1226-1410)."""
+        check(lib().cnq_scene_apply_moves(self.handle, _stream_ptr(stream)))
 
     def __del__(self):
         if getattr(self, "handle", None):

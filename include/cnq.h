@@ -105,6 +105,25 @@ typedef struct cnq_placement {
  * arrays as for flatten (blas_cov: the triangle hierarchies' rows); then: */
 int cnq_flat_build_instanced(const float* tris9, const uint32_t* counts, uint32_t n_geometries,
                              const cnq_placement* objects, uint64_t n_objects, int object_prefilter, cnq_flat** out);
+/* The same with `movable` objects (lib.rs:1986-1993): the object hierarchy
+ * is `object_hierarchy_with_movers` (hierarchy.rs:331-372): node 1 roots
+ * the static objects, node 2 the movers' subtree (rebuilt as they move). */
+int cnq_flat_build_instanced_movers(const float* tris9, const uint32_t* counts, uint32_t n_geometries,
+                                    const cnq_placement* objects, uint64_t n_objects, const uint32_t* movable,
+                                    uint32_t n_movable, int object_prefilter, cnq_flat** out);
+
+/* hierarchy.rs:308-325 `MoverTree`. */
+typedef struct cnq_mover_tree {
+  uint32_t fixed_root[12];  /* the static objects' root (node 1), NodeGpu */
+  uint32_t first_node;      /* the movers' nodes after node 2 */
+  uint32_t nodes;
+  uint32_t first_item;      /* the movers' items */
+  uint32_t n_objects;
+  const uint32_t* objects;  /* the movable objects, in subtree order */
+} cnq_mover_tree;
+
+/* The flat's mover tree (`objects` points into the flat). */
+int cnq_flat_mover_tree(const cnq_flat* flat, cnq_mover_tree* out);
 /* sizes: n_xforms, coverage rows in blas_cov (with the object hierarchy's),
  * object_cov (the object hierarchy's first row; CNQ_NONE: none). */
 int cnq_flat_instanced_sizes(const cnq_flat* flat, uint64_t sizes[3]);
@@ -130,6 +149,19 @@ int cnq_scene_create(cnq_device* dev, const cnq_scene_desc* desc, cnq_scene** ou
 void cnq_scene_destroy(cnq_scene* scene);
 /* Bytes of HBM the scene holds. */
 uint64_t cnq_scene_bytes(const cnq_scene* scene);
+
+/* ---- Moving objects of an instanced scene (the reference's
+ * `Renderer::set_transform`, renderer.rs:442, applied by `apply_moves`,
+ * instanced.rs:1226-1410). set_movers gives the scene its mover tree, each
+ * geometry's object-space bounds (6 floats: lo, hi) and the movers'
+ * current placements (tree order). set_transform gives a mover a new
+ * rotation (row-major) and translation; apply_moves writes the moved
+ * objects' transforms and rebuilds the movers' subtree (`mover_subtree`,
+ * `union_root`, hierarchy.rs:374-411) on `stream`. */
+int cnq_scene_set_movers(cnq_scene* scene, const cnq_mover_tree* tree, const float* geometry_bounds,
+                         uint32_t n_geometries, const cnq_placement* placements);
+int cnq_scene_set_transform(cnq_scene* scene, uint32_t object, const float* rotation, const float* translation);
+int cnq_scene_apply_moves(cnq_scene* scene, void* stream);
 
 /* ---- Queries. MaskConeKernel::irradiance_with (lib.rs:1718) and
  * `coverage` (lib.rs:3289): host buffers in and out, synchronous. out_cov

@@ -288,9 +288,54 @@ static float host_front_area(const float* cov, V3 v) {
   return lerp(lerp(f[0], f[1], fs), lerp(f[2], f[3], fs), ft);
 }
 
+// hierarchy.rs:374-384 `union_root`
+static NodeGpu union_root(const NodeGpu& a, const NodeGpu& b) {
+  NodeGpu r{};
+  for (int k = 0; k < 3; k++) {
+    r.lo[k] = gmin(a.lo[k], b.lo[k]);
+    r.hi[k] = gmax(a.hi[k], b.hi[k]);
+  }
+  r.info[0] = 1;
+  r.info[2] = 1;
+  return r;
+}
+
+// hierarchy.rs:331-411: the static objects' hierarchy under node 1, the
+// movers' subtree under node 2 (its other nodes from first_node).
+static void object_hierarchy_with_movers(const std::vector<Box>& bounds, const std::vector<uint32_t>& movable,
+                                         std::vector<NodeGpu>& nodes, std::vector<uint32_t>& items,
+                                         uint32_t& first_node, uint32_t& first_item) {
+  std::vector<uint8_t> keep(bounds.size(), 1);
+  for (uint32_t m : movable) keep[m] = 0;
+  std::vector<NodeGpu> fixed;
+  object_hierarchy(bounds, fixed, items, keep.data());
+  first_node = (uint32_t)fixed.size() + 2;
+  first_item = (uint32_t)items.size();
+  // the movers' subtree over their bounds, in `movable` order
+  std::vector<Box> mb;
+  for (uint32_t m : movable) mb.push_back(bounds[m]);
+  std::vector<NodeGpu> local(1, NodeGpu{});
+  std::vector<uint32_t> order(movable.size());
+  for (size_t k = 0; k < order.size(); k++) order[k] = (uint32_t)k;
+  Builder b{&local, &mb, 2, first_node - 1, first_item};
+  b.split(0, order.data(), order.size(), 0);
+  for (NodeGpu& n : local) n.info[2] = 1;
+  nodes.assign(first_node, NodeGpu{});
+  for (size_t i = 0; i < fixed.size(); i++) {
+    NodeGpu n = fixed[i];
+    if (n.info[1] == 0) n.info[0] = n.info[0] == 0 ? 1 : n.info[0] + 2;
+    nodes[i == 0 ? 1 : i + 2] = n;
+  }
+  nodes[2] = local[0];
+  nodes.insert(nodes.end(), local.begin() + 1, local.end());
+  nodes[0] = union_root(nodes[1], nodes[2]);
+  for (uint32_t k : order) items.push_back(movable[k]);
+}
+
 static void flatten_instanced(const float* tris, const uint32_t* counts, uint32_t n_geometries,
                               const std::vector<Placed>& objects, bool object_prefilter, OFlat& f,
-                              std::vector<float>& xforms, uint32_t& object_cov) {
+                              std::vector<float>& xforms, uint32_t& object_cov,
+                              const std::vector<uint32_t>& movable = {}) {
   flatten(tris, counts, n_geometries, f);
   // Blended geometries: coverage times the first object's opacity.
   std::vector<std::pair<uint32_t, uint32_t>> roots;
@@ -333,7 +378,12 @@ static void flatten_instanced(const float* tris, const uint32_t* counts, uint32_
   }
   if (f.ranges.empty()) f.ranges.insert(f.ranges.end(), {0, 0, BNONE, BNONE});
   if (xforms.empty()) xforms.assign(16, 0.0f);
-  object_hierarchy(bounds, f.nodes, f.items);
+  if (movable.empty()) {
+    object_hierarchy(bounds, f.nodes, f.items);
+  } else {
+    uint32_t fn, fi;
+    object_hierarchy_with_movers(bounds, movable, f.nodes, f.items, fn, fi);
+  }
   object_cov = BNONE;
   if (!object_prefilter) return;
   V3 dirs[27];
@@ -404,7 +454,8 @@ int oracle_flatten(const float* tris, const uint32_t* counts, uint32_t n_meshes,
 // outputs, sizes = [n_tris, n_ranges, n_blas, n_blas_items, n_nodes,
 // n_items, n_cov_rows, object_cov]; then it fills them (xforms: 16 per object).
 int oracle_flatten_instanced(const float* tris, const uint32_t* counts, uint32_t n_geometries, const uint32_t* placements,
-                             uint64_t n_objects, int object_prefilter, uint64_t* sizes, float* out_tris,
+                             uint64_t n_objects, const uint32_t* movable, uint32_t n_movable, int object_prefilter,
+                             uint64_t* sizes, float* out_tris,
                              uint32_t* out_ranges, float* out_blas, uint32_t* out_blas_items, float* out_blas_cov,
                              float* out_nodes, uint32_t* out_items, float* out_xforms) {
   std::vector<Placed> objs(n_objects);
@@ -423,7 +474,8 @@ int oracle_flatten_instanced(const float* tris, const uint32_t* counts, uint32_t
   OFlat f;
   std::vector<float> xforms;
   uint32_t object_cov = BNONE;
-  flatten_instanced(tris, counts, n_geometries, objs, object_prefilter != 0, f, xforms, object_cov);
+  flatten_instanced(tris, counts, n_geometries, objs, object_prefilter != 0, f, xforms, object_cov,
+                    std::vector<uint32_t>(movable, movable + n_movable));
   if (!out_tris) {
     sizes[0] = f.tris.size() / 12;
     sizes[1] = f.ranges.size() / 4;

@@ -46,7 +46,8 @@ def lib() -> C.CDLL:
             "oracle_cone3_visible": (None, [_P, C.c_uint32, _P, _P, _P, C.c_uint32, C.c_int, _P, _P]),
             "oracle_sampled_visible": (C.c_double, [_P, _P, _P, C.c_uint64, C.c_uint32]),
             "oracle_mesh_hierarchy": (C.c_int, [_P, _P, C.c_uint32, _P, _P, _P, _P]),
-            "oracle_flatten_instanced": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_uint64, C.c_int] + [_P] * 9),
+            "oracle_flatten_instanced": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_uint64, _P, C.c_uint32, C.c_int]
+                                         + [_P] * 9),
             "oracle_block_planes": (None, [_P, _P, C.c_uint32, _P, _P]),
             "oracle_mark_blocks": (None, [_P, C.c_uint32, _P, C.c_uint32, _P]),
             "oracle_bake_static": (None, [_P] * 9 + [_P, _P, C.c_uint32, _P, _P, _P, C.c_int]),
@@ -107,14 +108,16 @@ class OracleInstancedFlat:
     library's placement layout (geometry u32, rotation 9 f32 row-major,
     scale, translation 3, clear)."""
 
-    def __init__(self, tris9, counts, placements, object_prefilter=True):
+    def __init__(self, tris9, counts, placements, object_prefilter=True, movable=None):
         tris9 = np.ascontiguousarray(tris9, dtype=np.float32)
         counts = np.ascontiguousarray(counts, dtype=np.uint32)
         pl = np.ascontiguousarray(placements)
         assert pl.dtype.itemsize == 60
         sizes = np.zeros(8, np.uint64)
         L = lib()
-        args = (_p(tris9), _p(counts), len(counts), _p(pl), len(pl), int(object_prefilter), _p(sizes))
+        mv = np.ascontiguousarray(list(movable or []), dtype=np.uint32)
+        args = (_p(tris9), _p(counts), len(counts), _p(pl), len(pl), _p(mv), len(mv), int(object_prefilter),
+                _p(sizes))
         L.oracle_flatten_instanced(*args, 0, 0, 0, 0, 0, 0, 0, 0)
         nt, nr, nb, nbi, nn, ni, ncov, ocov = (int(x) for x in sizes)
         self.tris = np.zeros((nt, 12), np.float32)

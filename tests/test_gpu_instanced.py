@@ -89,3 +89,42 @@ def test_object_splats_change_results_only_where_prefiltered(device):
     off = scene.render_view(cam, 96, 64, lights, _lib.default_options(prefilter=3.0))
     assert np.array_equal(on[0], off[0])          # camera hits do not splat
     assert not np.array_equal(on[1], off[1])
+
+
+def test_moves_equal_a_fresh_build_and_the_oracle(device):
+    """Movers (renderer.rs:442 set_transform, instanced.rs:1226 apply_moves):
+    after moving some of them, every query equals a scene built afresh with
+    the moved placements, and the oracle's, bit for bit (caps off)."""
+    tris9, counts, pl, lights = _world()
+    movable = list(range(1, 120, 3))
+    f = Flat.build_instanced(tris9, counts, pl, object_prefilter=True, movable=movable)
+    scene = Scene(device, f)
+    rng = np.random.default_rng(11)
+    moved = pl.copy()
+    for m in movable[::2]:
+        ang = rng.uniform(0, 2 * np.pi)
+        c, s = np.cos(ang), np.sin(ang)
+        rot = np.array([[c, -s, 0], [s, c, 0], [0, 0, 1]], np.float32)
+        t = (moved[m]["translation"] + rng.uniform(-4, 4, 3) * np.array([1, 1, 0.3])).astype(np.float32)
+        scene.set_transform(m, rot, t)
+        moved[m]["rotation"] = rot.reshape(9)
+        moved[m]["translation"] = t
+    scene.apply_moves()
+    fresh = Flat.build_instanced(tris9, counts, moved, object_prefilter=True, movable=movable)
+    fresh_scene = Scene(device, fresh)
+    o = O.OracleInstancedFlat(tris9, counts, moved, object_prefilter=True, movable=movable)
+    cam = camera_uniform([-24.0, -24.0, 2.5], [5.0, 5.0, 0.5], 55.0, 96, 64)
+    for prefilter in (0.0, 3.0):
+        kw = dict(hierarchy=2, prefilter=prefilter, object_cov=f.object_cov, light_cone_cap=O.NONE,
+                  wide_cone_cap=O.NONE)
+        got = scene.render_view(cam, 96, 64, lights, _lib.default_options(**kw))
+        ref = fresh_scene.render_view(cam, 96, 64, lights, _lib.default_options(**kw))
+        for a, b in zip(got, ref):
+            assert np.array_equal(np.ascontiguousarray(a).view(np.uint32), np.ascontiguousarray(b).view(np.uint32))
+        rh, recv, _ = O.primary(o, cam, 96, 64, lights, hierarchy=2)
+        assert np.array_equal(got[0], rh)
+        rv, rf, rt, _, capped, _ = O.irradiance(o, recv, lights, O.opts_array(
+            hierarchy=2, prefilter=prefilter, object_cov=f.object_cov, light_cone_cap=O.NONE, wide_cone_cap=O.NONE))
+        assert np.array_equal(got[1].view(np.uint32), rv.view(np.uint32)) and np.array_equal(got[2], rf)
+    with pytest.raises(Exception):
+        scene.set_transform(0, np.eye(3), [0, 0, 0])    # the ground is static

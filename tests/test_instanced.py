@@ -143,3 +143,36 @@ def test_boulder_field_shape():
     d = np.linalg.norm(pl["translation"][1:, :2] - eye[:2], axis=1)
     assert np.all(d >= 4.0) and np.all(np.diff(d) >= -1e-3)   # nearest first
     assert len(lights) == 32 and np.all(lights[:, 3] == 1.0)
+
+
+def test_movers_hierarchy_equals_oracle_and_is_laid_out():
+    """object_hierarchy_with_movers (hierarchy.rs:331-411): node 0 over the
+    static root (node 1) and the movers' root (node 2), no prefiltered
+    coverage on either or on the movers' subtree; the product equals the
+    oracle; moving the movers and building afresh changes only the movers'
+    subtree, whose shape depends only on their count."""
+    geoms, tris9, counts, pl = _world(n=60, seed=5, blend=False)
+    movable = [3, 17, 18, 40, 59, 7, 22]
+    f = Flat.build_instanced(tris9, counts, pl, object_prefilter=True, movable=movable)
+    o = O.OracleInstancedFlat(tris9, counts, pl, object_prefilter=True, movable=movable)
+    for name in ("tris", "ranges", "blas", "blas_items", "blas_cov", "nodes", "items", "xforms"):
+        _same_bits(getattr(f, name), getattr(o, name))
+    t = f.movers
+    assert sorted(t["objects"].tolist()) == sorted(movable)
+    nodes, items = f.nodes, f.items
+    assert nodes[0][8] == 1 and nodes[0][9] == 0 and nodes[0][10] == 1
+    assert nodes[2][10] == 1 and all(nodes[t["first_node"]:, 10] == 1)
+    assert sorted(items[t["first_item"]:].tolist()) == sorted(movable)
+    assert not set(items[: t["first_item"]].tolist()) & set(movable)
+    lo0, hi0 = nodes[0][0:3].view(np.float32), nodes[0][4:7].view(np.float32)
+    for k in (1, 2):
+        assert np.all(nodes[k][0:3].view(np.float32) >= lo0) and np.all(nodes[k][4:7].view(np.float32) <= hi0)
+    # move every mover and rebuild: the static part and the layout stay
+    moved = pl.copy()
+    for m in movable:
+        moved[m]["translation"] = moved[m]["translation"] + np.float32(3.0)
+    g = Flat.build_instanced(tris9, counts, moved, object_prefilter=True, movable=movable)
+    assert len(g.nodes) == len(f.nodes) and g.movers["first_node"] == t["first_node"]
+    _same_bits(g.nodes[3:t["first_node"]], f.nodes[3:t["first_node"]])
+    _same_bits(g.nodes[1], f.nodes[1])
+    _same_bits(g.items[: t["first_item"]], f.items[: t["first_item"]])

@@ -81,6 +81,21 @@ class RequestInfo(C.Structure):
 REQ_QUEUED, REQ_ADMITTED, REQ_DONE, REQ_RELEASED = range(4)
 
 
+class VcacheSurfaces(C.Structure):
+    _fields_ = [("tri_chart", C.c_void_p), ("tri_uv", C.c_void_p), ("normals", C.c_void_p),
+                ("n_charts", C.c_uint32), ("chart_size", C.c_void_p), ("chart_grid", C.c_void_p)]
+
+
+class VcacheConfig(C.Structure):
+    _fields_ = [("pages", C.c_uint32), ("append_pages", C.c_uint32), ("candidates", C.c_uint32),
+                ("base_texel", C.c_float), ("light_ratio", C.c_uint32), ("visible_frames", C.c_uint32)]
+
+
+class MovedBox(C.Structure):
+    _fields_ = [("object", C.c_uint32), ("old_lo", C.c_float * 3), ("old_hi", C.c_float * 3),
+                ("new_lo", C.c_float * 3), ("new_hi", C.c_float * 3)]
+
+
 # Every symbol include/cnq.h declares, with its ctypes signature.
 _P = C.c_void_p
 SIGNATURES = {
@@ -143,6 +158,18 @@ SIGNATURES = {
     "cnq_cache_destroy": (None, [_P]),
     "cnq_cache_update": (C.c_int, [_P, _P, C.c_uint32, _P, _P]),
     "cnq_cache_read": (C.c_int, [_P, _P, _P, _P]),
+    "cnq_scene_last_moves": (C.c_int, [_P, _P, C.c_uint32, C.POINTER(C.c_uint32)]),
+    "cnq_vcache_config_default": (None, [C.POINTER(VcacheConfig)]),
+    "cnq_vcache_create": (C.c_int, [_P, C.POINTER(VcacheSurfaces), _P, C.c_uint32, C.POINTER(VcacheConfig),
+                                    C.POINTER(Options), C.POINTER(_P)]),
+    "cnq_vcache_destroy": (None, [_P]),
+    "cnq_vcache_grid": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.c_uint32, _P]),
+    "cnq_vcache_request": (C.c_int, [_P, _P, C.c_uint32, _P]),
+    "cnq_vcache_evict": (C.c_int, [_P, _P, C.c_uint32]),
+    "cnq_vcache_set_moved": (C.c_int, [_P, _P, C.c_uint32, _P]),
+    "cnq_vcache_frame": (C.c_int, [_P, _P, _P]),
+    "cnq_vcache_read": (C.c_int, [_P, _P, C.c_uint32, _P, _P]),
+    "cnq_vcache_state": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P]),
 }
 
 _lib = None

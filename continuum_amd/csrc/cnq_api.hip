@@ -660,6 +660,10 @@ int cnq_scene_set_movers(cnq_scene* s, const cnq_mover_tree* t, const float* geo
                              reinterpret_cast<const cnq::host::Placement*>(placements) + t->n_objects);
   s->mover_dirty.assign(t->n_objects, 0);
   s->mover_of = std::move(mover_of);
+  s->mover_bounds.resize(t->n_objects);
+  for (uint32_t k = 0; k < t->n_objects; k++)
+    s->mover_bounds[k] = cnq::host::world_bounds(s->mover_placements[k], s->geometry_bounds[s->mover_placements[k].geometry]);
+  s->last_moves.clear();
   s->has_movers = true;
   return CNQ_OK;
 }
@@ -681,6 +685,7 @@ int cnq_scene_apply_moves(cnq_scene* s, void* stream) {
   if (!s->has_movers) return fail(CNQ_INVALID_ARG, "the scene has no movers");
   bool any = false;
   for (uint8_t d : s->mover_dirty) any = any || d;
+  s->last_moves.clear();
   if (!any) return CNQ_OK;
   HIP_TRY(hipSetDevice(s->dev->ordinal), "hipSetDevice");
   hipStream_t st = (hipStream_t)stream;
@@ -693,6 +698,15 @@ int cnq_scene_apply_moves(cnq_scene* s, void* stream) {
   for (size_t k = 0; k < t.objects.size(); k++) {
     if (!s->mover_dirty[k]) continue;
     s->mover_dirty[k] = 0;
+    // The move, for the view cache's stale passes (instanced.rs:1284-1306).
+    cnq_moved mv;
+    mv.object = t.objects[k];
+    std::memcpy(mv.old_lo, s->mover_bounds[k].lo, 12);
+    std::memcpy(mv.old_hi, s->mover_bounds[k].hi, 12);
+    std::memcpy(mv.new_lo, bounds[k].lo, 12);
+    std::memcpy(mv.new_hi, bounds[k].hi, 12);
+    s->last_moves.push_back(mv);
+    s->mover_bounds[k] = bounds[k];
     const cnq::host::Placement& p = s->mover_placements[k];
     const float* r = p.rotation;
     const float rows[16] = {r[0], r[1], r[2], p.translation[0], r[3], r[4], r[5], p.translation[1],
@@ -719,6 +733,14 @@ int cnq_scene_apply_moves(cnq_scene* s, void* stream) {
   HIP_TRY(hipMemcpyAsync(s->items.p + t.first_item, items.data(), items.size() * 4, hipMemcpyHostToDevice, st),
           "movers' items");
   HIP_TRY(hipStreamSynchronize(st), "apply moves");
+  return CNQ_OK;
+}
+
+int cnq_scene_last_moves(cnq_scene* s, cnq_moved* out, uint32_t cap, uint32_t* n) {
+  if (!s || !n || (cap && !out)) return fail(CNQ_INVALID_ARG, "null argument");
+  *n = (uint32_t)s->last_moves.size();
+  if (*n > cap) return fail(CNQ_CAPACITY, "more moves than `cap`");
+  if (*n) std::memcpy(out, s->last_moves.data(), *n * sizeof(cnq_moved));
   return CNQ_OK;
 }
 

@@ -82,7 +82,7 @@ __global__ __launch_bounds__(64) void bake_static_kernel(
   __shared__ float s_poly[POLY_SLOTS * 64];
   const uint32_t lane = lane_id();
   Lane L{s_acc + lane, s_poly + lane, ACC_CELLS, 0u};
-  const Ctx x{blas, static_nodes, tri_leaf, tris, ranges, static_items, blas_cov, xforms,
+  const Ctx x{blas, static_nodes, tri_leaf, tris, ranges, static_items, nullptr, blas_cov, xforms,
               p.n_objects, 2u, p.profile_stage, NONE};
   const uint32_t slot = blockIdx.x;
   const uint64_t i = 64ull * slot_block[slot] + lane;
@@ -112,7 +112,7 @@ __global__ __launch_bounds__(64) void rebake_kernel(
   __shared__ float s_poly[POLY_SLOTS * 64];
   const uint32_t lane = lane_id();
   Lane L{s_acc + lane, s_poly + lane, ACC_CELLS, 0u};
-  const Ctx x{blas, nullptr, tri_leaf, tris, ranges, nullptr, blas_cov, xforms,
+  const Ctx x{blas, nullptr, tri_leaf, tris, ranges, nullptr, nullptr, blas_cov, xforms,
               p.n_objects, 2u, p.profile_stage, NONE};
   const uint32_t count = *n_dirty;
   for (uint32_t j = blockIdx.x; j < count; j += gridDim.x) {

@@ -40,7 +40,7 @@ __global__ __launch_bounds__(64) void cone_items_kernel(
   __shared__ float s_poly[POLY_SLOTS * 64];
   const uint32_t lane = lane_id();
   Lane L{s_acc + lane, s_poly + lane, ACC_CELLS, 0u};
-  const Ctx x{blas, nodes, tri_leaf, tris, ranges, items, blas_cov, xforms,
+  const Ctx x{blas, nodes, tri_leaf, tris, ranges, items, nullptr, blas_cov, xforms,
               p.n_objects, p.hierarchy, p.profile_stage, p.object_cov};
   const uint32_t tw_log2 = p.tile_w_log2;
   for (uint32_t round = 0;; round++) {

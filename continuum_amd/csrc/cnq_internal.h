@@ -78,6 +78,8 @@ struct cnq_scene {
   std::vector<cnq::host::Placement> mover_placements;
   std::vector<uint8_t> mover_dirty;
   std::vector<uint32_t> mover_of;  // per object: its index among the movers, or NONE
+  std::vector<cnq::host::Box> mover_bounds;  // each mover's world box as last applied
+  std::vector<cnq_moved> last_moves;         // the last apply_moves's moves
   bool instanced = false;
 
   cnq::SceneArgs args() const {

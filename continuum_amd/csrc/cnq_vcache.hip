@@ -1,0 +1,1190 @@
+// The view cache's page passes on gfx950: pages of 9 x 9 texels on geometry
+// charts, keyed (object, chart | mip << 24, page x | page y << 16) and found
+// through an open-addressing page table; each texel holds, per light, the
+// irradiance (or its shadowed share of the unshadowed light) through one
+// mask cone from the texel's place on the surface.
+//
+// Follows crates/continuum-gpu/src/mask_cones.wgsl: `vc_hash`/`vc_find`
+// :3123-3150, `vc_grid` :3181, `uv_barycentric` :3250, `vc_surface(_at)`
+// :3274-3326, `page_texel` :3338, `vc_update` :3394, `vc_bake` :3432-3554
+// (static lights), `vc_fill` :3693-3836, and the movers' stale passes
+// `world_box` :3867, `light_box` :3887, `hull_meets` :3903, `boxes_meet`
+// :3919, `may_be_shadowed` :3967, `mark_invalid` :3988, `append_page`
+// :3998, `append_jobs` :4018, `fail_entry` :4029, `vc_stale` :4233-4379,
+// `vc_stale_texels` :4385-4432; host side crates/continuum-gpu/src/vcache.rs
+// (planning :3940-4050, eviction :3425-3440, `set_moved` :2426-2492).
+//
+// Organisation on this GPU: bake jobs are consecutive (page, texel, light)
+// triples, so a wave's 64 lanes are 64 neighbouring texels of one page and
+// light (at most two): one packet through the cone walk of cnq_walk.h. The
+// fill runs one wave per page (a lane per texel); the stale passes one lane
+// per slot and per candidate texel, appending with atomics as the WGSL does.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <unordered_map>
+
+#include "cnq_internal.h"
+#include "cnq_walk.h"
+#include "vcache_host.h"
+
+namespace cnq {
+
+constexpr uint32_t VC_STRIDE = vc::STRIDE;
+constexpr uint32_t VC_TEXELS = vc::PAGE_TEXELS;
+
+// mover_stats words (WGSL): 0 pages appended, 1 invalid marks, 2 visible
+// pages tested, 3 appends that did not fit, 4 candidates, 5 jobs appended,
+// 6 pages made stale for a re-bake.
+constexpr uint32_t VC_STATS = 8;
+
+struct VcArgs {
+  const vc::Chart* __restrict__ charts;
+  const uint32_t* __restrict__ cells;
+  const uint32_t* __restrict__ cell_tris;
+  const float* __restrict__ tri_uv;    // 6 per triangle
+  const float* __restrict__ normals;   // 9 per triangle
+  const float* __restrict__ tris;      // 12 per triangle
+  const uint32_t* __restrict__ ranges;
+  const float* __restrict__ xforms;
+  const float* __restrict__ lights;    // 16 per light
+  const float* __restrict__ moved;     // 4 per vec4
+  uint32_t* page_table;
+  uint32_t* slot_info;                 // 12 per slot
+  uint32_t* slot_stale;
+  uint32_t* slot_used;
+  float* pool;
+  uint32_t* bake_list;                 // 4 per entry
+  float* bake_bounds;                  // 8 per entry
+  uint32_t* bake_jobs;                 // 2 per job
+  uint32_t* candidates;                // 2 per candidate
+  uint32_t* invalid_list;
+  uint32_t* stats;
+  uint32_t n_slots, n_lights, channels, n_movers;
+  uint32_t plan_pages, plan_jobs, append_pages, append_jobs;
+  uint32_t candidate_cap, invalid_cap, table_mask;
+  uint32_t frame, visible_frames, light_ratio;
+  float inv_base;
+};
+
+namespace {
+
+struct Box3 {
+  f3 lo, hi;
+};
+
+__device__ __forceinline__ float surface_offset(float distance) { return mx(1e-4f, 2e-6f * distance); }
+
+__device__ __forceinline__ f3 row3(const float* xf, int r) { return f3{xf[4 * r], xf[4 * r + 1], xf[4 * r + 2]}; }
+
+__device__ __forceinline__ f3 rotate_to_world(const float* xf, f3 d) {
+  return f3{dot3(row3(xf, 0), d), dot3(row3(xf, 1), d), dot3(row3(xf, 2), d)};
+}
+
+__device__ __forceinline__ f3 translation(const float* xf) { return f3{xf[3], xf[7], xf[11]}; }
+
+__device__ __forceinline__ f3 vabs3(f3 a) { return f3{fabsf(a.x), fabsf(a.y), fabsf(a.z)}; }
+__device__ __forceinline__ f3 vmax3(f3 a, f3 b) { return f3{mx(a.x, b.x), mx(a.y, b.y), mx(a.z, b.z)}; }
+__device__ __forceinline__ float max_el(f3 a) { return mx(a.x, mx(a.y, a.z)); }
+
+__device__ __forceinline__ float object_scale(const VcArgs& a, uint32_t m) {
+  const uint32_t x = a.ranges[4 * m + 3];
+  return x == NONE ? 1.0f : a.xforms[16 * (size_t)x + 12];
+}
+
+// WGSL :3181-3184
+__device__ __forceinline__ void vc_grid(const VcArgs& a, uint32_t c, uint32_t m, uint32_t mip, uint32_t n[2]) {
+  const vc::Chart ch = a.charts[c];
+  const float s = object_scale(a, m);
+  const float inv = ldexpf(a.inv_base, -(int)mip);
+  const float nx = ceilf((ch.size[0] * s) * inv), ny = ceilf((ch.size[1] * s) * inv);
+  const uint32_t ux = nx >= 4294967296.0f ? 0xffffffffu : (uint32_t)nx;
+  const uint32_t uy = ny >= 4294967296.0f ? 0xffffffffu : (uint32_t)ny;
+  n[0] = ux > 1u ? ux : 1u;
+  n[1] = uy > 1u ? uy : 1u;
+}
+
+// WGSL :3250-3260
+__device__ __forceinline__ void uv_barycentric(const VcArgs& a, uint32_t t, f2 uv, float b[3]) {
+  const float* q = a.tri_uv + 6 * (size_t)t;
+  const f2 o{q[0], q[1]};
+  const f2 e1 = f2{q[2], q[3]} - o, e2 = f2{q[4], q[5]} - o, d = uv - o;
+  const float det = e1.x * e2.y - e1.y * e2.x;
+  if (fabsf(det) < 1e-30f) {
+    b[0] = b[1] = b[2] = -1.0f;
+    return;
+  }
+  const float s = (d.x * e2.y - d.y * e2.x) / det;
+  const float u = (e1.x * d.y - e1.y * d.x) / det;
+  b[0] = 1.0f - s - u;
+  b[1] = s;
+  b[2] = u;
+}
+
+struct TexelSurface {
+  f3 apex, normal;
+  bool ok;
+  uint32_t tri;
+};
+
+// WGSL :3283-3326 (objects' lighting levels are always present here)
+__device__ __forceinline__ TexelSurface vc_surface_at(const VcArgs& a, uint32_t m, uint32_t c, f2 uv) {
+  const vc::Chart ch = a.charts[c];
+  const uint32_t ux = (uint32_t)(uv.x * (float)ch.grid[0]), uy = (uint32_t)(uv.y * (float)ch.grid[1]);
+  const uint32_t cx = ux < ch.grid[0] - 1u ? ux : ch.grid[0] - 1u;
+  const uint32_t cy = uy < ch.grid[1] - 1u ? uy : ch.grid[1] - 1u;
+  const uint32_t cell = ch.first_cell + cy * ch.grid[0] + cx;
+  const uint32_t first = a.cells[2 * cell], count = a.cells[2 * cell + 1];
+  uint32_t best_t = NONE;
+  float best_b[3] = {0.0f, 0.0f, 0.0f};
+  float best_score = -3.4e38f;
+  for (uint32_t q = first; q < first + count; q++) {
+    const uint32_t t = a.cell_tris[q];
+    float b[3];
+    uv_barycentric(a, t, uv, b);
+    const float score = mn(b[0], mn(b[1], b[2]));
+    if (score > best_score) {
+      best_score = score;
+      best_t = t;
+      best_b[0] = b[0], best_b[1] = b[1], best_b[2] = b[2];
+    }
+  }
+  float b0[3];
+  for (int k = 0; k < 3; k++) b0[k] = clampf(best_b[k], 0.0f, 1.0f);
+  const float sum = mx(b0[0] + b0[1] + b0[2], 1e-20f);
+  const float bx = b0[0] / sum, by = b0[1] / sum, bz = b0[2] / sum;
+  const uint32_t t = best_t != NONE ? best_t : 0u;
+  const float* tr = a.tris + 12 * (size_t)t;
+  const f3 v0 = ld3(tr), v1 = ld3(tr + 4), v2 = ld3(tr + 8);
+  const float* nr = a.normals + 9 * (size_t)t;
+  const f3 p = v0 * bx + v1 * by + v2 * bz;
+  const f3 interpolated = ld3(nr) * bx + ld3(nr + 3) * by + ld3(nr + 6) * bz;
+  const f3 face = cross3(v1 - v0, v2 - v0);
+  const f3 nrm = dot3(interpolated, interpolated) > 1e-20f ? interpolated : face;
+  f3 pw = p;
+  f3 nw = norm3(nrm);
+  f3 geometric = face;
+  const uint32_t x = a.ranges[4 * m + 3];
+  if (x != NONE) {
+    const float* xf = a.xforms + 16 * (size_t)x;
+    pw = rotate_to_world(xf, p) * xf[12] + translation(xf);
+    nw = norm3(rotate_to_world(xf, nrm));
+    geometric = rotate_to_world(xf, face);
+  }
+  TexelSurface s;
+  s.apex = pw + norm3(geometric) * surface_offset(len3(pw));
+  s.normal = nw;
+  s.ok = best_t != NONE;
+  s.tri = t;
+  return s;
+}
+
+struct PageTexel {
+  uint32_t e[4];
+  uint32_t m, c, n[2], texel[2];
+  bool inside;
+};
+
+// WGSL :3338-3350
+__device__ __forceinline__ PageTexel page_texel(const VcArgs& a, uint32_t page_index, uint32_t k) {
+  PageTexel t;
+  const uint32_t* e = a.bake_list + 4 * (size_t)page_index;
+  t.e[0] = e[0], t.e[1] = e[1], t.e[2] = e[2], t.e[3] = e[3];
+  t.m = t.e[0];
+  t.c = t.e[1] & vc::KEY_CHART_MASK;
+  const uint32_t mip = t.e[1] >> vc::KEY_MIP_SHIFT;
+  vc_grid(a, t.c, t.m, mip, t.n);
+  t.texel[0] = (t.e[2] & 0xffffu) * vc::PAGE + k % VC_STRIDE;
+  t.texel[1] = (t.e[2] >> 16) * vc::PAGE + k / VC_STRIDE;
+  t.inside = t.texel[0] < t.n[0] && t.texel[1] < t.n[1];
+  return t;
+}
+
+// ---- the movers' tests
+
+// WGSL :3867-3882
+__device__ __forceinline__ Box3 world_box(const VcArgs& a, uint32_t m, f3 lo, f3 hi) {
+  Box3 b{lo, hi};
+  const uint32_t x = a.ranges[4 * m + 3];
+  if (x != NONE) {
+    const float* xf = a.xforms + 16 * (size_t)x;
+    const float s = xf[12];
+    const f3 c = (lo + hi) * 0.5f;
+    const f3 e = (hi - lo) * 0.5f;
+    const f3 wc = rotate_to_world(xf, c) * s + translation(xf);
+    const f3 we = f3{dot3(vabs3(row3(xf, 0)), e), dot3(vabs3(row3(xf, 1)), e), dot3(vabs3(row3(xf, 2)), e)} * s;
+    b = Box3{wc - we, wc + we};
+  }
+  const f3 far = vmax3(vmax3(vabs3(b.lo), vabs3(b.hi)), f3{1.0f, 1.0f, 1.0f});
+  const float pad = 2.0f * surface_offset(len3(far)) + 1e-5f * max_el(far);
+  return Box3{b.lo - f3{pad, pad, pad}, b.hi + f3{pad, pad, pad}};
+}
+
+__device__ __forceinline__ f3 light_center_of(const float* r) {
+  if (r[3] > 1.5f) return ld3(r);
+  return ld3(r) + (ld3(r + 4) + ld3(r + 8)) * 0.5f;
+}
+
+// WGSL :3887-3898
+__device__ __forceinline__ Box3 light_box(const float* r) {
+  const f3 c = light_center_of(r);
+  float radius;
+  if (r[3] > 1.5f) radius = 1.5f * r[4];
+  else radius = 0.5f * (len3(ld3(r + 4)) + len3(ld3(r + 8)));
+  const f3 far = vmax3(vabs3(c), f3{1.0f, 1.0f, 1.0f});
+  radius = radius * 1.001f + 1e-5f * max_el(far) + 1e-4f;
+  return Box3{c - f3{radius, radius, radius}, c + f3{radius, radius, radius}};
+}
+
+__device__ __forceinline__ float comp(f3 v, int a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
+
+// WGSL :3903-3917
+__device__ __forceinline__ bool hull_meets(const Box3& p, const Box3& l, const Box3& m) {
+  float s0 = 0.0f, s1 = 1.0f;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const float b0 = comp(l.lo, a) - comp(p.lo, a);
+    const float c0 = comp(m.hi, a) - comp(p.lo, a);
+    if (b0 > 0.0f) s1 = mn(s1, c0 / b0);
+    else if (b0 < 0.0f) s0 = mx(s0, c0 / b0);
+    else if (c0 < 0.0f) return false;
+    const float b1 = comp(l.hi, a) - comp(p.hi, a);
+    const float c1 = comp(m.lo, a) - comp(p.hi, a);
+    if (b1 > 0.0f) s0 = mx(s0, c1 / b1);
+    else if (b1 < 0.0f) s1 = mn(s1, c1 / b1);
+    else if (c1 > 0.0f) return false;
+  }
+  return s0 <= s1 + 1e-6f;
+}
+
+__device__ __forceinline__ bool boxes_meet(const Box3& a, const Box3& b) {
+  return a.lo.x <= b.hi.x && a.lo.y <= b.hi.y && a.lo.z <= b.hi.z && b.lo.x <= a.hi.x && b.lo.y <= a.hi.y &&
+         b.lo.z <= a.hi.z;
+}
+
+__device__ __forceinline__ Box3 moved_box(const VcArgs& a, uint32_t i) {
+  return Box3{ld3(a.moved + 4 * (size_t)i), ld3(a.moved + 4 * (size_t)(i + 1))};
+}
+
+__device__ __forceinline__ uint32_t moved_w(const VcArgs& a, uint32_t i) {
+  return __float_as_uint(a.moved[4 * (size_t)i + 3]);
+}
+
+// WGSL :3967-3982
+__device__ __forceinline__ bool may_be_shadowed(const VcArgs& a, const Box3& p, uint32_t l, uint32_t s) {
+  const uint32_t lights_n = a.n_lights;
+  if (!boxes_meet(p, moved_box(a, 2 * l))) return false;
+  const Box3 lb = light_box(a.lights + 16 * (size_t)l);
+  const uint32_t first = 2 * lights_n + 2, stride = 4 + 4 * lights_n;
+  for (uint32_t k = 0; k < a.n_movers; k++) {
+    const uint32_t e = first + k * stride;
+    const uint32_t slice = moved_w(a, e + 3);
+    if (slice != 0u && s % (slice >> 16) != (slice & 0xffffu)) continue;
+    const uint32_t r = e + 4 + 4 * l;
+    if (boxes_meet(p, moved_box(a, r)) && hull_meets(p, lb, moved_box(a, e))) return true;
+    if (boxes_meet(p, moved_box(a, r + 2)) && hull_meets(p, lb, moved_box(a, e + 2))) return true;
+  }
+  return false;
+}
+
+// WGSL :3988-3996
+__device__ __forceinline__ void mark_invalid(const VcArgs& a, uint32_t s) {
+  const uint32_t old = atomicExch(&a.slot_stale[s], 1u);
+  if (old == 0u || old == 2u || old == 3u) {
+    const uint32_t q = atomicAdd(&a.stats[1], 1u);
+    if (q < a.invalid_cap) a.invalid_list[q] = s;
+  }
+}
+
+// WGSL :3998-4012
+__device__ __forceinline__ uint32_t append_page(const VcArgs& a, uint32_t s, const uint32_t key[3], f3 lo, f3 hi) {
+  const uint32_t p = atomicAdd(&a.stats[0], 1u);
+  if (p >= a.append_pages) {
+    atomicAdd(&a.stats[3], 1u);
+    return NONE;
+  }
+  const uint32_t entry = a.plan_pages + p;
+  uint32_t* e = a.bake_list + 4 * (size_t)entry;
+  e[0] = key[0], e[1] = key[1], e[2] = key[2], e[3] = s;
+  float* b = a.bake_bounds + 8 * (size_t)entry;
+  b[0] = lo.x, b[1] = lo.y, b[2] = lo.z, b[3] = 0.0f;
+  b[4] = hi.x, b[5] = hi.y, b[6] = hi.z, b[7] = 0.0f;
+  return entry;
+}
+
+// WGSL :4018-4027: one add; past the capacity, what of the range lies
+// within it is written as empty jobs.
+__device__ __forceinline__ uint32_t append_jobs(const VcArgs& a, uint32_t count) {
+  const uint32_t old = atomicAdd(&a.stats[5], count);
+  if (old + count <= a.append_jobs) return a.plan_jobs + old;
+  const uint32_t end = old + count < a.append_jobs ? old + count : a.append_jobs;
+  for (uint32_t k = old; k < end; k++) {
+    a.bake_jobs[2 * (size_t)(a.plan_jobs + k)] = NONE;
+    a.bake_jobs[2 * (size_t)(a.plan_jobs + k) + 1] = 0u;
+  }
+  return NONE;
+}
+
+// WGSL :4029-4035
+__device__ __forceinline__ void fail_entry(const VcArgs& a, uint32_t entry, uint32_t s) {
+  a.bake_bounds[8 * (size_t)entry + 3] = 1.0f;
+  atomicExch(&a.slot_stale[s], 1u);
+  const uint32_t q = atomicAdd(&a.stats[1], 1u);
+  if (q < a.invalid_cap) a.invalid_list[q] = s;
+}
+
+}  // namespace
+
+// WGSL :3394-3398, and this build's use marks and cleared slots: updates
+// are [n_table x (entry, key, slot)] [n_used slots] [n_cleared slots].
+__global__ __launch_bounds__(256) void vc_update_kernel(VcArgs a, const uint32_t* __restrict__ upd, uint32_t n_table,
+                                                        uint32_t n_used, uint32_t n_cleared) {
+  const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+  if (i < n_table) {
+    const uint32_t* u = upd + 5 * (size_t)i;
+    uint32_t* e = a.page_table + 4 * (size_t)u[0];
+    e[0] = u[1], e[1] = u[2], e[2] = u[3], e[3] = u[4];
+  } else if (i < n_table + n_used) {
+    a.slot_used[upd[5 * (size_t)n_table + (i - n_table)]] = a.frame;
+  } else if (i < n_table + n_used + n_cleared) {
+    const uint32_t s = upd[5 * (size_t)n_table + n_used + (i - n_table - n_used)];
+    for (int k = 0; k < 12; k++) a.slot_info[12 * (size_t)s + k] = 0u;
+  }
+}
+
+// WGSL :4233-4379, rigid movers (no deforming catch-up, no sky): one lane
+// per slot.
+__global__ __launch_bounds__(256) void vc_stale_kernel(VcArgs a) {
+  const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+  if (s >= a.n_slots) return;
+  const uint32_t* info = a.slot_info + 12 * (size_t)s;
+  if (info[3] == 0u) return;
+  const uint32_t mark = __atomic_load_n(&a.slot_stale[s], __ATOMIC_RELAXED);
+  if (mark == 1u) return;
+  const uint32_t m = info[0];
+  const uint32_t key[3] = {info[0], info[1], info[2]};
+  const bool visible = a.slot_used[s] + a.visible_frames >= a.frame;
+  if (visible) atomicAdd(&a.stats[2], 1u);
+  const f3 lo{__uint_as_float(info[4]), __uint_as_float(info[5]), __uint_as_float(info[6])};
+  const f3 hi{__uint_as_float(info[8]), __uint_as_float(info[9]), __uint_as_float(info[10])};
+  const Box3 page = world_box(a, m, lo, hi);
+  const uint32_t lights_n = a.n_lights;
+  const uint32_t first = 2 * lights_n + 2, stride = 4 + 4 * lights_n;
+  // A page on a moved object is stale whole.
+  bool own = false;
+  uint32_t own_mask = 0;
+  if (boxes_meet(page, moved_box(a, 2 * lights_n))) {
+    for (uint32_t k = 0; k < a.n_movers; k++) {
+      if (moved_w(a, first + k * stride) == m) {
+        own = true;
+        own_mask = moved_w(a, first + k * stride + 2);
+      }
+    }
+  }
+  if (own && own_mask != 0u) {
+    if (!visible) {
+      mark_invalid(a, s);
+      return;
+    }
+    const uint32_t entry = append_page(a, s, key, lo, hi);
+    if (entry == NONE) {
+      mark_invalid(a, s);
+      return;
+    }
+    const uint32_t all = lights_n >= 32 ? 0xffffffffu : (1u << lights_n) - 1u;
+    const uint32_t updated = (uint32_t)__builtin_popcount(own_mask & all);
+    const uint32_t j = append_jobs(a, VC_TEXELS * updated);
+    if (j == NONE) {
+      atomicAdd(&a.stats[3], 1u);
+      fail_entry(a, entry, s);
+      return;
+    }
+    atomicAdd(&a.stats[6], 1u);
+    uint32_t at = j;
+    for (uint32_t l = 0; l < lights_n; l++) {
+      if ((own_mask & (1u << l)) == 0u) continue;
+      for (uint32_t k = 0; k < VC_TEXELS; k++) {
+        a.bake_jobs[2 * (size_t)(at + k)] = entry;
+        a.bake_jobs[2 * (size_t)(at + k) + 1] = k | (l << 8);
+      }
+      at += VC_TEXELS;
+    }
+    return;
+  }
+  bool candidate = false;
+  for (uint32_t l = 0; l < lights_n && !candidate; l++) candidate = may_be_shadowed(a, page, l, s);
+  if (!candidate) return;
+  if (mark != 0u) {
+    mark_invalid(a, s);
+    return;
+  }
+  const uint32_t c = atomicAdd(&a.stats[4], 1u);
+  if (c >= a.candidate_cap) {
+    mark_invalid(a, s);
+    return;
+  }
+  uint32_t entry = NONE;
+  if (visible) {
+    entry = append_page(a, s, key, lo, hi);
+    if (entry == NONE) {
+      a.candidates[2 * (size_t)c] = NONE;
+      a.candidates[2 * (size_t)c + 1] = NONE;
+      // Only rigid movers reach here, so the reference's `lag_only` is false.
+      mark_invalid(a, s);
+      return;
+    }
+  }
+  a.candidates[2 * (size_t)c] = s;
+  a.candidates[2 * (size_t)c + 1] = entry;
+}
+
+// WGSL :4385-4432: one lane per candidate page's texel.
+__global__ __launch_bounds__(256) void vc_stale_texels_kernel(VcArgs a) {
+  const uint32_t j = blockIdx.x * 256u + threadIdx.x;
+  const uint32_t c = j / VC_TEXELS, k = j % VC_TEXELS;
+  const uint32_t listed = __atomic_load_n(&a.stats[4], __ATOMIC_RELAXED);
+  if (c >= (listed < a.candidate_cap ? listed : a.candidate_cap)) return;
+  const uint32_t s = a.candidates[2 * (size_t)c], cand_entry = a.candidates[2 * (size_t)c + 1];
+  if (s == NONE) return;
+  const uint32_t* info = a.slot_info + 12 * (size_t)s;
+  const uint32_t m = info[0];
+  const uint32_t chart = info[1] & vc::KEY_CHART_MASK, mip = info[1] >> vc::KEY_MIP_SHIFT;
+  uint32_t n[2];
+  vc_grid(a, chart, m, mip, n);
+  const uint32_t tx = (info[2] & 0xffffu) * vc::PAGE + k % VC_STRIDE;
+  const uint32_t ty = (info[2] >> 16) * vc::PAGE + k / VC_STRIDE;
+  if (tx >= n[0] || ty >= n[1]) return;
+  const TexelSurface surface =
+      vc_surface_at(a, m, chart, f2{((float)tx + 0.5f) / (float)n[0], ((float)ty + 0.5f) / (float)n[1]});
+  if (!surface.ok) return;
+  const f3 far = vmax3(vabs3(surface.apex), f3{1.0f, 1.0f, 1.0f});
+  const float pad = 1e-5f * max_el(far);
+  const Box3 apex{surface.apex - f3{pad, pad, pad}, surface.apex + f3{pad, pad, pad}};
+  for (uint32_t l = 0; l < a.n_lights; l++) {
+    if (!may_be_shadowed(a, apex, l, s)) continue;
+    if (cand_entry == NONE) {
+      mark_invalid(a, s);
+      return;
+    }
+    const uint32_t at = append_jobs(a, 1u);
+    if (at == NONE) {
+      atomicAdd(&a.stats[3], 1u);
+      fail_entry(a, cand_entry, s);
+      return;
+    }
+    a.bake_jobs[2 * (size_t)at] = cand_entry;
+    a.bake_jobs[2 * (size_t)at + 1] = k | (l << 8);
+    if (atomicExch(&a.slot_stale[s], 1u) == 0u) atomicAdd(&a.stats[6], 1u);
+  }
+}
+
+// WGSL :3432-3554 for static lights: one wave per 64 consecutive jobs, a
+// grid-stride loop over the planned and appended jobs.
+__global__ __launch_bounds__(64) void vc_bake_kernel(VcArgs a, const NodeC* __restrict__ blas,
+                                                     const NodeC* __restrict__ nodes,
+                                                     const float* __restrict__ tri_leaf,
+                                                     const uint32_t* __restrict__ items,
+                                                     const uint32_t* __restrict__ blas_items,
+                                                     const float* __restrict__ blas_cov, QueryParams p) {
+  __shared__ float s_acc[ACC_CELLS * 64];
+  __shared__ float s_poly[POLY_SLOTS * 64];
+  const uint32_t lane = lane_id();
+  Lane L{s_acc + lane, s_poly + lane, ACC_CELLS, 0u};
+  const Ctx x{blas, nodes, tri_leaf, a.tris, a.ranges, items, blas_items, blas_cov, a.xforms,
+              p.n_objects, p.hierarchy, p.profile_stage, p.object_cov};
+  const uint32_t appended = a.stats[5] < a.append_jobs ? a.stats[5] : a.append_jobs;
+  const uint32_t total = a.plan_jobs + appended;
+  for (uint32_t base = blockIdx.x * 64u; base < total; base += gridDim.x * 64u) {
+    const uint32_t j = base + lane;
+    const uint32_t* job = a.bake_jobs + 2 * (size_t)(j < total ? j : total - 1u);
+    const uint32_t jx = job[0], jy = job[1];
+    const bool listed = j < total && jx != NONE;
+    const uint32_t k = jy & 0xffu;
+    const uint32_t l = (jy >> 8) < a.n_lights - 1u ? (jy >> 8) : a.n_lights - 1u;
+    const PageTexel t = page_texel(a, jx == NONE ? 0u : jx, k);
+    const TexelSurface s = vc_surface_at(
+        a, t.m, t.c, f2{((float)t.texel[0] + 0.5f) / (float)t.n[0], ((float)t.texel[1] + 0.5f) / (float)t.n[1]});
+    const bool on = listed && t.inside && s.ok;
+    const LightRect r = reinterpret_cast<const LightRect*>(a.lights)[l];
+    // unshadowed_rect (WGSL :2135-2141)
+    float open = 0.0f;
+    if (faces_light(r, s.apex, s.normal))
+      open = is_sphere(r) ? spot_falloff(r, norm3(s.apex - f3{r.corner.x, r.corner.y, r.corner.z})) *
+                                sphere_unshadowed(s.apex, s.normal, r)
+                          : rect_open(L, s.apex, s.normal, r);
+    const f3 toward = norm3(light_center(r) - s.apex);
+    const bool back = on && open <= 0.0f && faces_light(r, s.apex, toward);
+    float seen = open;
+    if (back) {
+      seen = 0.0f;
+      if (faces_light(r, s.apex, toward))
+        seen = is_sphere(r) ? spot_falloff(r, norm3(s.apex - f3{r.corner.x, r.corner.y, r.corner.z})) *
+                                  sphere_unshadowed(s.apex, toward, r)
+                            : rect_open(L, s.apex, toward, r);
+    }
+    const Skip skip{back ? t.m : NONE, back ? s.tri : NONE};
+    const QueryOut o =
+        evaluate_at<false, true>(x, p, L, listed, s.apex, back ? toward : s.normal, on, r, nullptr, nullptr, skip);
+    const float value = o.visible;
+    const bool last_empty = o.touched == 0ull, last_full = o.full == ~0ull;
+    float ratio = !on ? 0.0f : (seen > 0.0f ? clampf(value / seen, 0.0f, 1.0f) : 1.0f);
+    if (on && seen > 0.0f && last_empty) ratio = 1.0f;
+    if (on && seen > 0.0f && last_full) ratio = 0.0f;
+    const float light_value = back ? 0.0f : value;
+    if (listed && t.inside)
+      a.pool[((size_t)t.e[3] * VC_TEXELS + k) * a.channels + l] = a.light_ratio ? ratio : (on ? light_value : 0.0f);
+  }
+}
+
+// WGSL :3693-3836 without sky or bounce light: one wave per baked page.
+__global__ __launch_bounds__(64) void vc_fill_kernel(VcArgs a) {
+  const uint32_t lane = lane_id();
+  const uint32_t appended = a.stats[0] < a.append_pages ? a.stats[0] : a.append_pages;
+  const uint32_t count = a.plan_pages + appended;
+  for (uint32_t p = blockIdx.x; p < count; p += gridDim.x) {
+    const uint32_t* e = a.bake_list + 4 * (size_t)p;
+    const float* bb = a.bake_bounds + 8 * (size_t)p;
+    if (bb[3] != 0.0f) continue;  // a page whose stale texels did not all fit stays invalid
+    const uint32_t slot = e[3];
+    uint32_t* info = a.slot_info + 12 * (size_t)slot;
+    const float kind = bb[7];
+    if (lane == 0) {
+      const uint32_t prev_w = info[3];
+      const bool same = prev_w != 0u && info[0] == e[0] && info[1] == e[1] && info[2] == e[2];
+      const bool fresh = p < a.plan_pages && (kind == 0.0f || kind == 2.0f || kind == 5.0f);
+      const bool keep = same && !fresh;
+      const uint32_t charge = keep ? info[7] : 0u;
+      const uint32_t moving = (fresh && a.n_movers > 0u) ? 4u : 0u;
+      info[0] = e[0], info[1] = e[1], info[2] = e[2];
+      info[3] = 1u | (keep ? (prev_w & 0xffffff06u) : 0u) | moving;
+      info[4] = __float_as_uint(bb[0]), info[5] = __float_as_uint(bb[1]), info[6] = __float_as_uint(bb[2]);
+      info[7] = charge;
+      for (int q = 0; q < 4; q++) info[8 + q] = __float_as_uint(bb[4 + q]);
+      if (kind == 5.0f) {
+      } else if (kind >= 3.0f) {
+        atomicCAS(&a.slot_stale[slot], 3u, 0u);
+      } else {
+        atomicExch(&a.slot_stale[slot], kind == 2.0f ? 3u : 0u);
+      }
+      if (p < a.plan_pages) a.slot_used[slot] = a.frame;
+    }
+    const uint32_t c = e[1] & vc::KEY_CHART_MASK, mip = e[1] >> vc::KEY_MIP_SHIFT;
+    uint32_t n[2];
+    vc_grid(a, c, e[0], mip, n);
+    const uint32_t px = (e[2] & 0xffffu) * vc::PAGE, py = (e[2] >> 16) * vc::PAGE;
+    float* o = a.pool + (size_t)slot * VC_TEXELS * a.channels;
+    const uint32_t sky = a.channels - 1u;
+    for (uint32_t k = lane; k < VC_TEXELS; k += 64u) {
+      // No sky here: its channel is zero, in the border as inside.
+      o[k * a.channels + sky] = 0.0f;
+      const uint32_t tx = px + k % VC_STRIDE, ty = py + k / VC_STRIDE;
+      if (tx >= n[0] || ty >= n[1]) {
+        const uint32_t ix = (tx < n[0] - 1u ? tx : n[0] - 1u) - px;
+        const uint32_t iy = (ty < n[1] - 1u ? ty : n[1] - 1u) - py;
+        const uint32_t nearest = iy * VC_STRIDE + ix;
+        for (uint32_t ch = 0; ch < sky; ch++) o[k * a.channels + ch] = o[nearest * a.channels + ch];
+      }
+    }
+  }
+}
+
+// WGSL :3134-3160 (`vc_find`, a page shown unless marked), then the page's
+// texels: one wave per key.
+__global__ __launch_bounds__(64) void vc_read_kernel(VcArgs a, const uint32_t* __restrict__ keys, uint32_t n_keys,
+                                                     float* __restrict__ out, uint32_t* __restrict__ out_slot) {
+  const uint32_t lane = lane_id();
+  const uint32_t i = blockIdx.x;
+  if (i >= n_keys) return;
+  const uint32_t k0 = keys[3 * (size_t)i], k1 = keys[3 * (size_t)i + 1], k2 = keys[3 * (size_t)i + 2];
+  const uint32_t kk[3] = {k0, k1, k2};
+  uint32_t h = (kk[0] * 0x9E3779B1u) ^ (kk[1] * 0x85EBCA77u) ^ (kk[2] * 0xC2B2AE3Du);
+  h ^= h >> 15;
+  h *= 0x2C1B3C6Du;
+  h ^= h >> 12;
+  uint32_t at = h & a.table_mask;
+  uint32_t slot = NONE;
+  for (uint32_t probe = 0; probe < vc::MAX_PROBE; probe++) {
+    const uint32_t* e = a.page_table + 4 * (size_t)at;
+    if (e[0] == vc::EMPTY) break;
+    if (e[0] == k0 && e[1] == k1 && e[2] == k2) {
+      const uint32_t stale = a.slot_stale[e[3]];
+      slot = (stale != 0u && stale != 3u) ? NONE : e[3];
+      break;
+    }
+    at = (at + 1u) & a.table_mask;
+  }
+  if (lane == 0) out_slot[i] = slot;
+  const uint32_t row = VC_TEXELS * a.channels;
+  float* dst = out + (size_t)i * row;
+  for (uint32_t q = lane; q < row; q += 64u) dst[q] = slot == NONE ? 0.0f : a.pool[(size_t)slot * row + q];
+}
+
+}  // namespace cnq
+
+// ---------------------------------------------------------------- host side
+
+using namespace cnq::detail;
+namespace vc = cnq::vc;
+using cnq::NONE;
+
+namespace {
+
+struct Key {
+  uint32_t k[3];
+  bool operator==(const Key& o) const { return k[0] == o.k[0] && k[1] == o.k[1] && k[2] == o.k[2]; }
+};
+struct KeyHash {
+  size_t operator()(const Key& a) const { return vc::page_hash(a.k) * 0x9E3779B97F4A7C15ull ^ a.k[2]; }
+};
+
+}  // namespace
+
+struct cnq_vcache {
+  cnq_scene* scene = nullptr;
+  cnq_vcache_config cfg;
+  cnq_options opts;
+  uint32_t n_lights = 0, channels = 0;
+  float inv_base = 1.0f;
+  std::vector<float> lights;
+  std::vector<vc::Box> light_boxes;
+  vc::ChartTables tables;
+  std::vector<float> scale;  // per object
+  vc::PageTable table;
+  std::unordered_map<Key, std::pair<uint32_t, uint32_t>, KeyHash> map;  // key -> (slot, entry)
+  std::vector<Key> slot_key;
+  std::vector<uint32_t> last_used, alloc_frame, free_slots;
+  uint32_t slot_extent = 0;
+  uint32_t frame = 1;
+  // The next frame's plan.
+  std::vector<uint32_t> plan_list, plan_jobs, used_marks, cleared;
+  std::vector<float> plan_bounds;
+  // Objects moved since the last frame.
+  std::vector<float> moved;
+  uint32_t n_movers = 0;
+  std::vector<std::pair<uint32_t, uint32_t>> mover_ids;
+  uint64_t last_stats[10] = {0};
+  std::vector<uint32_t> upd_host;  // staging for flush_updates (alive until the stream syncs)
+
+  DevBuf<vc::Chart> d_charts;
+  DevBuf<uint32_t> d_cells, d_cell_tris, d_page_table, d_slot_info, d_slot_stale, d_slot_used, d_bake_list,
+      d_bake_jobs, d_candidates, d_invalid, d_stats, d_updates;
+  DevBuf<float> d_tri_uv, d_normals, d_lights, d_pool, d_bake_bounds, d_moved;
+  uint32_t list_cap = 0, job_cap = 0;  // bake list / jobs capacity (plan + append)
+
+  void release() {
+    d_charts.release(), d_cells.release(), d_cell_tris.release(), d_page_table.release(), d_slot_info.release();
+    d_slot_stale.release(), d_slot_used.release(), d_bake_list.release(), d_bake_jobs.release();
+    d_candidates.release(), d_invalid.release(), d_stats.release(), d_updates.release(), d_tri_uv.release();
+    d_normals.release(), d_lights.release(), d_pool.release(), d_bake_bounds.release(), d_moved.release();
+  }
+  uint32_t append_jobs() const { return cfg.append_pages * vc::PAGE_TEXELS * n_lights; }
+
+  cnq::VcArgs args() {
+    cnq::VcArgs a;
+    a.charts = d_charts.p;
+    a.cells = d_cells.p;
+    a.cell_tris = d_cell_tris.p;
+    a.tri_uv = d_tri_uv.p;
+    a.normals = d_normals.p;
+    a.tris = scene->tris.p;
+    a.ranges = scene->ranges.p;
+    a.xforms = scene->xforms.p;
+    a.lights = d_lights.p;
+    a.moved = d_moved.p;
+    a.page_table = d_page_table.p;
+    a.slot_info = d_slot_info.p;
+    a.slot_stale = d_slot_stale.p;
+    a.slot_used = d_slot_used.p;
+    a.pool = d_pool.p;
+    a.bake_list = d_bake_list.p;
+    a.bake_bounds = d_bake_bounds.p;
+    a.bake_jobs = d_bake_jobs.p;
+    a.candidates = d_candidates.p;
+    a.invalid_list = d_invalid.p;
+    a.stats = d_stats.p;
+    a.n_slots = slot_extent;
+    a.n_lights = n_lights;
+    a.channels = channels;
+    a.n_movers = n_movers;
+    a.plan_pages = (uint32_t)(plan_list.size() / 4);
+    a.plan_jobs = (uint32_t)(plan_jobs.size() / 2);
+    a.append_pages = cfg.append_pages;
+    a.append_jobs = append_jobs();
+    a.candidate_cap = cfg.candidates;
+    a.invalid_cap = cfg.pages;
+    a.table_mask = table.len() - 1;
+    a.frame = frame;
+    a.visible_frames = cfg.visible_frames;
+    a.light_ratio = cfg.light_ratio;
+    a.inv_base = inv_base;
+    return a;
+  }
+
+  // Frees slot `slot` (its key leaves the table; its slot info is cleared
+  // before the next frame's passes).
+  void evict_slot(uint32_t slot) {
+    const Key k = slot_key[slot];
+    auto it = map.find(k);
+    if (it == map.end()) return;
+    const uint32_t entry = it->second.second;
+    map.erase(it);
+    auto& m = map;
+    table.remove(entry, [&m](const uint32_t* moved_key, uint32_t i) {
+      Key mk{{moved_key[0], moved_key[1], moved_key[2]}};
+      m[mk].second = i;
+    });
+    slot_key[slot] = Key{{vc::EMPTY, vc::EMPTY, vc::EMPTY}};
+    free_slots.push_back(slot);
+    cleared.push_back(slot);
+  }
+};
+
+namespace {
+
+// Sends the page-table entries changed since the last flush, the use marks
+// and the cleared slots (`vc_update`), on `st`; the caller synchronizes.
+int flush_updates(cnq_vcache* c, hipStream_t st) {
+  const std::vector<uint32_t> dirty = c->table.take_dirty();
+  std::vector<uint32_t>& upd = c->upd_host;
+  upd.clear();
+  for (uint32_t i : dirty) {
+    const uint32_t* e = &c->table.entries[4 * (size_t)i];
+    upd.insert(upd.end(), {i, e[0], e[1], e[2], e[3]});
+  }
+  upd.insert(upd.end(), c->used_marks.begin(), c->used_marks.end());
+  upd.insert(upd.end(), c->cleared.begin(), c->cleared.end());
+  const uint32_t n_table = (uint32_t)dirty.size(), n_used = (uint32_t)c->used_marks.size(),
+                 n_cleared = (uint32_t)c->cleared.size();
+  c->used_marks.clear();
+  c->cleared.clear();
+  const uint32_t n_upd = n_table + n_used + n_cleared;
+  if (n_upd == 0) return CNQ_OK;
+  if (upd.size() > c->d_updates.n) {
+    c->d_updates.release();
+    HIP_TRY(c->d_updates.alloc(upd.size() + 1024), "table updates");
+  }
+  HIP_TRY(hipMemcpyAsync(c->d_updates.p, upd.data(), upd.size() * 4, hipMemcpyHostToDevice, st), "table updates");
+  cnq::VcArgs a = c->args();
+  hipLaunchKernelGGL(cnq::vc_update_kernel, dim3((n_upd + 255) / 256), dim3(256), 0, st, a, c->d_updates.p, n_table,
+                     n_used, n_cleared);
+  HIP_TRY(hipGetLastError(), "vc_update launch");
+  return CNQ_OK;
+}
+
+uint32_t next_pow2(uint64_t v) {
+  uint64_t p = 1;
+  while (p < v) p <<= 1;
+  return (uint32_t)p;
+}
+
+}  // namespace
+
+extern "C" {
+
+void cnq_vcache_config_default(cnq_vcache_config* c) {
+  if (!c) return;
+  c->pages = 4096;
+  c->append_pages = 256;
+  c->candidates = 4096;
+  c->base_texel = 0.05f;
+  c->light_ratio = 1;
+  c->visible_frames = 1;
+}
+
+int cnq_vcache_create(cnq_scene* s, const cnq_vcache_surfaces* sf, const float* lights, uint32_t n_lights,
+                      const cnq_vcache_config* cfg, const cnq_options* opts, cnq_vcache** out) {
+  if (!s || !sf || !lights || !cfg || !out) return fail(CNQ_INVALID_ARG, "null argument");
+  *out = nullptr;
+  int rc = check_options(opts);
+  if (rc != CNQ_OK) return rc;
+  if (n_lights == 0 || n_lights > vc::MAX_LIGHTS)
+    return fail(CNQ_INVALID_ARG, "the view cache takes 1 to 16 lights (lib.rs:751 MAX_LIGHTS)");
+  if (cfg->pages == 0 || cfg->pages > (1u << 24)) return fail(CNQ_INVALID_ARG, "pages must be 1 .. 2^24");
+  if (!(cfg->base_texel > 0.0f)) return fail(CNQ_INVALID_ARG, "base_texel must be > 0");
+  if (s->n_tris == 0) return fail(CNQ_INVALID_ARG, "the scene has no triangles");
+  if (sf->n_charts == 0 || sf->n_charts >= (1u << 24))
+    return fail(CNQ_INVALID_ARG, "1 .. 2^24 - 1 charts (page keys hold 24 bits)");
+  if (!sf->tri_chart || !sf->tri_uv || !sf->normals || !sf->chart_size || !sf->chart_grid)
+    return fail(CNQ_INVALID_ARG, "surface arrays are null");
+  rc = check_object_cov(s, opts);
+  if (rc != CNQ_OK) return rc;
+  HIP_TRY(hipSetDevice(s->dev->ordinal), "hipSetDevice");
+  auto c = new cnq_vcache();
+  c->scene = s;
+  c->cfg = *cfg;
+  c->opts = *opts;
+  c->n_lights = n_lights;
+  c->channels = n_lights + 1;
+  c->inv_base = 1.0f / cfg->base_texel;
+  c->lights.assign(lights, lights + 16 * (size_t)n_lights);
+  for (uint32_t l = 0; l < n_lights; l++) c->light_boxes.push_back(vc::light_box(lights + 16 * (size_t)l));
+  // Triangles (object space) for the cells' boxes, from the device copy.
+  std::vector<float> tris(12 * (size_t)s->n_tris);
+  std::vector<float> xf(16 * (size_t)s->n_xforms);
+  hipError_t e = hipMemcpy(tris.data(), s->tris.p, tris.size() * 4, hipMemcpyDeviceToHost);
+  if (e == hipSuccess && !xf.empty()) e = hipMemcpy(xf.data(), s->xforms.p, xf.size() * 4, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    delete c;
+    return hip_fail(e, "scene copy");
+  }
+  if (!vc::chart_tables(tris.data(), s->n_tris, sf->tri_chart, sf->tri_uv, sf->chart_size, sf->chart_grid,
+                        sf->n_charts, c->tables)) {
+    delete c;
+    return fail(CNQ_INVALID_ARG, "a triangle's chart is out of range, or a chart has an empty grid");
+  }
+  c->scale.resize(s->n_objects);
+  for (uint32_t m = 0; m < s->n_objects; m++) {
+    const uint32_t x = s->h_ranges[4 * m + 3];
+    c->scale[m] = x == NONE ? 1.0f : xf[16 * (size_t)x + 12];
+  }
+  const uint32_t P = cfg->pages;
+  c->table = vc::PageTable(std::max<uint32_t>(next_pow2(4ull * P), 1024u));
+  c->slot_key.assign(P, Key{{vc::EMPTY, vc::EMPTY, vc::EMPTY}});
+  c->last_used.assign(P, 0);
+  c->alloc_frame.assign(P, 0);
+  for (uint32_t k = P; k > 0; k--) c->free_slots.push_back(k - 1);
+  c->list_cap = 64 + cfg->append_pages;
+  c->job_cap = 64 * vc::PAGE_TEXELS * n_lights + c->append_jobs();
+  const size_t pool = (size_t)P * vc::PAGE_TEXELS * c->channels;
+  const uint32_t stats_zero[cnq::VC_STATS] = {0};
+  e = c->d_charts.upload(c->tables.charts.data(), c->tables.charts.size());
+  if (e == hipSuccess) e = c->d_cells.upload(c->tables.cells.data(), c->tables.cells.size());
+  if (e == hipSuccess) e = c->d_cell_tris.upload(c->tables.cell_tris.data(), c->tables.cell_tris.size());
+  if (e == hipSuccess) e = c->d_tri_uv.upload(sf->tri_uv, 6 * (size_t)s->n_tris);
+  if (e == hipSuccess) e = c->d_normals.upload(sf->normals, 9 * (size_t)s->n_tris);
+  if (e == hipSuccess) e = c->d_lights.upload(lights, 16 * (size_t)n_lights);
+  if (e == hipSuccess) e = c->d_page_table.upload(c->table.entries.data(), c->table.entries.size());
+  if (e == hipSuccess) e = c->d_slot_info.alloc(12 * (size_t)P);
+  if (e == hipSuccess) e = hipMemset(c->d_slot_info.p, 0, c->d_slot_info.bytes());
+  if (e == hipSuccess) e = c->d_slot_stale.alloc(P);
+  if (e == hipSuccess) e = hipMemset(c->d_slot_stale.p, 0, c->d_slot_stale.bytes());
+  if (e == hipSuccess) e = c->d_slot_used.alloc(P);
+  if (e == hipSuccess) e = hipMemset(c->d_slot_used.p, 0, c->d_slot_used.bytes());
+  if (e == hipSuccess) e = c->d_pool.alloc(pool);
+  if (e == hipSuccess) e = hipMemset(c->d_pool.p, 0, c->d_pool.bytes());
+  if (e == hipSuccess) e = c->d_bake_list.alloc(4 * (size_t)c->list_cap);
+  if (e == hipSuccess) e = hipMemset(c->d_bake_list.p, 0, c->d_bake_list.bytes());
+  if (e == hipSuccess) e = c->d_bake_bounds.alloc(8 * (size_t)c->list_cap);
+  if (e == hipSuccess) e = hipMemset(c->d_bake_bounds.p, 0, c->d_bake_bounds.bytes());
+  if (e == hipSuccess) e = c->d_bake_jobs.alloc(2 * (size_t)c->job_cap);
+  if (e == hipSuccess) e = hipMemset(c->d_bake_jobs.p, 0, c->d_bake_jobs.bytes());
+  if (e == hipSuccess) e = c->d_candidates.alloc(2 * (size_t)std::max<uint32_t>(cfg->candidates, 1));
+  if (e == hipSuccess) e = c->d_invalid.alloc(P);
+  if (e == hipSuccess) e = c->d_stats.upload(stats_zero, cnq::VC_STATS);
+  if (e == hipSuccess) e = c->d_moved.alloc(4 * (2 * (size_t)n_lights + 2));
+  if (e == hipSuccess) e = hipMemset(c->d_moved.p, 0, c->d_moved.bytes());
+  if (e != hipSuccess) {
+    c->release();
+    delete c;
+    return hip_fail(e, "view cache allocation");
+  }
+  *out = c;
+  return CNQ_OK;
+}
+
+void cnq_vcache_destroy(cnq_vcache* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->scene->dev->ordinal);
+  (void)hipDeviceSynchronize();
+  c->release();
+  delete c;
+}
+
+int cnq_vcache_grid(cnq_vcache* c, uint32_t object, uint32_t chart, uint32_t mip, uint32_t out[2]) {
+  if (!c || !out) return fail(CNQ_INVALID_ARG, "null argument");
+  if (object >= c->scene->n_objects || chart >= c->tables.charts.size() || mip > 30)
+    return fail(CNQ_INVALID_ARG, "object, chart or mip out of range");
+  vc::grid(c->tables.charts[chart], c->scale[object], c->inv_base, mip, out);
+  return CNQ_OK;
+}
+
+int cnq_vcache_request(cnq_vcache* c, const uint32_t* keys, uint32_t n, uint32_t* slots) {
+  if (!c || (n && (!keys || !slots))) return fail(CNQ_INVALID_ARG, "null argument");
+  for (uint32_t i = 0; i < n; i++) {
+    const uint32_t* k = keys + 3 * (size_t)i;
+    const uint32_t m = k[0], chart = k[1] & vc::KEY_CHART_MASK, mip = k[1] >> vc::KEY_MIP_SHIFT;
+    if (m >= c->scene->n_objects || chart >= c->tables.charts.size() || mip > 30)
+      return fail(CNQ_INVALID_ARG, "key " + std::to_string(i) + ": object, chart or mip out of range");
+  }
+  for (uint32_t i = 0; i < n; i++) {
+    const uint32_t* k = keys + 3 * (size_t)i;
+    const Key key{{k[0], k[1], k[2]}};
+    slots[i] = NONE;
+    auto it = c->map.find(key);
+    if (it != c->map.end()) {
+      const uint32_t slot = it->second.first;
+      if (c->last_used[slot] != c->frame) c->used_marks.push_back(slot);
+      c->last_used[slot] = c->frame;
+      slots[i] = slot;
+      continue;
+    }
+    const uint32_t m = k[0], chart = k[1] & vc::KEY_CHART_MASK, mip = k[1] >> vc::KEY_MIP_SHIFT;
+    const uint32_t px = k[2] & 0xffffu, py = k[2] >> 16;
+    // vcache.rs:3943-3957: the grid one larger (the shaders' f32 grid may
+    // be), the texels inside it and one past them.
+    uint32_t g[2];
+    vc::grid(c->tables.charts[chart], c->scale[m], c->inv_base, mip, g);
+    uint32_t listed[2];
+    bool empty = false;
+    const uint32_t pp[2] = {px, py};
+    for (int a = 0; a < 2; a++) {
+      const uint32_t nn = g[a] + 1;
+      const uint32_t start = std::min(pp[a] * vc::PAGE, nn);
+      const uint32_t inside = std::min(nn - start, vc::STRIDE);
+      // The reference's slack of one texel covers a shader grid one larger
+      // than the host's; here both are the same f32 products, so a page
+      // starting past the grid holds no texel and is refused.
+      if (inside == 0 || (uint64_t)pp[a] * vc::PAGE >= g[a]) empty = true;
+      listed[a] = std::min(inside + 1, vc::STRIDE);
+    }
+    if (empty || c->free_slots.empty()) continue;
+    const uint32_t slot = c->free_slots.back();
+    c->free_slots.pop_back();
+    const uint32_t entry = c->table.insert(key.k, slot);
+    if (entry == vc::EMPTY) {
+      c->free_slots.push_back(slot);
+      continue;
+    }
+    c->slot_extent = std::max(c->slot_extent, slot + 1);
+    c->map[key] = {slot, entry};
+    c->slot_key[slot] = key;
+    c->last_used[slot] = c->frame;
+    c->alloc_frame[slot] = c->frame;
+    const uint32_t index = (uint32_t)(c->plan_list.size() / 4);
+    c->plan_list.insert(c->plan_list.end(), {k[0], k[1], k[2], slot});
+    float b[8];
+    vc::page_bounds(c->tables, chart, c->scale[m], c->inv_base, mip, px, py, b);
+    b[3] = 0.0f;
+    b[7] = 0.0f;  // kind 0: lights baked, no sky or bounce samples
+    c->plan_bounds.insert(c->plan_bounds.end(), b, b + 8);
+    for (uint32_t l = 0; l < c->n_lights; l++)
+      for (uint32_t y = 0; y < listed[1]; y++)
+        for (uint32_t x = 0; x < listed[0]; x++)
+          c->plan_jobs.insert(c->plan_jobs.end(), {index, (y * vc::STRIDE + x) | (l << 8)});
+    slots[i] = slot;
+  }
+  return CNQ_OK;
+}
+
+int cnq_vcache_evict(cnq_vcache* c, const uint32_t* keys, uint32_t n) {
+  if (!c || (n && !keys)) return fail(CNQ_INVALID_ARG, "null argument");
+  for (uint32_t i = 0; i < n; i++) {
+    const uint32_t* k = keys + 3 * (size_t)i;
+    auto it = c->map.find(Key{{k[0], k[1], k[2]}});
+    if (it == c->map.end()) continue;
+    const uint32_t slot = it->second.first;
+    if (c->alloc_frame[slot] == c->frame) {
+      // Planned this frame and not baked yet: take it out of the plan.
+      return fail(CNQ_INVALID_ARG, "a page planned this frame is evicted only after its frame");
+    }
+    c->evict_slot(slot);
+  }
+  // The device's table follows at once, so reads no longer find the pages.
+  HIP_TRY(hipSetDevice(c->scene->dev->ordinal), "hipSetDevice");
+  const int rc = flush_updates(c, nullptr);
+  if (rc != CNQ_OK) return rc;
+  HIP_TRY(hipStreamSynchronize(nullptr), "view cache eviction");
+  return CNQ_OK;
+}
+
+int cnq_vcache_set_moved(cnq_vcache* c, const cnq_moved* moved, uint32_t n, const float* scene_box) {
+  if (!c || (n && !moved)) return fail(CNQ_INVALID_ARG, "null argument");
+  std::vector<vc::Moved> list(n);
+  for (uint32_t i = 0; i < n; i++) {
+    if (moved[i].object >= c->scene->n_objects) return fail(CNQ_INVALID_ARG, "moved object out of range");
+    list[i].object = moved[i].object;
+    std::memcpy(list[i].old_box.lo, moved[i].old_lo, 12);
+    std::memcpy(list[i].old_box.hi, moved[i].old_hi, 12);
+    std::memcpy(list[i].new_box.lo, moved[i].new_lo, 12);
+    std::memcpy(list[i].new_box.hi, moved[i].new_hi, 12);
+  }
+  vc::Box sb;
+  if (scene_box) {
+    std::memcpy(sb.lo, scene_box, 12);
+    std::memcpy(sb.hi, scene_box + 3, 12);
+  } else {
+    // instanced.rs:1395-1406: the object hierarchy's root box with the
+    // movers' old boxes, padded.
+    cnq::NodeC root;
+    HIP_TRY(hipSetDevice(c->scene->dev->ordinal), "hipSetDevice");
+    HIP_TRY(hipMemcpy(&root, c->scene->nodes.p, sizeof(root), hipMemcpyDeviceToHost), "root box");
+    float lo[3], hi[3];
+    for (int a = 0; a < 3; a++) lo[a] = root.lo[a], hi[a] = root.hi[a];
+    for (const vc::Moved& m : list)
+      for (int a = 0; a < 3; a++) {
+        lo[a] = std::min(lo[a], m.old_box.lo[a]);
+        hi[a] = std::max(hi[a], m.old_box.hi[a]);
+      }
+    float e = 1.0f;
+    for (int a = 0; a < 3; a++) e = std::max(e, std::max(std::fabs(lo[a]), std::fabs(hi[a])));
+    const float pad = 1e-3f * e;
+    for (int a = 0; a < 3; a++) sb.lo[a] = lo[a] - pad, sb.hi[a] = hi[a] + pad;
+  }
+  c->moved = vc::pack_moved(list, c->light_boxes, sb, c->mover_ids);
+  c->n_movers = n;
+  return CNQ_OK;
+}
+
+int cnq_vcache_frame(cnq_vcache* c, void* stream, uint64_t stats[10]) {
+  if (!c) return fail(CNQ_INVALID_ARG, "null argument");
+  cnq_scene* s = c->scene;
+  HIP_TRY(hipSetDevice(s->dev->ordinal), "hipSetDevice");
+  hipStream_t st = (hipStream_t)stream;
+  const uint32_t plan_pages = (uint32_t)(c->plan_list.size() / 4), plan_jobs = (uint32_t)(c->plan_jobs.size() / 2);
+  // Capacities for this frame's plan and appends.
+  if (plan_pages + c->cfg.append_pages > c->list_cap) {
+    const uint32_t cap = plan_pages + c->cfg.append_pages + 64;
+    c->d_bake_list.release(), c->d_bake_bounds.release();
+    HIP_TRY(c->d_bake_list.alloc(4 * (size_t)cap), "bake list");
+    HIP_TRY(c->d_bake_bounds.alloc(8 * (size_t)cap), "bake bounds");
+    HIP_TRY(hipMemset(c->d_bake_list.p, 0, c->d_bake_list.bytes()), "bake list");
+    c->list_cap = cap;
+  }
+  if ((uint64_t)plan_jobs + c->append_jobs() > c->job_cap) {
+    const uint64_t cap = (uint64_t)plan_jobs + c->append_jobs() + 64;
+    if (cap > 0xffffffffull) return fail(CNQ_CAPACITY, "too many bake jobs in one frame");
+    c->d_bake_jobs.release();
+    HIP_TRY(c->d_bake_jobs.alloc(2 * cap), "bake jobs");
+    HIP_TRY(hipMemset(c->d_bake_jobs.p, 0, c->d_bake_jobs.bytes()), "bake jobs");
+    c->job_cap = (uint32_t)cap;
+  }
+  const size_t moved_floats = std::max<size_t>(c->moved.size(), 4 * (2 * (size_t)c->n_lights + 2));
+  if (moved_floats > c->d_moved.n) {
+    c->d_moved.release();
+    HIP_TRY(c->d_moved.alloc(moved_floats), "moved boxes");
+  }
+  if (plan_pages) {
+    HIP_TRY(hipMemcpyAsync(c->d_bake_list.p, c->plan_list.data(), c->plan_list.size() * 4, hipMemcpyHostToDevice, st),
+            "bake list");
+    HIP_TRY(hipMemcpyAsync(c->d_bake_bounds.p, c->plan_bounds.data(), c->plan_bounds.size() * 4,
+                           hipMemcpyHostToDevice, st), "bake bounds");
+  }
+  if (plan_jobs)
+    HIP_TRY(hipMemcpyAsync(c->d_bake_jobs.p, c->plan_jobs.data(), c->plan_jobs.size() * 4, hipMemcpyHostToDevice, st),
+            "bake jobs");
+  if (!c->moved.empty())
+    HIP_TRY(hipMemcpyAsync(c->d_moved.p, c->moved.data(), c->moved.size() * 4, hipMemcpyHostToDevice, st), "moved");
+  HIP_TRY(fill_u32(c->d_stats.p, 0, cnq::VC_STATS, st), "stats clear");
+  // Page-table entries changed since the last frame, use marks, cleared slots.
+  int rc = flush_updates(c, st);
+  if (rc != CNQ_OK) return rc;
+  cnq::VcArgs a = c->args();
+  if (c->moved.empty()) a.n_movers = 0;
+  if (a.n_movers > 0 && c->slot_extent > 0) {
+    hipLaunchKernelGGL(cnq::vc_stale_kernel, dim3((c->slot_extent + 255) / 256), dim3(256), 0, st, a);
+    HIP_TRY(hipGetLastError(), "vc_stale launch");
+    const uint64_t threads = (uint64_t)c->cfg.candidates * vc::PAGE_TEXELS;
+    if (threads) {
+      hipLaunchKernelGGL(cnq::vc_stale_texels_kernel, dim3((uint32_t)((threads + 255) / 256)), dim3(256), 0, st, a);
+      HIP_TRY(hipGetLastError(), "vc_stale_texels launch");
+    }
+  }
+  const cnq::QueryParams p = query_params(s, &c->opts, 64);
+  const uint64_t job_waves = ((uint64_t)plan_jobs + (a.n_movers ? a.append_jobs : 0) + 63) / 64;
+  if (job_waves) {
+    const uint32_t grid = (uint32_t)std::min<uint64_t>(job_waves, 8192);
+    hipLaunchKernelGGL(cnq::vc_bake_kernel, dim3(grid), dim3(64), 0, st, a, s->blas.p, s->nodes.p, s->tri_leaf.p,
+                       s->items.p, s->blas_items.p, s->blas_cov.p, p);
+    HIP_TRY(hipGetLastError(), "vc_bake launch");
+  }
+  const uint32_t page_blocks = std::min<uint32_t>(plan_pages + (a.n_movers ? c->cfg.append_pages : 0), 4096);
+  if (page_blocks) {
+    hipLaunchKernelGGL(cnq::vc_fill_kernel, dim3(page_blocks), dim3(64), 0, st, a);
+    HIP_TRY(hipGetLastError(), "vc_fill launch");
+  }
+  uint32_t dstats[cnq::VC_STATS];
+  HIP_TRY(hipMemcpyAsync(dstats, c->d_stats.p, sizeof(dstats), hipMemcpyDeviceToHost, st), "stats");
+  HIP_TRY(hipStreamSynchronize(st), "view cache frame");
+  // Pages marked invalid leave the cache (vcache.rs:3425-3440); past the
+  // list's capacity, every marked slot does (:3444-3452).
+  std::vector<uint32_t> invalid;
+  uint64_t evicted = 0;
+  if (dstats[1] > 0) {
+    if (dstats[1] <= c->cfg.pages) {
+      invalid.resize(dstats[1]);
+      HIP_TRY(hipMemcpy(invalid.data(), c->d_invalid.p, invalid.size() * 4, hipMemcpyDeviceToHost), "invalid list");
+    } else {
+      std::vector<uint32_t> marks(c->slot_extent);
+      HIP_TRY(hipMemcpy(marks.data(), c->d_slot_stale.p, marks.size() * 4, hipMemcpyDeviceToHost), "stale marks");
+      for (uint32_t sl = 0; sl < c->slot_extent; sl++)
+        if (marks[sl] != 0) invalid.push_back(sl);
+    }
+  }
+  // In slot order (the list's order is the atomics'), so the free list, and
+  // the slots the next pages take, do not depend on it.
+  std::sort(invalid.begin(), invalid.end());
+  for (uint32_t slot : invalid) {
+    if (c->slot_key[slot].k[0] == vc::EMPTY || c->alloc_frame[slot] >= c->frame) continue;
+    c->evict_slot(slot);
+    evicted++;
+  }
+  // The device's table follows the evictions at once (the reference sends
+  // them with the next frame; reads between frames see the same pages).
+  rc = flush_updates(c, st);
+  if (rc != CNQ_OK) return rc;
+  HIP_TRY(hipStreamSynchronize(st), "view cache evictions");
+  const uint64_t st_out[10] = {plan_pages,
+                               plan_jobs,
+                               std::min(dstats[0], c->cfg.append_pages),
+                               std::min<uint64_t>(dstats[5], c->append_jobs()),
+                               dstats[4],
+                               dstats[1],
+                               dstats[6],
+                               dstats[3],
+                               dstats[2],
+                               evicted};
+  std::memcpy(c->last_stats, st_out, sizeof(st_out));
+  if (stats) std::memcpy(stats, st_out, sizeof(st_out));
+  c->plan_list.clear(), c->plan_bounds.clear(), c->plan_jobs.clear(), c->used_marks.clear();
+  c->moved.clear();
+  c->n_movers = 0;
+  c->frame++;
+  return CNQ_OK;
+}
+
+int cnq_vcache_read(cnq_vcache* c, const uint32_t* keys, uint32_t n, float* texels, uint32_t* slots) {
+  if (!c || (n && (!keys || !texels || !slots))) return fail(CNQ_INVALID_ARG, "null argument");
+  if (n == 0) return CNQ_OK;
+  HIP_TRY(hipSetDevice(c->scene->dev->ordinal), "hipSetDevice");
+  DevBuf<uint32_t> dk, ds;
+  DevBuf<float> dt;
+  const size_t row = (size_t)vc::PAGE_TEXELS * c->channels;
+  hipError_t e = dk.upload(keys, 3 * (size_t)n);
+  if (e == hipSuccess) e = ds.alloc(n);
+  if (e == hipSuccess) e = dt.alloc(row * n);
+  if (e == hipSuccess) {
+    cnq::VcArgs a = c->args();
+    hipLaunchKernelGGL(cnq::vc_read_kernel, dim3(n), dim3(64), 0, 0, a, dk.p, n, dt.p, ds.p);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(texels, dt.p, row * n * 4, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(slots, ds.p, 4 * (size_t)n, hipMemcpyDeviceToHost);
+  dk.release(), ds.release(), dt.release();
+  if (e != hipSuccess) return hip_fail(e, "view cache read");
+  return CNQ_OK;
+}
+
+int cnq_vcache_state(cnq_vcache* c, uint32_t* slot_info, uint32_t* slot_stale, uint32_t* slot_used, float* pool,
+                     uint32_t* table_device, uint32_t* table_host, uint32_t* sizes) {
+  if (!c) return fail(CNQ_INVALID_ARG, "null argument");
+  HIP_TRY(hipSetDevice(c->scene->dev->ordinal), "hipSetDevice");
+  const uint32_t P = c->cfg.pages;
+  if (sizes) {
+    sizes[0] = P;
+    sizes[1] = c->channels;
+    sizes[2] = c->table.len();
+    sizes[3] = c->slot_extent;
+    sizes[4] = c->frame;
+  }
+  if (slot_info) HIP_TRY(hipMemcpy(slot_info, c->d_slot_info.p, 48 * (size_t)P, hipMemcpyDeviceToHost), "slot info");
+  if (slot_stale) HIP_TRY(hipMemcpy(slot_stale, c->d_slot_stale.p, 4 * (size_t)P, hipMemcpyDeviceToHost), "stale");
+  if (slot_used) HIP_TRY(hipMemcpy(slot_used, c->d_slot_used.p, 4 * (size_t)P, hipMemcpyDeviceToHost), "used");
+  if (pool) HIP_TRY(hipMemcpy(pool, c->d_pool.p, c->d_pool.bytes(), hipMemcpyDeviceToHost), "pool");
+  if (table_device)
+    HIP_TRY(hipMemcpy(table_device, c->d_page_table.p, c->d_page_table.bytes(), hipMemcpyDeviceToHost), "table");
+  if (table_host) std::memcpy(table_host, c->table.entries.data(), c->table.entries.size() * 4);
+  return CNQ_OK;
+}
+
+}  // extern "C"

@@ -336,6 +336,7 @@ struct Ctx {
   const float* __restrict__ tris;
   const uint32_t* __restrict__ ranges;
   const uint32_t* __restrict__ items;
+  const uint32_t* __restrict__ blas_items;
   const float* __restrict__ blas_cov;
   const float* __restrict__ xforms;
   uint32_t n_objects, hierarchy, profile_stage, object_cov;
@@ -352,6 +353,13 @@ struct Walk {
   bool truncated;  // deeper than this stack's 64 (the walk stopped)
   uint32_t work[6];  // object nodes tested, blas nodes tested, triangles tested, triangles drawn,
                      // meshes entered, blas nodes splatted
+};
+
+// A triangle the cone does not see (object, triangle in `tris`): a cache
+// texel's own, when it looks for its light below its horizon (WGSL
+// :587-591, :728). Read only by walks instantiated with SKIP.
+struct Skip {
+  uint32_t obj, tri;
 };
 
 // WGSL :623-638 (the cone in an object's space, for transform `xf`)
@@ -597,9 +605,9 @@ __device__ __forceinline__ void clear_acc(Lane& L) {
 }
 
 // WGSL :719-829 for mesh `m` (uniform), for the lanes with `on`.
-template <bool COUNT>
+template <bool COUNT, bool SKIP = false>
 __device__ __forceinline__ void add_mesh_coverage(const Ctx& x, const Cone& wc, uint32_t m, bool on, Walk& w,
-                                                  Stack& st, uint32_t base, Lane& L) {
+                                                  Stack& st, uint32_t base, Lane& L, Skip skip = Skip{NONE, NONE}) {
   if (x.profile_stage == 3) return;
   if (!ballot(on)) return;
   if (COUNT && on) w.work[4]++;
@@ -611,6 +619,7 @@ __device__ __forceinline__ void add_mesh_coverage(const Ctx& x, const Cone& wc, 
     c = cone_to_object(wc, x.xforms + 16 * (size_t)xform);
     weight = 1.0f - mx(x.xforms[16 * (size_t)xform + 14], 0.0f);
   }
+  const uint32_t cover_skip = (SKIP && m == skip.obj) ? skip.tri : NONE;
   clear_acc(L);
   bool touched = false;
   if (x.hierarchy == 2) {
@@ -646,7 +655,7 @@ __device__ __forceinline__ void add_mesh_coverage(const Ctx& x, const Cone& wc, 
           for (uint32_t k = k0; k < k1; k++) {
             const float* t = x.tri_leaf + 9 * (size_t)k;
             const f3 v0{t[0], t[1], t[2]}, v1{t[3], t[4], t[5]}, v2{t[6], t[7], t[8]};
-            if (descend) {
+            if (descend && (!SKIP || x.blas_items[k] != cover_skip)) {
               if (COUNT) w.work[2]++;
               if (cover_triangle(c, L, v0, v1, v2, weight, x.profile_stage, w.work[3])) touched = true;
             }
@@ -666,6 +675,7 @@ __device__ __forceinline__ void add_mesh_coverage(const Ctx& x, const Cone& wc, 
   } else if (on) {
     for (uint32_t t = first; t < first + count; t++) {
       const float* tr = x.tris + 12 * (size_t)t;
+      if (SKIP && t == cover_skip) continue;
       if (COUNT) w.work[2]++;
       if (cover_triangle(c, L, ld3(tr), ld3(tr + 4), ld3(tr + 8), weight, x.profile_stage, w.work[3])) touched = true;
     }
@@ -708,8 +718,9 @@ __device__ __forceinline__ bool splat_object_node(const Ctx& x, const Cone& c, W
 }
 
 // WGSL :1160-1227 for the lanes with `traced`.
-template <bool COUNT>
-__device__ __forceinline__ void scene_coverage(const Ctx& x, const Cone& c, bool traced, Walk& w, Lane& L) {
+template <bool COUNT, bool SKIP = false>
+__device__ __forceinline__ void scene_coverage(const Ctx& x, const Cone& c, bool traced, Walk& w, Lane& L,
+                                               Skip skip = Skip{NONE, NONE}) {
   w.cone_work = 0;
   Stack st{0u, 0u, 0u};
   if (x.hierarchy >= 1) {
@@ -736,7 +747,7 @@ __device__ __forceinline__ void scene_coverage(const Ctx& x, const Cone& c, bool
       if (!dm) continue;
       const uint32_t count = node.b & ~NODE_NO_SPLAT;
       if (count > 0) {
-        for (uint32_t k = node.a; k < node.a + count; k++) add_mesh_coverage<COUNT>(x, c, x.items[k], descend, w, st, sp, L);
+        for (uint32_t k = node.a; k < node.a + count; k++) add_mesh_coverage<COUNT, SKIP>(x, c, x.items[k], descend, w, st, sp, L, skip);
       } else {
         uint32_t nearc = node.a, farc = node.a + 1;
         const NodeC na = x.nodes[nearc];
@@ -763,7 +774,7 @@ __device__ __forceinline__ void scene_coverage(const Ctx& x, const Cone& c, bool
       }
     }
   } else {
-    for (uint32_t m = 0; m < x.n_objects; m++) add_mesh_coverage<COUNT>(x, c, m, traced && w.full < BINS, w, st, 0, L);
+    for (uint32_t m = 0; m < x.n_objects; m++) add_mesh_coverage<COUNT, SKIP>(x, c, m, traced && w.full < BINS, w, st, 0, L, skip);
   }
 }
 
@@ -775,13 +786,13 @@ struct QueryOut {
   bool capped, overflow, truncated;
 };
 
-template <bool COUNT>
-__device__ __forceinline__ QueryOut evaluate_query(const Ctx& x, const QueryParams& p, Lane& L, bool valid,
-                                                   const float* __restrict__ rc, const LightRect& r,
-                                                   float* __restrict__ cov_out, uint64_t* __restrict__ work_out) {
-  const f3 point = ld3(rc);
-  const f3 normal = ld3(rc + 4);
-  const bool active = valid && rc[7] >= 0.0f;
+// `active`: the lane traces (a valid query at a point that is not a sky or
+// light pixel); `skip`: the triangle this cone does not see (with SKIP).
+template <bool COUNT, bool SKIP = false>
+__device__ __forceinline__ QueryOut evaluate_at(const Ctx& x, const QueryParams& p, Lane& L, bool valid, f3 point,
+                                                f3 normal, bool active, const LightRect& r,
+                                                float* __restrict__ cov_out, uint64_t* __restrict__ work_out,
+                                                Skip skip = Skip{NONE, NONE}) {
   Walk w;
 #pragma unroll
   for (uint32_t b = 0; b < BINS; b++) w.cov[b] = 0.0f;
@@ -805,7 +816,7 @@ __device__ __forceinline__ QueryOut evaluate_query(const Ctx& x, const QueryPara
     lc = light_cone_rect(point, r, p.wide_extent);
     const bool traced = on && lc.ok;
     w.cone_cap = lc.wide_window ? p.wide_cone_cap : p.light_cone_cap;
-    if (p.profile_stage != 2) scene_coverage<COUNT>(x, lc.c, traced, w, L);
+    if (p.profile_stage != 2) scene_coverage<COUNT, SKIP>(x, lc.c, traced, w, L, skip);
     capped = traced && w.cone_work >= w.cone_cap;
     if (w.cone_work >= w.cone_cap && lc.wide_window && !lc.wide_clipped) {
 #pragma unroll
@@ -857,6 +868,15 @@ __device__ __forceinline__ QueryOut evaluate_query(const Ctx& x, const QueryPara
     }
   }
   return o;
+}
+
+// One query at receiver `rc` (the reference's ReceiverGpu: point, normal;
+// a negative normal.w traces nothing).
+template <bool COUNT>
+__device__ __forceinline__ QueryOut evaluate_query(const Ctx& x, const QueryParams& p, Lane& L, bool valid,
+                                                   const float* __restrict__ rc, const LightRect& r,
+                                                   float* __restrict__ cov_out, uint64_t* __restrict__ work_out) {
+  return evaluate_at<COUNT>(x, p, L, valid, ld3(rc), ld3(rc + 4), valid && rc[7] >= 0.0f, r, cov_out, work_out);
 }
 
 }  // namespace cnq

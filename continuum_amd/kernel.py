@@ -217,6 +217,15 @@ class Scene:
         (instanced.rs:1226-1410)."""
         check(lib().cnq_scene_apply_moves(self.handle, _stream_ptr(stream)))
 
+    def last_moves(self) -> list:
+        """The last apply_moves's moves: (object, old box (lo, hi), new box),
+        as `_lib.MovedBox` records for ViewCache.set_moved."""
+        cap = len(getattr(self, "_mover_objects", ())) or 1
+        out = (_lib.MovedBox * cap)()
+        n = C.c_uint32()
+        check(lib().cnq_scene_last_moves(self.handle, out, cap, C.byref(n)))
+        return list(out[: n.value])
+
     def __del__(self):
         if getattr(self, "handle", None):
             self._destroy(self.handle)
@@ -548,6 +557,107 @@ class LiveLightmap:
     def rebaked_blocks(self) -> int:
         """LiveRenderer::rebaked_blocks (live.rs:1083)."""
         return len(self.read()[1])
+
+
+class ViewCache:
+    """The view cache's pages (the reference's `VirtualCache`, vcache.rs):
+    9 x 9 texels per page of one chart of one object at one mip, baked with
+    one mask cone per texel and light, kept current as objects move. Keys
+    are (object, chart | mip << 24, page x | page y << 16).
+
+    `surface` is a continuum_amd.surfaces.Surface over the scene's triangles
+    (charts numbered globally); `lights` LightGpu rows (at most 16)."""
+
+    def __init__(self, scene: Scene, surface, lights: np.ndarray, opts: Optional[_lib.Options] = None, **config):
+        self.scene = scene
+        self.opts = opts or default_options()
+        cfg = _lib.VcacheConfig()
+        lib().cnq_vcache_config_default(C.byref(cfg))
+        for k, v in config.items():
+            setattr(cfg, k, v)
+        self.config = cfg
+        self._tri_chart = np.ascontiguousarray(surface.tri_chart, np.uint32)
+        self._tri_uv = np.ascontiguousarray(surface.tri_uv, np.float32).reshape(-1, 6)
+        self._normals = np.ascontiguousarray(surface.normals, np.float32).reshape(-1, 9)
+        self._chart_size = np.ascontiguousarray(surface.chart_size, np.float32).reshape(-1, 2)
+        self._chart_grid = np.ascontiguousarray(surface.chart_grid, np.uint32).reshape(-1, 2)
+        n_tris = len(scene.flat.tris)
+        if not (len(self._tri_chart) == len(self._tri_uv) == len(self._normals) == n_tris):
+            raise ValueError("one chart, 6 chart coordinates and 9 normal floats per scene triangle")
+        sf = _lib.VcacheSurfaces(_ptr(self._tri_chart), _ptr(self._tri_uv), _ptr(self._normals),
+                                 len(self._chart_size), _ptr(self._chart_size), _ptr(self._chart_grid))
+        self.lights = np.ascontiguousarray(lights, np.float32).reshape(-1, 16)
+        self.channels = len(self.lights) + 1
+        h = C.c_void_p()
+        check(lib().cnq_vcache_create(scene.handle, C.byref(sf), _ptr(self.lights), len(self.lights), C.byref(cfg),
+                                      C.byref(self.opts), C.byref(h)))
+        self.handle = h
+        self._destroy = lib().cnq_vcache_destroy
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            self._destroy(self.handle)
+            self.handle = None
+
+    @staticmethod
+    def key(obj: int, chart: int, mip: int, px: int, py: int) -> tuple:
+        return (int(obj), int(chart) | (int(mip) << 24), int(px) | (int(py) << 16))
+
+    def grid(self, obj: int, chart: int, mip: int) -> tuple:
+        out = np.zeros(2, np.uint32)
+        check(lib().cnq_vcache_grid(self.handle, int(obj), int(chart), int(mip), _ptr(out)))
+        return int(out[0]), int(out[1])
+
+    def pages_of(self, obj: int, chart: int, mip: int) -> list:
+        """Every page key of a chart at a mip."""
+        gx, gy = self.grid(obj, chart, mip)
+        return [self.key(obj, chart, mip, x, y) for y in range((gy + 7) // 8) for x in range((gx + 7) // 8)]
+
+    def request(self, keys) -> np.ndarray:
+        k = np.ascontiguousarray(keys, np.uint32).reshape(-1, 3)
+        slots = np.zeros(len(k), np.uint32)
+        check(lib().cnq_vcache_request(self.handle, _ptr(k), len(k), _ptr(slots)))
+        return slots
+
+    def evict(self, keys) -> None:
+        k = np.ascontiguousarray(keys, np.uint32).reshape(-1, 3)
+        check(lib().cnq_vcache_evict(self.handle, _ptr(k), len(k)))
+
+    def set_moved(self, moved, scene_box=None) -> None:
+        arr = (_lib.MovedBox * max(len(moved), 1))(*moved)
+        sb = None if scene_box is None else np.ascontiguousarray(scene_box, np.float32).reshape(6)
+        check(lib().cnq_vcache_set_moved(self.handle, arr, len(moved), 0 if sb is None else _ptr(sb)))
+
+    def frame(self, stream=None) -> dict:
+        st = np.zeros(10, np.uint64)
+        check(lib().cnq_vcache_frame(self.handle, _stream_ptr(stream), _ptr(st)))
+        names = ("planned_pages", "planned_jobs", "appended_pages", "appended_jobs", "candidates", "invalid",
+                 "stale_pages", "overflow", "visible_tested", "evicted")
+        return {n: int(v) for n, v in zip(names, st)}
+
+    def read(self, keys):
+        """(texels (n, 81, channels), slots: NONE where absent or stale)."""
+        k = np.ascontiguousarray(keys, np.uint32).reshape(-1, 3)
+        tex = np.zeros((len(k), 81, self.channels), np.float32)
+        slots = np.zeros(len(k), np.uint32)
+        if len(k):
+            check(lib().cnq_vcache_read(self.handle, _ptr(k), len(k), _ptr(tex), _ptr(slots)))
+        return tex, slots
+
+    def state(self) -> dict:
+        sizes = np.zeros(5, np.uint32)
+        check(lib().cnq_vcache_state(self.handle, 0, 0, 0, 0, 0, 0, _ptr(sizes)))
+        pages, ch, tlen = int(sizes[0]), int(sizes[1]), int(sizes[2])
+        info = np.zeros((pages, 12), np.uint32)
+        stale = np.zeros(pages, np.uint32)
+        used = np.zeros(pages, np.uint32)
+        pool = np.zeros((pages, 81, ch), np.float32)
+        td = np.zeros((tlen, 4), np.uint32)
+        th = np.zeros((tlen, 4), np.uint32)
+        check(lib().cnq_vcache_state(self.handle, _ptr(info), _ptr(stale), _ptr(used), _ptr(pool), _ptr(td), _ptr(th),
+                                     _ptr(sizes)))
+        return dict(slot_info=info, slot_stale=stale, slot_used=used, pool=pool, table_device=td, table_host=th,
+                    slot_extent=int(sizes[3]), frame=int(sizes[4]))
 
 
 def _stream_ptr(stream) -> int:

@@ -392,19 +392,23 @@ def courtyard(seed: int = 0xC0_0003, width: int = 1280, height: int = 720, detai
                      notes={"curtains": curtains, "eye": eye, "target": target, "fov": 60.0, "seed": seed})
 
 
-def boulder(coef, nu: int = 100, nv: int = 50) -> np.ndarray:
-    """A lumpy boulder about 1 m across, as experiments/room3d/src/field.rs:30-58
-    builds it (a sphere with low-frequency bumps, squashed vertically, on a
-    `Surface::parametric` nu x nv grid, degenerate pole triangles dropped,
-    wound outward), with the three bump amplitudes `coef`."""
+def _boulder_grid(coef, nu: int, nv: int) -> np.ndarray:
     a1, a2, a3 = coef
     u = np.arange(nu + 1, dtype=np.float64) / nu
     v = np.arange(nv + 1, dtype=np.float64) / nv
     th, ph = np.meshgrid(2 * np.pi * u, np.pi * v, indexing="ij")
     r = (1.0 + a1 * np.sin(3 * th + 1) * np.sin(2 * ph) + a2 * np.cos(5 * th) * np.sin(3 * ph + 0.5)
          + a3 * np.sin(9 * th + 2 * ph))
-    grid = np.stack([np.sin(ph) * np.cos(th) * r * 0.5, np.sin(ph) * np.sin(th) * r * 0.5, -np.cos(ph) * r * 0.33],
+    return np.stack([np.sin(ph) * np.cos(th) * r * 0.5, np.sin(ph) * np.sin(th) * r * 0.5, -np.cos(ph) * r * 0.33],
                     axis=-1).astype(F32)
+
+
+def boulder(coef, nu: int = 100, nv: int = 50) -> np.ndarray:
+    """A lumpy boulder about 1 m across, as experiments/room3d/src/field.rs:30-58
+    builds it (a sphere with low-frequency bumps, squashed vertically, on a
+    `Surface::parametric` nu x nv grid, degenerate pole triangles dropped,
+    wound outward), with the three bump amplitudes `coef`."""
+    grid = _boulder_grid(coef, nu, nv)
     a, b, c, d = grid[:-1, :-1], grid[1:, :-1], grid[1:, 1:], grid[:-1, 1:]
     tris = np.stack([np.stack([a, b, c], axis=2), np.stack([a, c, d], axis=2)], axis=2).reshape(-1, 3, 3)
     e1, e2 = tris[:, 1] - tris[:, 0], tris[:, 2] - tris[:, 0]
@@ -507,3 +511,51 @@ def meshes_to_arrays(meshes) -> tuple[np.ndarray, np.ndarray]:
         return np.zeros((0, 9), dtype=F32), counts
     tris = np.ascontiguousarray(np.concatenate([np.asarray(m, dtype=F32).reshape(-1, 9) for m in meshes]))
     return tris, counts
+
+
+def boulder_surface(coef, nu: int = 100, nv: int = 50):
+    """`boulder` with its surface: one parametric chart over the nu x nv
+    grid (continuum_amd.surfaces.parametric), smooth normals; its triangles
+    equal boulder(coef, nu, nv)'s."""
+    from .surfaces import parametric
+    grid = _boulder_grid(coef, nu, nv)
+    tris = boulder(coef, nu, nv)
+    a, b, c = grid[0, 0], grid[1, 0], grid[1, 1]
+    plain = parametric(grid, flip=False)
+    flip = not np.array_equal(plain.tris[0], tris[0])
+    return plain if not flip else parametric(grid, flip=True)
+
+
+def vcache_world(seed: int = 0xC0_0006, boulders: int = 6, boxes: int = 4, half: float = 10.0):
+    """A small instanced world for the view cache: a 2*half m ground quad
+    (geometry 0), a box (geometry 1) and a boulder (geometry 2, 24 x 12
+    grid) placed `boxes` and `boulders` times on the ground, the first box
+    and the first boulder movable; a rect light 6 m up and a sphere light.
+
+    Returns (tris9, counts, placements, surface over the geometries,
+    lights, movable objects). Seeded with numpy's PCG64 stream `seed`."""
+    from .kernel import PLACEMENT_DTYPE
+    from .surfaces import concat, from_triangles
+    rng = np.random.default_rng(seed)
+    ground = quad_pair([-half, -half, 0.0], [2 * half, 0, 0], [0, 2 * half, 0])
+    box = box_mesh([-0.5, -0.4, 0.0], [0.5, 0.4, 0.9])
+    rock = boulder_surface((0.2, 0.1, 0.05), 24, 12)
+    geoms = [ground, box, rock.tris]
+    surface = concat([from_triangles(ground), from_triangles(box), rock])
+    tris9, counts = meshes_to_arrays(geoms)
+    n = 1 + boxes + boulders
+    pl = np.zeros(n, dtype=PLACEMENT_DTYPE)
+    pl[0]["rotation"] = [1, 0, 0, 0, 1, 0, 0, 0, 1]
+    pl[0]["scale"] = 1.0
+    for k in range(1, n):
+        g = 1 if k <= boxes else 2
+        ang = rng.uniform(0, 2 * np.pi)
+        c, s_ = np.cos(ang), np.sin(ang)
+        pl[k]["geometry"] = g
+        pl[k]["rotation"] = np.array([c, -s_, 0, s_, c, 0, 0, 0, 1], np.float32)
+        pl[k]["scale"] = rng.uniform(0.7, 1.5)
+        z = 0.0 if g == 1 else 0.25 * float(pl[k]["scale"])
+        pl[k]["translation"] = [rng.uniform(-0.6 * half, 0.6 * half), rng.uniform(-0.6 * half, 0.6 * half), z]
+    lights = np.stack([rect_light([-1.0, -1.0, 6.0], [2.0, 0, 0], [0, 2.0, 0], one_sided=False),
+                       sphere_light([6.0, -4.0, 5.0], 0.5)])
+    return tris9, counts, pl, surface, lights, [1, 1 + boxes]

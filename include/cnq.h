@@ -359,6 +359,101 @@ int cnq_cache_update(cnq_cache* cache, const uint32_t* meshes, uint32_t n, const
  * blocks the last update re-baked (nullable; any order) and their count. */
 int cnq_cache_read(cnq_cache* cache, float* irradiance, uint32_t* blocks, uint32_t* n_blocks);
 
+/* ---- The view cache's pages (the reference's `VirtualCache`,
+ * crates/continuum-gpu/src/vcache.rs, and its passes in mask_cones.wgsl).
+ * A page is 9 x 9 texels (8 and a border) of one chart of one object at one
+ * mip, keyed (object, chart | mip << 24, page x | page y << 16); texel
+ * (x, y) of a page holds, per light, the irradiance per unit radiance (or,
+ * with `light_ratio`, the light's shadowed share of its unshadowed light)
+ * through one mask cone from the texel's place on the surface, then one
+ * sky channel (zero here: no sky). Resident pages are found through an
+ * open-addressing page table of (key, slot) entries. When objects move,
+ * the stale passes find the texels whose cones the moved boxes can reach
+ * and re-bake them (pages in view) or drop their pages (out of view). */
+#define CNQ_PAGE 8          /* vcache.rs:35 PAGE */
+#define CNQ_PAGE_TEXELS 81  /* vcache.rs:37 PAGE_TEXELS */
+#define CNQ_VC_MAX_LIGHTS 16 /* lib.rs:751 MAX_LIGHTS */
+
+typedef struct cnq_vcache cnq_vcache;
+
+/* The scene's surfaces (continuum::mesh::Surface, crates/continuum/src/
+ * mesh.rs:20-39), per triangle in the scene's order (geometries, then their
+ * triangles): its global chart, its corners' chart coordinates (6 floats,
+ * in [0, 1]^2) and shading normals (9 floats); per global chart: its size
+ * along u and v (object units) and its cell grid. */
+typedef struct {
+  const uint32_t* tri_chart;
+  const float* tri_uv;
+  const float* normals;
+  uint32_t n_charts;
+  const float* chart_size;
+  const uint32_t* chart_grid;
+} cnq_vcache_surfaces;
+
+typedef struct {
+  uint32_t pages;          /* resident pages (CacheOptions::pages) */
+  uint32_t append_pages;   /* pages the stale pass may add to a frame's bake (vcache.rs append_pages) */
+  uint32_t candidates;     /* candidate pages a frame's stale pass may list */
+  float base_texel;        /* a mip-0 texel's size in object units (CacheOptions::base_texel) */
+  uint32_t light_ratio;    /* 1: texels keep the shadowed share (CacheOptions::light_ratio) */
+  uint32_t visible_frames; /* a page used within this many frames is in view (re-baked when stale) */
+} cnq_vcache_config;
+
+/* An object moved since the last frame (vcache.rs:62-76 `Moved`, rigid):
+ * its world box before and after. */
+typedef struct {
+  uint32_t object;
+  float old_lo[3], old_hi[3], new_lo[3], new_hi[3];
+} cnq_moved;
+
+/* The moves the scene's last cnq_scene_apply_moves made (instanced.rs:
+ * 1284-1306: each moved object's world box before and after, `world_bounds`
+ * of its placement), for cnq_vcache_set_moved. */
+int cnq_scene_last_moves(cnq_scene* scene, cnq_moved* out, uint32_t cap, uint32_t* n);
+
+void cnq_vcache_config_default(cnq_vcache_config* cfg);
+/* The view cache over `scene` (which must outlive it) with `n_lights`
+ * static lights (LightGpu rows): chart cell tables (vcache.rs:704
+ * `chart_tables`), an empty page table (vcache.rs:1497, 4 entries per page
+ * rounded up to a power of two, at least 1,024), the texel pool. */
+int cnq_vcache_create(cnq_scene* scene, const cnq_vcache_surfaces* surfaces, const float* lights, uint32_t n_lights,
+                      const cnq_vcache_config* cfg, const cnq_options* opts, cnq_vcache** out);
+void cnq_vcache_destroy(cnq_vcache* cache);
+/* Texels along u and v of `chart` on `object` at `mip` (vcache.rs:2104
+ * `grid`, WGSL :3181 `vc_grid`). */
+int cnq_vcache_grid(cnq_vcache* cache, uint32_t object, uint32_t chart, uint32_t mip, uint32_t out[2]);
+/* The pages the next frame uses (the page requests of VirtualCache::plan,
+ * vcache.rs:3940-4050): a resident page is marked used; an absent one takes
+ * a free slot, enters the page table and is planned for the next frame's
+ * bake (every texel and light). slots[i]: its slot, or CNQ_NONE (no free
+ * slot, a probe run past 64 entries, or a page outside its chart). */
+int cnq_vcache_request(cnq_vcache* cache, const uint32_t* keys, uint32_t n, uint32_t* slots);
+/* Frees resident pages (vcache.rs:3515 `evict_queued`'s removal). */
+int cnq_vcache_evict(cnq_vcache* cache, const uint32_t* keys, uint32_t n);
+/* Objects moved (the scene already moved: cnq_scene_apply_moves) for the
+ * next frame's stale passes (VirtualCache::set_moved, vcache.rs:2426).
+ * `scene_box` (6 floats) holds every texel's apex; NULL takes the object
+ * hierarchy's root with the old boxes, padded (instanced.rs:1395-1406). */
+int cnq_vcache_set_moved(cnq_vcache* cache, const cnq_moved* moved, uint32_t n, const float* scene_box);
+/* One frame on `stream`: page-table updates (`vc_update`), the stale passes
+ * if anything moved (`vc_stale`, `vc_stale_texels`), the bake of planned
+ * and appended texels (`vc_bake`), the fill (`vc_fill`); then waits and
+ * evicts the pages marked invalid (vcache.rs:3425). stats: [planned pages,
+ * planned jobs, appended pages, appended jobs, candidates, invalid marks,
+ * pages made stale for a re-bake, appends that did not fit, visible pages
+ * tested, pages evicted]. */
+int cnq_vcache_frame(cnq_vcache* cache, void* stream, uint64_t stats[10]);
+/* Pages by key, as the shading read finds them (WGSL :3134 `vc_find`):
+ * texels (81 x (lights + 1) floats per key, texel-major) and slots[i]
+ * (CNQ_NONE, texels zero, where absent or marked stale). */
+int cnq_vcache_read(cnq_vcache* cache, const uint32_t* keys, uint32_t n, float* texels, uint32_t* slots);
+/* Diagnostics (any pointer may be NULL): slot info (12 per slot), stale
+ * marks, last use, the pool (81 x channels per slot), the device and host
+ * page tables (4 per entry); sizes: [pages, channels, table entries, slots
+ * ever used, next frame]. */
+int cnq_vcache_state(cnq_vcache* cache, uint32_t* slot_info, uint32_t* slot_stale, uint32_t* slot_used, float* pool,
+                     uint32_t* table_device, uint32_t* table_host, uint32_t* sizes);
+
 #ifdef __cplusplus
 }
 #endif

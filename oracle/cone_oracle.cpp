@@ -84,6 +84,9 @@ struct Query {
   bool wide_clipped = false;
   bool wide_window = false;
   bool stack_overflow = false;
+  // WGSL :587-591: a triangle the cone does not see (object, triangle), and
+  // that triangle while the walk is in its object.
+  uint32_t skip_obj = NONE, skip_tri = NONE, cover_skip = NONE;
   uint32_t stack[STACK];
   const Scene* s;
   const Options* o;
@@ -499,7 +502,7 @@ static uint32_t node_cone_test(const Cone& c, Query& qs, Acc& acc, uint32_t n, V
 
 // WGSL :594-616 (opaque triangles: no transmission)
 static bool cover_triangle(const Cone& c, Query& qs, Acc& acc, uint32_t t) {
-  if (qs.o->profile_stage == 4) return false;
+  if (qs.o->profile_stage == 4 || t == qs.cover_skip) return false;
   const float* tr = qs.s->tris + 12u * t;
   V3 v0 = xyz(tr), v1 = xyz(tr + 4), v2 = xyz(tr + 8);
   V3 n = cross(v1 - v0, v2 - v0);
@@ -560,6 +563,7 @@ static void add_mesh_coverage(const Cone& world_cone, Query& qs, uint32_t m, flo
   const Scene& s = *qs.s;
   Cone c = cone_to_object(s, world_cone, m);
   const uint32_t* range = s.ranges + 4 * m;
+  qs.cover_skip = m == qs.skip_obj ? qs.skip_tri : NONE;
   qs.tri_weight = 1.0f;
   if (range[3] != NONE) {
     float clear = s.xforms[16 * range[3] + 14];
@@ -1008,10 +1012,12 @@ struct Result {
 
 // WGSL :1286-1321 and :2125-2128 (hit_irradiance_rect).
 static Result hit_irradiance_rect(const Scene& s, const Options& o, V3 point, V3 normal, const Light& r,
-                                  float* cov_out) {
+                                  float* cov_out, uint32_t skip_obj = NONE, uint32_t skip_tri = NONE) {
   Query qs;
   qs.s = &s;
   qs.o = &o;
+  qs.skip_obj = skip_obj;
+  qs.skip_tri = skip_tri;
   Result res{0.0f, 0, 0, false, false};
   float spot = spot_falloff(r, normalize(point - light_center(r)));
   bool on = spot > 0.0f && faces_light(r, point, normal);
@@ -1347,6 +1353,41 @@ int oracle_bin_weights(const float* apex, const float* normal, const float* ligh
   for (uint32_t b = 0; b < 64; b++) out64[b] = clipped_bin_weight(lc.c, xyz(normal), b);
   *out_total = window_open(lc.c, xyz(normal));
   return 1;
+}
+
+// One query (WGSL :2125 hit_irradiance_rect with `on` = true) with the
+// triangle `skip_tri` of object `skip_obj` unseen (NONE: none), for the view
+// cache's restatement: visible, then the full and touched masks.
+void oracle_query_skip(const float* tris, const uint32_t* ranges, uint32_t n_objects, const float* blas,
+                       const uint32_t* blas_items, const float* blas_cov, const float* nodes, const uint32_t* items,
+                       const float* xforms, const uint32_t* opts, const float* point, const float* normal,
+                       const float* light, uint32_t skip_obj, uint32_t skip_tri, float* visible, uint64_t* masks) {
+  Scene s = make_scene(tris, ranges, n_objects, blas, blas_items, blas_cov, nodes, items, xforms);
+  Options o = make_options(opts);
+  Result r = hit_irradiance_rect(s, o, xyz(point), xyz(normal), *reinterpret_cast<const Light*>(light), nullptr,
+                                 skip_obj, skip_tri);
+  *visible = r.visible;
+  masks[0] = r.full;
+  masks[1] = r.touched;
+}
+
+// WGSL :2135-2141 `unshadowed_rect`.
+float oracle_unshadowed_rect(const float* point, const float* normal, const float* light) {
+  const Light& r = *reinterpret_cast<const Light*>(light);
+  V3 p = xyz(point), n = xyz(normal);
+  if (!faces_light(r, p, n)) return 0.0f;
+  if (is_sphere(r)) return spot_falloff(r, normalize(p - xyz(r.corner))) * sphere_unshadowed(p, n, r);
+  return rect_open(p, n, r);
+}
+
+// WGSL :880 `faces_light`, :866 `light_center`.
+int oracle_faces_light(const float* light, const float* point, const float* normal) {
+  return faces_light(*reinterpret_cast<const Light*>(light), xyz(point), xyz(normal)) ? 1 : 0;
+}
+
+void oracle_light_center(const float* light, float* out) {
+  V3 c = light_center(*reinterpret_cast<const Light*>(light));
+  out[0] = c.x, out[1] = c.y, out[2] = c.z;
 }
 
 float oracle_rect_open(const float* point, const float* normal, const float* light) {

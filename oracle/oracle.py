@@ -53,6 +53,27 @@ def lib() -> C.CDLL:
             "oracle_bake_static": (None, [_P] * 9 + [_P, _P, C.c_uint32, _P, _P, _P, C.c_int]),
             "oracle_rebake": (None, [_P] * 7 + [_P, _P, _P, _P, C.c_uint32, _P, _P, _P, C.c_uint32, _P, _P,
                                                 C.c_int]),
+            "oracle_query_skip": (None, [_P] * 12 + [_P, C.c_uint32, C.c_uint32, _P, _P]),
+            "oracle_unshadowed_rect": (C.c_float, [_P, _P, _P]),
+            "oracle_vc_create": (_P, [_P, C.c_uint32, _P, C.c_uint32] + [_P] * 7 + [_P, _P, _P, C.c_uint32, _P, _P,
+                                                                                     _P, C.c_uint32, _P, C.c_float]),
+            "oracle_vc_destroy": (None, [_P]),
+            "oracle_vc_set_scene": (None, [_P] * 8),
+            "oracle_vc_grid": (None, [_P, C.c_uint32, C.c_uint32, C.c_uint32, _P]),
+            "oracle_vc_request": (None, [_P, _P, C.c_uint32, _P]),
+            "oracle_vc_evict": (None, [_P, _P, C.c_uint32]),
+            "oracle_vc_set_moved": (None, [_P, _P, C.c_uint32, _P]),
+            "oracle_vc_moved": (None, [_P, _P, _P]),
+            "oracle_vc_frame": (None, [_P, _P]),
+            "oracle_vc_read": (None, [_P, _P, C.c_uint32, _P, _P]),
+            "oracle_vc_state": (None, [_P] * 7),
+            "oracle_vc_hash": (C.c_uint32, [_P]),
+            "oracle_vc_table_check": (C.c_int64, [C.c_uint32, C.c_uint32]),
+            "oracle_vc_shadow_region": (C.c_int, [_P, _P, _P, _P]),
+            "oracle_vc_hull_meets": (C.c_int, [_P, _P, _P]),
+            "oracle_vc_surface": (None, [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P]),
+            "oracle_vc_world_box": (None, [_P, C.c_uint32, _P, _P, _P]),
+            "oracle_vc_page_bounds": (None, [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -285,3 +306,165 @@ def rebake(flat, texels, slots, static_cov, blocks, light, moving, opts, threads
                         _p(flat.blas_cov), _p(xf), _p(texels), _p(slots), _p(sc), _p(blocks), len(blocks), _p(lt),
                         _p(mv), _p(boxes), len(moving), _p(opts), _p(out), threads or threads_default())
     return out
+
+
+# ---- the view cache's page passes (vcache_oracle.cpp)
+
+def vc_hash(key) -> int:
+    k = np.ascontiguousarray(key, np.uint32).reshape(3)
+    return int(lib().oracle_vc_hash(_p(k)))
+
+
+def vc_table_check(length: int = 1024, steps: int = 20_000) -> int:
+    """The reference's page-table deletion test (vcache.rs:4308-4349) on the
+    oracle's table: entries moved by deletions, or -1 on a failure."""
+    return int(lib().oracle_vc_table_check(length, steps))
+
+
+def shadow_region(m, l, s):
+    out = np.zeros(6, np.float32)
+    ok = lib().oracle_vc_shadow_region(_p(np.ascontiguousarray(m, np.float32)), _p(np.ascontiguousarray(l, np.float32)),
+                                       _p(np.ascontiguousarray(s, np.float32)), _p(out))
+    return out if ok else None
+
+
+def hull_meets(p, l, m) -> bool:
+    return bool(lib().oracle_vc_hull_meets(_p(np.ascontiguousarray(p, np.float32)),
+                                           _p(np.ascontiguousarray(l, np.float32)),
+                                           _p(np.ascontiguousarray(m, np.float32))))
+
+
+def query_skip(flat, opts, point, normal, light, skip_obj=NONE, skip_tri=NONE):
+    """One hit_irradiance_rect with one triangle unseen: (visible, full, touched)."""
+    vis = np.zeros(1, np.float32)
+    masks = np.zeros(2, np.uint64)
+    xf = flat.xforms if len(flat.xforms) else np.zeros((1, 16), np.float32)
+    lib().oracle_query_skip(_p(flat.tris), _p(flat.ranges), len(flat.ranges), _p(flat.blas), _p(flat.blas_items),
+                            _p(flat.blas_cov), _p(flat.nodes), _p(flat.items), _p(xf), _p(opts),
+                            _p(np.ascontiguousarray(point, np.float32)), _p(np.ascontiguousarray(normal, np.float32)),
+                            _p(np.ascontiguousarray(light, np.float32)), skip_obj, skip_tri, _p(vis), _p(masks))
+    return float(vis[0]), int(masks[0]), int(masks[1])
+
+
+class OracleViewCache:
+    """The view cache restated on the CPU, driven as the product's ViewCache
+    (request / evict / set_moved / frame / read / state)."""
+
+    def __init__(self, flat, surface, lights, opts, pages=4096, append_pages=256, candidates=4096, base_texel=0.05,
+                 light_ratio=1, visible_frames=1):
+        self._keep = []
+        self.flat = flat
+        self.lights = np.ascontiguousarray(lights, np.float32).reshape(-1, 16)
+        self.channels = len(self.lights) + 1
+        self.opts = np.zeros(8, np.uint32)
+        self.opts[: len(opts)] = opts
+        self._surf = [np.ascontiguousarray(surface.tri_chart, np.uint32), np.ascontiguousarray(surface.tri_uv, np.float32),
+                      np.ascontiguousarray(surface.normals, np.float32),
+                      np.ascontiguousarray(surface.chart_size, np.float32),
+                      np.ascontiguousarray(surface.chart_grid, np.uint32)]
+        cfg = np.array([pages, append_pages, candidates, light_ratio, visible_frames], np.uint32)
+        self._scene = self._scene_arrays(flat)
+        sa = self._scene
+        h = lib().oracle_vc_create(_p(sa["tris"]), len(sa["tris"]), _p(sa["ranges"]), len(sa["ranges"]), _p(sa["blas"]),
+                                   _p(sa["blas_items"]), _p(sa["blas_cov"]), _p(sa["nodes"]), _p(sa["items"]),
+                                   _p(sa["xforms"]), _p(self.opts), _p(self._surf[0]), _p(self._surf[1]),
+                                   _p(self._surf[2]), len(self._surf[3]), _p(self._surf[3]), _p(self._surf[4]),
+                                   _p(self.lights), len(self.lights), _p(cfg), base_texel)
+        if not h:
+            raise ValueError("bad charts")
+        self.handle = h
+        self._destroy = lib().oracle_vc_destroy
+
+    @staticmethod
+    def _scene_arrays(flat):
+        xf = flat.xforms if len(flat.xforms) else np.zeros((1, 16), np.float32)
+        return {k: np.ascontiguousarray(v) for k, v in dict(
+            tris=flat.tris, ranges=flat.ranges, blas=flat.blas, blas_items=flat.blas_items, blas_cov=flat.blas_cov,
+            nodes=flat.nodes, items=flat.items, xforms=xf).items()}
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            self._destroy(self.handle)
+            self.handle = None
+
+    def set_scene(self, flat) -> None:
+        """The scene after a move (same triangles)."""
+        self._scene = self._scene_arrays(flat)
+        sa = self._scene
+        lib().oracle_vc_set_scene(self.handle, _p(sa["ranges"]), _p(sa["blas"]), _p(sa["blas_items"]),
+                                  _p(sa["blas_cov"]), _p(sa["nodes"]), _p(sa["items"]), _p(sa["xforms"]))
+
+    def grid(self, obj, chart, mip):
+        out = np.zeros(2, np.uint32)
+        lib().oracle_vc_grid(self.handle, obj, chart, mip, _p(out))
+        return int(out[0]), int(out[1])
+
+    def request(self, keys):
+        k = np.ascontiguousarray(keys, np.uint32).reshape(-1, 3)
+        slots = np.zeros(len(k), np.uint32)
+        lib().oracle_vc_request(self.handle, _p(k), len(k), _p(slots))
+        return slots
+
+    def evict(self, keys):
+        k = np.ascontiguousarray(keys, np.uint32).reshape(-1, 3)
+        lib().oracle_vc_evict(self.handle, _p(k), len(k))
+
+    def set_moved(self, moved, scene_box):
+        """moved: (object, old lo, old hi, new lo, new hi) tuples."""
+        rows = np.zeros((max(len(moved), 1), 13), np.float32)
+        for i, (obj, olo, ohi, nlo, nhi) in enumerate(moved):
+            rows[i, 0] = np.array([obj], np.uint32).view(np.float32)[0]
+            rows[i, 1:4], rows[i, 4:7], rows[i, 7:10], rows[i, 10:13] = olo, ohi, nlo, nhi
+        sb = np.ascontiguousarray(scene_box, np.float32).reshape(6)
+        lib().oracle_vc_set_moved(self.handle, _p(rows), len(moved), _p(sb))
+
+    def moved_layout(self) -> np.ndarray:
+        n = C.c_uint32()
+        lib().oracle_vc_moved(self.handle, None, C.byref(n))
+        out = np.zeros(n.value, np.float32)
+        lib().oracle_vc_moved(self.handle, _p(out), C.byref(n))
+        return out
+
+    def frame(self) -> dict:
+        st = np.zeros(10, np.uint64)
+        lib().oracle_vc_frame(self.handle, _p(st))
+        names = ("planned_pages", "planned_jobs", "appended_pages", "appended_jobs", "candidates", "invalid",
+                 "stale_pages", "overflow", "visible_tested", "evicted")
+        return {n: int(v) for n, v in zip(names, st)}
+
+    def read(self, keys):
+        k = np.ascontiguousarray(keys, np.uint32).reshape(-1, 3)
+        tex = np.zeros((len(k), 81, self.channels), np.float32)
+        slots = np.zeros(len(k), np.uint32)
+        if len(k):
+            lib().oracle_vc_read(self.handle, _p(k), len(k), _p(tex), _p(slots))
+        return tex, slots
+
+    def state(self) -> dict:
+        sizes = np.zeros(5, np.uint32)
+        lib().oracle_vc_state(self.handle, None, None, None, None, None, _p(sizes))
+        pages, ch, tlen = int(sizes[0]), int(sizes[1]), int(sizes[2])
+        info = np.zeros((pages, 12), np.uint32)
+        stale = np.zeros(pages, np.uint32)
+        used = np.zeros(pages, np.uint32)
+        pool = np.zeros((pages, 81, ch), np.float32)
+        table = np.zeros((tlen, 4), np.uint32)
+        lib().oracle_vc_state(self.handle, _p(info), _p(stale), _p(used), _p(pool), _p(table), _p(sizes))
+        return dict(slot_info=info, slot_stale=stale, slot_used=used, pool=pool, table=table,
+                    slot_extent=int(sizes[3]), frame=int(sizes[4]))
+
+    def surface(self, obj, chart, mip, tx, ty):
+        out = np.zeros(8, np.float32)
+        lib().oracle_vc_surface(self.handle, obj, chart, mip, tx, ty, _p(out))
+        return out[0:3].copy(), out[3:6].copy(), bool(out[6]), int(out[7:8].view(np.uint32)[0])
+
+    def world_box(self, obj, lo, hi):
+        out = np.zeros(6, np.float32)
+        lib().oracle_vc_world_box(self.handle, obj, _p(np.ascontiguousarray(lo, np.float32)),
+                                  _p(np.ascontiguousarray(hi, np.float32)), _p(out))
+        return out
+
+    def page_bounds(self, obj, chart, mip, px, py):
+        out = np.zeros(8, np.float32)
+        lib().oracle_vc_page_bounds(self.handle, obj, chart, mip, px, py, _p(out))
+        return out

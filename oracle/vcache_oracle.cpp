@@ -1,0 +1,1034 @@
+// TEST INFRASTRUCTURE ONLY (see oracle_math.h): a CPU restatement of the
+// reference's view-cache page passes, used as the checker by tests/. Never
+// linked by the product.
+//
+// Follows crates/continuum-gpu/src/vcache.rs (page table :904-1093, chart
+// tables :688-790, grid :2104, page_bounds :2676, set_moved :2426-2492,
+// planning :3940-4050, eviction :3425-3452), hierarchy.rs:418-457
+// (shadow_region), instanced.rs:512-522 (host light boxes), and the WGSL
+// passes in crates/continuum-gpu/src/mask_cones.wgsl (cited "WGSL :line"):
+// vc_find :3134, vc_grid :3181, uv_barycentric :3250, vc_surface_at :3283,
+// page_texel :3338, vc_update :3394, vc_bake :3432 (static lights),
+// vc_fill :3693, world_box :3867, light_box :3887, hull_meets :3903,
+// boxes_meet :3919, may_be_shadowed :3967, mark_invalid :3988, append_page
+// :3998, append_jobs :4018, fail_entry :4029, vc_stale :4233,
+// vc_stale_texels :4385. The passes run one invocation after another, in
+// index order; where the GPU's atomics order appends differently, only the
+// order of list entries differs, never a texel's value.
+//
+// Left out as the product leaves them out: sky and bounce channels (the sky
+// channel is zero), moving lights and their lattice, deforming objects'
+// catch-up slices (every move is rigid, so the reference's `lag_only` is
+// false wherever it is asked and no page is deferred), area texels, wide
+// windows' ray pass, absent lighting levels.
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <vector>
+
+#include "oracle_math.h"
+
+extern "C" void oracle_query_skip(const float* tris, const uint32_t* ranges, uint32_t n_objects, const float* blas,
+                                  const uint32_t* blas_items, const float* blas_cov, const float* nodes,
+                                  const uint32_t* items, const float* xforms, const uint32_t* opts,
+                                  const float* point, const float* normal, const float* light, uint32_t skip_obj,
+                                  uint32_t skip_tri, float* visible, uint64_t* masks);
+extern "C" float oracle_unshadowed_rect(const float* point, const float* normal, const float* light);
+extern "C" int oracle_faces_light(const float* light, const float* point, const float* normal);
+extern "C" void oracle_light_center(const float* light, float* out);
+
+namespace vco {
+
+using orc::V3;
+
+constexpr uint32_t NONE = 0xffffffffu;
+constexpr uint32_t PAGE = 8, STRIDE = 9, TEXELS = 81;
+constexpr uint32_t MAX_PROBE = 64;
+constexpr uint32_t MIP_SHIFT = 24, CHART_MASK = 0xffffffu;
+
+// Rust's f32::min / f32::max.
+static float rmin(float a, float b) { return std::isnan(a) ? b : (std::isnan(b) ? a : std::min(a, b)); }
+static float rmax(float a, float b) { return std::isnan(a) ? b : (std::isnan(b) ? a : std::max(a, b)); }
+static uint32_t sat_u32(float f) {
+  if (!(f > 0.0f)) return 0;
+  if (f >= 4294967296.0f) return NONE;
+  return (uint32_t)f;
+}
+static V3 ld(const float* p) { return V3{p[0], p[1], p[2]}; }
+static float comp(V3 v, int a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
+
+// vcache.rs:904-913
+uint32_t hash3(const uint32_t* k) {
+  uint32_t h = (k[0] * 0x9E3779B1u) ^ (k[1] * 0x85EBCA77u) ^ (k[2] * 0xC2B2AE3Du);
+  h ^= h >> 15;
+  h *= 0x2C1B3C6Du;
+  h ^= h >> 12;
+  return h;
+}
+
+// vcache.rs:992-1093
+struct Table {
+  std::vector<uint32_t> e;  // 4 per entry
+  uint32_t len() const { return (uint32_t)(e.size() / 4); }
+  void init(uint32_t n) {
+    e.assign(4 * (size_t)n, 0);
+    for (uint32_t i = 0; i < n; i++) e[4 * i] = NONE;
+  }
+  uint32_t insert(const uint32_t* k, uint32_t slot) {
+    uint32_t mask = len() - 1, i = hash3(k) & mask;
+    for (uint32_t p = 0; p < MAX_PROBE; p++) {
+      if (e[4 * i] == NONE) {
+        e[4 * i] = k[0], e[4 * i + 1] = k[1], e[4 * i + 2] = k[2], e[4 * i + 3] = slot;
+        return i;
+      }
+      i = (i + 1) & mask;
+    }
+    return NONE;
+  }
+  // Knuth's Algorithm R; `moved` gets (key, new entry).
+  template <class F>
+  void remove(uint32_t i, F moved) {
+    uint32_t mask = len() - 1;
+    e[4 * i] = NONE, e[4 * i + 1] = e[4 * i + 2] = e[4 * i + 3] = 0;
+    uint32_t j = i;
+    for (;;) {
+      j = (j + 1) & mask;
+      if (e[4 * j] == NONE) return;
+      uint32_t k[3] = {e[4 * j], e[4 * j + 1], e[4 * j + 2]};
+      uint32_t home = hash3(k) & mask;
+      bool stays = i <= j ? (i < home && home <= j) : (i < home || home <= j);
+      if (!stays) {
+        for (int q = 0; q < 4; q++) e[4 * i + q] = e[4 * j + q];
+        moved(k, i);
+        e[4 * j] = NONE, e[4 * j + 1] = e[4 * j + 2] = e[4 * j + 3] = 0;
+        i = j;
+      }
+    }
+  }
+  uint32_t find(const uint32_t* k) const {
+    uint32_t mask = len() - 1, i = hash3(k) & mask;
+    for (uint32_t p = 0; p < MAX_PROBE; p++) {
+      if (e[4 * i] == NONE) return NONE;
+      if (e[4 * i] == k[0] && e[4 * i + 1] == k[1] && e[4 * i + 2] == k[2]) return e[4 * i + 3];
+      i = (i + 1) & mask;
+    }
+    return NONE;
+  }
+};
+
+struct BoxF {
+  float lo[3], hi[3];
+};
+
+struct State {
+  // scene (caller-owned arrays)
+  const float *tris, *blas, *blas_cov, *nodes, *xforms;
+  const uint32_t *ranges, *blas_items, *items;
+  uint32_t n_objects, n_tris;
+  uint32_t opts[8];
+  // surfaces
+  std::vector<float> tri_uv, normals;
+  std::vector<float> chart_size;
+  std::vector<uint32_t> chart_grid, first_cell, cells, cell_tris;
+  std::vector<float> cell_bounds;
+  uint32_t n_charts;
+  // lights, config
+  std::vector<float> lights;
+  uint32_t n_lights, channels;
+  uint32_t pages, append_pages, candidate_cap, light_ratio, visible_frames;
+  float inv_base;
+  // host manager
+  Table table;
+  std::map<std::vector<uint32_t>, std::pair<uint32_t, uint32_t>> map;
+  std::vector<std::vector<uint32_t>> slot_key;
+  std::vector<uint32_t> last_used, alloc_frame, free_slots;
+  uint32_t slot_extent = 0, frame = 1;
+  std::vector<uint32_t> plan_list, plan_jobs, used_marks, cleared;
+  std::vector<float> plan_bounds;
+  std::vector<float> moved;
+  uint32_t n_movers = 0;
+  std::vector<std::pair<uint32_t, uint32_t>> mover_ids;
+  // "device" state
+  std::vector<uint32_t> slot_info, slot_stale, slot_used;
+  std::vector<float> pool;
+  std::vector<uint32_t> bake_list, bake_jobs, candidates, invalid_list;
+  std::vector<float> bake_bounds;
+  uint32_t stats[8];
+};
+
+static float object_scale(const State& s, uint32_t m) {
+  uint32_t x = s.ranges[4 * m + 3];
+  return x == NONE ? 1.0f : s.xforms[16 * x + 12];
+}
+
+// vcache.rs:2104-2111 (and WGSL :3181, the same products)
+static void grid_of(const State& s, uint32_t c, uint32_t m, uint32_t mip, uint32_t n[2]) {
+  float inv = std::ldexp(s.inv_base, -(int)mip);
+  float sc = object_scale(s, m);
+  for (int a = 0; a < 2; a++) {
+    uint32_t v = sat_u32(std::ceil((s.chart_size[2 * c + a] * sc) * inv));
+    n[a] = v > 1 ? v : 1;
+  }
+}
+
+// vcache.rs:704-790
+static bool build_tables(State& s, const uint32_t* tri_chart) {
+  s.first_cell.resize(s.n_charts);
+  uint64_t total = 0;
+  for (uint32_t c = 0; c < s.n_charts; c++) {
+    if (s.chart_grid[2 * c] == 0 || s.chart_grid[2 * c + 1] == 0) return false;
+    s.first_cell[c] = (uint32_t)total;
+    total += (uint64_t)s.chart_grid[2 * c] * s.chart_grid[2 * c + 1];
+  }
+  std::vector<std::vector<uint32_t>> lists(total);
+  for (uint32_t t = 0; t < s.n_tris; t++) {
+    uint32_t c = tri_chart[t];
+    if (c >= s.n_charts) return false;
+    const float* uv = &s.tri_uv[6 * t];
+    uint32_t g[2] = {s.chart_grid[2 * c], s.chart_grid[2 * c + 1]};
+    uint32_t r0[2], r1[2];
+    for (int a = 0; a < 2; a++) {
+      float lo = std::floor(rmin(rmin(uv[a], uv[2 + a]), uv[4 + a]) * (float)g[a] + 1e-3f);
+      float hi = std::floor(rmax(rmax(uv[a], uv[2 + a]), uv[4 + a]) * (float)g[a] - 1e-3f);
+      r0[a] = std::min(sat_u32(rmax(lo, 0.0f)), g[a] - 1);
+      r1[a] = std::min(sat_u32(rmax(rmax(hi, lo), 0.0f)), g[a] - 1);
+    }
+    for (uint32_t y = r0[1]; y <= r1[1]; y++)
+      for (uint32_t x = r0[0]; x <= r1[0]; x++) lists[s.first_cell[c] + y * g[0] + x].push_back(t);
+  }
+  s.cells.clear();
+  s.cell_tris.clear();
+  s.cell_bounds.clear();
+  for (auto& list : lists) {
+    s.cells.push_back((uint32_t)s.cell_tris.size());
+    s.cells.push_back((uint32_t)list.size());
+    float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+    for (uint32_t t : list)
+      for (int v = 0; v < 3; v++)
+        for (int a = 0; a < 3; a++) {
+          lo[a] = rmin(lo[a], s.tris[12 * t + 4 * v + a]);
+          hi[a] = rmax(hi[a], s.tris[12 * t + 4 * v + a]);
+        }
+    s.cell_bounds.insert(s.cell_bounds.end(), {lo[0], lo[1], lo[2], hi[0], hi[1], hi[2]});
+    s.cell_tris.insert(s.cell_tris.end(), list.begin(), list.end());
+  }
+  if (s.cell_tris.empty()) s.cell_tris.push_back(0);
+  return true;
+}
+
+// vcache.rs:2676-2698
+static void page_bounds(const State& s, uint32_t m, uint32_t c, uint32_t mip, uint32_t px, uint32_t py, float out[8]) {
+  uint32_t n[2];
+  grid_of(s, c, m, mip, n);
+  uint32_t g[2] = {s.chart_grid[2 * c], s.chart_grid[2 * c + 1]};
+  uint32_t pg[2] = {px, py}, a0[2], a1[2];
+  for (int a = 0; a < 2; a++) {
+    uint32_t x0 = pg[a] * PAGE, x1 = std::min(pg[a] * PAGE + PAGE, n[a] - 1);
+    auto cell = [&](uint32_t x) { return sat_u32(((float)x + 0.5f) / (float)n[a] * (float)g[a]); };
+    uint32_t c0 = cell(x0);
+    a0[a] = c0 == 0 ? 0 : c0 - 1;
+    a1[a] = std::min(cell(x1) + 1, g[a] - 1);
+  }
+  float lo[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, hi[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+  for (uint32_t y = a0[1]; y <= a1[1]; y++)
+    for (uint32_t x = a0[0]; x <= a1[0]; x++) {
+      uint64_t cell = (uint64_t)s.first_cell[c] + (uint64_t)y * g[0] + x;
+      if (cell >= s.cell_bounds.size() / 6) continue;
+      for (int q = 0; q < 3; q++) {
+        lo[q] = rmin(lo[q], s.cell_bounds[6 * cell + q]);
+        hi[q] = rmax(hi[q], s.cell_bounds[6 * cell + 3 + q]);
+      }
+    }
+  if (lo[0] > hi[0]) lo[0] = lo[1] = lo[2] = hi[0] = hi[1] = hi[2] = 0.0f;
+  out[0] = lo[0], out[1] = lo[1], out[2] = lo[2], out[3] = 0.0f;
+  out[4] = hi[0], out[5] = hi[1], out[6] = hi[2], out[7] = 0.0f;
+}
+
+// hierarchy.rs:418-457
+bool shadow_region(const BoxF& m, const BoxF& l, const BoxF& sc, BoxF& out) {
+  float dl[3], dh[3];
+  for (int a = 0; a < 3; a++) dl[a] = m.lo[a] - l.hi[a], dh[a] = m.hi[a] - l.lo[a];
+  float t0 = 0.0f, t1 = INFINITY;
+  for (int a = 0; a < 3; a++) {
+    float b = dl[a], c = sc.hi[a] - m.lo[a];
+    if (b > 0.0f) t1 = rmin(t1, c / b);
+    else if (b < 0.0f) t0 = rmax(t0, c / b);
+    else if (c < 0.0f) return false;
+    b = dh[a], c = sc.lo[a] - m.hi[a];
+    if (b > 0.0f) t0 = rmax(t0, c / b);
+    else if (b < 0.0f) t1 = rmin(t1, c / b);
+    else if (c > 0.0f) return false;
+  }
+  if (t0 > t1 * (1.0f + 1e-6f) + 1e-6f) return false;
+  if (!std::isfinite(t1)) {
+    out = sc;
+    return true;
+  }
+  float lo[3], hi[3], big = 0.0f;
+  for (int a = 0; a < 3; a++) {
+    lo[a] = rmax(rmin(m.lo[a] + dl[a] * t0, m.lo[a] + dl[a] * t1), sc.lo[a]);
+    hi[a] = rmin(rmax(m.hi[a] + dh[a] * t0, m.hi[a] + dh[a] * t1), sc.hi[a]);
+    big = rmax(big, rmax(std::fabs(lo[a]), std::fabs(hi[a])));
+  }
+  float pad = 1e-4f * rmax(big, 1.0f);
+  for (int a = 0; a < 3; a++) out.lo[a] = lo[a] - pad, out.hi[a] = hi[a] + pad;
+  return true;
+}
+
+// instanced.rs:512-522
+BoxF host_light_box(const float* r) {
+  float c[3], rad;
+  if (r[3] > 1.5f) {
+    for (int a = 0; a < 3; a++) c[a] = r[a];
+    rad = 1.5f * r[4];
+  } else {
+    for (int a = 0; a < 3; a++) c[a] = r[a] + (r[4 + a] + r[8 + a]) * 0.5f;
+    float lu = std::sqrt(r[4] * r[4] + r[5] * r[5] + r[6] * r[6]);
+    float lv = std::sqrt(r[8] * r[8] + r[9] * r[9] + r[10] * r[10]);
+    rad = 0.5f * (lu + lv);
+  }
+  float big = rmax(rmax(rmax(std::fabs(c[0]), std::fabs(c[1])), std::fabs(c[2])), 1.0f);
+  rad = rad * 1.01f + 2e-5f * big + 1e-3f;
+  BoxF b;
+  for (int a = 0; a < 3; a++) b.lo[a] = c[a] - rad, b.hi[a] = c[a] + rad;
+  return b;
+}
+
+// vcache.rs:2426-2492 for rigid moves (every light's mask and own bit set).
+static void set_moved(State& s, const float* moved, uint32_t n, const BoxF& sc) {
+  auto bits = [](uint32_t u) {
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+  };
+  BoxF empty{{FLT_MAX, FLT_MAX, FLT_MAX}, {-FLT_MAX, -FLT_MAX, -FLT_MAX}};
+  auto grow = [](BoxF a, const BoxF& b) {
+    for (int q = 0; q < 3; q++) a.lo[q] = rmin(a.lo[q], b.lo[q]), a.hi[q] = rmax(a.hi[q], b.hi[q]);
+    return a;
+  };
+  std::vector<BoxF> unions(s.n_lights, empty);
+  BoxF all = empty;
+  std::vector<float> entries;
+  uint32_t every = s.n_lights >= 32 ? NONE : (1u << s.n_lights) - 1u;
+  for (uint32_t i = 0; i < n; i++) {
+    const float* m = moved + 13 * i;  // object bits, old lo, old hi, new lo, new hi
+    uint32_t object;
+    std::memcpy(&object, m, 4);
+    BoxF old{{m[1], m[2], m[3]}, {m[4], m[5], m[6]}}, nw{{m[7], m[8], m[9]}, {m[10], m[11], m[12]}};
+    all = grow(grow(all, old), nw);
+    uint32_t id = (uint32_t)s.mover_ids.size();
+    bool seen = false;
+    for (auto& p : s.mover_ids)
+      if (p.first == object) id = p.second, seen = true;
+    if (!seen) s.mover_ids.push_back({object, id});
+    entries.insert(entries.end(), {old.lo[0], old.lo[1], old.lo[2], bits(object)});
+    entries.insert(entries.end(), {old.hi[0], old.hi[1], old.hi[2], bits(id)});
+    entries.insert(entries.end(), {nw.lo[0], nw.lo[1], nw.lo[2], bits(every)});
+    entries.insert(entries.end(), {nw.hi[0], nw.hi[1], nw.hi[2], 0.0f});
+    for (uint32_t l = 0; l < s.n_lights; l++) {
+      BoxF lb = host_light_box(&s.lights[16 * l]);
+      for (const BoxF* b : {&old, &nw}) {
+        BoxF r;
+        if (!shadow_region(*b, lb, sc, r)) r = empty;
+        if (r.lo[0] <= r.hi[0]) unions[l] = grow(unions[l], r);
+        entries.insert(entries.end(), {r.lo[0], r.lo[1], r.lo[2], 0.0f});
+        entries.insert(entries.end(), {r.hi[0], r.hi[1], r.hi[2], 0.0f});
+      }
+    }
+  }
+  s.moved.clear();
+  for (auto& u : unions) {
+    s.moved.insert(s.moved.end(), {u.lo[0], u.lo[1], u.lo[2], 0.0f});
+    s.moved.insert(s.moved.end(), {u.hi[0], u.hi[1], u.hi[2], 0.0f});
+  }
+  s.moved.insert(s.moved.end(), {all.lo[0], all.lo[1], all.lo[2], 0.0f});
+  s.moved.insert(s.moved.end(), {all.hi[0], all.hi[1], all.hi[2], 0.0f});
+  s.moved.insert(s.moved.end(), entries.begin(), entries.end());
+  s.n_movers = n;
+}
+
+// ---- the WGSL passes
+
+struct Box3 {
+  V3 lo, hi;
+};
+
+static float surface_offset(float d) { return orc::fmax_(1e-4f, 2e-6f * d); }
+
+static V3 rot_world(const float* xf, V3 d) {
+  return V3{orc::dot(ld(xf), d), orc::dot(ld(xf + 4), d), orc::dot(ld(xf + 8), d)};
+}
+
+// WGSL :3867-3882
+static Box3 world_box(const State& s, uint32_t m, V3 lo, V3 hi) {
+  Box3 b{lo, hi};
+  uint32_t x = s.ranges[4 * m + 3];
+  if (x != NONE) {
+    const float* xf = s.xforms + 16 * x;
+    float sc = xf[12];
+    V3 c = (lo + hi) * 0.5f, e = (hi - lo) * 0.5f;
+    V3 wc = rot_world(xf, c) * sc + V3{xf[3], xf[7], xf[11]};
+    V3 we = V3{orc::dot(orc::vabs(ld(xf)), e), orc::dot(orc::vabs(ld(xf + 4)), e), orc::dot(orc::vabs(ld(xf + 8)), e)} * sc;
+    b = Box3{wc - we, wc + we};
+  }
+  V3 far = orc::vmax(orc::vmax(orc::vabs(b.lo), orc::vabs(b.hi)), V3{1.0f, 1.0f, 1.0f});
+  float pad = 2.0f * surface_offset(orc::length(far)) + 1e-5f * orc::fmax_(far.x, orc::fmax_(far.y, far.z));
+  return Box3{b.lo - V3{pad, pad, pad}, b.hi + V3{pad, pad, pad}};
+}
+
+// WGSL :3887-3898
+static Box3 light_box(const float* r) {
+  float cc[3];
+  oracle_light_center(r, cc);
+  V3 c{cc[0], cc[1], cc[2]};
+  float radius = r[3] > 1.5f ? 1.5f * r[4] : 0.5f * (orc::length(ld(r + 4)) + orc::length(ld(r + 8)));
+  V3 far = orc::vmax(orc::vabs(c), V3{1.0f, 1.0f, 1.0f});
+  radius = radius * 1.001f + 1e-5f * orc::fmax_(far.x, orc::fmax_(far.y, far.z)) + 1e-4f;
+  return Box3{c - V3{radius, radius, radius}, c + V3{radius, radius, radius}};
+}
+
+// WGSL :3903-3917
+bool hull_meets(const Box3& p, const Box3& l, const Box3& m) {
+  float s0 = 0.0f, s1 = 1.0f;
+  for (int a = 0; a < 3; a++) {
+    float b0 = comp(l.lo, a) - comp(p.lo, a), c0 = comp(m.hi, a) - comp(p.lo, a);
+    if (b0 > 0.0f) s1 = orc::fmin_(s1, c0 / b0);
+    else if (b0 < 0.0f) s0 = orc::fmax_(s0, c0 / b0);
+    else if (c0 < 0.0f) return false;
+    float b1 = comp(l.hi, a) - comp(p.hi, a), c1 = comp(m.lo, a) - comp(p.hi, a);
+    if (b1 > 0.0f) s0 = orc::fmax_(s0, c1 / b1);
+    else if (b1 < 0.0f) s1 = orc::fmin_(s1, c1 / b1);
+    else if (c1 > 0.0f) return false;
+  }
+  return s0 <= s1 + 1e-6f;
+}
+
+static bool boxes_meet(const Box3& a, const Box3& b) {
+  return a.lo.x <= b.hi.x && a.lo.y <= b.hi.y && a.lo.z <= b.hi.z && b.lo.x <= a.hi.x && b.lo.y <= a.hi.y &&
+         b.lo.z <= a.hi.z;
+}
+
+static Box3 moved_box(const State& s, uint32_t i) { return Box3{ld(&s.moved[4 * i]), ld(&s.moved[4 * (i + 1)])}; }
+static uint32_t moved_w(const State& s, uint32_t i) {
+  uint32_t u;
+  std::memcpy(&u, &s.moved[4 * i + 3], 4);
+  return u;
+}
+
+// WGSL :3967-3982
+static bool may_be_shadowed(const State& s, const Box3& p, uint32_t l, uint32_t slot) {
+  if (!boxes_meet(p, moved_box(s, 2 * l))) return false;
+  Box3 lb = light_box(&s.lights[16 * l]);
+  uint32_t first = 2 * s.n_lights + 2, stride = 4 + 4 * s.n_lights;
+  for (uint32_t k = 0; k < s.n_movers; k++) {
+    uint32_t e = first + k * stride;
+    uint32_t slice = moved_w(s, e + 3);
+    if (slice != 0 && slot % (slice >> 16) != (slice & 0xffffu)) continue;
+    uint32_t r = e + 4 + 4 * l;
+    if (boxes_meet(p, moved_box(s, r)) && hull_meets(p, lb, moved_box(s, e))) return true;
+    if (boxes_meet(p, moved_box(s, r + 2)) && hull_meets(p, lb, moved_box(s, e + 2))) return true;
+  }
+  return false;
+}
+
+struct Surface {
+  V3 apex, normal;
+  bool ok;
+  uint32_t tri;
+};
+
+// WGSL :3250-3260
+static void uv_barycentric(const State& s, uint32_t t, float u0, float v0, float b[3]) {
+  const float* q = &s.tri_uv[6 * t];
+  float e1x = q[2] - q[0], e1y = q[3] - q[1], e2x = q[4] - q[0], e2y = q[5] - q[1];
+  float dx = u0 - q[0], dy = v0 - q[1];
+  float det = e1x * e2y - e1y * e2x;
+  if (std::fabs(det) < 1e-30f) {
+    b[0] = b[1] = b[2] = -1.0f;
+    return;
+  }
+  float sv = (dx * e2y - dy * e2x) / det;
+  float uv = (e1x * dy - e1y * dx) / det;
+  b[0] = 1.0f - sv - uv, b[1] = sv, b[2] = uv;
+}
+
+// WGSL :3283-3326
+static Surface surface_at(const State& s, uint32_t m, uint32_t c, float u, float v) {
+  uint32_t g[2] = {s.chart_grid[2 * c], s.chart_grid[2 * c + 1]};
+  uint32_t cx = std::min(sat_u32(u * (float)g[0]), g[0] - 1), cy = std::min(sat_u32(v * (float)g[1]), g[1] - 1);
+  uint32_t cell = s.first_cell[c] + cy * g[0] + cx;
+  uint32_t first = s.cells[2 * cell], count = s.cells[2 * cell + 1];
+  uint32_t best_t = NONE;
+  float best_b[3] = {0.0f, 0.0f, 0.0f}, best_score = -3.4e38f;
+  for (uint32_t q = first; q < first + count; q++) {
+    uint32_t t = s.cell_tris[q];
+    float b[3];
+    uv_barycentric(s, t, u, v, b);
+    float score = orc::fmin_(b[0], orc::fmin_(b[1], b[2]));
+    if (score > best_score) {
+      best_score = score, best_t = t;
+      best_b[0] = b[0], best_b[1] = b[1], best_b[2] = b[2];
+    }
+  }
+  float b0[3];
+  for (int k = 0; k < 3; k++) b0[k] = orc::clampf(best_b[k], 0.0f, 1.0f);
+  float sum = orc::fmax_(b0[0] + b0[1] + b0[2], 1e-20f);
+  float bx = b0[0] / sum, by = b0[1] / sum, bz = b0[2] / sum;
+  uint32_t t = best_t != NONE ? best_t : 0;
+  const float* tr = s.tris + 12 * t;
+  V3 v0 = ld(tr), v1 = ld(tr + 4), v2 = ld(tr + 8);
+  const float* nr = &s.normals[9 * t];
+  V3 p = v0 * bx + v1 * by + v2 * bz;
+  V3 in = ld(nr) * bx + ld(nr + 3) * by + ld(nr + 6) * bz;
+  V3 face = orc::cross(v1 - v0, v2 - v0);
+  V3 nrm = orc::dot(in, in) > 1e-20f ? in : face;
+  V3 pw = p, nw = orc::normalize(nrm), geometric = face;
+  uint32_t x = s.ranges[4 * m + 3];
+  if (x != NONE) {
+    const float* xf = s.xforms + 16 * x;
+    pw = rot_world(xf, p) * xf[12] + V3{xf[3], xf[7], xf[11]};
+    nw = orc::normalize(rot_world(xf, nrm));
+    geometric = rot_world(xf, face);
+  }
+  Surface out;
+  out.apex = pw + orc::normalize(geometric) * surface_offset(orc::length(pw));
+  out.normal = nw;
+  out.ok = best_t != NONE;
+  out.tri = t;
+  return out;
+}
+
+static Surface texel_surface(const State& s, uint32_t m, uint32_t c, const uint32_t n[2], uint32_t tx, uint32_t ty) {
+  return surface_at(s, m, c, ((float)tx + 0.5f) / (float)n[0], ((float)ty + 0.5f) / (float)n[1]);
+}
+
+static void mark_invalid(State& s, uint32_t slot) {
+  uint32_t old = s.slot_stale[slot];
+  s.slot_stale[slot] = 1;
+  if (old == 0 || old == 2 || old == 3) {
+    uint32_t q = s.stats[1]++;
+    if (q < s.pages) s.invalid_list[q] = slot;
+  }
+}
+
+static uint32_t append_page(State& s, uint32_t slot, const uint32_t* key, V3 lo, V3 hi) {
+  uint32_t p = s.stats[0]++;
+  if (p >= s.append_pages) {
+    s.stats[3]++;
+    return NONE;
+  }
+  uint32_t entry = (uint32_t)(s.plan_list.size() / 4) + p;
+  s.bake_list[4 * entry] = key[0], s.bake_list[4 * entry + 1] = key[1], s.bake_list[4 * entry + 2] = key[2];
+  s.bake_list[4 * entry + 3] = slot;
+  float* b = &s.bake_bounds[8 * entry];
+  b[0] = lo.x, b[1] = lo.y, b[2] = lo.z, b[3] = 0.0f, b[4] = hi.x, b[5] = hi.y, b[6] = hi.z, b[7] = 0.0f;
+  return entry;
+}
+
+static uint32_t append_jobs_cap(const State& s) { return s.append_pages * TEXELS * s.n_lights; }
+
+static uint32_t append_jobs(State& s, uint32_t count) {
+  uint32_t old = s.stats[5];
+  s.stats[5] += count;
+  uint32_t plan = (uint32_t)(s.plan_jobs.size() / 2), cap = append_jobs_cap(s);
+  if (old + count <= cap) return plan + old;
+  for (uint32_t k = old; k < std::min(old + count, cap); k++) {
+    s.bake_jobs[2 * (plan + k)] = NONE;
+    s.bake_jobs[2 * (plan + k) + 1] = 0;
+  }
+  return NONE;
+}
+
+static void fail_entry(State& s, uint32_t entry, uint32_t slot) {
+  s.bake_bounds[8 * entry + 3] = 1.0f;
+  s.slot_stale[slot] = 1;
+  uint32_t q = s.stats[1]++;
+  if (q < s.pages) s.invalid_list[q] = slot;
+}
+
+static float f_of(uint32_t u) {
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
+// WGSL :4233-4379 (rigid moves, no sky)
+static void vc_stale(State& s, uint32_t slot) {
+  const uint32_t* info = &s.slot_info[12 * slot];
+  if (info[3] == 0) return;
+  uint32_t mark = s.slot_stale[slot];
+  if (mark == 1) return;
+  uint32_t m = info[0];
+  uint32_t key[3] = {info[0], info[1], info[2]};
+  bool visible = s.slot_used[slot] + s.visible_frames >= s.frame;
+  if (visible) s.stats[2]++;
+  V3 lo{f_of(info[4]), f_of(info[5]), f_of(info[6])}, hi{f_of(info[8]), f_of(info[9]), f_of(info[10])};
+  Box3 page = world_box(s, m, lo, hi);
+  uint32_t L = s.n_lights, first = 2 * L + 2, stride = 4 + 4 * L;
+  bool own = false;
+  uint32_t own_mask = 0;
+  if (boxes_meet(page, moved_box(s, 2 * L)))
+    for (uint32_t k = 0; k < s.n_movers; k++)
+      if (moved_w(s, first + k * stride) == m) own = true, own_mask = moved_w(s, first + k * stride + 2);
+  if (own && own_mask != 0) {
+    if (!visible) return mark_invalid(s, slot);
+    uint32_t entry = append_page(s, slot, key, lo, hi);
+    if (entry == NONE) return mark_invalid(s, slot);
+    uint32_t all = L >= 32 ? NONE : (1u << L) - 1u;
+    uint32_t updated = (uint32_t)__builtin_popcount(own_mask & all);
+    uint32_t j = append_jobs(s, TEXELS * updated);
+    if (j == NONE) {
+      s.stats[3]++;
+      return fail_entry(s, entry, slot);
+    }
+    s.stats[6]++;
+    uint32_t at = j;
+    for (uint32_t l = 0; l < L; l++) {
+      if (!(own_mask & (1u << l))) continue;
+      for (uint32_t k = 0; k < TEXELS; k++) s.bake_jobs[2 * (at + k)] = entry, s.bake_jobs[2 * (at + k) + 1] = k | (l << 8);
+      at += TEXELS;
+    }
+    return;
+  }
+  bool candidate = false;
+  for (uint32_t l = 0; l < L && !candidate; l++) candidate = may_be_shadowed(s, page, l, slot);
+  if (!candidate) return;
+  if (mark != 0) return mark_invalid(s, slot);
+  uint32_t c = s.stats[4]++;
+  if (c >= s.candidate_cap) return mark_invalid(s, slot);
+  uint32_t entry = NONE;
+  if (visible) {
+    entry = append_page(s, slot, key, lo, hi);
+    if (entry == NONE) {
+      s.candidates[2 * c] = NONE, s.candidates[2 * c + 1] = NONE;
+      return mark_invalid(s, slot);
+    }
+  }
+  s.candidates[2 * c] = slot, s.candidates[2 * c + 1] = entry;
+}
+
+// WGSL :4385-4432
+static void vc_stale_texel(State& s, uint32_t j) {
+  uint32_t c = j / TEXELS, k = j % TEXELS;
+  if (c >= std::min(s.stats[4], s.candidate_cap)) return;
+  uint32_t slot = s.candidates[2 * c], entry = s.candidates[2 * c + 1];
+  if (slot == NONE) return;
+  const uint32_t* info = &s.slot_info[12 * slot];
+  uint32_t m = info[0], chart = info[1] & CHART_MASK, mip = info[1] >> MIP_SHIFT;
+  uint32_t n[2];
+  grid_of(s, chart, m, mip, n);
+  uint32_t tx = (info[2] & 0xffffu) * PAGE + k % STRIDE, ty = (info[2] >> 16) * PAGE + k / STRIDE;
+  if (tx >= n[0] || ty >= n[1]) return;
+  Surface sf = texel_surface(s, m, chart, n, tx, ty);
+  if (!sf.ok) return;
+  V3 far = orc::vmax(orc::vabs(sf.apex), V3{1.0f, 1.0f, 1.0f});
+  float pad = 1e-5f * orc::fmax_(far.x, orc::fmax_(far.y, far.z));
+  Box3 apex{sf.apex - V3{pad, pad, pad}, sf.apex + V3{pad, pad, pad}};
+  for (uint32_t l = 0; l < s.n_lights; l++) {
+    if (!may_be_shadowed(s, apex, l, slot)) continue;
+    if (entry == NONE) return mark_invalid(s, slot);
+    uint32_t at = append_jobs(s, 1);
+    if (at == NONE) {
+      s.stats[3]++;
+      return fail_entry(s, entry, slot);
+    }
+    s.bake_jobs[2 * at] = entry, s.bake_jobs[2 * at + 1] = k | (l << 8);
+    if (s.slot_stale[slot] == 0) s.stats[6]++;
+    s.slot_stale[slot] = 1;
+  }
+}
+
+// WGSL :3432-3554, static lights: one job.
+static void vc_bake_job(State& s, uint32_t j, uint32_t total) {
+  uint32_t jx = s.bake_jobs[2 * std::min(j, total - 1)], jy = s.bake_jobs[2 * std::min(j, total - 1) + 1];
+  bool listed = j < total && jx != NONE;
+  if (!listed) return;  // an unlisted invocation writes nothing
+  uint32_t k = jy & 0xffu, l = std::min(jy >> 8, s.n_lights - 1);
+  const uint32_t* e = &s.bake_list[4 * jx];
+  uint32_t m = e[0], c = e[1] & CHART_MASK, mip = e[1] >> MIP_SHIFT;
+  uint32_t n[2];
+  grid_of(s, c, m, mip, n);
+  uint32_t tx = (e[2] & 0xffffu) * PAGE + k % STRIDE, ty = (e[2] >> 16) * PAGE + k / STRIDE;
+  bool inside = tx < n[0] && ty < n[1];
+  if (!inside) return;
+  Surface sf = texel_surface(s, m, c, n, tx, ty);
+  bool on = sf.ok;
+  const float* light = &s.lights[16 * l];
+  float apex[3] = {sf.apex.x, sf.apex.y, sf.apex.z}, nrm[3] = {sf.normal.x, sf.normal.y, sf.normal.z};
+  float open = oracle_unshadowed_rect(apex, nrm, light);
+  float cc[3];
+  oracle_light_center(light, cc);
+  V3 toward = orc::normalize(V3{cc[0], cc[1], cc[2]} - sf.apex);
+  float tw[3] = {toward.x, toward.y, toward.z};
+  bool back = on && open <= 0.0f && oracle_faces_light(light, apex, tw);
+  float seen = back ? oracle_unshadowed_rect(apex, tw, light) : open;
+  float value = 0.0f;
+  uint64_t masks[2] = {0, ~0ull};
+  if (on)
+    oracle_query_skip(s.tris, s.ranges, s.n_objects, s.blas, s.blas_items, s.blas_cov, s.nodes, s.items, s.xforms,
+                      s.opts, apex, back ? tw : nrm, light, back ? m : NONE, back ? sf.tri : NONE, &value, masks);
+  bool last_empty = masks[1] == 0, last_full = masks[0] == ~0ull;
+  float ratio = !on ? 0.0f : (seen > 0.0f ? orc::clampf(value / seen, 0.0f, 1.0f) : 1.0f);
+  if (on && seen > 0.0f && last_empty) ratio = 1.0f;
+  if (on && seen > 0.0f && last_full) ratio = 0.0f;
+  float light_value = back ? 0.0f : value;
+  s.pool[((size_t)e[3] * TEXELS + k) * s.channels + l] = s.light_ratio ? ratio : (on ? light_value : 0.0f);
+}
+
+// WGSL :3693-3836 without sky or bounce light: one page.
+static void vc_fill(State& s, uint32_t p) {
+  uint32_t plan_pages = (uint32_t)(s.plan_list.size() / 4);
+  const uint32_t* e = &s.bake_list[4 * p];
+  const float* bb = &s.bake_bounds[8 * p];
+  if (bb[3] != 0.0f) return;
+  uint32_t slot = e[3];
+  uint32_t* info = &s.slot_info[12 * slot];
+  float kind = bb[7];
+  uint32_t prev_w = info[3];
+  bool same = prev_w != 0 && info[0] == e[0] && info[1] == e[1] && info[2] == e[2];
+  bool fresh = p < plan_pages && (kind == 0.0f || kind == 2.0f || kind == 5.0f);
+  bool keep = same && !fresh;
+  uint32_t charge = keep ? info[7] : 0;
+  uint32_t moving = fresh && s.n_movers > 0 ? 4 : 0;
+  uint32_t b[8];
+  std::memcpy(b, bb, 32);
+  info[0] = e[0], info[1] = e[1], info[2] = e[2], info[3] = 1u | (keep ? prev_w & 0xffffff06u : 0u) | moving;
+  info[4] = b[0], info[5] = b[1], info[6] = b[2], info[7] = charge;
+  info[8] = b[4], info[9] = b[5], info[10] = b[6], info[11] = b[7];
+  if (kind == 5.0f) {
+  } else if (kind >= 3.0f) {
+    if (s.slot_stale[slot] == 3) s.slot_stale[slot] = 0;
+  } else {
+    s.slot_stale[slot] = kind == 2.0f ? 3 : 0;
+  }
+  uint32_t c = e[1] & CHART_MASK, mip = e[1] >> MIP_SHIFT;
+  uint32_t n[2];
+  grid_of(s, c, e[0], mip, n);
+  uint32_t px = (e[2] & 0xffffu) * PAGE, py = (e[2] >> 16) * PAGE;
+  float* o = &s.pool[(size_t)slot * TEXELS * s.channels];
+  uint32_t sky = s.channels - 1;
+  for (uint32_t k = 0; k < TEXELS; k++) o[k * s.channels + sky] = 0.0f;
+  for (uint32_t k = 0; k < TEXELS; k++) {
+    uint32_t tx = px + k % STRIDE, ty = py + k / STRIDE;
+    if (tx >= n[0] || ty >= n[1]) {
+      uint32_t ix = std::min(tx, n[0] - 1) - px, iy = std::min(ty, n[1] - 1) - py;
+      uint32_t nearest = iy * STRIDE + ix;
+      for (uint32_t ch = 0; ch < s.channels; ch++) o[k * s.channels + ch] = o[nearest * s.channels + ch];
+    }
+  }
+  if (p < plan_pages) s.slot_used[slot] = s.frame;
+}
+
+static void evict_slot(State& s, uint32_t slot) {
+  auto it = s.map.find(s.slot_key[slot]);
+  if (it == s.map.end()) return;
+  uint32_t entry = it->second.second;
+  s.map.erase(it);
+  s.table.remove(entry, [&](const uint32_t* k, uint32_t i) { s.map[std::vector<uint32_t>(k, k + 3)].second = i; });
+  s.slot_key[slot].clear();
+  s.free_slots.push_back(slot);
+  s.cleared.push_back(slot);
+}
+
+}  // namespace vco
+
+using namespace vco;
+
+extern "C" {
+
+void* oracle_vc_create(const float* tris, uint32_t n_tris, const uint32_t* ranges, uint32_t n_objects,
+                       const float* blas, const uint32_t* blas_items, const float* blas_cov, const float* nodes,
+                       const uint32_t* items, const float* xforms, const uint32_t* opts, const uint32_t* tri_chart,
+                       const float* tri_uv, const float* normals, uint32_t n_charts, const float* chart_size,
+                       const uint32_t* chart_grid, const float* lights, uint32_t n_lights, const uint32_t* cfg,
+                       float base_texel) {
+  State* s = new State();
+  s->tris = tris, s->ranges = ranges, s->n_objects = n_objects, s->blas = blas, s->blas_items = blas_items;
+  s->blas_cov = blas_cov, s->nodes = nodes, s->items = items, s->xforms = xforms, s->n_tris = n_tris;
+  std::memcpy(s->opts, opts, sizeof(s->opts));
+  s->tri_uv.assign(tri_uv, tri_uv + 6 * (size_t)n_tris);
+  s->normals.assign(normals, normals + 9 * (size_t)n_tris);
+  s->n_charts = n_charts;
+  s->chart_size.assign(chart_size, chart_size + 2 * (size_t)n_charts);
+  s->chart_grid.assign(chart_grid, chart_grid + 2 * (size_t)n_charts);
+  s->lights.assign(lights, lights + 16 * (size_t)n_lights);
+  s->n_lights = n_lights, s->channels = n_lights + 1;
+  s->pages = cfg[0], s->append_pages = cfg[1], s->candidate_cap = cfg[2], s->light_ratio = cfg[3];
+  s->visible_frames = cfg[4];
+  s->inv_base = 1.0f / base_texel;
+  if (!build_tables(*s, tri_chart)) {
+    delete s;
+    return nullptr;
+  }
+  uint32_t len = 1;
+  while (len < 4 * s->pages) len <<= 1;
+  s->table.init(std::max<uint32_t>(len, 1024));
+  s->slot_key.assign(s->pages, {});
+  s->last_used.assign(s->pages, 0);
+  s->alloc_frame.assign(s->pages, 0);
+  for (uint32_t k = s->pages; k > 0; k--) s->free_slots.push_back(k - 1);
+  s->slot_info.assign(12 * (size_t)s->pages, 0);
+  s->slot_stale.assign(s->pages, 0);
+  s->slot_used.assign(s->pages, 0);
+  s->pool.assign((size_t)s->pages * TEXELS * s->channels, 0.0f);
+  s->invalid_list.assign(s->pages, 0);
+  s->candidates.assign(2 * (size_t)std::max<uint32_t>(s->candidate_cap, 1), 0);
+  return s;
+}
+
+void oracle_vc_destroy(void* h) { delete static_cast<State*>(h); }
+
+// The scene after a move (caller-owned arrays, as at creation).
+void oracle_vc_set_scene(void* h, const uint32_t* ranges, const float* blas, const uint32_t* blas_items,
+                         const float* blas_cov, const float* nodes, const uint32_t* items, const float* xforms) {
+  State& s = *static_cast<State*>(h);
+  s.ranges = ranges, s.blas = blas, s.blas_items = blas_items, s.blas_cov = blas_cov, s.nodes = nodes;
+  s.items = items, s.xforms = xforms;
+}
+
+void oracle_vc_grid(void* h, uint32_t object, uint32_t chart, uint32_t mip, uint32_t* out) {
+  grid_of(*static_cast<State*>(h), chart, object, mip, out);
+}
+
+// vcache.rs:3940-4050 without a budget: as cnq_vcache_request.
+void oracle_vc_request(void* h, const uint32_t* keys, uint32_t n, uint32_t* slots) {
+  State& s = *static_cast<State*>(h);
+  for (uint32_t i = 0; i < n; i++) {
+    const uint32_t* k = keys + 3 * i;
+    std::vector<uint32_t> key(k, k + 3);
+    slots[i] = NONE;
+    auto it = s.map.find(key);
+    if (it != s.map.end()) {
+      uint32_t slot = it->second.first;
+      if (s.last_used[slot] != s.frame) s.used_marks.push_back(slot);
+      s.last_used[slot] = s.frame;
+      slots[i] = slot;
+      continue;
+    }
+    uint32_t m = k[0], c = k[1] & CHART_MASK, mip = k[1] >> MIP_SHIFT, px = k[2] & 0xffffu, py = k[2] >> 16;
+    uint32_t g[2];
+    grid_of(s, c, m, mip, g);
+    uint32_t pg[2] = {px, py}, listed[2];
+    bool empty = false;
+    for (int a = 0; a < 2; a++) {
+      uint32_t nn = g[a] + 1;
+      uint32_t inside = std::min(nn - std::min(pg[a] * PAGE, nn), STRIDE);
+      if (inside == 0 || (uint64_t)pg[a] * PAGE >= g[a]) empty = true;
+      listed[a] = std::min(inside + 1, STRIDE);
+    }
+    if (empty || s.free_slots.empty()) continue;
+    uint32_t slot = s.free_slots.back();
+    s.free_slots.pop_back();
+    uint32_t entry = s.table.insert(k, slot);
+    if (entry == NONE) {
+      s.free_slots.push_back(slot);
+      continue;
+    }
+    s.slot_extent = std::max(s.slot_extent, slot + 1);
+    s.map[key] = {slot, entry};
+    s.slot_key[slot] = key;
+    s.last_used[slot] = s.frame;
+    s.alloc_frame[slot] = s.frame;
+    uint32_t index = (uint32_t)(s.plan_list.size() / 4);
+    s.plan_list.insert(s.plan_list.end(), {k[0], k[1], k[2], slot});
+    float b[8];
+    page_bounds(s, m, c, mip, px, py, b);
+    s.plan_bounds.insert(s.plan_bounds.end(), b, b + 8);
+    for (uint32_t l = 0; l < s.n_lights; l++)
+      for (uint32_t y = 0; y < listed[1]; y++)
+        for (uint32_t x = 0; x < listed[0]; x++) s.plan_jobs.insert(s.plan_jobs.end(), {index, (y * STRIDE + x) | (l << 8)});
+    slots[i] = slot;
+  }
+}
+
+void oracle_vc_evict(void* h, const uint32_t* keys, uint32_t n) {
+  State& s = *static_cast<State*>(h);
+  for (uint32_t i = 0; i < n; i++) {
+    auto it = s.map.find(std::vector<uint32_t>(keys + 3 * i, keys + 3 * i + 3));
+    if (it != s.map.end()) evict_slot(s, it->second.first);
+  }
+  // As the product: the evicted slots' info clears at once.
+  for (uint32_t slot : s.cleared) std::fill(&s.slot_info[12 * slot], &s.slot_info[12 * slot + 12], 0u);
+  s.cleared.clear();
+}
+
+// moved: per object 13 floats (object bits, old lo, old hi, new lo, new hi); scene box 6 floats.
+void oracle_vc_set_moved(void* h, const float* moved, uint32_t n, const float* scene_box) {
+  State& s = *static_cast<State*>(h);
+  BoxF sc{{scene_box[0], scene_box[1], scene_box[2]}, {scene_box[3], scene_box[4], scene_box[5]}};
+  set_moved(s, moved, n, sc);
+}
+
+void oracle_vc_moved(void* h, float* out, uint32_t* n_floats) {
+  State& s = *static_cast<State*>(h);
+  if (out) std::memcpy(out, s.moved.data(), s.moved.size() * 4);
+  *n_floats = (uint32_t)s.moved.size();
+}
+
+// One frame, as cnq_vcache_frame: stats as there.
+void oracle_vc_frame(void* h, uint64_t* out_stats) {
+  State& s = *static_cast<State*>(h);
+  uint32_t plan_pages = (uint32_t)(s.plan_list.size() / 4), plan_jobs = (uint32_t)(s.plan_jobs.size() / 2);
+  uint32_t app_jobs = append_jobs_cap(s);
+  s.bake_list.assign(4 * (size_t)(plan_pages + s.append_pages + 1), 0);
+  s.bake_bounds.assign(8 * (size_t)(plan_pages + s.append_pages + 1), 0.0f);
+  s.bake_jobs.assign(2 * ((size_t)plan_jobs + app_jobs + 1), 0);
+  std::copy(s.plan_list.begin(), s.plan_list.end(), s.bake_list.begin());
+  std::copy(s.plan_bounds.begin(), s.plan_bounds.end(), s.bake_bounds.begin());
+  std::copy(s.plan_jobs.begin(), s.plan_jobs.end(), s.bake_jobs.begin());
+  std::memset(s.stats, 0, sizeof(s.stats));
+  // vc_update (WGSL :3394) and this build's use marks and cleared slots.
+  for (uint32_t slot : s.used_marks) s.slot_used[slot] = s.frame;
+  for (uint32_t slot : s.cleared) std::fill(&s.slot_info[12 * slot], &s.slot_info[12 * slot + 12], 0u);
+  s.cleared.clear();
+  bool moving = !s.moved.empty() && s.n_movers > 0;
+  if (!moving) s.n_movers = 0;
+  if (moving) {
+    for (uint32_t slot = 0; slot < s.slot_extent; slot++) vc_stale(s, slot);
+    uint64_t threads = (uint64_t)s.candidate_cap * TEXELS;
+    for (uint64_t j = 0; j < threads; j++) vc_stale_texel(s, (uint32_t)j);
+  }
+  uint32_t total = plan_jobs + std::min(s.stats[5], app_jobs);
+  for (uint32_t j = 0; j < total; j++) vc_bake_job(s, j, total);
+  uint32_t pages = plan_pages + std::min(s.stats[0], s.append_pages);
+  for (uint32_t p = 0; p < pages; p++) vc_fill(s, p);
+  std::vector<uint32_t> invalid;
+  if (s.stats[1] <= s.pages) {
+    invalid.assign(s.invalid_list.begin(), s.invalid_list.begin() + s.stats[1]);
+  } else {
+    for (uint32_t slot = 0; slot < s.slot_extent; slot++)
+      if (s.slot_stale[slot] != 0) invalid.push_back(slot);
+  }
+  std::sort(invalid.begin(), invalid.end());
+  uint64_t evicted = 0;
+  for (uint32_t slot : invalid) {
+    if (s.slot_key[slot].empty() || s.alloc_frame[slot] >= s.frame) continue;
+    evict_slot(s, slot);
+    evicted++;
+  }
+  // The product sends evictions to its device at once: their slots' info clears now.
+  for (uint32_t slot : s.cleared) std::fill(&s.slot_info[12 * slot], &s.slot_info[12 * slot + 12], 0u);
+  s.cleared.clear();
+  uint64_t st[10] = {plan_pages, plan_jobs, std::min(s.stats[0], s.append_pages), std::min(s.stats[5], app_jobs),
+                     s.stats[4], s.stats[1], s.stats[6], s.stats[3], s.stats[2], evicted};
+  if (out_stats) std::memcpy(out_stats, st, sizeof(st));
+  s.plan_list.clear(), s.plan_bounds.clear(), s.plan_jobs.clear(), s.used_marks.clear();
+  s.moved.clear();
+  s.n_movers = 0;
+  s.frame++;
+}
+
+// WGSL :3134 vc_find, then the page's texels.
+void oracle_vc_read(void* h, const uint32_t* keys, uint32_t n, float* texels, uint32_t* slots) {
+  State& s = *static_cast<State*>(h);
+  size_t row = (size_t)TEXELS * s.channels;
+  for (uint32_t i = 0; i < n; i++) {
+    uint32_t slot = s.table.find(keys + 3 * i);
+    if (slot != NONE && s.slot_stale[slot] != 0 && s.slot_stale[slot] != 3) slot = NONE;
+    slots[i] = slot;
+    for (size_t q = 0; q < row; q++) texels[i * row + q] = slot == NONE ? 0.0f : s.pool[slot * row + q];
+  }
+}
+
+void oracle_vc_state(void* h, uint32_t* slot_info, uint32_t* slot_stale, uint32_t* slot_used, float* pool,
+                     uint32_t* table, uint32_t* sizes) {
+  State& s = *static_cast<State*>(h);
+  if (sizes) {
+    sizes[0] = s.pages, sizes[1] = s.channels, sizes[2] = s.table.len(), sizes[3] = s.slot_extent, sizes[4] = s.frame;
+  }
+  if (slot_info) std::memcpy(slot_info, s.slot_info.data(), s.slot_info.size() * 4);
+  if (slot_stale) std::memcpy(slot_stale, s.slot_stale.data(), s.slot_stale.size() * 4);
+  if (slot_used) std::memcpy(slot_used, s.slot_used.data(), s.slot_used.size() * 4);
+  if (pool) std::memcpy(pool, s.pool.data(), s.pool.size() * 4);
+  if (table) std::memcpy(table, s.table.e.data(), s.table.e.size() * 4);
+}
+
+// Building blocks, for the tests that pin them.
+uint32_t oracle_vc_hash(const uint32_t* k) { return hash3(k); }
+
+// The reference's own page-table test (vcache.rs:4308-4349): `steps`
+// random inserts and removes with its LCG; returns the number of entries
+// deletions moved, or -1 if a live key was not findable or a removed one
+// still was.
+int64_t oracle_vc_table_check(uint32_t len, uint32_t steps) {
+  Table t;
+  t.init(len);
+  uint64_t rng = 12345;
+  auto next = [&]() {
+    rng = rng * 6364136223846793005ull + 1442695040888963407ull;
+    return (uint32_t)(rng >> 33);
+  };
+  std::map<std::vector<uint32_t>, uint32_t> live;
+  std::vector<std::vector<uint32_t>> order;  // live keys, for picking one
+  int64_t moves = 0;
+  for (uint32_t step = 0; step < steps; step++) {
+    bool remove = live.size() > 400 || (!live.empty() && next() % 3 == 0);
+    if (remove) {
+      uint32_t pick = next() % (uint32_t)order.size();
+      std::vector<uint32_t> k = order[pick];
+      order.erase(order.begin() + pick);
+      uint32_t entry = live[k];
+      live.erase(k);
+      t.remove(entry, [&](const uint32_t* mk, uint32_t i) {
+        live[std::vector<uint32_t>(mk, mk + 3)] = i;
+        moves++;
+      });
+      if (t.find(k.data()) != NONE) return -1;
+    } else {
+      std::vector<uint32_t> k = {next() % 64, next() % 4, next() % 64};
+      if (live.count(k)) continue;
+      uint32_t entry = t.insert(k.data(), step);
+      if (entry != NONE) {
+        live[k] = entry;
+        order.push_back(k);
+      }
+    }
+    if (step % 997 == 0) {
+      for (auto& kv : live) {
+        const uint32_t* e = &t.e[4 * kv.second];
+        if (e[0] != kv.first[0] || e[1] != kv.first[1] || e[2] != kv.first[2]) return -1;
+        if (t.find(kv.first.data()) != e[3]) return -1;
+      }
+    }
+  }
+  return moves;
+}
+
+int oracle_vc_shadow_region(const float* m, const float* l, const float* sc, float* out) {
+  BoxF mb{{m[0], m[1], m[2]}, {m[3], m[4], m[5]}}, lb{{l[0], l[1], l[2]}, {l[3], l[4], l[5]}};
+  BoxF sb{{sc[0], sc[1], sc[2]}, {sc[3], sc[4], sc[5]}}, r;
+  if (!shadow_region(mb, lb, sb, r)) return 0;
+  for (int a = 0; a < 3; a++) out[a] = r.lo[a], out[3 + a] = r.hi[a];
+  return 1;
+}
+
+int oracle_vc_hull_meets(const float* p, const float* l, const float* m) {
+  Box3 pb{ld(p), ld(p + 3)}, lb{ld(l), ld(l + 3)}, mb{ld(m), ld(m + 3)};
+  return hull_meets(pb, lb, mb) ? 1 : 0;
+}
+
+// A texel's place: apex, normal, ok, triangle (8 floats).
+void oracle_vc_surface(void* h, uint32_t m, uint32_t c, uint32_t mip, uint32_t tx, uint32_t ty, float* out) {
+  State& s = *static_cast<State*>(h);
+  uint32_t n[2];
+  grid_of(s, c, m, mip, n);
+  Surface sf = texel_surface(s, m, c, n, tx, ty);
+  out[0] = sf.apex.x, out[1] = sf.apex.y, out[2] = sf.apex.z;
+  out[3] = sf.normal.x, out[4] = sf.normal.y, out[5] = sf.normal.z;
+  out[6] = sf.ok ? 1.0f : 0.0f;
+  std::memcpy(&out[7], &sf.tri, 4);
+}
+
+void oracle_vc_world_box(void* h, uint32_t m, const float* lo, const float* hi, float* out) {
+  Box3 b = world_box(*static_cast<State*>(h), m, ld(lo), ld(hi));
+  out[0] = b.lo.x, out[1] = b.lo.y, out[2] = b.lo.z, out[3] = b.hi.x, out[4] = b.hi.y, out[5] = b.hi.z;
+}
+
+void oracle_vc_page_bounds(void* h, uint32_t m, uint32_t c, uint32_t mip, uint32_t px, uint32_t py, float* out) {
+  page_bounds(*static_cast<State*>(h), m, c, mip, px, py, out);
+}
+
+}  // extern "C"

@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -289,6 +290,89 @@ def served_form(device, n_requests: int, seconds: float) -> dict:
     }
 
 
+def view_cache_form(device, boulders: int = 512, moves: int = 5) -> dict:
+    """The view cache over the boulder field: pages (9 x 9 texels, mip 0,
+    base texel 5 cm) of the `boulders` boulders nearest the camera and of the
+    ground within 24 m of it (mip 1), lit by the sun and the three rect
+    lights nearest the camera. Frame 0 bakes every page (one cone per texel
+    and light); then the 16 nearest boulders, movable, shift 25 cm a frame
+    and the stale passes re-bake what their moves reach. Times are wall
+    clock around each frame (host planning, uploads and the readback of the
+    invalid list included)."""
+    import numpy as np
+
+    from continuum_amd import _lib
+    from continuum_amd.kernel import Flat, Scene, ViewCache
+    from continuum_amd.scenes import boulder_field, boulder_field_surface
+
+    t_build = time.perf_counter()
+    tris9, counts, pl, lights, eye, _ = boulder_field()
+    surf = boulder_field_surface()
+    movers = list(range(1, 17))
+    flat = Flat.build_instanced(tris9, counts, pl, object_prefilter=True, movable=movers)
+    scene = Scene(device, flat)
+    t_build = time.perf_counter() - t_build
+    centre = lights[1:, 0:3] + 0.5 * (lights[1:, 4:7] + lights[1:, 8:11])
+    nearest = 1 + np.argsort(np.linalg.norm(centre[:, :2] - eye[:2], axis=1))[:3]
+    vlights = np.ascontiguousarray(lights[[0, *nearest]])
+    opts = _lib.default_options(hierarchy=2, prefilter=1.0, object_cov=flat.object_cov)
+    vc = ViewCache(scene, surf, vlights, opts, pages=32768, append_pages=8192, candidates=32768, base_texel=0.05)
+    keys = []
+    for m in range(1, boulders + 1):
+        keys += vc.pages_of(m, int(pl[m]["geometry"]), 0)
+    gx, gy = vc.grid(0, 0, 1)
+    texel = 2100.0 / gx
+    for py in range(gy // 8 + 1):
+        y = -1050.0 + (py * 8 + 4) * texel
+        if abs(y - eye[1]) > 24.0:
+            continue
+        for px in range(gx // 8 + 1):
+            x = -1050.0 + (px * 8 + 4) * texel
+            if math.hypot(x - eye[0], y - eye[1]) <= 24.0:
+                keys.append(vc.key(0, 0, 1, px, py))
+    slots = vc.request(keys)
+    t0 = time.perf_counter()
+    st = vc.frame()
+    bake_s = time.perf_counter() - t0
+    # The same bake again, counting the walks' work (a second cache, so the
+    # first frame's timing stays uncounted).
+    scene.set_count_work(True)
+    scene.counters(reset=True)
+    vc2 = ViewCache(scene, surf, vlights, opts, pages=32768, append_pages=8192, candidates=32768, base_texel=0.05)
+    vc2.request(keys)
+    vc2.frame()
+    capped, work = scene.counters(reset=True)
+    scene.set_count_work(False)
+    del vc2
+    cones = max(st["planned_jobs"], 1)
+    out = {
+        "workload": f"boulder field view cache: {boulders} nearest boulders (mip 0) + ground within 24 m (mip 1), "
+                    "base texel 5 cm, sun + 3 rect lights",
+        "pages": len(keys), "resident": int((slots != _lib.NONE).sum()), "host_build_s": round(t_build, 2),
+        "bake": {"cones": st["planned_jobs"], "s": bake_s, "cones_per_s": st["planned_jobs"] / bake_s,
+                 "per_cone": {"object_nodes": int(work[0]) / cones, "blas_nodes": int(work[1]) / cones,
+                              "triangles_tested": int(work[2]) / cones, "meshes_entered": int(work[4]) / cones,
+                              "splats": int(work[5]) / cones, "capped": int(capped[0])}},
+    }
+    frames = []
+    for k in range(moves):
+        for m in movers:
+            t = pl[m]["translation"].copy()
+            t[0] += 0.25 * (k + 1)
+            scene.set_transform(m, pl[m]["rotation"], t)
+        scene.apply_moves()
+        vc.set_moved(scene.last_moves())
+        vc.request(keys)
+        t0 = time.perf_counter()
+        st = vc.frame()
+        dt = time.perf_counter() - t0
+        frames.append(dict(ms=dt * 1e3, candidates=st["candidates"], stale_pages=st["stale_pages"],
+                           rebaked_cones=st["appended_jobs"], evicted=st["evicted"], overflow=st["overflow"]))
+    out["moves"] = {"movers": len(movers), "shift_m": 0.25, "frames": frames,
+                    "ms_mean": float(np.mean([f["ms"] for f in frames]))}
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -301,6 +385,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-forms", action="store_true", help="skip the courtyard's plain/batched forms")
     ap.add_argument("--no-served", action="store_true", help="skip the served form over the boulder field")
+    ap.add_argument("--no-vcache", action="store_true", help="skip the view cache over the boulder field")
+    ap.add_argument("--only-form", choices=["courtyard", "served", "view_cache"],
+                    help="run one form alone and print its JSON (for profiling it); no headline line")
     ap.add_argument("--served-requests", type=int, default=64, help="requests in the saturated served run")
     ap.add_argument("--served-seconds", type=float, default=10.0, help="length of the steady served run")
     args = ap.parse_args()
@@ -317,6 +404,17 @@ def main() -> int:
     from continuum_amd import _lib
     from continuum_amd.kernel import Device, Flat, Scene
     from continuum_amd.scenes import meshes_to_arrays, rock_field
+
+    if args.only_form:
+        device = Device(0)
+        if args.only_form == "courtyard":
+            out = courtyard_forms(device, steps=5)
+        elif args.only_form == "served":
+            out = served_form(device, n_requests=args.served_requests, seconds=args.served_seconds)
+        else:
+            out = view_cache_form(device)
+        print(json.dumps({"form": args.only_form, **out}))
+        return 0
 
     t0 = time.perf_counter()
     sc = rock_field(width=args.width, height=args.height)
@@ -486,6 +584,8 @@ def main() -> int:
     if not args.no_served:
         result.setdefault("forms", {})["served"] = served_form(device, n_requests=args.served_requests,
                                                                seconds=args.served_seconds)
+    if not args.no_vcache:
+        result.setdefault("forms", {})["view_cache"] = view_cache_form(device)
 
     print(json.dumps(result))
     return 0

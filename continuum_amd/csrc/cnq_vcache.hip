@@ -480,13 +480,17 @@ __global__ __launch_bounds__(256) void vc_stale_texels_kernel(VcArgs a) {
 }
 
 // WGSL :3432-3554 for static lights: one wave per 64 consecutive jobs, a
-// grid-stride loop over the planned and appended jobs.
+// grid-stride loop over the planned and appended jobs. With COUNT, the
+// walks' work goes to `work` (as the cone kernel's) and capped walks to
+// counters[0].
+template <bool COUNT>
 __global__ __launch_bounds__(64) void vc_bake_kernel(VcArgs a, const NodeC* __restrict__ blas,
                                                      const NodeC* __restrict__ nodes,
                                                      const float* __restrict__ tri_leaf,
                                                      const uint32_t* __restrict__ items,
                                                      const uint32_t* __restrict__ blas_items,
-                                                     const float* __restrict__ blas_cov, QueryParams p) {
+                                                     const float* __restrict__ blas_cov, QueryParams p,
+                                                     uint64_t* __restrict__ work, uint32_t* __restrict__ counters) {
   __shared__ float s_acc[ACC_CELLS * 64];
   __shared__ float s_poly[POLY_SLOTS * 64];
   const uint32_t lane = lane_id();
@@ -524,8 +528,12 @@ __global__ __launch_bounds__(64) void vc_bake_kernel(VcArgs a, const NodeC* __re
                             : rect_open(L, s.apex, toward, r);
     }
     const Skip skip{back ? t.m : NONE, back ? s.tri : NONE};
-    const QueryOut o =
-        evaluate_at<false, true>(x, p, L, listed, s.apex, back ? toward : s.normal, on, r, nullptr, nullptr, skip);
+    const QueryOut o = evaluate_at<COUNT, true>(x, p, L, listed, s.apex, back ? toward : s.normal, on, r, nullptr,
+                                                COUNT ? work : nullptr, skip);
+    if (COUNT) {
+      const uint64_t capped = ballot(listed && o.capped);
+      if (lane == 0 && capped) atomicAdd(&counters[0], (uint32_t)__builtin_popcountll(capped));
+    }
     const float value = o.visible;
     const bool last_empty = o.touched == 0ull, last_full = o.full == ~0ull;
     float ratio = !on ? 0.0f : (seen > 0.0f ? clampf(value / seen, 0.0f, 1.0f) : 1.0f);
@@ -1084,8 +1092,12 @@ int cnq_vcache_frame(cnq_vcache* c, void* stream, uint64_t stats[10]) {
   const uint64_t job_waves = ((uint64_t)plan_jobs + (a.n_movers ? a.append_jobs : 0) + 63) / 64;
   if (job_waves) {
     const uint32_t grid = (uint32_t)std::min<uint64_t>(job_waves, 8192);
-    hipLaunchKernelGGL(cnq::vc_bake_kernel, dim3(grid), dim3(64), 0, st, a, s->blas.p, s->nodes.p, s->tri_leaf.p,
-                       s->items.p, s->blas_items.p, s->blas_cov.p, p);
+    if (s->count_work)
+      hipLaunchKernelGGL(cnq::vc_bake_kernel<true>, dim3(grid), dim3(64), 0, st, a, s->blas.p, s->nodes.p,
+                         s->tri_leaf.p, s->items.p, s->blas_items.p, s->blas_cov.p, p, s->work.p, s->counters.p);
+    else
+      hipLaunchKernelGGL(cnq::vc_bake_kernel<false>, dim3(grid), dim3(64), 0, st, a, s->blas.p, s->nodes.p,
+                         s->tri_leaf.p, s->items.p, s->blas_items.p, s->blas_cov.p, p, nullptr, nullptr);
     HIP_TRY(hipGetLastError(), "vc_bake launch");
   }
   const uint32_t page_blocks = std::min<uint32_t>(plan_pages + (a.n_movers ? c->cfg.append_pages : 0), 4096);

@@ -559,3 +559,16 @@ def vcache_world(seed: int = 0xC0_0006, boulders: int = 6, boxes: int = 4, half:
     lights = np.stack([rect_light([-1.0, -1.0, 6.0], [2.0, 0, 0], [0, 2.0, 0], one_sided=False),
                        sphere_light([6.0, -4.0, 5.0], 0.5)])
     return tris9, counts, pl, surface, lights, [1, 1 + boxes]
+
+
+def boulder_field_surface(seed: int = 0xC0_0005, kinds: int = 10):
+    """The surfaces of boulder_field's geometries (same seed, same shapes):
+    the ground quad's one chart, then each boulder's parametric chart, as
+    one continuum_amd.surfaces.Surface with charts numbered globally (chart
+    g is geometry g's)."""
+    from .surfaces import concat, from_triangles
+    rng = np.random.default_rng(seed)
+    h = 1050.0
+    ground = quad_pair([-h, -h, 0.0], [2 * h, 0, 0], [0, 2 * h, 0])
+    coefs = [rng.uniform([0.12, 0.06, 0.02], [0.30, 0.18, 0.10]) for _ in range(kinds)]
+    return concat([from_triangles(ground)] + [boulder_surface(c) for c in coefs])

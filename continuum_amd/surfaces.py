@@ -94,18 +94,29 @@ def smooth_normals(tris) -> np.ndarray:
     order = np.lexsort((np.arange(len(keys)), keys[:, 2], keys[:, 1], keys[:, 0]))
     sk = keys[order]
     starts = np.concatenate([[0], np.nonzero(np.any(sk[1:] != sk[:-1], axis=1))[0] + 1, [len(sk)]])
-    out = np.zeros((len(tris) * 3, 3), dtype=F32)
-    for g in range(len(starts) - 1):
-        members = order[starts[g]:starts[g + 1]]
-        fs = face[members // 3]
-        for c in members:
-            n = face[c // 3]
-            acc = np.zeros(3, dtype=F32)
-            for f in fs:
-                if F32(F32(f[0] * n[0] + f[1] * n[1]) + F32(f[2] * n[2])) > 0:
-                    acc = (acc + f).astype(F32)
-            out[c] = _normalize_or_zero(acc)
-    return out.reshape(-1, 9)
+    sizes = np.diff(starts)
+    group_of_sorted = np.repeat(np.arange(len(sizes)), sizes)
+    rank = np.arange(len(sk)) - starts[group_of_sorted]
+    members = np.full((len(sizes), int(sizes.max()) if len(sizes) else 1), -1, np.int64)
+    members[group_of_sorted, rank] = order
+    group = np.empty(len(keys), np.int64)
+    group[order] = group_of_sorted
+    n = face[np.arange(len(keys)) // 3]
+    acc = np.zeros((len(keys), 3), F32)
+    # members in corner order, each summed in turn (f32, as the reference's iterator sum)
+    for j in range(members.shape[1]):
+        m = members[group, j]
+        f = face[np.maximum(m, 0) // 3]
+        d = ((f[:, 0] * n[:, 0]).astype(F32) + (f[:, 1] * n[:, 1]).astype(F32)).astype(F32) + (f[:, 2] * n[:, 2])
+        take = (m >= 0) & (d.astype(F32) > 0)
+        acc = np.where(take[:, None], (acc + f).astype(F32), acc)
+    ln = np.sqrt(((acc[:, 0] * acc[:, 0]).astype(F32) + (acc[:, 1] * acc[:, 1]).astype(F32)).astype(F32)
+                 + (acc[:, 2] * acc[:, 2]).astype(F32)).astype(F32)
+    with np.errstate(divide="ignore"):
+        r = (F32(1.0) / ln).astype(F32)
+    ok = np.isfinite(r) & (r > 0)
+    out = np.where(ok[:, None], (acc * r[:, None]).astype(F32), F32(0.0))
+    return out.reshape(-1, 9).astype(F32)
 
 
 def parametric(grid: np.ndarray, keep: np.ndarray | None = None, flip: bool = False) -> Surface:

@@ -73,15 +73,15 @@ __device__ __forceinline__ bool faces_rect(const LightRect& r, f3 point, f3 norm
 
 // WGSL :2468-2487: one wave per block with a slot; the static meshes'
 // coverage of each texel's cone, uncapped, into static_cov.
-__global__ __launch_bounds__(64) void bake_static_kernel(
+__global__ __launch_bounds__(64, WALK_WAVES) void bake_static_kernel(
     const NodeC* __restrict__ blas, const NodeC* __restrict__ static_nodes, const float* __restrict__ tri_leaf,
     const float* __restrict__ tris, const uint32_t* __restrict__ ranges, const uint32_t* __restrict__ static_items,
     const float* __restrict__ blas_cov, const float* __restrict__ xforms, const float* __restrict__ texels,
     const uint32_t* __restrict__ slot_block, float* __restrict__ static_cov, LightRect light, QueryParams p) {
   __shared__ float s_acc[ACC_CELLS * 64];
-  __shared__ float s_poly[POLY_SLOTS * 64];
   const uint32_t lane = lane_id();
-  Lane L{s_acc + lane, s_poly + lane, ACC_CELLS, 0u};
+  float poly[POLY_SLOTS];
+  Lane L{s_acc + lane, poly, ACC_CELLS, 0u};
   const Ctx x{blas, static_nodes, tri_leaf, tris, ranges, static_items, nullptr, blas_cov, xforms,
               p.n_objects, 2u, p.profile_stage, NONE};
   const uint32_t slot = blockIdx.x;
@@ -102,16 +102,16 @@ __global__ __launch_bounds__(64) void bake_static_kernel(
 // count mark_blocks left): each texel's cached static coverage, plus every
 // moving mesh whose padded bounds its cone may touch, then the window's
 // visible weight, into the texel's irradiance.
-__global__ __launch_bounds__(64) void rebake_kernel(
+__global__ __launch_bounds__(64, WALK_WAVES) void rebake_kernel(
     const NodeC* __restrict__ blas, const float* __restrict__ tri_leaf, const float* __restrict__ tris,
     const uint32_t* __restrict__ ranges, const float* __restrict__ blas_cov, const float* __restrict__ xforms,
     const float* __restrict__ texels, const uint32_t* __restrict__ block_slot, const float* __restrict__ static_cov,
     const uint32_t* __restrict__ dirty, const uint32_t* __restrict__ n_dirty, float* __restrict__ irradiance,
     LightRect light, QueryParams p, MovingMeshes mv) {
   __shared__ float s_acc[ACC_CELLS * 64];
-  __shared__ float s_poly[POLY_SLOTS * 64];
   const uint32_t lane = lane_id();
-  Lane L{s_acc + lane, s_poly + lane, ACC_CELLS, 0u};
+  float poly[POLY_SLOTS];
+  Lane L{s_acc + lane, poly, ACC_CELLS, 0u};
   const Ctx x{blas, nullptr, tri_leaf, tris, ranges, nullptr, nullptr, blas_cov, xforms,
               p.n_objects, 2u, p.profile_stage, NONE};
   const uint32_t count = *n_dirty;

@@ -28,7 +28,7 @@ __device__ __forceinline__ uint32_t find_view(const ViewDesc* __restrict__ views
 // view.out_off + l * w * h + y * w + x. Direct mode: one item per block;
 // persistent mode: a fixed grid takes items from `work_counter` in order.
 template <bool COUNT, bool PERSISTENT>
-__global__ __launch_bounds__(64) void cone_items_kernel(
+__global__ __launch_bounds__(64, WALK_WAVES) void cone_items_kernel(
     const NodeC* __restrict__ blas, const NodeC* __restrict__ nodes, const float* __restrict__ tri_leaf,
     const float* __restrict__ tris, const uint32_t* __restrict__ ranges, const uint32_t* __restrict__ items,
     const float* __restrict__ blas_cov, const float* __restrict__ xforms, const float* __restrict__ receivers,
@@ -37,9 +37,9 @@ __global__ __launch_bounds__(64) void cone_items_kernel(
     uint32_t* __restrict__ counters, uint64_t* __restrict__ work_out, uint32_t* __restrict__ work_counter,
     QueryParams p, ViewDesc v0) {
   __shared__ float s_acc[ACC_CELLS * 64];
-  __shared__ float s_poly[POLY_SLOTS * 64];
   const uint32_t lane = lane_id();
-  Lane L{s_acc + lane, s_poly + lane, ACC_CELLS, 0u};
+  float poly[POLY_SLOTS];
+  Lane L{s_acc + lane, poly, ACC_CELLS, 0u};
   const Ctx x{blas, nodes, tri_leaf, tris, ranges, items, nullptr, blas_cov, xforms,
               p.n_objects, p.hierarchy, p.profile_stage, p.object_cov};
   const uint32_t tw_log2 = p.tile_w_log2;

@@ -484,7 +484,7 @@ __global__ __launch_bounds__(256) void vc_stale_texels_kernel(VcArgs a) {
 // walks' work goes to `work` (as the cone kernel's) and capped walks to
 // counters[0].
 template <bool COUNT>
-__global__ __launch_bounds__(64) void vc_bake_kernel(VcArgs a, const NodeC* __restrict__ blas,
+__global__ __launch_bounds__(64, WALK_WAVES) void vc_bake_kernel(VcArgs a, const NodeC* __restrict__ blas,
                                                      const NodeC* __restrict__ nodes,
                                                      const float* __restrict__ tri_leaf,
                                                      const uint32_t* __restrict__ items,
@@ -492,9 +492,9 @@ __global__ __launch_bounds__(64) void vc_bake_kernel(VcArgs a, const NodeC* __re
                                                      const float* __restrict__ blas_cov, QueryParams p,
                                                      uint64_t* __restrict__ work, uint32_t* __restrict__ counters) {
   __shared__ float s_acc[ACC_CELLS * 64];
-  __shared__ float s_poly[POLY_SLOTS * 64];
   const uint32_t lane = lane_id();
-  Lane L{s_acc + lane, s_poly + lane, ACC_CELLS, 0u};
+  float poly[POLY_SLOTS];
+  Lane L{s_acc + lane, poly, ACC_CELLS, 0u};
   const Ctx x{blas, nodes, tri_leaf, a.tris, a.ranges, items, blas_items, blas_cov, a.xforms,
               p.n_objects, p.hierarchy, p.profile_stage, p.object_cov};
   const uint32_t appended = a.stats[5] < a.append_jobs ? a.stats[5] : a.append_jobs;

@@ -27,6 +27,11 @@ PEAK_HBM_GBPS = 8000.0
 PROFILE_DIR = ROOT / "profiles"
 
 
+# The plain launch's tile: 16 x 4 shading points a wave (a sweep over 4, 8,
+# 16, 32 and 64 on the rock field puts it first).
+PLAIN_TILE_W = 16
+
+
 def algorithmic_bytes(work, queries: int, receivers: int, lights: int) -> dict:
     """Bytes a launch must move, per unit of work the kernel's counters
     report: 32 B per receiver read, 64 B per light read, 20 B of results
@@ -97,7 +102,7 @@ def courtyard_forms(device, steps: int) -> dict:
     def frame(f):
         scene.refit_device(poses[f], stream)
         scene.primary_device(cam, W, H, lights, opts, hits, recv, stream)
-        scene.view_query_device(recv, W, H, lights, opts, vis, full, touched, tile_w=8, stream=stream)
+        scene.view_query_device(recv, W, H, lights, opts, vis, full, touched, tile_w=PLAIN_TILE_W, stream=stream)
 
     frame(0)
     torch.cuda.synchronize()
@@ -135,7 +140,7 @@ def courtyard_forms(device, steps: int) -> dict:
     def sequential():
         for k, (_, w, h, lf, n) in enumerate(views):
             scene.primary_device(v_cams[k], w, h, v_lights[k], opts, s_hits, s_recv, stream)
-            scene.view_query_device(s_recv, w, h, v_lights[k], opts, s_vis, s_full, s_touched, tile_w=8,
+            scene.view_query_device(s_recv, w, h, v_lights[k], opts, s_vis, s_full, s_touched, tile_w=PLAIN_TILE_W,
                                     stream=stream)
 
     sequential()
@@ -276,7 +281,7 @@ def served_form(device, n_requests: int, seconds: float) -> dict:
     q = 0
     for k, (_, w, h, i) in enumerate(views):
         scene.primary_device(cams[k], w, h, lts[k], opts, hits, recv, stream)
-        scene.view_query_device(recv, w, h, lts[k], opts, vis, full, touched, tile_w=8, stream=stream)
+        scene.view_query_device(recv, w, h, lts[k], opts, vis, full, touched, tile_w=PLAIN_TILE_W, stream=stream)
         q += w * h * len(i)
     torch.cuda.synchronize()
     t_plain = time.perf_counter() - t0
@@ -380,7 +385,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--width", type=int, default=1280)
     ap.add_argument("--height", type=int, default=720)
-    ap.add_argument("--tile-w", type=int, default=8, help="pixels across a wave's tile (tile_w x 64/tile_w)")
+    ap.add_argument("--tile-w", type=int, default=PLAIN_TILE_W, help="pixels across a wave's tile (tile_w x 64/tile_w)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-forms", action="store_true", help="skip the courtyard's plain/batched forms")

@@ -613,7 +613,7 @@ int cnq_render_view(cnq_scene* s, const float* camera, uint32_t width, uint32_t 
   }
   rc = cnq_primary_device(s, cam.p, width, height, l.p, n_lights, o, h.p, recv.p, nullptr);
   if (rc == CNQ_OK && n_lights)
-    rc = cnq_view_irradiance_device(s, recv.p, width, height, l.p, n_lights, o, 8, v.p, f.p, t.p, nullptr);
+    rc = cnq_view_irradiance_device(s, recv.p, width, height, l.p, n_lights, o, 16, v.p, f.p, t.p, nullptr);
   if (rc == CNQ_OK) {
     e = hipDeviceSynchronize();
     if (e == hipSuccess && out_hits) e = hipMemcpy(out_hits, h.p, 16 * px, hipMemcpyDeviceToHost);
@@ -947,7 +947,7 @@ int batch_enqueue(cnq_batch* b, const cnq_options* o, const cnq_tuning* t, hipSt
 extern "C" {
 
 void cnq_tuning_default(cnq_tuning* t) {
-  t->tile_w = 8;
+  t->tile_w = 16;
   t->persistent_blocks = 0;
 }
 
@@ -1071,7 +1071,7 @@ int cnq_batch_autotune(cnq_batch* b, const cnq_options* o, void* stream, cnq_tun
   hipStream_t st = (hipStream_t)stream;
   int cus = 256;
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->scene->dev->ordinal);
-  const uint32_t tiles[] = {8, 16, 4, 64};
+  const uint32_t tiles[] = {16, 8, 32, 4, 64};
   const uint32_t per_cu[] = {0, 4, 8};
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0), "event");

@@ -419,6 +419,9 @@ __device__ __forceinline__ bool in_grid(f2 q) {
   return !(q.x < 0.0f || q.y < 0.0f || q.x > g || q.y > g);
 }
 
+__device__ __forceinline__ bool rasterize_general(const Cone& c, Lane& L, f3 v0, f3 v1, f3 v2, float weight,
+                                                  bool far_clip, float su, float sv);
+
 // WGSL :350-451. A triangle wholly in front of the near plane, before the far
 // plane and inside the window projects to a triangle inside the grid, drawn
 // straight from registers; any other case builds its polygons in LDS.
@@ -451,6 +454,13 @@ __device__ __forceinline__ bool rasterize(const Cone& c, Lane& L, f3 v0, f3 v1, 
       return true;
     }
   }
+  return rasterize_general(c, L, v0, v1, v2, weight, far_clip, su, sv);
+}
+
+// The general path of `rasterize`: the triangle clipped as a polygon.
+__device__ __forceinline__ bool rasterize_general(const Cone& c, Lane& L, f3 v0, f3 v1, f3 v2, float weight,
+                                                  bool far_clip, float su, float sv) {
+  const float g = (float)GRID;
   // General path: the triangle as a polygon at P3A.
   L.put3(P3A, 0, v0);
   L.put3(P3A, 1, v1);

@@ -266,7 +266,7 @@ class Scene:
                                           C.byref(opts), vis.data_ptr(), full.data_ptr(), touched.data_ptr(),
                                           cov.data_ptr() if cov is not None else 0, stream.cuda_stream))
 
-    def view_query_device(self, receivers, width, height, lights, opts, vis, full, touched, tile_w=8,
+    def view_query_device(self, receivers, width, height, lights, opts, vis, full, touched, tile_w=16,
                           stream=None) -> None:
         """Cone queries over a view's receivers (device tensors), each wave
         taking a tile_w x 64/tile_w patch of pixels."""
@@ -363,7 +363,7 @@ class Batch:
         return int(lib().cnq_batch_queries(self.handle))
 
     @staticmethod
-    def tuning(tile_w: int = 8, persistent_blocks: int = 0) -> _lib.Tuning:
+    def tuning(tile_w: int = 16, persistent_blocks: int = 0) -> _lib.Tuning:
         return _lib.Tuning(tile_w, persistent_blocks)
 
     def run(self, opts: _lib.Options, tuning: _lib.Tuning, stream=None) -> None:

@@ -89,6 +89,8 @@ __global__ __launch_bounds__(64, WALK_WAVES) void bake_static_kernel(
   const float* rc = texels + 8 * i;
   const f3 point = ld3(rc), normal = ld3(rc + 4);
   Walk w;
+  float cov[BINS];
+  w.cov = cov;
   uncapped_walk(w, p.prefilter);
   const bool facing = faces_rect(light, point, normal);
   const LightCone lc = light_cone_rect(point, light, p.wide_extent);
@@ -127,6 +129,8 @@ __global__ __launch_bounds__(64, WALK_WAVES) void rebake_kernel(
       const LightCone lc = light_cone_rect(point, light, p.wide_extent);
       const bool on = facing && lc.ok;
       Walk w;
+      float cov[BINS];
+      w.cov = cov;
       uncapped_walk(w, p.prefilter);
       const float* cached = static_cov + (size_t)slot * BINS * 64 + lane;
 #pragma unroll

@@ -344,7 +344,11 @@ struct Ctx {
 
 // Per-lane walk state.
 struct Walk {
-  float cov[64];
+  // The 64 coverage values, in the lane's private memory: a walk reads and
+  // writes them only when a mesh's accumulator is folded in (`combine`) and
+  // at the end, so keeping them out of registers leaves the triangle and
+  // clipping code room without spills (the caller provides the array).
+  float* cov;
   uint32_t full;
   uint32_t cone_work;
   uint32_t cone_cap;
@@ -578,9 +582,8 @@ __device__ __forceinline__ bool cover_triangle(const Cone& c, Lane& L, f3 v0, f3
 __device__ __forceinline__ void combine(Walk& w, Lane& L, bool snap) {
   float running = 0.0f;
   const uint32_t end = L.span_hi < BINS ? L.span_hi : BINS;
-#pragma unroll
-  for (uint32_t b = 0; b < BINS; b++) {
-    if (b >= L.span_lo && b < end) {
+  for (uint32_t b = L.span_lo; b < end; b++) {
+    {
       running += L.get(b);
       float covered = fabsf(running);
       if (snap) {
@@ -794,6 +797,8 @@ __device__ __forceinline__ QueryOut evaluate_at(const Ctx& x, const QueryParams&
                                                 float* __restrict__ cov_out, uint64_t* __restrict__ work_out,
                                                 Skip skip = Skip{NONE, NONE}) {
   Walk w;
+  float cov[BINS];
+  w.cov = cov;
 #pragma unroll
   for (uint32_t b = 0; b < BINS; b++) w.cov[b] = 0.0f;
   w.full = 0;

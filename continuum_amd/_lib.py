@@ -114,6 +114,7 @@ SIGNATURES = {
     "cnq_scene_set_movers": (C.c_int, [_P, C.POINTER(MoverTree), _P, C.c_uint32, _P]),
     "cnq_scene_set_transform": (C.c_int, [_P, C.c_uint32, _P, _P]),
     "cnq_scene_apply_moves": (C.c_int, [_P, _P]),
+    "cnq_scene_set_pose": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32, _P]),
     "cnq_flat_instanced_sizes": (C.c_int, [_P, _P]),
     "cnq_flat_copy_instanced": (C.c_int, [_P, _P, _P]),
     "cnq_scene_create": (C.c_int, [_P, C.POINTER(SceneDesc), C.POINTER(_P)]),

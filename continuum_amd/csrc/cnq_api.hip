@@ -736,6 +736,78 @@ int cnq_scene_apply_moves(cnq_scene* s, void* stream) {
   return CNQ_OK;
 }
 
+int cnq_scene_set_pose(cnq_scene* s, uint32_t geometry, const float* tris9, uint32_t n_tris, void* stream) {
+  if (!s || !tris9) return fail(CNQ_INVALID_ARG, "null argument");
+  if (!s->has_movers) return fail(CNQ_INVALID_ARG, "poses need a scene with movers (cnq_scene_set_movers)");
+  if (geometry >= s->geometry_bounds.size()) return fail(CNQ_INVALID_ARG, "geometry out of range");
+  // The geometry's triangles and hierarchy, from one of its movers' ranges.
+  uint32_t first = NONE, count = 0, root = NONE;
+  for (size_t k = 0; k < s->mover_placements.size(); k++) {
+    if (s->mover_placements[k].geometry != geometry) continue;
+    const uint32_t* r = &s->h_ranges[4 * (size_t)s->movers.objects[k]];
+    first = r[0], count = r[1], root = r[2];
+    break;
+  }
+  if (first == NONE) return fail(CNQ_INVALID_ARG, "no movable object places this geometry");
+  if (n_tris != count) return fail(CNQ_INVALID_ARG, "a pose keeps the geometry's triangle count");
+  // instanced.rs: deforming geometries are placed by movable objects only.
+  for (uint32_t m = 0; m < s->n_objects; m++) {
+    const uint32_t* r = &s->h_ranges[4 * (size_t)m];
+    if (r[0] == first && r[1] == count && r[2] == root && s->mover_of[m] == NONE)
+      return fail(CNQ_INVALID_ARG, "object " + std::to_string(m) + " places the posed geometry but is not movable");
+  }
+  HIP_TRY(hipSetDevice(s->dev->ordinal), "hipSetDevice");
+  hipStream_t st = (hipStream_t)stream;
+  // The geometry's leaves (each leaf climbs to one root), found once.
+  auto it = s->pose_leaves.find(root);
+  if (it == s->pose_leaves.end() && root != NONE) {
+    std::vector<uint32_t> parent(s->n_blas), leaves(s->leaves.n);
+    HIP_TRY(hipMemcpy(parent.data(), s->parent.p, parent.size() * 4, hipMemcpyDeviceToHost), "parents");
+    HIP_TRY(hipMemcpy(leaves.data(), s->leaves.p, leaves.size() * 4, hipMemcpyDeviceToHost), "leaves");
+    std::vector<uint32_t> mine;
+    for (uint32_t leaf : leaves) {
+      uint32_t n = leaf;
+      while (parent[n] != NONE) n = parent[n];
+      if (n == root) mine.push_back(leaf);
+    }
+    cnq::detail::DevBuf<uint32_t> d;
+    HIP_TRY(d.upload(mine.data(), mine.size()), "pose leaves");
+    it = s->pose_leaves.emplace(root, d).first;
+  }
+  // Triangles: xyz of each vertex into the 16-byte rows (w keeps its code).
+  if (count)
+    HIP_TRY(hipMemcpy2DAsync(s->tris.p + 12 * (size_t)first, 16, tris9, 12, 12, 3 * (size_t)count,
+                             hipMemcpyHostToDevice, st), "pose triangles");
+  if (root != NONE && it->second.n) {
+    HIP_TRY(fill_u32(s->arrivals.p, 0, s->n_blas, st), "refit clear");
+    cnq::RefitArgs a;
+    a.tris = s->tris.p;
+    a.blas_items = s->blas_items.p;
+    a.blas = s->blas.p;
+    a.blas_cov = s->blas_cov.p;
+    a.tri_leaf = s->tri_leaf.p;
+    a.parent = s->parent.p;
+    a.arrivals = s->arrivals.p;
+    a.leaves = it->second.p;
+    a.n_leaves = (uint32_t)it->second.n;
+    HIP_TRY(cnq::launch_refit(a, st), "pose refit");
+  }
+  // Its new bounds (hierarchy.rs:751 mesh_bounds) and its instances moved.
+  cnq::host::Box b;
+  for (int c = 0; c < 3; c++) b.lo[c] = 3.40282347e38f, b.hi[c] = -3.40282347e38f;
+  for (uint64_t v = 0; v < 3 * (uint64_t)count; v++)
+    for (int c = 0; c < 3; c++) {
+      const float x = tris9[3 * v + c];
+      b.lo[c] = x < b.lo[c] ? x : b.lo[c];
+      b.hi[c] = x > b.hi[c] ? x : b.hi[c];
+    }
+  s->geometry_bounds[geometry] = b;
+  for (size_t k = 0; k < s->mover_placements.size(); k++)
+    if (s->mover_placements[k].geometry == geometry) s->mover_dirty[k] = 1;
+  HIP_TRY(hipStreamSynchronize(st), "pose");
+  return CNQ_OK;
+}
+
 int cnq_scene_last_moves(cnq_scene* s, cnq_moved* out, uint32_t cap, uint32_t* n) {
   if (!s || !n || (cap && !out)) return fail(CNQ_INVALID_ARG, "null argument");
   *n = (uint32_t)s->last_moves.size();

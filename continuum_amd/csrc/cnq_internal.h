@@ -4,6 +4,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <string>
 #include <vector>
 
@@ -80,6 +81,7 @@ struct cnq_scene {
   std::vector<uint32_t> mover_of;  // per object: its index among the movers, or NONE
   std::vector<cnq::host::Box> mover_bounds;  // each mover's world box as last applied
   std::vector<cnq_moved> last_moves;         // the last apply_moves's moves
+  std::map<uint32_t, cnq::detail::DevBuf<uint32_t>> pose_leaves;  // per posed geometry's root: its leaves
   bool instanced = false;
 
   cnq::SceneArgs args() const {
@@ -100,6 +102,8 @@ struct cnq_scene {
     tris.release(), tri_leaf.release(), blas_cov.release(), xforms.release(), mesh_box.release();
     blas_items.release(), ranges.release(), items.release(), parent.release(), leaves.release();
     arrivals.release(), counters.release(), work.release(), blas.release(), nodes.release();
+    for (auto& kv : pose_leaves) kv.second.release();
+    pose_leaves.clear();
   }
   uint64_t bytes() const {
     return tris.bytes() + tri_leaf.bytes() + blas_cov.bytes() + xforms.bytes() + mesh_box.bytes() +

@@ -217,6 +217,14 @@ class Scene:
         (instanced.rs:1226-1410)."""
         check(lib().cnq_scene_apply_moves(self.handle, _stream_ptr(stream)))
 
+    def set_pose(self, geometry: int, tris, stream=None) -> None:
+        """InstancedRenderer::apply_poses (instanced.rs:1154) for one
+        deforming geometry: new triangles (its count, its order), its
+        hierarchy refit, its instances (all movable) marked moved; call
+        apply_moves next."""
+        t = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+        check(lib().cnq_scene_set_pose(self.handle, int(geometry), _ptr(t), len(t), _stream_ptr(stream)))
+
     def last_moves(self) -> list:
         """The last apply_moves's moves: (object, old box (lo, hi), new box),
         as `_lib.MovedBox` records for ViewCache.set_moved."""

@@ -162,6 +162,14 @@ int cnq_scene_set_movers(cnq_scene* scene, const cnq_mover_tree* tree, const flo
                          uint32_t n_geometries, const cnq_placement* placements);
 int cnq_scene_set_transform(cnq_scene* scene, uint32_t object, const float* rotation, const float* translation);
 int cnq_scene_apply_moves(cnq_scene* scene, void* stream);
+/* A new pose of a deforming geometry (instanced.rs:1154-1223 `apply_poses`):
+ * its `n_tris` triangles (9 floats each, the geometry's count and order)
+ * replace its vertices, its triangle hierarchy is refit on the GPU (same
+ * topology and leaf order, `refit_triangle_hierarchy`, hierarchy.rs:579),
+ * its bounds are recomputed (`mesh_bounds`), and every object placing it
+ * (all must be movable) is marked moved, so the next cnq_scene_apply_moves
+ * rebuilds the movers' subtree over their new boxes. */
+int cnq_scene_set_pose(cnq_scene* scene, uint32_t geometry, const float* tris9, uint32_t n_tris, void* stream);
 
 /* ---- Queries. MaskConeKernel::irradiance_with (lib.rs:1718) and
  * `coverage` (lib.rs:3289): host buffers in and out, synchronous. out_cov

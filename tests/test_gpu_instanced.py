@@ -128,3 +128,77 @@ def test_moves_equal_a_fresh_build_and_the_oracle(device):
         assert np.array_equal(got[1].view(np.uint32), rv.view(np.uint32)) and np.array_equal(got[2], rf)
     with pytest.raises(Exception):
         scene.set_transform(0, np.eye(3), [0, 0, 0])    # the ground is static
+
+
+def _cloth(nu=24, nv=16, phase=0.0):
+    """A 2 m x 1.5 m sheet on an nu x nv grid, rippled by `phase`."""
+    u = np.linspace(-1.0, 1.0, nu + 1)
+    v = np.linspace(-0.75, 0.75, nv + 1)
+    x, y = np.meshgrid(u, v, indexing="ij")
+    z = 0.6 + 0.15 * np.sin(3 * x + phase) * np.cos(2 * y - phase)
+    g = np.stack([x, y, z], axis=-1).astype(np.float32)
+    a, b, c, d = g[:-1, :-1], g[1:, :-1], g[1:, 1:], g[:-1, 1:]
+    return np.stack([np.stack([a, b, c], axis=2), np.stack([a, c, d], axis=2)], axis=2).reshape(-1, 3, 3)
+
+
+def test_poses_match_the_oracle(device):
+    """Deforming geometry (instanced.rs:1154 apply_poses): a sheet placed by
+    two movable objects takes a new pose; its hierarchy is refit on the GPU
+    (bit for bit the oracle's `refit_triangle_hierarchy`), the movers'
+    subtree follows its new bounds, and every query equals the oracle's on
+    the refit scene. A geometry with a static instance refuses poses."""
+    geoms = [quad_pair([-12, -12, 0], [24, 0, 0], [0, 24, 0]), box_mesh([-0.5, -0.3, 0.0], [0.5, 0.3, 0.8]),
+             _cloth()]
+    tris9, counts = meshes_to_arrays(geoms)
+    rng = np.random.default_rng(4)
+    n = 14
+    pl = np.zeros(n, dtype=PLACEMENT_DTYPE)
+    pl["rotation"][:] = [1, 0, 0, 0, 1, 0, 0, 0, 1]
+    pl["scale"][:] = 1.0
+    for k in range(1, n):
+        pl[k]["geometry"] = 2 if k in (3, 9) else 1
+        pl[k]["translation"] = [rng.uniform(-8, 8), rng.uniform(-8, 8), 0.0]
+    movable = [3, 9, 5]
+    lights = np.stack([rect_light([-2.0, -2.0, 5.0], [4.0, 0, 0], [0, 4.0, 0]), sphere_light([30.0, -20.0, 40.0], 2.0)])
+    f = Flat.build_instanced(tris9, counts, pl, object_prefilter=True, movable=movable)
+    scene = Scene(device, f)
+    posed = _cloth(phase=1.3).reshape(-1, 9)
+    off = np.concatenate([[0], np.cumsum(counts)]).astype(int)
+    print("pose: sheet of", len(posed), "triangles on objects 3 and 9", flush=True)
+    scene.set_pose(2, posed)
+    scene.apply_moves()
+    with pytest.raises(Exception):
+        scene.set_pose(1, tris9[off[1]:off[2]])  # boxes 1, 2, ... are static
+    # the oracle's scene: the new triangles, the original hierarchy refit, and
+    # the object hierarchy over the new bounds
+    new9 = tris9.copy()
+    new9[off[2]:off[3]] = posed
+    fresh = Flat.build_instanced(new9, counts, pl, object_prefilter=True, movable=movable)
+    o = O.OracleInstancedFlat(tris9, counts, pl, object_prefilter=True, movable=movable)
+    nb = len(o.blas)
+    blas_r, cov_r = O.refit(o, fresh.tris)
+    got_blas, got_cov = scene.read_blas()
+    cols = [0, 1, 2, 4, 5, 6, 8, 9]
+    assert np.array_equal(got_blas[:, cols], blas_r[:, cols])
+    assert np.array_equal(got_cov[:nb].view(np.uint32), cov_r[:nb].view(np.uint32))
+
+    class Assembled:
+        pass
+
+    a = Assembled()
+    a.tris, a.ranges, a.blas, a.blas_items = fresh.tris, fresh.ranges, blas_r, o.blas_items
+    a.blas_cov = np.ascontiguousarray(np.concatenate([cov_r[:nb], fresh.blas_cov[nb:]]))
+    a.nodes, a.items, a.xforms = fresh.nodes, fresh.items, fresh.xforms
+    cam = camera_uniform([-10.0, -9.0, 3.0], [0.0, 0.0, 0.5], 60.0, 96, 64)
+    for prefilter in (0.0, 2.0):
+        kw = dict(hierarchy=2, prefilter=prefilter, object_cov=f.object_cov, light_cone_cap=O.NONE,
+                  wide_cone_cap=O.NONE)
+        hits, vis, full, touched = scene.render_view(cam, 96, 64, lights, _lib.default_options(**kw))
+        rh, recv, _ = O.primary(a, cam, 96, 64, lights, hierarchy=2)
+        assert np.array_equal(hits, rh)
+        rv, rf, rt, _, capped, _ = O.irradiance(a, recv, lights, O.opts_array(**kw))
+        assert np.array_equal(vis.view(np.uint32), rv.view(np.uint32)) and np.array_equal(full, rf)
+        assert np.array_equal(touched, rt)
+    # the sheet's pixels see the new pose: its hits are not the rest pose's
+    rest = Scene(device, f).render_view(cam, 96, 64, lights, _lib.default_options(hierarchy=2))
+    assert not np.array_equal(rest[0], hits)

@@ -91,6 +91,11 @@ class VcacheConfig(C.Structure):
                 ("base_texel", C.c_float), ("light_ratio", C.c_uint32), ("visible_frames", C.c_uint32)]
 
 
+class VcacheShadeParams(C.Structure):
+    _fields_ = [("pixels_per_texel", C.c_float), ("refine_levels", C.c_uint32), ("request_tile", C.c_uint32),
+                ("pixel_angle", C.c_float)]
+
+
 class MovedBox(C.Structure):
     _fields_ = [("object", C.c_uint32), ("old_lo", C.c_float * 3), ("old_hi", C.c_float * 3),
                 ("new_lo", C.c_float * 3), ("new_hi", C.c_float * 3)]
@@ -171,6 +176,8 @@ SIGNATURES = {
     "cnq_vcache_frame": (C.c_int, [_P, _P, _P]),
     "cnq_vcache_read": (C.c_int, [_P, _P, C.c_uint32, _P, _P]),
     "cnq_vcache_state": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P]),
+    "cnq_vcache_shade": (C.c_int, [_P, _P, C.c_uint32, C.c_uint32, _P, _P, _P, C.POINTER(VcacheShadeParams), _P, _P]),
+    "cnq_vcache_plan": (C.c_int, [_P, C.c_uint32, _P]),
 }
 
 _lib = None

@@ -38,6 +38,11 @@ constexpr uint32_t VC_TEXELS = vc::PAGE_TEXELS;
 // pages tested, 3 appends that did not fit, 4 candidates, 5 jobs appended,
 // 6 pages made stale for a re-bake.
 constexpr uint32_t VC_STATS = 8;
+// The request set's entries (vcache.rs:85 REQUEST_SET) and the most
+// requests one shading pass lists.
+constexpr uint32_t VC_REQUEST_SET = 1u << 18;
+constexpr uint32_t VC_REQUEST_CAP = 1u << 16;
+constexpr uint32_t VC_EVICT_PER_FRAME = 2048;  // vcache.rs:41
 
 struct VcArgs {
   const vc::Chart* __restrict__ charts;
@@ -597,6 +602,300 @@ __global__ __launch_bounds__(64) void vc_fill_kernel(VcArgs a) {
   }
 }
 
+// ---- shading from the cache (WGSL :5907-6130, the direct light's part)
+
+struct ShadeArgs {
+  const float* __restrict__ camera;  // 20 floats: eye, forward, right, up, params
+  const uint32_t* __restrict__ hits; // 4 per pixel: object, triangle, distance bits, seen light + 1
+  const uint32_t* __restrict__ tri_chart;
+  float* __restrict__ out;           // per light and pixel: l * w * h + p
+  uint32_t* __restrict__ out_page;   // per pixel: the shown page's mip, top bit set when none was found
+  uint32_t* request_set;             // fingerprints (zeroed before the pass)
+  uint32_t* request_prio;            // per set entry: the most urgent priority asked (all ones before)
+  uint32_t* requests;                // 4 per request: key, then its set entry (its priority after the gather)
+  uint32_t* counts;                  // [requests, pixels that fell back]
+  uint32_t w, h, set_mask, request_cap, tile, refine;
+  float pixel_angle, pixels_per_texel;
+};
+
+namespace {
+
+__device__ __forceinline__ uint32_t key_hash(uint32_t k0, uint32_t k1, uint32_t k2) {
+  uint32_t h = (k0 * 0x9E3779B1u) ^ (k1 * 0x85EBCA77u) ^ (k2 * 0xC2B2AE3Du);
+  h ^= h >> 15;
+  h *= 0x2C1B3C6Du;
+  h ^= h >> 12;
+  return h;
+}
+
+// WGSL :3134-3160 without stale marks other than 0 and 3 (no ledger here).
+__device__ __forceinline__ uint32_t vc_find(const VcArgs& a, uint32_t k0, uint32_t k1, uint32_t k2) {
+  uint32_t at = key_hash(k0, k1, k2) & a.table_mask;
+  for (uint32_t probe = 0; probe < vc::MAX_PROBE; probe++) {
+    const uint32_t* e = a.page_table + 4 * (size_t)at;
+    if (e[0] == vc::EMPTY) return NONE;
+    if (e[0] == k0 && e[1] == k1 && e[2] == k2) {
+      const uint32_t stale = a.slot_stale[e[3]];
+      return (stale != 0u && stale != 3u) ? NONE : e[3];
+    }
+    at = (at + 1u) & a.table_mask;
+  }
+  return NONE;
+}
+
+__device__ __forceinline__ f3 point_to_object(const VcArgs& a, uint32_t m, f3 p) {
+  const uint32_t x = a.ranges[4 * m + 3];
+  if (x == NONE) return p;
+  const float* xf = a.xforms + 16 * (size_t)x;
+  const f3 d = p - translation(xf);
+  // rotate_to_object: the rows' transpose (WGSL :61)
+  return (row3(xf, 0) * d.x + row3(xf, 1) * d.y + row3(xf, 2) * d.z) * xf[13];
+}
+
+// WGSL :1780-1793
+__device__ __forceinline__ void barycentric(const VcArgs& a, uint32_t m, uint32_t t, f3 world_p, float b[3]) {
+  const f3 p = point_to_object(a, m, world_p);
+  const float* tr = a.tris + 12 * (size_t)t;
+  const f3 v0 = ld3(tr);
+  const f3 e1 = ld3(tr + 4) - v0, e2 = ld3(tr + 8) - v0;
+  const f3 d = p - v0;
+  const float d11 = dot3(e1, e1), d12 = dot3(e1, e2), d22 = dot3(e2, e2);
+  const float denom = d11 * d22 - d12 * d12;
+  const float sv = (d22 * dot3(d, e1) - d12 * dot3(d, e2)) / denom;
+  const float u = (d11 * dot3(d, e2) - d12 * dot3(d, e1)) / denom;
+  b[0] = 1.0f - sv - u;
+  b[1] = sv;
+  b[2] = u;
+}
+
+__device__ __forceinline__ f3 normal_to_world(const VcArgs& a, uint32_t m, f3 n) {
+  const uint32_t x = a.ranges[4 * m + 3];
+  if (x == NONE) return n;
+  return rotate_to_world(a.xforms + 16 * (size_t)x, n);
+}
+
+// WGSL :3187-3196
+__device__ __forceinline__ uint32_t vc_top_mip(const VcArgs& a, uint32_t c, uint32_t m) {
+  const vc::Chart ch = a.charts[c];
+  const float ratio = mx(ch.size[0], ch.size[1]) * object_scale(a, m) * a.inv_base;
+  uint32_t top = 0;
+  if (ratio > 1.0f) {
+    const float l = ceilf(log2f(ratio));
+    top = l < 30.0f ? (uint32_t)l : 30u;
+  }
+  uint32_t n[2];
+  if (top > 0) {
+    vc_grid(a, c, m, top - 1, n);
+    if (n[0] == 1 && n[1] == 1) top -= 1;
+  }
+  vc_grid(a, c, m, top, n);
+  if (n[0] != 1 || n[1] != 1) top += 1;
+  return top;
+}
+
+struct PageRef {
+  uint32_t key[3];
+  uint32_t base[2];
+  float frac[2];
+  bool coarsest;
+};
+
+// WGSL :3218-3231
+__device__ __forceinline__ PageRef vc_page(const VcArgs& a, uint32_t m, uint32_t c, f2 uv, uint32_t mip) {
+  uint32_t n[2];
+  vc_grid(a, c, m, mip, n);
+  const float tx = clampf(uv.x * (float)n[0] - 0.5f, 0.0f, (float)(n[0] - 1u));
+  const float ty = clampf(uv.y * (float)n[1] - 0.5f, 0.0f, (float)(n[1] - 1u));
+  const uint32_t bx = (uint32_t)floorf(tx), by = (uint32_t)floorf(ty);
+  const uint32_t px = bx / vc::PAGE, py = by / vc::PAGE;
+  PageRef r;
+  r.key[0] = m;
+  r.key[1] = c | (mip << vc::KEY_MIP_SHIFT);
+  r.key[2] = px | (py << 16);
+  r.base[0] = bx - px * vc::PAGE;
+  r.base[1] = by - py * vc::PAGE;
+  r.frac[0] = tx - floorf(tx);
+  r.frac[1] = ty - floorf(ty);
+  r.coarsest = n[0] == 1 && n[1] == 1;
+  return r;
+}
+
+// WGSL :3242-3247
+__device__ __forceinline__ float vc_bilinear(const VcArgs& a, uint32_t slot, const PageRef& r, uint32_t ch) {
+  const uint32_t o = slot * VC_TEXELS + r.base[1] * VC_STRIDE + r.base[0];
+  const float* p = a.pool;
+  const uint32_t C = a.channels;
+  const float x0 = mixf(p[(size_t)o * C + ch], p[(size_t)(o + 1) * C + ch], r.frac[0]);
+  const float x1 = mixf(p[(size_t)(o + VC_STRIDE) * C + ch], p[(size_t)(o + VC_STRIDE + 1) * C + ch], r.frac[0]);
+  return mixf(x0, x1, r.frac[1]);
+}
+
+// WGSL :5907-5925 (a strong compare-and-swap: no spurious failures to
+// retry). A page several pixels ask for keeps the most urgent (smallest)
+// of their priorities, whichever pixel lists it.
+__device__ __forceinline__ void vc_request(const ShadeArgs& s, const uint32_t k[3], uint32_t priority) {
+  const uint32_t fp = key_hash(k[0], k[1], k[2]) | 1u;
+  uint32_t i = key_hash(k[2], k[0], k[1]) & s.set_mask;
+  for (uint32_t probe = 0; probe < 32u; probe++) {
+    const uint32_t old = atomicCAS(&s.request_set[i], 0u, fp);
+    if (old == 0u) {
+      atomicMin(&s.request_prio[i], priority);
+      const uint32_t n = atomicAdd(&s.counts[0], 1u);
+      if (n < s.request_cap) {
+        uint32_t* r = s.requests + 4 * (size_t)n;
+        r[0] = k[0], r[1] = k[1], r[2] = k[2], r[3] = i;
+      }
+      return;
+    }
+    if (old == fp) {
+      atomicMin(&s.request_prio[i], priority);
+      return;
+    }
+    i = (i + 1u) & s.set_mask;
+  }
+}
+
+}  // namespace
+
+// WGSL :5937-6024 and :6078-6130 for the lights' direct part: one lane per
+// pixel of the camera pass's hits; per light the cached share times the
+// unshadowed light at the pixel (or, without light_ratio, the cached value),
+// the unshadowed light where no page is resident. One pixel per request
+// tile (a different one each frame) asks for the page one to `refine` mips
+// finer than the one it found, or for the coarsest missing one.
+__global__ __launch_bounds__(256) void vc_shade_kernel(VcArgs a, ShadeArgs s) {
+  const uint32_t p = blockIdx.x * 256u + threadIdx.x;
+  const uint32_t n_px = s.w * s.h;
+  const bool in_bounds = p < n_px;
+  const uint32_t x = in_bounds ? p % s.w : 0u, y = in_bounds ? p / s.w : 0u;
+  const uint32_t sample = (a.frame * 7u) % (s.tile * s.tile);
+  const bool is_sample = in_bounds && x % s.tile == sample % s.tile && y % s.tile == sample / s.tile;
+  uint32_t req_key[3] = {0u, 0u, 0u}, priority = 0u;
+  bool fell_back = false;
+  if (in_bounds) {
+    const float* cam = s.camera;
+    const float* pp = cam + 16;
+    const float aspect = pp[0] / pp[1];
+    const float sx = (((float)x + 0.5f + cam[11]) / pp[0] * 2.0f - 1.0f) * pp[2] * aspect;
+    const float sy = (1.0f - ((float)y + 0.5f + cam[15]) / pp[1] * 2.0f) * pp[2];
+    const f3 dir = norm3(ld3(cam + 4) + ld3(cam + 8) * sx + ld3(cam + 12) * sy);
+    const uint32_t* v = s.hits + 4 * (size_t)p;
+    const uint32_t m = v[0], t = v[1];
+    if (m != NONE && v[3] == 0u) {
+      const float distance = __uint_as_float(v[2]);
+      const float* tr = a.tris + 12 * (size_t)t;
+      const f3 v0 = ld3(tr);
+      const f3 face = cross3(ld3(tr + 4) - v0, ld3(tr + 8) - v0);
+      f3 hn = norm3(face);
+      if (a.ranges[4 * m + 3] != NONE) hn = norm3(normal_to_world(a, m, hn));
+      if (dot3(hn, dir) > 0.0f) hn = -hn;
+      const f3 surface = ld3(cam) + dir * distance;
+      const uint32_t c = s.tri_chart[t];
+      float b[3];
+      barycentric(a, m, t, surface, b);
+      const float* q = a.tri_uv + 6 * (size_t)t;
+      const f2 uv = f2{q[0], q[1]} * b[0] + f2{q[2], q[3]} * b[1] + f2{q[4], q[5]} * b[2];
+      // WGSL :3198-3202 `vc_mip`
+      const float ratio = distance * s.pixel_angle * s.pixels_per_texel * a.inv_base;
+      uint32_t want = 0;
+      if (ratio > 1.0f) {
+        const float lr = rintf(log2f(ratio));
+        want = lr < 30.0f ? (uint32_t)lr : 30u;
+      }
+      const uint32_t top = vc_top_mip(a, c, m);
+      const uint32_t wanted = want < top ? want : top;
+      // WGSL :6024-6070 `vc_find_shown`
+      uint32_t mip = wanted, slot = NONE;
+      uint32_t finer[4][3] = {{0u, 0u, 0u}, {0u, 0u, 0u}, {0u, 0u, 0u}, {0u, 0u, 0u}};
+      PageRef r;
+      bool found = false;
+      for (;;) {
+        r = vc_page(a, m, c, uv, mip);
+        slot = vc_find(a, r.key[0], r.key[1], r.key[2]);
+        if (slot != NONE) {
+          a.slot_used[slot] = a.frame;
+          found = true;
+          if (mip > wanted) {
+            const uint32_t step = s.refine;
+            const uint32_t* pick = finer[0];
+            if (step >= 2u && mip >= wanted + 2u) pick = finer[1];
+            if (step >= 3u && mip >= wanted + 3u) pick = finer[2];
+            if (step >= 4u && mip >= wanted + 4u) pick = finer[3];
+            req_key[0] = pick[0], req_key[1] = pick[1], req_key[2] = pick[2];
+            const uint32_t gap = mip - wanted;
+            priority = 32u - (gap < 30u ? gap : 30u);
+          }
+          break;
+        }
+        for (int k = 0; k < 3; k++) {
+          finer[3][k] = finer[2][k];
+          finer[2][k] = finer[1][k];
+          finer[1][k] = finer[0][k];
+          finer[0][k] = r.key[k];
+        }
+        if (r.coarsest || mip >= 30u) break;
+        mip += 1;
+      }
+      if (!found) {
+        req_key[0] = finer[0][0], req_key[1] = finer[0][1], req_key[2] = finer[0][2];
+        priority = 1u;
+      }
+      fell_back = !found;
+      const f3 point = surface + hn * surface_offset(distance);
+      // WGSL :2046-2059 `shading_normal`, toward the camera
+      const float* nr = a.normals + 9 * (size_t)t;
+      const f3 sn = ld3(nr) * b[0] + ld3(nr + 3) * b[1] + ld3(nr + 6) * b[2];
+      f3 normal = norm3(normal_to_world(a, m, dot3(sn, sn) > 1e-20f ? sn : face));
+      if (dot3(normal_to_world(a, m, face), -dir) < 0.0f) normal = -normal;
+      for (uint32_t l = 0; l < a.n_lights; l++) {
+        const float* lr = a.lights + 16 * (size_t)l;
+        float e;
+        if (found && !a.light_ratio) {
+          e = vc_bilinear(a, slot, r, l);
+        } else {
+          const float share = found ? vc_bilinear(a, slot, r, l) : 1.0f;
+          e = 0.0f;
+          if (share > 0.0f) {
+            // unshadowed_rect (WGSL :2135)
+            float open = 0.0f;
+            const LightRect lt = reinterpret_cast<const LightRect*>(a.lights)[l];
+            if (faces_light(lt, point, normal)) {
+              if (lr[3] > 1.5f) {
+                open = spot_falloff(lt, norm3(point - ld3(lr))) * sphere_unshadowed(point, normal, lt);
+              } else {
+                // rect_open without its clipped case's workspace: the
+                // horizon clip runs in the lane's registers
+                float poly[POLY_SLOTS];
+                float acc_dummy[1];
+                Lane L{acc_dummy, poly, 0u, 0u};
+                open = rect_open(L, point, normal, lt);
+              }
+            }
+            e = share * open;
+          }
+        }
+        s.out[(size_t)l * n_px + p] = e;
+      }
+      s.out_page[p] = found ? mip : (0x80000000u | mip);
+    } else {
+      for (uint32_t l = 0; l < a.n_lights; l++) s.out[(size_t)l * n_px + p] = 0.0f;
+      s.out_page[p] = NONE;
+    }
+  }
+  if (is_sample && priority > 0u) vc_request(s, req_key, priority);
+  const uint64_t fb = ballot(fell_back);
+  if (lane_id() == 0 && fb) atomicAdd(&s.counts[1], (uint32_t)__builtin_popcountll(fb));
+}
+
+// After the shading pass: each listed request takes its entry's priority.
+__global__ __launch_bounds__(256) void vc_request_gather_kernel(ShadeArgs s) {
+  const uint32_t n = blockIdx.x * 256u + threadIdx.x;
+  const uint32_t count = s.counts[0] < s.request_cap ? s.counts[0] : s.request_cap;
+  if (n >= count) return;
+  uint32_t* r = s.requests + 4 * (size_t)n;
+  r[3] = s.request_prio[r[3]];
+}
+
 // WGSL :3134-3160 (`vc_find`, a page shown unless marked), then the page's
 // texels: one wave per key.
 __global__ __launch_bounds__(64) void vc_read_kernel(VcArgs a, const uint32_t* __restrict__ keys, uint32_t n_keys,
@@ -672,11 +971,15 @@ struct cnq_vcache {
   uint32_t n_movers = 0;
   std::vector<std::pair<uint32_t, uint32_t>> mover_ids;
   uint64_t last_stats[10] = {0};
+  uint32_t shade_requests = 0;  // requests the last shading pass listed
+  DevBuf<float> d_camera;
   std::vector<uint32_t> upd_host;  // staging for flush_updates (alive until the stream syncs)
 
   DevBuf<vc::Chart> d_charts;
   DevBuf<uint32_t> d_cells, d_cell_tris, d_page_table, d_slot_info, d_slot_stale, d_slot_used, d_bake_list,
       d_bake_jobs, d_candidates, d_invalid, d_stats, d_updates;
+  // Shading from the cache: each triangle's chart, the request set and list.
+  DevBuf<uint32_t> d_tri_chart, d_request_set, d_request_prio, d_requests, d_shade_counts;
   DevBuf<float> d_tri_uv, d_normals, d_lights, d_pool, d_bake_bounds, d_moved;
   uint32_t list_cap = 0, job_cap = 0;  // bake list / jobs capacity (plan + append)
 
@@ -685,6 +988,9 @@ struct cnq_vcache {
     d_slot_stale.release(), d_slot_used.release(), d_bake_list.release(), d_bake_jobs.release();
     d_candidates.release(), d_invalid.release(), d_stats.release(), d_updates.release(), d_tri_uv.release();
     d_normals.release(), d_lights.release(), d_pool.release(), d_bake_bounds.release(), d_moved.release();
+    d_tri_chart.release(), d_request_set.release(), d_request_prio.release(), d_requests.release();
+    d_shade_counts.release();
+    d_camera.release();
   }
   uint32_t append_jobs() const { return cfg.append_pages * vc::PAGE_TEXELS * n_lights; }
 
@@ -861,6 +1167,11 @@ int cnq_vcache_create(cnq_scene* s, const cnq_vcache_surfaces* sf, const float* 
   if (e == hipSuccess) e = c->d_cell_tris.upload(c->tables.cell_tris.data(), c->tables.cell_tris.size());
   if (e == hipSuccess) e = c->d_tri_uv.upload(sf->tri_uv, 6 * (size_t)s->n_tris);
   if (e == hipSuccess) e = c->d_normals.upload(sf->normals, 9 * (size_t)s->n_tris);
+  if (e == hipSuccess) e = c->d_tri_chart.upload(sf->tri_chart, s->n_tris);
+  if (e == hipSuccess) e = c->d_request_set.alloc(cnq::VC_REQUEST_SET);
+  if (e == hipSuccess) e = c->d_request_prio.alloc(cnq::VC_REQUEST_SET);
+  if (e == hipSuccess) e = c->d_requests.alloc(4 * (size_t)cnq::VC_REQUEST_CAP);
+  if (e == hipSuccess) e = c->d_shade_counts.alloc(2);
   if (e == hipSuccess) e = c->d_lights.upload(lights, 16 * (size_t)n_lights);
   if (e == hipSuccess) e = c->d_page_table.upload(c->table.entries.data(), c->table.entries.size());
   if (e == hipSuccess) e = c->d_slot_info.alloc(12 * (size_t)P);
@@ -1196,6 +1507,161 @@ int cnq_vcache_state(cnq_vcache* c, uint32_t* slot_info, uint32_t* slot_stale, u
   if (table_device)
     HIP_TRY(hipMemcpy(table_device, c->d_page_table.p, c->d_page_table.bytes(), hipMemcpyDeviceToHost), "table");
   if (table_host) std::memcpy(table_host, c->table.entries.data(), c->table.entries.size() * 4);
+  return CNQ_OK;
+}
+
+
+int cnq_vcache_shade(cnq_vcache* c, const float* camera, uint32_t width, uint32_t height, const uint32_t* hits,
+                     float* out, uint32_t* out_page, const cnq_vcache_shade_params* sp, void* stream,
+                     uint32_t counts[2]) {
+  if (!c || !camera || !hits || !out || !out_page || !sp) return fail(CNQ_INVALID_ARG, "null argument");
+  if (width == 0 || height == 0 || (uint64_t)width * height > 0xffffffffull)
+    return fail(CNQ_INVALID_ARG, "bad view size");
+  if (sp->request_tile == 0 || sp->refine_levels == 0 || sp->refine_levels > 4)
+    return fail(CNQ_INVALID_ARG, "request_tile >= 1, refine_levels 1..4");
+  HIP_TRY(hipSetDevice(c->scene->dev->ordinal), "hipSetDevice");
+  hipStream_t st = (hipStream_t)stream;
+  // The camera goes by value through a small device copy.
+  if (c->d_camera.n < 20) HIP_TRY(c->d_camera.alloc(20), "camera");
+  HIP_TRY(hipMemcpyAsync(c->d_camera.p, camera, 80, hipMemcpyHostToDevice, st), "camera");
+  HIP_TRY(fill_u32(c->d_request_set.p, 0, cnq::VC_REQUEST_SET, st), "request set clear");
+  HIP_TRY(fill_u32(c->d_request_prio.p, 0xffffffffu, cnq::VC_REQUEST_SET, st), "request priorities clear");
+  HIP_TRY(fill_u32(c->d_shade_counts.p, 0, 2, st), "shade counts clear");
+  cnq::VcArgs a = c->args();
+  cnq::ShadeArgs s;
+  s.camera = c->d_camera.p;
+  s.hits = hits;
+  s.tri_chart = c->d_tri_chart.p;
+  s.out = out;
+  s.out_page = out_page;
+  s.request_set = c->d_request_set.p;
+  s.request_prio = c->d_request_prio.p;
+  s.requests = c->d_requests.p;
+  s.counts = c->d_shade_counts.p;
+  s.w = width;
+  s.h = height;
+  s.set_mask = cnq::VC_REQUEST_SET - 1;
+  s.request_cap = cnq::VC_REQUEST_CAP;
+  s.tile = sp->request_tile;
+  s.refine = sp->refine_levels;
+  s.pixel_angle = sp->pixel_angle;
+  s.pixels_per_texel = sp->pixels_per_texel;
+  const uint64_t px = (uint64_t)width * height;
+  hipLaunchKernelGGL(cnq::vc_shade_kernel, dim3((uint32_t)((px + 255) / 256)), dim3(256), 0, st, a, s);
+  HIP_TRY(hipGetLastError(), "vc_shade launch");
+  hipLaunchKernelGGL(cnq::vc_request_gather_kernel, dim3(cnq::VC_REQUEST_CAP / 256), dim3(256), 0, st, s);
+  HIP_TRY(hipGetLastError(), "request gather launch");
+  uint32_t got[2];
+  HIP_TRY(hipMemcpyAsync(got, c->d_shade_counts.p, 8, hipMemcpyDeviceToHost, st), "shade counts");
+  HIP_TRY(hipStreamSynchronize(st), "vc_shade");
+  c->shade_requests = std::min(got[0], cnq::VC_REQUEST_CAP);
+  if (counts) counts[0] = got[0], counts[1] = got[1];
+  return CNQ_OK;
+}
+
+int cnq_vcache_plan(cnq_vcache* c, uint32_t budget, uint64_t stats[4]) {
+  if (!c) return fail(CNQ_INVALID_ARG, "null argument");
+  HIP_TRY(hipSetDevice(c->scene->dev->ordinal), "hipSetDevice");
+  const uint32_t P = c->cfg.pages;
+  // The pages pixels used (vcache.rs:3485 `merge_used`).
+  std::vector<uint32_t> used(P);
+  HIP_TRY(hipMemcpy(used.data(), c->d_slot_used.p, 4 * (size_t)P, hipMemcpyDeviceToHost), "slot use");
+  for (uint32_t k = 0; k < P; k++) c->last_used[k] = std::max(c->last_used[k], used[k]);
+  std::vector<uint32_t> req(4 * (size_t)c->shade_requests);
+  if (!req.empty())
+    HIP_TRY(hipMemcpy(req.data(), c->d_requests.p, req.size() * 4, hipMemcpyDeviceToHost), "requests");
+  c->shade_requests = 0;
+  // vcache.rs:3740-3790: the non-resident pages by (priority, key), at most
+  // `budget` of them, in that order.
+  struct Order {
+    uint32_t priority, k0, k1, k2;
+    bool operator<(const Order& o) const {
+      if (priority != o.priority) return priority < o.priority;
+      if (k0 != o.k0) return k0 < o.k0;
+      if (k1 != o.k1) return k1 < o.k1;
+      return k2 < o.k2;
+    }
+  };
+  std::vector<Order> order;
+  for (size_t i = 0; i < req.size(); i += 4) {
+    const Key k{{req[i], req[i + 1], req[i + 2]}};
+    if (c->map.count(k)) continue;
+    order.push_back(Order{req[i + 3], req[i], req[i + 1], req[i + 2]});
+  }
+  const uint32_t cap = std::max<uint32_t>(budget, vc::PAGE_TEXELS);
+  if (order.size() > cap) {
+    std::nth_element(order.begin(), order.begin() + cap, order.end());
+    order.resize(cap);
+  }
+  std::sort(order.begin(), order.end());
+  uint64_t spent = 0, planned = 0, evicted = 0;
+  bool evicted_now = false;
+  for (const Order& o : order) {
+    const uint32_t k[3] = {o.k0, o.k1, o.k2};
+    const Key key{{k[0], k[1], k[2]}};
+    if (c->map.count(key)) continue;  // a fingerprint collision can list a page twice
+    const uint32_t m = k[0], chart = k[1] & vc::KEY_CHART_MASK, mip = k[1] >> vc::KEY_MIP_SHIFT;
+    if (m >= c->scene->n_objects || chart >= c->tables.charts.size() || mip > 30) continue;
+    const uint32_t pp[2] = {k[2] & 0xffffu, k[2] >> 16};
+    uint32_t g[2];
+    vc::grid(c->tables.charts[chart], c->scale[m], c->inv_base, mip, g);
+    uint32_t inside[2], listed[2];
+    bool empty = false;
+    for (int ax = 0; ax < 2; ax++) {
+      const uint32_t nn = g[ax] + 1;
+      inside[ax] = std::min(nn - std::min(pp[ax] * vc::PAGE, nn), vc::STRIDE);
+      if (inside[ax] == 0 || (uint64_t)pp[ax] * vc::PAGE >= g[ax]) empty = true;
+      listed[ax] = std::min(inside[ax] + 1, vc::STRIDE);
+    }
+    if (empty) continue;
+    const uint64_t cost = (uint64_t)inside[0] * inside[1] * c->n_lights;
+    if (spent + cost > budget && !c->plan_list.empty()) break;
+    if (c->free_slots.empty() && !evicted_now) {
+      // vcache.rs:3541-3557 `evict_now(2)`: pages unused for more than two
+      // frames, least recently used first, up to a frame's eviction count.
+      evicted_now = true;
+      std::vector<std::pair<uint32_t, uint32_t>> stale;
+      for (uint32_t sl = 0; sl < P; sl++)
+        if (c->slot_key[sl].k[0] != vc::EMPTY && c->last_used[sl] + 2 < c->frame) stale.push_back({c->last_used[sl], sl});
+      std::sort(stale.begin(), stale.end());
+      for (size_t q = 0; q < stale.size() && q < cnq::VC_EVICT_PER_FRAME; q++) {
+        c->evict_slot(stale[q].second);
+        evicted++;
+      }
+    }
+    if (c->free_slots.empty()) break;
+    const uint32_t slot = c->free_slots.back();
+    c->free_slots.pop_back();
+    const uint32_t entry = c->table.insert(k, slot);
+    if (entry == vc::EMPTY) {
+      c->free_slots.push_back(slot);
+      continue;
+    }
+    c->slot_extent = std::max(c->slot_extent, slot + 1);
+    c->map[key] = {slot, entry};
+    c->slot_key[slot] = key;
+    c->last_used[slot] = c->frame;
+    c->alloc_frame[slot] = c->frame;
+    const uint32_t index = (uint32_t)(c->plan_list.size() / 4);
+    c->plan_list.insert(c->plan_list.end(), {k[0], k[1], k[2], slot});
+    float b[8];
+    vc::page_bounds(c->tables, chart, c->scale[m], c->inv_base, mip, pp[0], pp[1], b);
+    b[3] = 0.0f;
+    b[7] = 0.0f;
+    c->plan_bounds.insert(c->plan_bounds.end(), b, b + 8);
+    for (uint32_t l = 0; l < c->n_lights; l++)
+      for (uint32_t y = 0; y < listed[1]; y++)
+        for (uint32_t x = 0; x < listed[0]; x++)
+          c->plan_jobs.insert(c->plan_jobs.end(), {index, (y * vc::STRIDE + x) | (l << 8)});
+    spent += cost;
+    planned++;
+  }
+  if (stats) {
+    stats[0] = req.size() / 4;
+    stats[1] = planned;
+    stats[2] = spent;
+    stats[3] = evicted;
+  }
   return CNQ_OK;
 }
 

@@ -652,6 +652,34 @@ class ViewCache:
             check(lib().cnq_vcache_read(self.handle, _ptr(k), len(k), _ptr(tex), _ptr(slots)))
         return tex, slots
 
+    @staticmethod
+    def shade_params(camera, pixels_per_texel: float = 2.0, refine_levels: int = 3, request_tile: int = 4):
+        """The shading pass's parameters for a camera uniform (pixel angle
+        2 tan(fov/2) / height, from its params)."""
+        cam = np.asarray(camera, np.float32).reshape(20)
+        angle = np.float32(np.float32(2.0) * cam[18]) / cam[17]
+        return _lib.VcacheShadeParams(pixels_per_texel, refine_levels, request_tile, float(angle))
+
+    def shade_device(self, camera, width: int, height: int, hits, out, out_page, params=None, stream=None):
+        """Shading from the cache for a camera pass's hits (device tensors):
+        out (lights, h, w) f32, out_page (h, w) u32. Returns (requests,
+        pixels without a page)."""
+        cam = np.ascontiguousarray(camera, np.float32).reshape(20)
+        p = params or self.shade_params(cam)
+        counts = np.zeros(2, np.uint32)
+        if stream is None:
+            import torch
+
+            stream = torch.cuda.current_stream()
+        check(lib().cnq_vcache_shade(self.handle, _ptr(cam), int(width), int(height), hits.data_ptr(), out.data_ptr(),
+                                     out_page.data_ptr(), C.byref(p), _stream_ptr(stream), _ptr(counts)))
+        return int(counts[0]), int(counts[1])
+
+    def plan(self, budget: int) -> dict:
+        st = np.zeros(4, np.uint64)
+        check(lib().cnq_vcache_plan(self.handle, int(budget), _ptr(st)))
+        return dict(requests=int(st[0]), planned=int(st[1]), cones=int(st[2]), evicted=int(st[3]))
+
     def state(self) -> dict:
         sizes = np.zeros(5, np.uint32)
         check(lib().cnq_vcache_state(self.handle, 0, 0, 0, 0, 0, 0, _ptr(sizes)))

@@ -455,6 +455,37 @@ int cnq_vcache_frame(cnq_vcache* cache, void* stream, uint64_t stats[10]);
  * texels (81 x (lights + 1) floats per key, texel-major) and slots[i]
  * (CNQ_NONE, texels zero, where absent or marked stale). */
 int cnq_vcache_read(cnq_vcache* cache, const uint32_t* keys, uint32_t n, float* texels, uint32_t* slots);
+/* Shading from the cache (WGSL :5937 `vc_shade`, the direct light's part).
+ * `hits` (device, 4 u32 per pixel as cnq_primary_device writes them) are a
+ * camera pass of `camera`; per pixel its surface point and chart
+ * coordinates pick the wanted mip (`vc_mip`: texels spanning
+ * `pixels_per_texel` pixels of angle `pixel_angle`), then the page shown
+ * (`vc_find_shown`: that mip's, else the finest coarser one resident), and
+ * per light `out[l * w * h + p]` (device) is the page's share times the
+ * unshadowed light at the pixel (or, without light_ratio, the cached value;
+ * the unshadowed light where no page is resident). `out_page[p]`: the
+ * shown page's mip (top bit: none found; CNQ_NONE: no surface). One pixel per
+ * `request_tile` x `request_tile` (a different one each frame) requests the
+ * page one to `refine_levels` mips finer than the one it found, or the
+ * coarsest missing one (`vc_request`). counts: [requests, pixels without a
+ * page]. */
+typedef struct {
+  float pixels_per_texel;  /* CacheOptions::pixels_per_texel (2) */
+  uint32_t refine_levels;  /* CacheOptions::refine_levels (3), 1..4 */
+  uint32_t request_tile;   /* CacheOptions::request_tile (4) */
+  float pixel_angle;       /* 2 tan(fov / 2) / height */
+} cnq_vcache_shade_params;
+int cnq_vcache_shade(cnq_vcache* cache, const float* camera, uint32_t width, uint32_t height, const uint32_t* hits,
+                     float* out, uint32_t* out_page, const cnq_vcache_shade_params* params, void* stream,
+                     uint32_t counts[2]);
+/* VirtualCache::plan (vcache.rs:3704-3810, 3928-4050) over the last shading
+ * pass's requests: the non-resident pages by (priority, key), at most
+ * `budget` cones of bake work (a page costs its texels inside the grid times
+ * the lights; the first page always fits), pages unused for more than two
+ * frames evicted least recently used first when no slot is free. The
+ * planned pages bake in the next cnq_vcache_frame. stats: [requests, pages
+ * planned, cones planned, pages evicted]. */
+int cnq_vcache_plan(cnq_vcache* cache, uint32_t budget, uint64_t stats[4]);
 /* Diagnostics (any pointer may be NULL): slot info (12 per slot), stale
  * marks, last use, the pool (81 x channels per slot), the device and host
  * page tables (4 per entry); sizes: [pages, channels, table entries, slots

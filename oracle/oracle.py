@@ -74,6 +74,8 @@ def lib() -> C.CDLL:
             "oracle_vc_surface": (None, [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P]),
             "oracle_vc_world_box": (None, [_P, C.c_uint32, _P, _P, _P]),
             "oracle_vc_page_bounds": (None, [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P]),
+            "oracle_vc_shade": (None, [_P, _P, C.c_uint32, C.c_uint32, _P, _P, _P, _P, _P, C.c_uint32, _P]),
+            "oracle_vc_plan": (None, [_P, _P, C.c_uint32, C.c_uint32, _P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -323,15 +325,14 @@ def vc_table_check(length: int = 1024, steps: int = 20_000) -> int:
 
 def shadow_region(m, l, s):
     out = np.zeros(6, np.float32)
-    ok = lib().oracle_vc_shadow_region(_p(np.ascontiguousarray(m, np.float32)), _p(np.ascontiguousarray(l, np.float32)),
-                                       _p(np.ascontiguousarray(s, np.float32)), _p(out))
+    m, l, s = (np.ascontiguousarray(x, np.float32) for x in (m, l, s))  # alive across the call
+    ok = lib().oracle_vc_shadow_region(_p(m), _p(l), _p(s), _p(out))
     return out if ok else None
 
 
 def hull_meets(p, l, m) -> bool:
-    return bool(lib().oracle_vc_hull_meets(_p(np.ascontiguousarray(p, np.float32)),
-                                           _p(np.ascontiguousarray(l, np.float32)),
-                                           _p(np.ascontiguousarray(m, np.float32))))
+    p, l, m = (np.ascontiguousarray(x, np.float32) for x in (p, l, m))  # alive across the call
+    return bool(lib().oracle_vc_hull_meets(_p(p), _p(l), _p(m)))
 
 
 def query_skip(flat, opts, point, normal, light, skip_obj=NONE, skip_tri=NONE):
@@ -339,10 +340,10 @@ def query_skip(flat, opts, point, normal, light, skip_obj=NONE, skip_tri=NONE):
     vis = np.zeros(1, np.float32)
     masks = np.zeros(2, np.uint64)
     xf = flat.xforms if len(flat.xforms) else np.zeros((1, 16), np.float32)
+    point, normal, light = (np.ascontiguousarray(x, np.float32) for x in (point, normal, light))
     lib().oracle_query_skip(_p(flat.tris), _p(flat.ranges), len(flat.ranges), _p(flat.blas), _p(flat.blas_items),
-                            _p(flat.blas_cov), _p(flat.nodes), _p(flat.items), _p(xf), _p(opts),
-                            _p(np.ascontiguousarray(point, np.float32)), _p(np.ascontiguousarray(normal, np.float32)),
-                            _p(np.ascontiguousarray(light, np.float32)), skip_obj, skip_tri, _p(vis), _p(masks))
+                            _p(flat.blas_cov), _p(flat.nodes), _p(flat.items), _p(xf), _p(opts), _p(point),
+                            _p(normal), _p(light), skip_obj, skip_tri, _p(vis), _p(masks))
     return float(vis[0]), int(masks[0]), int(masks[1])
 
 
@@ -453,6 +454,32 @@ class OracleViewCache:
         return dict(slot_info=info, slot_stale=stale, slot_used=used, pool=pool, table=table,
                     slot_extent=int(sizes[3]), frame=int(sizes[4]))
 
+    def shade(self, camera, width, height, hits, pixels_per_texel=2.0, refine_levels=3, request_tile=4,
+              pixel_angle=None):
+        """Shading from the cache: (out (lights, h, w), out_page (h, w),
+        requests (n, 4), counts [requests, pixels without a page])."""
+        cam = np.ascontiguousarray(camera, np.float32).reshape(20)
+        if pixel_angle is None:
+            pixel_angle = np.float32(np.float32(2.0) * cam[18]) / cam[17]
+        params = np.zeros(4, np.float32)
+        params[0], params[3] = pixels_per_texel, pixel_angle
+        params[1:3] = np.array([refine_levels, request_tile], np.uint32).view(np.float32)
+        h = np.ascontiguousarray(hits, np.uint32).reshape(-1, 4)
+        out = np.zeros((len(self.lights), height, width), np.float32)
+        page = np.zeros((height, width), np.uint32)
+        cap = 1 << 16
+        req = np.zeros((cap, 4), np.uint32)
+        counts = np.zeros(2, np.uint32)
+        lib().oracle_vc_shade(self.handle, _p(cam), width, height, _p(h), _p(params), _p(out), _p(page), _p(req), cap,
+                              _p(counts))
+        return out, page, req[: min(int(counts[0]), cap)].copy(), counts
+
+    def plan(self, requests, budget):
+        r = np.ascontiguousarray(requests, np.uint32).reshape(-1, 4)
+        st = np.zeros(4, np.uint64)
+        lib().oracle_vc_plan(self.handle, _p(r), len(r), budget, _p(st))
+        return dict(requests=int(st[0]), planned=int(st[1]), cones=int(st[2]), evicted=int(st[3]))
+
     def surface(self, obj, chart, mip, tx, ty):
         out = np.zeros(8, np.float32)
         lib().oracle_vc_surface(self.handle, obj, chart, mip, tx, ty, _p(out))
@@ -460,8 +487,8 @@ class OracleViewCache:
 
     def world_box(self, obj, lo, hi):
         out = np.zeros(6, np.float32)
-        lib().oracle_vc_world_box(self.handle, obj, _p(np.ascontiguousarray(lo, np.float32)),
-                                  _p(np.ascontiguousarray(hi, np.float32)), _p(out))
+        lo, hi = np.ascontiguousarray(lo, np.float32), np.ascontiguousarray(hi, np.float32)
+        lib().oracle_vc_world_box(self.handle, obj, _p(lo), _p(hi), _p(out))
         return out
 
     def page_bounds(self, obj, chart, mip, px, py):

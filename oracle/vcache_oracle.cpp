@@ -22,6 +22,7 @@
 // false wherever it is asked and no page is deferred), area texels, wide
 // windows' ray pass, absent lighting levels.
 #include <algorithm>
+#include <array>
 #include <cfloat>
 #include <cmath>
 #include <cstdint>
@@ -131,6 +132,7 @@ struct State {
   uint32_t opts[8];
   // surfaces
   std::vector<float> tri_uv, normals;
+  std::vector<uint32_t> tri_chart;
   std::vector<float> chart_size;
   std::vector<uint32_t> chart_grid, first_cell, cells, cell_tris;
   std::vector<float> cell_bounds;
@@ -722,6 +724,110 @@ static void vc_fill(State& s, uint32_t p) {
   if (p < plan_pages) s.slot_used[slot] = s.frame;
 }
 
+// ---- shading from the cache (WGSL :5907-6130, the direct light's part)
+
+static uint32_t find_shown_slot(const State& s, const uint32_t* k) {
+  uint32_t slot = s.table.find(k);
+  if (slot != NONE && s.slot_stale[slot] != 0 && s.slot_stale[slot] != 3) slot = NONE;
+  return slot;
+}
+
+// WGSL :3187-3196
+static uint32_t top_mip(const State& s, uint32_t c, uint32_t m) {
+  float ratio = orc::fmax_(s.chart_size[2 * c], s.chart_size[2 * c + 1]) * object_scale(s, m) * s.inv_base;
+  uint32_t top = 0;
+  if (ratio > 1.0f) {
+    float l = std::ceil(std::log2(ratio));
+    top = l < 30.0f ? (uint32_t)l : 30u;
+  }
+  uint32_t n[2];
+  if (top > 0) {
+    grid_of(s, c, m, top - 1, n);
+    if (n[0] == 1 && n[1] == 1) top -= 1;
+  }
+  grid_of(s, c, m, top, n);
+  if (n[0] != 1 || n[1] != 1) top += 1;
+  return top;
+}
+
+struct PageRef {
+  uint32_t key[3], base[2];
+  float frac[2];
+  bool coarsest;
+};
+
+// WGSL :3218-3231
+static PageRef page_ref(const State& s, uint32_t m, uint32_t c, float u, float v, uint32_t mip) {
+  uint32_t n[2];
+  grid_of(s, c, m, mip, n);
+  float tx = orc::clampf(u * (float)n[0] - 0.5f, 0.0f, (float)(n[0] - 1)), ty = orc::clampf(v * (float)n[1] - 0.5f, 0.0f, (float)(n[1] - 1));
+  uint32_t bx = (uint32_t)std::floor(tx), by = (uint32_t)std::floor(ty);
+  uint32_t px = bx / PAGE, py = by / PAGE;
+  PageRef r;
+  r.key[0] = m, r.key[1] = c | (mip << MIP_SHIFT), r.key[2] = px | (py << 16);
+  r.base[0] = bx - px * PAGE, r.base[1] = by - py * PAGE;
+  r.frac[0] = tx - std::floor(tx), r.frac[1] = ty - std::floor(ty);
+  r.coarsest = n[0] == 1 && n[1] == 1;
+  return r;
+}
+
+// WGSL :3242-3247
+static float bilinear(const State& s, uint32_t slot, const PageRef& r, uint32_t ch) {
+  size_t o = (size_t)slot * TEXELS + r.base[1] * STRIDE + r.base[0], C = s.channels;
+  float a = orc::mixf(s.pool[o * C + ch], s.pool[(o + 1) * C + ch], r.frac[0]);
+  float b = orc::mixf(s.pool[(o + STRIDE) * C + ch], s.pool[(o + STRIDE + 1) * C + ch], r.frac[0]);
+  return orc::mixf(a, b, r.frac[1]);
+}
+
+static V3 point_to_object(const State& s, uint32_t m, V3 p) {
+  uint32_t x = s.ranges[4 * m + 3];
+  if (x == NONE) return p;
+  const float* xf = s.xforms + 16 * x;
+  V3 d = p - V3{xf[3], xf[7], xf[11]};
+  return (ld(xf) * d.x + ld(xf + 4) * d.y + ld(xf + 8) * d.z) * xf[13];
+}
+
+static V3 normal_world(const State& s, uint32_t m, V3 n) {
+  uint32_t x = s.ranges[4 * m + 3];
+  return x == NONE ? n : rot_world(s.xforms + 16 * x, n);
+}
+
+struct Request {
+  uint32_t k[3], priority;
+};
+
+// WGSL :5907-5925 on the CPU: a set of fingerprints, first come first
+// listed; a page several pixels ask for keeps the smallest priority.
+struct RequestSet {
+  std::vector<uint32_t> set, prio;
+  std::vector<Request> list;
+  std::vector<uint32_t> entry;  // per listed request, its set entry
+  void request(const uint32_t* k, uint32_t priority) {
+    uint32_t mask = (uint32_t)set.size() - 1;
+    uint32_t fp = hash3(k) | 1u;
+    uint32_t k2[3] = {k[2], k[0], k[1]};
+    uint32_t i = hash3(k2) & mask;
+    for (uint32_t probe = 0; probe < 32; probe++) {
+      if (set[i] == 0) {
+        set[i] = fp;
+        prio[i] = std::min(prio[i], priority);
+        list.push_back(Request{{k[0], k[1], k[2]}, 0});
+        entry.push_back(i);
+        return;
+      }
+      if (set[i] == fp) {
+        prio[i] = std::min(prio[i], priority);
+        return;
+      }
+      i = (i + 1) & mask;
+    }
+  }
+};
+
+}  // namespace vco
+
+namespace vco {
+
 static void evict_slot(State& s, uint32_t slot) {
   auto it = s.map.find(s.slot_key[slot]);
   if (it == s.map.end()) return;
@@ -750,6 +856,7 @@ void* oracle_vc_create(const float* tris, uint32_t n_tris, const uint32_t* range
   s->blas_cov = blas_cov, s->nodes = nodes, s->items = items, s->xforms = xforms, s->n_tris = n_tris;
   std::memcpy(s->opts, opts, sizeof(s->opts));
   s->tri_uv.assign(tri_uv, tri_uv + 6 * (size_t)n_tris);
+  s->tri_chart.assign(tri_chart, tri_chart + n_tris);
   s->normals.assign(normals, normals + 9 * (size_t)n_tris);
   s->n_charts = n_charts;
   s->chart_size.assign(chart_size, chart_size + 2 * (size_t)n_charts);
@@ -1029,6 +1136,202 @@ void oracle_vc_world_box(void* h, uint32_t m, const float* lo, const float* hi, 
 
 void oracle_vc_page_bounds(void* h, uint32_t m, uint32_t c, uint32_t mip, uint32_t px, uint32_t py, float* out) {
   page_bounds(*static_cast<State*>(h), m, c, mip, px, py, out);
+}
+
+
+// Shading from the cache for one view's hits (4 u32 per pixel), as
+// cnq_vcache_shade: out (lights x pixels), out_page, requests (4 u32 each,
+// in pixel order), counts [requests, pixels without a page].
+void oracle_vc_shade(void* h, const float* cam, uint32_t w, uint32_t hgt, const uint32_t* hits, const float* params,
+                     float* out, uint32_t* out_page, uint32_t* requests, uint32_t request_cap, uint32_t* counts) {
+  State& s = *static_cast<State*>(h);
+  float ppt = params[0], angle = params[3];
+  uint32_t refine, tile;
+  std::memcpy(&refine, &params[1], 4);
+  std::memcpy(&tile, &params[2], 4);
+  RequestSet rs;
+  rs.set.assign(1u << 18, 0u);
+  rs.prio.assign(1u << 18, 0xffffffffu);
+  uint32_t fell = 0;
+  const uint32_t npx = w * hgt;
+  const uint32_t sample = (s.frame * 7u) % (tile * tile);
+  for (uint32_t p = 0; p < npx; p++) {
+    uint32_t x = p % w, y = p / w;
+    bool is_sample = x % tile == sample % tile && y % tile == sample / tile;
+    uint32_t req[3] = {0, 0, 0}, priority = 0;
+    const float* pp = cam + 16;
+    float aspect = pp[0] / pp[1];
+    float sx = (((float)x + 0.5f + cam[11]) / pp[0] * 2.0f - 1.0f) * pp[2] * aspect;
+    float sy = (1.0f - ((float)y + 0.5f + cam[15]) / pp[1] * 2.0f) * pp[2];
+    V3 dir = orc::normalize(ld(cam + 4) + ld(cam + 8) * sx + ld(cam + 12) * sy);
+    const uint32_t* v = hits + 4 * (size_t)p;
+    uint32_t m = v[0], t = v[1];
+    if (m == NONE || v[3] != 0) {
+      for (uint32_t l = 0; l < s.n_lights; l++) out[(size_t)l * npx + p] = 0.0f;
+      out_page[p] = NONE;
+      continue;
+    }
+    float distance;
+    std::memcpy(&distance, &v[2], 4);
+    const float* tr = s.tris + 12 * t;
+    V3 v0 = ld(tr), face = orc::cross(ld(tr + 4) - v0, ld(tr + 8) - v0);
+    V3 hn = orc::normalize(face);
+    if (s.ranges[4 * m + 3] != NONE) hn = orc::normalize(normal_world(s, m, hn));
+    if (orc::dot(hn, dir) > 0.0f) hn = -hn;
+    V3 surface = ld(cam) + dir * distance;
+    uint32_t c = s.tri_chart[t];
+    // WGSL :1780-1793 barycentric, :2339 chart_uv
+    V3 po = point_to_object(s, m, surface);
+    V3 e1 = ld(tr + 4) - v0, e2 = ld(tr + 8) - v0, d = po - v0;
+    float d11 = orc::dot(e1, e1), d12 = orc::dot(e1, e2), d22 = orc::dot(e2, e2);
+    float denom = d11 * d22 - d12 * d12;
+    float bs = (d22 * orc::dot(d, e1) - d12 * orc::dot(d, e2)) / denom;
+    float bu = (d11 * orc::dot(d, e2) - d12 * orc::dot(d, e1)) / denom;
+    float b[3] = {1.0f - bs - bu, bs, bu};
+    const float* q = &s.tri_uv[6 * t];
+    float u = q[0] * b[0] + q[2] * b[1] + q[4] * b[2], vv = q[1] * b[0] + q[3] * b[1] + q[5] * b[2];
+    // WGSL :3198-3202 vc_mip
+    float ratio = distance * angle * ppt * s.inv_base;
+    uint32_t want = 0;
+    if (ratio > 1.0f) {
+      float lr = std::nearbyint(std::log2(ratio));
+      want = lr < 30.0f ? (uint32_t)lr : 30u;
+    }
+    uint32_t top = top_mip(s, c, m), wanted = std::min(want, top);
+    uint32_t mip = wanted, slot = NONE, finer[4][3] = {};
+    PageRef r;
+    bool found = false;
+    for (;;) {
+      r = page_ref(s, m, c, u, vv, mip);
+      slot = find_shown_slot(s, r.key);
+      if (slot != NONE) {
+        s.slot_used[slot] = s.frame;
+        found = true;
+        if (mip > wanted) {
+          const uint32_t* pick = finer[0];
+          if (refine >= 2 && mip >= wanted + 2) pick = finer[1];
+          if (refine >= 3 && mip >= wanted + 3) pick = finer[2];
+          if (refine >= 4 && mip >= wanted + 4) pick = finer[3];
+          req[0] = pick[0], req[1] = pick[1], req[2] = pick[2];
+          priority = 32u - std::min(mip - wanted, 30u);
+        }
+        break;
+      }
+      for (int k = 0; k < 3; k++) {
+        finer[3][k] = finer[2][k], finer[2][k] = finer[1][k], finer[1][k] = finer[0][k], finer[0][k] = r.key[k];
+      }
+      if (r.coarsest || mip >= 30) break;
+      mip++;
+    }
+    if (!found) {
+      req[0] = finer[0][0], req[1] = finer[0][1], req[2] = finer[0][2];
+      priority = 1;
+      fell++;
+    }
+    V3 point = surface + hn * surface_offset(distance);
+    const float* nr = &s.normals[9 * t];
+    V3 sn = ld(nr) * b[0] + ld(nr + 3) * b[1] + ld(nr + 6) * b[2];
+    V3 normal = orc::normalize(normal_world(s, m, orc::dot(sn, sn) > 1e-20f ? sn : face));
+    if (orc::dot(normal_world(s, m, face), -dir) < 0.0f) normal = -normal;
+    float pt[3] = {point.x, point.y, point.z}, nm[3] = {normal.x, normal.y, normal.z};
+    for (uint32_t l = 0; l < s.n_lights; l++) {
+      float e;
+      if (found && !s.light_ratio) {
+        e = bilinear(s, slot, r, l);
+      } else {
+        float share = found ? bilinear(s, slot, r, l) : 1.0f;
+        e = share > 0.0f ? share * oracle_unshadowed_rect(pt, nm, &s.lights[16 * l]) : 0.0f;
+      }
+      out[(size_t)l * npx + p] = e;
+    }
+    out_page[p] = found ? mip : (0x80000000u | mip);
+    if (is_sample && priority > 0) rs.request(req, priority);
+  }
+  uint32_t n = (uint32_t)std::min<size_t>(rs.list.size(), request_cap);
+  for (uint32_t i = 0; i < n; i++) {
+    requests[4 * i] = rs.list[i].k[0], requests[4 * i + 1] = rs.list[i].k[1], requests[4 * i + 2] = rs.list[i].k[2];
+    requests[4 * i + 3] = rs.prio[rs.entry[i]];
+  }
+  counts[0] = (uint32_t)rs.list.size();
+  counts[1] = fell;
+}
+
+// VirtualCache::plan without sky, bounce or lighting levels (vcache.rs:
+// 3704-3810, 3928-4050, `evict_now` :3541 with age 2), as cnq_vcache_plan,
+// over `n` requests (4 u32 each). stats: [requests, planned, cones, evicted].
+void oracle_vc_plan(void* h, const uint32_t* requests, uint32_t n, uint32_t budget, uint64_t* stats) {
+  State& s = *static_cast<State*>(h);
+  for (uint32_t k = 0; k < s.pages; k++) s.last_used[k] = std::max(s.last_used[k], s.slot_used[k]);
+  std::vector<std::array<uint32_t, 4>> order;
+  for (uint32_t i = 0; i < n; i++) {
+    const uint32_t* r = requests + 4 * i;
+    if (s.map.count(std::vector<uint32_t>(r, r + 3))) continue;
+    order.push_back({r[3], r[0], r[1], r[2]});
+  }
+  uint32_t cap = std::max(budget, TEXELS);
+  if (order.size() > cap) {
+    std::nth_element(order.begin(), order.begin() + cap, order.end());
+    order.resize(cap);
+  }
+  std::sort(order.begin(), order.end());
+  uint64_t spent = 0, planned = 0, evicted = 0;
+  bool evicted_now = false;
+  for (auto& o : order) {
+    uint32_t k[3] = {o[1], o[2], o[3]};
+    std::vector<uint32_t> key(k, k + 3);
+    if (s.map.count(key)) continue;
+    uint32_t m = k[0], c = k[1] & CHART_MASK, mip = k[1] >> MIP_SHIFT, pg[2] = {k[2] & 0xffffu, k[2] >> 16};
+    if (m >= s.n_objects || c >= s.n_charts || mip > 30) continue;
+    uint32_t g[2], inside[2], listed[2];
+    grid_of(s, c, m, mip, g);
+    bool empty = false;
+    for (int a = 0; a < 2; a++) {
+      uint32_t nn = g[a] + 1;
+      inside[a] = std::min(nn - std::min(pg[a] * PAGE, nn), STRIDE);
+      if (inside[a] == 0 || (uint64_t)pg[a] * PAGE >= g[a]) empty = true;
+      listed[a] = std::min(inside[a] + 1, STRIDE);
+    }
+    if (empty) continue;
+    uint64_t cost = (uint64_t)inside[0] * inside[1] * s.n_lights;
+    if (spent + cost > budget && !s.plan_list.empty()) break;
+    if (s.free_slots.empty() && !evicted_now) {
+      evicted_now = true;
+      std::vector<std::pair<uint32_t, uint32_t>> stale;
+      for (uint32_t sl = 0; sl < s.pages; sl++)
+        if (!s.slot_key[sl].empty() && s.last_used[sl] + 2 < s.frame) stale.push_back({s.last_used[sl], sl});
+      std::sort(stale.begin(), stale.end());
+      for (size_t q = 0; q < stale.size() && q < 2048; q++) {
+        evict_slot(s, stale[q].second);
+        evicted++;
+      }
+      for (uint32_t sl : s.cleared) std::fill(&s.slot_info[12 * sl], &s.slot_info[12 * sl + 12], 0u);
+      s.cleared.clear();
+    }
+    if (s.free_slots.empty()) break;
+    uint32_t slot = s.free_slots.back();
+    s.free_slots.pop_back();
+    uint32_t entry = s.table.insert(k, slot);
+    if (entry == NONE) {
+      s.free_slots.push_back(slot);
+      continue;
+    }
+    s.slot_extent = std::max(s.slot_extent, slot + 1);
+    s.map[key] = {slot, entry};
+    s.slot_key[slot] = key;
+    s.last_used[slot] = s.frame;
+    s.alloc_frame[slot] = s.frame;
+    uint32_t index = (uint32_t)(s.plan_list.size() / 4);
+    s.plan_list.insert(s.plan_list.end(), {k[0], k[1], k[2], slot});
+    float b[8];
+    page_bounds(s, m, c, mip, pg[0], pg[1], b);
+    s.plan_bounds.insert(s.plan_bounds.end(), b, b + 8);
+    for (uint32_t l = 0; l < s.n_lights; l++)
+      for (uint32_t y = 0; y < listed[1]; y++)
+        for (uint32_t x = 0; x < listed[0]; x++) s.plan_jobs.insert(s.plan_jobs.end(), {index, (y * STRIDE + x) | (l << 8)});
+    spent += cost;
+    planned++;
+  }
+  stats[0] = n, stats[1] = planned, stats[2] = spent, stats[3] = evicted;
 }
 
 }  // extern "C"

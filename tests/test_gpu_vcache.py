@@ -193,3 +193,39 @@ def test_request_and_evict_churn_matches_the_oracle(device):
     ta, la = vc.read(keys)
     tb, lb = ov.read(keys)
     assert np.array_equal(la, lb) and np.array_equal(ta.view(np.uint32), tb.view(np.uint32))
+
+
+def test_shading_and_planning_match_the_oracle(device):
+    """Shading from the cache (WGSL :5937 vc_shade, the direct light's part)
+    and planning (vcache.rs:3704 plan) over ten frames of one view: per
+    pixel and light the shaded irradiance bit for bit, the page shown, the
+    requests (as a set), the pages planned and every slot and texel after
+    each frame's bake."""
+    import torch
+
+    from continuum_amd.scenes import camera_uniform
+
+    w = _setup(device, pages=4096, append_pages=256, candidates=1024, base_texel=0.02)
+    vc, ov = w["vc"], w["ov"]
+    W, H = 64, 48
+    cam = camera_uniform([-7.0, -6.0, 2.5], [0.0, 0.0, 0.3], 60.0, W, H)
+    hits, _, _, _ = w["scene"].render_view(cam, W, H, w["lights"], w["opts"])
+    oh, _, _ = O.primary(w["f"], cam, W, H, w["lights"], hierarchy=2)
+    assert np.array_equal(hits, oh)
+    hits_d = torch.from_numpy(np.ascontiguousarray(hits).view(np.int32)).to("cuda")
+    out_d = torch.zeros((len(w["lights"]), H, W), dtype=torch.float32, device="cuda")
+    page_d = torch.zeros((H, W), dtype=torch.int32, device="cuda")
+    for frame in range(10):
+        print(f"shade frame {frame}", flush=True)
+        counts = vc.shade_device(cam, W, H, hits_d, out_d, page_d)
+        torch.cuda.synchronize()
+        oout, opage, oreq, ocnt = ov.shade(cam, W, H, hits)
+        got = out_d.cpu().numpy()
+        assert np.array_equal(got.view(np.uint32), oout.view(np.uint32)), f"frame {frame}: shading differs"
+        assert np.array_equal(page_d.cpu().numpy().view(np.uint32), opage)
+        assert counts == (int(ocnt[0]), int(ocnt[1]))
+        st_a, st_b = vc.plan(4000), ov.plan(oreq, 4000)
+        assert st_a == st_b, (frame, st_a, st_b)
+        assert vc.frame() == ov.frame()
+        _same_state(vc, ov, f"shade frame {frame}")
+    assert counts[1] == 0  # every pixel found a page

@@ -281,3 +281,70 @@ def test_moves_keep_every_shown_page_current(touch, visible_frames, append_pages
         # pages the moves reach out of view, or past the append capacity, are dropped
         assert st["evicted"] > 0 and (~shown).sum() == st["evicted"]
         assert changed[~shown].sum() > 0 or append_pages < len(keys)
+
+
+# ---- shading from the cache and planning (WGSL :5907-6130; vcache.rs:3704)
+
+def _view(W=64, H=48):
+    from continuum_amd.scenes import camera_uniform
+    tris9, counts, pl, surf, lights, movable = vcache_world()
+    f = Flat.build_instanced(tris9, counts, pl, object_prefilter=True, movable=movable)
+    opts = O.opts_array(hierarchy=2, prefilter=1.0, object_cov=f.object_cov)
+    cam = camera_uniform([-7.0, -6.0, 2.5], [0.0, 0.0, 0.3], 60.0, W, H)
+    hits, recv, _ = O.primary(f, cam, W, H, lights, hierarchy=2)
+    return f, surf, lights, opts, cam, hits, recv
+
+
+def test_shading_falls_back_to_unshadowed_light_then_converges():
+    f, surf, lights, opts, cam, hits, recv = _view()
+    W, H = 64, 48
+    vc = O.OracleViewCache(f, surf, lights, opts, pages=4096, append_pages=256, candidates=1024, base_texel=0.02)
+    surface = hits[:, 0] != NONE
+    out, page, req, cnt = vc.shade(cam, W, H, hits)
+    # no page yet: every surface pixel takes the unshadowed light at its point
+    # (checked where shading normals are the face's: the ground and the boxes,
+    # objects 0-4)
+    assert cnt[1] == surface.sum() and np.all((page.reshape(-1)[surface] >> 31) == 1)
+    flat = surface & (hits[:, 0] <= 4)
+    assert flat.sum() > 100
+    for i in np.nonzero(flat)[0][::37]:
+        pt, nm = recv[i, 0:3].copy(), recv[i, 4:7].copy()  # kept alive across the calls
+        for l in range(len(lights)):
+            open_ = float(O.lib().oracle_unshadowed_rect(O._p(pt), O._p(nm), O._p(lights[l])))
+            assert abs(out[l].reshape(-1)[i] - open_) <= 1e-5 * max(1.0, abs(open_)) + 1e-6
+    # requests ask for the coarsest missing page, once each
+    assert len(req) == cnt[0] > 0 and len({tuple(r[:3]) for r in req}) == len(req)
+    assert np.all(req[:, 3] == 1)
+    last = np.full(W * H, 99)
+    budget = 4000
+    for frame in range(12):
+        st = vc.plan(req, budget)
+        assert st["cones"] <= budget or st["planned"] == 1
+        vc.frame()
+        out, page, req, cnt = vc.shade(cam, W, H, hits)
+        shown = page.reshape(-1)
+        found = surface & ((shown >> 31) == 0)
+        # coarse to fine: a pixel's page once found never gets coarser
+        mip = np.where(found, shown & 0xFF, 99)
+        assert np.all(mip[found] <= last[found])
+        last = np.where(found, mip, last)
+    assert cnt[1] == 0                       # every pixel has a page
+    assert len(req) <= 4                     # little left to refine
+
+
+def test_plan_evicts_the_least_recently_used():
+    f, surf, lights, opts, cam, hits, recv = _view()
+    W, H = 64, 48
+    vc = O.OracleViewCache(f, surf, lights, opts, pages=40, append_pages=16, candidates=64, base_texel=0.02)
+    for frame in range(8):
+        out, page, req, cnt = vc.shade(cam, W, H, hits)
+        st = vc.plan(req, 100000)
+        vc.frame()
+    s = vc.state()
+    resident = (s["slot_info"][:, 3] != 0).sum()
+    assert resident <= 40
+    # pages in use this view stay; evictions (when any) took pages unused for
+    # more than two frames
+    out, page, req, cnt = vc.shade(cam, W, H, hits)
+    used = s["slot_used"]
+    assert np.all(used[s["slot_info"][:, 3] != 0] > 0)

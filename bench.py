@@ -375,7 +375,74 @@ def view_cache_form(device, boulders: int = 512, moves: int = 5) -> dict:
                            rebaked_cones=st["appended_jobs"], evicted=st["evicted"], overflow=st["overflow"]))
     out["moves"] = {"movers": len(movers), "shift_m": 0.25, "frames": frames,
                     "ms_mean": float(np.mean([f["ms"] for f in frames]))}
+    out["view"] = view_from_cache(device, scene, surf, vlights, opts, eye)
     return out
+
+
+def view_from_cache(device, scene, surf, lights, opts, eye, width: int = 1280, height: int = 720,
+                    budget: int = 2_000_000, max_frames: int = 40) -> dict:
+    """A 1280x720 view shaded from a fresh view cache (the reference's
+    renderer with a cache): each frame shades from the pages resident, plans
+    the pages its sample pixels asked for (at most `budget` cones) and bakes
+    them. Sample pixels move through each request tile frame by frame, so
+    finer pages keep arriving for a while after every pixel has one; pages
+    off their chart's grid are asked for and never made. Reported: the frame
+    from which every pixel reads a resident page, the frames, the shading
+    pass at the end, and the plain per-view launch (camera pass and a cone
+    per pixel and light) for the same view and lights."""
+    import numpy as np
+    import torch
+
+    from continuum_amd.kernel import ViewCache
+    from continuum_amd.scenes import camera_uniform
+
+    dev = torch.device("cuda:0")
+    cam = camera_uniform(eye, [eye[0] + 10.0, eye[1] + 10.0, 0.4], 60.0, width, height)
+    px = width * height
+    hits = torch.zeros((px, 4), dtype=torch.int32, device=dev)
+    recv = torch.zeros((px, 8), dtype=torch.float32, device=dev)
+    lt = torch.from_numpy(np.ascontiguousarray(lights)).to(dev)
+    scene.primary_device(torch.from_numpy(cam).to(dev), width, height, lt, opts, hits, recv)
+    torch.cuda.synchronize()
+    vc = ViewCache(scene, surf, lights, opts, pages=1 << 17, append_pages=256, candidates=4096, base_texel=0.05)
+    out = torch.zeros((len(lights), height, width), dtype=torch.float32, device=dev)
+    page = torch.zeros((height, width), dtype=torch.int32, device=dev)
+    frames, settled = [], None
+    for k in range(max_frames):
+        t0 = time.perf_counter()
+        req, fell = vc.shade_device(cam, width, height, hits, out, page)
+        t1 = time.perf_counter()
+        st = vc.plan(budget)
+        tp = time.perf_counter()
+        vc.frame()
+        t2 = time.perf_counter()
+        frames.append(dict(frame=k, shade_ms=(t1 - t0) * 1e3, plan_ms=(tp - t1) * 1e3, bake_ms=(t2 - tp) * 1e3,
+                           requests=req, pixels_without_page=fell,
+                           cones=st["cones"], pages=st["planned"]))
+        if fell == 0 and settled is None:
+            settled = k
+    shade = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        vc.shade_device(cam, width, height, hits, out, page)
+        shade.append((time.perf_counter() - t0) * 1e3)
+    vis = torch.zeros(len(lights) * px, dtype=torch.float32, device=dev)
+    full = torch.zeros(len(lights) * px, dtype=torch.int64, device=dev)
+    touched = torch.zeros(len(lights) * px, dtype=torch.int64, device=dev)
+    plain = []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        scene.primary_device(torch.from_numpy(cam).to(dev), width, height, lt, opts, hits, recv)
+        scene.view_query_device(recv, width, height, lt, opts, vis, full, touched, tile_w=PLAIN_TILE_W)
+        torch.cuda.synchronize()
+        plain.append((time.perf_counter() - t0) * 1e3)
+    return {"width": width, "height": height, "budget_cones_per_frame": budget, "every_pixel_from_frame": settled,
+            "frames": frames[:8] + frames[8::8] + ([frames[-1]] if (len(frames) - 9) % 8 else []),
+            "cones_baked": int(sum(f["cones"] for f in frames)),
+            "bake_ms_total": float(sum(f["bake_ms"] for f in frames)),
+            "resident_pages": vc.state()["slot_extent"], "shade_ms_settled": float(np.median(shade)),
+            "plain_view_ms": float(np.median(plain))}
 
 
 def main() -> int:

@@ -379,6 +379,58 @@ def view_cache_form(device, boulders: int = 512, moves: int = 5) -> dict:
     return out
 
 
+def sky_form(scene, flat, sc, recv, h_recv, check: bool = True, stride: int = 64) -> dict:
+    """A uniform sky over the rock field's shading points (its 1280x720
+    camera pass): six cube-face cones per point (the reference's
+    `sky_irradiance`, the sky term of its per-pixel shading), faces summed
+    in order. Timed with HIP events on the launch stream; every `stride`th
+    point checked against the oracle bit for bit."""
+    import numpy as np
+    import torch
+
+    from continuum_amd import _lib
+
+    dev = recv.device
+    px = recv.shape[0]
+    out = torch.zeros(px, dtype=torch.float32, device=dev)
+    faces = torch.zeros(6 * px, dtype=torch.float32, device=dev)
+    full = torch.zeros(6 * px, dtype=torch.int64, device=dev)
+    opts = _lib.default_options(hierarchy=2, prefilter=sc.prefilter)
+    stream = torch.cuda.current_stream()
+    scene.sky_device(recv, opts, out, faces, full, stream=stream)
+    torch.cuda.synchronize()
+    ms = []
+    for _ in range(5):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        scene.sky_device(recv, opts, out, faces, full, stream=stream)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms.append(e0.elapsed_time(e1))
+    t = float(np.median(ms))
+    lit = int((h_recv[:, 7] >= 0).sum())
+    res = {"workload": f"uniform sky over the rock field's {sc.width}x{sc.height} shading points, six cube-face "
+                       "cones each (independent occluders, horizon cull, 1-bin prefilter)",
+           "points": px, "traced_points": lit, "face_cones": 6 * px, "ms": t, "face_cones_per_s": 6 * px / t * 1e3,
+           "mean_sky_irradiance": float(out.mean().item())}
+    if check:
+        from oracle import oracle as O
+
+        sample = h_recv[::stride]
+        t0 = time.perf_counter()
+        rv, rf, rfull = O.sky(flat, sample, O.opts_array(hierarchy=2, prefilter=sc.prefilter))
+        t_cpu = time.perf_counter() - t0
+        gv = out.cpu().numpy()[::stride]
+        gf = faces.cpu().numpy().reshape(6, px)[:, ::stride]
+        gfull = full.cpu().numpy().view(np.uint64).reshape(6, px)[:, ::stride]
+        res["parity"] = {"sample_points": len(sample),
+                         "sum_bits_differ": int((gv.view(np.uint32) != rv.view(np.uint32)).sum()),
+                         "face_bits_differ": int((gf.view(np.uint32) != rf.view(np.uint32)).sum()),
+                         "full_masks_differ": int((gfull != rfull).sum())}
+        res["cpu_oracle_face_cones_per_s"] = 6 * len(sample) / t_cpu
+    return res
+
+
 def view_from_cache(device, scene, surf, lights, opts, eye, width: int = 1280, height: int = 720,
                     budget: int = 2_000_000, max_frames: int = 40) -> dict:
     """A 1280x720 view shaded from a fresh view cache (the reference's
@@ -458,6 +510,7 @@ def main() -> int:
     ap.add_argument("--no-forms", action="store_true", help="skip the courtyard's plain/batched forms")
     ap.add_argument("--no-served", action="store_true", help="skip the served form over the boulder field")
     ap.add_argument("--no-vcache", action="store_true", help="skip the view cache over the boulder field")
+    ap.add_argument("--no-sky", action="store_true", help="skip the sky light over the rock field")
     ap.add_argument("--only-form", choices=["courtyard", "served", "view_cache"],
                     help="run one form alone and print its JSON (for profiling it); no headline line")
     ap.add_argument("--served-requests", type=int, default=64, help="requests in the saturated served run")
@@ -650,9 +703,11 @@ def main() -> int:
             "camera_hits_differ": int((hits.cpu().numpy().view(np.uint32) != rh).any(axis=1).sum()),
         }
 
+    if not args.no_sky:
+        result.setdefault("forms", {})["sky"] = sky_form(scene, flat, sc, recv, h_recv, check=not args.no_cpu_baseline)
     if not args.no_forms:
         del scene
-        result["forms"] = courtyard_forms(device, steps=5)
+        result.setdefault("forms", {}).update(courtyard_forms(device, steps=5))
     if not args.no_served:
         result.setdefault("forms", {})["served"] = served_form(device, n_requests=args.served_requests,
                                                                seconds=args.served_seconds)

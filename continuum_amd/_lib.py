@@ -33,6 +33,14 @@ class Options(C.Structure):
     ]
 
 
+class SkyOptions(C.Structure):
+    _fields_ = [
+        ("max_combine", C.c_uint32),
+        ("horizon_cull", C.c_uint32),
+        ("prefilter_scale", C.c_float),
+    ]
+
+
 class SceneDesc(C.Structure):
     _fields_ = [
         ("tris", C.c_void_p), ("n_tris", C.c_uint64),
@@ -128,6 +136,10 @@ SIGNATURES = {
     "cnq_irradiance": (C.c_int, [_P, _P, C.c_uint64, _P, C.c_uint32, C.POINTER(Options), _P, _P, _P, _P]),
     "cnq_irradiance_device": (
         C.c_int, [_P, _P, C.c_uint64, _P, C.c_uint32, C.POINTER(Options), _P, _P, _P, _P, _P]),
+    "cnq_sky_options_default": (None, [C.POINTER(SkyOptions)]),
+    "cnq_sky_irradiance": (C.c_int, [_P, _P, C.c_uint64, C.POINTER(Options), C.POINTER(SkyOptions), _P, _P, _P]),
+    "cnq_sky_irradiance_device": (
+        C.c_int, [_P, _P, C.c_uint64, C.POINTER(Options), C.POINTER(SkyOptions), _P, _P, _P, _P]),
     "cnq_view_irradiance_device": (
         C.c_int, [_P, _P, C.c_uint32, C.c_uint32, _P, C.c_uint32, C.POINTER(Options), C.c_uint32, _P, _P, _P, _P]),
     "cnq_tuning_default": (None, [C.POINTER(Tuning)]),
@@ -209,6 +221,14 @@ def lib() -> C.CDLL:
 def check(rc: int) -> None:
     if rc != OK:
         raise CnqError(rc, lib().cnq_last_error().decode(errors="replace"))
+
+
+def default_sky_options(**kw) -> SkyOptions:
+    o = SkyOptions()
+    lib().cnq_sky_options_default(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
 
 
 def default_options(**kw) -> Options:

@@ -39,7 +39,7 @@ HIP_FLAGS = [
 CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wextra"]
 
 HIP_SOURCES = ["cnq_api.hip", "cnq_cone.hip", "cnq_primary.hip", "cnq_refit.hip", "cnq_cache.hip", "cnq_server.hip",
-               "cnq_vcache.hip"]
+               "cnq_vcache.hip", "cnq_sky.hip"]
 CXX_SOURCES = ["bvh_build.cpp", "cache_host.cpp", "vcache_host.cpp"]
 ORACLE_SOURCES = ["cone_oracle.cpp", "bvh_oracle.cpp", "cone3_oracle.cpp", "vcache_oracle.cpp"]
 

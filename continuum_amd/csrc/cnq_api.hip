@@ -554,6 +554,79 @@ int cnq_irradiance(cnq_scene* s, const float* receivers, uint64_t n, const float
   return rc;
 }
 
+void cnq_sky_options_default(cnq_sky_options* sky) {
+  if (!sky) return;
+  sky->max_combine = 0;
+  sky->horizon_cull = 1;
+  sky->prefilter_scale = 1.0f;
+}
+
+int cnq_sky_irradiance_device(cnq_scene* s, const float* receivers, uint64_t n, const cnq_options* o,
+                              const cnq_sky_options* sky, float* out, float* out_faces, uint64_t* out_full,
+                              void* stream) {
+  if (!s) return fail(CNQ_INVALID_ARG, "scene is null");
+  int rc = check_options(o);
+  if (rc) return rc;
+  if ((rc = check_object_cov(s, o))) return rc;
+  if (n == 0) return CNQ_OK;
+  if (!receivers || !out) return fail(CNQ_INVALID_ARG, "null buffer");
+  if (n > 0xffffffffull || 6 * ((n + 63) / 64) > 0x7fffffffull) return fail(CNQ_CAPACITY, "too many receivers for one call");
+  cnq_sky_options so;
+  cnq_sky_options_default(&so);
+  if (sky) so = *sky;
+  HIP_TRY(hipSetDevice(s->dev->ordinal), "hipSetDevice");
+  if (!out_faces) {
+    if (!s->sky_faces.p || s->sky_faces.n < 6 * n) {
+      s->sky_faces.release();
+      HIP_TRY(s->sky_faces.alloc(6 * n), "sky face buffer");
+    }
+    out_faces = s->sky_faces.p;
+  }
+  cnq::SkyArgs a;
+  a.s = s->args();
+  a.receivers = receivers;
+  a.out = out;
+  a.out_faces = out_faces;
+  a.out_full = out_full;
+  a.work = s->work.p;
+  a.n = (uint32_t)n;
+  a.p = query_params(s, o, 64);
+  a.sky = cnq::SkyParams{so.max_combine, so.horizon_cull, so.prefilter_scale};
+  HIP_TRY(cnq::launch_sky(a, s->count_work, (hipStream_t)stream), "sky launch");
+  return CNQ_OK;
+}
+
+int cnq_sky_irradiance(cnq_scene* s, const float* receivers, uint64_t n, const cnq_options* o,
+                       const cnq_sky_options* sky, float* out, float* out_faces, uint64_t* out_full) {
+  if (!s) return fail(CNQ_INVALID_ARG, "scene is null");
+  int rc = check_options(o);
+  if (rc) return rc;
+  if (n == 0) return CNQ_OK;
+  if (!receivers || !out) return fail(CNQ_INVALID_ARG, "null buffer");
+  HIP_TRY(hipSetDevice(s->dev->ordinal), "hipSetDevice");
+  DevBuf<float> r, v, fa;
+  DevBuf<uint64_t> fu;
+  auto cleanup = [&]() { r.release(), v.release(), fa.release(), fu.release(); };
+  hipError_t e = r.upload(receivers, 8 * n);
+  if (e == hipSuccess) e = v.alloc(n);
+  if (e == hipSuccess) e = fa.alloc(6 * n);
+  if (e == hipSuccess && out_full) e = fu.alloc(6 * n);
+  if (e != hipSuccess) {
+    cleanup();
+    return hip_fail(e, "sky buffers");
+  }
+  rc = cnq_sky_irradiance_device(s, r.p, n, o, sky, v.p, fa.p, out_full ? fu.p : nullptr, nullptr);
+  if (rc == CNQ_OK) {
+    e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(out, v.p, 4 * n, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && out_faces) e = hipMemcpy(out_faces, fa.p, 24 * n, hipMemcpyDeviceToHost);
+    if (e == hipSuccess && out_full) e = hipMemcpy(out_full, fu.p, 48 * n, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = hip_fail(e, "sky results");
+  }
+  cleanup();
+  return rc;
+}
+
 int cnq_primary_device(cnq_scene* s, const float* camera, uint32_t width, uint32_t height, const float* lights,
                        uint32_t n_lights, const cnq_options* o, uint32_t* hits, float* receivers, void* stream) {
   if (!s) return fail(CNQ_INVALID_ARG, "scene is null");

@@ -125,6 +125,9 @@ struct Cone {
   float far_offset;
   f3 baxis;  // bounding circular cone: unit axis, cos and sin of the half-angle
   float bcos, bsin;
+  // The receiver's horizon (WGSL :147): read only by sky walks (`SKY`),
+  // whose cones cull geometry wholly below it; light cones have none.
+  f3 horizon;
 };
 
 __device__ __forceinline__ f3 direction(const Cone& c, float s, float t) {
@@ -201,17 +204,20 @@ __device__ __forceinline__ f3 box_corner(f3 lo, f3 hi, uint32_t k) {
   return f3{(k & 1u) ? hi.x : lo.x, (k & 2u) ? hi.y : lo.y, (k & 4u) ? hi.z : lo.z};
 }
 
-// WGSL :234-255 (no horizon: light cones have none)
+// WGSL :234-255; the horizon test only with SKY (light cones have none)
+template <bool SKY = false>
 __device__ __forceinline__ bool may_touch_box(const Cone& c, f3 lo, f3 hi) {
   // The bounding-cone test first (a cheap bound settles most boxes; both
   // tests must pass, so their order does not change the answer).
   const int quick = in_bound_quick(c, lo, hi);
   if (quick == 0) return false;
   bool far_out = true, near_out = true, out_u0 = true, out_u1 = true, out_v0 = true, out_v1 = true;
+  bool below = SKY;
 #pragma unroll
   for (uint32_t k = 0; k < 8; k++) {
     f3 corner = box_corner(lo, hi, k);
     if (c.far_offset - dot3(c.forward, corner) >= 0.0f) far_out = false;
+    if (SKY && dot3(c.horizon, corner - c.apex) >= 0.0f) below = false;
     f3 p = to_local(c, corner);
     if (p.z >= NEAR) near_out = false;
     if (p.x - c.w0 * p.z >= 0.0f) out_u0 = false;
@@ -219,16 +225,20 @@ __device__ __forceinline__ bool may_touch_box(const Cone& c, f3 lo, f3 hi) {
     if (p.y - c.w1 * p.z >= 0.0f) out_v0 = false;
     if (c.w3 * p.z - p.y >= 0.0f) out_v1 = false;
   }
-  if (far_out || near_out || out_u0 || out_u1 || out_v0 || out_v1) return false;
+  if (far_out || near_out || out_u0 || out_u1 || out_v0 || out_v1 || below) return false;
   return quick == 1 || in_bound(c, (lo + hi) * 0.5f, 0.5f * len3(hi - lo));
 }
 
-// WGSL :213-232
+// WGSL :213-232 (the horizon test only with SKY)
+template <bool SKY = false>
 __device__ __forceinline__ bool may_touch(const Cone& c, f3 v0, f3 v1, f3 v2) {
   float f0 = c.far_offset - dot3(c.forward, v0);
   float f1 = c.far_offset - dot3(c.forward, v1);
   float f2 = c.far_offset - dot3(c.forward, v2);
   if (f0 < 0.0f && f1 < 0.0f && f2 < 0.0f) return false;
+  if (SKY && dot3(c.horizon, v0 - c.apex) < 0.0f && dot3(c.horizon, v1 - c.apex) < 0.0f &&
+      dot3(c.horizon, v2 - c.apex) < 0.0f)
+    return false;
   f3 p0 = to_local(c, v0), p1 = to_local(c, v1), p2 = to_local(c, v2);
   if (p0.z < NEAR && p1.z < NEAR && p2.z < NEAR) return false;
   if (p0.x - c.w0 * p0.z < 0.0f && p1.x - c.w0 * p1.z < 0.0f && p2.x - c.w0 * p2.z < 0.0f) return false;

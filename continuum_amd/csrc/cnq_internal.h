@@ -68,6 +68,7 @@ struct cnq_scene {
   cnq::detail::DevBuf<float> tris, tri_leaf, blas_cov, xforms, mesh_box;
   cnq::detail::DevBuf<uint32_t> blas_items, ranges, items, parent, leaves, arrivals, counters;
   cnq::detail::DevBuf<uint64_t> work;
+  cnq::detail::DevBuf<float> sky_faces;  // cnq_sky_irradiance_device's per-face values when the caller has none
   cnq::detail::DevBuf<cnq::NodeC> blas, nodes;
   std::vector<uint32_t> h_ranges;  // for host-side checks and object rebuilds
   // Moving objects (instanced scenes): the subtree's layout, each geometry's
@@ -101,7 +102,7 @@ struct cnq_scene {
   void release() {
     tris.release(), tri_leaf.release(), blas_cov.release(), xforms.release(), mesh_box.release();
     blas_items.release(), ranges.release(), items.release(), parent.release(), leaves.release();
-    arrivals.release(), counters.release(), work.release(), blas.release(), nodes.release();
+    arrivals.release(), counters.release(), work.release(), sky_faces.release(), blas.release(), nodes.release();
     for (auto& kv : pose_leaves) kv.second.release();
     pose_leaves.clear();
   }

@@ -158,7 +158,32 @@ struct CacheArgs {
   uint32_t n_blocks, n_slots;
 };
 
+// The sky walk's options: the reference's params.w bits 1 (`sky_max`: meshes
+// combine by the maximum, a diagnostic) and 2 (`sky_horizon_cull` off: walk
+// geometry below the receiver's horizon too), and its sky prefilter scale
+// (light.objects.y, `sky_prefilter`).
+struct SkyParams {
+  uint32_t max_combine, horizon_cull;
+  float prefilter_scale;
+};
+
+// The sky over a list of receivers: work items are (face, tile of 64
+// receivers), face-major; face f of receiver i goes to out_faces[f * n + i]
+// (and its full bins to out_full, when given), then out[i] sums the six.
+struct SkyArgs {
+  SceneArgs s;
+  const float* __restrict__ receivers;
+  float* __restrict__ out;
+  float* __restrict__ out_faces;
+  uint64_t* __restrict__ out_full;
+  uint64_t* __restrict__ work;
+  uint32_t n;
+  QueryParams p;
+  SkyParams sky;
+};
+
 hipError_t launch_cone_items(const ItemArgs& a, bool count_work, uint32_t persistent_blocks, hipStream_t stream);
+hipError_t launch_sky(const SkyArgs& a, bool count_work, hipStream_t stream);
 hipError_t launch_primary(const PrimaryArgs& a, hipStream_t stream);
 hipError_t launch_refit(const RefitArgs& a, hipStream_t stream);
 hipError_t launch_bake_static(const CacheArgs& a, hipStream_t stream);

@@ -353,6 +353,9 @@ struct Walk {
   uint32_t cone_work;
   uint32_t cone_cap;
   float prefilter_bins;
+  // Sky walks (SKY) only: combine by the maximum with nearest-first order
+  // (params.w bit 1, a diagnostic) instead of as independent occluders.
+  bool sky_max;
   bool overflow;   // deeper than the reference's 48 entries
   bool truncated;  // deeper than this stack's 64 (the walk stopped)
   uint32_t work[6];  // object nodes tested, blas nodes tested, triangles tested, triangles drawn,
@@ -385,6 +388,7 @@ __device__ __forceinline__ Cone cone_to_object(const Cone& c, const float* __res
   o.baxis = rot(c.baxis);
   o.bcos = c.bcos;
   o.bsin = c.bsin;
+  o.horizon = rot(c.horizon);
   return o;
 }
 
@@ -481,11 +485,13 @@ __device__ __forceinline__ bool draw_hull(Lane& L, float alpha) {
 }
 
 // WGSL :519-574: 0 culled, 1 splatted with nothing drawn, 2 splatted and drawn, 3 descend.
+template <bool SKY = false>
 __device__ __forceinline__ uint32_t node_cone_test(const Cone& c, Lane& L, float prefilter_bins, f3 lo, f3 hi,
                                                    const float* __restrict__ cov27) {
   const float g = (float)GRID;
   f2 mnv{3.4e38f, 3.4e38f}, mxv{-3.4e38f, -3.4e38f};
   bool far_out = true, near_out = true, out_u0 = true, out_u1 = true, out_v0 = true, out_v1 = true;
+  bool below = SKY;
   bool inside = prefilter_bins > 0.0f;
   // The bounding-cone test first (see may_touch_box).
   const int quick = in_bound_quick(c, lo, hi);
@@ -493,6 +499,7 @@ __device__ __forceinline__ uint32_t node_cone_test(const Cone& c, Lane& L, float
 #pragma unroll
   for (uint32_t k = 0; k < 8; k++) {
     f3 corner = box_corner(lo, hi, k);
+    if (SKY && dot3(c.horizon, corner - c.apex) >= 0.0f) below = false;
     bool in_far = c.far_offset - dot3(c.forward, corner) >= 0.0f;
     if (in_far)
       far_out = false;
@@ -508,7 +515,7 @@ __device__ __forceinline__ uint32_t node_cone_test(const Cone& c, Lane& L, float
     if (p.y - c.w1 * p.z >= 0.0f) out_v0 = false;
     if (c.w3 * p.z - p.y >= 0.0f) out_v1 = false;
   }
-  if (far_out || near_out || out_u0 || out_u1 || out_v0 || out_v1) return 0;
+  if (far_out || near_out || out_u0 || out_u1 || out_v0 || out_v1 || below) return 0;
   if (quick == 2 && !in_bound(c, (lo + hi) * 0.5f, 0.5f * len3(hi - lo))) return 0;
   if (!inside) return 3;
   // Most nodes that get here project far wider than the prefilter's bins.
@@ -565,6 +572,7 @@ __device__ __forceinline__ uint32_t node_cone_test(const Cone& c, Lane& L, float
 }
 
 // WGSL :594-616 (opaque triangles)
+template <bool SKY = false>
 __device__ __forceinline__ bool cover_triangle(const Cone& c, Lane& L, f3 v0, f3 v1, f3 v2, float weight,
                                                uint32_t stage, uint32_t& drawn_count) {
   if (stage == 4) return false;
@@ -573,12 +581,15 @@ __device__ __forceinline__ bool cover_triangle(const Cone& c, Lane& L, f3 v0, f3
   if (dot3(n, direction(c, 0.0f, 0.0f)) >= 0.0f && dot3(n, direction(c, 1.0f, 0.0f)) >= 0.0f &&
       dot3(n, direction(c, 0.0f, 1.0f)) >= 0.0f && dot3(n, direction(c, 1.0f, 1.0f)) >= 0.0f)
     return false;
-  if (!may_touch(c, v0, v1, v2)) return false;
+  if (!may_touch<SKY>(c, v0, v1, v2)) return false;
   drawn_count++;
   return rasterize(c, L, v0, v1, v2, weight);
 }
 
-// Adds one mesh's accumulation buffer to the coverage (WGSL :809-828).
+// Adds one mesh's accumulation buffer to the coverage (WGSL :809-828): by
+// the maximum, or with SKY (unless `sky_max`) as independent occluders,
+// 1 - (1 - before)(1 - covered) (WGSL :820-821, :1141-1142).
+template <bool SKY = false>
 __device__ __forceinline__ void combine(Walk& w, Lane& L, bool snap) {
   float running = 0.0f;
   const uint32_t end = L.span_hi < BINS ? L.span_hi : BINS;
@@ -593,7 +604,10 @@ __device__ __forceinline__ void combine(Walk& w, Lane& L, bool snap) {
       }
       if (covered > 1e-6f) {
         const float before = w.cov[b];
-        w.cov[b] = mx(before, mn(covered, 1.0f));
+        if (SKY && !w.sky_max)
+          w.cov[b] = 1.0f - (1.0f - before) * (1.0f - mn(covered, 1.0f));
+        else
+          w.cov[b] = mx(before, mn(covered, 1.0f));
         if (before < 1.0f && w.cov[b] >= 1.0f) w.full++;
       }
     }
@@ -608,7 +622,7 @@ __device__ __forceinline__ void clear_acc(Lane& L) {
 }
 
 // WGSL :719-829 for mesh `m` (uniform), for the lanes with `on`.
-template <bool COUNT, bool SKIP = false>
+template <bool COUNT, bool SKIP = false, bool SKY = false>
 __device__ __forceinline__ void add_mesh_coverage(const Ctx& x, const Cone& wc, uint32_t m, bool on, Walk& w,
                                                   Stack& st, uint32_t base, Lane& L, Skip skip = Skip{NONE, NONE}) {
   if (x.profile_stage == 3) return;
@@ -644,7 +658,7 @@ __device__ __forceinline__ void add_mesh_coverage(const Ctx& x, const Cone& wc, 
         bool descend = false;
         if (mine) {
           if (COUNT) w.work[1]++;
-          const uint32_t test = node_cone_test(c, L, w.prefilter_bins, f3{node.lo[0], node.lo[1], node.lo[2]},
+          const uint32_t test = node_cone_test<SKY>(c, L, w.prefilter_bins, f3{node.lo[0], node.lo[1], node.lo[2]},
                                                f3{node.hi[0], node.hi[1], node.hi[2]}, x.blas_cov + 27 * (size_t)n);
           if (test == 2) touched = true;
           if (COUNT && (test == 1 || test == 2)) w.work[5]++;
@@ -660,7 +674,7 @@ __device__ __forceinline__ void add_mesh_coverage(const Ctx& x, const Cone& wc, 
             const f3 v0{t[0], t[1], t[2]}, v1{t[3], t[4], t[5]}, v2{t[6], t[7], t[8]};
             if (descend && (!SKIP || x.blas_items[k] != cover_skip)) {
               if (COUNT) w.work[2]++;
-              if (cover_triangle(c, L, v0, v1, v2, weight, x.profile_stage, w.work[3])) touched = true;
+              if (cover_triangle<SKY>(c, L, v0, v1, v2, weight, x.profile_stage, w.work[3])) touched = true;
             }
           }
         } else {
@@ -680,14 +694,16 @@ __device__ __forceinline__ void add_mesh_coverage(const Ctx& x, const Cone& wc, 
       const float* tr = x.tris + 12 * (size_t)t;
       if (SKIP && t == cover_skip) continue;
       if (COUNT) w.work[2]++;
-      if (cover_triangle(c, L, ld3(tr), ld3(tr + 4), ld3(tr + 8), weight, x.profile_stage, w.work[3])) touched = true;
+      if (cover_triangle<SKY>(c, L, ld3(tr), ld3(tr + 4), ld3(tr + 8), weight, x.profile_stage, w.work[3]))
+        touched = true;
     }
   }
-  if (touched) combine(w, L, true);
+  if (touched) combine<SKY>(w, L, true);
 }
 
 // WGSL :1096-1150 (object-level splat): true when the node was splatted
 // (coverage combined if anything was drawn); false to descend.
+template <bool SKY = false>
 __device__ __forceinline__ bool splat_object_node(const Ctx& x, const Cone& c, Walk& w, uint32_t n, f3 lo, f3 hi,
                                                   Lane& L) {
   if (x.object_cov == NONE || w.prefilter_bins <= 0.0f) return false;
@@ -716,12 +732,14 @@ __device__ __forceinline__ bool splat_object_node(const Ctx& x, const Cone& c, W
   float alpha = 1.0f - exp_c(-node_front_area(x.blas_cov + 27 * (size_t)(x.object_cov + n), v) / silhouette);
   if (alpha <= 0.0f) return true;
   clear_acc(L);
-  if (draw_hull(L, alpha)) combine(w, L, false);
+  if (draw_hull(L, alpha)) combine<SKY>(w, L, false);
   return true;
 }
 
-// WGSL :1160-1227 for the lanes with `traced`.
-template <bool COUNT, bool SKIP = false>
+// WGSL :1160-1227 for the lanes with `traced`. Sky walks (SKY, unless
+// `sky_max`) keep the children's fixed order, as their products would round
+// differently in another.
+template <bool COUNT, bool SKIP = false, bool SKY = false>
 __device__ __forceinline__ void scene_coverage(const Ctx& x, const Cone& c, bool traced, Walk& w, Lane& L,
                                                Skip skip = Skip{NONE, NONE}) {
   w.cone_work = 0;
@@ -741,8 +759,8 @@ __device__ __forceinline__ void scene_coverage(const Ctx& x, const Cone& c, bool
       bool descend = false;
       if (mine) {
         if (COUNT) w.work[0]++;
-        if (may_touch_box(c, lo, hi)) {
-          const bool splatted = (node.b & NODE_NO_SPLAT) == 0 && splat_object_node(x, c, w, n, lo, hi, L);
+        if (may_touch_box<SKY>(c, lo, hi)) {
+          const bool splatted = (node.b & NODE_NO_SPLAT) == 0 && splat_object_node<SKY>(x, c, w, n, lo, hi, L);
           descend = !splatted;
         }
       }
@@ -750,21 +768,27 @@ __device__ __forceinline__ void scene_coverage(const Ctx& x, const Cone& c, bool
       if (!dm) continue;
       const uint32_t count = node.b & ~NODE_NO_SPLAT;
       if (count > 0) {
-        for (uint32_t k = node.a; k < node.a + count; k++) add_mesh_coverage<COUNT, SKIP>(x, c, x.items[k], descend, w, st, sp, L, skip);
+        for (uint32_t k = node.a; k < node.a + count; k++)
+          add_mesh_coverage<COUNT, SKIP, SKY>(x, c, x.items[k], descend, w, st, sp, L, skip);
       } else {
         uint32_t nearc = node.a, farc = node.a + 1;
-        const NodeC na = x.nodes[nearc];
-        const NodeC nb = x.nodes[farc];
-        const bool use_bound = c.bcos >= -1.0f && (c.baxis.x != 0.0f || c.baxis.y != 0.0f || c.baxis.z != 0.0f);
-        // (componentwise selects keep the cone in registers)
-        const f3 axis{use_bound ? c.baxis.x : c.forward.x, use_bound ? c.baxis.y : c.forward.y,
-                      use_bound ? c.baxis.z : c.forward.z};
-        const float da = dot3(axis, (f3{na.lo[0], na.lo[1], na.lo[2]} + f3{na.hi[0], na.hi[1], na.hi[2]}) * 0.5f - c.apex);
-        const float db = dot3(axis, (f3{nb.lo[0], nb.lo[1], nb.lo[2]} + f3{nb.hi[0], nb.hi[1], nb.hi[2]}) * 0.5f - c.apex);
-        // packet_first: the first lane's order for the whole packet.
-        if (uni(db < da ? 1u : 0u)) {
-          nearc = node.a + 1;
-          farc = node.a;
+        if (!SKY || uni(w.sky_max ? 1u : 0u)) {
+          const NodeC na = x.nodes[nearc];
+          const NodeC nb = x.nodes[farc];
+          const bool use_bound =
+              c.bcos >= -1.0f && (c.baxis.x != 0.0f || c.baxis.y != 0.0f || c.baxis.z != 0.0f);
+          // (componentwise selects keep the cone in registers)
+          const f3 axis{use_bound ? c.baxis.x : c.forward.x, use_bound ? c.baxis.y : c.forward.y,
+                        use_bound ? c.baxis.z : c.forward.z};
+          const float da =
+              dot3(axis, (f3{na.lo[0], na.lo[1], na.lo[2]} + f3{na.hi[0], na.hi[1], na.hi[2]}) * 0.5f - c.apex);
+          const float db =
+              dot3(axis, (f3{nb.lo[0], nb.lo[1], nb.lo[2]} + f3{nb.hi[0], nb.hi[1], nb.hi[2]}) * 0.5f - c.apex);
+          // packet_first: the first lane's order for the whole packet.
+          if (uni(db < da ? 1u : 0u)) {
+            nearc = node.a + 1;
+            farc = node.a;
+          }
         }
         if (sp + 2 > REF_STACK) w.overflow = w.overflow || descend;
         if (sp + 2 > STACK_LIMIT) {
@@ -777,7 +801,8 @@ __device__ __forceinline__ void scene_coverage(const Ctx& x, const Cone& c, bool
       }
     }
   } else {
-    for (uint32_t m = 0; m < x.n_objects; m++) add_mesh_coverage<COUNT, SKIP>(x, c, m, traced && w.full < BINS, w, st, 0, L, skip);
+    for (uint32_t m = 0; m < x.n_objects; m++)
+      add_mesh_coverage<COUNT, SKIP, SKY>(x, c, m, traced && w.full < BINS, w, st, 0, L, skip);
   }
 }
 
@@ -882,6 +907,95 @@ __device__ __forceinline__ QueryOut evaluate_query(const Ctx& x, const QueryPara
                                                    const float* __restrict__ rc, const LightRect& r,
                                                    float* __restrict__ cov_out, uint64_t* __restrict__ work_out) {
   return evaluate_at<COUNT>(x, p, L, valid, ld3(rc), ld3(rc + 4), valid && rc[7] >= 0.0f, r, cov_out, work_out);
+}
+
+// ---- Sky light: a uniform sky through six cube-face mask cones (WGSL
+// :1323-1510), faces on the world axes, no skipped object.
+
+// WGSL :1326: far enough that every scene lies inside a sky cone.
+constexpr float SKY_FAR = 1e7f;
+
+// WGSL :1444-1478: cube face `f` (axis f / 2, negative when f is odd) around
+// `apex`; `above`: any of its directions lies above the horizon of `normal`.
+__device__ __forceinline__ Cone sky_face(uint32_t f, f3 apex, f3 normal, bool horizon_cull, bool& above) {
+  const uint32_t axis = f >> 1;
+  const float sign = (f & 1u) ? -1.0f : 1.0f;
+  const f3 forward{axis == 0 ? sign : 0.0f, axis == 1 ? sign : 0.0f, axis == 2 ? sign : 0.0f};
+  const uint32_t ra = axis == 2 ? 0u : axis + 1u;
+  const f3 right{ra == 0 ? 1.0f : 0.0f, ra == 1 ? 1.0f : 0.0f, ra == 2 ? 1.0f : 0.0f};
+  Cone c = Cone{};
+  c.apex = apex;
+  c.right = right;
+  c.up = cross3(forward, right);
+  c.forward = forward;
+  c.w0 = -1.0f;
+  c.w1 = -1.0f;
+  c.w2 = 1.0f;
+  c.w3 = 1.0f;
+  c.far_offset = dot3(forward, apex) + SKY_FAR;
+  c.horizon = horizon_cull ? normal : f3{0.0f, 0.0f, 0.0f};
+  bound_cone(c, forward);
+  above = false;
+#pragma unroll
+  for (uint32_t k = 0; k < 4; k++)
+    if (dot3(normal, direction(c, (float)(k & 1u), (float)(k >> 1u))) > 0.0f) above = true;
+  return c;
+}
+
+struct SkyOut {
+  float value;    // irradiance per unit sky radiance through the face
+  uint64_t full;  // bins with coverage 1
+};
+
+// WGSL :1505-1511 (`sky_face_irradiance`): face `f` at `point` for the lanes
+// with `active` (every lane of the wave takes part in the walk).
+template <bool COUNT>
+__device__ __forceinline__ SkyOut evaluate_sky_face(const Ctx& x, const QueryParams& p, const SkyParams& sp,
+                                                    Lane& L, bool valid, f3 point, f3 normal, bool active,
+                                                    uint32_t f, uint64_t* __restrict__ work_out) {
+  Walk w;
+  float cov[BINS];
+  w.cov = cov;
+#pragma unroll
+  for (uint32_t b = 0; b < BINS; b++) w.cov[b] = 0.0f;
+  w.full = 0;
+  w.cone_work = 0;
+  w.cone_cap = 0xffffffffu;  // sky walks have no cap (WGSL :1059)
+  w.overflow = false;
+  w.truncated = false;
+  w.sky_max = sp.max_combine != 0;
+  w.prefilter_bins = p.prefilter * (sp.prefilter_scale > 0.0f ? sp.prefilter_scale : 1.0f);
+#pragma unroll
+  for (int k = 0; k < 6; k++) w.work[k] = 0;
+  SkyOut o{0.0f, 0ull};
+  if (p.profile_stage == 1) return o;
+  bool above;
+  const Cone c = sky_face(f, point, normal, sp.horizon_cull != 0, above);
+  above = above && active;
+  if (p.profile_stage != 2) scene_coverage<COUNT, false, true>(x, c, above, w, L);
+#pragma unroll
+  for (uint32_t b = 0; b < BINS; b++) o.full |= (uint64_t)(w.cov[b] >= 1.0f) << b;
+  // WGSL :1492-1501 (`sky_face_light`)
+  if (above) {
+    float blocked = 0.0f;
+    for (uint32_t b = 0; b < BINS; b++) {
+      const float covered = w.cov[b];
+      if (covered > 0.0f) blocked += covered * clipped_bin_weight(L, c, normal, b);
+    }
+    bool all_above;
+    o.value = mx(window_weight4(L, c, normal, all_above) - blocked, 0.0f);
+  }
+  if (COUNT && work_out) {
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      uint32_t v = valid ? w.work[k] : 0u;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) v += (uint32_t)__shfl_xor((int)v, off, 64);
+      if (lane == 0) atomicAdd(reinterpret_cast<unsigned long long*>(&work_out[k]), (unsigned long long)v);
+    }
+  }
+  return o;
 }
 
 }  // namespace cnq

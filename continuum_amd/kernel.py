@@ -274,6 +274,36 @@ class Scene:
                                           C.byref(opts), vis.data_ptr(), full.data_ptr(), touched.data_ptr(),
                                           cov.data_ptr() if cov is not None else 0, stream.cuda_stream))
 
+    def sky(self, receivers: np.ndarray, opts: _lib.Options, sky: Optional[_lib.SkyOptions] = None):
+        """The sky's irradiance per unit radiance at each receiver through six
+        cube-face cones (WGSL :1411 sky_irradiance), host buffers: (total
+        (n,), per face (6, n), full bins per face (6, n))."""
+        receivers = np.ascontiguousarray(receivers, dtype=np.float32).reshape(-1, 8)
+        n = len(receivers)
+        out = np.zeros(n, np.float32)
+        faces = np.zeros((6, n), np.float32)
+        full = np.zeros((6, n), np.uint64)
+        so = sky if sky is not None else _lib.default_sky_options()
+        check(lib().cnq_sky_irradiance(self.handle, _ptr(receivers), n, C.byref(opts), C.byref(so), _ptr(out),
+                                       _ptr(faces), _ptr(full)))
+        return out, faces, full
+
+    def sky_device(self, receivers, opts, out, faces=None, full=None, sky=None, stream=None) -> None:
+        """The same on device tensors, enqueued on `stream` (the current
+        stream by default): out (n,), faces (6 * n) and full (6 * n) optional."""
+        import torch
+
+        if stream is None:
+            stream = torch.cuda.current_stream()
+        n = receivers.numel() // 8
+        if out.numel() < n or (faces is not None and faces.numel() < 6 * n) or (
+                full is not None and full.numel() < 6 * n):
+            raise ValueError("output tensor too small")
+        so = sky if sky is not None else _lib.default_sky_options()
+        check(lib().cnq_sky_irradiance_device(self.handle, receivers.data_ptr(), n, C.byref(opts), C.byref(so),
+                                              out.data_ptr(), faces.data_ptr() if faces is not None else 0,
+                                              full.data_ptr() if full is not None else 0, stream.cuda_stream))
+
     def view_query_device(self, receivers, width, height, lights, opts, vis, full, touched, tile_w=16,
                           stream=None) -> None:
         """Cone queries over a view's receivers (device tensors), each wave
@@ -721,6 +751,9 @@ class MaskConeKernel:
         self._wide = True
         self._nearest_first = True
         self._profile_stage = 0
+        self._sky_max = False
+        self._sky_prefilter = 1.0
+        self._sky_horizon_cull = True
 
     @classmethod
     def new(cls, ordinal: int = 0) -> Optional["MaskConeKernel"]:
@@ -799,6 +832,29 @@ class MaskConeKernel:
         best = time.perf_counter() - t0
         lightmap.irradiance = vis
         return best
+
+    def set_sky_max(self, on: bool) -> None:
+        """Sky cones combine meshes by the maximum (a diagnostic, lib.rs:103)."""
+        self._sky_max = bool(on)
+
+    def set_sky_prefilter(self, scale: float) -> None:
+        """The sky cones' prefilter as a multiple of the lights' (lib.rs:104)."""
+        self._sky_prefilter = float(scale)
+
+    def set_sky_horizon_cull(self, on: bool) -> None:
+        """Sky walks skip geometry wholly below the receiver's horizon (lib.rs:105)."""
+        self._sky_horizon_cull = bool(on)
+
+    def sky_options(self) -> _lib.SkyOptions:
+        return _lib.default_sky_options(max_combine=int(self._sky_max), horizon_cull=int(self._sky_horizon_cull),
+                                        prefilter_scale=self._sky_prefilter)
+
+    def sky_irradiance(self, receivers, meshes, hierarchy=Hierarchy.TRIANGLES) -> np.ndarray:
+        """Per receiver, the irradiance per unit radiance of a uniform sky
+        through six cube-face mask cones (WGSL :1411 `sky_irradiance`, the
+        sky term of `per_pixel_hit_color` :2282-2289)."""
+        out, _, _ = self.scene(meshes).sky(receivers_array(receivers), self.options(hierarchy), self.sky_options())
+        return out
 
     def coverage(self, receivers, light, meshes) -> np.ndarray:
         """Per receiver, the 64 bins' coverage (lib.rs:3289; every triangle tested)."""

@@ -184,6 +184,33 @@ int cnq_irradiance_device(cnq_scene* scene, const float* receivers, uint64_t n, 
                           uint32_t n_lights, const cnq_options* opts, float* out_visible, uint64_t* out_full,
                           uint64_t* out_touched, float* out_cov, void* stream);
 
+/* ---- The sky light: a uniform sky through six cube-face mask cones
+ * (mask_cones.wgsl :1323-1510, `MaskConeKernel::set_sky` lib.rs:1520). */
+typedef struct cnq_sky_options {
+  uint32_t max_combine;  /* params.w bit 1 (`sky_max`, lib.rs:103): meshes combine by the maximum (a
+                            diagnostic); 0: as independent occluders, 1 - (1 - a)(1 - b) (WGSL :820) */
+  uint32_t horizon_cull; /* `sky_horizon_cull` (lib.rs:105): walks skip geometry wholly below the receiver's
+                            horizon (default 1) */
+  float prefilter_scale; /* `sky_prefilter` (lib.rs:104): the sky cones' prefilter is cnq_options.prefilter
+                            times this (default 1) */
+} cnq_sky_options;
+
+void cnq_sky_options_default(cnq_sky_options* sky);
+
+/* `sky_irradiance` (WGSL :1411-1426) at each receiver (ReceiverGpu, 32 B; a
+ * negative normal.w traces nothing): out[i] the irradiance per unit sky
+ * radiance, the six faces' light summed in order (+x, -x, +y, -y, +z, -z;
+ * each face's `right` axis the next world axis). out_faces (nullable, 6 x n:
+ * face f of receiver i at f * n + i) and out_full (nullable, 6 x n: the
+ * face's bins with coverage 1) give the faces apart. Device buffers,
+ * enqueued on `stream`; without out_faces the scene keeps a 6 x n buffer. */
+int cnq_sky_irradiance_device(cnq_scene* scene, const float* receivers, uint64_t n, const cnq_options* opts,
+                              const cnq_sky_options* sky, float* out, float* out_faces, uint64_t* out_full,
+                              void* stream);
+/* The same on host buffers (copied in and out; returns when done). */
+int cnq_sky_irradiance(cnq_scene* scene, const float* receivers, uint64_t n, const cnq_options* opts,
+                       const cnq_sky_options* sky, float* out, float* out_faces, uint64_t* out_full);
+
 /* The same over a width x height view's receivers (pixel y * width + x),
  * each wave taking a tile_w x (64 / tile_w) patch of pixels (tile_w a power
  * of two up to 64): the cone part of the WGSL `render` entry

@@ -87,6 +87,10 @@ struct Query {
   // WGSL :587-591: a triangle the cone does not see (object, triangle), and
   // that triangle while the walk is in its object.
   uint32_t skip_obj = NONE, skip_tri = NONE, cover_skip = NONE;
+  // WGSL :653 `independent_occluders`: sky cones combine meshes as
+  // independent occluders, 1 - (1 - a)(1 - b), and walk children in a fixed
+  // order (:1195-1201); light cones take the maximum, nearest child first.
+  bool independent = false;
   uint32_t stack[STACK];
   const Scene* s;
   const Options* o;
@@ -537,7 +541,7 @@ static Cone cone_to_object(const Scene& s, const Cone& c, uint32_t m) {
 }
 
 // Combines one mesh's accumulation buffer into `cov` (WGSL :809-828).
-static void combine_mesh(const Acc& acc, float* cov, uint32_t& full, bool snap) {
+static void combine_mesh(const Acc& acc, float* cov, uint32_t& full, bool snap, bool independent) {
   float running = 0.0f;
   uint32_t end = std::min(acc.span_hi, BINS);
   for (uint32_t b = acc.span_lo; b < end; b++) {
@@ -550,7 +554,10 @@ static void combine_mesh(const Acc& acc, float* cov, uint32_t& full, bool snap) 
     }
     if (covered > 1e-6f) {
       float before = cov[b];
-      cov[b] = fmax_(before, fmin_(covered, 1.0f));
+      if (independent)
+        cov[b] = 1.0f - (1.0f - before) * (1.0f - fmin_(covered, 1.0f));
+      else
+        cov[b] = fmax_(before, fmin_(covered, 1.0f));
       if (before < 1.0f && cov[b] >= 1.0f) full++;
     }
   }
@@ -611,7 +618,7 @@ static void add_mesh_coverage(const Cone& world_cone, Query& qs, uint32_t m, flo
     for (uint32_t t = range[0]; t < range[0] + range[1]; t++)
       if (cover_triangle(c, qs, acc, t)) touched = true;
   }
-  if (touched) combine_mesh(acc, cov, full, true);
+  if (touched) combine_mesh(acc, cov, full, true, qs.independent);
 }
 
 // WGSL :1096-1150 (object-level splat; light cones combine by the maximum)
@@ -644,15 +651,17 @@ static void scene_coverage(const Cone& c, Query& qs, float* cov, uint32_t& full)
           add_mesh_coverage(c, qs, s.items[k], cov, full, sp);
       } else {
         uint32_t nearc = node.info[0], farc = node.info[0] + 1;
-        const Node& a = s.nodes[nearc];
-        const Node& b = s.nodes[farc];
-        bool use_bound = c.bound[3] >= -1.0f && (c.bound[0] != 0.0f || c.bound[1] != 0.0f || c.bound[2] != 0.0f);
-        V3 axis = use_bound ? xyz(c.bound) : c.forward;
-        float da = dot(axis, (xyz(a.lo) + xyz(a.hi)) * 0.5f - c.apex);
-        float db = dot(axis, (xyz(b.lo) + xyz(b.hi)) * 0.5f - c.apex);
-        if (db < da) {
-          nearc = node.info[0] + 1;
-          farc = node.info[0];
+        if (!qs.independent) {
+          const Node& a = s.nodes[nearc];
+          const Node& b = s.nodes[farc];
+          bool use_bound = c.bound[3] >= -1.0f && (c.bound[0] != 0.0f || c.bound[1] != 0.0f || c.bound[2] != 0.0f);
+          V3 axis = use_bound ? xyz(c.bound) : c.forward;
+          float da = dot(axis, (xyz(a.lo) + xyz(a.hi)) * 0.5f - c.apex);
+          float db = dot(axis, (xyz(b.lo) + xyz(b.hi)) * 0.5f - c.apex);
+          if (db < da) {
+            nearc = node.info[0] + 1;
+            farc = node.info[0];
+          }
         }
         if (sp + 2 > STACK) {
           qs.stack_overflow = true;
@@ -720,7 +729,7 @@ static bool splat_object_node(const Cone& c, Query& qs, uint32_t n, V3 lo, V3 hi
   acc.span_lo = 68;
   acc.span_hi = 0;
   draw_poly(acc, q, alpha);
-  combine_mesh(acc, cov, full, false);
+  combine_mesh(acc, cov, full, false, qs.independent);
   return true;
 }
 
@@ -1054,6 +1063,81 @@ static Result hit_irradiance_rect(const Scene& s, const Options& o, V3 point, V3
   return res;
 }
 
+// ---- Sky light: a uniform sky through six cube-face mask cones (WGSL
+// :1323-1510), faces on the world axes (`sky_frame` NONE), no skipped object.
+
+// WGSL :1326: far enough that every scene lies inside a sky cone.
+static const float SKY_FAR = 1e7f;
+
+struct SkyFace {
+  Cone c;
+  bool above;
+};
+
+// WGSL :1444-1478: cube face `f` around `apex`, its horizon `normal` when
+// `horizon_cull` (params.w bit 2 clear), and whether any of its directions
+// lies above that horizon (and `on`).
+static SkyFace sky_face(bool on, uint32_t f, V3 apex, V3 normal, bool horizon_cull) {
+  const uint32_t axis = f / 2u;
+  const float sign = (f & 1u) ? -1.0f : 1.0f;
+  float fw[3] = {0.0f, 0.0f, 0.0f}, rt[3] = {0.0f, 0.0f, 0.0f};
+  fw[axis] = sign;
+  rt[(axis + 1u) % 3u] = 1.0f;
+  const V3 forward{fw[0], fw[1], fw[2]}, right{rt[0], rt[1], rt[2]};
+  SkyFace face;
+  face.c = zero_cone();
+  face.c.apex = apex;
+  face.c.right = right;
+  face.c.up = cross(forward, right);
+  face.c.forward = forward;
+  face.c.window[0] = -1.0f;
+  face.c.window[1] = -1.0f;
+  face.c.window[2] = 1.0f;
+  face.c.window[3] = 1.0f;
+  face.c.far_normal = forward;
+  face.c.far_offset = dot(forward, apex) + SKY_FAR;
+  if (horizon_cull) face.c.horizon = normal;
+  face.c = bound_cone(face.c, forward);
+  bool above = false;
+  for (uint32_t k = 0; k < 4; k++)
+    if (dot(normal, direction(face.c, (float)(k & 1u), (float)(k >> 1u))) > 0.0f) above = true;
+  face.above = above && on;
+  return face;
+}
+
+struct SkyOptions {
+  bool max_combine = false;   // params.w bit 1 (`sky_max`, a diagnostic)
+  bool horizon_cull = true;   // params.w bit 2 clear (`sky_horizon_cull`)
+  float prefilter_scale = 1.0f;  // light.objects.y (`sky_prefilter`)
+};
+
+// WGSL :1482-1488 (`sky_face_coverage`) and :1492-1501 (`sky_face_light`):
+// the face's irradiance per unit sky radiance; its coverage into `cov`.
+static float sky_face_irradiance(const Scene& s, const Options& o, const SkyOptions& so, bool on, uint32_t f,
+                                 V3 apex, V3 normal, float* cov, uint64_t* full_mask) {
+  if (o.profile_stage == 1) return 0.0f;
+  Query qs;
+  qs.s = &s;
+  qs.o = &o;
+  const float scale = so.prefilter_scale > 0.0f ? so.prefilter_scale : 1.0f;
+  qs.prefilter_bins = o.prefilter * scale;
+  const SkyFace face = sky_face(on, f, apex, normal, so.horizon_cull);
+  for (uint32_t b = 0; b < BINS; b++) cov[b] = 0.0f;
+  uint32_t full = 0;
+  qs.independent = !so.max_combine;
+  // A lane with `above` false follows the packet without acting: no coverage.
+  if (o.profile_stage != 2 && face.above) scene_coverage(face.c, qs, cov, full);
+  uint64_t fm = 0;
+  for (uint32_t b = 0; b < BINS; b++)
+    if (cov[b] >= 1.0f) fm |= 1ull << b;
+  if (full_mask) *full_mask = fm;
+  if (!face.above) return 0.0f;
+  float blocked = 0.0f;
+  for (uint32_t b = 0; b < BINS; b++)
+    if (cov[b] > 0.0f) blocked += cov[b] * clipped_bin_weight(face.c, normal, b);
+  return fmax_(clipped_window_weight(face.c, normal) - blocked, 0.0f);
+}
+
 // ---- The lighting cache: lightmap blocks of 8 x 8 texels re-baked when
 // meshes move (lightmap.rs:256-333; WGSL :2408-2520; live.rs:515-720, 815-900).
 
@@ -1297,6 +1381,51 @@ int oracle_irradiance(const float* tris, const uint32_t* ranges, uint32_t n_obje
 
 // Single building blocks, for the tests that pin them.
 float oracle_atan2(float y, float x) { return atan2_c(y, x); }
+// WGSL :1416-1426 (`sky_irradiance_if`) per receiver (ReceiverGpu; a
+// negative normal.w traces nothing): out[i] the sum over the six faces in
+// order, out_faces[f * n + i] each face's share, out_full[f * n + i] the
+// face's full bins (coverage 1). sky_opts: [max_combine, horizon_cull,
+// prefilter_scale bits].
+void oracle_sky(const float* tris, const uint32_t* ranges, uint32_t n_objects, const float* blas,
+                const uint32_t* blas_items, const float* blas_cov, const float* nodes, const uint32_t* items,
+                const float* xforms, const float* receivers, uint64_t n, const uint32_t* opts,
+                const uint32_t* sky_opts, float* out, float* out_faces, uint64_t* out_full, int threads) {
+  Scene s = make_scene(tris, ranges, n_objects, blas, blas_items, blas_cov, nodes, items, xforms);
+  Options o = make_options(opts);
+  SkyOptions so;
+  so.max_combine = sky_opts[0] != 0;
+  so.horizon_cull = sky_opts[1] != 0;
+  std::memcpy(&so.prefilter_scale, &sky_opts[2], 4);
+  parallel_for(n, threads, [&](uint64_t i) {
+    const float* rc = receivers + 8 * i;
+    const bool on = rc[7] >= 0.0f;
+    float total = 0.0f;
+    for (uint32_t f = 0; f < 6; f++) {
+      float cov[64];
+      uint64_t fm = 0;
+      const float e = sky_face_irradiance(s, o, so, on, f, xyz(rc), xyz(rc + 4), cov, &fm);
+      total += e;
+      if (out_faces) out_faces[f * n + i] = e;
+      if (out_full) out_full[f * n + i] = fm;
+    }
+    out[i] = total;
+  });
+}
+
+// One sky face's coverage for one point (tests): cov[64].
+float oracle_sky_face(const float* tris, const uint32_t* ranges, uint32_t n_objects, const float* blas,
+                      const uint32_t* blas_items, const float* blas_cov, const float* nodes, const uint32_t* items,
+                      const float* xforms, const float* point, const float* normal, uint32_t face,
+                      const uint32_t* opts, const uint32_t* sky_opts, float* cov) {
+  Scene s = make_scene(tris, ranges, n_objects, blas, blas_items, blas_cov, nodes, items, xforms);
+  Options o = make_options(opts);
+  SkyOptions so;
+  so.max_combine = sky_opts[0] != 0;
+  so.horizon_cull = sky_opts[1] != 0;
+  std::memcpy(&so.prefilter_scale, &sky_opts[2], 4);
+  return sky_face_irradiance(s, o, so, true, face, xyz(point), xyz(normal), cov, nullptr);
+}
+
 float oracle_acos(float y) { return acos_c(y); }
 float oracle_exp(float x) { return exp_c(x); }
 

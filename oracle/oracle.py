@@ -33,6 +33,8 @@ def lib() -> C.CDLL:
             "oracle_refit": (None, [_P, _P, C.c_uint32, _P, _P, _P, C.c_uint32]),
             "oracle_atan2": (C.c_float, [C.c_float, C.c_float]),
             "oracle_acos": (C.c_float, [C.c_float]),
+            "oracle_sky": (None, [_P] * 9 + [_P, C.c_uint64, _P, _P, _P, _P, _P, C.c_int]),
+            "oracle_sky_face": (C.c_float, [_P] * 9 + [_P, _P, C.c_uint32, _P, _P, _P]),
             "oracle_exp": (C.c_float, [C.c_float]),
             "oracle_draw_polygon": (None, [_P, C.c_uint32, C.c_float, _P, _P]),
             "oracle_light_cone": (C.c_int, [_P, _P, _P, _P]),
@@ -176,6 +178,38 @@ def irradiance(flat, receivers: np.ndarray, lights: np.ndarray, opts: np.ndarray
     of = lib().oracle_irradiance(*scene_args(flat), _p(receivers), n, _p(lights), nl, _p(opts), _p(vis), _p(full),
                                  _p(touched), _p(cov), _p(capped), threads or threads_default())
     return vis, full, touched, cov, capped, of
+
+
+def sky_options(max_combine=False, horizon_cull=True, prefilter_scale=1.0) -> np.ndarray:
+    """[max_combine, horizon_cull, prefilter_scale bits] as oracle_sky reads them."""
+    o = np.zeros(3, np.uint32)
+    o[0], o[1] = int(bool(max_combine)), int(bool(horizon_cull))
+    o[2] = np.float32(prefilter_scale).view(np.uint32)
+    return o
+
+
+def sky(flat, receivers: np.ndarray, opts: np.ndarray, sky_opts=None, threads=None):
+    """Sky irradiance per unit radiance through six cube-face cones (WGSL
+    :1416 sky_irradiance_if): (total (n,), per face (6, n), full bins (6, n))."""
+    receivers = np.ascontiguousarray(receivers, dtype=np.float32).reshape(-1, 8)
+    n = len(receivers)
+    so = sky_options() if sky_opts is None else np.ascontiguousarray(sky_opts, np.uint32)
+    out = np.zeros(n, np.float32)
+    faces = np.zeros((6, n), np.float32)
+    full = np.zeros((6, n), np.uint64)
+    lib().oracle_sky(*scene_args(flat), _p(receivers), n, _p(opts), _p(so), _p(out), _p(faces), _p(full),
+                     threads or threads_default())
+    return out, faces, full
+
+
+def sky_face(flat, point, normal, face: int, opts: np.ndarray, sky_opts=None):
+    """One cube face at one point: (irradiance, coverage (64,))."""
+    p = np.ascontiguousarray(point, np.float32)
+    nrm = np.ascontiguousarray(normal, np.float32)
+    so = sky_options() if sky_opts is None else np.ascontiguousarray(sky_opts, np.uint32)
+    cov = np.zeros(64, np.float32)
+    e = lib().oracle_sky_face(*scene_args(flat), _p(p), _p(nrm), int(face), _p(opts), _p(so), _p(cov))
+    return float(e), cov
 
 
 def primary(flat, camera: np.ndarray, width: int, height: int, lights: np.ndarray, hierarchy=2, nearest_first=1,

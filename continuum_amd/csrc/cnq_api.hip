@@ -246,7 +246,9 @@ int cnq_device_create(int ordinal, cnq_device** out) {
   if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return fail(CNQ_NO_DEVICE, "no HIP device");
   if (ordinal < 0 || ordinal >= count) return fail(CNQ_NO_DEVICE, "no HIP device " + std::to_string(ordinal));
   HIP_TRY(hipSetDevice(ordinal), "hipSetDevice");
-  cnq_device* d = new (std::nothrow) cnq_device{ordinal};
+  int cus = 0;
+  HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ordinal), "compute units");
+  cnq_device* d = new (std::nothrow) cnq_device{ordinal, (uint32_t)(cus > 0 ? cus : 1)};
   if (!d) return fail(CNQ_OOM, "out of host memory");
   *out = d;
   return CNQ_OK;

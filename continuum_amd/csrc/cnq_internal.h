@@ -56,6 +56,7 @@ uint64_t layout_view(ViewDesc& v, uint32_t tile_w, uint64_t item_off);
 
 struct cnq_device {
   int ordinal;
+  uint32_t compute_units;  // hipDeviceProp_t::multiProcessorCount
 };
 
 struct cnq_scene {

@@ -70,6 +70,7 @@ struct VcArgs {
   uint32_t plan_pages, plan_jobs, append_pages, append_jobs;
   uint32_t candidate_cap, invalid_cap, table_mask;
   uint32_t frame, visible_frames, light_ratio;
+  uint32_t bake_group;                 // jobs a bake wave takes (a power of two, 4..64)
   float inv_base;
 };
 
@@ -504,11 +505,17 @@ __global__ __launch_bounds__(64, WALK_WAVES) void vc_bake_kernel(VcArgs a, const
               p.n_objects, p.hierarchy, p.profile_stage, p.object_cov};
   const uint32_t appended = a.stats[5] < a.append_jobs ? a.stats[5] : a.append_jobs;
   const uint32_t total = a.plan_jobs + appended;
-  for (uint32_t base = blockIdx.x * 64u; base < total; base += gridDim.x * 64u) {
+  // A wave takes `bake_group` consecutive jobs (its first lanes): all 64 when
+  // the bake fills the GPU; fewer when it would leave it mostly idle, as a
+  // packet's walk visits the union of its lanes' nodes, and a narrower
+  // packet's union is shorter (the frame ends with its slowest wave).
+  const uint32_t group = a.bake_group;
+  for (uint32_t base = blockIdx.x * group; base < total; base += gridDim.x * group) {
     const uint32_t j = base + lane;
-    const uint32_t* job = a.bake_jobs + 2 * (size_t)(j < total ? j : total - 1u);
+    const bool mine = lane < group && j < total;
+    const uint32_t* job = a.bake_jobs + 2 * (size_t)(mine ? j : total - 1u);
     const uint32_t jx = job[0], jy = job[1];
-    const bool listed = j < total && jx != NONE;
+    const bool listed = mine && jx != NONE;
     const uint32_t k = jy & 0xffu;
     const uint32_t l = (jy >> 8) < a.n_lights - 1u ? (jy >> 8) : a.n_lights - 1u;
     const PageTexel t = page_texel(a, jx == NONE ? 0u : jx, k);
@@ -1031,6 +1038,7 @@ struct cnq_vcache {
     a.frame = frame;
     a.visible_frames = cfg.visible_frames;
     a.light_ratio = cfg.light_ratio;
+    a.bake_group = 64;
     a.inv_base = inv_base;
     return a;
   }
@@ -1400,7 +1408,17 @@ int cnq_vcache_frame(cnq_vcache* c, void* stream, uint64_t stats[10]) {
     }
   }
   const cnq::QueryParams p = query_params(s, &c->opts, 64);
-  const uint64_t job_waves = ((uint64_t)plan_jobs + (a.n_movers ? a.append_jobs : 0) + 63) / 64;
+  // Jobs per wave: 64, or with a bake that would leave most of the GPU idle
+  // (fewer waves than it holds: 4 SIMDs a compute unit, WALK_WAVES each),
+  // the fewest (down to 4) that still fit it. A frame with moves appends an
+  // unknown number of jobs, so it keeps 64.
+  const uint64_t jobs = (uint64_t)plan_jobs + (a.n_movers ? a.append_jobs : 0);
+  const uint64_t resident = (uint64_t)s->dev->compute_units * 4 * cnq::WALK_WAVES;
+  uint32_t group = 64;
+  if (!a.n_movers)
+    while (group > 4 && (jobs + group / 2 - 1) / (group / 2) <= resident) group /= 2;
+  a.bake_group = group;
+  const uint64_t job_waves = (jobs + group - 1) / group;
   if (job_waves) {
     const uint32_t grid = (uint32_t)std::min<uint64_t>(job_waves, 8192);
     if (s->count_work)

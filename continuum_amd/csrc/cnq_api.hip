@@ -201,7 +201,50 @@ cnq::ItemArgs item_args(cnq_scene* s, const float* receivers, const float* light
   a.counters = s->counters.p;
   a.work = s->count_work ? s->work.p : nullptr;
   a.work_counter = nullptr;
+  a.order = nullptr;
+  a.item_cost = nullptr;
   return a;
+}
+
+// Longest items first for a view launched again: the launch records each
+// item's time and a one-block pass turns them into the next launch's order
+// (cnq::launch_item_order). The order buffer is read and rewritten only on
+// the stream the scene first ordered on, so a launch never reads it while it
+// changes; launches on other streams run in natural order. Results do not
+// depend on the order: every item writes only its own queries.
+hipError_t order_items(cnq_scene* s, cnq::ItemArgs& a, uint64_t items, const uint64_t key[4], hipStream_t stream) {
+  ItemOrder& o = s->order;
+  if (items == 0 || items > 0x7fffffffull) return hipSuccess;
+  if (!o.bound) {
+    o.bound = true;
+    o.stream = stream;
+  }
+  if (o.stream != stream) return hipSuccess;
+  const bool same = o.ready && o.key[0] == key[0] && o.key[1] == key[1] && o.key[2] == key[2] && o.key[3] == key[3];
+  if (!same) {
+    if (o.cost.n < items) {
+      hipError_t e = hipStreamSynchronize(stream);  // no launch still reads the old buffers
+      if (e != hipSuccess) return e;
+      o.cost.release(), o.order.release();
+      if ((e = o.cost.alloc(items)) != hipSuccess || (e = o.order.alloc(items)) != hipSuccess) {
+        o.cost.release(), o.order.release();
+        o.ready = false;
+        return e;
+      }
+    }
+    for (int k = 0; k < 4; k++) o.key[k] = key[k];
+    o.ready = false;
+  }
+  a.order = o.ready ? o.order.p : nullptr;
+  a.item_cost = o.cost.p;
+  return hipSuccess;
+}
+
+hipError_t reorder_items(cnq_scene* s, const cnq::ItemArgs& a, uint64_t items, hipStream_t stream) {
+  if (!a.item_cost) return hipSuccess;
+  hipError_t e = cnq::launch_item_order(s->order.cost.p, s->order.order.p, (uint32_t)items, stream);
+  if (e == hipSuccess) s->order.ready = true;
+  return e;
 }
 
 // Launches the items of `a` in direct mode, at most 2^31 - 1 blocks a launch.
@@ -516,7 +559,11 @@ int cnq_view_irradiance_device(cnq_scene* s, const float* receivers, uint32_t wi
   v.ptile_off = 0;
   const uint64_t items = layout_view(v, tile_w, 0);
   if (items > 0xffffffffull) return fail(CNQ_CAPACITY, "too many queries for one call");
+  const uint64_t key[4] = {((uint64_t)width << 32) | height, ((uint64_t)n_lights << 32) | tile_w, items,
+                           (uint64_t)(uintptr_t)receivers};
+  HIP_TRY(order_items(s, a, items, key, (hipStream_t)stream), "item order buffers");
   HIP_TRY(launch_items_direct(a, items, s->count_work, (hipStream_t)stream), "cone query launch");
+  HIP_TRY(reorder_items(s, a, items, (hipStream_t)stream), "item order");
   return CNQ_OK;
 }
 

@@ -35,7 +35,7 @@ __global__ __launch_bounds__(64, WALK_WAVES) void cone_items_kernel(
     const float* __restrict__ lights, const ViewDesc* __restrict__ views, float* __restrict__ out_visible,
     uint64_t* __restrict__ out_full, uint64_t* __restrict__ out_touched, float* __restrict__ out_cov,
     uint32_t* __restrict__ counters, uint64_t* __restrict__ work_out, uint32_t* __restrict__ work_counter,
-    QueryParams p, ViewDesc v0) {
+    const uint32_t* __restrict__ order, uint32_t* __restrict__ item_cost, QueryParams p, ViewDesc v0) {
   __shared__ float s_acc[ACC_CELLS * 64];
   const uint32_t lane = lane_id();
   float poly[POLY_SLOTS];
@@ -50,9 +50,10 @@ __global__ __launch_bounds__(64, WALK_WAVES) void cone_items_kernel(
       if (lane == 0) got = atomicAdd(work_counter, 1u);
       item = uni(got);
     } else {
-      item = p.item_begin + blockIdx.x;
+      item = p.item_begin + (order ? order[blockIdx.x] : blockIdx.x);
     }
     if (item >= p.n_items) break;
+    const uint64_t started = item_cost ? wall_clock64() : 0;
     const ViewDesc v = p.n_views == 1 ? v0 : views[find_view(views, p.n_views, item)];
     const uint32_t local = item - v.item_off;
     const uint32_t light = local / v.tiles;
@@ -78,13 +79,18 @@ __global__ __launch_bounds__(64, WALK_WAVES) void cone_items_kernel(
       if (of_m) atomicAdd(&counters[1], (uint32_t)__builtin_popcountll(of_m));
       if (tr_m) atomicAdd(&counters[2], (uint32_t)__builtin_popcountll(tr_m));
     }
+    if (item_cost && lane == 0) {
+      const uint64_t t = wall_clock64() - started;
+      item_cost[item - p.item_begin] = t < 0xffffffffull ? (uint32_t)t : 0xffffffffu;
+    }
     if (!PERSISTENT) break;
   }
 }
 
 #define CNQ_ITEMS_ARGS                                                                                         \
   s.blas, s.nodes, s.tri_leaf, s.tris, s.ranges, s.items, s.blas_cov, s.xforms, a.receivers, a.lights, a.views, \
-      a.out_visible, a.out_full, a.out_touched, a.out_cov, a.counters, a.work, a.work_counter, a.p, a.v0
+      a.out_visible, a.out_full, a.out_touched, a.out_cov, a.counters, a.work, a.work_counter, a.order, a.item_cost, \
+      a.p, a.v0
 
 hipError_t launch_cone_items(const ItemArgs& a, bool count_work, uint32_t persistent_blocks, hipStream_t stream) {
   if (a.p.n_items <= a.p.item_begin) return hipSuccess;
@@ -104,6 +110,37 @@ hipError_t launch_cone_items(const ItemArgs& a, bool count_work, uint32_t persis
     else
       hipLaunchKernelGGL((cone_items_kernel<false, false>), grid, block, 0, stream, CNQ_ITEMS_ARGS);
   }
+  return hipGetLastError();
+}
+
+// One block: items bucketed by the power of two of their time, buckets
+// longest first; within a bucket the order is the atomics'. A permutation of
+// [0, n) whatever the times.
+__global__ __launch_bounds__(1024) void item_order_kernel(const uint32_t* __restrict__ cost,
+                                                         uint32_t* __restrict__ order, uint32_t n) {
+  __shared__ uint32_t count[33], next[33];
+  const uint32_t t = threadIdx.x;
+  if (t < 33) count[t] = 0;
+  __syncthreads();
+  for (uint32_t i = t; i < n; i += 1024u) atomicAdd(&count[32u - (uint32_t)__builtin_clz(cost[i] | 1u)], 1u);
+  __syncthreads();
+  if (t == 0) {
+    uint32_t at = 0;
+    for (int b = 32; b >= 0; b--) {
+      next[b] = at;
+      at += count[b];
+    }
+  }
+  __syncthreads();
+  for (uint32_t i = t; i < n; i += 1024u) {
+    const uint32_t pos = atomicAdd(&next[32u - (uint32_t)__builtin_clz(cost[i] | 1u)], 1u);
+    order[pos] = i;
+  }
+}
+
+hipError_t launch_item_order(const uint32_t* cost, uint32_t* order, uint32_t n, hipStream_t stream) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(item_order_kernel, dim3(1), dim3(1024), 0, stream, cost, order, n);
   return hipGetLastError();
 }
 

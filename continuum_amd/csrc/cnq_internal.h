@@ -59,6 +59,15 @@ struct cnq_device {
   uint32_t compute_units;  // hipDeviceProp_t::multiProcessorCount
 };
 
+// A view's items in the order of their last times (longest first), for
+// launching the same view again (cnq_view_irradiance_device).
+struct ItemOrder {
+  bool bound = false, ready = false;
+  void* stream = nullptr;  // the one stream the order is read and written on
+  uint64_t key[4] = {0, 0, 0, 0};
+  cnq::detail::DevBuf<uint32_t> cost, order;
+};
+
 struct cnq_scene {
   cnq_device* dev;
   uint64_t n_tris;
@@ -70,6 +79,7 @@ struct cnq_scene {
   cnq::detail::DevBuf<uint32_t> blas_items, ranges, items, parent, leaves, arrivals, counters;
   cnq::detail::DevBuf<uint64_t> work;
   cnq::detail::DevBuf<float> sky_faces;  // cnq_sky_irradiance_device's per-face values when the caller has none
+  ItemOrder order;
   cnq::detail::DevBuf<cnq::NodeC> blas, nodes;
   std::vector<uint32_t> h_ranges;  // for host-side checks and object rebuilds
   // Moving objects (instanced scenes): the subtree's layout, each geometry's
@@ -103,7 +113,8 @@ struct cnq_scene {
   void release() {
     tris.release(), tri_leaf.release(), blas_cov.release(), xforms.release(), mesh_box.release();
     blas_items.release(), ranges.release(), items.release(), parent.release(), leaves.release();
-    arrivals.release(), counters.release(), work.release(), sky_faces.release(), blas.release(), nodes.release();
+    arrivals.release(), counters.release(), work.release(), sky_faces.release(), order.cost.release(),
+        order.order.release(), blas.release(), nodes.release();
     for (auto& kv : pose_leaves) kv.second.release();
     pose_leaves.clear();
   }

@@ -86,6 +86,11 @@ struct ItemArgs {
   uint32_t* __restrict__ counters;
   uint64_t* __restrict__ work;
   uint32_t* __restrict__ work_counter;  // persistent mode: the next item (zeroed before launch)
+  // Direct mode: wave w takes item item_begin + order[w] when `order` is
+  // given (a permutation of the launch's items), and writes the item's time
+  // (wall clock, 10 ns units) to item_cost[item - item_begin] when given.
+  const uint32_t* __restrict__ order;
+  uint32_t* __restrict__ item_cost;
   QueryParams p;
 };
 
@@ -184,6 +189,9 @@ struct SkyArgs {
 
 hipError_t launch_cone_items(const ItemArgs& a, bool count_work, uint32_t persistent_blocks, hipStream_t stream);
 hipError_t launch_sky(const SkyArgs& a, bool count_work, hipStream_t stream);
+// The next launch's order from this one's item times: longest first, by
+// powers of two (one block; the order within a power of two is arbitrary).
+hipError_t launch_item_order(const uint32_t* cost, uint32_t* order, uint32_t n, hipStream_t stream);
 hipError_t launch_primary(const PrimaryArgs& a, hipStream_t stream);
 hipError_t launch_refit(const RefitArgs& a, hipStream_t stream);
 hipError_t launch_bake_static(const CacheArgs& a, hipStream_t stream);

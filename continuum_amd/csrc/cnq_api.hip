@@ -250,6 +250,7 @@ hipError_t reorder_items(cnq_scene* s, const cnq::ItemArgs& a, uint64_t items, h
 // Launches the items of `a` in direct mode, at most 2^31 - 1 blocks a launch.
 hipError_t launch_items_direct(cnq::ItemArgs a, uint64_t n_items, bool count_work, hipStream_t stream) {
   const uint64_t per = 0x7fffffffull;
+  if (n_items > per) a.order = nullptr, a.item_cost = nullptr;  // an order covers one launch
   for (uint64_t i0 = 0; i0 < n_items; i0 += per) {
     a.p.item_begin = (uint32_t)i0;
     a.p.n_items = (uint32_t)(i0 + per < n_items ? i0 + per : n_items);
@@ -1061,6 +1062,10 @@ struct cnq_batch {
   DevBuf<cnq::ViewDesc> views;
   DevBuf<float> cameras, lights, receivers, visible;
   DevBuf<uint32_t> hits, work_counter;
+  // The cone pass's items in the order of their last times (longest first):
+  // identity after a layout, then rewritten after every cone pass, so a
+  // replayed batch starts its heaviest items first.
+  DevBuf<uint32_t> item_cost, item_order;
   DevBuf<uint64_t> full, touched;
   hipStream_t own_stream = nullptr;
   hipGraph_t graph = nullptr;
@@ -1074,6 +1079,7 @@ struct cnq_batch {
     exec = nullptr, graph = nullptr, own_stream = nullptr;
     views.release(), cameras.release(), lights.release(), receivers.release(), visible.release();
     hits.release(), work_counter.release(), full.release(), touched.release();
+    item_cost.release(), item_order.release();
   }
 };
 
@@ -1098,6 +1104,18 @@ int batch_layout(cnq_batch* b, uint32_t tile_w) {
   b->n_items = item;
   HIP_TRY(hipMemcpy(b->views.p, b->h_views.data(), b->h_views.size() * sizeof(cnq::ViewDesc), hipMemcpyHostToDevice),
           "view table");
+  // A fresh identity order for the new items.
+  b->item_cost.release(), b->item_order.release();
+  if (b->n_items <= 0x7fffffffull) {
+    std::vector<uint32_t> ident(b->n_items);
+    for (uint64_t i = 0; i < b->n_items; i++) ident[i] = (uint32_t)i;
+    hipError_t e = b->item_cost.alloc(b->n_items);
+    if (e == hipSuccess) e = b->item_order.upload(ident.data(), ident.size());
+    if (e != hipSuccess) {
+      b->item_cost.release(), b->item_order.release();
+      return hip_fail(e, "item order");
+    }
+  }
   b->layout_tile_w = tile_w;
   return CNQ_OK;
 }
@@ -1125,6 +1143,8 @@ int batch_enqueue(cnq_batch* b, const cnq_options* o, const cnq_tuning* t, hipSt
   a.views = b->views.p;
   a.v0 = b->h_views[0];
   a.work_counter = b->work_counter.p;
+  a.order = b->item_order.p;
+  a.item_cost = b->item_cost.p;
   if (t->persistent_blocks) {
     HIP_TRY(fill_u32(b->work_counter.p, 0, 1, st), "work counter");
     a.p.item_begin = 0;
@@ -1133,6 +1153,7 @@ int batch_enqueue(cnq_batch* b, const cnq_options* o, const cnq_tuning* t, hipSt
   } else {
     HIP_TRY(launch_items_direct(a, b->n_items, s->count_work, st), "batch cone pass");
   }
+  if (a.item_cost) HIP_TRY(cnq::launch_item_order(a.item_cost, b->item_order.p, (uint32_t)b->n_items, st), "item order");
   return CNQ_OK;
 }
 

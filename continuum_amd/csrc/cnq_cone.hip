@@ -48,7 +48,9 @@ __global__ __launch_bounds__(64, WALK_WAVES) void cone_items_kernel(
     if (PERSISTENT) {
       uint32_t got = 0;
       if (lane == 0) got = atomicAdd(work_counter, 1u);
-      item = uni(got);
+      got = uni(got);
+      if (got >= p.n_items) break;
+      item = order ? order[got] : got;
     } else {
       item = p.item_begin + (order ? order[blockIdx.x] : blockIdx.x);
     }

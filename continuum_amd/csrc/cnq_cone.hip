@@ -116,55 +116,33 @@ hipError_t launch_cone_items(const ItemArgs& a, bool count_work, uint32_t persis
 }
 
 // One block: items bucketed by the power of two of their time, buckets
-// longest first, each bucket in item order (neighbouring tiles of similar
-// time stay together, sharing the cache): thread t counts and then places
-// the items of its contiguous chunk. A permutation of [0, n) whatever the
-// times.
-constexpr uint32_t ORDER_THREADS = 256, ORDER_BUCKETS = 33;
-__global__ __launch_bounds__(ORDER_THREADS) void item_order_kernel(const uint32_t* __restrict__ cost,
-                                                                  uint32_t* __restrict__ order, uint32_t n) {
-  __shared__ uint32_t at[ORDER_BUCKETS][ORDER_THREADS];
+// longest first; within a bucket the order is the atomics'. A permutation of
+// [0, n) whatever the times.
+__global__ __launch_bounds__(1024) void item_order_kernel(const uint32_t* __restrict__ cost,
+                                                         uint32_t* __restrict__ order, uint32_t n) {
+  __shared__ uint32_t count[33], next[33];
   const uint32_t t = threadIdx.x;
-  const uint32_t per = (n + ORDER_THREADS - 1) / ORDER_THREADS;
-  const uint32_t i0 = t * per < n ? t * per : n, i1 = i0 + per < n ? i0 + per : n;
-  for (uint32_t b = 0; b < ORDER_BUCKETS; b++) at[b][t] = 0;
-  for (uint32_t i = i0; i < i1; i++) at[32u - (uint32_t)__builtin_clz(cost[i] | 1u)][t]++;
+  if (t < 33) count[t] = 0;
   __syncthreads();
-  // Exclusive prefix over (bucket descending, thread): position f of that
-  // sequence is bucket 32 - f / THREADS, thread f % THREADS; thread t scans
-  // positions [33 t, 33 t + 33).
-  __shared__ uint32_t part[ORDER_THREADS];
-  uint32_t local = 0;
-  for (uint32_t k = 0; k < ORDER_BUCKETS; k++) {
-    const uint32_t f = t * ORDER_BUCKETS + k;
-    local += at[ORDER_BUCKETS - 1 - f / ORDER_THREADS][f % ORDER_THREADS];
-  }
-  part[t] = local;
+  for (uint32_t i = t; i < n; i += 1024u) atomicAdd(&count[32u - (uint32_t)__builtin_clz(cost[i] | 1u)], 1u);
   __syncthreads();
   if (t == 0) {
-    uint32_t sum = 0;
-    for (uint32_t u = 0; u < ORDER_THREADS; u++) {
-      const uint32_t c = part[u];
-      part[u] = sum;
-      sum += c;
+    uint32_t at = 0;
+    for (int b = 32; b >= 0; b--) {
+      next[b] = at;
+      at += count[b];
     }
   }
   __syncthreads();
-  uint32_t sum = part[t];
-  for (uint32_t k = 0; k < ORDER_BUCKETS; k++) {
-    const uint32_t f = t * ORDER_BUCKETS + k;
-    uint32_t& cell = at[ORDER_BUCKETS - 1 - f / ORDER_THREADS][f % ORDER_THREADS];
-    const uint32_t c = cell;
-    cell = sum;
-    sum += c;
+  for (uint32_t i = t; i < n; i += 1024u) {
+    const uint32_t pos = atomicAdd(&next[32u - (uint32_t)__builtin_clz(cost[i] | 1u)], 1u);
+    order[pos] = i;
   }
-  __syncthreads();
-  for (uint32_t i = i0; i < i1; i++) order[at[32u - (uint32_t)__builtin_clz(cost[i] | 1u)][t]++] = i;
 }
 
 hipError_t launch_item_order(const uint32_t* cost, uint32_t* order, uint32_t n, hipStream_t stream) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(item_order_kernel, dim3(1), dim3(ORDER_THREADS), 0, stream, cost, order, n);
+  hipLaunchKernelGGL(item_order_kernel, dim3(1), dim3(1024), 0, stream, cost, order, n);
   return hipGetLastError();
 }
 

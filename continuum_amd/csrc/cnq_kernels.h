@@ -190,7 +190,7 @@ struct SkyArgs {
 hipError_t launch_cone_items(const ItemArgs& a, bool count_work, uint32_t persistent_blocks, hipStream_t stream);
 hipError_t launch_sky(const SkyArgs& a, bool count_work, hipStream_t stream);
 // The next launch's order from this one's item times: longest first, by
-// powers of two, each power of two in item order (one block).
+// powers of two (one block; the order within a power of two is arbitrary).
 hipError_t launch_item_order(const uint32_t* cost, uint32_t* order, uint32_t n, hipStream_t stream);
 hipError_t launch_primary(const PrimaryArgs& a, hipStream_t stream);
 hipError_t launch_refit(const RefitArgs& a, hipStream_t stream);

@@ -212,8 +212,8 @@ cnq::ItemArgs item_args(cnq_scene* s, const float* receivers, const float* light
 // the stream the scene first ordered on, so a launch never reads it while it
 // changes; launches on other streams run in natural order. Results do not
 // depend on the order: every item writes only its own queries.
-hipError_t order_items(cnq_scene* s, cnq::ItemArgs& a, uint64_t items, const uint64_t key[4], hipStream_t stream) {
-  ItemOrder& o = s->order;
+hipError_t order_items(ItemOrder& o, const uint32_t*& order, uint32_t*& item_cost, uint64_t items,
+                       const uint64_t key[4], hipStream_t stream) {
   if (items == 0 || items > 0x7fffffffull) return hipSuccess;
   if (!o.bound) {
     o.bound = true;
@@ -235,15 +235,15 @@ hipError_t order_items(cnq_scene* s, cnq::ItemArgs& a, uint64_t items, const uin
     for (int k = 0; k < 4; k++) o.key[k] = key[k];
     o.ready = false;
   }
-  a.order = o.ready ? o.order.p : nullptr;
-  a.item_cost = o.cost.p;
+  order = o.ready ? o.order.p : nullptr;
+  item_cost = o.cost.p;
   return hipSuccess;
 }
 
-hipError_t reorder_items(cnq_scene* s, const cnq::ItemArgs& a, uint64_t items, hipStream_t stream) {
-  if (!a.item_cost) return hipSuccess;
-  hipError_t e = cnq::launch_item_order(s->order.cost.p, s->order.order.p, (uint32_t)items, stream);
-  if (e == hipSuccess) s->order.ready = true;
+hipError_t reorder_items(ItemOrder& o, const uint32_t* item_cost, uint64_t items, hipStream_t stream) {
+  if (!item_cost) return hipSuccess;
+  hipError_t e = cnq::launch_item_order(o.cost.p, o.order.p, (uint32_t)items, stream);
+  if (e == hipSuccess) o.ready = true;
   return e;
 }
 
@@ -562,9 +562,13 @@ int cnq_view_irradiance_device(cnq_scene* s, const float* receivers, uint32_t wi
   if (items > 0xffffffffull) return fail(CNQ_CAPACITY, "too many queries for one call");
   const uint64_t key[4] = {((uint64_t)width << 32) | height, ((uint64_t)n_lights << 32) | tile_w, items,
                            (uint64_t)(uintptr_t)receivers};
-  HIP_TRY(order_items(s, a, items, key, (hipStream_t)stream), "item order buffers");
+  const uint32_t* order = nullptr;
+  uint32_t* cost = nullptr;
+  HIP_TRY(order_items(s->order, order, cost, items, key, (hipStream_t)stream), "item order buffers");
+  a.order = order;
+  a.item_cost = cost;
   HIP_TRY(launch_items_direct(a, items, s->count_work, (hipStream_t)stream), "cone query launch");
-  HIP_TRY(reorder_items(s, a, items, (hipStream_t)stream), "item order");
+  HIP_TRY(reorder_items(s->order, a.item_cost, items, (hipStream_t)stream), "item order");
   return CNQ_OK;
 }
 
@@ -642,7 +646,17 @@ int cnq_sky_irradiance_device(cnq_scene* s, const float* receivers, uint64_t n, 
   a.n = (uint32_t)n;
   a.p = query_params(s, o, 64);
   a.sky = cnq::SkyParams{so.max_combine, so.horizon_cull, so.prefilter_scale};
+  a.order = nullptr;
+  a.item_cost = nullptr;
+  const uint64_t items = 6 * ((n + 63) / 64);
+  const uint64_t key[4] = {n, (uint64_t)(uintptr_t)receivers, items, 0};
+  const uint32_t* order = nullptr;
+  uint32_t* cost = nullptr;
+  HIP_TRY(order_items(s->sky_order, order, cost, items, key, (hipStream_t)stream), "sky order buffers");
+  a.order = order;
+  a.item_cost = cost;
   HIP_TRY(cnq::launch_sky(a, s->count_work, (hipStream_t)stream), "sky launch");
+  HIP_TRY(reorder_items(s->sky_order, a.item_cost, items, (hipStream_t)stream), "sky order");
   return CNQ_OK;
 }
 

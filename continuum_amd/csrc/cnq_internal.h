@@ -59,8 +59,8 @@ struct cnq_device {
   uint32_t compute_units;  // hipDeviceProp_t::multiProcessorCount
 };
 
-// A view's items in the order of their last times (longest first), for
-// launching the same view again (cnq_view_irradiance_device).
+// A launch's items in the order of their last times (longest first), for
+// launching the same work again (a view's cone queries, a list's sky).
 struct ItemOrder {
   bool bound = false, ready = false;
   void* stream = nullptr;  // the one stream the order is read and written on
@@ -79,7 +79,8 @@ struct cnq_scene {
   cnq::detail::DevBuf<uint32_t> blas_items, ranges, items, parent, leaves, arrivals, counters;
   cnq::detail::DevBuf<uint64_t> work;
   cnq::detail::DevBuf<float> sky_faces;  // cnq_sky_irradiance_device's per-face values when the caller has none
-  ItemOrder order;
+  ItemOrder order;      // cnq_view_irradiance_device's
+  ItemOrder sky_order;  // cnq_sky_irradiance_device's
   cnq::detail::DevBuf<cnq::NodeC> blas, nodes;
   std::vector<uint32_t> h_ranges;  // for host-side checks and object rebuilds
   // Moving objects (instanced scenes): the subtree's layout, each geometry's
@@ -114,7 +115,7 @@ struct cnq_scene {
     tris.release(), tri_leaf.release(), blas_cov.release(), xforms.release(), mesh_box.release();
     blas_items.release(), ranges.release(), items.release(), parent.release(), leaves.release();
     arrivals.release(), counters.release(), work.release(), sky_faces.release(), order.cost.release(),
-        order.order.release(), blas.release(), nodes.release();
+        order.order.release(), sky_order.cost.release(), sky_order.order.release(), blas.release(), nodes.release();
     for (auto& kv : pose_leaves) kv.second.release();
     pose_leaves.clear();
   }

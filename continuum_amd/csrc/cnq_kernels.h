@@ -182,6 +182,10 @@ struct SkyArgs {
   float* __restrict__ out_faces;
   uint64_t* __restrict__ out_full;
   uint64_t* __restrict__ work;
+  // Wave w takes item order[w] when given; item times go to item_cost
+  // when given (as the cone kernel's direct mode).
+  const uint32_t* __restrict__ order;
+  uint32_t* __restrict__ item_cost;
   uint32_t n;
   QueryParams p;
   SkyParams sky;

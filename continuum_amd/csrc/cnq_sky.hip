@@ -20,8 +20,8 @@ __global__ __launch_bounds__(64, WALK_WAVES) void sky_items_kernel(
     const NodeC* __restrict__ blas, const NodeC* __restrict__ nodes, const float* __restrict__ tri_leaf,
     const float* __restrict__ tris, const uint32_t* __restrict__ ranges, const uint32_t* __restrict__ items,
     const float* __restrict__ blas_cov, const float* __restrict__ xforms, const float* __restrict__ receivers,
-    float* __restrict__ out_faces, uint64_t* __restrict__ out_full, uint64_t* __restrict__ work_out, uint32_t n,
-    QueryParams p, SkyParams sp) {
+    float* __restrict__ out_faces, uint64_t* __restrict__ out_full, uint64_t* __restrict__ work_out,
+    const uint32_t* __restrict__ order, uint32_t* __restrict__ item_cost, uint32_t n, QueryParams p, SkyParams sp) {
   __shared__ float s_acc[ACC_CELLS * 64];
   const uint32_t lane = lane_id();
   float poly[POLY_SLOTS];
@@ -29,7 +29,9 @@ __global__ __launch_bounds__(64, WALK_WAVES) void sky_items_kernel(
   const Ctx x{blas, nodes, tri_leaf, tris, ranges, items, nullptr, blas_cov, xforms,
               p.n_objects, p.hierarchy, p.profile_stage, p.object_cov};
   const uint32_t tiles = (n + 63u) / 64u;
-  const uint32_t face = blockIdx.x / tiles, tile = blockIdx.x - face * tiles;
+  const uint32_t item = order ? order[blockIdx.x] : blockIdx.x;
+  const uint64_t started = item_cost ? wall_clock64() : 0;
+  const uint32_t face = item / tiles, tile = item - face * tiles;
   const uint32_t i = tile * 64u + lane;
   const bool valid = i < n;
   const float* rc = receivers + 8 * (size_t)(valid ? i : 0u);
@@ -38,6 +40,10 @@ __global__ __launch_bounds__(64, WALK_WAVES) void sky_items_kernel(
   if (valid) {
     out_faces[(size_t)face * n + i] = o.value;
     if (out_full) out_full[(size_t)face * n + i] = o.full;
+  }
+  if (item_cost && lane == 0) {
+    const uint64_t t = wall_clock64() - started;
+    item_cost[item] = t < 0xffffffffull ? (uint32_t)t : 0xffffffffu;
   }
 }
 
@@ -59,11 +65,12 @@ hipError_t launch_sky(const SkyArgs& a, bool count_work, hipStream_t stream) {
   const dim3 grid(6u * tiles), block(64);
   if (count_work)
     hipLaunchKernelGGL(sky_items_kernel<true>, grid, block, 0, stream, s.blas, s.nodes, s.tri_leaf, s.tris, s.ranges,
-                       s.items, s.blas_cov, s.xforms, a.receivers, a.out_faces, a.out_full, a.work, a.n, a.p, a.sky);
+                       s.items, s.blas_cov, s.xforms, a.receivers, a.out_faces, a.out_full, a.work, a.order, a.item_cost,
+                       a.n, a.p, a.sky);
   else
     hipLaunchKernelGGL(sky_items_kernel<false>, grid, block, 0, stream, s.blas, s.nodes, s.tri_leaf, s.tris,
-                       s.ranges, s.items, s.blas_cov, s.xforms, a.receivers, a.out_faces, a.out_full, nullptr, a.n,
-                       a.p, a.sky);
+                       s.ranges, s.items, s.blas_cov, s.xforms, a.receivers, a.out_faces, a.out_full, nullptr, a.order,
+                       a.item_cost, a.n, a.p, a.sky);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   hipLaunchKernelGGL(sky_sum_kernel, dim3((a.n + 255u) / 256u), dim3(256), 0, stream, a.out_faces, a.out, a.n);

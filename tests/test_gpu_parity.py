@@ -216,3 +216,49 @@ def test_caps_and_wide_cones_match(device):
         _assert_same(gf[ok], rf[ok], "full (uncapped)")
         _assert_same(gv[ok], rv[ok], "visible (uncapped)")
         assert counters[0] >= 0
+
+
+def test_repeated_view_launches_take_the_longest_items_first_with_equal_results(device):
+    """The view launch records each item's time and orders the next launch
+    of the same view longest first (and the sky's the same way): the
+    ordered launches' results equal the first, natural-order launch's bit
+    for bit, as every item writes only its own queries; a launch on another
+    stream runs in natural order and agrees too."""
+    import torch
+
+    sc = rock_field(width=320, height=180)
+    tris9, counts = meshes_to_arrays(sc.meshes)
+    flat = Flat.build_arrays(tris9, counts)
+    scene = Scene(device, flat)
+    p = _lib.default_options(hierarchy=2, prefilter=sc.prefilter)
+    dev = torch.device("cuda:0")
+    w, h, nl = sc.width, sc.height, len(sc.lights)
+    cam = torch.from_numpy(sc.camera).to(dev)
+    lights = torch.from_numpy(np.ascontiguousarray(sc.lights)).to(dev)
+    hits = torch.zeros((w * h, 4), dtype=torch.int32, device=dev)
+    recv = torch.zeros((w * h, 8), dtype=torch.float32, device=dev)
+    scene.primary_device(cam, w, h, lights, p, hits, recv)
+    runs = []
+    other = torch.cuda.Stream()
+    for k in range(4):
+        vis = torch.zeros(w * h * nl, dtype=torch.float32, device=dev)
+        full = torch.zeros(w * h * nl, dtype=torch.int64, device=dev)
+        touched = torch.zeros(w * h * nl, dtype=torch.int64, device=dev)
+        stream = other if k == 3 else torch.cuda.current_stream()
+        if k == 3:
+            other.wait_stream(torch.cuda.current_stream())
+        scene.view_query_device(recv, w, h, lights, p, vis, full, touched, tile_w=16, stream=stream)
+        torch.cuda.synchronize()
+        runs.append((vis.cpu().numpy(), full.cpu().numpy(), touched.cpu().numpy()))
+    for k in range(1, 4):
+        for a, b, what in zip(runs[0], runs[k], ("visible", "full", "touched")):
+            assert np.array_equal(np.ascontiguousarray(a).view(np.uint32), np.ascontiguousarray(b).view(np.uint32)), \
+                f"launch {k}: {what} differs"
+    sky = []
+    for k in range(3):
+        out = torch.zeros(w * h, dtype=torch.float32, device=dev)
+        scene.sky_device(recv, p, out)
+        torch.cuda.synchronize()
+        sky.append(out.cpu().numpy())
+    assert np.array_equal(sky[0].view(np.uint32), sky[1].view(np.uint32))
+    assert np.array_equal(sky[0].view(np.uint32), sky[2].view(np.uint32))

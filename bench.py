@@ -408,11 +408,21 @@ def sky_form(scene, flat, sc, recv, h_recv, check: bool = True, stride: int = 64
         torch.cuda.synchronize()
         ms.append(e0.elapsed_time(e1))
     t = float(np.median(ms))
+    # the walks' work per face cone, from one more launch with the counters on
+    scene.counters(reset=True)
+    scene.set_count_work(True)
+    scene.sky_device(recv, opts, out, faces, full, stream=stream)
+    torch.cuda.synchronize()
+    _, work = scene.counters(reset=True)
+    scene.set_count_work(False)
     lit = int((h_recv[:, 7] >= 0).sum())
     res = {"workload": f"uniform sky over the rock field's {sc.width}x{sc.height} shading points, six cube-face "
                        "cones each (independent occluders, horizon cull, 1-bin prefilter)",
            "points": px, "traced_points": lit, "face_cones": 6 * px, "ms": t, "face_cones_per_s": 6 * px / t * 1e3,
-           "mean_sky_irradiance": float(out.mean().item())}
+           "mean_sky_irradiance": float(out.mean().item()),
+           "per_face_cone": {"object_nodes": int(work[0]) / (6 * px), "blas_nodes": int(work[1]) / (6 * px),
+                             "triangles_tested": int(work[2]) / (6 * px), "triangles_drawn": int(work[3]) / (6 * px),
+                             "meshes_entered": int(work[4]) / (6 * px), "splats": int(work[5]) / (6 * px)}}
     if check:
         from oracle import oracle as O
 

@@ -219,6 +219,12 @@ int cnq_sky_irradiance(cnq_scene* scene, const float* receivers, uint64_t n, con
 int cnq_view_irradiance_device(cnq_scene* scene, const float* receivers, uint32_t width, uint32_t height,
                                const float* lights, uint32_t n_lights, const cnq_options* opts, uint32_t tile_w,
                                float* out_visible, uint64_t* out_full, uint64_t* out_touched, void* stream);
+/* (Both this and cnq_sky_irradiance_device record each work item's time
+ * and launch the same view again, on the same stream, heaviest items first;
+ * the scene keeps that order for one view (receivers buffer and sizes) and
+ * binds it to the first stream it was used on, where launches see it in
+ * stream order. Launches on other streams run in natural order. The results
+ * do not depend on the order.) */
 
 /* Camera rays for a width x height view (the WGSL `visibility` entry,
  * mask_cones.wgsl:2227, with `camera_hit` :2222): per pixel, hits (mesh,

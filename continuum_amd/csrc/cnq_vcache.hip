@@ -70,7 +70,8 @@ struct VcArgs {
   uint32_t plan_pages, plan_jobs, append_pages, append_jobs;
   uint32_t candidate_cap, invalid_cap, table_mask;
   uint32_t frame, visible_frames, light_ratio;
-  uint32_t bake_group;                 // jobs a bake wave takes (a power of two, 4..64)
+  uint32_t bake_group;                 // most jobs a bake wave takes at a time (64)
+  uint32_t bake_resident;              // waves the GPU holds at once
   float inv_base;
 };
 
@@ -505,13 +506,25 @@ __global__ __launch_bounds__(64, WALK_WAVES) void vc_bake_kernel(VcArgs a, const
               p.n_objects, p.hierarchy, p.profile_stage, p.object_cov};
   const uint32_t appended = a.stats[5] < a.append_jobs ? a.stats[5] : a.append_jobs;
   const uint32_t total = a.plan_jobs + appended;
-  // A wave takes `bake_group` consecutive jobs (its first lanes): all 64 when
-  // the bake fills the GPU; fewer when it would leave it mostly idle, as a
-  // packet's walk visits the union of its lanes' nodes, and a narrower
-  // packet's union is shorter (the frame ends with its slowest wave).
-  const uint32_t group = a.bake_group;
-  for (uint32_t base = blockIdx.x * group; base < total; base += gridDim.x * group) {
-    const uint32_t j = base + lane;
+  // Waves take jobs from a counter (stats[7], zero at the frame's start),
+  // consecutive jobs on their first lanes: 64 at a time while many remain,
+  // and fewer as the bake nears its end (down to 4 once fewer jobs remain
+  // than 4 for each wave the GPU holds). A packet's walk visits the union
+  // of its lanes' nodes, and a narrower packet's union is shorter, so the
+  // last waves, which the frame waits for, finish sooner.
+  for (;;) {
+    uint32_t start = 0, group = 0;
+    if (lane == 0) {
+      const uint32_t done = __atomic_load_n(&a.stats[7], __ATOMIC_RELAXED);
+      const uint32_t left = done < total ? total - done : 0u;
+      group = a.bake_group;
+      while (group > 4u && (uint64_t)left < (uint64_t)group * a.bake_resident) group >>= 1;
+      start = atomicAdd(&a.stats[7], group);
+    }
+    start = uni(start);
+    group = uni(group);
+    if (start >= total) break;
+    const uint32_t j = start + lane;
     const bool mine = lane < group && j < total;
     const uint32_t* job = a.bake_jobs + 2 * (size_t)(mine ? j : total - 1u);
     const uint32_t jx = job[0], jy = job[1];
@@ -1039,6 +1052,7 @@ struct cnq_vcache {
     a.visible_frames = cfg.visible_frames;
     a.light_ratio = cfg.light_ratio;
     a.bake_group = 64;
+    a.bake_resident = 1;
     a.inv_base = inv_base;
     return a;
   }
@@ -1408,19 +1422,15 @@ int cnq_vcache_frame(cnq_vcache* c, void* stream, uint64_t stats[10]) {
     }
   }
   const cnq::QueryParams p = query_params(s, &c->opts, 64);
-  // Jobs per wave: 64, or with a bake that would leave most of the GPU idle
-  // (fewer waves than it holds: 4 SIMDs a compute unit, WALK_WAVES each),
-  // the fewest (down to 4) that still fit it. A frame with moves appends an
-  // unknown number of jobs, so it keeps 64.
+  // As many waves as the GPU holds at once (4 SIMDs a compute unit,
+  // WALK_WAVES each), or fewer for a small bake; they take jobs from a
+  // counter, fewer at a time near the end (vc_bake_kernel).
   const uint64_t jobs = (uint64_t)plan_jobs + (a.n_movers ? a.append_jobs : 0);
-  const uint64_t resident = (uint64_t)s->dev->compute_units * 4 * cnq::WALK_WAVES;
-  uint32_t group = 64;
-  if (!a.n_movers)
-    while (group > 4 && (jobs + group / 2 - 1) / (group / 2) <= resident) group /= 2;
-  a.bake_group = group;
-  const uint64_t job_waves = (jobs + group - 1) / group;
-  if (job_waves) {
-    const uint32_t grid = (uint32_t)std::min<uint64_t>(job_waves, 8192);
+  const uint32_t resident = s->dev->compute_units * 4 * (uint32_t)cnq::WALK_WAVES;
+  a.bake_group = 64;
+  a.bake_resident = resident;
+  if (jobs) {
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((jobs + 3) / 4, resident);
     if (s->count_work)
       hipLaunchKernelGGL(cnq::vc_bake_kernel<true>, dim3(grid), dim3(64), 0, st, a, s->blas.p, s->nodes.p,
                          s->tri_leaf.p, s->items.p, s->blas_items.p, s->blas_cov.p, p, s->work.p, s->counters.p);

@@ -441,6 +441,67 @@ def sky_form(scene, flat, sc, recv, h_recv, check: bool = True, stride: int = 64
     return res
 
 
+def lightmap_form(device, size: float = 2.0, texel: float = 0.04, boxes: int = 8, frames: int = 10) -> dict:
+    """The lightmap cache (the reference's `LiveRenderer` lighting: block
+    planes, `mark_blocks`, `bake_static`, `rebake`): an 8 x 8 x 6 m room of
+    large quads with a pillar and eight dynamic boxes under a ceiling rect,
+    lightmap texels of 4 cm on the static surfaces (8 x 8 texel blocks).
+    The cache is built (static coverage of every lit block and the first
+    bake), then two boxes move 4 cm a frame and each update re-bakes the
+    blocks their swept boxes may shadow. Times are wall clock around each
+    call (host uploads included). At the end every texel is compared with a
+    fresh query of the moved scene, bit for bit."""
+    import numpy as np
+
+    from continuum_amd import _lib
+    from continuum_amd.kernel import Flat, LiveLightmap, Scene
+    from continuum_amd.lightmap import Lightmap
+    from continuum_amd.scenes import lit_room, meshes_to_arrays
+
+    sc = lit_room(boxes=boxes, size=size)
+    dyn = sc.notes["dynamic"]
+    t0 = time.perf_counter()
+    lm = Lightmap.from_meshes(sc.meshes, texel, lambda m: not dyn[m])
+    t_host = time.perf_counter() - t0
+    tris9, counts = meshes_to_arrays(sc.meshes)
+    scene = Scene(device, Flat.build_arrays(tris9, counts))
+    light = sc.lights[0]
+    t0 = time.perf_counter()
+    cache = LiveLightmap(scene, lm.texels, lm.blocks, light, dyn)
+    t_build = time.perf_counter() - t0
+    opts = _lib.default_options(hierarchy=2)
+    t0 = time.perf_counter()
+    scene.query(lm.texels, light[None], opts)
+    t_full = time.perf_counter() - t0
+    off = [0] + [int(x) for x in np.cumsum(counts)]
+    moved = tris9.copy()
+    movers = [i for i, d in enumerate(dyn) if d][:2]
+    ms, rebaked = [], []
+    for k in range(frames):
+        changes = []
+        for j, m in enumerate(movers):
+            new = moved[off[m]:off[m + 1]].copy()
+            new[:, 0::3] += np.float32(0.04 if j == 0 else -0.03)
+            new[:, 1::3] += np.float32(0.02)
+            moved[off[m]:off[m + 1]] = new
+            changes.append((m, new))
+        t0 = time.perf_counter()
+        cache.update_meshes(changes)
+        _, marked = cache.read()
+        ms.append((time.perf_counter() - t0) * 1e3)
+        rebaked.append(len(marked))
+    got = cache.read_texels()
+    fresh, _, _, _ = Scene(device, Flat.build_arrays(moved, counts)).query(lm.texels, light[None], opts)
+    return {"workload": f"lit room {4 * size:g} x {4 * size:g} x {3 * size:g} m, {boxes} dynamic boxes, ceiling rect, "
+                        f"{texel * 100:g} cm texels on the static surfaces; two boxes move each frame",
+            "texels": int(len(lm.texels)), "blocks": int(len(lm.blocks)), "host_lightmap_s": round(t_host, 1),
+            "build_and_first_bake_ms": t_build * 1e3, "full_query_ms": t_full * 1e3,
+            "update_ms_mean": float(np.mean(ms[1:])), "update_ms": [round(x, 2) for x in ms],
+            "blocks_rebaked_mean": float(np.mean(rebaked)),
+            "blocks_rebaked_share": float(np.mean(rebaked)) / len(lm.blocks),
+            "texels_differ_from_a_fresh_query": int((got.view(np.uint32) != fresh.view(np.uint32)).sum())}
+
+
 def view_from_cache(device, scene, surf, lights, opts, eye, width: int = 1280, height: int = 720,
                     budget: int = 2_000_000, max_frames: int = 40) -> dict:
     """A 1280x720 view shaded from a fresh view cache (the reference's
@@ -521,7 +582,8 @@ def main() -> int:
     ap.add_argument("--no-served", action="store_true", help="skip the served form over the boulder field")
     ap.add_argument("--no-vcache", action="store_true", help="skip the view cache over the boulder field")
     ap.add_argument("--no-sky", action="store_true", help="skip the sky light over the rock field")
-    ap.add_argument("--only-form", choices=["courtyard", "served", "view_cache"],
+    ap.add_argument("--no-lightmap", action="store_true", help="skip the lightmap cache over the lit room")
+    ap.add_argument("--only-form", choices=["courtyard", "served", "view_cache", "lightmap"],
                     help="run one form alone and print its JSON (for profiling it); no headline line")
     ap.add_argument("--served-requests", type=int, default=64, help="requests in the saturated served run")
     ap.add_argument("--served-seconds", type=float, default=10.0, help="length of the steady served run")
@@ -546,6 +608,8 @@ def main() -> int:
             out = courtyard_forms(device, steps=5)
         elif args.only_form == "served":
             out = served_form(device, n_requests=args.served_requests, seconds=args.served_seconds)
+        elif args.only_form == "lightmap":
+            out = lightmap_form(device)
         else:
             out = view_cache_form(device)
         print(json.dumps({"form": args.only_form, **out}))
@@ -723,6 +787,8 @@ def main() -> int:
                                                                seconds=args.served_seconds)
     if not args.no_vcache:
         result.setdefault("forms", {})["view_cache"] = view_cache_form(device)
+    if not args.no_lightmap:
+        result.setdefault("forms", {})["lightmap"] = lightmap_form(device)
 
     print(json.dumps(result))
     return 0

@@ -225,15 +225,19 @@ hipError_t order_items(ItemOrder& o, const uint32_t*& order, uint32_t*& item_cos
     if (o.cost.n < items) {
       hipError_t e = hipStreamSynchronize(stream);  // no launch still reads the old buffers
       if (e != hipSuccess) return e;
-      o.cost.release(), o.order.release();
-      if ((e = o.cost.alloc(items)) != hipSuccess || (e = o.order.alloc(items)) != hipSuccess) {
-        o.cost.release(), o.order.release();
+      o.cost.release(), o.order.release(), o.last.release();
+      if ((e = o.cost.alloc(items)) != hipSuccess || (e = o.order.alloc(items)) != hipSuccess ||
+          (e = o.last.alloc((items + 3) / 4 * 4)) != hipSuccess) {
+        o.cost.release(), o.order.release(), o.last.release();
         o.ready = false;
         return e;
       }
     }
     for (int k = 0; k < 4; k++) o.key[k] = key[k];
     o.ready = false;
+    // No times yet (a fill kernel, not a memset, so a captured launch replays it).
+    hipError_t e = fill_u32(reinterpret_cast<uint32_t*>(o.last.p), 0xffffffffu, (items + 3) / 4, stream);
+    if (e != hipSuccess) return e;
   }
   order = o.ready ? o.order.p : nullptr;
   item_cost = o.cost.p;
@@ -242,7 +246,7 @@ hipError_t order_items(ItemOrder& o, const uint32_t*& order, uint32_t*& item_cos
 
 hipError_t reorder_items(ItemOrder& o, const uint32_t* item_cost, uint64_t items, hipStream_t stream) {
   if (!item_cost) return hipSuccess;
-  hipError_t e = cnq::launch_item_order(o.cost.p, o.order.p, (uint32_t)items, stream);
+  hipError_t e = cnq::launch_item_order(o.cost.p, o.last.p, o.order.p, (uint32_t)items, stream);
   if (e == hipSuccess) o.ready = true;
   return e;
 }
@@ -1080,6 +1084,7 @@ struct cnq_batch {
   // identity after a layout, then rewritten after every cone pass, so a
   // replayed batch starts its heaviest items first.
   DevBuf<uint32_t> item_cost, item_order;
+  DevBuf<uint8_t> item_last;
   DevBuf<uint64_t> full, touched;
   hipStream_t own_stream = nullptr;
   hipGraph_t graph = nullptr;
@@ -1093,7 +1098,7 @@ struct cnq_batch {
     exec = nullptr, graph = nullptr, own_stream = nullptr;
     views.release(), cameras.release(), lights.release(), receivers.release(), visible.release();
     hits.release(), work_counter.release(), full.release(), touched.release();
-    item_cost.release(), item_order.release();
+    item_cost.release(), item_order.release(), item_last.release();
   }
 };
 
@@ -1119,14 +1124,16 @@ int batch_layout(cnq_batch* b, uint32_t tile_w) {
   HIP_TRY(hipMemcpy(b->views.p, b->h_views.data(), b->h_views.size() * sizeof(cnq::ViewDesc), hipMemcpyHostToDevice),
           "view table");
   // A fresh identity order for the new items.
-  b->item_cost.release(), b->item_order.release();
+  b->item_cost.release(), b->item_order.release(), b->item_last.release();
   if (b->n_items <= 0x7fffffffull) {
     std::vector<uint32_t> ident(b->n_items);
     for (uint64_t i = 0; i < b->n_items; i++) ident[i] = (uint32_t)i;
+    const std::vector<uint8_t> none(b->n_items, 0xffu);
     hipError_t e = b->item_cost.alloc(b->n_items);
     if (e == hipSuccess) e = b->item_order.upload(ident.data(), ident.size());
+    if (e == hipSuccess) e = b->item_last.upload(none.data(), none.size());
     if (e != hipSuccess) {
-      b->item_cost.release(), b->item_order.release();
+      b->item_cost.release(), b->item_order.release(), b->item_last.release();
       return hip_fail(e, "item order");
     }
   }
@@ -1167,7 +1174,9 @@ int batch_enqueue(cnq_batch* b, const cnq_options* o, const cnq_tuning* t, hipSt
   } else {
     HIP_TRY(launch_items_direct(a, b->n_items, s->count_work, st), "batch cone pass");
   }
-  if (a.item_cost) HIP_TRY(cnq::launch_item_order(a.item_cost, b->item_order.p, (uint32_t)b->n_items, st), "item order");
+  if (a.item_cost)
+    HIP_TRY(cnq::launch_item_order(a.item_cost, b->item_last.p, b->item_order.p, (uint32_t)b->n_items, st),
+            "item order");
   return CNQ_OK;
 }
 

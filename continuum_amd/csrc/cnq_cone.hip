@@ -116,16 +116,54 @@ hipError_t launch_cone_items(const ItemArgs& a, bool count_work, uint32_t persis
 }
 
 // One block: items bucketed by the power of two of their time, buckets
-// longest first; within a bucket the order is the atomics'. A permutation of
-// [0, n) whatever the times.
+// longest first (within a bucket the order is the atomics'), when this
+// launch's heaviest items (the top two powers of two) were mostly heavy in
+// the last launch too (the same view, or one that moved a little), or when
+// there is no last launch; otherwise natural order, since another view's
+// order would only scatter
+// the waves that run together. Comparing the heavy sets, not every item's
+// time, keeps the decision stable where the order itself shifts the times
+// a little. `last` keeps each item's bucket (0xff: none yet). A permutation
+// of [0, n) whatever the times.
+__device__ __forceinline__ uint32_t time_bucket(uint32_t t) { return 32u - (uint32_t)__builtin_clz(t | 1u); }
+
 __global__ __launch_bounds__(1024) void item_order_kernel(const uint32_t* __restrict__ cost,
-                                                         uint32_t* __restrict__ order, uint32_t n) {
-  __shared__ uint32_t count[33], next[33];
+                                                         uint8_t* __restrict__ last, uint32_t* __restrict__ order,
+                                                         uint32_t n) {
+  __shared__ uint32_t count[33], next[33], top_new, top_old, unknown, heavy, kept;
   const uint32_t t = threadIdx.x;
   if (t < 33) count[t] = 0;
+  if (t == 0) top_new = 0, top_old = 0, unknown = 0, heavy = 0, kept = 0;
   __syncthreads();
-  for (uint32_t i = t; i < n; i += 1024u) atomicAdd(&count[32u - (uint32_t)__builtin_clz(cost[i] | 1u)], 1u);
+  uint32_t my_top_new = 0, my_top_old = 0, my_unknown = 0;
+  for (uint32_t i = t; i < n; i += 1024u) {
+    const uint32_t b = time_bucket(cost[i]), was = last[i];
+    my_top_new = b > my_top_new ? b : my_top_new;
+    if (was == 0xffu)
+      my_unknown++;
+    else
+      my_top_old = was > my_top_old ? was : my_top_old;
+    atomicAdd(&count[b], 1u);
+  }
+  atomicMax(&top_new, my_top_new);
+  atomicMax(&top_old, my_top_old);
+  atomicAdd(&unknown, my_unknown);
   __syncthreads();
+  uint32_t my_heavy = 0, my_kept = 0;
+  for (uint32_t i = t; i < n; i += 1024u) {
+    const uint32_t b = time_bucket(cost[i]), was = last[i];
+    if (b + 1u >= top_new) {
+      my_heavy++;
+      if (was != 0xffu && was + 1u >= top_old) my_kept++;
+    }
+    last[i] = (uint8_t)b;
+  }
+  atomicAdd(&heavy, my_heavy);
+  atomicAdd(&kept, my_kept);
+  __syncthreads();
+  // After the first launch of a view (no history) the next is taken to be
+  // the same view; after that the heavy sets decide.
+  const bool similar = unknown == n || (unknown == 0 && 2u * kept >= heavy);
   if (t == 0) {
     uint32_t at = 0;
     for (int b = 32; b >= 0; b--) {
@@ -135,14 +173,18 @@ __global__ __launch_bounds__(1024) void item_order_kernel(const uint32_t* __rest
   }
   __syncthreads();
   for (uint32_t i = t; i < n; i += 1024u) {
-    const uint32_t pos = atomicAdd(&next[32u - (uint32_t)__builtin_clz(cost[i] | 1u)], 1u);
-    order[pos] = i;
+    if (similar) {
+      const uint32_t pos = atomicAdd(&next[time_bucket(cost[i])], 1u);
+      order[pos] = i;
+    } else {
+      order[i] = i;
+    }
   }
 }
 
-hipError_t launch_item_order(const uint32_t* cost, uint32_t* order, uint32_t n, hipStream_t stream) {
+hipError_t launch_item_order(const uint32_t* cost, uint8_t* last, uint32_t* order, uint32_t n, hipStream_t stream) {
   if (n == 0) return hipSuccess;
-  hipLaunchKernelGGL(item_order_kernel, dim3(1), dim3(1024), 0, stream, cost, order, n);
+  hipLaunchKernelGGL(item_order_kernel, dim3(1), dim3(1024), 0, stream, cost, last, order, n);
   return hipGetLastError();
 }
 

@@ -66,6 +66,7 @@ struct ItemOrder {
   void* stream = nullptr;  // the one stream the order is read and written on
   uint64_t key[4] = {0, 0, 0, 0};
   cnq::detail::DevBuf<uint32_t> cost, order;
+  cnq::detail::DevBuf<uint8_t> last;  // each item's last time, as a power of two (0xff: none)
 };
 
 struct cnq_scene {
@@ -115,7 +116,8 @@ struct cnq_scene {
     tris.release(), tri_leaf.release(), blas_cov.release(), xforms.release(), mesh_box.release();
     blas_items.release(), ranges.release(), items.release(), parent.release(), leaves.release();
     arrivals.release(), counters.release(), work.release(), sky_faces.release(), order.cost.release(),
-        order.order.release(), sky_order.cost.release(), sky_order.order.release(), blas.release(), nodes.release();
+        order.order.release(), order.last.release(), sky_order.cost.release(), sky_order.order.release(),
+        sky_order.last.release(), blas.release(), nodes.release();
     for (auto& kv : pose_leaves) kv.second.release();
     pose_leaves.clear();
   }

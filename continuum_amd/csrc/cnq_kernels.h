@@ -194,8 +194,10 @@ struct SkyArgs {
 hipError_t launch_cone_items(const ItemArgs& a, bool count_work, uint32_t persistent_blocks, hipStream_t stream);
 hipError_t launch_sky(const SkyArgs& a, bool count_work, hipStream_t stream);
 // The next launch's order from this one's item times: longest first, by
-// powers of two (one block; the order within a power of two is arbitrary).
-hipError_t launch_item_order(const uint32_t* cost, uint32_t* order, uint32_t n, hipStream_t stream);
+// powers of two, when the times resemble the last launch's (`last`: each
+// item's last power of two, 0xff for none), else natural order (one block;
+// the order within a power of two is arbitrary).
+hipError_t launch_item_order(const uint32_t* cost, uint8_t* last, uint32_t* order, uint32_t n, hipStream_t stream);
 hipError_t launch_primary(const PrimaryArgs& a, hipStream_t stream);
 hipError_t launch_refit(const RefitArgs& a, hipStream_t stream);
 hipError_t launch_bake_static(const CacheArgs& a, hipStream_t stream);

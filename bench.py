@@ -375,6 +375,10 @@ def view_cache_form(device, boulders: int = 512, moves: int = 5) -> dict:
                            rebaked_cones=st["appended_jobs"], evicted=st["evicted"], overflow=st["overflow"]))
     out["moves"] = {"movers": len(movers), "shift_m": 0.25, "frames": frames,
                     "ms_mean": float(np.mean([f["ms"] for f in frames]))}
+    # The reference's invariant after the moves: every page shown equals a
+    # fresh bake of the moved scene, bit for bit (`verify_cache`).
+    differ, compared = vc.verify()
+    out["moves"]["verify"] = {"values_compared": compared, "values_differ": differ}
     out["view"] = view_from_cache(device, scene, surf, vlights, opts, eye)
     return out
 

@@ -916,6 +916,23 @@ __global__ __launch_bounds__(256) void vc_request_gather_kernel(ShadeArgs s) {
   r[3] = s.request_prio[r[3]];
 }
 
+// `verify_cache`'s comparison (instanced.rs:3721-3729): the f32 bit
+// patterns of each listed slot's texels in the pool and in a fresh bake,
+// the differing ones counted (one wave per slot).
+__global__ __launch_bounds__(64) void vc_compare_kernel(const float* __restrict__ pool,
+                                                        const float* __restrict__ fresh,
+                                                        const uint32_t* __restrict__ slots, uint32_t n,
+                                                        uint32_t per_slot, unsigned long long* __restrict__ differ) {
+  for (uint32_t p = blockIdx.x; p < n; p += gridDim.x) {
+    const size_t base = (size_t)slots[p] * per_slot;
+    uint32_t d = 0;
+    for (uint32_t i = threadIdx.x; i < per_slot; i += 64u)
+      d += __float_as_uint(pool[base + i]) != __float_as_uint(fresh[base + i]) ? 1u : 0u;
+    for (int off = 32; off > 0; off >>= 1) d += (uint32_t)__shfl_xor((int)d, off, 64);
+    if (threadIdx.x == 0 && d) atomicAdd(differ, (unsigned long long)d);
+  }
+}
+
 // WGSL :3134-3160 (`vc_find`, a page shown unless marked), then the page's
 // texels: one wave per key.
 __global__ __launch_bounds__(64) void vc_read_kernel(VcArgs a, const uint32_t* __restrict__ keys, uint32_t n_keys,
@@ -1513,6 +1530,103 @@ int cnq_vcache_read(cnq_vcache* c, const uint32_t* keys, uint32_t n, float* texe
   if (e == hipSuccess) e = hipMemcpy(slots, ds.p, 4 * (size_t)n, hipMemcpyDeviceToHost);
   dk.release(), ds.release(), dt.release();
   if (e != hipSuccess) return hip_fail(e, "view cache read");
+  return CNQ_OK;
+}
+
+int cnq_vcache_verify(cnq_vcache* c, void* stream, uint64_t out[2]) {
+  if (!c || !out) return fail(CNQ_INVALID_ARG, "null argument");
+  if (!c->plan_list.empty()) return fail(CNQ_INVALID_ARG, "pages are planned but not baked: run cnq_vcache_frame first");
+  cnq_scene* s = c->scene;
+  HIP_TRY(hipSetDevice(s->dev->ordinal), "hipSetDevice");
+  hipStream_t st = (hipStream_t)stream;
+  out[0] = out[1] = 0;
+  // The pages shown: resident and not marked stale or invalid.
+  std::vector<uint32_t> marks(c->slot_extent);
+  HIP_TRY(hipStreamSynchronize(st), "verify");
+  if (c->slot_extent)
+    HIP_TRY(hipMemcpy(marks.data(), c->d_slot_stale.p, 4 * (size_t)c->slot_extent, hipMemcpyDeviceToHost), "marks");
+  std::vector<uint32_t> list, jobs, slots;
+  std::vector<float> bounds;
+  for (const auto& kv : c->map) {
+    const uint32_t slot = kv.second.first;
+    if (slot >= c->slot_extent || marks[slot] != 0) continue;
+    const uint32_t* k = kv.first.k;
+    const uint32_t m = k[0], chart = k[1] & vc::KEY_CHART_MASK, mip = k[1] >> vc::KEY_MIP_SHIFT;
+    const uint32_t pp[2] = {k[2] & 0xffffu, k[2] >> 16};
+    uint32_t g[2], listed[2];
+    vc::grid(c->tables.charts[chart], c->scale[m], c->inv_base, mip, g);
+    for (int ax = 0; ax < 2; ax++) {
+      const uint32_t nn = g[ax] + 1;
+      const uint32_t inside = std::min(nn - std::min(pp[ax] * vc::PAGE, nn), vc::STRIDE);
+      listed[ax] = std::min(inside + 1, vc::STRIDE);
+    }
+    const uint32_t index = (uint32_t)(list.size() / 4);
+    list.insert(list.end(), {k[0], k[1], k[2], slot});
+    float b[8];
+    vc::page_bounds(c->tables, chart, c->scale[m], c->inv_base, mip, pp[0], pp[1], b);
+    b[3] = 0.0f;
+    b[7] = 0.0f;
+    bounds.insert(bounds.end(), b, b + 8);
+    for (uint32_t l = 0; l < c->n_lights; l++)
+      for (uint32_t y = 0; y < listed[1]; y++)
+        for (uint32_t x = 0; x < listed[0]; x++) jobs.insert(jobs.end(), {index, (y * vc::STRIDE + x) | (l << 8)});
+    slots.push_back(slot);
+  }
+  const uint32_t n = (uint32_t)slots.size(), per_slot = vc::PAGE_TEXELS * c->channels;
+  if (n == 0) return CNQ_OK;
+  if (jobs.size() / 2 > 0xffffffffull) return fail(CNQ_CAPACITY, "too many texels to verify at once");
+  DevBuf<float> fresh, dbounds;
+  DevBuf<uint32_t> dlist, djobs, dslots, dstats;
+  DevBuf<unsigned long long> differ;
+  auto cleanup = [&]() {
+    fresh.release(), dbounds.release(), dlist.release(), djobs.release(), dslots.release(), dstats.release();
+    differ.release();
+  };
+  const uint32_t zeros[cnq::VC_STATS] = {0};
+  const unsigned long long zero = 0;
+  hipError_t e = fresh.alloc(c->d_pool.n);
+  if (e == hipSuccess) e = hipMemcpyAsync(fresh.p, c->d_pool.p, c->d_pool.bytes(), hipMemcpyDeviceToDevice, st);
+  if (e == hipSuccess) e = dbounds.upload(bounds.data(), bounds.size());
+  if (e == hipSuccess) e = dlist.upload(list.data(), list.size());
+  if (e == hipSuccess) e = djobs.upload(jobs.data(), jobs.size());
+  if (e == hipSuccess) e = dslots.upload(slots.data(), slots.size());
+  if (e == hipSuccess) e = dstats.upload(zeros, cnq::VC_STATS);
+  if (e == hipSuccess) e = differ.upload(&zero, 1);
+  if (e != hipSuccess) {
+    cleanup();
+    return hip_fail(e, "verify buffers");
+  }
+  cnq::VcArgs a = c->args();
+  a.pool = fresh.p;
+  a.bake_list = dlist.p;
+  a.bake_bounds = dbounds.p;
+  a.bake_jobs = djobs.p;
+  a.stats = dstats.p;
+  a.n_movers = 0;
+  a.plan_pages = n;
+  a.plan_jobs = (uint32_t)(jobs.size() / 2);
+  a.append_pages = 0;
+  a.append_jobs = 0;
+  const uint32_t resident = s->dev->compute_units * 4 * (uint32_t)cnq::WALK_WAVES;
+  a.bake_group = 64;
+  a.bake_resident = resident;
+  const cnq::QueryParams p = query_params(s, &c->opts, 64);
+  const uint32_t grid = (uint32_t)std::min<uint64_t>((a.plan_jobs + 3) / 4, resident);
+  hipLaunchKernelGGL(cnq::vc_bake_kernel<false>, dim3(grid), dim3(64), 0, st, a, s->blas.p, s->nodes.p, s->tri_leaf.p,
+                     s->items.p, s->blas_items.p, s->blas_cov.p, p, nullptr, nullptr);
+  e = hipGetLastError();
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(cnq::vc_compare_kernel, dim3(std::min<uint32_t>(n, 4096)), dim3(64), 0, st, c->d_pool.p,
+                       fresh.p, dslots.p, n, per_slot, differ.p);
+    e = hipGetLastError();
+  }
+  unsigned long long d = 0;
+  if (e == hipSuccess) e = hipMemcpyAsync(&d, differ.p, sizeof(d), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  cleanup();
+  if (e != hipSuccess) return hip_fail(e, "verify");
+  out[0] = d;
+  out[1] = (uint64_t)n * per_slot;
   return CNQ_OK;
 }
 

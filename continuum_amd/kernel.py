@@ -710,6 +710,13 @@ class ViewCache:
         check(lib().cnq_vcache_plan(self.handle, int(budget), _ptr(st)))
         return dict(requests=int(st[0]), planned=int(st[1]), cones=int(st[2]), evicted=int(st[3]))
 
+    def verify(self, stream=None) -> tuple:
+        """`verify_cache` (instanced.rs:3721): every page shown baked afresh
+        and compared bit for bit; (values that differ, values compared)."""
+        out = np.zeros(2, np.uint64)
+        check(lib().cnq_vcache_verify(self.handle, _stream_ptr(stream), _ptr(out)))
+        return int(out[0]), int(out[1])
+
     def state(self) -> dict:
         sizes = np.zeros(5, np.uint32)
         check(lib().cnq_vcache_state(self.handle, 0, 0, 0, 0, 0, 0, _ptr(sizes)))

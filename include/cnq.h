@@ -519,6 +519,11 @@ int cnq_vcache_shade(cnq_vcache* cache, const float* camera, uint32_t width, uin
  * planned pages bake in the next cnq_vcache_frame. stats: [requests, pages
  * planned, cones planned, pages evicted]. */
 int cnq_vcache_plan(cnq_vcache* cache, uint32_t budget, uint64_t stats[4]);
+/* `verify_cache` (instanced.rs:3721-3729): every page shown (resident, not
+ * marked stale or invalid) baked afresh into a copy of the pool and compared
+ * with the pool by f32 bit pattern. out: [values that differ, values
+ * compared]. Refuses to run while planned pages wait for their bake. */
+int cnq_vcache_verify(cnq_vcache* cache, void* stream, uint64_t out[2]);
 /* Diagnostics (any pointer may be NULL): slot info (12 per slot), stale
  * marks, last use, the pool (81 x channels per slot), the device and host
  * page tables (4 per entry); sizes: [pages, channels, table entries, slots

@@ -229,3 +229,28 @@ def test_shading_and_planning_match_the_oracle(device):
         assert vc.frame() == ov.frame()
         _same_state(vc, ov, f"shade frame {frame}")
     assert counts[1] == 0  # every pixel found a page
+
+
+def test_verify_finds_every_shown_page_current(device):
+    """`verify_cache` (instanced.rs:3721-3729): every page shown baked afresh
+    equals the pool bit for bit, after the first bake and after moves (the
+    reference's invariant); it refuses to run while planned pages wait."""
+    w = _setup(device)
+    vc = w["vc"]
+    keys = _keys(vc, w["pl"])
+    vc.request(keys)
+    with pytest.raises(_lib.CnqError):
+        vc.verify()
+    vc.frame()
+    shown = int((vc.read(keys)[1] != O.NONE).sum())
+    differ, compared = vc.verify()
+    print(f"verify after the bake: {differ} of {compared}", flush=True)
+    assert differ == 0 and compared == shown * 81 * vc.channels
+    for shifts in ([1.5, 0.5, 0.0], [-0.7, 1.0, 0.0]), ([0.0, -2.0, 0.0], [1.2, 0.3, 0.0]):
+        g, moves, box = _move(w, shifts)
+        vc.set_moved(moves, box)
+        vc.request(keys)
+        vc.frame()
+        differ, compared = vc.verify()
+        print(f"verify after a move: {differ} of {compared}", flush=True)
+        assert differ == 0 and compared > 0

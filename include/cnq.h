@@ -4,7 +4,7 @@
  * (crates/continuum-gpu/src/lib.rs:65) and the hierarchy it walks
  * (crates/continuum-gpu/src/hierarchy.rs). The reference has no FFI; these
  * are the entry points a Rust backend crate beside crates/continuum-gpu
- * binds, with the reference's data layouts:
+ * would bind, with the reference's data layouts:
  *
  *   receiver  8 x f32   ReceiverGpu (lib.rs:56-59): point.xyz, _, normal.xyz, w
  *                       (w < 0: trace nothing; the reference packs w = 0)

@@ -224,7 +224,9 @@ int cnq_view_irradiance_device(cnq_scene* scene, const float* receivers, uint32_
  * the scene keeps that order for one view (receivers buffer and sizes) and
  * binds it to the first stream it was used on, where launches see it in
  * stream order. Launches on other streams run in natural order. The results
- * do not depend on the order.) */
+ * do not depend on the order. The first launch of a larger view allocates
+ * the order buffers and waits for its stream, so a caller that captures
+ * these launches in a graph launches the view once before capturing.) */
 
 /* Camera rays for a width x height view (the WGSL `visibility` entry,
  * mask_cones.wgsl:2227, with `camera_hit` :2222): per pixel, hits (mesh,

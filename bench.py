@@ -379,6 +379,16 @@ def view_cache_form(device, boulders: int = 512, moves: int = 5) -> dict:
     # fresh bake of the moved scene, bit for bit (`verify_cache`).
     differ, compared = vc.verify()
     out["moves"]["verify"] = {"values_compared": compared, "values_differ": differ}
+    # A light edit: the nearest rect light moved 0.5 m, its channel of every
+    # resident page baked afresh; then verified again.
+    row = np.array(vc.lights[1], np.float32, copy=True)
+    row[0:2] += np.float32(0.5)
+    t0 = time.perf_counter()
+    st = vc.set_lights(row[None], 1)
+    dt = time.perf_counter() - t0
+    differ, compared = vc.verify()
+    out["light_edit"] = {"pages": st["pages"], "cones": st["cones"], "ms": dt * 1e3,
+                         "verify": {"values_compared": compared, "values_differ": differ}}
     out["view"] = view_from_cache(device, scene, surf, vlights, opts, eye)
     return out
 

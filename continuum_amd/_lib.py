@@ -190,6 +190,7 @@ SIGNATURES = {
     "cnq_vcache_state": (C.c_int, [_P, _P, _P, _P, _P, _P, _P, _P]),
     "cnq_vcache_shade": (C.c_int, [_P, _P, C.c_uint32, C.c_uint32, _P, _P, _P, C.POINTER(VcacheShadeParams), _P, _P]),
     "cnq_vcache_verify": (C.c_int, [_P, _P, _P]),
+    "cnq_vcache_set_lights": (C.c_int, [_P, _P, C.c_uint32, C.c_uint32, _P, _P]),
     "cnq_vcache_plan": (C.c_int, [_P, C.c_uint32, _P]),
 }
 

@@ -916,6 +916,30 @@ __global__ __launch_bounds__(256) void vc_request_gather_kernel(ShadeArgs s) {
   r[3] = s.request_prio[r[3]];
 }
 
+// The border texels of each listed page (entry p of the bake list) for
+// channels [ch0, ch1): copied from their nearest inside texel, as
+// vc_fill_kernel copies every channel (one wave per page).
+__global__ __launch_bounds__(64) void vc_border_kernel(VcArgs a, uint32_t n, uint32_t ch0, uint32_t ch1) {
+  const uint32_t lane = lane_id();
+  for (uint32_t p = blockIdx.x; p < n; p += gridDim.x) {
+    const uint32_t* e = a.bake_list + 4 * (size_t)p;
+    const uint32_t c = e[1] & vc::KEY_CHART_MASK, mip = e[1] >> vc::KEY_MIP_SHIFT;
+    uint32_t g[2];
+    vc_grid(a, c, e[0], mip, g);
+    const uint32_t px = (e[2] & 0xffffu) * vc::PAGE, py = (e[2] >> 16) * vc::PAGE;
+    float* o = a.pool + (size_t)e[3] * VC_TEXELS * a.channels;
+    for (uint32_t k = lane; k < VC_TEXELS; k += 64u) {
+      const uint32_t tx = px + k % VC_STRIDE, ty = py + k / VC_STRIDE;
+      if (tx >= g[0] || ty >= g[1]) {
+        const uint32_t ix = (tx < g[0] - 1u ? tx : g[0] - 1u) - px;
+        const uint32_t iy = (ty < g[1] - 1u ? ty : g[1] - 1u) - py;
+        const uint32_t nearest = iy * VC_STRIDE + ix;
+        for (uint32_t ch = ch0; ch < ch1; ch++) o[k * a.channels + ch] = o[nearest * a.channels + ch];
+      }
+    }
+  }
+}
+
 // `verify_cache`'s comparison (instanced.rs:3721-3729): the f32 bit
 // patterns of each listed slot's texels in the pool and in a fresh bake,
 // the differing ones counted (one wave per slot).
@@ -1533,6 +1557,119 @@ int cnq_vcache_read(cnq_vcache* c, const uint32_t* keys, uint32_t n, float* texe
   return CNQ_OK;
 }
 
+namespace {
+
+// Bake lists for `pages` (key, slot) of the cache: per page its entry and
+// box, and a job per listed texel (inside the grid and one past) for each
+// light in [l0, l1), as cnq_vcache_request lists a new page's.
+struct BakeLists {
+  std::vector<uint32_t> list, jobs, slots;
+  std::vector<float> bounds;
+};
+
+void list_pages(const cnq_vcache* c, const std::vector<std::pair<Key, uint32_t>>& pages, uint32_t l0, uint32_t l1,
+                BakeLists& b) {
+  for (const auto& pg : pages) {
+    const uint32_t* k = pg.first.k;
+    const uint32_t slot = pg.second;
+    const uint32_t m = k[0], chart = k[1] & vc::KEY_CHART_MASK, mip = k[1] >> vc::KEY_MIP_SHIFT;
+    const uint32_t pp[2] = {k[2] & 0xffffu, k[2] >> 16};
+    uint32_t g[2], listed[2];
+    vc::grid(c->tables.charts[chart], c->scale[m], c->inv_base, mip, g);
+    for (int ax = 0; ax < 2; ax++) {
+      const uint32_t nn = g[ax] + 1;
+      const uint32_t inside = std::min(nn - std::min(pp[ax] * vc::PAGE, nn), vc::STRIDE);
+      listed[ax] = std::min(inside + 1, vc::STRIDE);
+    }
+    const uint32_t index = (uint32_t)(b.list.size() / 4);
+    b.list.insert(b.list.end(), {k[0], k[1], k[2], slot});
+    float bb[8];
+    vc::page_bounds(c->tables, chart, c->scale[m], c->inv_base, mip, pp[0], pp[1], bb);
+    bb[3] = 0.0f;
+    bb[7] = 0.0f;
+    b.bounds.insert(b.bounds.end(), bb, bb + 8);
+    for (uint32_t l = l0; l < l1; l++)
+      for (uint32_t y = 0; y < listed[1]; y++)
+        for (uint32_t x = 0; x < listed[0]; x++) b.jobs.insert(b.jobs.end(), {index, (y * vc::STRIDE + x) | (l << 8)});
+    b.slots.push_back(slot);
+  }
+}
+
+// The bake of `b` into `pool` on `st` (device copies of the lists kept in
+// the DevBufs given), then the border texels of channels [ch0, ch1) when
+// ch0 < ch1.
+int bake_lists(cnq_vcache* c, const BakeLists& b, float* pool, hipStream_t st, uint32_t ch0, uint32_t ch1,
+               DevBuf<float>& dbounds, DevBuf<uint32_t>& dlist, DevBuf<uint32_t>& djobs, DevBuf<uint32_t>& dstats) {
+  cnq_scene* s = c->scene;
+  const uint32_t zeros[cnq::VC_STATS] = {0};
+  hipError_t e = dbounds.upload(b.bounds.data(), b.bounds.size());
+  if (e == hipSuccess) e = dlist.upload(b.list.data(), b.list.size());
+  if (e == hipSuccess) e = djobs.upload(b.jobs.data(), b.jobs.size());
+  if (e == hipSuccess) e = dstats.upload(zeros, cnq::VC_STATS);
+  if (e != hipSuccess) return hip_fail(e, "bake lists");
+  cnq::VcArgs a = c->args();
+  a.pool = pool;
+  a.bake_list = dlist.p;
+  a.bake_bounds = dbounds.p;
+  a.bake_jobs = djobs.p;
+  a.stats = dstats.p;
+  a.n_movers = 0;
+  a.plan_pages = (uint32_t)b.slots.size();
+  a.plan_jobs = (uint32_t)(b.jobs.size() / 2);
+  a.append_pages = 0;
+  a.append_jobs = 0;
+  const uint32_t resident = s->dev->compute_units * 4 * (uint32_t)cnq::WALK_WAVES;
+  a.bake_group = 64;
+  a.bake_resident = resident;
+  const cnq::QueryParams p = query_params(s, &c->opts, 64);
+  if (a.plan_jobs) {
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((a.plan_jobs + 3) / 4, resident);
+    if (s->count_work)
+      hipLaunchKernelGGL(cnq::vc_bake_kernel<true>, dim3(grid), dim3(64), 0, st, a, s->blas.p, s->nodes.p,
+                         s->tri_leaf.p, s->items.p, s->blas_items.p, s->blas_cov.p, p, s->work.p, s->counters.p);
+    else
+      hipLaunchKernelGGL(cnq::vc_bake_kernel<false>, dim3(grid), dim3(64), 0, st, a, s->blas.p, s->nodes.p,
+                         s->tri_leaf.p, s->items.p, s->blas_items.p, s->blas_cov.p, p, nullptr, nullptr);
+    HIP_TRY(hipGetLastError(), "bake launch");
+  }
+  if (ch0 < ch1 && a.plan_pages) {
+    hipLaunchKernelGGL(cnq::vc_border_kernel, dim3(std::min<uint32_t>(a.plan_pages, 4096)), dim3(64), 0, st, a,
+                       a.plan_pages, ch0, ch1);
+    HIP_TRY(hipGetLastError(), "border launch");
+  }
+  return CNQ_OK;
+}
+
+}  // namespace
+
+int cnq_vcache_set_lights(cnq_vcache* c, const float* lights, uint32_t first, uint32_t n, void* stream,
+                          uint64_t stats[2]) {
+  if (!c || (n && !lights)) return fail(CNQ_INVALID_ARG, "null argument");
+  if ((uint64_t)first + n > c->n_lights) return fail(CNQ_INVALID_ARG, "lights past the cache's light count");
+  if (stats) stats[0] = stats[1] = 0;
+  if (n == 0) return CNQ_OK;
+  HIP_TRY(hipSetDevice(c->scene->dev->ordinal), "hipSetDevice");
+  hipStream_t st = (hipStream_t)stream;
+  HIP_TRY(hipStreamSynchronize(st), "light edit");  // no pass on the stream still reads the old lights
+  std::copy(lights, lights + 16 * (size_t)n, c->lights.begin() + 16 * (size_t)first);
+  for (uint32_t l = first; l < first + n; l++) c->light_boxes[l] = vc::light_box(&c->lights[16 * (size_t)l]);
+  HIP_TRY(hipMemcpy(c->d_lights.p + 16 * (size_t)first, lights, 64 * (size_t)n, hipMemcpyHostToDevice), "lights");
+  // Every resident page's channels of these lights, baked afresh.
+  std::vector<std::pair<Key, uint32_t>> pages;
+  for (const auto& kv : c->map) pages.push_back({kv.first, kv.second.first});
+  BakeLists b;
+  list_pages(c, pages, first, first + n, b);
+  DevBuf<float> dbounds;
+  DevBuf<uint32_t> dlist, djobs, dstats;
+  int rc = bake_lists(c, b, c->d_pool.p, st, first, first + n, dbounds, dlist, djobs, dstats);
+  hipError_t e = hipStreamSynchronize(st);
+  dbounds.release(), dlist.release(), djobs.release(), dstats.release();
+  if (rc != CNQ_OK) return rc;
+  if (e != hipSuccess) return hip_fail(e, "light edit");
+  if (stats) stats[0] = b.slots.size(), stats[1] = b.jobs.size() / 2;
+  return CNQ_OK;
+}
+
 int cnq_vcache_verify(cnq_vcache* c, void* stream, uint64_t out[2]) {
   if (!c || !out) return fail(CNQ_INVALID_ARG, "null argument");
   if (!c->plan_list.empty()) return fail(CNQ_INVALID_ARG, "pages are planned but not baked: run cnq_vcache_frame first");
@@ -1545,36 +1682,16 @@ int cnq_vcache_verify(cnq_vcache* c, void* stream, uint64_t out[2]) {
   HIP_TRY(hipStreamSynchronize(st), "verify");
   if (c->slot_extent)
     HIP_TRY(hipMemcpy(marks.data(), c->d_slot_stale.p, 4 * (size_t)c->slot_extent, hipMemcpyDeviceToHost), "marks");
-  std::vector<uint32_t> list, jobs, slots;
-  std::vector<float> bounds;
+  std::vector<std::pair<Key, uint32_t>> pages;
   for (const auto& kv : c->map) {
     const uint32_t slot = kv.second.first;
-    if (slot >= c->slot_extent || marks[slot] != 0) continue;
-    const uint32_t* k = kv.first.k;
-    const uint32_t m = k[0], chart = k[1] & vc::KEY_CHART_MASK, mip = k[1] >> vc::KEY_MIP_SHIFT;
-    const uint32_t pp[2] = {k[2] & 0xffffu, k[2] >> 16};
-    uint32_t g[2], listed[2];
-    vc::grid(c->tables.charts[chart], c->scale[m], c->inv_base, mip, g);
-    for (int ax = 0; ax < 2; ax++) {
-      const uint32_t nn = g[ax] + 1;
-      const uint32_t inside = std::min(nn - std::min(pp[ax] * vc::PAGE, nn), vc::STRIDE);
-      listed[ax] = std::min(inside + 1, vc::STRIDE);
-    }
-    const uint32_t index = (uint32_t)(list.size() / 4);
-    list.insert(list.end(), {k[0], k[1], k[2], slot});
-    float b[8];
-    vc::page_bounds(c->tables, chart, c->scale[m], c->inv_base, mip, pp[0], pp[1], b);
-    b[3] = 0.0f;
-    b[7] = 0.0f;
-    bounds.insert(bounds.end(), b, b + 8);
-    for (uint32_t l = 0; l < c->n_lights; l++)
-      for (uint32_t y = 0; y < listed[1]; y++)
-        for (uint32_t x = 0; x < listed[0]; x++) jobs.insert(jobs.end(), {index, (y * vc::STRIDE + x) | (l << 8)});
-    slots.push_back(slot);
+    if (slot < c->slot_extent && marks[slot] == 0) pages.push_back({kv.first, slot});
   }
-  const uint32_t n = (uint32_t)slots.size(), per_slot = vc::PAGE_TEXELS * c->channels;
-  if (n == 0) return CNQ_OK;
-  if (jobs.size() / 2 > 0xffffffffull) return fail(CNQ_CAPACITY, "too many texels to verify at once");
+  if (pages.empty()) return CNQ_OK;
+  BakeLists b;
+  list_pages(c, pages, 0, c->n_lights, b);
+  if (b.jobs.size() / 2 > 0xffffffffull) return fail(CNQ_CAPACITY, "too many texels to verify at once");
+  const uint32_t n = (uint32_t)b.slots.size(), per_slot = vc::PAGE_TEXELS * c->channels;
   DevBuf<float> fresh, dbounds;
   DevBuf<uint32_t> dlist, djobs, dslots, dstats;
   DevBuf<unsigned long long> differ;
@@ -1582,44 +1699,23 @@ int cnq_vcache_verify(cnq_vcache* c, void* stream, uint64_t out[2]) {
     fresh.release(), dbounds.release(), dlist.release(), djobs.release(), dslots.release(), dstats.release();
     differ.release();
   };
-  const uint32_t zeros[cnq::VC_STATS] = {0};
   const unsigned long long zero = 0;
   hipError_t e = fresh.alloc(c->d_pool.n);
   if (e == hipSuccess) e = hipMemcpyAsync(fresh.p, c->d_pool.p, c->d_pool.bytes(), hipMemcpyDeviceToDevice, st);
-  if (e == hipSuccess) e = dbounds.upload(bounds.data(), bounds.size());
-  if (e == hipSuccess) e = dlist.upload(list.data(), list.size());
-  if (e == hipSuccess) e = djobs.upload(jobs.data(), jobs.size());
-  if (e == hipSuccess) e = dslots.upload(slots.data(), slots.size());
-  if (e == hipSuccess) e = dstats.upload(zeros, cnq::VC_STATS);
+  if (e == hipSuccess) e = dslots.upload(b.slots.data(), b.slots.size());
   if (e == hipSuccess) e = differ.upload(&zero, 1);
   if (e != hipSuccess) {
     cleanup();
     return hip_fail(e, "verify buffers");
   }
-  cnq::VcArgs a = c->args();
-  a.pool = fresh.p;
-  a.bake_list = dlist.p;
-  a.bake_bounds = dbounds.p;
-  a.bake_jobs = djobs.p;
-  a.stats = dstats.p;
-  a.n_movers = 0;
-  a.plan_pages = n;
-  a.plan_jobs = (uint32_t)(jobs.size() / 2);
-  a.append_pages = 0;
-  a.append_jobs = 0;
-  const uint32_t resident = s->dev->compute_units * 4 * (uint32_t)cnq::WALK_WAVES;
-  a.bake_group = 64;
-  a.bake_resident = resident;
-  const cnq::QueryParams p = query_params(s, &c->opts, 64);
-  const uint32_t grid = (uint32_t)std::min<uint64_t>((a.plan_jobs + 3) / 4, resident);
-  hipLaunchKernelGGL(cnq::vc_bake_kernel<false>, dim3(grid), dim3(64), 0, st, a, s->blas.p, s->nodes.p, s->tri_leaf.p,
-                     s->items.p, s->blas_items.p, s->blas_cov.p, p, nullptr, nullptr);
-  e = hipGetLastError();
-  if (e == hipSuccess) {
-    hipLaunchKernelGGL(cnq::vc_compare_kernel, dim3(std::min<uint32_t>(n, 4096)), dim3(64), 0, st, c->d_pool.p,
-                       fresh.p, dslots.p, n, per_slot, differ.p);
-    e = hipGetLastError();
+  int rc = bake_lists(c, b, fresh.p, st, 0, 0, dbounds, dlist, djobs, dstats);
+  if (rc != CNQ_OK) {
+    cleanup();
+    return rc;
   }
+  hipLaunchKernelGGL(cnq::vc_compare_kernel, dim3(std::min<uint32_t>(n, 4096)), dim3(64), 0, st, c->d_pool.p, fresh.p,
+                     dslots.p, n, per_slot, differ.p);
+  e = hipGetLastError();
   unsigned long long d = 0;
   if (e == hipSuccess) e = hipMemcpyAsync(&d, differ.p, sizeof(d), hipMemcpyDeviceToHost, st);
   if (e == hipSuccess) e = hipStreamSynchronize(st);

@@ -710,6 +710,16 @@ class ViewCache:
         check(lib().cnq_vcache_plan(self.handle, int(budget), _ptr(st)))
         return dict(requests=int(st[0]), planned=int(st[1]), cones=int(st[2]), evicted=int(st[3]))
 
+    def set_lights(self, lights, first: int = 0, stream=None) -> dict:
+        """Lights [first, first + n) replaced (moved, resized, re-aimed) and
+        their channels of every resident page baked afresh."""
+        rows = np.ascontiguousarray(lights, np.float32).reshape(-1, 16)
+        st = np.zeros(2, np.uint64)
+        check(lib().cnq_vcache_set_lights(self.handle, _ptr(rows), int(first), len(rows), _stream_ptr(stream),
+                                          _ptr(st)))
+        self.lights[first:first + len(rows)] = rows
+        return dict(pages=int(st[0]), cones=int(st[1]))
+
     def verify(self, stream=None) -> tuple:
         """`verify_cache` (instanced.rs:3721): every page shown baked afresh
         and compared bit for bit; (values that differ, values compared)."""

@@ -521,6 +521,15 @@ int cnq_vcache_shade(cnq_vcache* cache, const float* camera, uint32_t width, uin
  * planned pages bake in the next cnq_vcache_frame. stats: [requests, pages
  * planned, cones planned, pages evicted]. */
 int cnq_vcache_plan(cnq_vcache* cache, uint32_t budget, uint64_t stats[4]);
+/* A light edit (`Renderer::set_light`, renderer.rs:421): lights [first,
+ * first + n) replaced by `lights` (n x 16 floats) and those lights' channels
+ * of every resident page baked afresh, inside texels then border texels.
+ * The reference bounds a moving light's staleness with its error ledger and
+ * re-bakes only where the bound passes its epsilon; this re-bakes every
+ * page the light has, so each one equals a fresh bake. stats (nullable):
+ * [pages, cones]. Waits for `stream`. */
+int cnq_vcache_set_lights(cnq_vcache* cache, const float* lights, uint32_t first, uint32_t n, void* stream,
+                          uint64_t stats[2]);
 /* `verify_cache` (instanced.rs:3721-3729): every page shown (resident, not
  * marked stale or invalid) baked afresh into a copy of the pool and compared
  * with the pool by f32 bit pattern. out: [values that differ, values

@@ -68,6 +68,7 @@ def lib() -> C.CDLL:
             "oracle_vc_moved": (None, [_P, _P, _P]),
             "oracle_vc_frame": (None, [_P, _P]),
             "oracle_vc_read": (None, [_P, _P, C.c_uint32, _P, _P]),
+            "oracle_vc_set_lights": (None, [_P, _P, C.c_uint32, C.c_uint32]),
             "oracle_vc_state": (None, [_P] * 7),
             "oracle_vc_hash": (C.c_uint32, [_P]),
             "oracle_vc_table_check": (C.c_int64, [C.c_uint32, C.c_uint32]),
@@ -474,6 +475,10 @@ class OracleViewCache:
         if len(k):
             lib().oracle_vc_read(self.handle, _p(k), len(k), _p(tex), _p(slots))
         return tex, slots
+
+    def set_lights(self, lights, first: int = 0):
+        rows = np.ascontiguousarray(lights, np.float32).reshape(-1, 16)
+        lib().oracle_vc_set_lights(self.handle, _p(rows), int(first), len(rows))
 
     def state(self) -> dict:
         sizes = np.zeros(5, np.uint32)

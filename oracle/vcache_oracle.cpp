@@ -1029,6 +1029,53 @@ void oracle_vc_frame(void* h, uint64_t* out_stats) {
 }
 
 // WGSL :3134 vc_find, then the page's texels.
+// cnq_vcache_set_lights: lights [first, first + n) replaced, and those
+// lights' channels of every resident page baked afresh (inside texels, then
+// the border texels from their nearest inside texel, as vc_fill copies).
+void oracle_vc_set_lights(void* h, const float* lights, uint32_t first, uint32_t n) {
+  State& s = *static_cast<State*>(h);
+  std::copy(lights, lights + 16 * (size_t)n, s.lights.begin() + 16 * (size_t)first);
+  std::vector<uint32_t> list, jobs;
+  for (uint32_t slot = 0; slot < s.slot_key.size(); slot++) {
+    if (s.slot_key[slot].empty()) continue;
+    const uint32_t* k = s.slot_key[slot].data();
+    uint32_t m = k[0], c = k[1] & CHART_MASK, mip = k[1] >> MIP_SHIFT, pp[2] = {k[2] & 0xffffu, k[2] >> 16};
+    uint32_t g[2], listed[2];
+    grid_of(s, c, m, mip, g);
+    for (int ax = 0; ax < 2; ax++) {
+      uint32_t nn = g[ax] + 1;
+      uint32_t inside = std::min(nn - std::min(pp[ax] * PAGE, nn), STRIDE);
+      listed[ax] = std::min(inside + 1, STRIDE);
+    }
+    uint32_t index = (uint32_t)(list.size() / 4);
+    list.insert(list.end(), {k[0], k[1], k[2], slot});
+    for (uint32_t l = first; l < first + n; l++)
+      for (uint32_t y = 0; y < listed[1]; y++)
+        for (uint32_t x = 0; x < listed[0]; x++) jobs.insert(jobs.end(), {index, (y * STRIDE + x) | (l << 8)});
+  }
+  std::swap(s.bake_list, list);
+  std::swap(s.bake_jobs, jobs);
+  const uint32_t total = (uint32_t)(s.bake_jobs.size() / 2);
+  for (uint32_t j = 0; j < total; j++) vc_bake_job(s, j, total);
+  for (size_t p = 0; p < s.bake_list.size() / 4; p++) {
+    const uint32_t* e = &s.bake_list[4 * p];
+    uint32_t c = e[1] & CHART_MASK, mip = e[1] >> MIP_SHIFT, g[2];
+    grid_of(s, c, e[0], mip, g);
+    uint32_t px = (e[2] & 0xffffu) * PAGE, py = (e[2] >> 16) * PAGE;
+    float* o = &s.pool[(size_t)e[3] * TEXELS * s.channels];
+    for (uint32_t k = 0; k < TEXELS; k++) {
+      uint32_t tx = px + k % STRIDE, ty = py + k / STRIDE;
+      if (tx >= g[0] || ty >= g[1]) {
+        uint32_t ix = std::min(tx, g[0] - 1) - px, iy = std::min(ty, g[1] - 1) - py;
+        for (uint32_t ch = first; ch < first + n; ch++)
+          o[k * s.channels + ch] = o[(iy * STRIDE + ix) * s.channels + ch];
+      }
+    }
+  }
+  std::swap(s.bake_list, list);
+  std::swap(s.bake_jobs, jobs);
+}
+
 void oracle_vc_read(void* h, const uint32_t* keys, uint32_t n, float* texels, uint32_t* slots) {
   State& s = *static_cast<State*>(h);
   size_t row = (size_t)TEXELS * s.channels;

@@ -254,3 +254,45 @@ def test_verify_finds_every_shown_page_current(device):
         differ, compared = vc.verify()
         print(f"verify after a move: {differ} of {compared}", flush=True)
         assert differ == 0 and compared > 0
+
+
+def test_light_edits_match_the_oracle_and_a_fresh_bake(device):
+    """A light edit (`Renderer::set_light`): the rect light moved and the
+    sphere light grown, each edit re-baking that light's channel of every
+    resident page; the state equals the oracle's, every page equals a fresh
+    bake under the edited lights, and `verify` finds nothing stale. A move
+    of the movers afterwards still matches the oracle (the shadow regions
+    follow the edited light's box)."""
+    w = _setup(device)
+    vc, ov = w["vc"], w["ov"]
+    keys = _keys(vc, w["pl"])
+    vc.request(keys), ov.request(keys)
+    vc.frame(), ov.frame()
+    lights = np.array(w["lights"], np.float32, copy=True)
+    rect = lights[0].copy()
+    rect[0:3] += np.float32([1.0, -0.5, 0.3])
+    sphere = lights[1].copy()
+    sphere[4] *= np.float32(1.5)
+    for idx, row in ((0, rect), (1, sphere)):
+        st = vc.set_lights(row[None], idx)
+        ov.set_lights(row[None], idx)
+        lights[idx] = row
+        print(f"light {idx} edited: {st}", flush=True)
+        assert st["pages"] > 0 and st["cones"] > 0
+        _same_state(vc, ov, f"after editing light {idx}")
+    differ, compared = vc.verify()
+    assert differ == 0 and compared > 0
+    fresh = ViewCache(w["scene"], w["surf"], lights, w["opts"], **w["kw"])
+    fresh.request(keys)
+    fresh.frame()
+    got, slots = vc.read(keys)
+    want, _ = fresh.read(keys)
+    shown = slots != O.NONE
+    assert shown.any() and np.array_equal(got[shown].view(np.uint32), want[shown].view(np.uint32))
+    g, moves, box = _move(w, ([1.5, 0.5, 0.0], [-0.7, 1.0, 0.0]))
+    vc.set_moved(moves, box)
+    ov.set_scene(g)
+    ov.set_moved(_oracle_moves(moves), box)
+    vc.request(keys), ov.request(keys)
+    assert vc.frame() == ov.frame()
+    _same_state(vc, ov, "after a move under the edited lights")

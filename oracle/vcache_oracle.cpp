@@ -17,8 +17,9 @@
 // order of list entries differs, never a texel's value.
 //
 // Left out as the product leaves them out: sky and bounce channels (the sky
-// channel is zero), moving lights and their lattice, deforming objects'
-// catch-up slices (every move is rigid, so the reference's `lag_only` is
+// channel is zero), the moving-light ledger and its lattice (a light edit
+// re-bakes the light's channel of every resident page, exactly), deforming
+// objects' catch-up slices (every move is rigid, so the reference's `lag_only` is
 // false wherever it is asked and no page is deferred), area texels, wide
 // windows' ray pass, absent lighting levels.
 #include <algorithm>

@@ -174,6 +174,13 @@ def courtyard_forms(device, steps: int) -> dict:
     return out
 
 
+# Served form: cone launches cut into pieces of at most 4M queries, and a
+# third slot that takes views of at most 1M queries (a 1280x720 view with one
+# light) ahead of larger ones.
+SERVED_LAUNCH_QUERIES = 1 << 22
+SERVED_EXPRESS_QUERIES = 1 << 20
+
+
 def served_form(device, n_requests: int, seconds: float) -> dict:
     """The served form over the 1M-instance boulder field (10 boulder shapes,
     a 32-light table): a seeded Poisson stream of view requests (a random
@@ -201,7 +208,8 @@ def served_form(device, n_requests: int, seconds: float) -> dict:
 
     def run(reqs):
         srv = Server(scene, lights, opts, pixel_capacity=40 << 20, query_capacity=400 << 20,
-                     max_lights=len(lights), max_views_per_launch=16, slots=2)
+                     max_lights=len(lights), max_views_per_launch=16, slots=3,
+                     launch_queries=SERVED_LAUNCH_QUERIES, express_queries=SERVED_EXPRESS_QUERIES)
         first_view = next(p for _, k, p in reqs if k == "view")
         warm = srv.submit_view(first_view[0], 64, 64, [0])
         srv.wait(warm)
@@ -247,10 +255,20 @@ def served_form(device, n_requests: int, seconds: float) -> dict:
         srv.close()
         lat = np.array([d[0] for d in done])
         q = sum(d[2] for d in done)
+        # Latency by the view's queries (pixels x lights): at most 1M (a
+        # 1280x720 view with one light), at most 4M, and larger.
+        sizes = np.array([d[2] for d in done])
+        by_size = {}
+        for name, lo, hi in (("le_1M_queries", 0, 1 << 20), ("le_4M_queries", (1 << 20) + 1, 1 << 22),
+                             ("gt_4M_queries", (1 << 22) + 1, 1 << 62)):
+            m = (sizes >= lo) & (sizes <= hi)
+            if m.any():
+                by_size[name] = {"views": int(m.sum()), "p50_ms": float(np.percentile(lat[m], 50)),
+                                 "p99_ms": float(np.percentile(lat[m], 99))}
         return {"requests": len(reqs), "views": len(done), "edits": counts_["edits"], "launches": counts_["launches"],
                 "queries": int(q), "s": wall, "queries_per_s": q / wall,
                 "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-                "queued_p50_ms": float(np.percentile([d[1] for d in done], 50))}
+                "queued_p50_ms": float(np.percentile([d[1] for d in done], 50)), "by_size": by_size}
 
     # Run 1: everything offered within the first 10 ms.
     burst = serving_requests(n_requests, n_requests / 0.01, lights)
@@ -289,6 +307,8 @@ def served_form(device, n_requests: int, seconds: float) -> dict:
         "workload": "boulder field: 10 boulder shapes x 999,987 instances + ground (100,002 triangles, 1.05M object "
                     "nodes), 32-light table; views 256^2..1920x1080 x 1-32 lights, 20% light edits",
         "instances": int(len(pl)), "host_build_s": round(t_build, 2), "scene_hbm_bytes": scene.hbm_bytes,
+        "server": {"slots": 3, "max_views_per_launch": 16, "launch_queries": SERVED_LAUNCH_QUERIES,
+                   "express_queries": SERVED_EXPRESS_QUERIES},
         "saturated": r1, "steady": dict(r2, offered_views_per_s=0.8 * rate),
         "plain_same_views": {"queries_per_s": q / t_plain, "s": t_plain},
         "served_over_plain": r1["queries_per_s"] / (q / t_plain),

@@ -72,7 +72,7 @@ class Tuning(C.Structure):
 class ServerConfig(C.Structure):
     _fields_ = [("pixel_capacity", C.c_uint64), ("query_capacity", C.c_uint64), ("max_lights", C.c_uint32),
                 ("max_view_lights", C.c_uint32), ("max_views_per_launch", C.c_uint32), ("slots", C.c_uint32),
-                ("tuning", Tuning)]
+                ("tuning", Tuning), ("launch_queries", C.c_uint64), ("express_queries", C.c_uint64)]
 
 
 class MoverTree(C.Structure):

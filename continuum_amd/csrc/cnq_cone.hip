@@ -49,8 +49,8 @@ __global__ __launch_bounds__(64, WALK_WAVES) void cone_items_kernel(
       uint32_t got = 0;
       if (lane == 0) got = atomicAdd(work_counter, 1u);
       got = uni(got);
-      if (got >= p.n_items) break;
-      item = order ? order[got] : got;
+      if (got >= p.n_items - p.item_begin) break;
+      item = p.item_begin + (order ? order[got] : got);
     } else {
       item = p.item_begin + (order ? order[blockIdx.x] : blockIdx.x);
     }

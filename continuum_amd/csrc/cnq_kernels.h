@@ -86,7 +86,8 @@ struct ItemArgs {
   uint32_t* __restrict__ counters;
   uint64_t* __restrict__ work;
   uint32_t* __restrict__ work_counter;  // persistent mode: the next item (zeroed before launch)
-  // Direct mode: wave w takes item item_begin + order[w] when `order` is
+  // Items [item_begin, n_items): wave w (direct mode) or the w-th grab
+  // (persistent mode) takes item item_begin + order[w] when `order` is
   // given (a permutation of the launch's items), and writes the item's time
   // (wall clock, 10 ns units) to item_cost[item - item_begin] when given.
   const uint32_t* __restrict__ order;

@@ -1,11 +1,16 @@
 // The served form: a request loop over one resident scene with continuous
 // batching. Clients submit view requests (camera, size, light indices) and
-// light edits from any thread; one scheduler thread admits them in arrival
-// order as HBM and launch slots free up, packs the admitted views of each
-// round into one camera pass and one cone launch on a free slot's stream,
-// and lets launches on different slots overlap on the device. A host
-// callback on each slot's stream stamps completion, so a request's latency
-// runs from submit to results readable.
+// light edits from any thread. A light edit applies to the table at once; a
+// view takes its lights from the table as it stands at submission. One
+// scheduler thread admits views as HBM and launch slots free up, in arrival
+// order, except that an express slot (the last, when configured) takes the
+// small views first, packs the admitted views of each round into one camera
+// pass and cone launches on a free slot's stream, and lets launches on
+// different slots overlap on the device. A launch's cone work can be cut
+// into pieces of a bounded number of queries, so the device turns to other
+// streams' launches between pieces. A host callback on each slot's stream
+// stamps completion, so a request's latency runs from submit to results
+// readable.
 //
 // Memory is reserved once: the shading points and hits of admitted views
 // live in a pixel arena (48 B per pixel), their query results in a query
@@ -93,6 +98,7 @@ struct Request {
   float camera[20];
   uint32_t w = 0, h = 0;
   std::vector<uint32_t> lights;
+  std::vector<float> rows;  // a view's lights, 16 floats each, as of submission
   uint32_t light_index = 0;
   float light[16];
   int32_t state = CNQ_REQ_QUEUED;
@@ -229,16 +235,21 @@ hipError_t launch_slot(cnq_server* sv, Slot& s) {
     a.views = s.views.p;
     a.v0 = s.h_views[0];
     a.work_counter = s.work_counter.p;
-    if (sv->cfg.tuning.persistent_blocks) {
-      e = fill_u32(s.work_counter.p, 0, 1, s.stream);
-      if (e != hipSuccess) return e;
-      a.p.item_begin = 0;
-      a.p.n_items = (uint32_t)items;
+    // Pieces of at most launch_queries queries (whole 64-query items), one
+    // after another on this stream; a direct launch also stays within the
+    // grid's 2^31 - 1 waves.
+    uint64_t piece = sv->cfg.launch_queries ? (sv->cfg.launch_queries + 63) / 64 : items;
+    if (!sv->cfg.tuning.persistent_blocks && piece > 0x7fffffffull) piece = 0x7fffffffull;
+    for (uint64_t b = 0; b < items; b += piece) {
+      a.p.item_begin = (uint32_t)b;
+      a.p.n_items = (uint32_t)(items - b < piece ? items : b + piece);
+      if (sv->cfg.tuning.persistent_blocks) {
+        e = fill_u32(s.work_counter.p, 0, 1, s.stream);
+        if (e != hipSuccess) return e;
+      }
       e = cnq::launch_cone_items(a, sc->count_work, sv->cfg.tuning.persistent_blocks, s.stream);
-    } else {
-      e = launch_items_direct(a, items, sc->count_work, s.stream);
+      if (e != hipSuccess) return e;
     }
-    if (e != hipSuccess) return e;
   }
   return hipLaunchHostFunc(s.stream, slot_finished, &s);
 }
@@ -264,28 +275,22 @@ void scheduler(cnq_server* sv) {
     bool any_busy = false;
     for (Slot& s : sv->slots) any_busy = any_busy || s.busy;
     if (sv->stop && !any_busy) break;
-    // Admit into free slots, in arrival order. A light edit at the head of
-    // the queue applies at once (no slot needed).
-    auto apply_edit = [sv](Request* r) {
-      std::memcpy(&sv->light_table[16 * (size_t)r->light_index], r->light, 64);
-      r->state = CNQ_REQ_DONE;
-      r->t_admit = r->t_done = Clock::now();
-      sv->edits_done++;
-      sv->pending.pop_front();
-      sv->done.notify_all();
-    };
+    // Admit views into free slots, in arrival order; the express slot
+    // passes over views larger than express_queries. A view that does not
+    // fit the arenas ends the slot's round.
     bool launched = false;
     if (!sv->stop && sv->error == CNQ_OK) {
-      while (!sv->pending.empty() && !sv->pending.front()->is_view) apply_edit(sv->pending.front());
-      for (Slot& s : sv->slots) {
+      for (size_t si = 0; si < sv->slots.size(); si++) {
+        Slot& s = sv->slots[si];
         if (s.busy || sv->pending.empty()) continue;
-        while (!sv->pending.empty()) {
-          Request* r = sv->pending.front();
-          if (!r->is_view) {
-            apply_edit(r);
+        const bool express = sv->cfg.express_queries && si + 1 == sv->slots.size();
+        for (auto it = sv->pending.begin(); it != sv->pending.end();) {
+          Request* r = *it;
+          if (s.reqs.size() >= sv->cfg.max_views_per_launch) break;
+          if (express && r->queries() > sv->cfg.express_queries) {
+            ++it;
             continue;
           }
-          if (s.reqs.size() >= sv->cfg.max_views_per_launch) break;
           uint64_t po, qo;
           if (!sv->px_alloc.take(r->pixels(), po)) break;
           if (!sv->q_alloc.take(r->queries(), qo)) {
@@ -294,15 +299,14 @@ void scheduler(cnq_server* sv) {
           }
           r->px_off = po;
           r->q_off = qo;
-          // The view's lights as the table holds them now.
-          for (size_t k = 0; k < r->lights.size(); k++)
-            std::memcpy(s.h_lights + 16 * (s.reqs.size() * sv->cfg.max_view_lights + k),
-                        &sv->light_table[16 * (size_t)r->lights[k]], 64);
+          if (!r->rows.empty())
+            std::memcpy(s.h_lights + 16 * s.reqs.size() * sv->cfg.max_view_lights, r->rows.data(),
+                        4 * r->rows.size());
           r->state = CNQ_REQ_ADMITTED;
           r->t_admit = Clock::now();
           r->launch = sv->launches + 1;
           s.reqs.push_back(r);
-          sv->pending.pop_front();
+          it = sv->pending.erase(it);
         }
         if (s.reqs.empty()) continue;
         s.launch = ++sv->launches;
@@ -339,6 +343,8 @@ void cnq_server_config_default(cnq_server_config* c) {
   c->max_views_per_launch = 64;
   c->slots = 2;
   cnq_tuning_default(&c->tuning);
+  c->launch_queries = 0;
+  c->express_queries = 0;
 }
 
 int cnq_server_create(cnq_scene* scene, const cnq_server_config* cfg, const cnq_options* opts, const float* lights,
@@ -349,6 +355,8 @@ int cnq_server_create(cnq_scene* scene, const cnq_server_config* cfg, const cnq_
   if (cfg->slots == 0 || cfg->slots > 8) return fail(CNQ_INVALID_ARG, "slots must be 1..8");
   if (cfg->max_views_per_launch == 0 || cfg->max_view_lights == 0 || cfg->max_lights == 0)
     return fail(CNQ_INVALID_ARG, "zero-sized server limits");
+  if (cfg->express_queries && cfg->slots < 2)
+    return fail(CNQ_INVALID_ARG, "an express slot needs at least two slots");
   const cnq_tuning& t = cfg->tuning;
   if (t.tile_w == 0 || t.tile_w > 64 || (t.tile_w & (t.tile_w - 1)))
     return fail(CNQ_INVALID_ARG, "tile_w must be 1..64, a power of two");
@@ -434,6 +442,7 @@ int cnq_server_submit_view(cnq_server* sv, const float* camera, uint32_t width, 
   r->w = width;
   r->h = height;
   r->lights.assign(lights, lights + n_lights);
+  r->rows.resize(16 * (size_t)n_lights);
   r->t_submit = Clock::now();
   std::lock_guard<std::mutex> g(sv->mu);
   if (sv->stop) {
@@ -444,6 +453,8 @@ int cnq_server_submit_view(cnq_server* sv, const float* camera, uint32_t width, 
     delete r;
     return fail(sv->error, sv->error_msg);
   }
+  for (uint32_t k = 0; k < n_lights; k++)
+    std::memcpy(r->rows.data() + 16 * (size_t)k, &sv->light_table[16 * (size_t)lights[k]], 64);
   r->id = sv->next_id++;
   sv->requests[r->id] = r;
   sv->pending.push_back(r);
@@ -466,10 +477,14 @@ int cnq_server_submit_light(cnq_server* sv, uint32_t index, const float* light, 
     delete r;
     return fail(CNQ_INVALID_ARG, "server is stopping");
   }
+  // Applied at once: views submitted from now on see the new light.
+  std::memcpy(&sv->light_table[16 * (size_t)index], light, 64);
+  r->state = CNQ_REQ_DONE;
+  r->t_admit = r->t_done = Clock::now();
+  sv->edits_done++;
   r->id = sv->next_id++;
   sv->requests[r->id] = r;
-  sv->pending.push_back(r);
-  sv->wake.notify_all();
+  sv->done.notify_all();
   *id = r->id;
   return CNQ_OK;
 }

@@ -309,12 +309,13 @@ int cnq_batch_set_lights(cnq_batch* batch, const float* lights, uint32_t n_light
  * the reference, which renders on the caller's thread; this is the
  * continuous-batching server its Renderer would sit behind). View requests
  * (a camera, a size, and 1..max_view_lights indices into the server's light
- * table) and light edits are admitted in arrival order: a view is admitted
- * when its results fit in the free HBM (48 B per pixel for shading points
- * and hits, 20 B per query) and a launch slot is free; admitted views go out
- * together as one camera pass and one cone launch on the slot's stream, and
- * launches on different slots overlap on the device. A light edit applies
- * to the views admitted after it. A view's memory stays held until
+ * table) are admitted in arrival order (the express slot, when configured,
+ * takes small views first): a view is admitted when its results fit in the
+ * free HBM (48 B per pixel for shading points and hits, 20 B per query) and a
+ * launch slot is free; admitted views go out together as one camera pass and
+ * cone launches on the slot's stream, and launches on different slots
+ * overlap on the device. A light edit applies to the table at once and so to
+ * the views submitted after it. A view's memory stays held until
  * cnq_server_release. Submit, wait, read and release may be called from any
  * host thread. */
 typedef struct cnq_server cnq_server;
@@ -327,6 +328,14 @@ typedef struct cnq_server_config {
   uint32_t max_views_per_launch; /* views batched into one launch */
   uint32_t slots;                /* launches in flight (one stream each) */
   cnq_tuning tuning;             /* cone launch shape */
+  uint64_t launch_queries;       /* cone launches carry at most this many queries (whole
+                                    64-query tiles); the rest of a launch's views follow as
+                                    further launches on its stream, so other slots' launches
+                                    reach the device in between (0: one launch) */
+  uint64_t express_queries;      /* the last slot takes only views of at most this many
+                                    queries, ahead of larger views that arrived earlier; the
+                                    other slots admit in arrival order (0: no express slot;
+                                    needs slots >= 2) */
 } cnq_server_config;
 
 enum { CNQ_REQ_QUEUED = 0, CNQ_REQ_ADMITTED = 1, CNQ_REQ_DONE = 2, CNQ_REQ_RELEASED = 3 };

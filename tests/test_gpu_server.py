@@ -139,3 +139,33 @@ def test_launches_overlap_on_the_device(device):
         _same(srv.read(r), ref)
         srv.release(r)
     srv.close()
+
+
+@pytest.mark.parametrize("persistent", [0, 512])
+def test_launches_cut_into_pieces_and_express_slot(device, persistent):
+    sc, scene, lights = _room(device)
+    opts = _lib.default_options(hierarchy=2)
+    big = camera_uniform([0.4, 0.4, 1.6], [3.6, 3.6, 0.8], 60.0, 512, 512)
+    small = camera_uniform([3.5, 0.5, 2.0], [0.5, 3.0, 0.5], 50.0, 48, 40)
+    # one view a launch, pieces of 4,096 queries (64 tiles); the last slot
+    # takes views of at most 48 x 40 x 2 queries
+    srv = Server(scene, lights, opts, max_views_per_launch=1, slots=2, launch_queries=4096,
+                 express_queries=48 * 40 * 2, persistent_blocks=persistent)
+    a = srv.submit_view(big, 512, 512, [0, 1, 2, 3])
+    b = srv.submit_view(big, 512, 512, [0, 1, 2, 3])
+    c = srv.submit_view(small, 48, 40, [1, 3])
+    for r in (a, b, c):
+        assert srv.wait(r)
+    ia, ib, ic = srv.info(a), srv.info(b), srv.info(c)
+    # b waits for the first slot; c, submitted after it, goes out on the
+    # express slot while a is still running
+    assert ib.launch != ic.launch and ic.admitted_at_ms <= ib.admitted_at_ms
+    ref = scene.render_view(big, 512, 512, lights, opts)
+    _same(srv.read(a), ref)
+    _same(srv.read(b), ref)
+    _same(srv.read(c), scene.render_view(small, 48, 40, lights[[1, 3]], opts))
+    for r in (a, b, c):
+        srv.release(r)
+    with pytest.raises(CnqError):
+        Server(scene, lights, opts, slots=1, express_queries=100)
+    srv.close()

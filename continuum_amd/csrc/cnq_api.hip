@@ -45,14 +45,20 @@ hipError_t fill_u32(uint32_t* p, uint32_t v, uint64_t n, hipStream_t s) {
 }
 
 // Per object: the min/max of its triangles' vertices (`mesh_bounds`,
-// hierarchy.rs:751), one block per object.
+// hierarchy.rs:751). Block (m, c) reduces part c of object m's vertices into
+// part[6 * (m * chunks + c) ..]; mesh_bounds_merge_kernel then reduces the
+// parts. Min and max are exact, so the grouping does not change the box
+// (only the sign of a zero, which the padding that follows erases).
 __global__ __launch_bounds__(256) void mesh_bounds_kernel(const float* __restrict__ tris,
-                                                          const uint32_t* __restrict__ ranges, float* out) {
-  const uint32_t m = blockIdx.x;
+                                                          const uint32_t* __restrict__ ranges, uint32_t chunks,
+                                                          float* part) {
+  const uint32_t m = blockIdx.x, c = blockIdx.y;
   const uint32_t first = ranges[4 * m], count = ranges[4 * m + 1];
+  const uint64_t verts = 3ull * count, per = (verts + chunks - 1) / chunks;
+  const uint64_t v0 = per * c, v1 = v0 + per < verts ? v0 + per : verts;
   float lo[3] = {3.40282347e38f, 3.40282347e38f, 3.40282347e38f};
   float hi[3] = {-3.40282347e38f, -3.40282347e38f, -3.40282347e38f};
-  for (uint64_t v = threadIdx.x; v < 3ull * count; v += blockDim.x) {
+  for (uint64_t v = v0 + threadIdx.x; v < v1; v += blockDim.x) {
     const float* p = tris + 4 * (3ull * first + v);
 #pragma unroll
     for (int k = 0; k < 3; k++) {
@@ -80,7 +86,23 @@ __global__ __launch_bounds__(256) void mesh_bounds_kernel(const float* __restric
     __syncthreads();
   }
   if (threadIdx.x == 0)
-    for (int k = 0; k < 6; k++) out[6 * m + k] = s[k][0];
+    for (int k = 0; k < 6; k++) part[6 * ((size_t)m * chunks + c) + k] = s[k][0];
+}
+
+__global__ __launch_bounds__(256) void mesh_bounds_merge_kernel(const float* __restrict__ part, uint32_t n_objects,
+                                                                uint32_t chunks, float* out) {
+  const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n_objects) return;
+  float b[6];
+  for (int k = 0; k < 6; k++) b[k] = part[6 * (size_t)m * chunks + k];
+  for (uint32_t c = 1; c < chunks; c++) {
+    const float* q = part + 6 * ((size_t)m * chunks + c);
+    for (int k = 0; k < 3; k++) {
+      if (q[k] < b[k]) b[k] = q[k];
+      if (q[3 + k] > b[3 + k]) b[3 + k] = q[3 + k];
+    }
+  }
+  for (int k = 0; k < 6; k++) out[6 * (size_t)m + k] = b[k];
 }
 
 }  // namespace detail
@@ -262,6 +284,59 @@ hipError_t launch_items_direct(cnq::ItemArgs a, uint64_t n_items, bool count_wor
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
+}
+
+int make_refit_plan(const std::vector<uint32_t>& child, const std::vector<uint32_t>& roots, RefitPlan& plan) {
+  // Breadth first from the roots (parents before children), then heights
+  // from the back of that order (children before parents).
+  const size_t n_nodes = child.size();
+  std::vector<uint32_t> order(roots);
+  for (size_t i = 0; i < order.size(); i++) {
+    const uint32_t c = child[order[i]];
+    if (c == NONE) continue;
+    if ((uint64_t)c + 1 >= n_nodes || order.size() + 2 > n_nodes)
+      return fail(CNQ_INVALID_ARG, "triangle hierarchy: a child outside the nodes, or a cycle");
+    order.push_back(c);
+    order.push_back(c + 1);
+  }
+  std::vector<uint8_t> height(n_nodes, 0);
+  std::vector<uint32_t> count(1, 0), leaves;
+  for (size_t i = order.size(); i-- > 0;) {
+    const uint32_t n = order[i], c = child[n];
+    if (c == NONE) {
+      leaves.push_back(n);
+      continue;
+    }
+    const uint32_t h = 1u + (height[c] > height[c + 1] ? height[c] : height[c + 1]);
+    if (h > 255) return fail(CNQ_INVALID_ARG, "triangle hierarchy deeper than 255 levels");
+    height[n] = (uint8_t)h;
+    if (count.size() < h + 1) count.resize(h + 1, 0);
+    count[h]++;
+  }
+  // Internal nodes by height (a counting sort), lowest first.
+  const uint32_t levels = count.size() > 1 ? (uint32_t)count.size() - 1 : 0;
+  plan.level_off.assign(levels + 1, 0);
+  for (uint32_t h = 1; h <= levels; h++) plan.level_off[h] = plan.level_off[h - 1] + count[h];
+  std::vector<uint32_t> next(plan.level_off), inner(levels ? plan.level_off[levels] : 0);
+  for (const uint32_t n : order)
+    if (child[n] != NONE) inner[next[height[n] - 1]++] = n;
+  hipError_t e = plan.leaves.upload(leaves.data(), leaves.size());
+  if (e == hipSuccess) e = plan.inner.upload(inner.data(), inner.size());
+  if (e != hipSuccess) return hip_fail(e, "refit plan");
+  return CNQ_OK;
+}
+
+hipError_t run_refit(cnq_scene* s, const RefitPlan& plan, hipStream_t stream) {
+  cnq::RefitArgs a;
+  a.tris = s->tris.p;
+  a.blas_items = s->blas_items.p;
+  a.blas = s->blas.p;
+  a.blas_cov = s->blas_cov.p;
+  a.tri_leaf = s->tri_leaf.p;
+  a.leaves = plan.leaves.p;
+  a.n_leaves = (uint32_t)plan.leaves.n;
+  const uint32_t levels = plan.level_off.empty() ? 0 : (uint32_t)plan.level_off.size() - 1;
+  return cnq::launch_refit(a, plan.inner.p, plan.level_off.data(), levels, stream);
 }
 
 int check_object_cov(cnq_scene* s, const cnq_options* o) {
@@ -455,15 +530,19 @@ int cnq_scene_create(cnq_device* dev, const cnq_scene_desc* d, cnq_scene** out) 
   for (uint32_t m = 0; m < d->n_objects; m++)
     if (d->ranges[4 * m + 3] != NONE) s->instanced = true;
   std::vector<NodeC> blas(d->n_blas), nodes(d->n_nodes);
-  std::vector<uint32_t> parent(d->n_blas, NONE), leaves;
+  std::vector<uint8_t> is_child(d->n_blas, 0);
+  s->h_child.assign(d->n_blas, NONE);
   for (uint32_t n = 0; n < d->n_blas; n++) {
     const uint32_t* g = d->blas + 12 * (size_t)n;
     blas[n] = compact(g, false);
-    if (g[9] > 0)
-      leaves.push_back(n);
-    else
-      parent[g[8]] = parent[g[8] + 1] = n;
+    if (g[9] == 0) {
+      s->h_child[n] = g[8];
+      is_child[g[8]] = is_child[g[8] + 1] = 1;
+    }
   }
+  std::vector<uint32_t> roots;
+  for (uint32_t n = 0; n < d->n_blas; n++)
+    if (!is_child[n]) roots.push_back(n);
   for (uint32_t n = 0; n < d->n_nodes; n++) nodes[n] = compact(d->nodes + 12 * (size_t)n, true);
   std::vector<float> tri_leaf(9 * d->n_blas_items);
   for (uint64_t k = 0; k < d->n_blas_items; k++) {
@@ -489,10 +568,12 @@ int cnq_scene_create(cnq_device* dev, const cnq_scene_desc* d, cnq_scene** out) 
   if (e == hipSuccess) e = s->nodes.upload(nodes.data(), nodes.size());
   if (e == hipSuccess) e = s->items.upload(d->items, d->n_items);
   if (e == hipSuccess) e = s->xforms.upload(d->xforms, 16 * (size_t)d->n_xforms);
-  if (e == hipSuccess) e = s->parent.upload(parent.data(), parent.size());
-  if (e == hipSuccess) e = s->leaves.upload(leaves.data(), leaves.size());
-  if (e == hipSuccess) e = s->arrivals.alloc(d->n_blas);
-  if (e == hipSuccess) e = s->mesh_box.alloc(6 * (size_t)d->n_objects);
+  // Mesh bounds in parts of about 8,192 vertices of the largest object (at
+  // most 128 parts an object), then the objects' boxes after the parts.
+  uint64_t max_count = 0;
+  for (uint32_t m = 0; m < d->n_objects; m++) max_count = std::max<uint64_t>(max_count, d->ranges[4 * (size_t)m + 1]);
+  s->bound_chunks = (uint32_t)std::min<uint64_t>(128, std::max<uint64_t>(1, (3 * max_count + 8191) / 8192));
+  if (e == hipSuccess) e = s->mesh_box.alloc(6 * (size_t)d->n_objects * (s->bound_chunks + 1));
   if (e == hipSuccess) e = s->counters.alloc(4);
   if (e == hipSuccess) e = s->work.alloc(6);
   if (e == hipSuccess) e = hipMemset(s->counters.p, 0, s->counters.bytes());
@@ -501,6 +582,11 @@ int cnq_scene_create(cnq_device* dev, const cnq_scene_desc* d, cnq_scene** out) 
     s->release();
     delete s;
     return hip_fail(e, e == hipErrorOutOfMemory ? "scene upload (out of device memory)" : "scene upload");
+  }
+  if ((rc = make_refit_plan(s->h_child, roots, s->refit))) {
+    s->release();
+    delete s;
+    return rc;
   }
   *out = s;
   return CNQ_OK;
@@ -899,40 +985,22 @@ int cnq_scene_set_pose(cnq_scene* s, uint32_t geometry, const float* tris9, uint
   }
   HIP_TRY(hipSetDevice(s->dev->ordinal), "hipSetDevice");
   hipStream_t st = (hipStream_t)stream;
-  // The geometry's leaves (each leaf climbs to one root), found once.
-  auto it = s->pose_leaves.find(root);
-  if (it == s->pose_leaves.end() && root != NONE) {
-    std::vector<uint32_t> parent(s->n_blas), leaves(s->leaves.n);
-    HIP_TRY(hipMemcpy(parent.data(), s->parent.p, parent.size() * 4, hipMemcpyDeviceToHost), "parents");
-    HIP_TRY(hipMemcpy(leaves.data(), s->leaves.p, leaves.size() * 4, hipMemcpyDeviceToHost), "leaves");
-    std::vector<uint32_t> mine;
-    for (uint32_t leaf : leaves) {
-      uint32_t n = leaf;
-      while (parent[n] != NONE) n = parent[n];
-      if (n == root) mine.push_back(leaf);
+  // The refit plan of the geometry's hierarchy, made once.
+  auto it = s->pose_plans.find(root);
+  if (it == s->pose_plans.end() && root != NONE) {
+    RefitPlan plan;
+    const int rc = make_refit_plan(s->h_child, std::vector<uint32_t>{root}, plan);
+    if (rc) {
+      plan.release();
+      return rc;
     }
-    cnq::detail::DevBuf<uint32_t> d;
-    HIP_TRY(d.upload(mine.data(), mine.size()), "pose leaves");
-    it = s->pose_leaves.emplace(root, d).first;
+    it = s->pose_plans.emplace(root, plan).first;
   }
   // Triangles: xyz of each vertex into the 16-byte rows (w keeps its code).
   if (count)
     HIP_TRY(hipMemcpy2DAsync(s->tris.p + 12 * (size_t)first, 16, tris9, 12, 12, 3 * (size_t)count,
                              hipMemcpyHostToDevice, st), "pose triangles");
-  if (root != NONE && it->second.n) {
-    HIP_TRY(fill_u32(s->arrivals.p, 0, s->n_blas, st), "refit clear");
-    cnq::RefitArgs a;
-    a.tris = s->tris.p;
-    a.blas_items = s->blas_items.p;
-    a.blas = s->blas.p;
-    a.blas_cov = s->blas_cov.p;
-    a.tri_leaf = s->tri_leaf.p;
-    a.parent = s->parent.p;
-    a.arrivals = s->arrivals.p;
-    a.leaves = it->second.p;
-    a.n_leaves = (uint32_t)it->second.n;
-    HIP_TRY(cnq::launch_refit(a, st), "pose refit");
-  }
+  if (root != NONE) HIP_TRY(run_refit(s, it->second, st), "pose refit");
   // Its new bounds (hierarchy.rs:751 mesh_bounds) and its instances moved.
   cnq::host::Box b;
   for (int c = 0; c < 3; c++) b.lo[c] = 3.40282347e38f, b.hi[c] = -3.40282347e38f;
@@ -963,24 +1031,18 @@ int cnq_scene_refit_device(cnq_scene* s, const float* tris, void* stream) {
   hipStream_t st = (hipStream_t)stream;
   if (s->n_tris)
     HIP_TRY(hipMemcpyAsync(s->tris.p, tris, 48 * s->n_tris, hipMemcpyDeviceToDevice, st), "refit copy");
-  HIP_TRY(fill_u32(s->arrivals.p, 0, s->n_blas, st), "refit clear");
-  cnq::RefitArgs a;
-  a.tris = s->tris.p;
-  a.blas_items = s->blas_items.p;
-  a.blas = s->blas.p;
-  a.blas_cov = s->blas_cov.p;
-  a.tri_leaf = s->tri_leaf.p;
-  a.parent = s->parent.p;
-  a.arrivals = s->arrivals.p;
-  a.leaves = s->leaves.p;
-  a.n_leaves = (uint32_t)s->leaves.n;
-  HIP_TRY(cnq::launch_refit(a, st), "refit launch");
+  HIP_TRY(run_refit(s, s->refit, st), "refit launch");
   // The object hierarchy over the new mesh bounds (live.rs:888-892 rebuilds
   // `mesh_hierarchy`): bounds on the device, the small tree on the host.
-  hipLaunchKernelGGL(mesh_bounds_kernel, dim3(s->n_objects), dim3(256), 0, st, s->tris.p, s->ranges.p, s->mesh_box.p);
+  float* boxes_d = s->mesh_box.p + 6 * (size_t)s->n_objects * s->bound_chunks;
+  hipLaunchKernelGGL(mesh_bounds_kernel, dim3(s->n_objects, s->bound_chunks), dim3(256), 0, st, s->tris.p,
+                     s->ranges.p, s->bound_chunks, s->mesh_box.p);
   HIP_TRY(hipGetLastError(), "mesh bounds launch");
+  hipLaunchKernelGGL(mesh_bounds_merge_kernel, dim3((s->n_objects + 255) / 256), dim3(256), 0, st, s->mesh_box.p,
+                     s->n_objects, s->bound_chunks, boxes_d);
+  HIP_TRY(hipGetLastError(), "mesh bounds merge launch");
   std::vector<float> mb(6 * (size_t)s->n_objects);
-  HIP_TRY(hipMemcpyAsync(mb.data(), s->mesh_box.p, mb.size() * 4, hipMemcpyDeviceToHost, st), "mesh bounds");
+  HIP_TRY(hipMemcpyAsync(mb.data(), boxes_d, mb.size() * 4, hipMemcpyDeviceToHost, st), "mesh bounds");
   HIP_TRY(hipStreamSynchronize(st), "refit sync");
   std::vector<cnq::host::Box> boxes(s->n_objects);
   for (uint32_t m = 0; m < s->n_objects; m++) {

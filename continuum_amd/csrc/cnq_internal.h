@@ -59,6 +59,19 @@ struct cnq_device {
   uint32_t compute_units;  // hipDeviceProp_t::multiProcessorCount
 };
 
+// A refit's schedule for some of the triangle hierarchies: their leaves,
+// and their internal nodes by height (inner[level_off[h], level_off[h + 1])
+// have height h + 1), built once on the host.
+struct RefitPlan {
+  cnq::detail::DevBuf<uint32_t> leaves, inner;
+  std::vector<uint32_t> level_off;
+  void release() {
+    leaves.release(), inner.release();
+    level_off.clear();
+  }
+  uint64_t bytes() const { return leaves.bytes() + inner.bytes(); }
+};
+
 // A launch's items in the order of their last times (longest first), for
 // launching the same work again (a view's cone queries, a list's sky).
 struct ItemOrder {
@@ -74,10 +87,13 @@ struct cnq_scene {
   uint64_t n_tris;
   uint32_t n_objects, n_blas, n_nodes, n_items, n_xforms;
   uint32_t n_cov_rows = 0;  // rows of 27 coverage samples (>= n_blas)
+  uint32_t bound_chunks = 1;  // parts of an object's vertices in the mesh-bounds pass
   uint64_t n_blas_items;
   bool count_work = false;
   cnq::detail::DevBuf<float> tris, tri_leaf, blas_cov, xforms, mesh_box;
-  cnq::detail::DevBuf<uint32_t> blas_items, ranges, items, parent, leaves, arrivals, counters;
+  cnq::detail::DevBuf<uint32_t> blas_items, ranges, items, counters;
+  RefitPlan refit;                  // every triangle hierarchy
+  std::vector<uint32_t> h_child;    // per node: its first child, NONE for a leaf
   cnq::detail::DevBuf<uint64_t> work;
   cnq::detail::DevBuf<float> sky_faces;  // cnq_sky_irradiance_device's per-face values when the caller has none
   ItemOrder order;      // cnq_view_irradiance_device's
@@ -95,7 +111,7 @@ struct cnq_scene {
   std::vector<uint32_t> mover_of;  // per object: its index among the movers, or NONE
   std::vector<cnq::host::Box> mover_bounds;  // each mover's world box as last applied
   std::vector<cnq_moved> last_moves;         // the last apply_moves's moves
-  std::map<uint32_t, cnq::detail::DevBuf<uint32_t>> pose_leaves;  // per posed geometry's root: its leaves
+  std::map<uint32_t, RefitPlan> pose_plans;  // per posed geometry's root: its hierarchy's refit
   bool instanced = false;
 
   cnq::SceneArgs args() const {
@@ -114,16 +130,16 @@ struct cnq_scene {
   }
   void release() {
     tris.release(), tri_leaf.release(), blas_cov.release(), xforms.release(), mesh_box.release();
-    blas_items.release(), ranges.release(), items.release(), parent.release(), leaves.release();
-    arrivals.release(), counters.release(), work.release(), sky_faces.release(), order.cost.release(),
+    blas_items.release(), ranges.release(), items.release(), refit.release();
+    counters.release(), work.release(), sky_faces.release(), order.cost.release(),
         order.order.release(), order.last.release(), sky_order.cost.release(), sky_order.order.release(),
         sky_order.last.release(), blas.release(), nodes.release();
-    for (auto& kv : pose_leaves) kv.second.release();
-    pose_leaves.clear();
+    for (auto& kv : pose_plans) kv.second.release();
+    pose_plans.clear();
   }
   uint64_t bytes() const {
     return tris.bytes() + tri_leaf.bytes() + blas_cov.bytes() + xforms.bytes() + mesh_box.bytes() +
-           blas_items.bytes() + ranges.bytes() + items.bytes() + parent.bytes() + leaves.bytes() + arrivals.bytes() +
+           blas_items.bytes() + ranges.bytes() + items.bytes() + refit.bytes() +
            counters.bytes() + work.bytes() + blas.bytes() + nodes.bytes();
   }
 };
@@ -144,6 +160,11 @@ ItemArgs item_args(cnq_scene* s, const float* receivers, const float* lights, fl
                    uint64_t* touched, float* cov);
 hipError_t launch_items_direct(ItemArgs a, uint64_t n_items, bool count_work, hipStream_t stream);
 int check_object_cov(cnq_scene* s, const cnq_options* o);
+// The refit schedule of the hierarchies under `roots` (child: per node its
+// first child, NONE for a leaf), and one refit of the scene's triangle
+// hierarchies by such a plan on `stream`.
+int make_refit_plan(const std::vector<uint32_t>& child, const std::vector<uint32_t>& roots, RefitPlan& plan);
+hipError_t run_refit(cnq_scene* s, const RefitPlan& plan, hipStream_t stream);
 
 }  // namespace detail
 }  // namespace cnq

@@ -118,9 +118,7 @@ struct RefitArgs {
   NodeC* blas;
   float* blas_cov;
   float* tri_leaf;
-  const uint32_t* __restrict__ parent;   // per node: parent index (NONE at a root)
-  uint32_t* arrivals;                    // per node: children finished (zeroed before)
-  const uint32_t* __restrict__ leaves;   // leaf node indices to start from
+  const uint32_t* __restrict__ leaves;   // the leaves to refit
   uint32_t n_leaves;
 };
 
@@ -200,7 +198,11 @@ hipError_t launch_sky(const SkyArgs& a, bool count_work, hipStream_t stream);
 // the order within a power of two is arbitrary).
 hipError_t launch_item_order(const uint32_t* cost, uint8_t* last, uint32_t* order, uint32_t n, hipStream_t stream);
 hipError_t launch_primary(const PrimaryArgs& a, hipStream_t stream);
-hipError_t launch_refit(const RefitArgs& a, hipStream_t stream);
+// Leaves first, then the internal nodes one height at a time: `inner` lists
+// them by height, level_off[h]..level_off[h + 1] (host array, n_levels + 1
+// entries) those of height h + 1.
+hipError_t launch_refit(const RefitArgs& a, const uint32_t* inner, const uint32_t* level_off, uint32_t n_levels,
+                        hipStream_t stream);
 hipError_t launch_bake_static(const CacheArgs& a, hipStream_t stream);
 hipError_t launch_mark_blocks(const CacheArgs& a, const SweptBoxes& boxes, hipStream_t stream);
 hipError_t launch_rebake(const CacheArgs& a, const MovingMeshes& moving, uint32_t grid, hipStream_t stream);

@@ -1,8 +1,9 @@
-// Triangle-hierarchy refit on the GPU: one bottom-up pass with an arrival
-// counter per node. A leaf's lane computes its box and 27 coverage samples
-// from the new pose, then climbs: the first child to finish a parent stops,
-// the second computes the parent from both children and carries on, so each
-// node is written once, after both of its children.
+// Triangle-hierarchy refit on the GPU, one height at a time: a pass over
+// the leaves computes each leaf's box and 27 coverage samples from the new
+// pose, then one pass per height computes the internal nodes of that height
+// (a leaf has height 0, a node one more than its taller child) from their
+// children, which earlier passes finished. Each node is written once; the
+// passes are ordered by the stream, so no node waits on another's atomics.
 //
 // Same results, bit for bit, as the reference's reverse sweep
 // (crates/continuum-gpu/src/hierarchy.rs:579-619 `refit_triangle_hierarchy`):
@@ -24,77 +25,83 @@ __device__ __forceinline__ f3 coverage_dir(int n) {
 __device__ __forceinline__ float gmx(float a, float b) { return a > b ? a : (b > a ? b : a); }
 __device__ __forceinline__ float gmn(float a, float b) { return a < b ? a : (b < a ? b : a); }
 
-__global__ __launch_bounds__(256) void refit_kernel(RefitArgs a) {
+__global__ __launch_bounds__(256) void refit_leaf_kernel(RefitArgs a) {
   const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= a.n_leaves) return;
-  uint32_t n = a.leaves[t];
-  {
-    NodeC node = a.blas[n];
-    float lo[3] = {3.40282347e38f, 3.40282347e38f, 3.40282347e38f};
-    float hi[3] = {-3.40282347e38f, -3.40282347e38f, -3.40282347e38f};
-    float cov[27];
+  const uint32_t n = a.leaves[t];
+  const NodeC node = a.blas[n];
+  float lo[3] = {3.40282347e38f, 3.40282347e38f, 3.40282347e38f};
+  float hi[3] = {-3.40282347e38f, -3.40282347e38f, -3.40282347e38f};
+  float cov[27];
 #pragma unroll
-    for (int k = 0; k < 27; k++) cov[k] = 0.0f;
-    for (uint32_t k = node.a; k < node.a + node.b; k++) {
-      const float* tr = a.tris + 12 * (size_t)a.blas_items[k];
-      f3 v0 = ld3(tr), v1 = ld3(tr + 4), v2 = ld3(tr + 8);
-      float* tl = a.tri_leaf + 9 * (size_t)k;
-      tl[0] = v0.x, tl[1] = v0.y, tl[2] = v0.z, tl[3] = v1.x, tl[4] = v1.y, tl[5] = v1.z;
-      tl[6] = v2.x, tl[7] = v2.y, tl[8] = v2.z;
-      // padded(mesh_bounds([t]))
-      float blo[3], bhi[3];
-      blo[0] = gmn(gmn(gmn(3.40282347e38f, v0.x), v1.x), v2.x);
-      blo[1] = gmn(gmn(gmn(3.40282347e38f, v0.y), v1.y), v2.y);
-      blo[2] = gmn(gmn(gmn(3.40282347e38f, v0.z), v1.z), v2.z);
-      bhi[0] = gmx(gmx(gmx(-3.40282347e38f, v0.x), v1.x), v2.x);
-      bhi[1] = gmx(gmx(gmx(-3.40282347e38f, v0.y), v1.y), v2.y);
-      bhi[2] = gmx(gmx(gmx(-3.40282347e38f, v0.z), v1.z), v2.z);
-      float m0 = gmx(fabsf(blo[0]), fabsf(bhi[0])), m1 = gmx(fabsf(blo[1]), fabsf(bhi[1])),
-            m2 = gmx(fabsf(blo[2]), fabsf(bhi[2]));
-      float pad = 1e-5f * gmx(gmx(gmx(m0, m1), m2), 1.0f);
-#pragma unroll
-      for (int c = 0; c < 3; c++) {
-        lo[c] = gmn(lo[c], blo[c] - pad);
-        hi[c] = gmx(hi[c], bhi[c] + pad);
-      }
-      f3 an = cross3(v1 - v0, v2 - v0) * 0.5f;
-#pragma unroll
-      for (int k2 = 0; k2 < 27; k2++) cov[k2] += gmx(dot3(an, coverage_dir(k2)), 0.0f);
-    }
-    NodeC* out = a.blas + n;
-    out->lo[0] = lo[0], out->lo[1] = lo[1], out->lo[2] = lo[2];
-    out->hi[0] = hi[0], out->hi[1] = hi[1], out->hi[2] = hi[2];
-    float* oc = a.blas_cov + 27 * (size_t)n;
-#pragma unroll
-    for (int k = 0; k < 27; k++) oc[k] = cov[k];
-  }
-  for (;;) {
-    const uint32_t p = a.parent[n];
-    if (p == NONE) return;
-    __threadfence();
-    if (atomicAdd(&a.arrivals[p], 1u) == 0u) return;  // the sibling finishes the parent
-    __threadfence();
-    const uint32_t left = a.blas[p].a;
-    const NodeC l = a.blas[left], r = a.blas[left + 1];
-    NodeC* out = a.blas + p;
+  for (int k = 0; k < 27; k++) cov[k] = 0.0f;
+  for (uint32_t k = node.a; k < node.a + node.b; k++) {
+    const float* tr = a.tris + 12 * (size_t)a.blas_items[k];
+    f3 v0 = ld3(tr), v1 = ld3(tr + 4), v2 = ld3(tr + 8);
+    float* tl = a.tri_leaf + 9 * (size_t)k;
+    tl[0] = v0.x, tl[1] = v0.y, tl[2] = v0.z, tl[3] = v1.x, tl[4] = v1.y, tl[5] = v1.z;
+    tl[6] = v2.x, tl[7] = v2.y, tl[8] = v2.z;
+    // padded(mesh_bounds([t]))
+    float blo[3], bhi[3];
+    blo[0] = gmn(gmn(gmn(3.40282347e38f, v0.x), v1.x), v2.x);
+    blo[1] = gmn(gmn(gmn(3.40282347e38f, v0.y), v1.y), v2.y);
+    blo[2] = gmn(gmn(gmn(3.40282347e38f, v0.z), v1.z), v2.z);
+    bhi[0] = gmx(gmx(gmx(-3.40282347e38f, v0.x), v1.x), v2.x);
+    bhi[1] = gmx(gmx(gmx(-3.40282347e38f, v0.y), v1.y), v2.y);
+    bhi[2] = gmx(gmx(gmx(-3.40282347e38f, v0.z), v1.z), v2.z);
+    float m0 = gmx(fabsf(blo[0]), fabsf(bhi[0])), m1 = gmx(fabsf(blo[1]), fabsf(bhi[1])),
+          m2 = gmx(fabsf(blo[2]), fabsf(bhi[2]));
+    float pad = 1e-5f * gmx(gmx(gmx(m0, m1), m2), 1.0f);
 #pragma unroll
     for (int c = 0; c < 3; c++) {
-      out->lo[c] = gmn(gmn(3.40282347e38f, l.lo[c]), r.lo[c]);
-      out->hi[c] = gmx(gmx(-3.40282347e38f, l.hi[c]), r.hi[c]);
+      lo[c] = gmn(lo[c], blo[c] - pad);
+      hi[c] = gmx(hi[c], bhi[c] + pad);
     }
-    const float* cl = a.blas_cov + 27 * (size_t)left;
-    const float* cr = a.blas_cov + 27 * (size_t)(left + 1);
-    float* oc = a.blas_cov + 27 * (size_t)p;
+    f3 an = cross3(v1 - v0, v2 - v0) * 0.5f;
 #pragma unroll
-    for (int k = 0; k < 27; k++) oc[k] = cl[k] + cr[k];
-    n = p;
+    for (int k2 = 0; k2 < 27; k2++) cov[k2] += gmx(dot3(an, coverage_dir(k2)), 0.0f);
   }
+  NodeC* out = a.blas + n;
+  out->lo[0] = lo[0], out->lo[1] = lo[1], out->lo[2] = lo[2];
+  out->hi[0] = hi[0], out->hi[1] = hi[1], out->hi[2] = hi[2];
+  float* oc = a.blas_cov + 27 * (size_t)n;
+#pragma unroll
+  for (int k = 0; k < 27; k++) oc[k] = cov[k];
 }
 
-hipError_t launch_refit(const RefitArgs& a, hipStream_t stream) {
+// Internal nodes nodes[0, n), all of one height: each from its left then
+// right child.
+__global__ __launch_bounds__(256) void refit_inner_kernel(RefitArgs a, const uint32_t* __restrict__ nodes, uint32_t n) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const uint32_t p = nodes[t];
+  const uint32_t left = a.blas[p].a;
+  const NodeC l = a.blas[left], r = a.blas[left + 1];
+  NodeC* out = a.blas + p;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    out->lo[c] = gmn(gmn(3.40282347e38f, l.lo[c]), r.lo[c]);
+    out->hi[c] = gmx(gmx(-3.40282347e38f, l.hi[c]), r.hi[c]);
+  }
+  const float* cl = a.blas_cov + 27 * (size_t)left;
+  const float* cr = a.blas_cov + 27 * (size_t)(left + 1);
+  float* oc = a.blas_cov + 27 * (size_t)p;
+#pragma unroll
+  for (int k = 0; k < 27; k++) oc[k] = cl[k] + cr[k];
+}
+
+hipError_t launch_refit(const RefitArgs& a, const uint32_t* inner, const uint32_t* level_off, uint32_t n_levels,
+                        hipStream_t stream) {
   if (a.n_leaves == 0) return hipSuccess;
-  hipLaunchKernelGGL(refit_kernel, dim3((a.n_leaves + 255) / 256), dim3(256), 0, stream, a);
-  return hipGetLastError();
+  hipLaunchKernelGGL(refit_leaf_kernel, dim3((a.n_leaves + 255) / 256), dim3(256), 0, stream, a);
+  hipError_t e = hipGetLastError();
+  for (uint32_t h = 0; h < n_levels && e == hipSuccess; h++) {
+    const uint32_t n = level_off[h + 1] - level_off[h];
+    if (n == 0) continue;
+    hipLaunchKernelGGL(refit_inner_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, a, inner + level_off[h], n);
+    e = hipGetLastError();
+  }
+  return e;
 }
 
 }  // namespace cnq

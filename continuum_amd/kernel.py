@@ -436,9 +436,12 @@ class Batch:
 
 
 class Server:
-    """The served form over one resident scene: view requests and light
-    edits admitted in arrival order, batched per launch, launches in flight
-    on several streams; see `cnq_server_*` in include/cnq.h."""
+    """The served form over one resident scene: view requests admitted in
+    arrival order (an express slot may take small views first), batched per
+    launch, launches in flight on several streams; light edits apply when
+    submitted. Keyword arguments set `cnq_server_config` fields (and
+    `tile_w`/`persistent_blocks` of its tuning); see `cnq_server_*` in
+    include/cnq.h."""
 
     def __init__(self, scene: Scene, lights: np.ndarray, opts: Optional[_lib.Options] = None, **config):
         self.scene = scene

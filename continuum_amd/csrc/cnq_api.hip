@@ -89,20 +89,26 @@ __global__ __launch_bounds__(256) void mesh_bounds_kernel(const float* __restric
     for (int k = 0; k < 6; k++) part[6 * ((size_t)m * chunks + c) + k] = s[k][0];
 }
 
-__global__ __launch_bounds__(256) void mesh_bounds_merge_kernel(const float* __restrict__ part, uint32_t n_objects,
-                                                                uint32_t chunks, float* out) {
-  const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= n_objects) return;
-  float b[6];
-  for (int k = 0; k < 6; k++) b[k] = part[6 * (size_t)m * chunks + k];
-  for (uint32_t c = 1; c < chunks; c++) {
-    const float* q = part + 6 * ((size_t)m * chunks + c);
-    for (int k = 0; k < 3; k++) {
-      if (q[k] < b[k]) b[k] = q[k];
-      if (q[3 + k] > b[3 + k]) b[3 + k] = q[3 + k];
+// Block m: object m's parts (at most 128), one a lane, reduced in LDS.
+__global__ __launch_bounds__(128) void mesh_bounds_merge_kernel(const float* __restrict__ part, uint32_t chunks,
+                                                                float* out) {
+  const uint32_t m = blockIdx.x, c = threadIdx.x;
+  __shared__ float s[6][128];
+  for (int k = 0; k < 6; k++)
+    s[k][c] = c < chunks ? part[6 * ((size_t)m * chunks + c) + k] : (k < 3 ? 3.40282347e38f : -3.40282347e38f);
+  __syncthreads();
+  for (uint32_t w = 64; w > 0; w >>= 1) {
+    if (c < w) {
+      for (int k = 0; k < 3; k++) {
+        const float a = s[k][c + w], b = s[3 + k][c + w];
+        if (a < s[k][c]) s[k][c] = a;
+        if (b > s[3 + k][c]) s[3 + k][c] = b;
+      }
     }
+    __syncthreads();
   }
-  for (int k = 0; k < 6; k++) out[6 * (size_t)m + k] = b[k];
+  if (c == 0)
+    for (int k = 0; k < 6; k++) out[6 * (size_t)m + k] = s[k][0];
 }
 
 }  // namespace detail
@@ -322,6 +328,7 @@ int make_refit_plan(const std::vector<uint32_t>& child, const std::vector<uint32
     if (child[n] != NONE) inner[next[height[n] - 1]++] = n;
   hipError_t e = plan.leaves.upload(leaves.data(), leaves.size());
   if (e == hipSuccess) e = plan.inner.upload(inner.data(), inner.size());
+  if (e == hipSuccess) e = plan.level_off_d.upload(plan.level_off.data(), plan.level_off.size());
   if (e != hipSuccess) return hip_fail(e, "refit plan");
   return CNQ_OK;
 }
@@ -336,7 +343,7 @@ hipError_t run_refit(cnq_scene* s, const RefitPlan& plan, hipStream_t stream) {
   a.leaves = plan.leaves.p;
   a.n_leaves = (uint32_t)plan.leaves.n;
   const uint32_t levels = plan.level_off.empty() ? 0 : (uint32_t)plan.level_off.size() - 1;
-  return cnq::launch_refit(a, plan.inner.p, plan.level_off.data(), levels, stream);
+  return cnq::launch_refit(a, plan.inner.p, plan.level_off.data(), plan.level_off_d.p, levels, stream);
 }
 
 int check_object_cov(cnq_scene* s, const cnq_options* o) {
@@ -1038,8 +1045,8 @@ int cnq_scene_refit_device(cnq_scene* s, const float* tris, void* stream) {
   hipLaunchKernelGGL(mesh_bounds_kernel, dim3(s->n_objects, s->bound_chunks), dim3(256), 0, st, s->tris.p,
                      s->ranges.p, s->bound_chunks, s->mesh_box.p);
   HIP_TRY(hipGetLastError(), "mesh bounds launch");
-  hipLaunchKernelGGL(mesh_bounds_merge_kernel, dim3((s->n_objects + 255) / 256), dim3(256), 0, st, s->mesh_box.p,
-                     s->n_objects, s->bound_chunks, boxes_d);
+  hipLaunchKernelGGL(mesh_bounds_merge_kernel, dim3(s->n_objects), dim3(128), 0, st, s->mesh_box.p, s->bound_chunks,
+                     boxes_d);
   HIP_TRY(hipGetLastError(), "mesh bounds merge launch");
   std::vector<float> mb(6 * (size_t)s->n_objects);
   HIP_TRY(hipMemcpyAsync(mb.data(), boxes_d, mb.size() * 4, hipMemcpyDeviceToHost, st), "mesh bounds");

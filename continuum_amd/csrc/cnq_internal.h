@@ -63,13 +63,13 @@ struct cnq_device {
 // and their internal nodes by height (inner[level_off[h], level_off[h + 1])
 // have height h + 1), built once on the host.
 struct RefitPlan {
-  cnq::detail::DevBuf<uint32_t> leaves, inner;
+  cnq::detail::DevBuf<uint32_t> leaves, inner, level_off_d;
   std::vector<uint32_t> level_off;
   void release() {
-    leaves.release(), inner.release();
+    leaves.release(), inner.release(), level_off_d.release();
     level_off.clear();
   }
-  uint64_t bytes() const { return leaves.bytes() + inner.bytes(); }
+  uint64_t bytes() const { return leaves.bytes() + inner.bytes() + level_off_d.bytes(); }
 };
 
 // A launch's items in the order of their last times (longest first), for

@@ -199,10 +199,10 @@ hipError_t launch_sky(const SkyArgs& a, bool count_work, hipStream_t stream);
 hipError_t launch_item_order(const uint32_t* cost, uint8_t* last, uint32_t* order, uint32_t n, hipStream_t stream);
 hipError_t launch_primary(const PrimaryArgs& a, hipStream_t stream);
 // Leaves first, then the internal nodes one height at a time: `inner` lists
-// them by height, level_off[h]..level_off[h + 1] (host array, n_levels + 1
-// entries) those of height h + 1.
-hipError_t launch_refit(const RefitArgs& a, const uint32_t* inner, const uint32_t* level_off, uint32_t n_levels,
-                        hipStream_t stream);
+// them by height, level_off[h]..level_off[h + 1] (n_levels + 1 entries, on
+// the host and, as level_off_d, on the device) those of height h + 1.
+hipError_t launch_refit(const RefitArgs& a, const uint32_t* inner, const uint32_t* level_off,
+                        const uint32_t* level_off_d, uint32_t n_levels, hipStream_t stream);
 hipError_t launch_bake_static(const CacheArgs& a, hipStream_t stream);
 hipError_t launch_mark_blocks(const CacheArgs& a, const SweptBoxes& boxes, hipStream_t stream);
 hipError_t launch_rebake(const CacheArgs& a, const MovingMeshes& moving, uint32_t grid, hipStream_t stream);

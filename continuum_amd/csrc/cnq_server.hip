@@ -99,8 +99,6 @@ struct Request {
   uint32_t w = 0, h = 0;
   std::vector<uint32_t> lights;
   std::vector<float> rows;  // a view's lights, 16 floats each, as of submission
-  uint32_t light_index = 0;
-  float light[16];
   int32_t state = CNQ_REQ_QUEUED;
   uint64_t px_off = 0, q_off = 0, launch = 0;
   Clock::time_point t_submit, t_admit, t_done;
@@ -469,8 +467,6 @@ int cnq_server_submit_light(cnq_server* sv, uint32_t index, const float* light, 
   Request* r = new (std::nothrow) Request;
   if (!r) return fail(CNQ_OOM, "out of host memory");
   r->is_view = false;
-  r->light_index = index;
-  std::memcpy(r->light, light, 64);
   r->t_submit = Clock::now();
   std::lock_guard<std::mutex> g(sv->mu);
   if (sv->stop) {

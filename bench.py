@@ -171,6 +171,17 @@ def courtyard_forms(device, steps: int) -> dict:
         batch.replay()
     torch.cuda.synchronize()
     t_batch = (time.perf_counter() - t0) / reps
+    # The sequential buffers still hold the last view: the batch's results
+    # for it must be the same bits.
+    _, lw, lh, _, ln = views[-1]
+    lq = lw * lh * ln
+    b_hits, b_vis, b_full, b_touched = batch.read(len(views) - 1)
+    last_view_differ = {
+        "hits": int((s_hits[:lw * lh].cpu().numpy().view(np.uint32) != b_hits).any(axis=1).sum()),
+        "visible": int((s_vis[:lq].cpu().numpy().view(np.uint32) != b_vis.view(np.uint32)).sum()),
+        "full": int((s_full[:lq].cpu().numpy().view(np.uint64) != b_full).sum()),
+        "touched": int((s_touched[:lq].cpu().numpy().view(np.uint64) != b_touched).sum()),
+        "queries": int(lq)}
     out["courtyard_64_views"] = {
         "workload": "64 views over the courtyard, sizes from {256^2, 640x360, 1280x720, 1920x1080}, 1-16 lights each",
         "queries": total_q,
@@ -178,6 +189,7 @@ def courtyard_forms(device, steps: int) -> dict:
         "batched_graph": {"queries_per_s": total_q / t_batch, "s": t_batch,
                           "tuning": {"tile_w": best.tile_w, "persistent_blocks": best.persistent_blocks}},
         "batched_over_sequential": t_seq / t_batch,
+        "last_view_batched_vs_plain_differ": last_view_differ,
     }
     return out
 

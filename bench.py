@@ -234,7 +234,7 @@ def served_form(device, n_requests: int, seconds: float) -> dict:
         warm = srv.submit_view(first_view[0], 64, 64, [0])
         srv.wait(warm)
         srv.release(warm)
-        ids, done = [], []
+        ids, done, kept = [], [], []
         lock = threading.Condition()
 
         def collector():
@@ -243,7 +243,7 @@ def served_form(device, n_requests: int, seconds: float) -> dict:
                 with lock:
                     while k >= len(ids):
                         lock.wait()
-                    rid, kind = ids[k]
+                    rid, kind, view = ids[k]
                 if rid is None:
                     return
                 k += 1
@@ -251,23 +251,34 @@ def served_form(device, n_requests: int, seconds: float) -> dict:
                     continue
                 srv.wait(rid)
                 info = srv.info(rid)
+                # The first view of at most 1M queries is kept, to compare
+                # with its own plain launch afterwards.
+                if not kept and info.width * info.height * info.n_lights <= 1 << 20:
+                    kept.append((view, srv.read(rid)))
                 done.append((info.latency_ms, info.queued_ms, info.width * info.height * info.n_lights,
                              time.perf_counter()))
                 srv.release(rid)
 
         th = threading.Thread(target=collector)
         th.start()
+        table = np.array(lights, np.float32, copy=True)  # the light table as each request is submitted
         t0 = time.perf_counter()
         for t_arr, kind, payload in reqs:
             delay = t0 + t_arr - time.perf_counter()
             if delay > 0:
                 time.sleep(delay)
             rid = srv.submit_view(*payload) if kind == "view" else srv.submit_light(*payload)
+            if kind == "view":
+                cam_, w_, h_, idx_ = payload
+                view = (cam_, w_, h_, table[list(idx_)].copy())
+            else:
+                table[payload[0]] = np.asarray(payload[1], np.float32).reshape(16)
+                view = None
             with lock:
-                ids.append((rid, kind))
+                ids.append((rid, kind, view))
                 lock.notify()
         with lock:
-            ids.append((None, None))
+            ids.append((None, None, None))
             lock.notify()
         th.join()
         wall = max(d[3] for d in done) - t0
@@ -288,17 +299,30 @@ def served_form(device, n_requests: int, seconds: float) -> dict:
         return {"requests": len(reqs), "views": len(done), "edits": counts_["edits"], "launches": counts_["launches"],
                 "queries": int(q), "s": wall, "queries_per_s": q / wall,
                 "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-                "queued_p50_ms": float(np.percentile([d[1] for d in done], 50)), "by_size": by_size}
+                "queued_p50_ms": float(np.percentile([d[1] for d in done], 50)), "by_size": by_size,
+                "_kept": kept}
 
     # Run 1: everything offered within the first 10 ms.
     burst = serving_requests(n_requests, n_requests / 0.01, lights)
     r1 = run(burst)
     views_per_s = r1["views"] / r1["s"]
+    # A served view of the burst against its plain launch (the host-read
+    # path), bit for bit.
+    served_vs_plain = None
+    if r1["_kept"]:
+        (cam_, w_, h_, rows), got = r1["_kept"][0]
+        want = scene.render_view(cam_, w_, h_, rows, opts)
+        served_vs_plain = {"queries": int(w_ * h_ * len(rows)),
+                           "differ": int(sum(int((np.ascontiguousarray(a).view(np.uint32).ravel()
+                                                  != np.ascontiguousarray(b).view(np.uint32).ravel()).sum())
+                                             for a, b in zip(got, want)))}
+    r1.pop("_kept")
     # Run 2: Poisson at half the drained rate, for about `seconds`.
     rate = 0.5 * views_per_s / 0.8
     n2 = max(20, int(rate * seconds))
     steady = serving_requests(n2, rate, lights, seed=0xC0_0008)
     r2 = run(steady)
+    r2.pop("_kept")
 
     # The same views of run 2 as plain launches, one after another.
     dev = torch.device("cuda:0")
@@ -332,6 +356,7 @@ def served_form(device, n_requests: int, seconds: float) -> dict:
         "saturated": r1, "steady": dict(r2, offered_views_per_s=0.8 * rate),
         "plain_same_views": {"queries_per_s": q / t_plain, "s": t_plain},
         "served_over_plain": r1["queries_per_s"] / (q / t_plain),
+        "burst_view_vs_its_plain_launch": served_vs_plain,
     }
 
 

@@ -7,7 +7,9 @@
 // libm's atan2f.
 #include <cmath>
 #include <cstdint>
+#include <algorithm>
 #include <cstring>
+#include <vector>
 
 #include "oracle_math.h"
 
@@ -322,6 +324,180 @@ static bool ray_triangle(V3 o, V3 d, const float* t9, float& t_out) {
   return t > 0.0f;
 }
 
+
+// cone3.rs:286-307: whether an axis-aligned box may meet the cone's slab and
+// window (conservative; the hierarchy walk's box test).
+static bool may_touch_box(const Cone& c, V3 lo, V3 hi) {
+  V3 corner[8];
+  for (int i = 0; i < 8; i++)
+    corner[i] = V3{(i & 1) ? hi.x : lo.x, (i & 2) ? hi.y : lo.y, (i & 4) ? hi.z : lo.z};
+  bool all_far = true;
+  for (int i = 0; i < 8; i++)
+    if (!(c.far_sd(corner[i]) < 0.0f)) all_far = false;
+  if (all_far) return false;
+  V3 l[8];
+  bool all_near = true;
+  for (int i = 0; i < 8; i++) {
+    l[i] = c.to_local(corner[i]);
+    if (!(l[i].z < c.near_)) all_near = false;
+  }
+  if (all_near) return false;
+  const float* w = c.window;
+  for (int which = 0; which < 4; which++) {
+    bool out = true;
+    for (int i = 0; i < 8 && out; i++) {
+      V3 p = l[i];
+      float f = which == 0 ? p.x - w[0] * p.z : which == 1 ? w[2] * p.z - p.x : which == 2 ? p.y - w[1] * p.z
+                                                                                        : w[3] * p.z - p.y;
+      if (!(f < 0.0f)) out = false;
+    }
+    if (out) return false;
+  }
+  return true;
+}
+
+// bvh.rs:25-165: the CPU model's hierarchy over every triangle of every
+// mesh: median splits on the longest axis of the triangles' centroids
+// ((a + b + c) / 3), keyed by the sum of the three vertices' coordinates on
+// that axis, at most 4 triangles a leaf. Equal keys fall in index order (the
+// reference leaves their order to select_nth_unstable_by).
+struct CpuBvh {
+  struct Node {
+    V3 lo, hi;
+    uint32_t start, count;
+  };
+  std::vector<Node> nodes;
+  std::vector<uint32_t> tri;  // triangle (global index) at each leaf position
+  std::vector<uint32_t> mesh_of;
+  const float* tris9 = nullptr;
+
+  V3 vert(uint32_t t, int k) const {
+    const float* p = tris9 + 9 * (size_t)t + 3 * k;
+    return V3{p[0], p[1], p[2]};
+  }
+
+  void split(uint32_t node, uint32_t start, uint32_t end) {
+    V3 lo{3.40282347e38f, 3.40282347e38f, 3.40282347e38f}, hi{-3.40282347e38f, -3.40282347e38f, -3.40282347e38f};
+    V3 clo = lo, chi = hi;
+    for (uint32_t i = start; i < end; i++) {
+      V3 a = vert(tri[i], 0), b = vert(tri[i], 1), cc = vert(tri[i], 2);
+      for (V3 p : {a, b, cc}) {
+        lo = V3{std::fmin(lo.x, p.x), std::fmin(lo.y, p.y), std::fmin(lo.z, p.z)};
+        hi = V3{std::fmax(hi.x, p.x), std::fmax(hi.y, p.y), std::fmax(hi.z, p.z)};
+      }
+      V3 sum = a + b + cc;
+      V3 cen{sum.x / 3.0f, sum.y / 3.0f, sum.z / 3.0f};
+      clo = V3{std::fmin(clo.x, cen.x), std::fmin(clo.y, cen.y), std::fmin(clo.z, cen.z)};
+      chi = V3{std::fmax(chi.x, cen.x), std::fmax(chi.y, cen.y), std::fmax(chi.z, cen.z)};
+    }
+    nodes[node].lo = lo;
+    nodes[node].hi = hi;
+    if (end - start <= 4) {
+      nodes[node].start = start;
+      nodes[node].count = end - start;
+      return;
+    }
+    V3 e = chi - clo;
+    int axis = (e.x >= e.y && e.x >= e.z) ? 0 : (e.y >= e.z ? 1 : 2);
+    auto key = [&](uint32_t t) {
+      auto co = [&](V3 v) { return axis == 0 ? v.x : axis == 1 ? v.y : v.z; };
+      return co(vert(t, 0)) + co(vert(t, 1)) + co(vert(t, 2));
+    };
+    uint32_t mid = (start + end) / 2;
+    std::stable_sort(tri.begin() + start, tri.begin() + end, [&](uint32_t a, uint32_t b) {
+      float ka = key(a), kb = key(b);
+      return ka < kb;
+    });
+    uint32_t left = (uint32_t)nodes.size();
+    nodes.push_back(Node{});
+    nodes.push_back(Node{});
+    nodes[node].start = left;
+    nodes[node].count = 0;
+    split(left, start, mid);
+    split(left + 1, mid, end);
+  }
+
+  void build(const float* t9, const uint32_t* counts, uint32_t n_meshes) {
+    tris9 = t9;
+    uint32_t total = 0;
+    for (uint32_t m = 0; m < n_meshes; m++) {
+      for (uint32_t k = 0; k < counts[m]; k++) mesh_of.push_back(m);
+      total += counts[m];
+    }
+    tri.resize(total);
+    for (uint32_t i = 0; i < total; i++) tri[i] = i;
+    nodes.assign(1, Node{});
+    split(0, 0, total);
+  }
+
+  // bvh.rs:116-165 `collect`: every triangle whose node boxes pass and which
+  // passes the triangle test, in the walk's order (children pushed left then
+  // right, so the right child is taken first); `visited` counts nodes.
+  template <class NodeTest, class TriTest>
+  std::vector<uint32_t> collect(NodeTest node_test, TriTest tri_test, uint64_t& visited) const {
+    std::vector<uint32_t> found;
+    std::vector<uint32_t> stack{0};
+    visited = 0;
+    while (!stack.empty()) {
+      uint32_t i = stack.back();
+      stack.pop_back();
+      visited++;
+      const Node& n = nodes[i];
+      if (!node_test(n.lo, n.hi)) continue;
+      if (n.count > 0) {
+        for (uint32_t k = n.start; k < n.start + n.count; k++)
+          if (tri_test(tri[k])) found.push_back(tri[k]);
+      } else {
+        stack.push_back(n.start);
+        stack.push_back(n.start + 1);
+      }
+    }
+    return found;
+  }
+};
+
+// experiments/room3d/src/lib.rs:121-171 `TileFrustum`: the prismatoid
+// between a floor tile [x0, y0, x1, y1] at z = 0 and a horizontal,
+// axis-aligned light at height h, as six planes n . p <= o.
+struct TileFrustum {
+  V3 n[6];
+  float o[6];
+  TileFrustum(const float tile[4], V3 corner, V3 eu, V3 ev) {
+    V3 a = corner, b = corner + eu + ev;
+    float lx0 = std::fmin(a.x, b.x), ly0 = std::fmin(a.y, b.y), lx1 = std::fmax(a.x, b.x),
+          ly1 = std::fmax(a.y, b.y), h = a.z;
+    auto side = [&](int k, float t, float l, V3 axis, float sign) {
+      V3 nn = (axis - V3{0.0f, 0.0f, 1.0f} * ((l - t) / h)) * sign;
+      n[k] = nn;
+      o[k] = sign * t;
+    };
+    side(0, tile[0], lx0, V3{1, 0, 0}, -1.0f);
+    side(1, tile[2], lx1, V3{1, 0, 0}, 1.0f);
+    side(2, tile[1], ly0, V3{0, 1, 0}, -1.0f);
+    side(3, tile[3], ly1, V3{0, 1, 0}, 1.0f);
+    n[4] = V3{-0.0f, -0.0f, -1.0f};
+    o[4] = 0.0f;
+    n[5] = V3{0.0f, 0.0f, 1.0f};
+    o[5] = h;
+  }
+  bool outside_one_plane(const V3* p, int count) const {
+    for (int k = 0; k < 6; k++) {
+      bool all = true;
+      for (int i = 0; i < count && all; i++)
+        if (!(dot(n[k], p[i]) > o[k])) all = false;
+      if (all) return true;
+    }
+    return false;
+  }
+  bool may_touch_box(V3 lo, V3 hi) const {
+    V3 corner[8];
+    for (int i = 0; i < 8; i++)
+      corner[i] = V3{(i & 1) ? hi.x : lo.x, (i & 2) ? hi.y : lo.y, (i & 4) ? hi.z : lo.z};
+    return !outside_one_plane(corner, 8);
+  }
+  bool may_touch(const V3* t) const { return !outside_one_plane(t, 3); }
+};
+
 }  // namespace c3
 
 using namespace c3;
@@ -403,6 +579,81 @@ double oracle_sampled_visible(const float* p3, const float* light12, const float
       }
     }
   return sum;
+}
+
+
+// bvh.rs:116 `Bvh::candidates` for the cone toward a rect light (corner,
+// edge_u, edge_v as light12's first three vec4s) from `apex` with receiver
+// normal `normal`: per mesh, the number of candidate triangles
+// (out_per_mesh, n_meshes entries); out_tris (nullable, capacity `cap`)
+// the candidates grouped by mesh in the walk's order (`group_by_mesh`: a
+// stable sort by mesh); returns the number of candidates, -1 when there is
+// no cone. *visited: nodes the walk visited.
+int64_t oracle_bvh_candidates(const float* tris9, const uint32_t* counts, uint32_t n_meshes, const float* apex3,
+                              const float* light12, const float* normal3, uint32_t* out_per_mesh, uint32_t* out_tris,
+                              uint64_t cap, uint64_t* visited) {
+  CpuBvh bvh;
+  bvh.build(tris9, counts, n_meshes);
+  V3 apex{apex3[0], apex3[1], apex3[2]}, nn{normal3[0], normal3[1], normal3[2]};
+  V3 corner{light12[0], light12[1], light12[2]}, eu{light12[4], light12[5], light12[6]},
+      ev{light12[8], light12[9], light12[10]};
+  Cone c;
+  if (!toward_rect_light(apex, corner, eu, ev, nn, c)) return -1;
+  uint64_t v = 0;
+  std::vector<uint32_t> found = bvh.collect([&](V3 lo, V3 hi) { return may_touch_box(c, lo, hi); },
+                                            [&](uint32_t t) {
+                                              V3 tv[3] = {bvh.vert(t, 0), bvh.vert(t, 1), bvh.vert(t, 2)};
+                                              return may_touch(c, tv);
+                                            },
+                                            v);
+  std::stable_sort(found.begin(), found.end(),
+                   [&](uint32_t a, uint32_t b) { return bvh.mesh_of[a] < bvh.mesh_of[b]; });
+  for (uint32_t m = 0; m < n_meshes; m++) out_per_mesh[m] = 0;
+  for (uint64_t i = 0; i < found.size(); i++) {
+    out_per_mesh[bvh.mesh_of[found[i]]]++;
+    if (out_tris && i < cap) out_tris[i] = found[i];
+  }
+  if (visited) *visited = v;
+  return (int64_t)found.size();
+}
+
+// cone3.rs:267 `may_touch` of every triangle (9 floats each) for the same cone.
+int oracle_cone3_may_touch(const float* tris9, uint64_t n, const float* apex3, const float* light12,
+                           const float* normal3, uint8_t* out) {
+  V3 apex{apex3[0], apex3[1], apex3[2]}, nn{normal3[0], normal3[1], normal3[2]};
+  V3 corner{light12[0], light12[1], light12[2]}, eu{light12[4], light12[5], light12[6]},
+      ev{light12[8], light12[9], light12[10]};
+  Cone c;
+  if (!toward_rect_light(apex, corner, eu, ev, nn, c)) return -1;
+  for (uint64_t t = 0; t < n; t++) {
+    const float* p = tris9 + 9 * t;
+    V3 tv[3] = {V3{p[0], p[1], p[2]}, V3{p[3], p[4], p[5]}, V3{p[6], p[7], p[8]}};
+    out[t] = may_touch(c, tv) ? 1 : 0;
+  }
+  return 0;
+}
+
+// TileFrustum's planes (6 x (n.xyz, o)) for a floor tile and a horizontal
+// light, and may_touch_box for boxes (6 floats each) / may_touch for
+// triangles (9 floats each).
+void oracle_tile_frustum(const float* tile4, const float* light12, float* planes24, const float* boxes6,
+                         uint64_t n_boxes, uint8_t* box_out, const float* tris9, uint64_t n_tris, uint8_t* tri_out) {
+  V3 corner{light12[0], light12[1], light12[2]}, eu{light12[4], light12[5], light12[6]},
+      ev{light12[8], light12[9], light12[10]};
+  TileFrustum f(tile4, corner, eu, ev);
+  for (int k = 0; k < 6 && planes24; k++) {
+    planes24[4 * k] = f.n[k].x, planes24[4 * k + 1] = f.n[k].y, planes24[4 * k + 2] = f.n[k].z;
+    planes24[4 * k + 3] = f.o[k];
+  }
+  for (uint64_t i = 0; i < n_boxes; i++) {
+    const float* b = boxes6 + 6 * i;
+    box_out[i] = f.may_touch_box(V3{b[0], b[1], b[2]}, V3{b[3], b[4], b[5]}) ? 1 : 0;
+  }
+  for (uint64_t t = 0; t < n_tris; t++) {
+    const float* p = tris9 + 9 * t;
+    V3 tv[3] = {V3{p[0], p[1], p[2]}, V3{p[3], p[4], p[5]}, V3{p[6], p[7], p[8]}};
+    tri_out[t] = f.may_touch(tv) ? 1 : 0;
+  }
 }
 
 }  // extern "C"

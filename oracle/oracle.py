@@ -46,6 +46,9 @@ def lib() -> C.CDLL:
             "oracle_coverage_visible": (C.c_float, [_P, C.c_uint32]),
             "oracle_coverage_open_length": (C.c_float, [_P, C.c_uint32, C.c_float, C.c_float]),
             "oracle_cone3_visible": (None, [_P, C.c_uint32, _P, _P, _P, C.c_uint32, C.c_int, _P, _P]),
+            "oracle_bvh_candidates": (C.c_int64, [_P, _P, C.c_uint32, _P, _P, _P, _P, _P, C.c_uint64, _P]),
+            "oracle_cone3_may_touch": (C.c_int, [_P, C.c_uint64, _P, _P, _P, _P]),
+            "oracle_tile_frustum": (None, [_P, _P, _P, _P, C.c_uint64, _P, _P, C.c_uint64, _P]),
             "oracle_sampled_visible": (C.c_double, [_P, _P, _P, C.c_uint64, C.c_uint32]),
             "oracle_mesh_hierarchy": (C.c_int, [_P, _P, C.c_uint32, _P, _P, _P, _P]),
             "oracle_flatten_instanced": (C.c_int, [_P, _P, C.c_uint32, _P, C.c_uint64, _P, C.c_uint32, C.c_int]
@@ -248,6 +251,53 @@ def cone3_visible(receivers6: np.ndarray, light: np.ndarray, meshes, cull=True):
     lib().oracle_cone3_visible(_p(r), len(r), _p(lt), _p(tris9), _p(counts), len(counts), int(cull), _p(out),
                                _p(total))
     return out, total
+
+
+def bvh_candidates(meshes, apex, light: np.ndarray, normal=(0.0, 0.0, 1.0)):
+    """bvh.rs:116 `Bvh::candidates` (the CPU model's hierarchy, leaves of 4)
+    for the cone from `apex` toward a rect light: (per-mesh candidate counts,
+    candidate triangle indices grouped by mesh, nodes visited), or None when
+    there is no cone."""
+    tris9 = np.ascontiguousarray(np.concatenate([np.asarray(m, np.float32).reshape(-1, 9) for m in meshes]))
+    counts = np.ascontiguousarray([len(m) for m in meshes], dtype=np.uint32)
+    a = np.ascontiguousarray(apex, np.float32)
+    lt = np.ascontiguousarray(light, np.float32).reshape(-1)[:12].copy()
+    nn = np.ascontiguousarray(normal, np.float32)
+    per = np.zeros(len(counts), np.uint32)
+    out = np.zeros(len(tris9), np.uint32)
+    visited = np.zeros(1, np.uint64)
+    n = lib().oracle_bvh_candidates(_p(tris9), _p(counts), len(counts), _p(a), _p(lt), _p(nn), _p(per), _p(out),
+                                    len(out), _p(visited))
+    if n < 0:
+        return None
+    return per, out[:n], int(visited[0])
+
+
+def cone3_may_touch(tris9: np.ndarray, apex, light: np.ndarray, normal=(0.0, 0.0, 1.0)):
+    """cone3.rs:267 `may_touch` per triangle (None when there is no cone)."""
+    t = np.ascontiguousarray(tris9, np.float32).reshape(-1, 9)
+    a = np.ascontiguousarray(apex, np.float32)
+    lt = np.ascontiguousarray(light, np.float32).reshape(-1)[:12].copy()
+    nn = np.ascontiguousarray(normal, np.float32)
+    out = np.zeros(len(t), np.uint8)
+    if lib().oracle_cone3_may_touch(_p(t), len(t), _p(a), _p(lt), _p(nn), _p(out)) < 0:
+        return None
+    return out.astype(bool)
+
+
+def tile_frustum(tile, light: np.ndarray, boxes=None, tris9=None):
+    """room3d TileFrustum (lib.rs:121-171): (planes (6, 4) as n.xyz, o with
+    n . p <= o inside, may_touch_box per box (6 floats), may_touch per
+    triangle (9 floats))."""
+    t4 = np.ascontiguousarray(tile, np.float32)
+    lt = np.ascontiguousarray(light, np.float32).reshape(-1)[:12].copy()
+    b = np.ascontiguousarray(np.zeros((0, 6)) if boxes is None else boxes, np.float32).reshape(-1, 6)
+    tr = np.ascontiguousarray(np.zeros((0, 9)) if tris9 is None else tris9, np.float32).reshape(-1, 9)
+    planes = np.zeros((6, 4), np.float32)
+    bo = np.zeros(len(b), np.uint8)
+    to = np.zeros(len(tr), np.uint8)
+    lib().oracle_tile_frustum(_p(t4), _p(lt), _p(planes), _p(b), len(b), _p(bo), _p(tr), len(tr), _p(to))
+    return planes, bo.astype(bool), to.astype(bool)
 
 
 def sampled_visible(point, light: np.ndarray, meshes, k: int) -> float:

@@ -161,6 +161,7 @@ SIGNATURES = {
     "cnq_scene_read_blas": (C.c_int, [_P, _P, _P]),
     "cnq_scene_counters": (C.c_int, [_P, C.c_int, _P, _P]),
     "cnq_scene_set_count_work": (C.c_int, [_P, C.c_int]),
+    "cnq_scene_set_query_flags": (C.c_int, [_P, _P, C.c_uint64]),
     "cnq_server_config_default": (None, [C.POINTER(ServerConfig)]),
     "cnq_server_create": (C.c_int, [_P, C.POINTER(ServerConfig), C.POINTER(Options), _P, C.c_uint32, C.POINTER(_P)]),
     "cnq_server_destroy": (None, [_P]),

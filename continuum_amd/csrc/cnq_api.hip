@@ -231,6 +231,8 @@ cnq::ItemArgs item_args(cnq_scene* s, const float* receivers, const float* light
   a.work_counter = nullptr;
   a.order = nullptr;
   a.item_cost = nullptr;
+  a.query_flags = s->query_flags;
+  a.n_flags = s->query_flags ? s->n_query_flags : 0;
   return a;
 }
 
@@ -240,9 +242,18 @@ cnq::ItemArgs item_args(cnq_scene* s, const float* receivers, const float* light
 // the stream the scene first ordered on, so a launch never reads it while it
 // changes; launches on other streams run in natural order. Results do not
 // depend on the order: every item writes only its own queries.
+// A launch being captured into a graph runs in natural order and records no
+// times: a graph would keep the order buffers' addresses, which a later,
+// larger view frees, and a later, smaller view rewrites only in part.
+bool capturing(hipStream_t stream) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  return hipStreamIsCapturing(stream, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone;
+}
+
 hipError_t order_items(ItemOrder& o, const uint32_t*& order, uint32_t*& item_cost, uint64_t items,
                        const uint64_t key[4], hipStream_t stream) {
   if (items == 0 || items > 0x7fffffffull) return hipSuccess;
+  if (capturing(stream)) return hipSuccess;
   if (!o.bound) {
     o.bound = true;
     o.stream = stream;
@@ -727,7 +738,10 @@ int cnq_sky_irradiance_device(cnq_scene* s, const float* receivers, uint64_t n, 
   if (sky) so = *sky;
   HIP_TRY(hipSetDevice(s->dev->ordinal), "hipSetDevice");
   if (!out_faces) {
+    if (capturing((hipStream_t)stream))
+      return fail(CNQ_INVALID_ARG, "a captured sky launch needs its own out_faces");
     if (!s->sky_faces.p || s->sky_faces.n < 6 * n) {
+      HIP_TRY(hipDeviceSynchronize(), "wait before the face buffer grows");  // no launch still reads it
       s->sky_faces.release();
       HIP_TRY(s->sky_faces.alloc(6 * n), "sky face buffer");
     }
@@ -1137,6 +1151,14 @@ int cnq_scene_set_count_work(cnq_scene* s, int on) {
   return CNQ_OK;
 }
 
+int cnq_scene_set_query_flags(cnq_scene* s, uint8_t* flags, uint64_t n) {
+  if (!s) return fail(CNQ_INVALID_ARG, "scene is null");
+  if (!flags && n) return fail(CNQ_INVALID_ARG, "null flags with a size");
+  s->query_flags = n ? flags : nullptr;
+  s->n_query_flags = n ? n : 0;
+  return CNQ_OK;
+}
+
 }  // extern "C"
 
 // ---- Batches: many views in one camera pass and one cone launch.
@@ -1181,14 +1203,25 @@ int check_tuning(const cnq_tuning* t) {
 }
 
 // Lays the batch's work items out for a cone tile width and uploads the
-// view table (synchronously: never during a capture).
+// view table (synchronously: never during a capture). A captured graph holds
+// the view table's and the item buffers' addresses and the old layout, so a
+// new layout first waits for the device (no launch still reads them) and
+// drops the graph: a replay then fails until the batch is captured again.
 int batch_layout(cnq_batch* b, uint32_t tile_w) {
   if (b->layout_tile_w == tile_w) return CNQ_OK;
+  std::vector<cnq::ViewDesc> laid(b->h_views);
   uint64_t item = 0;
-  for (auto& v : b->h_views) {
+  for (auto& v : laid) {
     item += layout_view(v, tile_w, item);
     if (item > 0xffffffffull) return fail(CNQ_CAPACITY, "a batch holds at most 2^32 - 1 work items");
   }
+  HIP_TRY(hipDeviceSynchronize(), "wait before a new layout");
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+  b->layout_tile_w = 0;
+  b->h_views.swap(laid);
   b->n_items = item;
   HIP_TRY(hipMemcpy(b->views.p, b->h_views.data(), b->h_views.size() * sizeof(cnq::ViewDesc), hipMemcpyHostToDevice),
           "view table");
@@ -1365,8 +1398,7 @@ int cnq_batch_capture(cnq_batch* b, const cnq_options* o, const cnq_tuning* t) {
 }
 
 int cnq_batch_replay(cnq_batch* b, void* stream) {
-  if (!b || !b->exec) return fail(CNQ_INVALID_ARG, "no captured graph");
-  if (b->layout_tile_w != b->graph_tuning.tile_w) return fail(CNQ_INVALID_ARG, "the batch was laid out again after capture");
+  if (!b || !b->exec) return fail(CNQ_INVALID_ARG, "no captured graph (none yet, or the batch was laid out again)");
   HIP_TRY(hipGraphLaunch(b->exec, (hipStream_t)stream), "graph launch");
   return CNQ_OK;
 }

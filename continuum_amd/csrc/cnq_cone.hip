@@ -35,7 +35,8 @@ __global__ __launch_bounds__(64, WALK_WAVES) void cone_items_kernel(
     const float* __restrict__ lights, const ViewDesc* __restrict__ views, float* __restrict__ out_visible,
     uint64_t* __restrict__ out_full, uint64_t* __restrict__ out_touched, float* __restrict__ out_cov,
     uint32_t* __restrict__ counters, uint64_t* __restrict__ work_out, uint32_t* __restrict__ work_counter,
-    const uint32_t* __restrict__ order, uint32_t* __restrict__ item_cost, QueryParams p, ViewDesc v0) {
+    const uint32_t* __restrict__ order, uint32_t* __restrict__ item_cost, uint8_t* __restrict__ query_flags,
+    uint64_t n_flags, QueryParams p, ViewDesc v0) {
   __shared__ float s_acc[ACC_CELLS * 64];
   const uint32_t lane = lane_id();
   float poly[POLY_SLOTS];
@@ -73,6 +74,8 @@ __global__ __launch_bounds__(64, WALK_WAVES) void cone_items_kernel(
       out_visible[q] = o.visible;
       out_full[q] = o.full;
       out_touched[q] = o.touched;
+      if (query_flags && q < n_flags)
+        query_flags[q] = (uint8_t)((o.capped ? 1u : 0u) | (o.overflow ? 2u : 0u) | (o.truncated ? 4u : 0u));
     }
     const uint64_t cap_m = ballot(valid && o.capped), of_m = ballot(valid && o.overflow),
                    tr_m = ballot(valid && o.truncated);
@@ -92,7 +95,7 @@ __global__ __launch_bounds__(64, WALK_WAVES) void cone_items_kernel(
 #define CNQ_ITEMS_ARGS                                                                                         \
   s.blas, s.nodes, s.tri_leaf, s.tris, s.ranges, s.items, s.blas_cov, s.xforms, a.receivers, a.lights, a.views, \
       a.out_visible, a.out_full, a.out_touched, a.out_cov, a.counters, a.work, a.work_counter, a.order, a.item_cost, \
-      a.p, a.v0
+      a.query_flags, a.n_flags, a.p, a.v0
 
 hipError_t launch_cone_items(const ItemArgs& a, bool count_work, uint32_t persistent_blocks, hipStream_t stream) {
   if (a.p.n_items <= a.p.item_begin) return hipSuccess;

@@ -90,6 +90,8 @@ struct cnq_scene {
   uint32_t bound_chunks = 1;  // parts of an object's vertices in the mesh-bounds pass
   uint64_t n_blas_items;
   bool count_work = false;
+  uint8_t* query_flags = nullptr;  // cnq_scene_set_query_flags: caller-owned device bytes
+  uint64_t n_query_flags = 0;
   cnq::detail::DevBuf<float> tris, tri_leaf, blas_cov, xforms, mesh_box;
   cnq::detail::DevBuf<uint32_t> blas_items, ranges, items, counters;
   RefitPlan refit;                  // every triangle hierarchy

@@ -92,6 +92,11 @@ struct ItemArgs {
   // (wall clock, 10 ns units) to item_cost[item - item_begin] when given.
   const uint32_t* __restrict__ order;
   uint32_t* __restrict__ item_cost;
+  // Nullable: per query q < n_flags of this launch's outputs, bit 0 the walk
+  // stopped at its cap, bit 1 it went deeper than the reference's 48-entry
+  // stack, bit 2 this build truncated it.
+  uint8_t* __restrict__ query_flags;
+  uint64_t n_flags;
   QueryParams p;
 };
 

@@ -228,6 +228,8 @@ hipError_t launch_slot(cnq_server* sv, Slot& s) {
   if (e != hipSuccess) return e;
   if (items) {
     cnq::ItemArgs a = item_args(sc, sv->receivers.p, s.lights.p, sv->visible.p, sv->full.p, sv->touched.p, nullptr);
+    a.query_flags = nullptr;  // the server's arena layout is its own
+    a.n_flags = 0;
     a.p = query_params(sc, &sv->opts, tile_w);
     a.p.n_views = n;
     a.views = s.views.p;

@@ -571,9 +571,33 @@ __device__ __forceinline__ uint32_t node_cone_test(const Cone& c, Lane& L, float
   return draw_hull(L, alpha) ? 2 : 1;
 }
 
-// WGSL :594-616 (opaque triangles)
+// How a mesh's triangles cover (WGSL :578-586, set per mesh :730-742): the
+// blend weight of an alpha-blended object, or a transmissive object's
+// factor (0: opaque) and whether it is a volume (two surfaces).
+struct Cover {
+  float weight;
+  float transmit;
+  bool volume;
+};
+
+__device__ __forceinline__ Cover cover_of(float clear) {
+  Cover k{1.0f - mx(clear, 0.0f), mx(-clear, 0.0f), false};
+  if (clear <= -2.0f) {
+    k.transmit = -2.0f - clear;
+    k.volume = true;
+  }
+  return k;
+}
+
+// x^5 as ((x x)(x x)) x, for Schlick's reflectance.
+__device__ __forceinline__ float pow5(float x) {
+  const float x2 = x * x;
+  return (x2 * x2) * x;
+}
+
+// WGSL :594-616
 template <bool SKY = false>
-__device__ __forceinline__ bool cover_triangle(const Cone& c, Lane& L, f3 v0, f3 v1, f3 v2, float weight,
+__device__ __forceinline__ bool cover_triangle(const Cone& c, Lane& L, f3 v0, f3 v1, f3 v2, const Cover& k,
                                                uint32_t stage, uint32_t& drawn_count) {
   if (stage == 4) return false;
   f3 n = cross3(v1 - v0, v2 - v0);
@@ -583,6 +607,16 @@ __device__ __forceinline__ bool cover_triangle(const Cone& c, Lane& L, f3 v0, f3
     return false;
   if (!may_touch<SKY>(c, v0, v1, v2)) return false;
   drawn_count++;
+  float weight = k.weight;
+  if (k.transmit > 0.0f) {
+    // WGSL :608-614: thin glass passes what Fresnel's reflectance (Schlick's,
+    // F0 0.04, at the cone's centre direction) leaves, times the factor.
+    const float cos_t = fabsf(dot3(norm3(n), norm3(direction(c, 0.5f, 0.5f))));
+    const float f = 0.04f + 0.96f * pow5(1.0f - cos_t);
+    float passes = k.transmit * (1.0f - f);
+    if (k.volume) passes *= 1.0f - f;
+    weight = 1.0f - passes;
+  }
   return rasterize(c, L, v0, v1, v2, weight);
 }
 
@@ -631,10 +665,10 @@ __device__ __forceinline__ void add_mesh_coverage(const Ctx& x, const Cone& wc, 
   const uint32_t first = x.ranges[4 * m], count = x.ranges[4 * m + 1], root = x.ranges[4 * m + 2],
                  xform = x.ranges[4 * m + 3];
   Cone c = wc;
-  float weight = 1.0f;
+  Cover weight{1.0f, 0.0f, false};
   if (xform != NONE) {
     c = cone_to_object(wc, x.xforms + 16 * (size_t)xform);
-    weight = 1.0f - mx(x.xforms[16 * (size_t)xform + 14], 0.0f);
+    weight = cover_of(x.xforms[16 * (size_t)xform + 14]);
   }
   const uint32_t cover_skip = (SKIP && m == skip.obj) ? skip.tri : NONE;
   clear_acc(L);

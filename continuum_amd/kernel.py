@@ -369,6 +369,16 @@ class Scene:
     def set_count_work(self, on: bool) -> None:
         check(lib().cnq_scene_set_count_work(self.handle, int(on)))
 
+    def set_query_flags(self, flags=None) -> None:
+        """Per-query walk flags (a device uint8 tensor, or None for none):
+        bit 0 capped, bit 1 deeper than 48 entries, bit 2 truncated. The
+        tensor must stay alive while launches write it."""
+        self._query_flags = flags
+        if flags is None:
+            check(lib().cnq_scene_set_query_flags(self.handle, None, 0))
+        else:
+            check(lib().cnq_scene_set_query_flags(self.handle, flags.data_ptr(), int(flags.numel())))
+
 
 class Batch:
     """Many views of one scene, each with its own camera, size and lights, in

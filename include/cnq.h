@@ -203,7 +203,10 @@ void cnq_sky_options_default(cnq_sky_options* sky);
  * each face's `right` axis the next world axis). out_faces (nullable, 6 x n:
  * face f of receiver i at f * n + i) and out_full (nullable, 6 x n: the
  * face's bins with coverage 1) give the faces apart. Device buffers,
- * enqueued on `stream`; without out_faces the scene keeps a 6 x n buffer. */
+ * enqueued on `stream`; without out_faces the scene keeps one 6 x n buffer
+ * for the faces, shared by every such call: calls without out_faces on
+ * different streams must not overlap, and a launch captured into a graph
+ * must pass out_faces (CNQ_INVALID_ARG otherwise). */
 int cnq_sky_irradiance_device(cnq_scene* scene, const float* receivers, uint64_t n, const cnq_options* opts,
                               const cnq_sky_options* sky, float* out, float* out_faces, uint64_t* out_full,
                               void* stream);
@@ -223,10 +226,9 @@ int cnq_view_irradiance_device(cnq_scene* scene, const float* receivers, uint32_
  * and launch the same view again, on the same stream, heaviest items first;
  * the scene keeps that order for one view (receivers buffer and sizes) and
  * binds it to the first stream it was used on, where launches see it in
- * stream order. Launches on other streams run in natural order. The results
- * do not depend on the order. The first launch of a larger view allocates
- * the order buffers and waits for its stream, so a caller that captures
- * these launches in a graph launches the view once before capturing.) */
+ * stream order. Launches on other streams, and launches captured into a
+ * graph, run in natural order and record no times. The results do not
+ * depend on the order.) */
 
 /* Camera rays for a width x height view (the WGSL `visibility` entry,
  * mask_cones.wgsl:2227, with `camera_hit` :2222): per pixel, hits (mesh,
@@ -263,6 +265,17 @@ int cnq_scene_read_blas(cnq_scene* scene, uint32_t* blas, float* blas_cov);
  * splatted. */
 int cnq_scene_counters(cnq_scene* scene, int reset, uint32_t counters[3], uint64_t work[6]);
 int cnq_scene_set_count_work(cnq_scene* scene, int on);
+/* Per-query walk flags for the queries launched after this call (NULL or
+ * n = 0: none): a device buffer of n bytes; every cone-query launch on the
+ * scene (cnq_irradiance[_device], cnq_view_irradiance_device,
+ * cnq_render_view, batches: the batch's concatenated output, view after
+ * view) writes, for each query whose output index is below n, bit 0 the walk
+ * stopped at its cone cap (light_cone_cap / wide_cone_cap; the walk's
+ * stop in `scene_coverage_if`, WGSL :1160-1227), bit 1 it went deeper than the reference's 48-entry stack, bit 2
+ * this build truncated it. A capped walk's coverage depends on the order
+ * nodes were visited in; the flags name exactly those queries. The buffer
+ * must outlive the launches that write it. Served views write none. */
+int cnq_scene_set_query_flags(cnq_scene* scene, uint8_t* flags, uint64_t n);
 
 /* ---- Batches: many views of uneven size and light count in one camera
  * pass and one cone launch (no counterpart in the reference, which renders

@@ -79,6 +79,10 @@ struct Cone {
 struct Query {
   float prefilter_bins = 0.0f;
   float tri_weight = 1.0f;
+  // WGSL :583-586: a transmissive object's factor (0: none) and whether it
+  // is a volume (light passes two surfaces).
+  float tri_transmit = 0.0f;
+  bool tri_volume = false;
   uint32_t cone_work = 0;
   uint32_t cone_cap = NONE;
   bool wide_clipped = false;
@@ -504,7 +508,14 @@ static uint32_t node_cone_test(const Cone& c, Query& qs, Acc& acc, uint32_t n, V
   return 2;
 }
 
-// WGSL :594-616 (opaque triangles: no transmission)
+// x^5 for Schlick's reflectance, by products: ((x x)(x x)) x. (The WGSL's
+// `pow(x, 5.0)` is left to the driver; the kernels use these products too.)
+static inline float pow5(float x) {
+  const float x2 = x * x;
+  return (x2 * x2) * x;
+}
+
+// WGSL :594-616
 static bool cover_triangle(const Cone& c, Query& qs, Acc& acc, uint32_t t) {
   if (qs.o->profile_stage == 4 || t == qs.cover_skip) return false;
   const float* tr = qs.s->tris + 12u * t;
@@ -515,6 +526,15 @@ static bool cover_triangle(const Cone& c, Query& qs, Acc& acc, uint32_t t) {
       dot(n, direction(c, 0.0f, 1.0f)) >= 0.0f && dot(n, direction(c, 1.0f, 1.0f)) >= 0.0f)
     return false;
   if (!may_touch(c, v0, v1, v2)) return false;
+  if (qs.tri_transmit > 0.0f) {
+    // WGSL :608-614: thin glass passes what Fresnel's reflectance (Schlick's,
+    // F0 0.04, at the cone's centre direction) leaves, times the factor.
+    const float cos_t = std::fabs(dot(normalize(n), normalize(direction(c, 0.5f, 0.5f))));
+    const float f = 0.04f + 0.96f * pow5(1.0f - cos_t);
+    float passes = qs.tri_transmit * (1.0f - f);
+    if (qs.tri_volume) passes *= 1.0f - f;
+    qs.tri_weight = 1.0f - passes;
+  }
   return rasterize(c, qs, acc, v0, v1, v2);
 }
 
@@ -572,9 +592,18 @@ static void add_mesh_coverage(const Cone& world_cone, Query& qs, uint32_t m, flo
   const uint32_t* range = s.ranges + 4 * m;
   qs.cover_skip = m == qs.skip_obj ? qs.skip_tri : NONE;
   qs.tri_weight = 1.0f;
+  qs.tri_transmit = 0.0f;
+  qs.tri_volume = false;
   if (range[3] != NONE) {
+    // WGSL :733-742: blended (clear > 0), transmissive (clear < 0), or a
+    // transmissive volume (clear <= -2).
     float clear = s.xforms[16 * range[3] + 14];
     qs.tri_weight = 1.0f - fmax_(clear, 0.0f);
+    qs.tri_transmit = fmax_(-clear, 0.0f);
+    if (clear <= -2.0f) {
+      qs.tri_transmit = -2.0f - clear;
+      qs.tri_volume = true;
+    }
   }
   Acc acc;
   for (int b = 0; b < 68; b++) acc.a[b] = 0.0f;

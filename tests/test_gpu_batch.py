@@ -107,3 +107,31 @@ def test_light_edits_change_only_what_they_touch(device):
     for k in range(len(views)):
         for a, b in zip(batch.read(k), fresh.read(k)):
             assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_a_new_layout_drops_the_captured_graph(device):
+    """A captured graph holds the view table and the item buffers of its
+    layout; laying the batch out for another tile width (a run or an
+    autotune) drops the graph, so a replay is refused until the batch is
+    captured again, and the new capture replays to the same bits."""
+    from continuum_amd._lib import CnqError
+
+    sc, lights, views = _views()
+    tris9, counts = meshes_to_arrays(sc.meshes)
+    scene = Scene(device, Flat.build_arrays(tris9, counts))
+    opts = _lib.default_options(hierarchy=2)
+    batch = Batch(scene, views, lights)
+    batch.run(opts, Batch.tuning(16, 0))
+    ref = [batch.read(k) for k in range(len(views))]
+    batch.capture(opts, Batch.tuning(16, 64))
+    batch.replay()
+    batch.run(opts, Batch.tuning(8, 0))
+    batch.run(opts, Batch.tuning(16, 0))  # back at the captured width: still a new layout
+    with pytest.raises(CnqError):
+        batch.replay()
+    batch.capture(opts, Batch.tuning(16, 64))
+    for _ in range(2):
+        batch.replay()
+        for k in range(len(views)):
+            for a, b in zip(batch.read(k), ref[k]):
+                assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), k

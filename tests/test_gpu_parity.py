@@ -10,7 +10,8 @@ import pytest
 
 from continuum_amd import _lib
 from continuum_amd.kernel import Flat, MaskConeKernel, Scene, receivers_array
-from continuum_amd.scenes import box_mesh, meshes_to_arrays, rect_light, rock_field, room_scene, sphere_light
+from continuum_amd.scenes import (box_mesh, camera_uniform, meshes_to_arrays, rect_light, rock_field, room_scene,
+                                  sphere_light)
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -262,3 +263,58 @@ def test_repeated_view_launches_take_the_longest_items_first_with_equal_results(
         sky.append(out.cpu().numpy())
     assert np.array_equal(sky[0].view(np.uint32), sky[1].view(np.uint32))
     assert np.array_equal(sky[0].view(np.uint32), sky[2].view(np.uint32))
+
+
+def test_a_captured_view_launch_replays_after_other_views(device):
+    """A view launch captured into a graph runs in natural order and keeps
+    no order buffer of the scene's: launches of a larger and a smaller view
+    between the replays (which reallocate and rewrite the scene's order) do
+    not change what a replay computes. Each replay follows eager work and a
+    synchronization, and equals the eager launch bit for bit."""
+    import torch
+
+    sc = room_scene(width=96, height=64)
+    tris9, counts = meshes_to_arrays(sc.meshes)
+    scene = Scene(device, Flat.build_arrays(tris9, counts))
+    p = _lib.default_options(hierarchy=2)
+    dev = torch.device("cuda:0")
+    lights = torch.from_numpy(np.ascontiguousarray(sc.lights)).to(dev)
+
+    def view(w, h):
+        # the room's camera (eye (0.4, 0.4, 1.6) toward (3.6, 3.6, 0.8), 60 degrees) at w x h
+        cam = torch.from_numpy(camera_uniform((0.4, 0.4, 1.6), (3.6, 3.6, 0.8), 60.0, w, h)).to(dev)
+        hits = torch.zeros((w * h, 4), dtype=torch.int32, device=dev)
+        recv = torch.zeros((w * h, 8), dtype=torch.float32, device=dev)
+        scene.primary_device(cam, w, h, lights, p, hits, recv)
+        outs = (torch.zeros(w * h, dtype=torch.float32, device=dev), torch.zeros(w * h, dtype=torch.int64, device=dev),
+                torch.zeros(w * h, dtype=torch.int64, device=dev))
+        return recv, outs
+
+    w, h = sc.width, sc.height
+    recv, eager = view(w, h)
+    for _ in range(2):  # the second launch runs ordered by the first one's times
+        scene.view_query_device(recv, w, h, lights, p, *eager, tile_w=16)
+    torch.cuda.synchronize()
+    want = [t.cpu().numpy() for t in eager]
+    _, graph_out = view(w, h)
+    g = torch.cuda.CUDAGraph()
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        with torch.cuda.graph(g):
+            scene.view_query_device(recv, w, h, lights, p, *graph_out, tile_w=16)
+    torch.cuda.current_stream().wait_stream(side)
+    big_recv, big = view(2 * w, 2 * h)
+    small_recv, small = view(w // 2, h // 2)
+    for r in range(3):
+        for t in graph_out:
+            t.zero_()
+        vw, vh, vr, vo = (2 * w, 2 * h, big_recv, big) if r % 2 == 0 else (w // 2, h // 2, small_recv, small)
+        scene.view_query_device(vr, vw, vh, lights, p, *vo, tile_w=16)
+        scene.view_query_device(vr, vw, vh, lights, p, *vo, tile_w=16)
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for a, b in zip(graph_out, want):
+            assert np.array_equal(np.ascontiguousarray(a.cpu().numpy()).view(np.uint32),
+                                  np.ascontiguousarray(b).view(np.uint32)), r

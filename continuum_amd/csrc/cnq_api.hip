@@ -17,6 +17,14 @@
 using cnq::NodeC;
 using cnq::NONE;
 
+// The face values of cnq_sky_irradiance_device_noalloc (6 per receiver, at
+// most CNQ_SKY_NOALLOC_MAX receivers a call): a static array in the
+// library's code object, one copy per GPU the library runs on, held until
+// the process ends. Host code never passes this name to a kernel: it is a
+// host-side stand-in, and each device's address comes from
+// hipGetSymbolAddress at cnq_device_create (cnq_device::sky_scratch).
+static __device__ __align__(256) float scratch[6ull * CNQ_SKY_NOALLOC_MAX];
+
 namespace cnq {
 namespace detail {
 
@@ -389,7 +397,10 @@ int cnq_device_create(int ordinal, cnq_device** out) {
   HIP_TRY(hipSetDevice(ordinal), "hipSetDevice");
   int cus = 0;
   HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ordinal), "compute units");
-  cnq_device* d = new (std::nothrow) cnq_device{ordinal, (uint32_t)(cus > 0 ? cus : 1)};
+  float* sky_scratch = nullptr;
+  HIP_TRY(hipGetSymbolAddress(reinterpret_cast<void**>(&sky_scratch), HIP_SYMBOL(scratch)), "face scratch address");
+  if (!sky_scratch) return fail(CNQ_HIP_ERROR, "face scratch address is null");
+  cnq_device* d = new (std::nothrow) cnq_device{ordinal, (uint32_t)(cus > 0 ? cus : 1), sky_scratch};
   if (!d) return fail(CNQ_OOM, "out of host memory");
   *out = d;
   return CNQ_OK;
@@ -768,6 +779,37 @@ int cnq_sky_irradiance_device(cnq_scene* s, const float* receivers, uint64_t n, 
   a.item_cost = cost;
   HIP_TRY(cnq::launch_sky(a, s->count_work, (hipStream_t)stream), "sky launch");
   HIP_TRY(reorder_items(s->sky_order, a.item_cost, items, (hipStream_t)stream), "sky order");
+  return CNQ_OK;
+}
+
+int cnq_sky_irradiance_device_noalloc(cnq_scene* s, const float* receivers, uint64_t n, const cnq_options* o,
+                                      const cnq_sky_options* sky, float* out, uint64_t* out_full, void* stream) {
+  if (!s) return fail(CNQ_INVALID_ARG, "scene is null");
+  int rc = check_options(o);
+  if (rc) return rc;
+  if ((rc = check_object_cov(s, o))) return rc;
+  if (n == 0) return CNQ_OK;
+  if (!receivers || !out) return fail(CNQ_INVALID_ARG, "null buffer");
+  if (n > CNQ_SKY_NOALLOC_MAX)
+    return fail(CNQ_CAPACITY, "at most " + std::to_string(CNQ_SKY_NOALLOC_MAX) + " receivers a call without allocation");
+  if (!s->dev->sky_scratch) return fail(CNQ_HIP_ERROR, "no face scratch on this device");
+  cnq_sky_options so;
+  cnq_sky_options_default(&so);
+  if (sky) so = *sky;
+  HIP_TRY(hipSetDevice(s->dev->ordinal), "hipSetDevice");
+  cnq::SkyArgs a;
+  a.s = s->args();
+  a.receivers = receivers;
+  a.out = out;
+  a.out_faces = s->dev->sky_scratch;  // every face of every receiver is written before the sum reads it
+  a.out_full = out_full;
+  a.work = s->work.p;
+  a.n = (uint32_t)n;
+  a.p = query_params(s, o, 64);
+  a.sky = cnq::SkyParams{so.max_combine, so.horizon_cull, so.prefilter_scale};
+  a.order = nullptr;  // natural order: no order buffers
+  a.item_cost = nullptr;
+  HIP_TRY(cnq::launch_sky(a, s->count_work, (hipStream_t)stream), "sky launch");
   return CNQ_OK;
 }
 

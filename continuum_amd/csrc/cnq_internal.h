@@ -57,6 +57,10 @@ uint64_t layout_view(ViewDesc& v, uint32_t tile_w, uint64_t item_off);
 struct cnq_device {
   int ordinal;
   uint32_t compute_units;  // hipDeviceProp_t::multiProcessorCount
+  // This device's copy of the library's static face scratch (the address
+  // hipGetSymbolAddress gives with the device current), for
+  // cnq_sky_irradiance_device_noalloc.
+  float* sky_scratch;
 };
 
 // A refit's schedule for some of the triangle hierarchies: their leaves,

@@ -214,6 +214,20 @@ int cnq_sky_irradiance_device(cnq_scene* scene, const float* receivers, uint64_t
 int cnq_sky_irradiance(cnq_scene* scene, const float* receivers, uint64_t n, const cnq_options* opts,
                        const cnq_sky_options* sky, float* out, float* out_faces, uint64_t* out_full);
 
+/* No counterpart in the reference: for callers that cannot allocate (code
+ * under graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * cnq_sky_irradiance_device without out_faces and without allocating: the
+ * per-face values go to a static array of the library's (6 x
+ * CNQ_SKY_NOALLOC_MAX floats, 24 MiB, one copy per GPU, held until the
+ * process ends), items run in natural order (no order buffers), and nothing
+ * is allocated or freed. At most CNQ_SKY_NOALLOC_MAX receivers a call
+ * (CNQ_CAPACITY beyond). Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU (other streams, other scenes, other threads)
+ * race. Results equal cnq_sky_irradiance_device's bit for bit. */
+#define CNQ_SKY_NOALLOC_MAX (1u << 20)
+int cnq_sky_irradiance_device_noalloc(cnq_scene* scene, const float* receivers, uint64_t n, const cnq_options* opts,
+                                      const cnq_sky_options* sky, float* out, uint64_t* out_full, void* stream);
+
 /* The same over a width x height view's receivers (pixel y * width + x),
  * each wave taking a tile_w x (64 / tile_w) patch of pixels (tile_w a power
  * of two up to 64): the cone part of the WGSL `render` entry

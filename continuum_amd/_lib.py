@@ -66,7 +66,8 @@ class View(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("tile_w", C.c_uint32), ("persistent_blocks", C.c_uint32)]
+    _fields_ = [("tile_w", C.c_uint32), ("persistent_blocks", C.c_uint32), ("waves_per_simd", C.c_uint32),
+                ("ray_stack", C.c_uint32)]
 
 
 class ServerConfig(C.Structure):
@@ -164,6 +165,7 @@ SIGNATURES = {
     "cnq_scene_counters": (C.c_int, [_P, C.c_int, _P, _P]),
     "cnq_scene_set_count_work": (C.c_int, [_P, C.c_int]),
     "cnq_scene_set_query_flags": (C.c_int, [_P, _P, C.c_uint64]),
+    "cnq_scene_set_normals": (C.c_int, [_P, _P, C.c_uint64]),
     "cnq_server_config_default": (None, [C.POINTER(ServerConfig)]),
     "cnq_server_create": (C.c_int, [_P, C.POINTER(ServerConfig), C.POINTER(Options), _P, C.c_uint32, C.POINTER(_P)]),
     "cnq_server_destroy": (None, [_P]),

@@ -299,13 +299,14 @@ hipError_t reorder_items(ItemOrder& o, const uint32_t* item_cost, uint64_t items
 }
 
 // Launches the items of `a` in direct mode, at most 2^31 - 1 blocks a launch.
-hipError_t launch_items_direct(cnq::ItemArgs a, uint64_t n_items, bool count_work, hipStream_t stream) {
+hipError_t launch_items_direct(cnq::ItemArgs a, uint64_t n_items, bool count_work, hipStream_t stream,
+                               uint32_t waves) {
   const uint64_t per = 0x7fffffffull;
   if (n_items > per) a.order = nullptr, a.item_cost = nullptr;  // an order covers one launch
   for (uint64_t i0 = 0; i0 < n_items; i0 += per) {
     a.p.item_begin = (uint32_t)i0;
     a.p.n_items = (uint32_t)(i0 + per < n_items ? i0 + per : n_items);
-    hipError_t e = cnq::launch_cone_items(a, count_work, 0, stream);
+    hipError_t e = cnq::launch_cone_items(a, count_work, 0, stream, waves);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
@@ -653,7 +654,7 @@ int cnq_irradiance_device(cnq_scene* s, const float* receivers, uint64_t n, cons
   v.n_lights = n_lights;
   v.ptile_off = 0;
   const uint64_t items = layout_view(v, 64, 0);
-  HIP_TRY(launch_items_direct(a, items, s->count_work, (hipStream_t)stream), "cone query launch");
+  HIP_TRY(launch_items_direct(a, items, s->count_work, (hipStream_t)stream, 2), "cone query launch");
   return CNQ_OK;
 }
 
@@ -686,7 +687,7 @@ int cnq_view_irradiance_device(cnq_scene* s, const float* receivers, uint32_t wi
   HIP_TRY(order_items(s->order, order, cost, items, key, (hipStream_t)stream), "item order buffers");
   a.order = order;
   a.item_cost = cost;
-  HIP_TRY(launch_items_direct(a, items, s->count_work, (hipStream_t)stream), "cone query launch");
+  HIP_TRY(launch_items_direct(a, items, s->count_work, (hipStream_t)stream, 2), "cone query launch");
   HIP_TRY(reorder_items(s->order, a.item_cost, items, (hipStream_t)stream), "item order");
   return CNQ_OK;
 }
@@ -872,6 +873,7 @@ int cnq_primary_device(cnq_scene* s, const float* camera, uint32_t width, uint32
   a.hits = hits;
   a.receivers = receivers;
   a.counters = s->counters.p + 3;
+  a.normals = s->normals.p;
   HIP_TRY(cnq::launch_primary(a, (hipStream_t)stream), "primary launch");
   return CNQ_OK;
 }
@@ -1193,6 +1195,24 @@ int cnq_scene_set_count_work(cnq_scene* s, int on) {
   return CNQ_OK;
 }
 
+int cnq_scene_set_normals(cnq_scene* s, const float* normals, uint64_t n_tris) {
+  if (!s) return fail(CNQ_INVALID_ARG, "scene is null");
+  HIP_TRY(hipSetDevice(s->dev->ordinal), "hipSetDevice");
+  if (!normals) {
+    HIP_TRY(hipDeviceSynchronize(), "wait before dropping the normals");  // no camera pass still reads them
+    s->normals.release();
+    return CNQ_OK;
+  }
+  if (n_tris != s->n_tris) return fail(CNQ_INVALID_ARG, "normals: one row of 9 per triangle of the scene");
+  if (!s->normals.p) {
+    HIP_TRY(s->normals.alloc(9 * n_tris), "normals buffer");
+  } else {
+    HIP_TRY(hipDeviceSynchronize(), "wait before replacing the normals");
+  }
+  HIP_TRY(hipMemcpy(s->normals.p, normals, 36 * n_tris, hipMemcpyHostToDevice), "normals upload");
+  return CNQ_OK;
+}
+
 int cnq_scene_set_query_flags(cnq_scene* s, uint8_t* flags, uint64_t n) {
   if (!s) return fail(CNQ_INVALID_ARG, "scene is null");
   if (!flags && n) return fail(CNQ_INVALID_ARG, "null flags with a size");
@@ -1241,6 +1261,9 @@ int check_tuning(const cnq_tuning* t) {
   if (!t) return fail(CNQ_INVALID_ARG, "tuning is null");
   if (t->tile_w == 0 || t->tile_w > 64 || (t->tile_w & (t->tile_w - 1)))
     return fail(CNQ_INVALID_ARG, "tile_w must be 1..64, a power of two");
+  if (t->waves_per_simd > 2) return fail(CNQ_INVALID_ARG, "waves_per_simd must be 0, 1 or 2");
+  if (t->ray_stack != 0 && t->ray_stack != 32 && t->ray_stack != 64)
+    return fail(CNQ_INVALID_ARG, "ray_stack must be 0, 32 or 64");
   return CNQ_OK;
 }
 
@@ -1301,7 +1324,8 @@ int batch_enqueue(cnq_batch* b, const cnq_options* o, const cnq_tuning* t, hipSt
   pa.hits = b->hits.p;
   pa.receivers = b->receivers.p;
   pa.counters = s->counters.p + 3;
-  HIP_TRY(cnq::launch_primary(pa, st), "batch camera pass");
+  pa.normals = s->normals.p;
+  HIP_TRY(cnq::launch_primary(pa, st, t->ray_stack ? t->ray_stack : 64), "batch camera pass");
   cnq::ItemArgs a = item_args(s, b->receivers.p, b->lights.p, b->visible.p, b->full.p, b->touched.p, nullptr);
   a.p = query_params(s, o, t->tile_w);
   a.p.n_views = b->n_views;
@@ -1314,9 +1338,9 @@ int batch_enqueue(cnq_batch* b, const cnq_options* o, const cnq_tuning* t, hipSt
     HIP_TRY(fill_u32(b->work_counter.p, 0, 1, st), "work counter");
     a.p.item_begin = 0;
     a.p.n_items = (uint32_t)b->n_items;
-    HIP_TRY(cnq::launch_cone_items(a, s->count_work, t->persistent_blocks, st), "batch cone pass");
+    HIP_TRY(cnq::launch_cone_items(a, s->count_work, t->persistent_blocks, st, t->waves_per_simd), "batch cone pass");
   } else {
-    HIP_TRY(launch_items_direct(a, b->n_items, s->count_work, st), "batch cone pass");
+    HIP_TRY(launch_items_direct(a, b->n_items, s->count_work, st, t->waves_per_simd), "batch cone pass");
   }
   if (a.item_cost)
     HIP_TRY(cnq::launch_item_order(a.item_cost, b->item_last.p, b->item_order.p, (uint32_t)b->n_items, st),
@@ -1331,6 +1355,8 @@ extern "C" {
 void cnq_tuning_default(cnq_tuning* t) {
   t->tile_w = 16;
   t->persistent_blocks = 0;
+  t->waves_per_simd = 2;
+  t->ray_stack = 64;
 }
 
 int cnq_batch_create(cnq_scene* s, const cnq_view* views, uint32_t n_views, const float* lights, uint32_t n_lights,
@@ -1454,32 +1480,59 @@ int cnq_batch_autotune(cnq_batch* b, const cnq_options* o, void* stream, cnq_tun
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->scene->dev->ordinal);
   const uint32_t tiles[] = {16, 8, 32, 4, 64};
   const uint32_t per_cu[] = {0, 4, 8};
+  const uint32_t waves[] = {2, 1};
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0), "event");
   HIP_TRY(hipEventCreate(&e1), "event");
   float best_t = 3.4e38f;
   cnq_tuning best_c{};
+  // Two timed runs of a tuning after a warm one (which also lays the batch out).
+  auto time_it = [&](const cnq_tuning& c, float& total) -> int {
+    int r = cnq_batch_run(b, o, &c, st);
+    total = 0.0f;
+    for (int k = 0; k < 2 && !r; k++) {
+      (void)hipEventRecord(e0, st);
+      r = batch_enqueue(b, o, &c, st);
+      (void)hipEventRecord(e1, st);
+      (void)hipEventSynchronize(e1);
+      float ms = 0.0f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      total += ms;
+    }
+    return r;
+  };
   for (uint32_t tw : tiles) {
     for (uint32_t k : per_cu) {
-      cnq_tuning c{tw, k * (uint32_t)cus};
-      if ((rc = cnq_batch_run(b, o, &c, st))) break;  // warm
-      float total = 0.0f;
-      for (int r = 0; r < 2 && !rc; r++) {
-        (void)hipEventRecord(e0, st);
-        rc = batch_enqueue(b, o, &c, st);
-        (void)hipEventRecord(e1, st);
-        (void)hipEventSynchronize(e1);
-        float ms = 0.0f;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        total += ms;
+      for (uint32_t wv : waves) {
+        const cnq_tuning c{tw, k * (uint32_t)cus, wv, 64};
+        float total = 0.0f;
+        if ((rc = time_it(c, total))) break;
+        if (total < best_t) {
+          best_t = total;
+          best_c = c;
+        }
       }
       if (rc) break;
-      if (total < best_t) {
-        best_t = total;
-        best_c = c;
-      }
     }
     if (rc) break;
+  }
+  // The shallower ray stack, where it truncates no camera ray of the batch
+  // (the scene's truncation counter does not move).
+  if (!rc) {
+    uint32_t before[4] = {0, 0, 0, 0}, after[4] = {0, 0, 0, 0};
+    cnq_tuning c = best_c;
+    c.ray_stack = 32;
+    hipError_t e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(before, b->scene->counters.p, sizeof(before), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) rc = hip_fail(e, "read counters");
+    float total = 0.0f;
+    if (!rc) rc = time_it(c, total);
+    if (!rc && (e = hipMemcpy(after, b->scene->counters.p, sizeof(after), hipMemcpyDeviceToHost)) != hipSuccess)
+      rc = hip_fail(e, "read counters");
+    if (!rc && after[3] == before[3] && total < best_t) {
+      best_t = total;
+      best_c = c;
+    }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

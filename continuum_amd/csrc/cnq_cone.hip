@@ -97,23 +97,38 @@ __global__ __launch_bounds__(64, WALK_WAVES) void cone_items_kernel(
       a.out_visible, a.out_full, a.out_touched, a.out_cov, a.counters, a.work, a.work_counter, a.order, a.item_cost, \
       a.query_flags, a.n_flags, a.p, a.v0
 
-hipError_t launch_cone_items(const ItemArgs& a, bool count_work, uint32_t persistent_blocks, hipStream_t stream) {
+// Occupancy: the walk is compiled for two waves per SIMD (eight a CU: the
+// accumulators' 17 KB of LDS a wave allow that). With `waves` 1 a launch
+// asks for unused LDS on top, 40 KB a wave in all, so a CU holds four of its
+// waves: half the waves in flight, half the lanes' private memory resident
+// in the caches.
+constexpr uint32_t LDS_PER_CU = 160u * 1024u;
+
+uint32_t occupancy_pad(uint32_t waves) {
+  if (waves != 1) return 0;
+  const uint32_t per_wave = LDS_PER_CU / 4u;  // four waves a CU
+  return per_wave - (uint32_t)(ACC_CELLS * 64u * sizeof(float));
+}
+
+hipError_t launch_cone_items(const ItemArgs& a, bool count_work, uint32_t persistent_blocks, hipStream_t stream,
+                             uint32_t waves) {
   if (a.p.n_items <= a.p.item_begin) return hipSuccess;
   const SceneArgs& s = a.s;
-  dim3 block(64);
+  const dim3 block(64);
+  const uint32_t pad = occupancy_pad(waves);
+  const uint32_t n = a.p.n_items - a.p.item_begin;
   if (persistent_blocks) {
-    const uint32_t n = a.p.n_items - a.p.item_begin;
-    dim3 grid(persistent_blocks < n ? persistent_blocks : n);
+    const dim3 grid(persistent_blocks < n ? persistent_blocks : n);
     if (count_work)
-      hipLaunchKernelGGL((cone_items_kernel<true, true>), grid, block, 0, stream, CNQ_ITEMS_ARGS);
+      hipLaunchKernelGGL((cone_items_kernel<true, true>), grid, block, pad, stream, CNQ_ITEMS_ARGS);
     else
-      hipLaunchKernelGGL((cone_items_kernel<false, true>), grid, block, 0, stream, CNQ_ITEMS_ARGS);
+      hipLaunchKernelGGL((cone_items_kernel<false, true>), grid, block, pad, stream, CNQ_ITEMS_ARGS);
   } else {
-    dim3 grid(a.p.n_items - a.p.item_begin);
+    const dim3 grid(n);
     if (count_work)
-      hipLaunchKernelGGL((cone_items_kernel<true, false>), grid, block, 0, stream, CNQ_ITEMS_ARGS);
+      hipLaunchKernelGGL((cone_items_kernel<true, false>), grid, block, pad, stream, CNQ_ITEMS_ARGS);
     else
-      hipLaunchKernelGGL((cone_items_kernel<false, false>), grid, block, 0, stream, CNQ_ITEMS_ARGS);
+      hipLaunchKernelGGL((cone_items_kernel<false, false>), grid, block, pad, stream, CNQ_ITEMS_ARGS);
   }
   return hipGetLastError();
 }

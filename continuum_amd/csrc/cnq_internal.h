@@ -97,6 +97,7 @@ struct cnq_scene {
   uint8_t* query_flags = nullptr;  // cnq_scene_set_query_flags: caller-owned device bytes
   uint64_t n_query_flags = 0;
   cnq::detail::DevBuf<float> tris, tri_leaf, blas_cov, xforms, mesh_box;
+  cnq::detail::DevBuf<float> normals;  // cnq_scene_set_normals: 9 per triangle (none: face normals)
   cnq::detail::DevBuf<uint32_t> blas_items, ranges, items, counters;
   RefitPlan refit;                  // every triangle hierarchy
   std::vector<uint32_t> h_child;    // per node: its first child, NONE for a leaf
@@ -135,7 +136,7 @@ struct cnq_scene {
     return s;
   }
   void release() {
-    tris.release(), tri_leaf.release(), blas_cov.release(), xforms.release(), mesh_box.release();
+    tris.release(), tri_leaf.release(), blas_cov.release(), xforms.release(), mesh_box.release(), normals.release();
     blas_items.release(), ranges.release(), items.release(), refit.release();
     counters.release(), work.release(), sky_faces.release(), order.cost.release(),
         order.order.release(), order.last.release(), sky_order.cost.release(), sky_order.order.release(),
@@ -145,6 +146,7 @@ struct cnq_scene {
   }
   uint64_t bytes() const {
     return tris.bytes() + tri_leaf.bytes() + blas_cov.bytes() + xforms.bytes() + mesh_box.bytes() +
+           (normals.p ? normals.bytes() : 0) +
            blas_items.bytes() + ranges.bytes() + items.bytes() + refit.bytes() +
            counters.bytes() + work.bytes() + blas.bytes() + nodes.bytes();
   }
@@ -164,7 +166,7 @@ namespace detail {
 QueryParams query_params(cnq_scene* s, const cnq_options* o, uint32_t tile_w);
 ItemArgs item_args(cnq_scene* s, const float* receivers, const float* lights, float* vis, uint64_t* full,
                    uint64_t* touched, float* cov);
-hipError_t launch_items_direct(ItemArgs a, uint64_t n_items, bool count_work, hipStream_t stream);
+hipError_t launch_items_direct(ItemArgs a, uint64_t n_items, bool count_work, hipStream_t stream, uint32_t waves);
 int check_object_cov(cnq_scene* s, const cnq_options* o);
 // The refit schedule of the hierarchies under `roots` (child: per node its
 // first child, NONE for a leaf), and one refit of the scene's triangle

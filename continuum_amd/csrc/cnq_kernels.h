@@ -115,6 +115,9 @@ struct PrimaryArgs {
   uint32_t* __restrict__ hits;   // 4 per pixel
   float* __restrict__ receivers; // 8 per pixel
   uint32_t* __restrict__ counters;  // nullable: [stack overflow]
+  // Nullable: per triangle (the scene's order) its corners' shading normals,
+  // 9 floats; the receivers then take the interpolated shading normal.
+  const float* __restrict__ normals = nullptr;
 };
 
 struct RefitArgs {
@@ -195,14 +198,18 @@ struct SkyArgs {
   SkyParams sky;
 };
 
-hipError_t launch_cone_items(const ItemArgs& a, bool count_work, uint32_t persistent_blocks, hipStream_t stream);
+// `waves`: waves per SIMD in flight (2, the most the walk's registers and
+// LDS allow, or 1: the launch reserves LDS so a CU holds four waves).
+hipError_t launch_cone_items(const ItemArgs& a, bool count_work, uint32_t persistent_blocks, hipStream_t stream,
+                             uint32_t waves = 2);
 hipError_t launch_sky(const SkyArgs& a, bool count_work, hipStream_t stream);
 // The next launch's order from this one's item times: longest first, by
 // powers of two, when the times resemble the last launch's (`last`: each
 // item's last power of two, 0xff for none), else natural order (one block;
 // the order within a power of two is arbitrary).
 hipError_t launch_item_order(const uint32_t* cost, uint8_t* last, uint32_t* order, uint32_t n, hipStream_t stream);
-hipError_t launch_primary(const PrimaryArgs& a, hipStream_t stream);
+// `depth`: the rays' stack, 64 entries (default) or 32.
+hipError_t launch_primary(const PrimaryArgs& a, hipStream_t stream, uint32_t depth = 64);
 // Leaves first, then the internal nodes one height at a time: `inner` lists
 // them by height, level_off[h]..level_off[h + 1] (n_levels + 1 entries, on
 // the host and, as level_off_d, on the device) those of height h + 1.

@@ -18,6 +18,9 @@ struct Hit {
 };
 
 constexpr float EDGE_TOLERANCE = 1e-5f;
+// The ray stack's depth: 64 entries a lane by default (16 KB of LDS a
+// 64-lane block), or 32 (8 KB: more waves in flight); rays deeper than the
+// depth stop and are counted as truncated.
 constexpr uint32_t RSTACK = 64;
 
 __device__ __forceinline__ float ray_triangle(f3 o, f3 d, f3 v0, f3 v1, f3 v2) {
@@ -66,6 +69,7 @@ struct RayStack {
   bool overflow;
   bool truncated;
   bool nearest_first;
+  uint32_t depth;  // entries
   __device__ __forceinline__ void set(uint32_t i, uint32_t v) { base[i * 64u] = v; }
   __device__ __forceinline__ uint32_t get(uint32_t i) const { return base[i * 64u]; }
 };
@@ -73,7 +77,7 @@ struct RayStack {
 __device__ __forceinline__ void push_children(RayStack& st, f3 eye, f3 inv, uint32_t left, const NodeC& l,
                                               const NodeC& r, float limit, uint32_t& sp) {
   if (sp + 2 > REF_STACK) st.overflow = true;
-  if (sp + 2 > RSTACK) {
+  if (sp + 2 > st.depth) {
     st.truncated = true;
     return;
   }
@@ -122,7 +126,7 @@ __device__ __forceinline__ Hit nearest_in_mesh(const PrimaryArgs& a, RayStack& s
   if (a.hierarchy == 2) {
     if (root == NONE) return hit;
     f3 inv = recip3(dir);
-    if (base >= RSTACK) {
+    if (base >= st.depth) {
       st.truncated = true;
       return hit;
     }
@@ -179,8 +183,45 @@ __device__ __forceinline__ float rect_distance(const LightRect& l, f3 eye, f3 di
   return t;
 }
 
+// WGSL :2047-2058 `shading_normal` (with `barycentric` :1780-1793 and
+// `normal_to_world` :81-85): the corners' normals of triangle `t` of object
+// `m` interpolated at world point `p` (the face's where that is degenerate),
+// in world space, on the side `toward` points to.
+__device__ __forceinline__ f3 shading_normal(const SceneArgs& s, const float* __restrict__ normals, uint32_t m,
+                                             uint32_t t, f3 p_world, f3 toward) {
+  const uint32_t xform = s.ranges[4 * m + 3];
+  const float* xf = s.xforms + 16 * (size_t)(xform == NONE ? 0 : xform);
+  const f3 r0 = ld3(xf), r1 = ld3(xf + 4), r2 = ld3(xf + 8);
+  f3 p = p_world;
+  if (xform != NONE) {
+    const f3 d = p_world - f3{xf[3], xf[7], xf[11]};
+    p = (r0 * d.x + r1 * d.y + r2 * d.z) * xf[13];
+  }
+  const float* tr = s.tris + 12 * (size_t)t;
+  const f3 v0 = ld3(tr), e1 = ld3(tr + 4) - v0, e2 = ld3(tr + 8) - v0;
+  const f3 d = p - v0;
+  const float d11 = dot3(e1, e1), d12 = dot3(e1, e2), d22 = dot3(e2, e2);
+  const float denom = d11 * d22 - d12 * d12;
+  const float bs = (d22 * dot3(d, e1) - d12 * dot3(d, e2)) / denom;
+  const float bu = (d11 * dot3(d, e2) - d12 * dot3(d, e1)) / denom;
+  const float bt = 1.0f - bs - bu;
+  const f3 face = cross3(e1, e2);
+  const float* nr = normals + 9 * (size_t)t;
+  const f3 sn = ld3(nr) * bt + ld3(nr + 3) * bs + ld3(nr + 6) * bu;
+  const f3 pick = dot3(sn, sn) > 1e-20f ? sn : face;
+  f3 n = pick, geometric = face;
+  if (xform != NONE) {
+    n = f3{dot3(r0, pick), dot3(r1, pick), dot3(r2, pick)};
+    geometric = f3{dot3(r0, face), dot3(r1, face), dot3(r2, face)};
+  }
+  n = norm3(n);
+  if (dot3(geometric, toward) < 0.0f) n = -n;
+  return n;
+}
+
+template <uint32_t DEPTH>
 __global__ __launch_bounds__(64) void primary_kernel(PrimaryArgs a) {
-  __shared__ uint32_t s_stack[RSTACK * 64];
+  __shared__ uint32_t s_stack[DEPTH * 64];
   const uint32_t lane = threadIdx.x & 63u;
   // 8x8 pixel tiles: one wave per tile, so a wave's rays are neighbours.
   const uint32_t tile = blockIdx.x;
@@ -210,7 +251,7 @@ __global__ __launch_bounds__(64) void primary_kernel(PrimaryArgs a) {
   float sy = (1.0f - ((float)y + 0.5f + cam[15]) / p[1] * 2.0f) * p[2];
   const f3 dir = norm3(ld3(cam + 4) + ld3(cam + 8) * sx + ld3(cam + 12) * sy);
   const f3 eye = ld3(cam);
-  RayStack st{s_stack + lane, false, false, a.nearest_first != 0};
+  RayStack st{s_stack + lane, false, false, a.nearest_first != 0, DEPTH};
   Hit hit{3.4e38f, NONE, NONE, f3{0.0f, 0.0f, 0.0f}};
   const SceneArgs& s = a.s;
   if (a.hierarchy >= 1) {
@@ -252,7 +293,7 @@ __global__ __launch_bounds__(64) void primary_kernel(PrimaryArgs a) {
   if (lit) {
     f3 surface = eye + dir * hit.distance;
     point = surface + hit.normal * mx(1e-4f, 2e-6f * hit.distance);
-    normal = hit.normal;
+    normal = a.normals ? shading_normal(s, a.normals, hit.mesh, hit.tri, surface, -dir) : hit.normal;
   }
   float* r = a.receivers + 8 * px;
   r[0] = point.x, r[1] = point.y, r[2] = point.z, r[3] = 0.0f;
@@ -260,9 +301,12 @@ __global__ __launch_bounds__(64) void primary_kernel(PrimaryArgs a) {
   if (a.counters && (st.overflow || st.truncated)) atomicAdd(&a.counters[0], 1u);
 }
 
-hipError_t launch_primary(const PrimaryArgs& a, hipStream_t stream) {
+hipError_t launch_primary(const PrimaryArgs& a, hipStream_t stream, uint32_t depth) {
   if (a.n_tiles == 0) return hipSuccess;
-  hipLaunchKernelGGL(primary_kernel, dim3(a.n_tiles), dim3(64), 0, stream, a);
+  if (depth == 32)
+    hipLaunchKernelGGL(primary_kernel<32>, dim3(a.n_tiles), dim3(64), 0, stream, a);
+  else
+    hipLaunchKernelGGL(primary_kernel<RSTACK>, dim3(a.n_tiles), dim3(64), 0, stream, a);
   return hipGetLastError();
 }
 

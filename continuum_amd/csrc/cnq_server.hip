@@ -224,7 +224,8 @@ hipError_t launch_slot(cnq_server* sv, Slot& s) {
   pa.hits = sv->hits.p;
   pa.receivers = sv->receivers.p;
   pa.counters = sc->counters.p + 3;
-  e = cnq::launch_primary(pa, s.stream);
+  pa.normals = sc->normals.p;
+  e = cnq::launch_primary(pa, s.stream, sv->cfg.tuning.ray_stack ? sv->cfg.tuning.ray_stack : 64);
   if (e != hipSuccess) return e;
   if (items) {
     cnq::ItemArgs a = item_args(sc, sv->receivers.p, s.lights.p, sv->visible.p, sv->full.p, sv->touched.p, nullptr);
@@ -247,7 +248,8 @@ hipError_t launch_slot(cnq_server* sv, Slot& s) {
         e = fill_u32(s.work_counter.p, 0, 1, s.stream);
         if (e != hipSuccess) return e;
       }
-      e = cnq::launch_cone_items(a, sc->count_work, sv->cfg.tuning.persistent_blocks, s.stream);
+      e = cnq::launch_cone_items(a, sc->count_work, sv->cfg.tuning.persistent_blocks, s.stream,
+                                 sv->cfg.tuning.waves_per_simd);
       if (e != hipSuccess) return e;
     }
   }

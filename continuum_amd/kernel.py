@@ -369,6 +369,16 @@ class Scene:
     def set_count_work(self, on: bool) -> None:
         check(lib().cnq_scene_set_count_work(self.handle, int(on)))
 
+    def set_normals(self, normals=None) -> None:
+        """Corner shading normals, 9 floats per triangle in the flat's order
+        (None: face normals): the camera pass's receivers then take the
+        interpolated shading normal."""
+        if normals is None:
+            check(lib().cnq_scene_set_normals(self.handle, None, 0))
+            return
+        nrm = np.ascontiguousarray(normals, dtype=np.float32).reshape(-1, 9)
+        check(lib().cnq_scene_set_normals(self.handle, _ptr(nrm), len(nrm)))
+
     def set_query_flags(self, flags=None) -> None:
         """Per-query walk flags (a device uint8 tensor, or None for none):
         bit 0 capped, bit 1 deeper than 48 entries, bit 2 truncated. The
@@ -411,8 +421,8 @@ class Batch:
         return int(lib().cnq_batch_queries(self.handle))
 
     @staticmethod
-    def tuning(tile_w: int = 16, persistent_blocks: int = 0) -> _lib.Tuning:
-        return _lib.Tuning(tile_w, persistent_blocks)
+    def tuning(tile_w: int = 16, persistent_blocks: int = 0, waves_per_simd: int = 2, ray_stack: int = 64) -> _lib.Tuning:
+        return _lib.Tuning(tile_w, persistent_blocks, waves_per_simd, ray_stack)
 
     def run(self, opts: _lib.Options, tuning: _lib.Tuning, stream=None) -> None:
         check(lib().cnq_batch_run(self.handle, C.byref(opts), C.byref(tuning), _stream_ptr(stream)))

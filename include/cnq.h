@@ -247,10 +247,23 @@ int cnq_view_irradiance_device(cnq_scene* scene, const float* receivers, uint32_
 /* Camera rays for a width x height view (the WGSL `visibility` entry,
  * mask_cones.wgsl:2227, with `camera_hit` :2222): per pixel, hits (mesh,
  * triangle, distance bits, seen light + 1) and receivers (the shading point
- * offset by `surface_offset`, the face normal toward the camera; w = -1
- * where the pixel sees the sky or a light). Device buffers. */
+ * offset by `surface_offset`, the face normal toward the camera or, with
+ * cnq_scene_set_normals, the interpolated shading normal; w = -1 where the
+ * pixel sees the sky or a light). Device buffers. */
 int cnq_primary_device(cnq_scene* scene, const float* camera, uint32_t width, uint32_t height, const float* lights,
                        uint32_t n_lights, const cnq_options* opts, uint32_t* hits, float* receivers, void* stream);
+
+/* Corner shading normals (the reference's per-triangle `normals`, vec4 x 3
+ * per triangle in the scene's triangle order, lib.rs:3002
+ * `render_buffers_with`; `Renderer::set_vertices(g, tris, normals)`,
+ * renderer.rs:430): 9 floats per triangle, n_tris the scene's count; NULL
+ * drops them. With normals, every camera pass (cnq_primary_device,
+ * cnq_render_view, batches, served views) gives its receivers the
+ * interpolated shading normal (`shading_normal`, mask_cones.wgsl
+ * :2047-2058: the corners' normals at the hit's barycentrics, the face's
+ * where they sum to nothing, turned to the world, on the camera's side);
+ * the shading point stays offset along the face normal. Synchronous. */
+int cnq_scene_set_normals(cnq_scene* scene, const float* normals, uint64_t n_tris);
 
 /* The plain per-view launch (MaskConeKernel::render_instanced_lights,
  * lib.rs:1857, without shading): camera rays, then one query per pixel and
@@ -302,12 +315,21 @@ typedef struct cnq_view {
   uint32_t light_first, light_count;
 } cnq_view;
 
-/* How the cone launch is shaped: the pixel tile a wave takes (tile_w x
- * 64 / tile_w) and, if non-zero, a persistent grid of that many one-wave
- * blocks that take work items from a counter (else one block per item). */
+/* How the passes are shaped: the pixel tile a cone wave takes (tile_w x
+ * 64 / tile_w); if non-zero, a persistent grid of that many one-wave blocks
+ * that take work items from a counter (else one block per item); the cone
+ * walk's waves in flight per SIMD (2, the most its registers and LDS allow,
+ * or 1: half the waves resident, half the lanes' private memory in the
+ * caches; 0 means 2); and the camera rays' stack depth (64 entries, or 32:
+ * half the LDS, more waves in flight; a ray deeper than the depth stops and
+ * is counted as truncated; 0 means 64). Results do not depend on the
+ * tuning, except that a ray stack too shallow for the scene truncates rays
+ * (the autotuner takes 32 only where it truncates none). */
 typedef struct cnq_tuning {
   uint32_t tile_w;
   uint32_t persistent_blocks;
+  uint32_t waves_per_simd;
+  uint32_t ray_stack;
 } cnq_tuning;
 
 typedef struct cnq_batch cnq_batch;
@@ -324,7 +346,8 @@ int cnq_batch_run(cnq_batch* batch, const cnq_options* opts, const cnq_tuning* t
 int cnq_batch_capture(cnq_batch* batch, const cnq_options* opts, const cnq_tuning* tuning);
 int cnq_batch_replay(cnq_batch* batch, void* stream);
 /* Time every candidate tuning on this batch (tile shapes x persistent grid
- * sizes) and return the fastest; results do not depend on the tuning. */
+ * sizes x waves per SIMD, then the shallower ray stack if it truncates no
+ * ray) and return the fastest; results do not depend on the tuning. */
 int cnq_batch_autotune(cnq_batch* batch, const cnq_options* opts, void* stream, cnq_tuning* best, float* best_ms);
 /* Synchronize and copy one view's hits and query results. */
 int cnq_batch_read(cnq_batch* batch, uint32_t view, uint32_t* hits, float* out_visible, uint64_t* out_full,

@@ -1801,6 +1801,32 @@ static float rect_distance(const Light& l, V3 eye, V3 dir) {
   return t;
 }
 
+// WGSL :2047-2058 `shading_normal` (with `barycentric` :1780-1793,
+// `point_to_object` :74-78, `normal_to_world` :81-85): triangle t's corner
+// normals (9 floats per triangle in `normals`) at world point p, the face's
+// where they sum to nothing, in world space on the side `toward` points to.
+static V3 shading_normal(const Scene& s, const float* normals, uint32_t m, uint32_t t, V3 p_world, V3 toward) {
+  const uint32_t x = s.ranges[4 * m + 3];
+  V3 p = point_to_object(s, m, p_world);
+  const float* tr = s.tris + 12u * t;
+  V3 v0 = xyz(tr), e1 = xyz(tr + 4) - v0, e2 = xyz(tr + 8) - v0;
+  V3 d = p - v0;
+  float d11 = dot(e1, e1), d12 = dot(e1, e2), d22 = dot(e2, e2);
+  float denom = d11 * d22 - d12 * d12;
+  float bs = (d22 * dot(d, e1) - d12 * dot(d, e2)) / denom;
+  float bu = (d11 * dot(d, e2) - d12 * dot(d, e1)) / denom;
+  float bt = 1.0f - bs - bu;
+  V3 face = cross(e1, e2);
+  const float* nr = normals + 9u * t;
+  V3 sn = xyz(nr) * bt + xyz(nr + 3) * bs + xyz(nr + 6) * bu;
+  V3 pick = dot(sn, sn) > 1e-20f ? sn : face;
+  V3 n = x == NONE ? pick : rotate_to_world(s, x, pick);
+  V3 geometric = x == NONE ? face : rotate_to_world(s, x, face);
+  n = normalize(n);
+  if (dot(geometric, toward) < 0.0f) n = -n;
+  return n;
+}
+
 // camera_dir (WGSL :1769-1774). cam: eye4 forward4 right4 up4 params4.
 static V3 camera_dir(const float* cam, uint32_t x, uint32_t y) {
   const float* p = cam + 16;
@@ -1819,12 +1845,15 @@ extern "C" {
 // distance bits, seen light index + 1) and receivers[8*p] = (point, 0,
 // normal, 0 where a surface is shaded, -1 where the pixel sees the sky or a
 // light and no cone is traced). The shading normal is the hit
-// triangle's face normal toward the camera (flat shading). Returns the
-// number of rays whose stack would overflow 48 entries.
+// triangle's face normal toward the camera (flat shading), or with
+// `normals` (nullable; 9 floats per triangle) the interpolated one
+// (`shading_normal`, as `per_pixel_hit_color` takes it, WGSL :2280). Returns
+// the number of rays whose stack would overflow 48 entries.
 int oracle_primary(const float* tris, const uint32_t* ranges, uint32_t n_objects, const float* blas,
                    const uint32_t* blas_items, const float* nodes, const uint32_t* items, const float* xforms,
                    uint32_t hierarchy, uint32_t nearest_first, const float* cam, uint32_t w, uint32_t h,
-                   const float* lights, uint32_t n_lights, uint32_t* hits, float* receivers, int threads) {
+                   const float* lights, uint32_t n_lights, uint32_t* hits, float* receivers, int threads,
+                   const float* normals) {
   Scene s = make_scene(tris, ranges, n_objects, blas, blas_items, nullptr, nodes, items, xforms);
   const Light* ls = reinterpret_cast<const Light*>(lights);
   std::atomic<uint64_t> next{0};
@@ -1864,7 +1893,7 @@ int oracle_primary(const float* tris, const uint32_t* ranges, uint32_t n_objects
         if (lit) {
           V3 surface = eye + dir * hit.distance;
           point = surface + hit.normal * fmax_(1e-4f, 2e-6f * hit.distance);
-          normal = hit.normal;
+          normal = normals ? shading_normal(s, normals, hit.mesh, hit.tri, surface, -dir) : hit.normal;
         }
         r[0] = point.x, r[1] = point.y, r[2] = point.z, r[3] = 0.0f;
         r[4] = normal.x, r[5] = normal.y, r[6] = normal.z, r[7] = lit ? 0.0f : -1.0f;

@@ -28,7 +28,7 @@ def lib() -> C.CDLL:
         sig = {
             "oracle_irradiance": (C.c_int, [_P] * 9 + [_P, C.c_uint64, _P, C.c_uint32, _P, _P, _P, _P, _P, _P, C.c_int]),
             "oracle_primary": (C.c_int, [_P, _P, C.c_uint32, _P, _P, _P, _P, _P, C.c_uint32, C.c_uint32, _P,
-                                         C.c_uint32, C.c_uint32, _P, C.c_uint32, _P, _P, C.c_int]),
+                                         C.c_uint32, C.c_uint32, _P, C.c_uint32, _P, _P, C.c_int, _P]),
             "oracle_flatten": (C.c_int, [_P, _P, C.c_uint32, _P, _P, _P, _P, _P, _P, _P, _P]),
             "oracle_refit": (None, [_P, _P, C.c_uint32, _P, _P, _P, C.c_uint32]),
             "oracle_atan2": (C.c_float, [C.c_float, C.c_float]),
@@ -217,17 +217,23 @@ def sky_face(flat, point, normal, face: int, opts: np.ndarray, sky_opts=None):
 
 
 def primary(flat, camera: np.ndarray, width: int, height: int, lights: np.ndarray, hierarchy=2, nearest_first=1,
-            threads=None):
-    """(hits (px, 4) u32, receivers (px, 8) f32, overflows)."""
+            threads=None, normals=None):
+    """(hits (px, 4) u32, receivers (px, 8) f32, overflows). With `normals`
+    (9 f32 per triangle, the flat's triangle order) the receivers take the
+    interpolated shading normal."""
     camera = np.ascontiguousarray(camera, dtype=np.float32)
     lights = np.ascontiguousarray(lights, dtype=np.float32).reshape(-1, 16)
     px = width * height
     hits = np.zeros((px, 4), np.uint32)
     recv = np.zeros((px, 8), np.float32)
     xf = flat.xforms if len(flat.xforms) else np.zeros((1, 16), np.float32)
+    if normals is not None:
+        nrm = np.ascontiguousarray(normals, dtype=np.float32).reshape(-1, 9)
+        assert len(nrm) == len(flat.tris), "one row of 9 normals per triangle"
     of = lib().oracle_primary(_p(flat.tris), _p(flat.ranges), len(flat.ranges), _p(flat.blas), _p(flat.blas_items),
                               _p(flat.nodes), _p(flat.items), _p(xf), hierarchy, nearest_first, _p(camera), width,
-                              height, _p(lights), len(lights), _p(hits), _p(recv), threads or threads_default())
+                              height, _p(lights), len(lights), _p(hits), _p(recv), threads or threads_default(),
+                              _p(nrm) if normals is not None else None)
     return hits, recv, of
 
 

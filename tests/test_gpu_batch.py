@@ -69,8 +69,14 @@ def test_every_tuning_and_every_replay_agree(device):
         for pb in (0, 7, 256):
             batch.run(opts, Batch.tuning(tw, pb))
             same((tw, pb))
+    # occupancy (one wave per SIMD: LDS reserved) and the ray stack's depth
+    for wv, rs in ((1, 64), (2, 32), (1, 32), (0, 0)):
+        for tw, pb in ((16, 0), (8, 512)):
+            batch.run(opts, Batch.tuning(tw, pb, wv, rs))
+            same((tw, pb, wv, rs))
     best, ms = batch.autotune(opts)
-    assert best.tile_w in (4, 8, 16, 64) and ms > 0
+    assert best.tile_w in (4, 8, 16, 32, 64) and best.waves_per_simd in (1, 2) and best.ray_stack in (32, 64)
+    assert ms > 0
     batch.run(opts, best)
     same("autotuned")
     # Graph replays, with eager work (a single-view query) and synchronisation between them.

@@ -1,13 +1,18 @@
 #!/usr/bin/env python3
 """Cone-visibility throughput on one MI355X.
 
-A step is one plain per-view launch over the 1M-triangle rock field: the
-camera pass (1280 x 720 shading points) and one cone query per shading point
-and light (3 lights: a 0.53 degree sun disk and two area lights), on inputs
-already resident in HBM. Prints one JSON line: queries/s over the timed
-steps, p50/p99 view latency, the cone kernel's roofline, the oracle's
-throughput on the host's cores for a bounded sample of the same queries, and
-a bit-for-bit comparison of that sample.
+A step is one frame of the 3M-triangle courtyard as the plain per-view
+launch renders it: the curtains (1.2M triangles) move by a seeded sine field
+and their triangle hierarchies are refit on the GPU, then the camera pass
+(1280 x 720 shading points) and one cone query per shading point and light
+(4 lights: a 0.53 degree sun disk and three rectangles), on inputs already
+resident in HBM. Prints one JSON line: queries/s over the timed steps,
+p50/p99 frame latency, the cone kernel's roofline, the oracle's throughput
+on the host's cores for a bounded sample of the same queries (and one
+thread's ns per query), and a bit-for-bit comparison of that sample. Under
+`forms`: the courtyard's 64-view batch against 64 sequential plain
+launches, the rock field's plain frame and sky, the served boulder field,
+the view cache and the lightmap cache.
 """
 from __future__ import annotations
 
@@ -23,6 +28,11 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 METRIC = "cone-visibility queries/s; p50/p99 view latency, batched/served vs plain"
+
+
+def note(msg: str) -> None:
+    """A progress line on stderr as each leg ends (stdout keeps the one JSON line)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
 PEAK_HBM_GBPS = 8000.0
 PROFILE_DIR = ROOT / "profiles"
 
@@ -66,75 +76,196 @@ def read_traffic(workload: str):
     return None
 
 
-def courtyard_forms(device, steps: int) -> dict:
-    """The 3M-triangle courtyard: (1) the plain per-view frame with its
-    per-frame refit (curtains displaced by a sine field of time), camera pass
-    and cones for 1280x720 x 4 lights; (2) 64 views of uneven size and 1-16
-    lights as 64 sequential plain launches; (3) the same views as one batch
-    (autotuned on a subset, captured as a hipGraph and replayed)."""
+def posed_oracle_flat(flat, counts, pose):
+    """The oracle's arrays after a refit to `pose`: its refit of every
+    triangle hierarchy and its object hierarchy over the posed meshes."""
+    import types
+
+    from oracle import oracle as O
+
+    rblas, rcov = O.refit(flat, pose)
+    nodes, items = O.mesh_hierarchy(pose.reshape(-1, 3, 4)[:, :, :3].reshape(-1, 9), counts)
+    return types.SimpleNamespace(tris=pose, ranges=flat.ranges, blas=rblas, blas_items=flat.blas_items,
+                                 blas_cov=rcov, nodes=nodes, items=items, xforms=flat.xforms)
+
+
+def plain_frames(scene, sc, steps: int, warmup: int, tile_w: int, poses=None) -> dict:
+    """The plain per-view launch, `steps` frames after `warmup`: (the
+    refit to frame f's pose, when `poses` is given,) the camera pass and one
+    cone query per pixel and light, all on torch's current stream. HIP
+    events bracket each pass of each timed frame; the wall clock brackets
+    the frames (the refit reads the meshes' bounds back to the host once a
+    frame, so frames do not overlap). Before the warmup, the first launch of
+    the view (natural item order, no times yet) is timed on its own. After
+    the timed frames: one frame with the walk counters on (the roofline's
+    bytes), and one with per-query flags (the capped walks)."""
     import numpy as np
     import torch
 
     from continuum_amd import _lib
-    from continuum_amd.kernel import Batch, Flat, Scene
-    from continuum_amd.scenes import courtyard, courtyard_frame, meshes_to_arrays
-    from continuum_amd.workloads import courtyard_views, view_queries
 
-    out = {}
-    sc = courtyard()
-    tris9, counts = meshes_to_arrays(sc.meshes)
-    flat = Flat.build_arrays(tris9, counts)
-    scene = Scene(device, flat)
-    opts = _lib.default_options(hierarchy=2, prefilter=sc.prefilter)
     dev = torch.device("cuda:0")
-    stream = torch.cuda.current_stream()
+    opts = _lib.default_options(hierarchy=2, prefilter=sc.prefilter)
     W, H, nl = sc.width, sc.height, len(sc.lights)
+    px, Q = W * H, W * H * nl
     cam = torch.from_numpy(sc.camera).to(dev)
     lights = torch.from_numpy(np.ascontiguousarray(sc.lights)).to(dev)
-    hits = torch.empty((W * H, 4), dtype=torch.int32, device=dev)
-    recv = torch.empty((W * H, 8), dtype=torch.float32, device=dev)
-    vis = torch.empty(W * H * nl, dtype=torch.float32, device=dev)
-    full = torch.empty(W * H * nl, dtype=torch.int64, device=dev)
-    touched = torch.empty(W * H * nl, dtype=torch.int64, device=dev)
-    poses = [torch.from_numpy(courtyard_frame(sc, f, flat.tris, flat.ranges)).to(dev) for f in range(steps + 1)]
-    torch.cuda.synchronize()
+    hits = torch.empty((px, 4), dtype=torch.int32, device=dev)
+    recv = torch.empty((px, 8), dtype=torch.float32, device=dev)
+    vis = torch.empty(Q, dtype=torch.float32, device=dev)
+    full = torch.empty(Q, dtype=torch.int64, device=dev)
+    touched = torch.empty(Q, dtype=torch.int64, device=dev)
+    stream = torch.cuda.current_stream()
 
-    def frame(f):
-        scene.refit_device(poses[f], stream)
+    def frame(f, ev=None):
+        if ev is not None:
+            ev[0].record(stream)
+        if poses is not None:
+            scene.refit_device(poses[f % len(poses)], stream)
+        if ev is not None:
+            ev[1].record(stream)
         scene.primary_device(cam, W, H, lights, opts, hits, recv, stream)
-        scene.view_query_device(recv, W, H, lights, opts, vis, full, touched, tile_w=PLAIN_TILE_W, stream=stream)
+        if ev is not None:
+            ev[2].record(stream)
+        scene.view_query_device(recv, W, H, lights, opts, vis, full, touched, tile_w=tile_w, stream=stream)
+        if ev is not None:
+            ev[3].record(stream)
 
-    frame(0)
+    first = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    frame(0, first)
     torch.cuda.synchronize()
-    # The refit alone, wall clock: leaves, internal nodes by height, mesh
-    # bounds, the host's object hierarchy over them and its upload.
-    refit_ms = []
-    for f in range(1, min(steps, 10) + 1):
-        t0 = time.perf_counter()
-        scene.refit_device(poses[f], stream)
-        torch.cuda.synchronize()
-        refit_ms.append((time.perf_counter() - t0) * 1e3)
-    lat = []
+    first_cone_ms = first[2].elapsed_time(first[3])
+    for k in range(warmup):
+        frame(1 + k)
+    torch.cuda.synchronize()
+    scene.counters(reset=True)
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(4)] for _ in range(steps)]
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for f in range(1, steps + 1):
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(stream)
-        frame(f)
-        e1.record(stream)
-        lat.append((e0, e1))
+    for k in range(steps):
+        frame(1 + warmup + k, ev[k])
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    ms = np.array([a.elapsed_time(b) for a, b in lat])
-    q = W * H * nl
-    out["courtyard_plain"] = {
-        "workload": "courtyard 3.05M triangles, 1280x720 x 4 lights, per-frame GPU refit of the curtains",
-        "queries_per_s": steps * q / wall, "ms_per_frame": wall / steps * 1e3,
-        "p50_ms": float(np.percentile(ms, 50)), "p99_ms": float(np.percentile(ms, 99)),
-        "triangles": int(sc.n_tris), "refit_ms_p50": float(np.percentile(refit_ms, 50))}
+    lat = np.array([e[0].elapsed_time(e[3]) for e in ev])
+    refit_ms = np.array([e[0].elapsed_time(e[1]) for e in ev])
+    prim_ms = np.array([e[1].elapsed_time(e[2]) for e in ev])
+    cone_ms = np.array([e[2].elapsed_time(e[3]) for e in ev])
+    counters, _ = scene.counters(reset=True)
+    last = 1 + warmup + steps - 1
+    # untimed: the same (last) frame with the walk counters on, then with flags
+    scene.set_count_work(True)
+    frame(last)
+    torch.cuda.synchronize()
+    _, work = scene.counters(reset=True)
+    scene.set_count_work(False)
+    flags = torch.zeros(Q, dtype=torch.uint8, device=dev)
+    scene.set_query_flags(flags)
+    frame(last)
+    torch.cuda.synchronize()
+    scene.set_query_flags(None)
+    h_flags = flags.cpu().numpy()
+    h_recv = recv.cpu().numpy()
+    lit = int((h_recv[:, 7] >= 0).sum())
+    return {
+        "W": W, "H": H, "nl": nl, "Q": Q, "lit": lit, "wall": wall, "steps": steps, "last_frame": last,
+        "lat": lat, "refit_ms": refit_ms, "prim_ms": prim_ms, "cone_ms": cone_ms, "first_cone_ms": first_cone_ms,
+        "counters": counters, "work": work, "flags": h_flags, "h_recv": h_recv,
+        "h_hits": hits.cpu().numpy(), "h_vis": vis.cpu().numpy(), "h_full": full.cpu().numpy().view(np.uint64),
+        "h_touched": touched.cpu().numpy().view(np.uint64), "recv": recv,
+    }
 
+
+def roofline_of(m, workload: str) -> dict:
+    """The cone kernel's roofline from a plain_frames measurement: the
+    algorithmic bytes of one launch (the counted walk's work, priced per
+    unit) over its mean time in the timed frames."""
+    traced_q = m["lit"] * m["nl"]
+    algo = algorithmic_bytes(m["work"], traced_q, m["lit"], m["nl"])
+    cone_s = float(m["cone_ms"].mean()) / 1e3
+    achieved = algo["total"] / cone_s / 1e9
+    w = m["work"]
+    return {"bound": "hbm", "kernel": "cone_items_kernel", "achieved": achieved, "peak": PEAK_HBM_GBPS,
+            "unit": "GB/s", "frac": achieved / PEAK_HBM_GBPS, "traffic": read_traffic(workload),
+            "algorithmic_bytes_per_launch": algo["total"], "kernel_ms": float(m["cone_ms"].mean()),
+            "work": {"object_nodes": int(w[0]), "blas_nodes": int(w[1]), "triangles_tested": int(w[2]),
+                     "triangles_drawn": int(w[3]), "meshes_entered": int(w[4]), "splats": int(w[5])},
+            "per_query": {k: int(x) / max(traced_q, 1) for k, x in
+                          zip(("object_nodes", "blas_nodes", "triangles_tested", "triangles_drawn",
+                               "meshes_entered", "splats"), w)}}
+
+
+def oracle_check(m, oflat, sc, cpu_seconds: float, single_seconds: float = 0.0, camera: bool = True) -> dict:
+    """The oracle on the host's cores over a bounded sample of the measured
+    frame's queries (every stride-th shading point, all lights; receivers
+    from the GPU camera pass), its rate, and the sample compared bit for bit
+    with the GPU's results (queries capped on either side excluded and
+    counted). With `single_seconds`, also one thread's ns per query on a
+    smaller sample. With `camera`, every pixel's camera hit is compared."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    threads = O.threads_default()
+    o_opts = O.opts_array(hierarchy=2, prefilter=sc.prefilter)
+    W, H, nl, Q = m["W"], m["H"], m["nl"], m["Q"]
+    px = W * H
+    h_recv = m["h_recv"]
+    probe = h_recv[::997]
+    t = time.perf_counter()
+    O.irradiance(oflat, probe, sc.lights, o_opts, threads=threads)
+    rate = len(probe) * nl / max(time.perf_counter() - t, 1e-6)
+    stride = max(1, int(np.ceil(Q / max(rate * cpu_seconds, 1.0))))
+    pix = np.arange(0, px, stride)
+    t = time.perf_counter()
+    rv, rf, rt, _, capped, _ = O.irradiance(oflat, h_recv[pix], sc.lights, o_opts, threads=threads)
+    t_cpu = time.perf_counter() - t
+    sq = len(pix) * nl
+    q = (np.arange(nl)[:, None] * px + pix[None, :]).reshape(-1)
+    gv, gf, gt = m["h_vis"][q], m["h_full"][q], m["h_touched"][q]
+    skip = (capped != 0) | ((m["flags"][q] & 1) != 0)
+    ok = ~skip
+    out = {"cpu_baseline": {
+        "value": sq / t_cpu, "unit": "queries/s", "cores": threads, "kind": "port",
+        "sample": f"every {stride}th shading point of the {W}x{H} view x {nl} lights = {sq} queries "
+                  f"({t_cpu:.1f} s on {threads} threads; cones only, receivers from the GPU camera pass)"}}
+    if single_seconds > 0:
+        s_stride = max(1, int(np.ceil(Q / max(rate / threads * single_seconds, 1.0))))
+        s_pix = np.arange(0, px, s_stride)
+        t = time.perf_counter()
+        O.irradiance(oflat, h_recv[s_pix], sc.lights, o_opts, threads=1)
+        t1 = time.perf_counter() - t
+        out["cpu_baseline"]["single_thread_ns_per_query"] = t1 / (len(s_pix) * nl) * 1e9
+        out["cpu_baseline"]["single_thread_sample"] = f"every {s_stride}th shading point x {nl} lights"
+    par = {"sample_queries": sq,
+           "visible_bits_differ": int((gv.view(np.uint32)[ok] != rv.view(np.uint32)[ok]).sum()),
+           "full_masks_differ": int((gf[ok] != rf[ok]).sum()),
+           "touched_masks_differ": int((gt[ok] != rt[ok]).sum()),
+           "capped_skipped": int(skip.sum()),
+           "capped_gpu_only": int(((m["flags"][q] & 1) != 0).sum() - ((capped != 0) & ((m["flags"][q] & 1) != 0)).sum()),
+           "capped_oracle_only": int(((capped != 0) & ((m["flags"][q] & 1) == 0)).sum())}
+    if camera:
+        rh, _, _ = O.primary(oflat, sc.camera, W, H, sc.lights, threads=threads)
+        par["camera_hits_differ"] = int((m["h_hits"].view(np.uint32) != rh).any(axis=1).sum())
+    out["parity"] = par
+    return out
+
+
+def courtyard_batch_form(device, scene, opts) -> dict:
+    """64 views of the courtyard (sizes from {256^2, 640x360, 1280x720,
+    1920x1080}, 1-16 lights each): as 64 sequential plain launches (camera
+    pass + cone pass per view), and as one batch (autotuned on every eighth
+    view, captured as a hipGraph and replayed). The batch's last view is
+    compared with its plain launch bit for bit."""
+    import numpy as np
+    import torch
+
+    from continuum_amd.kernel import Batch
+    from continuum_amd.workloads import courtyard_views, view_queries
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
     views, vlights = courtyard_views()
     total_q = view_queries(views)
-    # 64 sequential plain launches (camera pass + cone pass per view).
     maxpx = max(w * h for _, w, h, _, _ in views)
     maxq = max(w * h * n for _, w, h, _, n in views)
     s_hits = torch.empty((maxpx, 4), dtype=torch.int32, device=dev)
@@ -171,27 +302,43 @@ def courtyard_forms(device, steps: int) -> dict:
         batch.replay()
     torch.cuda.synchronize()
     t_batch = (time.perf_counter() - t0) / reps
-    # The sequential buffers still hold the last view: the batch's results
-    # for it must be the same bits.
-    _, lw, lh, _, ln = views[-1]
+    # The batch's last view against its own plain launch, bit for bit: both
+    # once more (untimed) with per-query flags, the batch uncaptured (the
+    # graph holds no flags). Walks capped on either side are excluded: a
+    # capped walk keeps the nodes its packet reached, and the autotuned tile
+    # can group other pixels into a wave than the plain launch's 16 x 4.
+    _, lw, lh, lf, ln = views[-1]
     lq = lw * lh * ln
+    p_flags = torch.zeros(lq, dtype=torch.uint8, device=dev)
+    scene.set_query_flags(p_flags)
+    scene.primary_device(v_cams[-1], lw, lh, v_lights[-1], opts, s_hits, s_recv, stream)
+    scene.view_query_device(s_recv, lw, lh, v_lights[-1], opts, s_vis, s_full, s_touched, tile_w=PLAIN_TILE_W,
+                            stream=stream)
+    b_flags = torch.zeros(total_q, dtype=torch.uint8, device=dev)
+    scene.set_query_flags(b_flags)
+    batch.run(opts, best)
+    torch.cuda.synchronize()
+    scene.set_query_flags(None)
     b_hits, b_vis, b_full, b_touched = batch.read(len(views) - 1)
+    capped = ((p_flags.cpu().numpy() | b_flags.cpu().numpy()[total_q - lq:]) & 1) != 0
+    ok = ~capped
     last_view_differ = {
         "hits": int((s_hits[:lw * lh].cpu().numpy().view(np.uint32) != b_hits).any(axis=1).sum()),
-        "visible": int((s_vis[:lq].cpu().numpy().view(np.uint32) != b_vis.view(np.uint32)).sum()),
-        "full": int((s_full[:lq].cpu().numpy().view(np.uint64) != b_full).sum()),
-        "touched": int((s_touched[:lq].cpu().numpy().view(np.uint64) != b_touched).sum()),
+        "visible": int((s_vis[:lq].cpu().numpy().view(np.uint32) != b_vis.view(np.uint32))[ok].sum()),
+        "full": int((s_full[:lq].cpu().numpy().view(np.uint64) != b_full)[ok].sum()),
+        "touched": int((s_touched[:lq].cpu().numpy().view(np.uint64) != b_touched)[ok].sum()),
+        "capped_excluded": int(capped.sum()),
         "queries": int(lq)}
-    out["courtyard_64_views"] = {
+    return {
         "workload": "64 views over the courtyard, sizes from {256^2, 640x360, 1280x720, 1920x1080}, 1-16 lights each",
         "queries": total_q,
         "sequential_plain": {"queries_per_s": total_q / t_seq, "s": t_seq},
         "batched_graph": {"queries_per_s": total_q / t_batch, "s": t_batch,
-                          "tuning": {"tile_w": best.tile_w, "persistent_blocks": best.persistent_blocks}},
+                          "tuning": {"tile_w": best.tile_w, "persistent_blocks": best.persistent_blocks,
+                                     "waves_per_simd": best.waves_per_simd, "ray_stack": best.ray_stack}},
         "batched_over_sequential": t_seq / t_batch,
         "last_view_batched_vs_plain_differ": last_view_differ,
     }
-    return out
 
 
 # Served form: cone launches cut into pieces of at most 4M queries, and a
@@ -199,6 +346,93 @@ def courtyard_forms(device, steps: int) -> dict:
 # light) ahead of larger ones.
 SERVED_LAUNCH_QUERIES = 1 << 22
 SERVED_EXPRESS_QUERIES = 1 << 20
+
+
+def serve_stream(scene, lights, opts, reqs) -> dict:
+    """Offers `reqs` (serving_requests' list) to a fresh server over `scene` at
+    their arrival times and collects each view as it finishes: throughput,
+    latency (overall and by view size), queueing; keeps the first view of at
+    most 1M queries with its results, for a check against its plain launch."""
+    import threading
+
+    import numpy as np
+
+    from continuum_amd.kernel import Server
+
+    srv = Server(scene, lights, opts, pixel_capacity=40 << 20, query_capacity=400 << 20,
+                 max_lights=len(lights), max_views_per_launch=16, slots=3,
+                 launch_queries=SERVED_LAUNCH_QUERIES, express_queries=SERVED_EXPRESS_QUERIES)
+    first_view = next(p for _, k, p in reqs if k == "view")
+    warm = srv.submit_view(first_view[0], 64, 64, [0])
+    srv.wait(warm)
+    srv.release(warm)
+    ids, done, kept = [], [], []
+    lock = threading.Condition()
+
+    def collector():
+        k = 0
+        while True:
+            with lock:
+                while k >= len(ids):
+                    lock.wait()
+                rid, kind, view = ids[k]
+            if rid is None:
+                return
+            k += 1
+            if kind != "view":
+                continue
+            srv.wait(rid)
+            info = srv.info(rid)
+            # The first view of at most 1M queries is kept, to compare
+            # with its own plain launch afterwards.
+            if not kept and info.width * info.height * info.n_lights <= 1 << 20:
+                kept.append((view, srv.read(rid)))
+            done.append((info.latency_ms, info.queued_ms, info.width * info.height * info.n_lights,
+                         time.perf_counter()))
+            srv.release(rid)
+
+    th = threading.Thread(target=collector)
+    th.start()
+    table = np.array(lights, np.float32, copy=True)  # the light table as each request is submitted
+    t0 = time.perf_counter()
+    for t_arr, kind, payload in reqs:
+        delay = t0 + t_arr - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        rid = srv.submit_view(*payload) if kind == "view" else srv.submit_light(*payload)
+        if kind == "view":
+            cam_, w_, h_, idx_ = payload
+            view = (cam_, w_, h_, table[list(idx_)].copy())
+        else:
+            table[payload[0]] = np.asarray(payload[1], np.float32).reshape(16)
+            view = None
+        with lock:
+            ids.append((rid, kind, view))
+            lock.notify()
+    with lock:
+        ids.append((None, None, None))
+        lock.notify()
+    th.join()
+    wall = max(d[3] for d in done) - t0
+    counts_ = srv.counts()
+    srv.close()
+    lat = np.array([d[0] for d in done])
+    q = sum(d[2] for d in done)
+    # Latency by the view's queries (pixels x lights): at most 1M (a
+    # 1280x720 view with one light), at most 4M, and larger.
+    sizes = np.array([d[2] for d in done])
+    by_size = {}
+    for name, lo, hi in (("le_1M_queries", 0, 1 << 20), ("le_4M_queries", (1 << 20) + 1, 1 << 22),
+                         ("gt_4M_queries", (1 << 22) + 1, 1 << 62)):
+        m = (sizes >= lo) & (sizes <= hi)
+        if m.any():
+            by_size[name] = {"views": int(m.sum()), "p50_ms": float(np.percentile(lat[m], 50)),
+                             "p99_ms": float(np.percentile(lat[m], 99))}
+    return {"requests": len(reqs), "views": len(done), "edits": counts_["edits"], "launches": counts_["launches"],
+            "queries": int(q), "s": wall, "queries_per_s": q / wall,
+            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+            "queued_p50_ms": float(np.percentile([d[1] for d in done], 50)), "by_size": by_size,
+            "_kept": kept}
 
 
 def served_form(device, n_requests: int, seconds: float) -> dict:
@@ -226,85 +460,9 @@ def served_form(device, n_requests: int, seconds: float) -> dict:
     t_build = time.perf_counter() - t_build
     opts = _lib.default_options(hierarchy=2, prefilter=1.0, object_cov=flat.object_cov)
 
-    def run(reqs):
-        srv = Server(scene, lights, opts, pixel_capacity=40 << 20, query_capacity=400 << 20,
-                     max_lights=len(lights), max_views_per_launch=16, slots=3,
-                     launch_queries=SERVED_LAUNCH_QUERIES, express_queries=SERVED_EXPRESS_QUERIES)
-        first_view = next(p for _, k, p in reqs if k == "view")
-        warm = srv.submit_view(first_view[0], 64, 64, [0])
-        srv.wait(warm)
-        srv.release(warm)
-        ids, done, kept = [], [], []
-        lock = threading.Condition()
-
-        def collector():
-            k = 0
-            while True:
-                with lock:
-                    while k >= len(ids):
-                        lock.wait()
-                    rid, kind, view = ids[k]
-                if rid is None:
-                    return
-                k += 1
-                if kind != "view":
-                    continue
-                srv.wait(rid)
-                info = srv.info(rid)
-                # The first view of at most 1M queries is kept, to compare
-                # with its own plain launch afterwards.
-                if not kept and info.width * info.height * info.n_lights <= 1 << 20:
-                    kept.append((view, srv.read(rid)))
-                done.append((info.latency_ms, info.queued_ms, info.width * info.height * info.n_lights,
-                             time.perf_counter()))
-                srv.release(rid)
-
-        th = threading.Thread(target=collector)
-        th.start()
-        table = np.array(lights, np.float32, copy=True)  # the light table as each request is submitted
-        t0 = time.perf_counter()
-        for t_arr, kind, payload in reqs:
-            delay = t0 + t_arr - time.perf_counter()
-            if delay > 0:
-                time.sleep(delay)
-            rid = srv.submit_view(*payload) if kind == "view" else srv.submit_light(*payload)
-            if kind == "view":
-                cam_, w_, h_, idx_ = payload
-                view = (cam_, w_, h_, table[list(idx_)].copy())
-            else:
-                table[payload[0]] = np.asarray(payload[1], np.float32).reshape(16)
-                view = None
-            with lock:
-                ids.append((rid, kind, view))
-                lock.notify()
-        with lock:
-            ids.append((None, None, None))
-            lock.notify()
-        th.join()
-        wall = max(d[3] for d in done) - t0
-        counts_ = srv.counts()
-        srv.close()
-        lat = np.array([d[0] for d in done])
-        q = sum(d[2] for d in done)
-        # Latency by the view's queries (pixels x lights): at most 1M (a
-        # 1280x720 view with one light), at most 4M, and larger.
-        sizes = np.array([d[2] for d in done])
-        by_size = {}
-        for name, lo, hi in (("le_1M_queries", 0, 1 << 20), ("le_4M_queries", (1 << 20) + 1, 1 << 22),
-                             ("gt_4M_queries", (1 << 22) + 1, 1 << 62)):
-            m = (sizes >= lo) & (sizes <= hi)
-            if m.any():
-                by_size[name] = {"views": int(m.sum()), "p50_ms": float(np.percentile(lat[m], 50)),
-                                 "p99_ms": float(np.percentile(lat[m], 99))}
-        return {"requests": len(reqs), "views": len(done), "edits": counts_["edits"], "launches": counts_["launches"],
-                "queries": int(q), "s": wall, "queries_per_s": q / wall,
-                "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-                "queued_p50_ms": float(np.percentile([d[1] for d in done], 50)), "by_size": by_size,
-                "_kept": kept}
-
     # Run 1: everything offered within the first 10 ms.
     burst = serving_requests(n_requests, n_requests / 0.01, lights)
-    r1 = run(burst)
+    r1 = serve_stream(scene, lights, opts, burst)
     views_per_s = r1["views"] / r1["s"]
     # A served view of the burst against its plain launch (the host-read
     # path), bit for bit.
@@ -321,7 +479,7 @@ def served_form(device, n_requests: int, seconds: float) -> dict:
     rate = 0.5 * views_per_s / 0.8
     n2 = max(20, int(rate * seconds))
     steady = serving_requests(n2, rate, lights, seed=0xC0_0008)
-    r2 = run(steady)
+    r2 = serve_stream(scene, lights, opts, steady)
     r2.pop("_kept")
 
     # The same views of run 2 as plain launches, one after another.
@@ -358,6 +516,50 @@ def served_form(device, n_requests: int, seconds: float) -> dict:
         "served_over_plain": r1["queries_per_s"] / (q / t_plain),
         "burst_view_vs_its_plain_launch": served_vs_plain,
     }
+
+
+def served_sweep_form(device, rates, seconds: float) -> dict:
+    """The served boulder field under Poisson streams at increasing offered
+    view rates (views are 80% of the requests, light edits the rest; each
+    stream about `seconds` long, with its own seed): per rate the views
+    done per second, queries/s, p50/p99 latency, and the drain (last
+    completion after the last arrival). A rate is stable when the server
+    keeps up (views done per second at least 90% of offered, over the
+    stream's span) and the drain is no longer than twice the p50 plus 0.5 s."""
+    import numpy as np
+
+    from continuum_amd import _lib
+    from continuum_amd.kernel import Flat, Scene
+    from continuum_amd.scenes import boulder_field
+    from continuum_amd.workloads import serving_requests
+
+    tris9, counts, pl, lights, _, _ = boulder_field()
+    flat = Flat.build_instanced(tris9, counts, pl, object_prefilter=True)
+    scene = Scene(device, flat)
+    opts = _lib.default_options(hierarchy=2, prefilter=1.0, object_cov=flat.object_cov)
+    out = []
+    for k, rate in enumerate(rates):
+        req_rate = rate / 0.8
+        n = max(20, int(req_rate * seconds))
+        reqs = serving_requests(n, req_rate, lights, seed=0xC0_0100 + k)
+        last_arrival = reqs[-1][0]
+        r = serve_stream(scene, lights, opts, reqs)
+        r.pop("_kept")
+        done_rate = r["views"] / max(r["s"], 1e-9)
+        drain_s = max(r["s"] - last_arrival, 0.0)
+        row = {"offered_views_per_s": rate, "views": r["views"], "views_done_per_s": done_rate,
+               "queries_per_s": r["queries_per_s"], "p50_ms": r["p50_ms"], "p99_ms": r["p99_ms"],
+               "queued_p50_ms": r["queued_p50_ms"], "drain_s": drain_s, "by_size": r["by_size"]}
+        row["stable"] = bool(done_rate >= 0.9 * rate and drain_s <= 2.0 * r["p50_ms"] / 1e3 + 0.5)
+        out.append(row)
+        note(f"served at {rate:g} views/s: {done_rate:.2f} done/s, p50 {r['p50_ms']:.0f} ms, "
+             f"p99 {r['p99_ms']:.0f} ms, drain {drain_s:.1f} s")
+    stable = [r["offered_views_per_s"] for r in out if r["stable"]]
+    mean_q = 0.5 * (256 * 256 + 1920 * 1080) * 16.5  # a C5 view's mean queries: pixels x lights
+    return {"workload": "boulder field, Poisson view/edit stream (80% views of 256^2..1920x1080 x 1-32 lights)",
+            "rates": out, "highest_stable_views_per_s": max(stable) if stable else None,
+            "target": {"requests_per_s": 2000, "views_per_s": 1600,
+                       "queries_per_s_needed": 1600 * mean_q}}
 
 
 def view_cache_form(device, boulders: int = 512, moves: int = 5) -> dict:
@@ -657,15 +859,17 @@ def main() -> int:
     ap.add_argument("--tile-w", type=int, default=PLAIN_TILE_W, help="pixels across a wave's tile (tile_w x 64/tile_w)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-forms", action="store_true", help="skip the courtyard's plain/batched forms")
+    ap.add_argument("--no-forms", action="store_true", help="skip the courtyard's 64-view batch")
+    ap.add_argument("--no-rock-field", action="store_true", help="skip the rock field's plain frame (and the sky)")
     ap.add_argument("--no-served", action="store_true", help="skip the served form over the boulder field")
     ap.add_argument("--no-vcache", action="store_true", help="skip the view cache over the boulder field")
     ap.add_argument("--no-sky", action="store_true", help="skip the sky light over the rock field")
     ap.add_argument("--no-lightmap", action="store_true", help="skip the lightmap cache over the lit room")
-    ap.add_argument("--only-form", choices=["courtyard", "served", "view_cache", "lightmap"],
+    ap.add_argument("--only-form", choices=["served", "served_sweep", "view_cache", "lightmap"],
                     help="run one form alone and print its JSON (for profiling it); no headline line")
     ap.add_argument("--served-requests", type=int, default=64, help="requests in the saturated served run")
     ap.add_argument("--served-seconds", type=float, default=10.0, help="length of the steady served run")
+    ap.add_argument("--served-rates", default="0.5,1,2,3,4", help="offered views/s of the served sweep")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -679,14 +883,14 @@ def main() -> int:
 
     from continuum_amd import _lib
     from continuum_amd.kernel import Device, Flat, Scene
-    from continuum_amd.scenes import meshes_to_arrays, rock_field
+    from continuum_amd.scenes import courtyard, courtyard_frame, meshes_to_arrays, rock_field
 
+    device = Device(0)
     if args.only_form:
-        device = Device(0)
-        if args.only_form == "courtyard":
-            out = courtyard_forms(device, steps=5)
-        elif args.only_form == "served":
+        if args.only_form == "served":
             out = served_form(device, n_requests=args.served_requests, seconds=args.served_seconds)
+        elif args.only_form == "served_sweep":
+            out = served_sweep_form(device, [float(x) for x in args.served_rates.split(",")], args.served_seconds)
         elif args.only_form == "lightmap":
             out = lightmap_form(device)
         else:
@@ -694,28 +898,7 @@ def main() -> int:
         print(json.dumps({"form": args.only_form, **out}))
         return 0
 
-    t0 = time.perf_counter()
-    sc = rock_field(width=args.width, height=args.height)
-    tris9, counts = meshes_to_arrays(sc.meshes)
-    flat = Flat.build_arrays(tris9, counts)
-    t_build = time.perf_counter() - t0
     dev = torch.device("cuda:0")
-    device = Device(0)
-    scene = Scene(device, flat)
-    opts = _lib.default_options(hierarchy=2, prefilter=sc.prefilter)
-
-    W, H = sc.width, sc.height
-    px = W * H
-    nl = len(sc.lights)
-    Q = px * nl
-    cam = torch.from_numpy(sc.camera).to(dev)
-    lights = torch.from_numpy(np.ascontiguousarray(sc.lights)).to(dev)
-    hits = torch.empty((px, 4), dtype=torch.int32, device=dev)
-    recv = torch.empty((px, 8), dtype=torch.float32, device=dev)
-    vis = torch.empty(Q, dtype=torch.float32, device=dev)
-    full = torch.empty(Q, dtype=torch.int64, device=dev)
-    touched = torch.empty(Q, dtype=torch.int64, device=dev)
-
     # A copy of known size (256 MiB read, 256 MiB written), for calibrating
     # HBM byte counters when this command runs under rocprofv3 --pmc.
     ca = torch.ones(64 << 20, dtype=torch.float32, device=dev)
@@ -723,151 +906,113 @@ def main() -> int:
     torch.neg(ca, out=cb)
     del ca, cb
 
-    stream = torch.cuda.current_stream()
-
-    def step(e0=None, e1=None, e2=None):
-        if e0 is not None:
-            e0.record(stream)
-        scene.primary_device(cam, W, H, lights, opts, hits, recv, stream)
-        if e1 is not None:
-            e1.record(stream)
-        scene.view_query_device(recv, W, H, lights, opts, vis, full, touched, tile_w=args.tile_w, stream=stream)
-        if e2 is not None:
-            e2.record(stream)
-
-    for _ in range(args.warmup):
-        step()
+    # ---- The headline: the courtyard frame (3.05M triangles; per frame the
+    # curtains move and their hierarchies are refit on the GPU, then the
+    # camera pass and 1280x720 x 4 lights of cone queries).
+    t0 = time.perf_counter()
+    sc = courtyard(width=args.width, height=args.height)
+    tris9, counts = meshes_to_arrays(sc.meshes)
+    flat = Flat.build_arrays(tris9, counts)
+    t_build = time.perf_counter() - t0
+    scene = Scene(device, flat)
+    n_frames = 1 + args.warmup + args.steps
+    h_poses = [courtyard_frame(sc, f, flat.tris, flat.ranges) for f in range(n_frames)]
+    poses = [torch.from_numpy(p).to(dev) for p in h_poses]
     torch.cuda.synchronize()
-    scene.counters(reset=True)
-
     K = args.steps
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(K)]
-    torch.cuda.synchronize()
-    t_start = time.perf_counter()
-    for k in range(K):
-        step(*ev[k])
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t_start
-
-    lat = np.array([ev[k][0].elapsed_time(ev[k][2]) for k in range(K)])
-    cone_ms = np.array([ev[k][1].elapsed_time(ev[k][2]) for k in range(K)])
-    prim_ms = np.array([ev[k][0].elapsed_time(ev[k][1]) for k in range(K)])
-    counters, _ = scene.counters(reset=True)
-
-    # One more step with the work counters on (untimed), for the roofline's bytes.
-    scene.set_count_work(True)
-    step()
-    torch.cuda.synchronize()
-    _, work = scene.counters(reset=True)
-    scene.set_count_work(False)
-
-    h_recv = recv.cpu().numpy()
-    lit = int((h_recv[:, 7] >= 0).sum())
-    traced_q = lit * nl
-    algo = algorithmic_bytes(work, traced_q, lit, nl)
-    cone_avg_s = float(cone_ms.mean()) / 1e3
-    achieved = algo["total"] / cone_avg_s / 1e9
-    traffic = read_traffic(sc.name)
-
+    note(f"courtyard built ({sc.n_tris} triangles, {t_build:.1f} s on the host)")
+    m = plain_frames(scene, sc, K, args.warmup, args.tile_w, poses=poses)
+    note(f"courtyard frames: {m['wall'] / K * 1e3:.2f} ms a frame, cone pass {m['cone_ms'].mean():.2f} ms")
+    counters = m["counters"]
+    flags = m["flags"]
     result = {
         "metric": METRIC,
-        "value": K * Q / wall,
+        "value": K * m["Q"] / m["wall"],
         "unit": "queries/s",
         "n_gpus": 1,
         "steps": K,
         "warmup": args.warmup,
-        "ms_per_step": wall / K * 1e3,
+        "ms_per_step": m["wall"] / K * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded rock field; geometry generated, no assets)",
+        "data": "synthetic (seeded courtyard; geometry generated, no assets)",
         "config": {
-            "workload": "rock field, plain per-view launch (camera pass + one cone per pixel and light)",
+            "workload": "courtyard frame: per-frame GPU refit of the displaced curtains, camera pass, one cone "
+                        "query per shading point and light (plain per-view launch)",
             "triangles": int(sc.n_tris),
+            "curtain_triangles": int(sum(len(sc.meshes[i]) for i in sc.notes["curtains"])),
             "objects": len(sc.meshes),
-            "shading_points": px,
-            "lights": nl,
-            "queries_per_step": Q,
-            "traced_queries_per_step": traced_q,
+            "shading_points": m["W"] * m["H"],
+            "lights": m["nl"],
+            "queries_per_step": m["Q"],
+            "traced_queries_per_step": m["lit"] * m["nl"],
             "hierarchy": "triangles",
             "prefilter_bins": sc.prefilter,
             "scene_hbm_bytes": scene.hbm_bytes,
             "host_build_s": round(t_build, 3),
         },
-        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
-                       "camera_pass_mean": float(prim_ms.mean()), "cone_pass_mean": float(cone_ms.mean())},
-        "roofline": {
-            "bound": "hbm",
-            "kernel": "cone_items_kernel",
-            "achieved": achieved,
-            "peak": PEAK_HBM_GBPS,
-            "unit": "GB/s",
-            "frac": achieved / PEAK_HBM_GBPS,
-            "traffic": traffic,
-            "algorithmic_bytes_per_launch": algo["total"],
-            "kernel_ms": float(cone_ms.mean()),
-            "work": {"object_nodes": int(work[0]), "blas_nodes": int(work[1]), "triangles_tested": int(work[2]),
-                     "triangles_drawn": int(work[3]), "meshes_entered": int(work[4]), "splats": int(work[5])},
-        },
-        "diagnostics": {"capped_walks": int(counters[0]), "deeper_than_48": int(counters[1]),
-                        "truncated": int(counters[2])},
+        "latency_ms": {"p50": float(np.percentile(m["lat"], 50)), "p99": float(np.percentile(m["lat"], 99)),
+                       "refit_mean": float(m["refit_ms"].mean()), "camera_pass_mean": float(m["prim_ms"].mean()),
+                       "cone_pass_mean": float(m["cone_ms"].mean()),
+                       "cone_pass_first_launch_natural_order": m["first_cone_ms"]},
+        "roofline": roofline_of(m, "courtyard"),
+        "diagnostics": {"capped_walks": int(counters[0]), "capped_share": int(counters[0]) / m["Q"],
+                        "deeper_than_48": int(counters[1]), "truncated": int(counters[2]),
+                        "flagged_capped_last_frame": int(((flags & 1) != 0).sum())},
     }
-
     if not args.no_cpu_baseline:
-        from oracle import oracle as O
-
-        threads = O.threads_default()
-        o_opts = O.opts_array(hierarchy=2, prefilter=sc.prefilter)
-        h_vis = vis.cpu().numpy().reshape(nl, px)
-        h_full = full.cpu().numpy().view(np.uint64).reshape(nl, px)
-        h_touched = touched.cpu().numpy().view(np.uint64).reshape(nl, px)
-        # calibrate on 1/1024 of the pixels, then a sample of about cpu_seconds
-        probe = h_recv[::1024]
-        t = time.perf_counter()
-        O.irradiance(flat, probe, sc.lights, o_opts, threads=threads)
-        rate = len(probe) * nl / max(time.perf_counter() - t, 1e-6)
-        stride = max(1, int(np.ceil(Q / max(rate * args.cpu_seconds, 1.0))))
-        sample = h_recv[::stride]
-        t = time.perf_counter()
-        rv, rf, rt, _, capped, _ = O.irradiance(flat, sample, sc.lights, o_opts, threads=threads)
-        t_cpu = time.perf_counter() - t
-        sq = len(sample) * nl
-        gv = h_vis[:, ::stride].reshape(-1)
-        gf = h_full[:, ::stride].reshape(-1)
-        gt = h_touched[:, ::stride].reshape(-1)
-        ok = capped == 0
-        result["cpu_baseline"] = {
-            "value": sq / t_cpu,
-            "unit": "queries/s",
-            "cores": threads,
-            "kind": "port",
-            "sample": f"every {stride}th shading point of the {W}x{H} view x {nl} lights = {sq} queries "
-                      f"({t_cpu:.1f} s on {threads} threads; cones only, receivers from the GPU camera pass)",
-        }
-        # camera pass: every pixel against the oracle
-        rh, _, _ = O.primary(flat, sc.camera, W, H, sc.lights, threads=threads)
-        result["parity"] = {
-            "sample_queries": sq,
-            "visible_bits_differ": int((gv.view(np.uint32)[ok] != rv.view(np.uint32)[ok]).sum()),
-            "full_masks_differ": int((gf[ok] != rf[ok]).sum()),
-            "touched_masks_differ": int((gt[ok] != rt[ok]).sum()),
-            "capped_skipped": int((~ok).sum()),
-            "camera_hits_differ": int((hits.cpu().numpy().view(np.uint32) != rh).any(axis=1).sum()),
-        }
-
-    if not args.no_sky:
-        result.setdefault("forms", {})["sky"] = sky_form(scene, flat, sc, recv, h_recv, check=not args.no_cpu_baseline)
+        oflat = posed_oracle_flat(flat, counts, h_poses[m["last_frame"]])
+        chk = oracle_check(m, oflat, sc, args.cpu_seconds, single_seconds=4.0)
+        result["cpu_baseline"] = chk["cpu_baseline"]
+        result["parity"] = chk["parity"]
+        note(f"oracle sample: {chk['cpu_baseline']['value']:.0f} queries/s, parity {chk['parity']}")
+    del m, poses, h_poses
+    torch.cuda.synchronize()
+    forms = result.setdefault("forms", {})
     if not args.no_forms:
-        del scene
-        result.setdefault("forms", {}).update(courtyard_forms(device, steps=5))
+        forms["courtyard_64_views"] = courtyard_batch_form(device, scene,
+                                                           _lib.default_options(hierarchy=2, prefilter=sc.prefilter))
+        b = forms["courtyard_64_views"]
+        b["batched_over_plain_frame_rate"] = b["batched_graph"]["queries_per_s"] / result["value"]
+        note(f"courtyard 64 views: batched/sequential {b['batched_over_sequential']:.2f}")
+    del scene, flat
+
+    # ---- The rock field's plain frame (1.03M triangles, 3 lights), and the
+    # sky over its shading points.
+    if not args.no_rock_field:
+        t0 = time.perf_counter()
+        rf = rock_field(width=args.width, height=args.height)
+        rt9, rc = meshes_to_arrays(rf.meshes)
+        rflat = Flat.build_arrays(rt9, rc)
+        t_rb = time.perf_counter() - t0
+        rscene = Scene(device, rflat)
+        rm = plain_frames(rscene, rf, K, args.warmup, args.tile_w)
+        leg = {"workload": "rock field 1.03M triangles, 1280x720 x 3 lights (sun disk + two rects), plain per-view "
+                           "launch (camera pass + cones)",
+               "queries_per_s": K * rm["Q"] / rm["wall"], "ms_per_frame": rm["wall"] / K * 1e3,
+               "p50_ms": float(np.percentile(rm["lat"], 50)), "p99_ms": float(np.percentile(rm["lat"], 99)),
+               "camera_pass_mean": float(rm["prim_ms"].mean()), "cone_pass_mean": float(rm["cone_ms"].mean()),
+               "cone_pass_first_launch_natural_order": rm["first_cone_ms"],
+               "capped_walks": int(rm["counters"][0]), "host_build_s": round(t_rb, 3),
+               "roofline": roofline_of(rm, "rock_field")}
+        if not args.no_cpu_baseline:
+            leg.update(oracle_check(rm, rflat, rf, cpu_seconds=4.0))
+        forms["rock_field_plain"] = leg
+        note(f"rock field: {leg['queries_per_s'] / 1e6:.1f} M queries/s")
+        if not args.no_sky:
+            forms["sky"] = sky_form(rscene, rflat, rf, rm["recv"], rm["h_recv"], check=not args.no_cpu_baseline)
+            note(f"sky: {forms['sky']['ms']:.1f} ms")
+        del rm, rscene, rflat
     if not args.no_served:
-        result.setdefault("forms", {})["served"] = served_form(device, n_requests=args.served_requests,
-                                                               seconds=args.served_seconds)
+        forms["served"] = served_form(device, n_requests=args.served_requests, seconds=args.served_seconds)
+        note("served form done")
     if not args.no_vcache:
-        result.setdefault("forms", {})["view_cache"] = view_cache_form(device)
+        forms["view_cache"] = view_cache_form(device)
+        note("view cache done")
     if not args.no_lightmap:
-        result.setdefault("forms", {})["lightmap"] = lightmap_form(device)
+        forms["lightmap"] = lightmap_form(device)
 
     print(json.dumps(result))
     return 0

@@ -37,9 +37,12 @@ PEAK_HBM_GBPS = 8000.0
 PROFILE_DIR = ROOT / "profiles"
 
 
-# The plain launch's tile: 16 x 4 shading points a wave (a sweep over 4, 8,
-# 16, 32 and 64 on the rock field puts it first).
+# The plain launch's tile: 16 x 4 shading points a wave on the rock field
+# (a sweep over 4, 8, 16, 32 and 64 puts it first there), 8 x 8 on the
+# courtyard (79.1 ms a frame against 82.7 ms at 16 x 4, 102.5 at 4 x 16 and
+# 108.2 at 32 x 2).
 PLAIN_TILE_W = 16
+COURTYARD_TILE_W = 8
 
 
 def algorithmic_bytes(work, queries: int, receivers: int, lights: int) -> dict:
@@ -856,7 +859,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--width", type=int, default=1280)
     ap.add_argument("--height", type=int, default=720)
-    ap.add_argument("--tile-w", type=int, default=PLAIN_TILE_W, help="pixels across a wave's tile (tile_w x 64/tile_w)")
+    ap.add_argument("--tile-w", type=int, default=0,
+                    help="pixels across a wave's tile (tile_w x 64/tile_w); 0: 8 on the courtyard, 16 on the rock field")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target length of the CPU baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-forms", action="store_true", help="skip the courtyard's 64-view batch")
@@ -921,7 +925,7 @@ def main() -> int:
     torch.cuda.synchronize()
     K = args.steps
     note(f"courtyard built ({sc.n_tris} triangles, {t_build:.1f} s on the host)")
-    m = plain_frames(scene, sc, K, args.warmup, args.tile_w, poses=poses)
+    m = plain_frames(scene, sc, K, args.warmup, args.tile_w or COURTYARD_TILE_W, poses=poses)
     note(f"courtyard frames: {m['wall'] / K * 1e3:.2f} ms a frame, cone pass {m['cone_ms'].mean():.2f} ms")
     counters = m["counters"]
     flags = m["flags"]
@@ -953,6 +957,7 @@ def main() -> int:
             "scene_hbm_bytes": scene.hbm_bytes,
             "host_build_s": round(t_build, 3),
         },
+        "tile": f"{args.tile_w or COURTYARD_TILE_W} x {64 // (args.tile_w or COURTYARD_TILE_W)} shading points a wave",
         "latency_ms": {"p50": float(np.percentile(m["lat"], 50)), "p99": float(np.percentile(m["lat"], 99)),
                        "refit_mean": float(m["refit_ms"].mean()), "camera_pass_mean": float(m["prim_ms"].mean()),
                        "cone_pass_mean": float(m["cone_ms"].mean()),
@@ -988,7 +993,7 @@ def main() -> int:
         rflat = Flat.build_arrays(rt9, rc)
         t_rb = time.perf_counter() - t0
         rscene = Scene(device, rflat)
-        rm = plain_frames(rscene, rf, K, args.warmup, args.tile_w)
+        rm = plain_frames(rscene, rf, K, args.warmup, args.tile_w or PLAIN_TILE_W)
         leg = {"workload": "rock field 1.03M triangles, 1280x720 x 3 lights (sun disk + two rects), plain per-view "
                            "launch (camera pass + cones)",
                "queries_per_s": K * rm["Q"] / rm["wall"], "ms_per_frame": rm["wall"] / K * 1e3,

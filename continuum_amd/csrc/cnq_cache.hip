@@ -81,7 +81,7 @@ __global__ __launch_bounds__(64, WALK_WAVES) void bake_static_kernel(
   __shared__ float s_acc[ACC_CELLS * 64];
   const uint32_t lane = lane_id();
   float poly[POLY_SLOTS];
-  Lane L{s_acc + lane, poly, ACC_CELLS, 0u};
+  Lane L{s_acc + lane, poly, POLY_STRIDE, ACC_CELLS, 0u};
   const Ctx x{blas, static_nodes, tri_leaf, tris, ranges, static_items, nullptr, blas_cov, xforms,
               p.n_objects, 2u, p.profile_stage, NONE};
   const uint32_t slot = blockIdx.x;
@@ -113,7 +113,7 @@ __global__ __launch_bounds__(64, WALK_WAVES) void rebake_kernel(
   __shared__ float s_acc[ACC_CELLS * 64];
   const uint32_t lane = lane_id();
   float poly[POLY_SLOTS];
-  Lane L{s_acc + lane, poly, ACC_CELLS, 0u};
+  Lane L{s_acc + lane, poly, POLY_STRIDE, ACC_CELLS, 0u};
   const Ctx x{blas, nullptr, tri_leaf, tris, ranges, nullptr, nullptr, blas_cov, xforms,
               p.n_objects, 2u, p.profile_stage, NONE};
   const uint32_t count = *n_dirty;

@@ -27,8 +27,14 @@ __device__ __forceinline__ uint32_t find_view(const ViewDesc* __restrict__ views
 // query of (view, light l, pixel y * w + x) is stored at
 // view.out_off + l * w * h + y * w + x. Direct mode: one item per block;
 // persistent mode: a fixed grid takes items from `work_counter` in order.
-template <bool COUNT, bool PERSISTENT>
-__global__ __launch_bounds__(64, WALK_WAVES) void cone_items_kernel(
+// FAST: the launch walks both hierarchies (hierarchy 2) with no profiling
+// stage, so those parameters are constants here and the other paths are not
+// compiled into this variant (a smaller kernel, kinder to the instruction
+// cache); other launches take the general variant. ONE: one wave per SIMD,
+// the clipper's workspace in LDS (lane-interleaved) instead of private
+// memory, and up to 512 registers a lane.
+template <bool COUNT, bool PERSISTENT, bool FAST, bool ONE = false>
+__global__ __launch_bounds__(64, ONE ? 1 : WALK_WAVES) void cone_items_kernel(
     const NodeC* __restrict__ blas, const NodeC* __restrict__ nodes, const float* __restrict__ tri_leaf,
     const float* __restrict__ tris, const uint32_t* __restrict__ ranges, const uint32_t* __restrict__ items,
     const float* __restrict__ blas_cov, const float* __restrict__ xforms, const float* __restrict__ receivers,
@@ -36,11 +42,17 @@ __global__ __launch_bounds__(64, WALK_WAVES) void cone_items_kernel(
     uint64_t* __restrict__ out_full, uint64_t* __restrict__ out_touched, float* __restrict__ out_cov,
     uint32_t* __restrict__ counters, uint64_t* __restrict__ work_out, uint32_t* __restrict__ work_counter,
     const uint32_t* __restrict__ order, uint32_t* __restrict__ item_cost, uint8_t* __restrict__ query_flags,
-    uint64_t n_flags, QueryParams p, ViewDesc v0) {
+    uint64_t n_flags, QueryParams pa, ViewDesc v0) {
   __shared__ float s_acc[ACC_CELLS * 64];
+  __shared__ float s_poly[ONE ? POLY_SLOTS * 64 : 1];
   const uint32_t lane = lane_id();
-  float poly[POLY_SLOTS];
-  Lane L{s_acc + lane, poly, ACC_CELLS, 0u};
+  float poly[ONE ? 1 : POLY_SLOTS];
+  Lane L{s_acc + lane, ONE ? s_poly + lane : poly, ONE ? POLY_STRIDE_LDS : POLY_STRIDE, ACC_CELLS, 0u};
+  QueryParams p = pa;
+  if (FAST) {
+    p.hierarchy = 2;
+    p.profile_stage = 0;
+  }
   const Ctx x{blas, nodes, tri_leaf, tris, ranges, items, nullptr, blas_cov, xforms,
               p.n_objects, p.hierarchy, p.profile_stage, p.object_cov};
   const uint32_t tw_log2 = p.tile_w_log2;
@@ -97,39 +109,39 @@ __global__ __launch_bounds__(64, WALK_WAVES) void cone_items_kernel(
       a.out_visible, a.out_full, a.out_touched, a.out_cov, a.counters, a.work, a.work_counter, a.order, a.item_cost, \
       a.query_flags, a.n_flags, a.p, a.v0
 
-// Occupancy: the walk is compiled for two waves per SIMD (eight a CU: the
-// accumulators' 17 KB of LDS a wave allow that). With `waves` 1 a launch
-// asks for unused LDS on top, 40 KB a wave in all, so a CU holds four of its
-// waves: half the waves in flight, half the lanes' private memory resident
-// in the caches.
-constexpr uint32_t LDS_PER_CU = 160u * 1024u;
-
-uint32_t occupancy_pad(uint32_t waves) {
-  if (waves != 1) return 0;
-  const uint32_t per_wave = LDS_PER_CU / 4u;  // four waves a CU
-  return per_wave - (uint32_t)(ACC_CELLS * 64u * sizeof(float));
-}
-
-hipError_t launch_cone_items(const ItemArgs& a, bool count_work, uint32_t persistent_blocks, hipStream_t stream,
-                             uint32_t waves) {
-  if (a.p.n_items <= a.p.item_begin) return hipSuccess;
+template <bool FAST, bool ONE>
+void launch_items_variant(const ItemArgs& a, bool count_work, uint32_t persistent_blocks, hipStream_t stream) {
   const SceneArgs& s = a.s;
   const dim3 block(64);
-  const uint32_t pad = occupancy_pad(waves);
   const uint32_t n = a.p.n_items - a.p.item_begin;
   if (persistent_blocks) {
     const dim3 grid(persistent_blocks < n ? persistent_blocks : n);
     if (count_work)
-      hipLaunchKernelGGL((cone_items_kernel<true, true>), grid, block, pad, stream, CNQ_ITEMS_ARGS);
+      hipLaunchKernelGGL((cone_items_kernel<true, true, FAST>), grid, block, 0, stream, CNQ_ITEMS_ARGS);
     else
-      hipLaunchKernelGGL((cone_items_kernel<false, true>), grid, block, pad, stream, CNQ_ITEMS_ARGS);
+      hipLaunchKernelGGL((cone_items_kernel<false, true, FAST, ONE>), grid, block, 0, stream, CNQ_ITEMS_ARGS);
   } else {
     const dim3 grid(n);
     if (count_work)
-      hipLaunchKernelGGL((cone_items_kernel<true, false>), grid, block, pad, stream, CNQ_ITEMS_ARGS);
+      hipLaunchKernelGGL((cone_items_kernel<true, false, FAST>), grid, block, 0, stream, CNQ_ITEMS_ARGS);
     else
-      hipLaunchKernelGGL((cone_items_kernel<false, false>), grid, block, pad, stream, CNQ_ITEMS_ARGS);
+      hipLaunchKernelGGL((cone_items_kernel<false, false, FAST, ONE>), grid, block, 0, stream, CNQ_ITEMS_ARGS);
   }
+}
+
+// Occupancy: two waves per SIMD (the clipper's workspace in private memory),
+// or with `waves` 1 the one-wave variant (workspace in LDS), for launches
+// without the work counters that walk both hierarchies unprofiled.
+hipError_t launch_cone_items(const ItemArgs& a, bool count_work, uint32_t persistent_blocks, hipStream_t stream,
+                             uint32_t waves) {
+  if (a.p.n_items <= a.p.item_begin) return hipSuccess;
+  const bool fast = a.p.hierarchy == 2 && a.p.profile_stage == 0;
+  if (fast && waves == 1 && !count_work)
+    launch_items_variant<true, true>(a, count_work, persistent_blocks, stream);
+  else if (fast)
+    launch_items_variant<true, false>(a, count_work, persistent_blocks, stream);
+  else
+    launch_items_variant<false, false>(a, count_work, persistent_blocks, stream);
   return hipGetLastError();
 }
 

@@ -257,10 +257,13 @@ __device__ __forceinline__ bool may_touch(const Cone& c, f3 v0, f3 v1, f3 v2) {
 constexpr uint32_t ACC_CELLS = 68;
 constexpr uint32_t POLY_SLOTS = 70;
 // The clipper's workspace lives in the lane's private memory (scratch, cached
-// in L1/L2): only triangles that cross the window, the near or the far plane
-// use it, and keeping it out of LDS leaves 17 KB of LDS per wave, so two
-// waves fit on each SIMD.
+// in L1/L2; consecutive slots, stride 1): only triangles that cross the
+// window, the near or the far plane use it, and keeping it out of LDS leaves
+// 17 KB of LDS per wave, so two waves fit on each SIMD. The cone kernel's
+// one-wave variant keeps it in LDS instead, lane-interleaved (stride 64):
+// 35 KB a wave, one wave per SIMD, no scratch round trips in the clips.
 constexpr uint32_t POLY_STRIDE = 1;
+constexpr uint32_t POLY_STRIDE_LDS = 64;
 // Waves per SIMD the walk kernels are compiled for (their registers are
 // limited to 256 a lane; the LDS, 17 KB a wave, allows it).
 constexpr int WALK_WAVES = 2;
@@ -268,30 +271,31 @@ constexpr uint32_t P3A = 0, P3B = 30, Q2A = 30, Q2B = 50;
 constexpr uint32_t MAX_POLY = 10;  // WGSL :11
 
 struct Lane {
-  float* acc;   // this lane's accumulation cell 0
-  float* poly;  // this lane's polygon slot 0
+  float* acc;       // this lane's accumulation cell 0
+  float* poly;      // this lane's polygon slot 0
+  uint32_t pstride;  // polygon slot i at poly[i * pstride]
   uint32_t span_lo, span_hi;
   __device__ __forceinline__ void add(uint32_t cell, float v) { acc[cell * 64u] += v; }
   __device__ __forceinline__ float get(uint32_t cell) const { return acc[cell * 64u]; }
   __device__ __forceinline__ void set(uint32_t cell, float v) { acc[cell * 64u] = v; }
   __device__ __forceinline__ f3 p3(uint32_t base, uint32_t i) const {
-    const float* p = poly + (base + 3u * i) * POLY_STRIDE;
-    return f3{p[0], p[POLY_STRIDE], p[2 * POLY_STRIDE]};
+    const float* p = poly + (base + 3u * i) * pstride;
+    return f3{p[0], p[pstride], p[2 * pstride]};
   }
   __device__ __forceinline__ void put3(uint32_t base, uint32_t i, f3 v) {
-    float* p = poly + (base + 3u * i) * POLY_STRIDE;
+    float* p = poly + (base + 3u * i) * pstride;
     p[0] = v.x;
-    p[POLY_STRIDE] = v.y;
-    p[2 * POLY_STRIDE] = v.z;
+    p[pstride] = v.y;
+    p[2 * pstride] = v.z;
   }
   __device__ __forceinline__ f2 p2(uint32_t base, uint32_t i) const {
-    const float* p = poly + (base + 2u * i) * POLY_STRIDE;
-    return f2{p[0], p[POLY_STRIDE]};
+    const float* p = poly + (base + 2u * i) * pstride;
+    return f2{p[0], p[pstride]};
   }
   __device__ __forceinline__ void put2(uint32_t base, uint32_t i, f2 v) {
-    float* p = poly + (base + 2u * i) * POLY_STRIDE;
+    float* p = poly + (base + 2u * i) * pstride;
     p[0] = v.x;
-    p[POLY_STRIDE] = v.y;
+    p[pstride] = v.y;
   }
 };
 

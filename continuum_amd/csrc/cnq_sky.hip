@@ -25,7 +25,7 @@ __global__ __launch_bounds__(64, WALK_WAVES) void sky_items_kernel(
   __shared__ float s_acc[ACC_CELLS * 64];
   const uint32_t lane = lane_id();
   float poly[POLY_SLOTS];
-  Lane L{s_acc + lane, poly, ACC_CELLS, 0u};
+  Lane L{s_acc + lane, poly, POLY_STRIDE, ACC_CELLS, 0u};
   const Ctx x{blas, nodes, tri_leaf, tris, ranges, items, nullptr, blas_cov, xforms,
               p.n_objects, p.hierarchy, p.profile_stage, p.object_cov};
   const uint32_t tiles = (n + 63u) / 64u;

@@ -501,7 +501,7 @@ __global__ __launch_bounds__(64, WALK_WAVES) void vc_bake_kernel(VcArgs a, const
   __shared__ float s_acc[ACC_CELLS * 64];
   const uint32_t lane = lane_id();
   float poly[POLY_SLOTS];
-  Lane L{s_acc + lane, poly, ACC_CELLS, 0u};
+  Lane L{s_acc + lane, poly, POLY_STRIDE, ACC_CELLS, 0u};
   const Ctx x{blas, nodes, tri_leaf, a.tris, a.ranges, items, blas_items, blas_cov, a.xforms,
               p.n_objects, p.hierarchy, p.profile_stage, p.object_cov};
   const uint32_t appended = a.stats[5] < a.append_jobs ? a.stats[5] : a.append_jobs;
@@ -887,7 +887,7 @@ __global__ __launch_bounds__(256) void vc_shade_kernel(VcArgs a, ShadeArgs s) {
                 // horizon clip runs in the lane's registers
                 float poly[POLY_SLOTS];
                 float acc_dummy[1];
-                Lane L{acc_dummy, poly, 0u, 0u};
+                Lane L{acc_dummy, poly, POLY_STRIDE, 0u, 0u};
                 open = rect_open(L, point, normal, lt);
               }
             }

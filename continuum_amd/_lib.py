@@ -141,6 +141,9 @@ SIGNATURES = {
     "cnq_sky_irradiance": (C.c_int, [_P, _P, C.c_uint64, C.POINTER(Options), C.POINTER(SkyOptions), _P, _P, _P]),
     "cnq_sky_irradiance_device": (
         C.c_int, [_P, _P, C.c_uint64, C.POINTER(Options), C.POINTER(SkyOptions), _P, _P, _P, _P]),
+    "cnq_sky_view_irradiance_device": (
+        C.c_int, [_P, _P, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(Options), C.POINTER(SkyOptions), _P, _P, _P,
+                  _P]),
     "cnq_sky_irradiance_device_noalloc": (
         C.c_int, [_P, _P, C.c_uint64, C.POINTER(Options), C.POINTER(SkyOptions), _P, _P, _P]),
     "cnq_view_irradiance_device": (

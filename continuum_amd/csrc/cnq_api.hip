@@ -735,16 +735,28 @@ void cnq_sky_options_default(cnq_sky_options* sky) {
   sky->prefilter_scale = 1.0f;
 }
 
-int cnq_sky_irradiance_device(cnq_scene* s, const float* receivers, uint64_t n, const cnq_options* o,
-                              const cnq_sky_options* sky, float* out, float* out_faces, uint64_t* out_full,
-                              void* stream) {
+}  // extern "C"
+
+namespace {
+
+// The sky launch over n receivers: a list (w 0) or a w x h view in tiles of
+// tile_w x 64 / tile_w pixels.
+int sky_device(cnq_scene* s, const float* receivers, uint64_t n, uint32_t w, uint32_t h, uint32_t tile_w,
+               const cnq_options* o, const cnq_sky_options* sky, float* out, float* out_faces, uint64_t* out_full,
+               void* stream) {
   if (!s) return fail(CNQ_INVALID_ARG, "scene is null");
   int rc = check_options(o);
   if (rc) return rc;
   if ((rc = check_object_cov(s, o))) return rc;
   if (n == 0) return CNQ_OK;
   if (!receivers || !out) return fail(CNQ_INVALID_ARG, "null buffer");
-  if (n > 0xffffffffull || 6 * ((n + 63) / 64) > 0x7fffffffull) return fail(CNQ_CAPACITY, "too many receivers for one call");
+  cnq::SkyArgs a;
+  a.n = (uint32_t)n;
+  a.w = w;
+  a.h = w ? h : 1u;
+  a.tile_w_log2 = w ? log2u(tile_w) : 6u;
+  if (n > 0xffffffffull || cnq::sky_items(a) > 0x7fffffffull)
+    return fail(CNQ_CAPACITY, "too many receivers for one call");
   cnq_sky_options so;
   cnq_sky_options_default(&so);
   if (sky) so = *sky;
@@ -759,20 +771,18 @@ int cnq_sky_irradiance_device(cnq_scene* s, const float* receivers, uint64_t n, 
     }
     out_faces = s->sky_faces.p;
   }
-  cnq::SkyArgs a;
   a.s = s->args();
   a.receivers = receivers;
   a.out = out;
   a.out_faces = out_faces;
   a.out_full = out_full;
   a.work = s->work.p;
-  a.n = (uint32_t)n;
   a.p = query_params(s, o, 64);
   a.sky = cnq::SkyParams{so.max_combine, so.horizon_cull, so.prefilter_scale};
   a.order = nullptr;
   a.item_cost = nullptr;
-  const uint64_t items = 6 * ((n + 63) / 64);
-  const uint64_t key[4] = {n, (uint64_t)(uintptr_t)receivers, items, 0};
+  const uint64_t items = cnq::sky_items(a);
+  const uint64_t key[4] = {n, (uint64_t)(uintptr_t)receivers, items, ((uint64_t)w << 32) | (w ? tile_w : 0u)};
   const uint32_t* order = nullptr;
   uint32_t* cost = nullptr;
   HIP_TRY(order_items(s->sky_order, order, cost, items, key, (hipStream_t)stream), "sky order buffers");
@@ -781,6 +791,25 @@ int cnq_sky_irradiance_device(cnq_scene* s, const float* receivers, uint64_t n, 
   HIP_TRY(cnq::launch_sky(a, s->count_work, (hipStream_t)stream), "sky launch");
   HIP_TRY(reorder_items(s->sky_order, a.item_cost, items, (hipStream_t)stream), "sky order");
   return CNQ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cnq_sky_irradiance_device(cnq_scene* s, const float* receivers, uint64_t n, const cnq_options* o,
+                              const cnq_sky_options* sky, float* out, float* out_faces, uint64_t* out_full,
+                              void* stream) {
+  return sky_device(s, receivers, n, 0, 1, 64, o, sky, out, out_faces, out_full, stream);
+}
+
+int cnq_sky_view_irradiance_device(cnq_scene* s, const float* receivers, uint32_t width, uint32_t height,
+                                   uint32_t tile_w, const cnq_options* o, const cnq_sky_options* sky, float* out,
+                                   float* out_faces, uint64_t* out_full, void* stream) {
+  if (tile_w == 0 || tile_w > 64 || (tile_w & (tile_w - 1)))
+    return fail(CNQ_INVALID_ARG, "tile_w must be 1..64, a power of two");
+  return sky_device(s, receivers, (uint64_t)width * height, width, height, tile_w, o, sky, out, out_faces, out_full,
+                    stream);
 }
 
 int cnq_sky_irradiance_device_noalloc(cnq_scene* s, const float* receivers, uint64_t n, const cnq_options* o,

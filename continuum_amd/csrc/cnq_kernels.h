@@ -179,9 +179,11 @@ struct SkyParams {
   float prefilter_scale;
 };
 
-// The sky over a list of receivers: work items are (face, tile of 64
-// receivers), face-major; face f of receiver i goes to out_faces[f * n + i]
-// (and its full bins to out_full, when given), then out[i] sums the six.
+// The sky over a w x h grid of receivers (a view, or a list as n x 1):
+// work items are (face, tile), face-major, a tile being tile_w x 64 / tile_w
+// receivers (64 x 1 for a list); face f of receiver i = y * w + x goes to
+// out_faces[f * n + i] (n = w * h; its full bins to out_full, when given),
+// then out[i] sums the six.
 struct SkyArgs {
   SceneArgs s;
   const float* __restrict__ receivers;
@@ -194,6 +196,7 @@ struct SkyArgs {
   const uint32_t* __restrict__ order;
   uint32_t* __restrict__ item_cost;
   uint32_t n;
+  uint32_t w = 0, h = 1, tile_w_log2 = 6;  // the grid (w 0: a list of n) and its tiles
   QueryParams p;
   SkyParams sky;
 };
@@ -203,6 +206,8 @@ struct SkyArgs {
 hipError_t launch_cone_items(const ItemArgs& a, bool count_work, uint32_t persistent_blocks, hipStream_t stream,
                              uint32_t waves = 2);
 hipError_t launch_sky(const SkyArgs& a, bool count_work, hipStream_t stream);
+// The sky launch's work items (six faces times its tiles).
+uint64_t sky_items(const SkyArgs& a);
 // The next launch's order from this one's item times: longest first, by
 // powers of two, when the times resemble the last launch's (`last`: each
 // item's last power of two, 0xff for none), else natural order (one block;

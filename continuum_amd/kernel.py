@@ -304,6 +304,24 @@ class Scene:
                                               out.data_ptr(), faces.data_ptr() if faces is not None else 0,
                                               full.data_ptr() if full is not None else 0, stream.cuda_stream))
 
+    def sky_view_device(self, receivers, width, height, opts, out, faces=None, full=None, sky=None, tile_w=8,
+                        stream=None) -> None:
+        """The sky over a view's receivers (device tensors), each wave taking a
+        tile_w x 64/tile_w patch of pixels; the layouts of sky_device."""
+        import torch
+
+        if stream is None:
+            stream = torch.cuda.current_stream()
+        n = width * height
+        if receivers.numel() < 8 * n or out.numel() < n or (faces is not None and faces.numel() < 6 * n) or (
+                full is not None and full.numel() < 6 * n):
+            raise ValueError("tensor too small")
+        so = sky if sky is not None else _lib.default_sky_options()
+        check(lib().cnq_sky_view_irradiance_device(self.handle, receivers.data_ptr(), width, height, tile_w,
+                                                   C.byref(opts), C.byref(so), out.data_ptr(),
+                                                   faces.data_ptr() if faces is not None else 0,
+                                                   full.data_ptr() if full is not None else 0, stream.cuda_stream))
+
     def view_query_device(self, receivers, width, height, lights, opts, vis, full, touched, tile_w=16,
                           stream=None) -> None:
         """Cone queries over a view's receivers (device tensors), each wave

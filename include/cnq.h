@@ -210,6 +210,15 @@ void cnq_sky_options_default(cnq_sky_options* sky);
 int cnq_sky_irradiance_device(cnq_scene* scene, const float* receivers, uint64_t n, const cnq_options* opts,
                               const cnq_sky_options* sky, float* out, float* out_faces, uint64_t* out_full,
                               void* stream);
+/* The same over a width x height view's receivers (pixel y * width + x;
+ * n = width * height in the layouts above), each wave taking a tile_w x
+ * 64 / tile_w patch of pixels (tile_w a power of two up to 64) instead of
+ * 64 consecutive receivers: the cones of a patch start close together, so
+ * their walks share more nodes. Results equal the list launch's bit for bit
+ * (sky walks have no node cap and keep a fixed child order). */
+int cnq_sky_view_irradiance_device(cnq_scene* scene, const float* receivers, uint32_t width, uint32_t height,
+                                   uint32_t tile_w, const cnq_options* opts, const cnq_sky_options* sky, float* out,
+                                   float* out_faces, uint64_t* out_full, void* stream);
 /* The same on host buffers (copied in and out; returns when done). */
 int cnq_sky_irradiance(cnq_scene* scene, const float* receivers, uint64_t n, const cnq_options* opts,
                        const cnq_sky_options* sky, float* out, float* out_faces, uint64_t* out_full);

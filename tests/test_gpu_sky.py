@@ -94,3 +94,39 @@ def test_kernel_object_sky_irradiance(device):
     recv = [((2.0, 2.0, 1.5), (0.0, 0.0, 1.0)), ((1.0, 1.0, 0.01), (0.0, 0.0, 1.0))]
     out = k.sky_irradiance(recv, sc.meshes)
     assert out.shape == (2,) and np.all(out >= 0) and out[0] < 1e-5 and np.all(out < np.pi)
+
+
+def test_view_tiled_sky_equals_the_list_launch_and_the_oracle(device):
+    """The sky over a view's receivers in pixel tiles (cnq_sky_view_irradiance_device)
+    equals the list launch and the oracle bit for bit, for tiles of 8 x 8,
+    16 x 4 and 64 x 1, on a view whose size is not a multiple of the tile."""
+    import torch
+
+    sc = rock_field(width=100, height=37)
+    tris9, counts = meshes_to_arrays(sc.meshes)
+    flat = Flat.build_arrays(tris9, counts)
+    scene = Scene(device, flat)
+    p = _lib.default_options(hierarchy=2, prefilter=sc.prefilter)
+    dev = torch.device("cuda:0")
+    w, h = sc.width, sc.height
+    n = w * h
+    cam = torch.from_numpy(sc.camera).to(dev)
+    lights = torch.from_numpy(np.ascontiguousarray(sc.lights)).to(dev)
+    hits = torch.zeros((n, 4), dtype=torch.int32, device=dev)
+    recv = torch.zeros((n, 8), dtype=torch.float32, device=dev)
+    scene.primary_device(cam, w, h, lights, p, hits, recv)
+    torch.cuda.synchronize()
+    want = scene.sky(recv.cpu().numpy(), p)
+    r = O.sky(flat, recv.cpu().numpy(), O.opts_array(hierarchy=2, prefilter=sc.prefilter))
+    _same(want[0], r[0], "list launch sum")
+    for tw in (8, 16, 64):
+        out = torch.zeros(n, dtype=torch.float32, device=dev)
+        faces = torch.zeros(6 * n, dtype=torch.float32, device=dev)
+        full = torch.zeros(6 * n, dtype=torch.int64, device=dev)
+        for _ in range(2):  # the second launch ordered by the first one's times
+            scene.sky_view_device(recv, w, h, p, out, faces, full, tile_w=tw)
+        torch.cuda.synchronize()
+        _same(out.cpu().numpy(), want[0], f"sum, tile {tw}")
+        _same(faces.cpu().numpy().reshape(6, n), want[1], f"faces, tile {tw}")
+        _same(full.cpu().numpy().view(np.uint64).reshape(6, n), want[2], f"full, tile {tw}")
+    assert (want[0] > 0).mean() > 0.5

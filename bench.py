@@ -972,7 +972,8 @@ def main() -> int:
                        "cone_pass_mean": float(m["cone_ms"].mean()),
                        "cone_pass_first_launch_natural_order": m["first_cone_ms"]},
         "roofline": roofline_of(m, "courtyard"),
-        "diagnostics": {"capped_walks": int(counters[0]), "capped_share": int(counters[0]) / m["Q"],
+        "diagnostics": {"capped_walks_timed_frames": int(counters[0]),
+                        "capped_share": int(counters[0]) / (m["Q"] * m["steps"]),
                         "deeper_than_48": int(counters[1]), "truncated": int(counters[2]),
                         "flagged_capped_last_frame": int(((flags & 1) != 0).sum())},
     }

@@ -200,6 +200,11 @@ SIGNATURES = {
     "cnq_vcache_verify": (C.c_int, [_P, _P, _P]),
     "cnq_vcache_set_lights": (C.c_int, [_P, _P, C.c_uint32, C.c_uint32, _P, _P]),
     "cnq_vcache_plan": (C.c_int, [_P, C.c_uint32, _P]),
+    "cnq_vcache_set_moving_lights": (C.c_int, [_P, _P, C.c_uint32, C.c_float]),
+    "cnq_vcache_move_light": (C.c_int, [_P, C.c_uint32, _P]),
+    "cnq_vcache_ledger_stats": (C.c_int, [_P, _P]),
+    "cnq_vcache_ledger_state": (C.c_int, [_P, _P, _P]),
+    "cnq_vcache_verify_ledger": (C.c_int, [_P, C.c_uint32, _P, _P]),
 }
 
 _lib = None

@@ -97,6 +97,64 @@ __device__ __forceinline__ float atan2_c(float y, float x) {
 
 __device__ __forceinline__ float acos_c(float y) { return atan2_c(sqrtf((1.0f - y) * (1.0f + y)), y); }
 
+// asin(x) = atan2(x, sqrt((1 - x)(1 + x))) for |x| <= 1.
+__device__ __forceinline__ float asin_c(float x) { return atan2_c(x, sqrtf((1.0f - x) * (1.0f + x))); }
+
+// Cosine and tangent: S. L. Moshier, Cephes Math Library (sinf.c `cosf`,
+// tanf.c): the argument reduced by multiples of pi/4 (pi/4 in three parts),
+// then minimax polynomials on [-pi/4, pi/4].
+constexpr float kFourOverPi = 1.27323954473516f;
+constexpr float kDP1 = 0.78515625f;
+constexpr float kDP2 = 2.4187564849853515625e-4f;
+constexpr float kDP3 = 3.77489497744594108e-8f;
+
+__device__ __forceinline__ float cos_c(float xin) {
+  float x = xin < 0.0f ? -xin : xin;
+  int j = (int)(kFourOverPi * x);
+  float y = (float)j;
+  if (j & 1) {
+    j += 1;
+    y += 1.0f;
+  }
+  j &= 7;
+  int sign = 1;
+  if (j > 3) {
+    j -= 4;
+    sign = -sign;
+  }
+  if (j > 1) sign = -sign;
+  x = ((x - y * kDP1) - y * kDP2) - y * kDP3;
+  const float z = x * x;
+  if (j == 1 || j == 2)
+    y = ((-1.9515295891e-4f * z + 8.3321608736e-3f) * z - 1.6666654611e-1f) * z * x + x;
+  else
+    y = ((2.443315711809948e-5f * z - 1.388731625493765e-3f) * z + 4.166664568298827e-2f) * z * z - 0.5f * z + 1.0f;
+  return sign < 0 ? -y : y;
+}
+
+__device__ __forceinline__ float tan_c(float xin) {
+  const float x = xin < 0.0f ? -xin : xin;
+  int j = (int)(kFourOverPi * x);
+  float y = (float)j;
+  if (j & 1) {
+    j += 1;
+    y += 1.0f;
+  }
+  const float z = ((x - y * kDP1) - y * kDP2) - y * kDP3;
+  const float zz = z * z;
+  if (x > 1.0e-4f)
+    y = (((((9.38540185543e-3f * zz + 3.11992232697e-3f) * zz + 2.44301354525e-2f) * zz + 5.34112807005e-2f) * zz +
+          1.33387994085e-1f) *
+             zz +
+         3.33331568548e-1f) *
+            zz * z +
+        z;
+  else
+    y = z;
+  if (j & 2) y = -1.0f / y;
+  return xin < 0.0f ? -y : y;
+}
+
 // Exponential after Cephes expf: x = n ln2 + r (ln2 in two parts), a
 // degree-6 polynomial for e^r, then scaling by 2^n.
 __device__ __forceinline__ float exp_c(float x) {
@@ -117,11 +175,13 @@ __device__ __forceinline__ float exp_c(float x) {
   return ldexpf(p, n);
 }
 
-// A light cone (WGSL :125-146). Its far plane's normal is always its
-// forward axis and a light cone has no horizon, so neither is stored.
+// A mask cone (WGSL :125-146). Light, sky and guard cones take their
+// forward axis as the far plane's normal; a lattice cone (a moving rect
+// light's, in the view cache) takes the light's plane.
 struct Cone {
   f3 apex, right, up, forward;
   float w0, w1, w2, w3;  // window u0, v0, u1, v1
+  f3 far_normal;
   float far_offset;
   f3 baxis;  // bounding circular cone: unit axis, cos and sin of the half-angle
   float bcos, bsin;
@@ -216,7 +276,7 @@ __device__ __forceinline__ bool may_touch_box(const Cone& c, f3 lo, f3 hi) {
 #pragma unroll
   for (uint32_t k = 0; k < 8; k++) {
     f3 corner = box_corner(lo, hi, k);
-    if (c.far_offset - dot3(c.forward, corner) >= 0.0f) far_out = false;
+    if (c.far_offset - dot3(c.far_normal, corner) >= 0.0f) far_out = false;
     if (SKY && dot3(c.horizon, corner - c.apex) >= 0.0f) below = false;
     f3 p = to_local(c, corner);
     if (p.z >= NEAR) near_out = false;
@@ -232,9 +292,9 @@ __device__ __forceinline__ bool may_touch_box(const Cone& c, f3 lo, f3 hi) {
 // WGSL :213-232 (the horizon test only with SKY)
 template <bool SKY = false>
 __device__ __forceinline__ bool may_touch(const Cone& c, f3 v0, f3 v1, f3 v2) {
-  float f0 = c.far_offset - dot3(c.forward, v0);
-  float f1 = c.far_offset - dot3(c.forward, v1);
-  float f2 = c.far_offset - dot3(c.forward, v2);
+  float f0 = c.far_offset - dot3(c.far_normal, v0);
+  float f1 = c.far_offset - dot3(c.far_normal, v1);
+  float f2 = c.far_offset - dot3(c.far_normal, v2);
   if (f0 < 0.0f && f1 < 0.0f && f2 < 0.0f) return false;
   if (SKY && dot3(c.horizon, v0 - c.apex) < 0.0f && dot3(c.horizon, v1 - c.apex) < 0.0f &&
       dot3(c.horizon, v2 - c.apex) < 0.0f)
@@ -440,9 +500,9 @@ __device__ __forceinline__ bool rasterize_general(const Cone& c, Lane& L, f3 v0,
 // plane and inside the window projects to a triangle inside the grid, drawn
 // straight from registers; any other case builds its polygons in LDS.
 __device__ __forceinline__ bool rasterize(const Cone& c, Lane& L, f3 v0, f3 v1, f3 v2, float weight) {
-  const float f0 = c.far_offset - dot3(c.forward, v0);
-  const float f1 = c.far_offset - dot3(c.forward, v1);
-  const float f2_ = c.far_offset - dot3(c.forward, v2);
+  const float f0 = c.far_offset - dot3(c.far_normal, v0);
+  const float f1 = c.far_offset - dot3(c.far_normal, v1);
+  const float f2_ = c.far_offset - dot3(c.far_normal, v2);
   const bool far_clip = f0 < 0.0f || f1 < 0.0f || f2_ < 0.0f;
   const float su = (float)GRID / (c.w2 - c.w0);
   const float sv = (float)GRID / (c.w3 - c.w1);
@@ -481,7 +541,7 @@ __device__ __forceinline__ bool rasterize_general(const Cone& c, Lane& L, f3 v0,
   L.put3(P3A, 2, v2);
   uint32_t n = 3;
   if (far_clip) {
-    n = clip_plane3(L, P3A, n, c.forward, c.far_offset, P3B);
+    n = clip_plane3(L, P3A, n, c.far_normal, c.far_offset, P3B);
     for (uint32_t i = 0; i < n; i++) L.put3(P3A, i, L.p3(P3B, i));
   }
   bool all_near_ok = true, all_in_window = true;

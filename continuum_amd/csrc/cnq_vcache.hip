@@ -11,8 +11,11 @@
 // `world_box` :3867, `light_box` :3887, `hull_meets` :3903, `boxes_meet`
 // :3919, `may_be_shadowed` :3967, `mark_invalid` :3988, `append_page`
 // :3998, `append_jobs` :4018, `fail_entry` :4029, `vc_stale` :4233-4379,
-// `vc_stale_texels` :4385-4432; host side crates/continuum-gpu/src/vcache.rs
-// (planning :3940-4050, eviction :3425-3440, `set_moved` :2426-2492).
+// `vc_stale_texels` :4385-4432, and the moving lights' error ledger :4457-5164
+// (guard cones, `ledger_texel`, `ledger_reset`, `light_shift`, the rect
+// lights' lattice, `vc_ledger`); host side crates/continuum-gpu/src/vcache.rs
+// (planning :3940-4050, eviction :3425-3440, `set_moved` :2426-2492, moving
+// lights :1399-1430 and :2255-2380).
 //
 // Organisation on this GPU: bake jobs are consecutive (page, texel, light)
 // triples, so a wave's 64 lanes are 64 neighbouring texels of one page and
@@ -36,8 +39,15 @@ constexpr uint32_t VC_TEXELS = vc::PAGE_TEXELS;
 
 // mover_stats words (WGSL): 0 pages appended, 1 invalid marks, 2 visible
 // pages tested, 3 appends that did not fit, 4 candidates, 5 jobs appended,
-// 6 pages made stale for a re-bake.
-constexpr uint32_t VC_STATS = 8;
+// 6 pages made stale for a re-bake, 7 (here) the bake's job counter; the
+// moving lights' ledger: 8 pages re-baked, 9 texels re-baked, 10 pages set
+// aside, 11 pages checked, 12 pages re-baking mixed texels, 13 pages held
+// whole, 14 jobs asked, 21 pages held on their lattice, 22 lattices that
+// may have grown, 23 unbounded texels of held pages.
+constexpr uint32_t VC_STATS = 24;
+// Frames of light poses kept for the ledger (vcache.rs:161 POSE_RING).
+constexpr uint32_t VC_POSE_RING = 1024;
+constexpr uint32_t LEDGER_WORDS_HOST = 48;  // words of a ledger record (vcache.rs:162)
 // The request set's entries (vcache.rs:85 REQUEST_SET) and the most
 // requests one shading pass lists.
 constexpr uint32_t VC_REQUEST_SET = 1u << 18;
@@ -73,9 +83,30 @@ struct VcArgs {
   uint32_t bake_group;                 // most jobs a bake wave takes at a time (64)
   uint32_t bake_resident;              // waves the GPU holds at once
   float inv_base;
+  // Moving lights (WGSL :4457-5164): per light its index among the moving
+  // ones (NONE: static); ledger records (slot x moving x 48 words); each
+  // record's certified bound; the light poses of the last VC_POSE_RING
+  // frames (frame % ring x lights x 16 floats).
+  const uint32_t* __restrict__ moving_index;
+  uint32_t* ledger;
+  float* ledger_bound;
+  const float* __restrict__ poses;
+  uint32_t n_moving;
+  uint32_t ledger_limit;   // jobs the ledger may append; top bit: a moving light moved
+  uint32_t ledger_record;  // 0: bakes leave the records alone (a check's bake)
+  float epsilon;
 };
 
 namespace {
+
+// WGSL :3123-3129 `vc_hash` (vcache.rs:904-913)
+__device__ __forceinline__ uint32_t key_hash(uint32_t k0, uint32_t k1, uint32_t k2) {
+  uint32_t h = (k0 * 0x9E3779B1u) ^ (k1 * 0x85EBCA77u) ^ (k2 * 0xC2B2AE3Du);
+  h ^= h >> 15;
+  h *= 0x2C1B3C6Du;
+  h ^= h >> 12;
+  return h;
+}
 
 struct Box3 {
   f3 lo, hi;
@@ -341,6 +372,401 @@ __device__ __forceinline__ void fail_entry(const VcArgs& a, uint32_t entry, uint
   if (q < a.invalid_cap) a.invalid_list[q] = s;
 }
 
+// ---- moving lights through the error ledger (WGSL :4457-5164)
+
+constexpr uint32_t LEDGER_WORDS = 48;      // WGSL :4481, vcache.rs:162
+constexpr uint32_t MAX_OFFSET = 1023;      // WGSL :4483
+constexpr float GUARD = 3.0f;              // WGSL :4482
+constexpr float LATTICE_MARGIN = 0.03f;    // WGSL :4704
+constexpr float LATTICE_KAPPA_MAX = 8.0f;  // WGSL :4708
+
+__device__ __forceinline__ uint32_t ledger_base(const VcArgs& a, uint32_t slot, uint32_t j) {
+  return (slot * a.n_moving + j) * LEDGER_WORDS;
+}
+
+__device__ __forceinline__ const float* pose_at(const VcArgs& a, uint32_t frame, uint32_t l) {
+  return a.poses + ((size_t)(frame % VC_POSE_RING) * a.n_lights + l) * 16;
+}
+
+__device__ __forceinline__ f3 rect_corner_of(const float* r, uint32_t k) {
+  return ld3(r) + ld3(r + 4) * (float)(k & 1u) + ld3(r + 8) * (float)((k >> 1u) & 1u);
+}
+
+__device__ __forceinline__ f3 vmin3(f3 a, f3 b) { return f3{mn(a.x, b.x), mn(a.y, b.y), mn(a.z, b.z)}; }
+
+// WGSL :4666-4687: the largest angle a point of light pose `a` moves to
+// reach pose `b`, seen from anywhere in box `page` (4: unbounded).
+__device__ __forceinline__ float light_shift(const float* la_, const float* lb_, const Box3& page) {
+  float moved = 0.0f;
+  if (la_[3] > 1.5f) {
+    moved = len3(ld3(lb_) - ld3(la_)) + fabsf(lb_[4] - la_[4]);
+  } else {
+#pragma unroll
+    for (uint32_t k = 0; k < 4; k++) moved = mx(moved, len3(rect_corner_of(lb_, k) - rect_corner_of(la_, k)));
+  }
+  if (moved <= 0.0f) return 0.0f;
+  const Box3 la = light_box(la_), lb = light_box(lb_);
+  const Box3 span{vmin3(la.lo, lb.lo), vmax3(la.hi, lb.hi)};
+  const f3 gap = vmax3(f3{0.0f, 0.0f, 0.0f}, vmax3(page.lo - span.hi, span.lo - page.hi));
+  const float d = len3(gap);
+  if (moved >= 2.0f * d) return 4.0f;
+  return 2.0f * asin_c(moved / (2.0f * d)) * 1.0001f;
+}
+
+struct LatticeReach {
+  float near, far, cos_min, half_diagonal;
+};
+
+// WGSL :4907-4942
+__device__ __forceinline__ LatticeReach lattice_reach(const float* a, const float* b, const Box3& page) {
+  LatticeReach out;
+  const Box3 la = light_box(a), lb = light_box(b);
+  const Box3 span{vmin3(la.lo, lb.lo), vmax3(la.hi, lb.hi)};
+  const f3 gap = vmax3(f3{0.0f, 0.0f, 0.0f}, vmax3(page.lo - span.hi, span.lo - page.hi));
+  out.near = len3(gap);
+  const f3 au = ld3(a + 4), av = ld3(a + 8), bu = ld3(b + 4), bv = ld3(b + 8);
+  out.half_diagonal = 0.5f * mx(len3(au + av), len3(bu + bv));
+  out.half_diagonal = mx(out.half_diagonal, 0.5f * mx(len3(au - av), len3(bu - bv)));
+  const f3 pc = (page.lo + page.hi) * 0.5f;
+  const float ph = 0.5f * len3(page.hi - page.lo);
+  const f3 ca = light_center_of(a) - pc, cb = light_center_of(b) - pc;
+  out.far = mx(len3(ca), len3(cb)) + ph + out.half_diagonal;
+  const f3 na = norm3(cross3(au, av)), nb = norm3(cross3(bu, bv));
+  const float cos_c = mn(fabsf(dot3(na, norm3(ca))), fabsf(dot3(nb, norm3(cb))));
+  out.cos_min = out.near > 0.0f ? cos_c - asin_c(mn((ph + out.half_diagonal) / out.near, 1.0f)) : -1.0f;
+  return out;
+}
+
+// WGSL :4949-4971
+__device__ __forceinline__ float lattice_bound(const float* a, const float* b, const Box3& page, float kappa) {
+  const float delta = light_shift(a, b, page);
+  if (delta <= 0.0f) return 0.0f;
+  if (delta >= 1.0f || kappa >= 1.0e30f) return 1.0f;
+  const LatticeReach r = lattice_reach(a, b, page);
+  if (r.cos_min <= 0.0f || r.near <= 2.0f * r.half_diagonal) return 1.0f;
+  const f3 au = ld3(a + 4), av = ld3(a + 8), bu = ld3(b + 4), bv = ld3(b + 8);
+  const float area = mn(len3(cross3(au, av)), len3(cross3(bu, bv)));
+  const float omega = area * r.cos_min / (r.far * r.far);
+  const float lu = mx(len3(au), len3(bu)), lv = mx(len3(av), len3(bv));
+  const float perimeter =
+      4.0f * (asin_c(mn(lu / (2.0f * r.near), 1.0f)) + asin_c(mn(lv / (2.0f * r.near), 1.0f)));
+  const float rho = mn(GUARD * asin_c(mn(r.half_diagonal / r.near, 1.0f)) * 1.0001f, 1.4f);
+  const float t = tan_c(rho);
+  const float sec2 = 1.0f + t * t;
+  const float moved = sec2 * delta;
+  const float sw = sec2 * perimeter * moved + 3.14159265f * moved * moved;
+  return mn(1.5f * kappa * sw / omega * 1.0001f, 1.0f);
+}
+
+// WGSL :4981-5000
+__device__ __forceinline__ bool lattice_grew(const float* a, const float* b, const Box3& page) {
+  const LatticeReach r = lattice_reach(a, b, page);
+  if (r.cos_min <= 0.0f || r.near <= 2.0f * r.half_diagonal) return true;
+  const f3 ca = light_center_of(a), cb = light_center_of(b);
+  const float t = len3(cb - ca);
+  float rest = 0.0f;
+#pragma unroll
+  for (uint32_t k = 0; k < 4; k++) rest = mx(rest, len3((rect_corner_of(b, k) - cb) - (rect_corner_of(a, k) - ca)));
+  const float diameter = 2.0f * asin_c(mn(r.half_diagonal / r.near, 1.0f));
+  const float rho = mn(GUARD * 0.5f * diameter * 1.0001f, 1.4f);
+  const float tr = tan_c(rho);
+  const float sec2 = 1.0f + tr * tr;
+  const float delta = light_shift(a, b, page);
+  const float spread = sec2 * (3.0f * (t / r.near) * (2.0f * r.half_diagonal / r.near) + rest / r.near) +
+                       3.0f * tr * sec2 * sec2 * diameter * delta;
+  const f3 au = ld3(a + 4), av = ld3(a + 8), bu = ld3(b + 4), bv = ld3(b + 8);
+  const float area = mn(len3(cross3(au, av)), len3(cross3(bu, bv)));
+  const float width = area * r.cos_min / (r.far * r.far) / (sec2 * diameter);
+  return 2.0f * spread * 1.0001f > LATTICE_MARGIN * width;
+}
+
+// WGSL :4643-4652
+__device__ __forceinline__ void ledger_reset(const VcArgs& a, uint32_t slot, uint32_t j) {
+  uint32_t* r = a.ledger + ledger_base(a, slot, j);
+  r[0] = a.frame;
+  r[1] = 0x7f7fffffu;
+  r[2] = 0u;
+  r[3] = a.frame;
+  r[4] = 0u;
+  for (uint32_t w = 5; w < LEDGER_WORDS; w++) r[w] = 0u;
+  a.ledger_bound[(size_t)slot * a.n_moving + j] = 0.0f;
+}
+
+__device__ __forceinline__ float hashed_share(uint32_t slot, uint32_t frame, uint32_t salt) {
+  return 0.5f + 0.5f * (float)(key_hash(slot, frame, salt) & 0xffffu) / 65535.0f;
+}
+
+// The guard cone (WGSL :4499-4529) around the direction to light `r`'s
+// centre from `apex`, and the light's angular radius there.
+struct GuardCone {
+  Cone c;
+  float alpha;
+};
+
+__device__ __forceinline__ GuardCone guard_cone(f3 apex, const LightRect& r) {
+  GuardCone out;
+  out.c = Cone{};
+  const f3 to = light_center(r) - apex;
+  const float d = mx(len3(to), 1e-6f);
+  const f3 forward = to / d;
+  float alpha = 0.0f;
+  if (is_sphere(r)) {
+    alpha = asin_c(mn(r.edge_u.x / d, 1.0f));
+  } else {
+    const float* rf = reinterpret_cast<const float*>(&r);
+#pragma unroll
+    for (uint32_t k = 0; k < 4; k++)
+      alpha = mx(alpha, acos_c(clampf(dot3(forward, norm3(rect_corner_of(rf, k) - apex)), -1.0f, 1.0f)));
+  }
+  const float wide = mn(GUARD * alpha, 1.4f);
+  const float t = tan_c(wide);
+  const f3 helper = fabsf(forward.x) > 0.9f ? f3{0.0f, 1.0f, 0.0f} : f3{1.0f, 0.0f, 0.0f};
+  const f3 right = norm3(cross3(helper, forward));
+  out.c.apex = apex;
+  out.c.right = right;
+  out.c.up = cross3(forward, right);
+  out.c.forward = forward;
+  out.c.w0 = -t;
+  out.c.w1 = -t;
+  out.c.w2 = t;
+  out.c.w3 = t;
+  out.c.far_normal = forward;
+  out.c.far_offset = dot3(forward, apex) + SKY_FAR;
+  bound_cone(out.c, forward);
+  out.alpha = alpha;
+  return out;
+}
+
+// WGSL :4532-4539
+__device__ __forceinline__ float below_horizon(f3 apex, f3 n, const LightRect& r) {
+  const f3 to = light_center(r) - apex;
+  const float d = mx(len3(to), 1e-6f);
+  float below = -dot3(n, to);
+  if (is_sphere(r)) below -= r.edge_u.x;
+  return asin_c(clampf(below / d, -1.0f, 1.0f));
+}
+
+struct LatticeFrame {
+  f3 forward, right, up;
+  float beta_x, beta_y, alpha;
+  bool ok;
+};
+
+// WGSL :4727-4760
+__device__ __forceinline__ LatticeFrame lattice_frame(f3 apex, const float* then) {
+  LatticeFrame f;
+  f.ok = false;
+  f.forward = f.right = f.up = f3{0.0f, 0.0f, 0.0f};
+  f.beta_x = f.beta_y = f.alpha = 0.0f;
+  const f3 to = light_center_of(then) - apex;
+  const float d = len3(to);
+  if (d <= 1e-6f) return f;
+  const f3 forward = to / d;
+  const f3 eu = ld3(then + 4), ev = ld3(then + 8);
+  f3 right = eu - forward * dot3(forward, eu);
+  if (len3(right) <= 1e-6f * len3(eu)) right = ev - forward * dot3(forward, ev);
+  if (len3(right) <= 0.0f) return f;
+  right = norm3(right);
+  const f3 up = cross3(forward, right);
+  float lox = 3.4e38f, loy = 3.4e38f, hix = -3.4e38f, hiy = -3.4e38f, alpha = 0.0f;
+  for (uint32_t k = 0; k < 4; k++) {
+    const f3 p = rect_corner_of(then, k) - apex;
+    const float z = dot3(forward, p);
+    if (z <= 1e-6f * d) return f;
+    const float ux = dot3(right, p) / z, uy = dot3(up, p) / z;
+    lox = mn(lox, ux), loy = mn(loy, uy), hix = mx(hix, ux), hiy = mx(hiy, uy);
+    alpha = mx(alpha, atan2_c(len3(cross3(forward, p)), z));
+  }
+  f.forward = forward;
+  f.right = right;
+  f.up = up;
+  const float share = (1.0f + LATTICE_MARGIN) / 7.0f;
+  f.beta_x = mx(hix - lox, 1e-12f) * share;
+  f.beta_y = mx(hiy - loy, 1e-12f) * share;
+  f.alpha = alpha * 1.0001f;
+  f.ok = true;
+  return f;
+}
+
+// WGSL :4766-4773, (1 + t^2)^1.5 as q sqrt(q)
+__device__ __forceinline__ float lattice_kappa(const LatticeFrame& f, f3 n) {
+  if (!f.ok) return 3.0e38f;
+  const float reach = mn(GUARD * f.alpha, 1.4f);
+  const float theta = atan2_c(len3(cross3(n, f.forward)), dot3(n, f.forward));
+  if (theta + reach >= 1.5706f) return 3.0e38f;
+  const float t = tan_c(reach);
+  const float q = 1.0f + t * t;
+  return cos_c(mx(theta - reach, 0.0f)) / cos_c(theta + reach) * (q * sqrtf(q)) * 1.0001f;
+}
+
+// The lattice cone (WGSL :4786-4826) of rect light `now` on the lattice of
+// pose `then`, and the footprint's corners in bins.
+struct LatticeCone {
+  Cone c;
+  f2 q[4];
+  bool ok;
+};
+
+__device__ __forceinline__ LatticeCone lattice_cone(f3 apex, const float* now, const float* then) {
+  LatticeCone out;
+  out.ok = false;
+  out.c = Cone{};
+#pragma unroll
+  for (int i = 0; i < 4; i++) out.q[i] = f2{0.0f, 0.0f};
+  const LatticeFrame f = lattice_frame(apex, then);
+  if (!f.ok) return out;
+  const f3 n = norm3(cross3(ld3(now + 4), ld3(now + 8)));
+  if (now[3] > 0.5f && dot3(n, apex - ld3(now)) <= 0.0f) return out;
+  f2 pts[4];
+  float lox = 3.4e38f, loy = 3.4e38f, hix = -3.4e38f, hiy = -3.4e38f;
+  for (uint32_t k = 0; k < 4; k++) {
+    const f3 p = rect_corner_of(now, k) - apex;
+    const float z = dot3(f.forward, p);
+    if (z <= 1e-6f) return out;
+    pts[k] = f2{dot3(f.right, p) / z, dot3(f.up, p) / z};
+    lox = mn(lox, pts[k].x), loy = mn(loy, pts[k].y), hix = mx(hix, pts[k].x), hiy = mx(hiy, pts[k].y);
+  }
+  const float ox = floorf(lox / f.beta_x) * f.beta_x, oy = floorf(loy / f.beta_y) * f.beta_y;
+  const float ex = ox + 8.0f * f.beta_x, ey = oy + 8.0f * f.beta_y;
+  if (hix > ex || hiy > ey) return out;
+  f3 far_n = n;
+  if (dot3(far_n, ld3(now) - apex) < 0.0f) far_n = f3{-far_n.x, -far_n.y, -far_n.z};
+  const float dist = dot3(far_n, ld3(now) - apex);
+  if (dist <= 1e-6f) return out;
+  out.c.apex = apex;
+  out.c.right = f.right;
+  out.c.up = f.up;
+  out.c.forward = f.forward;
+  out.c.w0 = ox;
+  out.c.w1 = oy;
+  out.c.w2 = ex;
+  out.c.w3 = ey;
+  out.c.far_normal = far_n;
+  out.c.far_offset = dot3(far_n, apex) + dist * ONE_MINUS_1EM5;
+  bound_cone(out.c, norm3(light_center_of(now) - apex));
+  const uint32_t order[4] = {0, 1, 3, 2};
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+    out.q[i] = f2{(pts[order[i]].x - ox) / f.beta_x, (pts[order[i]].y - oy) / f.beta_y};
+  out.ok = true;
+  return out;
+}
+
+// The footprint drawn into the lane's accumulator (WGSL :4830-4848 before
+// its prefix sum): bin b's area is min(|sum of cells 0..b|, 1).
+__device__ __forceinline__ void lattice_draw(Lane& L, const f2 q[4]) {
+  clear_acc(L);
+  const float g = (float)GRID;
+#pragma unroll
+  for (uint32_t i = 0; i < 4; i++) {
+    const f2 a{clampf(q[i].x, 0.0f, g), clampf(q[i].y, 0.0f, g)};
+    const f2 e{clampf(q[(i + 1) % 4].x, 0.0f, g), clampf(q[(i + 1) % 4].y, 0.0f, g)};
+    draw_edge(L, a, e, 1.0f);
+  }
+}
+
+struct LatticeOut {
+  float ratio;  // -1 where the lattice cannot hold the footprint, or not traced
+  bool empty, full;
+};
+
+// WGSL :4875-4899 `lattice_ratio` with :4853-4869 `lattice_share`, for the
+// lanes with `on` (every lane takes part in the walk).
+template <bool COUNT>
+__device__ __forceinline__ LatticeOut lattice_ratio(const Ctx& x, const QueryParams& p, Lane& L, bool on, f3 apex,
+                                                    f3 normal, const float* now, const float* then, float* cov,
+                                                    uint64_t* work) {
+  const LatticeCone lc = lattice_cone(apex, now, then);
+  const bool traced = on && lc.ok;
+  Walk w;
+  w.cov = cov;
+  w.full = 0;
+  lattice_draw(L, lc.q);
+  float running = 0.0f;
+  for (uint32_t b = 0; b < BINS; b++) {
+    running += L.get(b);
+    const float m = mn(fabsf(running), 1.0f);
+    w.cov[b] = traced && m <= 0.0f ? 1.0f : 0.0f;
+    if (w.cov[b] >= 1.0f) w.full++;
+  }
+  w.cone_work = 0;
+  w.cone_cap = NONE;
+  w.prefilter_bins = p.prefilter;
+  w.sky_max = false;
+  w.overflow = false;
+  w.truncated = false;
+#pragma unroll
+  for (int k = 0; k < 6; k++) w.work[k] = 0;
+  if (uni(ballot(traced) != 0ull ? 1u : 0u)) scene_coverage<COUNT>(x, lc.c, traced, w, L);
+  if (COUNT && work) {
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      uint32_t v = w.work[k];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) v += (uint32_t)__shfl_xor((int)v, off, 64);
+      if (lane_id() == 0) atomicAdd(reinterpret_cast<unsigned long long*>(&work[k]), (unsigned long long)v);
+    }
+  }
+  // The footprint again (the walk used the accumulator), then the share.
+  lattice_draw(L, lc.q);
+  LatticeOut o;
+  o.empty = true;
+  o.full = true;
+  running = 0.0f;
+  const float g = (float)GRID;
+  float total = 0.0f, open = 0.0f;
+  for (uint32_t b = 0; b < BINS; b++) {
+    running += L.get(b);
+    const float m = mn(fabsf(running), 1.0f);
+    const float cb = w.cov[b];
+    if (cb > 0.0f && m > 0.0f) o.empty = false;
+    if (cb < 1.0f) o.full = false;
+    if (m > 0.0f) {
+      const f3 d = direction(lc.c, ((float)(b % GRID) + 0.5f) / g, ((float)(b / GRID) + 0.5f) / g);
+      const float l2 = dot3(d, d);
+      const float wt = m * mx(dot3(normal, d), 0.0f) / (l2 * l2);
+      total += wt;
+      open += wt * (1.0f - cb);
+    }
+  }
+  o.ratio = !traced ? -1.0f : (total <= 0.0f ? 1.0f : open / total);
+  return o;
+}
+
+// The guard cone's walk (WGSL :4576-4583) for the lanes with `trace`:
+// whether any bin is touched.
+template <bool COUNT>
+__device__ __forceinline__ bool guard_touched(const Ctx& x, const QueryParams& p, Lane& L, const Cone& c,
+                                              bool trace, float* cov, uint64_t* work) {
+  Walk w;
+  w.cov = cov;
+#pragma unroll
+  for (uint32_t b = 0; b < BINS; b++) w.cov[b] = 0.0f;
+  w.full = 0;
+  w.cone_work = 0;
+  w.cone_cap = NONE;
+  w.prefilter_bins = p.prefilter;
+  w.sky_max = false;
+  w.overflow = false;
+  w.truncated = false;
+#pragma unroll
+  for (int k = 0; k < 6; k++) w.work[k] = 0;
+  if (uni(ballot(trace) != 0ull ? 1u : 0u)) scene_coverage<COUNT>(x, c, trace, w, L);
+  if (COUNT && work) {
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+      uint32_t v = w.work[k];
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) v += (uint32_t)__shfl_xor((int)v, off, 64);
+      if (lane_id() == 0) atomicAdd(reinterpret_cast<unsigned long long*>(&work[k]), (unsigned long long)v);
+    }
+  }
+  bool any = false;
+  for (uint32_t b = 0; b < BINS; b++) any = any || w.cov[b] > 0.0f;
+  return trace && any;
+}
+
+
 }  // namespace
 
 // WGSL :3394-3398, and this build's use marks and cleared slots: updates
@@ -486,11 +912,219 @@ __global__ __launch_bounds__(256) void vc_stale_texels_kernel(VcArgs a) {
   }
 }
 
-// WGSL :3432-3554 for static lights: one wave per 64 consecutive jobs, a
-// grid-stride loop over the planned and appended jobs. With COUNT, the
-// walks' work goes to `work` (as the cone kernel's) and capped walks to
-// counters[0].
+// WGSL :4655-4662 `vc_ledger_reset`: one lane per planned page, fresh
+// records for every moving light (version: this frame).
+__global__ __launch_bounds__(64) void vc_ledger_reset_kernel(VcArgs a) {
+  const uint32_t p = blockIdx.x * 64u + threadIdx.x;
+  if (p >= a.plan_pages) return;
+  const uint32_t slot = a.bake_list[4 * (size_t)p + 3];
+  for (uint32_t j = 0; j < a.n_moving; j++) ledger_reset(a, slot, j);
+}
+
+// WGSL :5007-5164 `vc_ledger`: one lane per slot. A page read in the frame
+// before re-bakes its texels whose bound exceeds ε (the whole page past its
+// slack or its pose ring); a page not read is set aside (kept, shown as
+// missing) until a pixel wants it.
+__global__ __launch_bounds__(256) void vc_ledger_kernel(VcArgs a) {
+  const uint32_t s = blockIdx.x * 256u + threadIdx.x;
+  if (s >= a.n_slots) return;
+  const uint32_t* info = a.slot_info + 12 * (size_t)s;
+  const uint32_t set_aside = __atomic_load_n(&a.slot_stale[s], __ATOMIC_RELAXED);
+  if (info[3] == 0u || set_aside == 1u) return;
+  const bool visible = a.slot_used[s] + 1u >= a.frame;
+  const bool lit = (a.ledger_limit >> 31) == 1u;
+  if (set_aside == 2u && !visible) return;
+  if (set_aside == 0u && !lit) return;
+  if (visible) atomicAdd(&a.stats[11], 1u);
+  const f3 lo{__uint_as_float(info[4]), __uint_as_float(info[5]), __uint_as_float(info[6])};
+  const f3 hi{__uint_as_float(info[8]), __uint_as_float(info[9]), __uint_as_float(info[10])};
+  const Box3 page = world_box(a, info[0], lo, hi);
+  const uint32_t key[3] = {info[0], info[1], info[2]};
+  const uint32_t limit = a.ledger_limit & 0x7fffffffu;
+  uint32_t entry = NONE;
+  bool counted = false;
+  for (uint32_t l = 0; l < a.n_lights; l++) {
+    const uint32_t j = a.moving_index[l];
+    if (j == NONE) continue;
+    uint32_t* rec = a.ledger + ledger_base(a, s, j);
+    const uint32_t version = rec[0];
+    if (version == 0u) continue;
+    const float* now = a.lights + 16 * (size_t)l;
+    const float slack = __uint_as_float(rec[1]);
+    const uint32_t page_flags = rec[4];
+    const bool mixed_page = (page_flags & 1u) != 0u;
+    const uint32_t age = a.frame - version;
+    bool whole = age + 1u >= (VC_POSE_RING < MAX_OFFSET ? VC_POSE_RING : MAX_OFFSET);
+    const bool lattice = !(now[3] > 1.5f);
+    if (lattice && !whole && (page_flags & 8u) != 0u) {
+      whole = true;
+      if (visible) atomicAdd(&a.stats[22], 1u);
+    }
+    uint32_t stale[3] = {0u, 0u, 0u};
+    uint32_t count = 0;
+    float bound = 0.0f;
+    bool held = false;
+    if (!whole) {
+      const float d0 = light_shift(pose_at(a, version, l), now, page);
+      if (d0 > slack * hashed_share(s, version, 0x5bd1e995u)) {
+        whole = true;
+      } else if (!mixed_page) {
+        if (visible) atomicAdd(&a.stats[13], 1u);
+        a.ledger_bound[(size_t)s * a.n_moving + j] = 0.0f;
+        continue;
+      } else {
+        if (visible) atomicAdd(&a.stats[12], 1u);
+        if (lattice) {
+          const uint32_t since = rec[3];
+          const float kappa = __uint_as_float(rec[2]);
+          if (a.frame - since < VC_POSE_RING) {
+            const float b = lattice_bound(pose_at(a, since, l), now, page, kappa);
+            held = b <= a.epsilon * hashed_share(s, since, 0x27d4eb2du);
+            if (held && lattice_grew(pose_at(a, version, l), now, page)) {
+              held = false;
+              if (visible) atomicAdd(&a.stats[22], 1u);
+            }
+            if (held) bound = b;
+          }
+        }
+        const uint32_t want = held ? 7u : 3u;
+        for (uint32_t k = 0; k < VC_TEXELS; k++) {
+          const uint32_t half = (rec[5 + k / 2] >> (16u * (k % 2u))) & 0xffffu;
+          if ((half & want) == want) {
+            stale[k / 32] |= 1u << (k % 32u);
+            count++;
+          }
+        }
+        if (held && visible) {
+          atomicAdd(&a.stats[21], 1u);
+          atomicAdd(&a.stats[23], count);
+        }
+        whole = count == VC_TEXELS && !held;
+      }
+    }
+    if (whole) count = VC_TEXELS;
+    a.ledger_bound[(size_t)s * a.n_moving + j] = bound;
+    if (count == 0u) continue;
+    if (!visible) {
+      if (atomicExch(&a.slot_stale[s], 2u) == 0u) atomicAdd(&a.stats[10], 1u);
+      return;
+    }
+    if (entry == NONE) {
+      entry = append_page(a, s, key, lo, hi);
+      if (entry == NONE) {
+        atomicExch(&a.slot_stale[s], 2u);
+        atomicAdd(&a.stats[10], 1u);
+        return;
+      }
+    }
+    uint32_t first = NONE;
+    if (atomicAdd(&a.stats[14], count) + count <= limit) first = append_jobs(a, count);
+    if (first == NONE) {
+      atomicAdd(&a.stats[10], 1u);
+      a.bake_bounds[8 * (size_t)entry + 3] = 1.0f;
+      atomicExch(&a.slot_stale[s], 2u);
+      return;
+    }
+    if (whole) {
+      ledger_reset(a, s, j);
+      for (uint32_t k = 0; k < VC_TEXELS; k++) {
+        a.bake_jobs[2 * (size_t)(first + k)] = entry;
+        a.bake_jobs[2 * (size_t)(first + k) + 1] = k | (l << 8);
+      }
+    } else {
+      uint32_t at = 0;
+      for (uint32_t k = 0; k < VC_TEXELS; k++) {
+        if ((stale[k / 32] & (1u << (k % 32u))) == 0u) continue;
+        a.bake_jobs[2 * (size_t)(first + at)] = entry;
+        a.bake_jobs[2 * (size_t)(first + at) + 1] = k | (l << 8);
+        at++;
+      }
+      if (!held) rec[3] = a.frame;
+    }
+    atomicAdd(&a.stats[9], count);
+    if (!counted) {
+      counted = true;
+      atomicAdd(&a.stats[8], 1u);
+    }
+  }
+  if (set_aside == 2u && entry == NONE) atomicExch(&a.slot_stale[s], 0u);
+}
+
+namespace {
+
+// WGSL :4547-4638 `ledger_texel` for the lanes with `rec_on` (a listed
+// texel of a moving light, records kept): its state (2 clear, 3 mixed, 0
+// fixed or off), the page's slack (the smallest guard, less the light's
+// shift since the version), κ on a lattice, and the page's flags. The guard
+// cone's walk runs as a packet for the lanes baked fresh at the version.
 template <bool COUNT>
+__device__ __forceinline__ void ledger_texel(const VcArgs& a, const Ctx& x, const QueryParams& p, Lane& L,
+                                             bool rec_on, bool on, uint32_t slot, uint32_t j, uint32_t l, uint32_t k,
+                                             f3 apex, f3 n, const LightRect& r, float open, const float* then,
+                                             bool lattice, bool overflow, float* cov, uint64_t* work) {
+  const uint32_t base = rec_on ? ledger_base(a, slot, j) : 0u;
+  const uint32_t version = rec_on ? a.ledger[base] : a.frame;
+  const uint32_t age = a.frame - version;
+  const uint32_t offset = age < MAX_OFFSET ? age : MAX_OFFSET;
+  uint32_t state = 0;
+  float guard = 3.4e38f;
+  float reach = 0.0f, margin = 3.4e38f, bin = 0.0f, kappa = 3.0e38f;
+  if (rec_on && lattice && on && open > 0.0f) {
+    const LatticeFrame f = lattice_frame(apex, then);
+    if (f.ok) {
+      reach = mn(GUARD * f.alpha, 1.4f);
+      bin = sqrtf(f.beta_x * f.beta_x + f.beta_y * f.beta_y);
+      margin = f.alpha + bin;
+      kappa = lattice_kappa(f, n);
+    }
+  }
+  const GuardCone g = guard_cone(apex, r);
+  const bool fresh = rec_on && on && open > 0.0f && offset == 0u;
+  const bool any = guard_touched<COUNT>(x, p, L, g.c, fresh, cov, work);
+  if (!rec_on) return;
+  if (on && open > 0.0f && offset != 0u) {
+    state = 3;
+    if (lattice) guard = reach - margin;
+  } else if (on && open > 0.0f) {
+    state = any ? 3u : 2u;
+    if (-below_horizon(apex, n, r) <= mx(2.0f * mn(GUARD * g.alpha, 1.4f), 0.1f)) state = 3;
+    guard = mn(GUARD * g.alpha, 1.4f) - g.alpha;
+    if (lattice) guard = mn(mn(GUARD * g.alpha, 1.4f), reach) - mx(margin, g.alpha + bin);
+  } else if (on) {
+    const float below = below_horizon(apex, n, r);
+    if (below >= 2.0f * g.alpha)
+      guard = below;
+    else
+      state = 3;
+  }
+  if ((state != 3u || lattice) && guard < 3.0e38f) {
+    float slack = guard;
+    if (offset != 0u) slack = guard - light_shift(pose_at(a, version, l), reinterpret_cast<const float*>(&r), Box3{apex, apex});
+    atomicMin(&a.ledger[base + 1], __float_as_uint(mx(slack, 0.0f)));
+  }
+  uint32_t code = 0;
+  if (lattice && state == 3u) {
+    if (on && open > 0.0f && kappa <= LATTICE_KAPPA_MAX)
+      atomicMax(&a.ledger[base + 2], __float_as_uint(kappa));
+    else
+      code = 4;
+  }
+  const uint32_t shift = 16u * (k % 2u), word = base + 5 + k / 2;
+  atomicAnd(&a.ledger[word], ~(0xffffu << shift));
+  atomicOr(&a.ledger[word], ((offset << 6) | code | state) << shift);
+  const uint32_t flags = (state == 3u ? 1u : 0u) | (offset != 0u ? 2u : 0u) | (overflow ? 8u : 0u);
+  if (flags != 0u) atomicOr(&a.ledger[base + 4], flags);
+}
+
+}  // namespace
+
+// WGSL :3432-3554: one wave per 64 consecutive jobs, a grid-stride loop
+// over the planned and appended jobs. With COUNT, the walks' work goes to
+// `work` (as the cone kernel's) and capped walks to counters[0]. With
+// MOVING (a cache with moving lights), a moving light's texel bakes as WGSL
+// :3448-3530 and :3551-3553 say: a rect light on its page's lattice, ratios
+// within ε of 0 or 1 snapped, and its ledger record kept.
+template <bool COUNT, bool MOVING>
 __global__ __launch_bounds__(64, WALK_WAVES) void vc_bake_kernel(VcArgs a, const NodeC* __restrict__ blas,
                                                      const NodeC* __restrict__ nodes,
                                                      const float* __restrict__ tri_leaf,
@@ -501,6 +1135,7 @@ __global__ __launch_bounds__(64, WALK_WAVES) void vc_bake_kernel(VcArgs a, const
   __shared__ float s_acc[ACC_CELLS * 64];
   const uint32_t lane = lane_id();
   float poly[POLY_SLOTS];
+  float wcov[MOVING ? BINS : 1];  // the lattice and guard walks' coverage
   Lane L{s_acc + lane, poly, POLY_STRIDE, ACC_CELLS, 0u};
   const Ctx x{blas, nodes, tri_leaf, a.tris, a.ranges, items, blas_items, blas_cov, a.xforms,
               p.n_objects, p.hierarchy, p.profile_stage, p.object_cov};
@@ -542,8 +1177,30 @@ __global__ __launch_bounds__(64, WALK_WAVES) void vc_bake_kernel(VcArgs a, const
       open = is_sphere(r) ? spot_falloff(r, norm3(s.apex - f3{r.corner.x, r.corner.y, r.corner.z})) *
                                 sphere_unshadowed(s.apex, s.normal, r)
                           : rect_open(L, s.apex, s.normal, r);
+    // A moving light (WGSL :3448-3472): a rect one bakes on its page's
+    // lattice, in the frame of its pose at the page's version, unless its
+    // bins' weights spread too far; both walks run for every lane.
+    uint32_t mj = NONE;
+    const float* lf = a.lights + 16 * (size_t)l;
+    const float* then = lf;
+    bool on_lattice = false, overflow = false, lattice = false;
+    LatticeOut lat{-1.0f, true, false};
+    if (MOVING) {
+      mj = a.moving_index[l];
+      lattice = mj != NONE && !is_sphere(r);
+      if (lattice && listed) {
+        const uint32_t version = a.ledger[ledger_base(a, t.e[3], mj)];
+        if (version != 0u && a.frame - version < VC_POSE_RING) then = pose_at(a, version, l);
+      }
+      on_lattice = lattice;
+      if (lattice && on && open > 0.0f)
+        on_lattice = lattice_kappa(lattice_frame(s.apex, then), s.normal) <= LATTICE_KAPPA_MAX;
+      lat = lattice_ratio<COUNT>(x, p, L, on && on_lattice && open > 0.0f, s.apex, s.normal, lf, then, wcov,
+                                 COUNT ? work : nullptr);
+      overflow = on && on_lattice && open > 0.0f && lat.ratio < 0.0f;
+    }
     const f3 toward = norm3(light_center(r) - s.apex);
-    const bool back = on && open <= 0.0f && faces_light(r, s.apex, toward);
+    const bool back = on && open <= 0.0f && mj == NONE && faces_light(r, s.apex, toward);
     float seen = open;
     if (back) {
       seen = 0.0f;
@@ -553,8 +1210,9 @@ __global__ __launch_bounds__(64, WALK_WAVES) void vc_bake_kernel(VcArgs a, const
                             : rect_open(L, s.apex, toward, r);
     }
     const Skip skip{back ? t.m : NONE, back ? s.tri : NONE};
-    const QueryOut o = evaluate_at<COUNT, true>(x, p, L, listed, s.apex, back ? toward : s.normal, on, r, nullptr,
-                                                COUNT ? work : nullptr, skip);
+    const QueryOut o = evaluate_at<COUNT, true>(x, p, L, listed, s.apex, back ? toward : s.normal,
+                                                on && (!on_lattice || overflow), r, nullptr, COUNT ? work : nullptr,
+                                                skip);
     if (COUNT) {
       const uint64_t capped = ballot(listed && o.capped);
       if (lane == 0 && capped) atomicAdd(&counters[0], (uint32_t)__builtin_popcountll(capped));
@@ -564,9 +1222,23 @@ __global__ __launch_bounds__(64, WALK_WAVES) void vc_bake_kernel(VcArgs a, const
     float ratio = !on ? 0.0f : (seen > 0.0f ? clampf(value / seen, 0.0f, 1.0f) : 1.0f);
     if (on && seen > 0.0f && last_empty) ratio = 1.0f;
     if (on && seen > 0.0f && last_full) ratio = 0.0f;
-    const float light_value = back ? 0.0f : value;
+    // A moving light's ratio within ε of 0 or 1 is that (WGSL :3515-3520).
+    if (MOVING && mj != NONE) {
+      if (ratio < a.epsilon) ratio = 0.0f;
+      if (ratio > 1.0f - a.epsilon) ratio = 1.0f;
+    }
+    float light_value = back ? 0.0f : value;
+    if (MOVING && on_lattice && !overflow) {
+      ratio = !on ? 0.0f : (open > 0.0f ? lat.ratio : 1.0f);
+      if (on && open > 0.0f && lat.empty) ratio = 1.0f;
+      if (on && open > 0.0f && lat.full) ratio = 0.0f;
+      light_value = ratio * open;
+    }
     if (listed && t.inside)
       a.pool[((size_t)t.e[3] * VC_TEXELS + k) * a.channels + l] = a.light_ratio ? ratio : (on ? light_value : 0.0f);
+    if (MOVING)
+      ledger_texel<COUNT>(a, x, p, L, mj != NONE && listed && t.inside && a.ledger_record != 0u, on, t.e[3], mj, l,
+                          k, s.apex, s.normal, r, open, then, lattice, overflow, wcov, COUNT ? work : nullptr);
   }
 }
 
@@ -640,15 +1312,9 @@ struct ShadeArgs {
 
 namespace {
 
-__device__ __forceinline__ uint32_t key_hash(uint32_t k0, uint32_t k1, uint32_t k2) {
-  uint32_t h = (k0 * 0x9E3779B1u) ^ (k1 * 0x85EBCA77u) ^ (k2 * 0xC2B2AE3Du);
-  h ^= h >> 15;
-  h *= 0x2C1B3C6Du;
-  h ^= h >> 12;
-  return h;
-}
-
-// WGSL :3134-3160 without stale marks other than 0 and 3 (no ledger here).
+// WGSL :3134-3160: a page marked stale (1) or set aside by the ledger (2)
+// is missing; one set aside is marked wanted, so the next frame's ledger
+// pass re-bakes what it lacks.
 __device__ __forceinline__ uint32_t vc_find(const VcArgs& a, uint32_t k0, uint32_t k1, uint32_t k2) {
   uint32_t at = key_hash(k0, k1, k2) & a.table_mask;
   for (uint32_t probe = 0; probe < vc::MAX_PROBE; probe++) {
@@ -656,6 +1322,7 @@ __device__ __forceinline__ uint32_t vc_find(const VcArgs& a, uint32_t k0, uint32
     if (e[0] == vc::EMPTY) return NONE;
     if (e[0] == k0 && e[1] == k1 && e[2] == k2) {
       const uint32_t stale = a.slot_stale[e[3]];
+      if (stale == 2u) a.slot_used[e[3]] = a.frame;
       return (stale != 0u && stale != 3u) ? NONE : e[3];
     }
     at = (at + 1u) & a.table_mask;
@@ -1032,6 +1699,18 @@ struct cnq_vcache {
   uint32_t n_movers = 0;
   std::vector<std::pair<uint32_t, uint32_t>> mover_ids;
   uint64_t last_stats[10] = {0};
+  // Moving lights through the ledger (cnq_vcache_set_moving_lights): per
+  // light its index among them (NONE: static), ε, whether one moved since
+  // the last frame, whether the table changed on the host since its upload,
+  // whether objects ever moved (the two do not mix), and the last frame's
+  // ledger counters.
+  std::vector<uint32_t> moving_index;
+  uint32_t n_moving = 0;
+  float epsilon = 0.01f;
+  bool lights_moved = false, lights_dirty = false, had_movers = false;
+  uint64_t last_ledger[10] = {0};
+  DevBuf<uint32_t> d_moving_index, d_ledger;
+  DevBuf<float> d_ledger_bound, d_poses;
   uint32_t shade_requests = 0;  // requests the last shading pass listed
   DevBuf<float> d_camera;
   std::vector<uint32_t> upd_host;  // staging for flush_updates (alive until the stream syncs)
@@ -1052,6 +1731,7 @@ struct cnq_vcache {
     d_tri_chart.release(), d_request_set.release(), d_request_prio.release(), d_requests.release();
     d_shade_counts.release();
     d_camera.release();
+    d_moving_index.release(), d_ledger.release(), d_ledger_bound.release(), d_poses.release();
   }
   uint32_t append_jobs() const { return cfg.append_pages * vc::PAGE_TEXELS * n_lights; }
 
@@ -1095,6 +1775,14 @@ struct cnq_vcache {
     a.bake_group = 64;
     a.bake_resident = 1;
     a.inv_base = inv_base;
+    a.moving_index = d_moving_index.p;
+    a.ledger = d_ledger.p;
+    a.ledger_bound = d_ledger_bound.p;
+    a.poses = d_poses.p;
+    a.n_moving = n_moving;
+    a.ledger_limit = 0;
+    a.ledger_record = 1;
+    a.epsilon = epsilon;
     return a;
   }
 
@@ -1118,6 +1806,30 @@ struct cnq_vcache {
 };
 
 namespace {
+
+// The bake over `a`'s jobs on `st`: the work-counting and moving-light
+// instantiations as the scene and the cache ask.
+hipError_t launch_bake(cnq_vcache* c, const cnq::VcArgs& a, const cnq::QueryParams& p, uint32_t grid,
+                       hipStream_t st) {
+  cnq_scene* s = c->scene;
+  const bool moving = c->n_moving > 0;
+  if (s->count_work) {
+    if (moving)
+      hipLaunchKernelGGL((cnq::vc_bake_kernel<true, true>), dim3(grid), dim3(64), 0, st, a, s->blas.p, s->nodes.p,
+                         s->tri_leaf.p, s->items.p, s->blas_items.p, s->blas_cov.p, p, s->work.p, s->counters.p);
+    else
+      hipLaunchKernelGGL((cnq::vc_bake_kernel<true, false>), dim3(grid), dim3(64), 0, st, a, s->blas.p, s->nodes.p,
+                         s->tri_leaf.p, s->items.p, s->blas_items.p, s->blas_cov.p, p, s->work.p, s->counters.p);
+  } else {
+    if (moving)
+      hipLaunchKernelGGL((cnq::vc_bake_kernel<false, true>), dim3(grid), dim3(64), 0, st, a, s->blas.p, s->nodes.p,
+                         s->tri_leaf.p, s->items.p, s->blas_items.p, s->blas_cov.p, p, nullptr, nullptr);
+    else
+      hipLaunchKernelGGL((cnq::vc_bake_kernel<false, false>), dim3(grid), dim3(64), 0, st, a, s->blas.p, s->nodes.p,
+                         s->tri_leaf.p, s->items.p, s->blas_items.p, s->blas_cov.p, p, nullptr, nullptr);
+  }
+  return hipGetLastError();
+}
 
 // Sends the page-table entries changed since the last flush, the use marks
 // and the cleared slots (`vc_update`), on `st`; the caller synchronizes.
@@ -1372,6 +2084,9 @@ int cnq_vcache_evict(cnq_vcache* c, const uint32_t* keys, uint32_t n) {
 
 int cnq_vcache_set_moved(cnq_vcache* c, const cnq_moved* moved, uint32_t n, const float* scene_box) {
   if (!c || (n && !moved)) return fail(CNQ_INVALID_ARG, "null argument");
+  if (c->n_moving && n)
+    return fail(CNQ_INVALID_ARG, "moving lights and moving objects together are not supported (vcache.rs:1410)");
+  if (n) c->had_movers = true;
   std::vector<vc::Moved> list(n);
   for (uint32_t i = 0; i < n; i++) {
     if (moved[i].object >= c->scene->n_objects) return fail(CNQ_INVALID_ARG, "moved object out of range");
@@ -1453,6 +2168,23 @@ int cnq_vcache_frame(cnq_vcache* c, void* stream, uint64_t stats[10]) {
   if (rc != CNQ_OK) return rc;
   cnq::VcArgs a = c->args();
   if (c->moved.empty()) a.n_movers = 0;
+  if (c->n_moving) {
+    // The lights as moved since the last frame, this frame's poses
+    // (vcache.rs:2261-2266), and fresh ledger records for the planned pages
+    // (WGSL :4655-4662).
+    if (c->lights_dirty) {
+      HIP_TRY(hipMemcpyAsync(c->d_lights.p, c->lights.data(), 64 * (size_t)c->n_lights, hipMemcpyHostToDevice, st),
+              "lights");
+      c->lights_dirty = false;
+    }
+    HIP_TRY(hipMemcpyAsync(c->d_poses.p + (size_t)(c->frame % cnq::VC_POSE_RING) * c->n_lights * 16,
+                           c->lights.data(), 64 * (size_t)c->n_lights, hipMemcpyHostToDevice, st),
+            "light poses");
+    if (plan_pages) {
+      hipLaunchKernelGGL(cnq::vc_ledger_reset_kernel, dim3((plan_pages + 63) / 64), dim3(64), 0, st, a);
+      HIP_TRY(hipGetLastError(), "vc_ledger_reset launch");
+    }
+  }
   if (a.n_movers > 0 && c->slot_extent > 0) {
     hipLaunchKernelGGL(cnq::vc_stale_kernel, dim3((c->slot_extent + 255) / 256), dim3(256), 0, st, a);
     HIP_TRY(hipGetLastError(), "vc_stale launch");
@@ -1462,25 +2194,27 @@ int cnq_vcache_frame(cnq_vcache* c, void* stream, uint64_t stats[10]) {
       HIP_TRY(hipGetLastError(), "vc_stale_texels launch");
     }
   }
+  // The ledger runs every frame with moving lights (instanced.rs:176-185):
+  // it re-bakes what a move took past ε, and pages set aside once wanted.
+  if (c->n_moving && c->slot_extent > 0) {
+    a.ledger_limit = std::min<uint32_t>(a.append_jobs, 0x7fffffffu) | (c->lights_moved ? 0x80000000u : 0u);
+    hipLaunchKernelGGL(cnq::vc_ledger_kernel, dim3((c->slot_extent + 255) / 256), dim3(256), 0, st, a);
+    HIP_TRY(hipGetLastError(), "vc_ledger launch");
+  }
+  const bool appends = a.n_movers > 0 || c->n_moving > 0;
   const cnq::QueryParams p = query_params(s, &c->opts, 64);
   // As many waves as the GPU holds at once (4 SIMDs a compute unit,
   // WALK_WAVES each), or fewer for a small bake; they take jobs from a
   // counter, fewer at a time near the end (vc_bake_kernel).
-  const uint64_t jobs = (uint64_t)plan_jobs + (a.n_movers ? a.append_jobs : 0);
+  const uint64_t jobs = (uint64_t)plan_jobs + (appends ? a.append_jobs : 0);
   const uint32_t resident = s->dev->compute_units * 4 * (uint32_t)cnq::WALK_WAVES;
   a.bake_group = 64;
   a.bake_resident = resident;
   if (jobs) {
     const uint32_t grid = (uint32_t)std::min<uint64_t>((jobs + 3) / 4, resident);
-    if (s->count_work)
-      hipLaunchKernelGGL(cnq::vc_bake_kernel<true>, dim3(grid), dim3(64), 0, st, a, s->blas.p, s->nodes.p,
-                         s->tri_leaf.p, s->items.p, s->blas_items.p, s->blas_cov.p, p, s->work.p, s->counters.p);
-    else
-      hipLaunchKernelGGL(cnq::vc_bake_kernel<false>, dim3(grid), dim3(64), 0, st, a, s->blas.p, s->nodes.p,
-                         s->tri_leaf.p, s->items.p, s->blas_items.p, s->blas_cov.p, p, nullptr, nullptr);
-    HIP_TRY(hipGetLastError(), "vc_bake launch");
+    HIP_TRY(launch_bake(c, a, p, grid, st), "vc_bake launch");
   }
-  const uint32_t page_blocks = std::min<uint32_t>(plan_pages + (a.n_movers ? c->cfg.append_pages : 0), 4096);
+  const uint32_t page_blocks = std::min<uint32_t>(plan_pages + (appends ? c->cfg.append_pages : 0), 4096);
   if (page_blocks) {
     hipLaunchKernelGGL(cnq::vc_fill_kernel, dim3(page_blocks), dim3(64), 0, st, a);
     HIP_TRY(hipGetLastError(), "vc_fill launch");
@@ -1528,6 +2262,9 @@ int cnq_vcache_frame(cnq_vcache* c, void* stream, uint64_t stats[10]) {
                                evicted};
   std::memcpy(c->last_stats, st_out, sizeof(st_out));
   if (stats) std::memcpy(stats, st_out, sizeof(st_out));
+  const uint32_t ledger_words[10] = {8, 9, 10, 11, 12, 13, 14, 21, 22, 23};
+  for (int i = 0; i < 10; i++) c->last_ledger[i] = dstats[ledger_words[i]];
+  c->lights_moved = false;
   c->plan_list.clear(), c->plan_bounds.clear(), c->plan_jobs.clear(), c->used_marks.clear();
   c->moved.clear();
   c->n_movers = 0;
@@ -1599,7 +2336,8 @@ void list_pages(const cnq_vcache* c, const std::vector<std::pair<Key, uint32_t>>
 // the DevBufs given), then the border texels of channels [ch0, ch1) when
 // ch0 < ch1.
 int bake_lists(cnq_vcache* c, const BakeLists& b, float* pool, hipStream_t st, uint32_t ch0, uint32_t ch1,
-               DevBuf<float>& dbounds, DevBuf<uint32_t>& dlist, DevBuf<uint32_t>& djobs, DevBuf<uint32_t>& dstats) {
+               DevBuf<float>& dbounds, DevBuf<uint32_t>& dlist, DevBuf<uint32_t>& djobs, DevBuf<uint32_t>& dstats,
+               bool record = true) {
   cnq_scene* s = c->scene;
   const uint32_t zeros[cnq::VC_STATS] = {0};
   hipError_t e = dbounds.upload(b.bounds.data(), b.bounds.size());
@@ -1618,19 +2356,14 @@ int bake_lists(cnq_vcache* c, const BakeLists& b, float* pool, hipStream_t st, u
   a.plan_jobs = (uint32_t)(b.jobs.size() / 2);
   a.append_pages = 0;
   a.append_jobs = 0;
+  a.ledger_record = record ? 1u : 0u;
   const uint32_t resident = s->dev->compute_units * 4 * (uint32_t)cnq::WALK_WAVES;
   a.bake_group = 64;
   a.bake_resident = resident;
   const cnq::QueryParams p = query_params(s, &c->opts, 64);
   if (a.plan_jobs) {
     const uint32_t grid = (uint32_t)std::min<uint64_t>((a.plan_jobs + 3) / 4, resident);
-    if (s->count_work)
-      hipLaunchKernelGGL(cnq::vc_bake_kernel<true>, dim3(grid), dim3(64), 0, st, a, s->blas.p, s->nodes.p,
-                         s->tri_leaf.p, s->items.p, s->blas_items.p, s->blas_cov.p, p, s->work.p, s->counters.p);
-    else
-      hipLaunchKernelGGL(cnq::vc_bake_kernel<false>, dim3(grid), dim3(64), 0, st, a, s->blas.p, s->nodes.p,
-                         s->tri_leaf.p, s->items.p, s->blas_items.p, s->blas_cov.p, p, nullptr, nullptr);
-    HIP_TRY(hipGetLastError(), "bake launch");
+    HIP_TRY(launch_bake(c, a, p, grid, st), "bake launch");
   }
   if (ch0 < ch1 && a.plan_pages) {
     hipLaunchKernelGGL(cnq::vc_border_kernel, dim3(std::min<uint32_t>(a.plan_pages, 4096)), dim3(64), 0, st, a,
@@ -1646,6 +2379,8 @@ int cnq_vcache_set_lights(cnq_vcache* c, const float* lights, uint32_t first, ui
                           uint64_t stats[2]) {
   if (!c || (n && !lights)) return fail(CNQ_INVALID_ARG, "null argument");
   if ((uint64_t)first + n > c->n_lights) return fail(CNQ_INVALID_ARG, "lights past the cache's light count");
+  for (uint32_t l = first; c->n_moving && l < first + n; l++)
+    if (c->moving_index[l] != cnq::NONE) return fail(CNQ_INVALID_ARG, "a moving light moves by cnq_vcache_move_light");
   if (stats) stats[0] = stats[1] = 0;
   if (n == 0) return CNQ_OK;
   HIP_TRY(hipSetDevice(c->scene->dev->ordinal), "hipSetDevice");
@@ -1708,7 +2443,7 @@ int cnq_vcache_verify(cnq_vcache* c, void* stream, uint64_t out[2]) {
     cleanup();
     return hip_fail(e, "verify buffers");
   }
-  int rc = bake_lists(c, b, fresh.p, st, 0, 0, dbounds, dlist, djobs, dstats);
+  int rc = bake_lists(c, b, fresh.p, st, 0, 0, dbounds, dlist, djobs, dstats, false);
   if (rc != CNQ_OK) {
     cleanup();
     return rc;
@@ -1723,6 +2458,142 @@ int cnq_vcache_verify(cnq_vcache* c, void* stream, uint64_t out[2]) {
   if (e != hipSuccess) return hip_fail(e, "verify");
   out[0] = d;
   out[1] = (uint64_t)n * per_slot;
+  return CNQ_OK;
+}
+
+int cnq_vcache_set_moving_lights(cnq_vcache* c, const uint32_t* lights, uint32_t n, float epsilon) {
+  if (!c || (n && !lights)) return fail(CNQ_INVALID_ARG, "null argument");
+  if (c->frame != 1 || c->n_moving) return fail(CNQ_INVALID_ARG, "moving lights are set once, before the first frame");
+  if (c->had_movers || c->n_movers)
+    return fail(CNQ_INVALID_ARG, "moving lights and moving objects together are not supported (vcache.rs:1410)");
+  if (!(epsilon >= 0.0f && epsilon < 0.5f)) return fail(CNQ_INVALID_ARG, "epsilon in [0, 0.5)");
+  if (!c->cfg.light_ratio) return fail(CNQ_INVALID_ARG, "moving lights need light_ratio (cached ratios)");
+  if (n == 0) return CNQ_OK;
+  std::vector<uint32_t> index(c->n_lights, cnq::NONE);
+  for (uint32_t j = 0; j < n; j++) {
+    if (lights[j] >= c->n_lights) return fail(CNQ_INVALID_ARG, "moving light past the light table");
+    if (index[lights[j]] != cnq::NONE) return fail(CNQ_INVALID_ARG, "a moving light listed twice");
+    index[lights[j]] = j;
+  }
+  HIP_TRY(hipSetDevice(c->scene->dev->ordinal), "hipSetDevice");
+  const size_t records = (size_t)c->cfg.pages * n;
+  hipError_t e = c->d_moving_index.upload(index.data(), index.size());
+  if (e == hipSuccess) e = c->d_ledger.alloc(records * cnq::LEDGER_WORDS_HOST);
+  if (e == hipSuccess) e = hipMemset(c->d_ledger.p, 0, c->d_ledger.bytes());
+  if (e == hipSuccess) e = c->d_ledger_bound.alloc(records);
+  if (e == hipSuccess) e = hipMemset(c->d_ledger_bound.p, 0, c->d_ledger_bound.bytes());
+  if (e == hipSuccess) e = c->d_poses.alloc((size_t)cnq::VC_POSE_RING * c->n_lights * 16);
+  if (e == hipSuccess) e = hipMemset(c->d_poses.p, 0, c->d_poses.bytes());
+  if (e != hipSuccess) {
+    c->d_moving_index.release(), c->d_ledger.release(), c->d_ledger_bound.release(), c->d_poses.release();
+    return hip_fail(e, "ledger buffers");
+  }
+  c->moving_index = index;
+  c->n_moving = n;
+  c->epsilon = epsilon;
+  return CNQ_OK;
+}
+
+int cnq_vcache_move_light(cnq_vcache* c, uint32_t index, const float* light) {
+  if (!c || !light) return fail(CNQ_INVALID_ARG, "null argument");
+  if (index >= c->n_lights || !c->n_moving || c->moving_index[index] == cnq::NONE)
+    return fail(CNQ_INVALID_ARG, "not a moving light (cnq_vcache_set_moving_lights)");
+  if ((light[3] > 1.5f) != (c->lights[16 * (size_t)index + 3] > 1.5f))
+    return fail(CNQ_INVALID_ARG, "a moving light keeps its kind (rect or sphere)");
+  std::copy(light, light + 16, c->lights.begin() + 16 * (size_t)index);
+  c->light_boxes[index] = vc::light_box(&c->lights[16 * (size_t)index]);
+  c->lights_moved = true;
+  c->lights_dirty = true;
+  return CNQ_OK;
+}
+
+int cnq_vcache_ledger_stats(cnq_vcache* c, uint64_t out[10]) {
+  if (!c || !out) return fail(CNQ_INVALID_ARG, "null argument");
+  std::memcpy(out, c->last_ledger, sizeof(c->last_ledger));
+  return CNQ_OK;
+}
+
+int cnq_vcache_ledger_state(cnq_vcache* c, uint32_t* records, float* bounds) {
+  if (!c) return fail(CNQ_INVALID_ARG, "null argument");
+  if (!c->n_moving) return fail(CNQ_INVALID_ARG, "no moving lights");
+  HIP_TRY(hipSetDevice(c->scene->dev->ordinal), "hipSetDevice");
+  if (records) HIP_TRY(hipMemcpy(records, c->d_ledger.p, c->d_ledger.bytes(), hipMemcpyDeviceToHost), "ledger");
+  if (bounds)
+    HIP_TRY(hipMemcpy(bounds, c->d_ledger_bound.p, c->d_ledger_bound.bytes(), hipMemcpyDeviceToHost), "bounds");
+  return CNQ_OK;
+}
+
+int cnq_vcache_verify_ledger(cnq_vcache* c, uint32_t light, void* stream, float out[6]) {
+  if (!c || !out) return fail(CNQ_INVALID_ARG, "null argument");
+  for (int i = 0; i < 6; i++) out[i] = 0.0f;
+  if (light >= c->n_lights || !c->n_moving || c->moving_index[light] == cnq::NONE)
+    return fail(CNQ_INVALID_ARG, "not a moving light");
+  if (!c->plan_list.empty()) return fail(CNQ_INVALID_ARG, "pages are planned but not baked: run cnq_vcache_frame first");
+  cnq_scene* s = c->scene;
+  HIP_TRY(hipSetDevice(s->dev->ordinal), "hipSetDevice");
+  hipStream_t st = (hipStream_t)stream;
+  HIP_TRY(hipStreamSynchronize(st), "verify ledger");
+  const uint32_t j = c->moving_index[light];
+  std::vector<uint32_t> marks(c->slot_extent);
+  if (c->slot_extent)
+    HIP_TRY(hipMemcpy(marks.data(), c->d_slot_stale.p, 4 * (size_t)c->slot_extent, hipMemcpyDeviceToHost), "marks");
+  std::vector<std::pair<Key, uint32_t>> pages;
+  for (const auto& kv : c->map) {
+    const uint32_t slot = kv.second.first;
+    if (slot < c->slot_extent && marks[slot] == 0) pages.push_back({kv.first, slot});
+  }
+  if (pages.empty()) return CNQ_OK;
+  std::sort(pages.begin(), pages.end(),
+            [](const std::pair<Key, uint32_t>& x, const std::pair<Key, uint32_t>& y) { return x.second < y.second; });
+  BakeLists b;
+  list_pages(c, pages, light, light + 1, b);
+  DevBuf<float> fresh, dbounds;
+  DevBuf<uint32_t> dlist, djobs, dstats;
+  auto cleanup = [&]() { fresh.release(), dbounds.release(), dlist.release(), djobs.release(), dstats.release(); };
+  hipError_t e = fresh.alloc(c->d_pool.n);
+  if (e == hipSuccess) e = hipMemcpyAsync(fresh.p, c->d_pool.p, c->d_pool.bytes(), hipMemcpyDeviceToDevice, st);
+  if (e != hipSuccess) {
+    cleanup();
+    return hip_fail(e, "verify ledger buffers");
+  }
+  int rc = bake_lists(c, b, fresh.p, st, 0, 0, dbounds, dlist, djobs, dstats, false);
+  if (rc != CNQ_OK) {
+    cleanup();
+    return rc;
+  }
+  std::vector<float> before(c->d_pool.n), after(c->d_pool.n), bound((size_t)c->cfg.pages * c->n_moving);
+  e = hipStreamSynchronize(st);
+  if (e == hipSuccess) e = hipMemcpy(before.data(), c->d_pool.p, c->d_pool.bytes(), hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(after.data(), fresh.p, c->d_pool.bytes(), hipMemcpyDeviceToHost);
+  if (e == hipSuccess)
+    e = hipMemcpy(bound.data(), c->d_ledger_bound.p, c->d_ledger_bound.bytes(), hipMemcpyDeviceToHost);
+  cleanup();
+  if (e != hipSuccess) return hip_fail(e, "verify ledger");
+  uint32_t n_pages = 0, violations = 0, over = 0, changed = 0;
+  float largest = 0.0f, largest_bound = 0.0f;
+  for (const auto& pg : pages) {
+    const uint32_t* k = pg.first.k;
+    const uint32_t slot = pg.second;
+    const uint32_t chart = k[1] & vc::KEY_CHART_MASK, mip = k[1] >> vc::KEY_MIP_SHIFT;
+    uint32_t g[2];
+    vc::grid(c->tables.charts[chart], c->scale[k[0]], c->inv_base, mip, g);
+    const uint32_t px = (k[2] & 0xffffu) * vc::PAGE, py = (k[2] >> 16) * vc::PAGE;
+    float change = 0.0f;
+    for (uint32_t t = 0; t < vc::PAGE_TEXELS; t++) {
+      if (px + t % vc::STRIDE >= g[0] || py + t / vc::STRIDE >= g[1]) continue;  // border texels copy inside ones
+      const size_t at = ((size_t)slot * vc::PAGE_TEXELS + t) * c->channels + light;
+      change = std::max(change, std::fabs(before[at] - after[at]));
+    }
+    const float bd = bound[(size_t)slot * c->n_moving + j];
+    n_pages++;
+    if (change > bd + 1e-5f) violations++;
+    if (change > c->epsilon) over++;
+    if (change > 0.0f) changed++;
+    largest = std::max(largest, change);
+    largest_bound = std::max(largest_bound, bd);
+  }
+  out[0] = (float)n_pages, out[1] = (float)violations, out[2] = (float)over, out[3] = (float)changed;
+  out[4] = largest, out[5] = largest_bound;
   return CNQ_OK;
 }
 

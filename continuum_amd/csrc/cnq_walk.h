@@ -85,6 +85,7 @@ __device__ __forceinline__ LightCone light_cone_sphere(f3 apex, const LightRect&
   out.c.w1 = -t;
   out.c.w2 = t;
   out.c.w3 = t;
+  out.c.far_normal = forward;
   out.c.far_offset = dot3(forward, apex) + (d - radius * s) * ONE_MINUS_1EM5;
   bound_cone(out.c, forward);
   out.ok = true;
@@ -146,6 +147,7 @@ __device__ __forceinline__ LightCone light_cone_rect(f3 apex, const LightRect& r
   out.c.w1 = lo.y;
   out.c.w2 = hi.x;
   out.c.w3 = hi.y;
+  out.c.far_normal = forward;
   out.c.far_offset = dot3(forward, apex) + dist * ONE_MINUS_1EM5;
   f3 axis = norm3(rel + (edge_u + edge_v) * 0.5f);
   if (out.wide_clipped) axis = norm3(direction(out.c, 0.5f, 0.5f));
@@ -384,7 +386,8 @@ __device__ __forceinline__ Cone cone_to_object(const Cone& c, const float* __res
   o.w1 = c.w1;
   o.w2 = c.w2;
   o.w3 = c.w3;
-  o.far_offset = (c.far_offset - dot3(c.forward, t)) * inv_scale;
+  o.far_normal = rot(c.far_normal);
+  o.far_offset = (c.far_offset - dot3(c.far_normal, t)) * inv_scale;
   o.baxis = rot(c.baxis);
   o.bcos = c.bcos;
   o.bsin = c.bsin;
@@ -500,7 +503,7 @@ __device__ __forceinline__ uint32_t node_cone_test(const Cone& c, Lane& L, float
   for (uint32_t k = 0; k < 8; k++) {
     f3 corner = box_corner(lo, hi, k);
     if (SKY && dot3(c.horizon, corner - c.apex) >= 0.0f) below = false;
-    bool in_far = c.far_offset - dot3(c.forward, corner) >= 0.0f;
+    bool in_far = c.far_offset - dot3(c.far_normal, corner) >= 0.0f;
     if (in_far)
       far_out = false;
     else
@@ -745,7 +748,7 @@ __device__ __forceinline__ bool splat_object_node(const Ctx& x, const Cone& c, W
   f2 mnv{3.4e38f, 3.4e38f}, mxv{-3.4e38f, -3.4e38f};
   for (uint32_t k = 0; k < 8; k++) {
     f3 corner = box_corner(lo, hi, k);
-    if (c.far_offset - dot3(c.forward, corner) < 0.0f) return false;
+    if (c.far_offset - dot3(c.far_normal, corner) < 0.0f) return false;
     f3 p = to_local(c, corner);
     if (p.z < NEAR) return false;
     f2 q{(p.x / p.z - c.w0) / (c.w2 - c.w0) * g, (p.y / p.z - c.w1) / (c.w3 - c.w1) * g};
@@ -966,6 +969,7 @@ __device__ __forceinline__ Cone sky_face(uint32_t f, f3 apex, f3 normal, bool ho
   c.w1 = -1.0f;
   c.w2 = 1.0f;
   c.w3 = 1.0f;
+  c.far_normal = forward;
   c.far_offset = dot3(forward, apex) + SKY_FAR;
   c.horizon = horizon_cull ? normal : f3{0.0f, 0.0f, 0.0f};
   bound_cone(c, forward);

@@ -761,6 +761,44 @@ class ViewCache:
         self.lights[first:first + len(rows)] = rows
         return dict(pages=int(st[0]), cones=int(st[1]))
 
+    LEDGER_STATS = ("pages", "texels", "set_aside", "checked", "mixed", "held_whole", "jobs", "held_lattice",
+                    "grew", "unbounded")
+
+    def set_moving_lights(self, lights, epsilon: float = 0.01) -> None:
+        """Lights (indices into the table) that move through the error ledger
+        (the reference's `moving` list), with threshold `epsilon`; before
+        the first frame."""
+        idx = np.ascontiguousarray(lights, np.uint32).reshape(-1)
+        check(lib().cnq_vcache_set_moving_lights(self.handle, _ptr(idx), len(idx), float(epsilon)))
+        self.moving = [int(i) for i in idx]
+
+    def move_light(self, index: int, light) -> None:
+        """Moving light `index` is `light` from the next frame."""
+        row = np.ascontiguousarray(light, np.float32).reshape(16)
+        check(lib().cnq_vcache_move_light(self.handle, int(index), _ptr(row)))
+        self.lights[int(index)] = row
+
+    def ledger_stats(self) -> dict:
+        out = np.zeros(10, np.uint64)
+        check(lib().cnq_vcache_ledger_stats(self.handle, _ptr(out)))
+        return {n: int(v) for n, v in zip(self.LEDGER_STATS, out)}
+
+    def ledger_state(self):
+        """(records (pages, moving, 48) u32, bounds (pages, moving) f32)."""
+        pages, m = int(self.config.pages), len(self.moving)
+        rec = np.zeros((pages, m, 48), np.uint32)
+        bound = np.zeros((pages, m), np.float32)
+        check(lib().cnq_vcache_ledger_state(self.handle, _ptr(rec), _ptr(bound)))
+        return rec, bound
+
+    def verify_ledger(self, light: int, stream=None) -> dict:
+        """`verify_ledger` (instanced.rs:3741): every page shown baked afresh
+        for moving light `light` against the cache."""
+        out = np.zeros(6, np.float32)
+        check(lib().cnq_vcache_verify_ledger(self.handle, int(light), _stream_ptr(stream), _ptr(out)))
+        return dict(pages=int(out[0]), violations=int(out[1]), over_epsilon=int(out[2]), changed=int(out[3]),
+                    largest_change=float(out[4]), largest_bound=float(out[5]))
+
     def verify(self, stream=None) -> tuple:
         """`verify_cache` (instanced.rs:3721): every page shown baked afresh
         and compared bit for bit; (values that differ, values compared)."""

@@ -63,6 +63,28 @@ def sphere_light(center, radius: float, radiance=(1.0, 1.0, 1.0)) -> np.ndarray:
     return np.array([*center, 2.0, radius, 0.0, 1.0, 0.0, 0.0, 0.0, 1.0, 0.0, *radiance, 0.0], dtype=F32)
 
 
+def sun_light(center, toward, distance: float, irradiance=(1.0, 1.0, 1.0)) -> np.ndarray:
+    """LightGpu of the reference's `Light::sun` (lib.rs:661-700, through
+    `Light::distant`): a one-sided square 0.5 degrees across, `distance` from
+    `center` toward `toward`, facing `center`; radiance irradiance / its
+    solid angle. f64 as there, then f32."""
+    size_angle = 2.0 * math.tan(math.radians(0.25))
+    solid_angle = size_angle * size_angle
+    t = np.asarray(toward, np.float64)
+    t = t / np.linalg.norm(t)
+    size = distance * math.sqrt(max(solid_angle, 1e-12))
+    w = -t
+    z = np.array([0.0, 0.0, 1.0])
+    side = np.cross(z, w) if np.linalg.norm(np.cross(w, z)) > 1e-6 else np.array([1.0, 0.0, 0.0])
+    a = side / np.linalg.norm(side) * size
+    b = np.cross(w, a)
+    b = b / np.linalg.norm(b) * size
+    middle = np.asarray(center, np.float64) + t * distance
+    solid = (size / distance) * (size / distance)
+    radiance = np.asarray(irradiance, np.float64) / solid
+    return rect_light(middle - (a + b) * 0.5, a, b, one_sided=True, radiance=tuple(radiance))
+
+
 def _glam_normalize(v: np.ndarray) -> np.ndarray:
     v = v.astype(F32)
     n = F32(math.sqrt(float(F32(v[0] * v[0]) + F32(v[1] * v[1]) + F32(v[2] * v[2]))))

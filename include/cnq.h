@@ -589,15 +589,52 @@ int cnq_vcache_shade(cnq_vcache* cache, const float* camera, uint32_t width, uin
  * planned pages bake in the next cnq_vcache_frame. stats: [requests, pages
  * planned, cones planned, pages evicted]. */
 int cnq_vcache_plan(cnq_vcache* cache, uint32_t budget, uint64_t stats[4]);
-/* A light edit (`Renderer::set_light`, renderer.rs:421): lights [first,
- * first + n) replaced by `lights` (n x 16 floats) and those lights' channels
- * of every resident page baked afresh, inside texels then border texels.
- * The reference bounds a moving light's staleness with its error ledger and
- * re-bakes only where the bound passes its epsilon; this re-bakes every
- * page the light has, so each one equals a fresh bake. stats (nullable):
- * [pages, cones]. Waits for `stream`. */
+/* A light edit of static lights (`Renderer::set_light`, renderer.rs:421,
+ * for a light that is not moving): lights [first, first + n) replaced by
+ * `lights` (n x 16 floats) and those lights' channels of every resident page
+ * baked afresh, inside texels then border texels, so each one equals a
+ * fresh bake. Refuses a moving light (cnq_vcache_move_light moves those).
+ * stats (nullable): [pages, cones]. Waits for `stream`. */
 int cnq_vcache_set_lights(cnq_vcache* cache, const float* lights, uint32_t first, uint32_t n, void* stream,
                           uint64_t stats[2]);
+
+/* Moving lights through the error ledger (WGSL :4457-5164; vcache.rs
+ * `VirtualCache::new`'s `moving` list, :1399-1430). Lights `lights[0..n)`
+ * of the table move: each texel keeps its light's shadowed share of the
+ * unshadowed light, and shading multiplies that by the unshadowed light
+ * where the light is now. Each page keeps a 48-word record per moving
+ * light: the frame it was baked at (its version), its slack (the smallest
+ * guard cone's margin of its clear and facing-away texels, less the light's
+ * shift since), and each texel's state (clear: nothing meets its guard cone,
+ * three times the light's angular radius; mixed; or facing away). A rect
+ * light bakes on a lattice fixed in its pose at the page's version, so a
+ * mixed texel's ratio moves only by a bounded amount (`lattice_bound`). Each
+ * frame the ledger pass re-bakes, for pages read in the frame before, the
+ * mixed texels whose bound passes `epsilon`, or the whole page past a hashed
+ * share (a half to all) of its slack; a page not read is set aside (kept,
+ * shown as missing) until a pixel wants it. Ratios within `epsilon` of 0 or
+ * 1 are stored as 0 or 1. Set once, before the first frame, with
+ * light_ratio on; not together with moving objects (vcache.rs:1410). */
+int cnq_vcache_set_moving_lights(cnq_vcache* cache, const uint32_t* lights, uint32_t n, float epsilon);
+/* Moving light `index` is `light` (16 floats, the same kind: rect or
+ * sphere) from the next frame (vcache.rs:2375 `set_light_pose`). */
+int cnq_vcache_move_light(cnq_vcache* cache, uint32_t index, const float* light);
+/* The last frame's ledger counters (WGSL mover_stats 8-14 and 21-23):
+ * [pages re-baked, texels re-baked, pages set aside, pages checked (read
+ * in the frame before), pages re-baking mixed texels, pages held whole,
+ * jobs asked, pages held on their lattice, lattices that may have grown,
+ * unbounded texels of held pages]. */
+int cnq_vcache_ledger_stats(cnq_vcache* cache, uint64_t out[10]);
+/* Diagnostics (either pointer may be NULL): the ledger records (pages x
+ * moving lights x 48 words) and each record's certified bound (pages x
+ * moving lights). */
+int cnq_vcache_ledger_state(cnq_vcache* cache, uint32_t* records, float* bounds);
+/* `verify_ledger` (instanced.rs:3741-3850) for moving light `light`: every
+ * page shown baked afresh for that light (its records untouched) and
+ * compared with the cache. out: [pages, pages whose largest change passes
+ * their certified bound + 1e-5, pages past epsilon, pages changed at all,
+ * the largest change, the largest bound]. */
+int cnq_vcache_verify_ledger(cnq_vcache* cache, uint32_t light, void* stream, float out[6]);
 /* `verify_cache` (instanced.rs:3721-3729): every page shown (resident, not
  * marked stale or invalid) baked afresh into a copy of the pool and compared
  * with the pool by f32 bit pattern. out: [values that differ, values

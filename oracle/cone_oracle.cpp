@@ -1911,3 +1911,293 @@ int oracle_primary(const float* tris, const uint32_t* ranges, uint32_t n_objects
 }
 
 }  // extern "C"
+
+// ---- Moving lights in the view cache: the guard cone and the lattice of a
+// moving rect light (WGSL :4457-4530 `guard_cone`, :4532 `below_horizon`,
+// :4689-4905 `rect_corner`, `lattice_frame`, `lattice_kappa`,
+// `lattice_cone`, `lattice_areas`, `lattice_share`, `lattice_ratio`). Walks
+// of these cones take no node cap: the WGSL sets `cone_cap` back to
+// unlimited after each light cone (:1305).
+
+namespace orc {
+
+constexpr float GUARD = 3.0f;           // WGSL :4482
+constexpr float LATTICE_MARGIN = 0.03f; // WGSL :4704
+
+static V3 rect_corner(const Light& r, uint32_t k) {
+  return xyz(r.corner) + xyz(r.edge_u) * (float)(k & 1u) + xyz(r.edge_v) * (float)((k >> 1u) & 1u);
+}
+
+struct GuardCone {
+  Cone c;
+  float alpha;  // the light's angular radius around its centre, seen from the apex
+};
+
+// WGSL :4499-4529
+static GuardCone guard_cone(V3 apex, const Light& r) {
+  GuardCone out;
+  out.c = zero_cone();
+  V3 to = light_center(r) - apex;
+  float d = fmax_(length(to), 1e-6f);
+  V3 forward = to / d;
+  float alpha = 0.0f;
+  if (is_sphere(r)) {
+    alpha = asin_c(fmin_(r.edge_u[0] / d, 1.0f));
+  } else {
+    for (uint32_t k = 0; k < 4; k++)
+      alpha = fmax_(alpha, acos_c(clampf(dot(forward, normalize(rect_corner(r, k) - apex)), -1.0f, 1.0f)));
+  }
+  float wide = fmin_(GUARD * alpha, 1.4f);
+  float t = tan_c(wide);
+  V3 helper = std::fabs(forward.x) > 0.9f ? V3{0.0f, 1.0f, 0.0f} : V3{1.0f, 0.0f, 0.0f};
+  V3 right = normalize(cross(helper, forward));
+  out.c.apex = apex;
+  out.c.right = right;
+  out.c.up = cross(forward, right);
+  out.c.forward = forward;
+  out.c.window[0] = -t;
+  out.c.window[1] = -t;
+  out.c.window[2] = t;
+  out.c.window[3] = t;
+  out.c.far_normal = forward;
+  out.c.far_offset = dot(forward, apex) + SKY_FAR;
+  out.c = bound_cone(out.c, forward);
+  out.alpha = alpha;
+  return out;
+}
+
+// WGSL :4532-4539
+static float below_horizon(V3 apex, V3 n, const Light& r) {
+  V3 to = light_center(r) - apex;
+  float d = fmax_(length(to), 1e-6f);
+  float below = -dot(n, to);
+  if (is_sphere(r)) below -= r.edge_u[0];
+  return asin_c(clampf(below / d, -1.0f, 1.0f));
+}
+
+struct LatticeFrame {
+  V3 forward, right, up;
+  V2 beta;  // a bin's size in the window's plane, per axis
+  float alpha;
+  bool ok;
+};
+
+// WGSL :4727-4760
+static LatticeFrame lattice_frame(V3 apex, const Light& then) {
+  LatticeFrame f;
+  std::memset(&f, 0, sizeof(f));
+  f.ok = false;
+  V3 to = light_center(then) - apex;
+  float d = length(to);
+  if (d <= 1e-6f) return f;
+  V3 forward = to / d;
+  V3 eu = xyz(then.edge_u), ev = xyz(then.edge_v);
+  V3 right = eu - forward * dot(forward, eu);
+  if (length(right) <= 1e-6f * length(eu)) right = ev - forward * dot(forward, ev);
+  if (length(right) <= 0.0f) return f;
+  right = normalize(right);
+  V3 up = cross(forward, right);
+  V2 lo{3.4e38f, 3.4e38f}, hi{-3.4e38f, -3.4e38f};
+  float alpha = 0.0f;
+  for (uint32_t k = 0; k < 4; k++) {
+    V3 p = rect_corner(then, k) - apex;
+    float z = dot(forward, p);
+    if (z <= 1e-6f * d) return f;
+    V2 uv{dot(right, p) / z, dot(up, p) / z};
+    lo = V2{fmin_(lo.x, uv.x), fmin_(lo.y, uv.y)};
+    hi = V2{fmax_(hi.x, uv.x), fmax_(hi.y, uv.y)};
+    alpha = fmax_(alpha, atan2_c(length(cross(forward, p)), z));
+  }
+  f.forward = forward;
+  f.right = right;
+  f.up = up;
+  const float share = (1.0f + LATTICE_MARGIN) / 7.0f;
+  f.beta = V2{fmax_(hi.x - lo.x, 1e-12f) * share, fmax_(hi.y - lo.y, 1e-12f) * share};
+  f.alpha = alpha * 1.0001f;
+  f.ok = true;
+  return f;
+}
+
+// WGSL :4766-4773; (1 + t^2)^1.5 as q * sqrt(q).
+static float lattice_kappa(const LatticeFrame& f, V3 n) {
+  if (!f.ok) return 3.0e38f;
+  float reach = fmin_(GUARD * f.alpha, 1.4f);
+  float theta = atan2_c(length(cross(n, f.forward)), dot(n, f.forward));
+  if (theta + reach >= 1.5706f) return 3.0e38f;
+  float t = tan_c(reach);
+  float q = 1.0f + t * t;
+  return cos_c(fmax_(theta - reach, 0.0f)) / cos_c(theta + reach) * (q * std::sqrt(q)) * 1.0001f;
+}
+
+struct Lattice {
+  Cone c;
+  Poly2 q;  // the light's footprint in bins, corners in order around it
+  bool ok;
+};
+
+// WGSL :4786-4826
+static Lattice lattice_cone(V3 apex, const Light& now, const Light& then) {
+  Lattice out;
+  std::memset(&out, 0, sizeof(out));
+  out.ok = false;
+  LatticeFrame f = lattice_frame(apex, then);
+  if (!f.ok) return out;
+  V3 n = normalize(cross(xyz(now.edge_u), xyz(now.edge_v)));
+  if (now.corner[3] > 0.5f && dot(n, apex - xyz(now.corner)) <= 0.0f) return out;
+  V2 pts[4];
+  V2 lo{3.4e38f, 3.4e38f}, hi{-3.4e38f, -3.4e38f};
+  for (uint32_t k = 0; k < 4; k++) {
+    V3 p = rect_corner(now, k) - apex;
+    float z = dot(f.forward, p);
+    if (z <= 1e-6f) return out;
+    pts[k] = V2{dot(f.right, p) / z, dot(f.up, p) / z};
+    lo = V2{fmin_(lo.x, pts[k].x), fmin_(lo.y, pts[k].y)};
+    hi = V2{fmax_(hi.x, pts[k].x), fmax_(hi.y, pts[k].y)};
+  }
+  V2 origin{std::floor(lo.x / f.beta.x) * f.beta.x, std::floor(lo.y / f.beta.y) * f.beta.y};
+  V2 end{origin.x + 8.0f * f.beta.x, origin.y + 8.0f * f.beta.y};
+  if (hi.x > end.x || hi.y > end.y) return out;
+  V3 far_n = n;
+  if (dot(far_n, xyz(now.corner) - apex) < 0.0f) far_n = -far_n;
+  float dist = dot(far_n, xyz(now.corner) - apex);
+  if (dist <= 1e-6f) return out;
+  out.c.apex = apex;
+  out.c.right = f.right;
+  out.c.up = f.up;
+  out.c.forward = f.forward;
+  out.c.window[0] = origin.x;
+  out.c.window[1] = origin.y;
+  out.c.window[2] = end.x;
+  out.c.window[3] = end.y;
+  out.c.far_normal = far_n;
+  out.c.far_offset = dot(far_n, apex) + dist * ONE_MINUS_1EM5;
+  out.c = bound_cone(out.c, normalize(light_center(now) - apex));
+  const uint32_t order[4] = {0, 1, 3, 2};
+  out.q.n = 4;
+  for (uint32_t i = 0; i < 4; i++)
+    out.q.v[i] = V2{(pts[order[i]].x - origin.x) / f.beta.x, (pts[order[i]].y - origin.y) / f.beta.y};
+  out.ok = true;
+  return out;
+}
+
+// WGSL :4830-4848: the footprint's area in each bin (in bins).
+static void lattice_areas(const Poly2& q, float m[64]) {
+  Acc acc;
+  for (uint32_t b = 0; b < 68; b++) acc.a[b] = 0.0f;
+  acc.span_lo = 68;
+  acc.span_hi = 0;
+  const float g = (float)GRID;
+  for (uint32_t i = 0; i < q.n; i++) draw_edge(acc, clamp2(q.v[i], g), clamp2(q.v[(i + 1) % q.n], g), 1.0f);
+  float running = 0.0f;
+  for (uint32_t b = 0; b < BINS; b++) {
+    running += acc.a[b];
+    m[b] = fmin_(std::fabs(running), 1.0f);
+  }
+}
+
+// WGSL :4853-4869
+static float lattice_share(const Cone& c, V3 normal, const float m[64], const float cov[64]) {
+  const float g = (float)GRID;
+  float total = 0.0f, open = 0.0f;
+  for (uint32_t b = 0; b < BINS; b++) {
+    float area = m[b];
+    if (area <= 0.0f) continue;
+    V3 d = direction(c, ((float)(b % GRID) + 0.5f) / g, ((float)(b / GRID) + 0.5f) / g);
+    float l2 = dot(d, d);
+    float w = area * fmax_(dot(normal, d), 0.0f) / (l2 * l2);
+    total += w;
+    open += w * (1.0f - cov[b]);
+  }
+  if (total <= 0.0f) return 1.0f;
+  return open / total;
+}
+
+}  // namespace orc
+
+extern "C" {
+
+// The guard cone's walk from `apex` toward `light` (WGSL :4576-4583): 1 if
+// any bin is touched; its angular radius to `alpha`.
+int oracle_guard_trace(const float* tris, const uint32_t* ranges, uint32_t n_objects, const float* blas,
+                       const uint32_t* blas_items, const float* blas_cov, const float* nodes, const uint32_t* items,
+                       const float* xforms, const uint32_t* opts, const float* apex, const float* light,
+                       int trace, float* alpha) {
+  Scene s = make_scene(tris, ranges, n_objects, blas, blas_items, blas_cov, nodes, items, xforms);
+  Options o = make_options(opts);
+  GuardCone g = guard_cone(xyz(apex), *reinterpret_cast<const Light*>(light));
+  *alpha = g.alpha;
+  if (!trace) return 0;
+  Query qs;
+  qs.s = &s;
+  qs.o = &o;
+  qs.prefilter_bins = o.prefilter;
+  qs.cone_cap = NONE;
+  float cov[64];
+  for (int b = 0; b < 64; b++) cov[b] = 0.0f;
+  uint32_t full = 0;
+  scene_coverage(g.c, qs, cov, full);
+  for (int b = 0; b < 64; b++)
+    if (cov[b] > 0.0f) return 1;
+  return 0;
+}
+
+float oracle_asin(float x) { return asin_c(x); }
+float oracle_cos(float x) { return cos_c(x); }
+float oracle_tan(float x) { return tan_c(x); }
+
+float oracle_below_horizon(const float* apex, const float* normal, const float* light) {
+  return below_horizon(xyz(apex), xyz(normal), *reinterpret_cast<const Light*>(light));
+}
+
+// Lattice frame of pose `then` seen from `apex`: out = forward3 right3 up3
+// beta2 alpha ok (13 floats); returns kappa for `normal`.
+float oracle_lattice_frame(const float* apex, const float* then, const float* normal, float* out) {
+  LatticeFrame f = lattice_frame(xyz(apex), *reinterpret_cast<const Light*>(then));
+  if (out) {
+    const float v[13] = {f.forward.x, f.forward.y, f.forward.z, f.right.x, f.right.y, f.right.z, f.up.x, f.up.y,
+                         f.up.z, f.beta.x, f.beta.y, f.alpha, f.ok ? 1.0f : 0.0f};
+    std::memcpy(out, v, sizeof(v));
+  }
+  return lattice_kappa(f, xyz(normal));
+}
+
+// WGSL :4875-4899 `lattice_ratio`: the ratio (-1 where the lattice cannot
+// hold the footprint or `on` is false) and flags[0] = last_empty,
+// flags[1] = last_full. With `areas`, the footprint's 64 bin areas.
+float oracle_lattice_ratio(const float* tris, const uint32_t* ranges, uint32_t n_objects, const float* blas,
+                           const uint32_t* blas_items, const float* blas_cov, const float* nodes,
+                           const uint32_t* items, const float* xforms, const uint32_t* opts, int on,
+                           const float* apex, const float* normal, const float* now, const float* then,
+                           int* flags, float* areas) {
+  Scene s = make_scene(tris, ranges, n_objects, blas, blas_items, blas_cov, nodes, items, xforms);
+  Options o = make_options(opts);
+  Lattice lc = lattice_cone(xyz(apex), *reinterpret_cast<const Light*>(now), *reinterpret_cast<const Light*>(then));
+  bool traced = on && lc.ok;
+  float m[64], cov[64];
+  lattice_areas(lc.q, m);
+  if (areas) std::memcpy(areas, m, sizeof(m));
+  uint32_t full = 0;
+  for (int b = 0; b < 64; b++) {
+    cov[b] = traced && m[b] <= 0.0f ? 1.0f : 0.0f;
+    if (cov[b] >= 1.0f) full++;
+  }
+  if (traced) {
+    Query qs;
+    qs.s = &s;
+    qs.o = &o;
+    qs.prefilter_bins = o.prefilter;
+    qs.cone_cap = NONE;
+    scene_coverage(lc.c, qs, cov, full);
+  }
+  bool last_empty = true, last_full = true;
+  for (int b = 0; b < 64; b++) {
+    if (cov[b] > 0.0f && m[b] > 0.0f) last_empty = false;
+    if (cov[b] < 1.0f) last_full = false;
+  }
+  flags[0] = last_empty ? 1 : 0;
+  flags[1] = last_full ? 1 : 0;
+  if (!traced) return -1.0f;
+  return lattice_share(lc.c, xyz(normal), m, cov);
+}
+
+}  // extern "C"

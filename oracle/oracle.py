@@ -82,6 +82,19 @@ def lib() -> C.CDLL:
             "oracle_vc_page_bounds": (None, [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, _P]),
             "oracle_vc_shade": (None, [_P, _P, C.c_uint32, C.c_uint32, _P, _P, _P, _P, _P, C.c_uint32, _P]),
             "oracle_vc_plan": (None, [_P, _P, C.c_uint32, C.c_uint32, _P]),
+            "oracle_vc_set_moving": (C.c_int, [_P, _P, C.c_uint32, C.c_float]),
+            "oracle_vc_move_light": (C.c_int, [_P, C.c_uint32, _P]),
+            "oracle_vc_ledger_stats": (None, [_P, _P]),
+            "oracle_vc_ledger_state": (None, [_P, _P, _P]),
+            "oracle_vc_verify_ledger": (None, [_P, C.c_uint32, _P]),
+            "oracle_guard_trace": (C.c_int, [_P] * 9 + [_P, _P, _P, C.c_int, _P]),
+            "oracle_below_horizon": (C.c_float, [_P, _P, _P]),
+            "oracle_lattice_frame": (C.c_float, [_P, _P, _P, _P]),
+            "oracle_lattice_ratio": (C.c_float, [_P] * 9 + [_P, C.c_int, _P, _P, _P, _P, _P, _P]),
+            "oracle_asin": (C.c_float, [C.c_float]),
+            "oracle_cos": (C.c_float, [C.c_float]),
+            "oracle_tan": (C.c_float, [C.c_float]),
+            "oracle_vc_light_shift": (C.c_float, [_P, _P, _P]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -438,6 +451,52 @@ def query_skip(flat, opts, point, normal, light, skip_obj=NONE, skip_tri=NONE):
     return float(vis[0]), int(masks[0]), int(masks[1])
 
 
+def light_shift(a, b, box) -> float:
+    """WGSL :4666 `light_shift`: the largest angle a point of light pose a
+    moves to reach pose b, seen from anywhere in box (lo, hi)."""
+    a = np.ascontiguousarray(a, np.float32).reshape(16)
+    b = np.ascontiguousarray(b, np.float32).reshape(16)
+    bx = np.ascontiguousarray(box, np.float32).reshape(6)
+    return float(lib().oracle_vc_light_shift(_p(a), _p(b), _p(bx)))
+
+
+def guard_alpha(apex, light) -> float:
+    """The guard cone's angular radius of `light` seen from `apex` (WGSL :4499)."""
+    p = np.ascontiguousarray(apex, np.float32).reshape(3)
+    lt = np.ascontiguousarray(light, np.float32).reshape(16)
+    alpha = C.c_float()
+    z = np.zeros(1, np.float32)
+    zu = np.zeros(8, np.uint32)
+    lib().oracle_guard_trace(_p(z), _p(zu), 0, _p(z), _p(zu), _p(z), _p(z), _p(zu), _p(z), _p(zu), _p(p), _p(lt), 0,
+                             C.byref(alpha))
+    return float(alpha.value)
+
+
+def lattice(apex, normal, now, then, flat=None, opts=None, on=False):
+    """WGSL :4727-4899 for one apex: (frame (13 floats: forward, right, up,
+    beta, alpha, ok), kappa, ratio, (last_empty, last_full), bin areas (64)).
+    With `flat` and `on` the lattice cone walks the scene."""
+    p = np.ascontiguousarray(apex, np.float32).reshape(3)
+    n = np.ascontiguousarray(normal, np.float32).reshape(3)
+    a = np.ascontiguousarray(now, np.float32).reshape(16)
+    t = np.ascontiguousarray(then, np.float32).reshape(16)
+    frame = np.zeros(13, np.float32)
+    kappa = float(lib().oracle_lattice_frame(_p(p), _p(t), _p(n), _p(frame)))
+    flags = np.zeros(2, np.int32)
+    areas = np.zeros(64, np.float32)
+    if flat is None:
+        z = np.zeros(1, np.float32)
+        zu = np.zeros(8, np.uint32)
+        args = (_p(z), _p(zu), 0, _p(z), _p(zu), _p(z), _p(z), _p(zu), _p(z))
+        o = np.zeros(8, np.uint32) if opts is None else np.ascontiguousarray(opts, np.uint32)
+    else:
+        args = scene_args(flat)
+        o = np.ascontiguousarray(opts, np.uint32)
+    ratio = float(lib().oracle_lattice_ratio(*args, _p(o), int(bool(on) and flat is not None), _p(p), _p(n), _p(a),
+                                             _p(t), _p(flags), _p(areas)))
+    return frame, kappa, ratio, (bool(flags[0]), bool(flags[1])), areas
+
+
 class OracleViewCache:
     """The view cache restated on the CPU, driven as the product's ViewCache
     (request / evict / set_moved / frame / read / state)."""
@@ -535,6 +594,41 @@ class OracleViewCache:
     def set_lights(self, lights, first: int = 0):
         rows = np.ascontiguousarray(lights, np.float32).reshape(-1, 16)
         lib().oracle_vc_set_lights(self.handle, _p(rows), int(first), len(rows))
+
+    LEDGER_STATS = ("pages", "texels", "set_aside", "checked", "mixed", "held_whole", "jobs", "held_lattice",
+                    "grew", "unbounded")
+
+    def set_moving(self, lights, epsilon: float = 0.01) -> None:
+        """Lights that move through the error ledger (before the first frame)."""
+        idx = np.ascontiguousarray(lights, np.uint32).reshape(-1)
+        self.moving = [int(i) for i in idx]
+        if lib().oracle_vc_set_moving(self.handle, _p(idx), len(idx), float(epsilon)) != 0:
+            raise ValueError("moving lights: before the first frame, distinct, within the light table")
+
+    def move_light(self, index: int, light) -> None:
+        row = np.ascontiguousarray(light, np.float32).reshape(16)
+        if lib().oracle_vc_move_light(self.handle, int(index), _p(row)) != 0:
+            raise ValueError("not a moving light")
+        self.lights[int(index)] = row
+
+    def ledger_stats(self) -> dict:
+        out = np.zeros(10, np.uint64)
+        lib().oracle_vc_ledger_stats(self.handle, _p(out))
+        return {n: int(v) for n, v in zip(self.LEDGER_STATS, out)}
+
+    def ledger_state(self, pages: int):
+        """(records (pages, moving, 48) u32, bounds (pages, moving) f32)."""
+        m = len(self.moving)
+        rec = np.zeros((pages, m, 48), np.uint32)
+        bound = np.zeros((pages, m), np.float32)
+        lib().oracle_vc_ledger_state(self.handle, _p(rec), _p(bound))
+        return rec, bound
+
+    def verify_ledger(self, light: int) -> dict:
+        out = np.zeros(6, np.float32)
+        lib().oracle_vc_verify_ledger(self.handle, int(light), _p(out))
+        return dict(pages=int(out[0]), violations=int(out[1]), over_epsilon=int(out[2]), changed=int(out[3]),
+                    largest_change=float(out[4]), largest_bound=float(out[5]))
 
     def state(self) -> dict:
         sizes = np.zeros(5, np.uint32)

@@ -96,4 +96,62 @@ inline float atan2_c(float y, float x) {
 // acos(y) = atan2(sqrt((1 - y)(1 + y)), y).
 inline float acos_c(float y) { return atan2_c(std::sqrt((1.0f - y) * (1.0f + y)), y); }
 
+// asin(x) = atan2(x, sqrt((1 - x)(1 + x))) for |x| <= 1.
+inline float asin_c(float x) { return atan2_c(x, std::sqrt((1.0f - x) * (1.0f + x))); }
+
+// Cosine and tangent: S. L. Moshier, Cephes Math Library (sinf.c `cosf`,
+// tanf.c): the argument reduced by multiples of pi/4 (pi/4 in three parts),
+// then minimax polynomials on [-pi/4, pi/4].
+constexpr float kFourOverPi = 1.27323954473516f;
+constexpr float kDP1 = 0.78515625f;
+constexpr float kDP2 = 2.4187564849853515625e-4f;
+constexpr float kDP3 = 3.77489497744594108e-8f;
+
+inline float cos_c(float xin) {
+  float x = xin < 0.0f ? -xin : xin;
+  int j = (int)(kFourOverPi * x);
+  float y = (float)j;
+  if (j & 1) {
+    j += 1;
+    y += 1.0f;
+  }
+  j &= 7;
+  int sign = 1;
+  if (j > 3) {
+    j -= 4;
+    sign = -sign;
+  }
+  if (j > 1) sign = -sign;
+  x = ((x - y * kDP1) - y * kDP2) - y * kDP3;
+  const float z = x * x;
+  if (j == 1 || j == 2)
+    y = ((-1.9515295891e-4f * z + 8.3321608736e-3f) * z - 1.6666654611e-1f) * z * x + x;
+  else
+    y = ((2.443315711809948e-5f * z - 1.388731625493765e-3f) * z + 4.166664568298827e-2f) * z * z - 0.5f * z + 1.0f;
+  return sign < 0 ? -y : y;
+}
+
+inline float tan_c(float xin) {
+  const float x = xin < 0.0f ? -xin : xin;
+  int j = (int)(kFourOverPi * x);
+  float y = (float)j;
+  if (j & 1) {
+    j += 1;
+    y += 1.0f;
+  }
+  const float z = ((x - y * kDP1) - y * kDP2) - y * kDP3;
+  const float zz = z * z;
+  if (x > 1.0e-4f)
+    y = (((((9.38540185543e-3f * zz + 3.11992232697e-3f) * zz + 2.44301354525e-2f) * zz + 5.34112807005e-2f) * zz +
+          1.33387994085e-1f) *
+             zz +
+         3.33331568548e-1f) *
+            zz * z +
+        z;
+  else
+    y = z;
+  if (j & 2) y = -1.0f / y;
+  return xin < 0.0f ? -y : y;
+}
+
 }  // namespace orc

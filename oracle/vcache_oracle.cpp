@@ -16,9 +16,12 @@
 // index order; where the GPU's atomics order appends differently, only the
 // order of list entries differs, never a texel's value.
 //
+// Moving lights go through the error ledger (WGSL :4457-5164: `ledger_texel`
+// :4547, `ledger_reset` :4643, `light_shift` :4666, the lattice's bounds
+// :4907-5000, `vc_ledger` :5007; the guard and lattice cones are restated
+// in cone_oracle.cpp), as in vc_bake :3448-3530 and vcache.rs:2255-2380.
 // Left out as the product leaves them out: sky and bounce channels (the sky
-// channel is zero), the moving-light ledger and its lattice (a light edit
-// re-bakes the light's channel of every resident page, exactly), deforming
+// channel is zero) and their ledgers, deforming
 // objects' catch-up slices (every move is rigid, so the reference's `lag_only` is
 // false wherever it is asked and no page is deferred), area texels, wide
 // windows' ray pass, absent lighting levels.
@@ -41,6 +44,17 @@ extern "C" void oracle_query_skip(const float* tris, const uint32_t* ranges, uin
 extern "C" float oracle_unshadowed_rect(const float* point, const float* normal, const float* light);
 extern "C" int oracle_faces_light(const float* light, const float* point, const float* normal);
 extern "C" void oracle_light_center(const float* light, float* out);
+extern "C" int oracle_guard_trace(const float* tris, const uint32_t* ranges, uint32_t n_objects, const float* blas,
+                                  const uint32_t* blas_items, const float* blas_cov, const float* nodes,
+                                  const uint32_t* items, const float* xforms, const uint32_t* opts, const float* apex,
+                                  const float* light, int trace, float* alpha);
+extern "C" float oracle_below_horizon(const float* apex, const float* normal, const float* light);
+extern "C" float oracle_lattice_frame(const float* apex, const float* then, const float* normal, float* out);
+extern "C" float oracle_lattice_ratio(const float* tris, const uint32_t* ranges, uint32_t n_objects,
+                                      const float* blas, const uint32_t* blas_items, const float* blas_cov,
+                                      const float* nodes, const uint32_t* items, const float* xforms,
+                                      const uint32_t* opts, int on, const float* apex, const float* normal,
+                                      const float* now, const float* then, int* flags, float* areas);
 
 namespace vco {
 
@@ -159,7 +173,19 @@ struct State {
   std::vector<float> pool;
   std::vector<uint32_t> bake_list, bake_jobs, candidates, invalid_list;
   std::vector<float> bake_bounds;
-  uint32_t stats[8];
+  // WGSL mover_stats: 0-7 as the stale passes (7 unused here), 8-14 and
+  // 21-23 the moving lights' ledger.
+  uint32_t stats[24];
+  // moving lights (WGSL :4457-5164): per light its index among the moving
+  // ones (NONE: static), the ledger records (slot x moving x 48 words), the
+  // poses ring (POSE_RING frames x lights x 16 floats), each page's certified
+  // bound, ε, and whether a moving light moved since the last frame.
+  std::vector<uint32_t> moving_index, ledger;
+  std::vector<float> poses, ledger_bound;
+  uint32_t n_moving = 0;
+  float epsilon = 0.01f;
+  bool lights_moved = false;
+  uint32_t last_ledger[10] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 static float object_scale(const State& s, uint32_t m) {
@@ -558,6 +584,338 @@ static float f_of(uint32_t u) {
   return f;
 }
 
+// ---- moving lights through the error ledger (WGSL :4457-5164)
+
+constexpr uint32_t LEDGER_WORDS = 48;        // WGSL :4481, vcache.rs:162
+constexpr uint32_t MAX_OFFSET = 1023;        // WGSL :4483
+constexpr uint32_t POSE_RING = 1024;         // vcache.rs:161
+constexpr float GUARD = 3.0f;                // WGSL :4482
+constexpr float LATTICE_MARGIN = 0.03f;      // WGSL :4704
+constexpr float LATTICE_KAPPA_MAX = 8.0f;    // WGSL :4708
+
+static uint32_t u_of(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  return u;
+}
+
+static bool sphere_light(const float* r) { return r[3] > 1.5f; }
+
+static V3 light_center_v(const float* r) {
+  float cc[3];
+  oracle_light_center(r, cc);
+  return V3{cc[0], cc[1], cc[2]};
+}
+
+static V3 rect_corner(const float* r, uint32_t k) {
+  return ld(r) + ld(r + 4) * (float)(k & 1u) + ld(r + 8) * (float)((k >> 1u) & 1u);
+}
+
+static const float* pose_at(const State& s, uint32_t frame, uint32_t l) {
+  return &s.poses[((size_t)(frame % POSE_RING) * s.n_lights + l) * 16];
+}
+
+static uint32_t ledger_base(const State& s, uint32_t slot, uint32_t j) {
+  return (slot * s.n_moving + j) * LEDGER_WORDS;
+}
+
+// WGSL :4666-4687: the largest angle a point of light pose `a` moves to
+// reach pose `b`, seen from anywhere in box `page` (4: unbounded).
+static float light_shift(const float* a, const float* b, const Box3& page) {
+  float moved = 0.0f;
+  if (sphere_light(a)) {
+    moved = orc::length(ld(b) - ld(a)) + std::fabs(b[4] - a[4]);
+  } else {
+    for (uint32_t k = 0; k < 4; k++) moved = orc::fmax_(moved, orc::length(rect_corner(b, k) - rect_corner(a, k)));
+  }
+  if (moved <= 0.0f) return 0.0f;
+  Box3 la = light_box(a), lb = light_box(b);
+  Box3 span{orc::vmin(la.lo, lb.lo), orc::vmax(la.hi, lb.hi)};
+  V3 gap = orc::vmax(V3{0.0f, 0.0f, 0.0f}, orc::vmax(page.lo - span.hi, span.lo - page.hi));
+  float d = orc::length(gap);
+  if (moved >= 2.0f * d) return 4.0f;
+  return 2.0f * orc::asin_c(moved / (2.0f * d)) * 1.0001f;
+}
+
+struct LatticeReach {
+  float near, far, cos_min, half_diagonal;
+};
+
+// WGSL :4907-4942
+static LatticeReach lattice_reach(const float* a, const float* b, const Box3& page) {
+  LatticeReach out;
+  Box3 la = light_box(a), lb = light_box(b);
+  Box3 span{orc::vmin(la.lo, lb.lo), orc::vmax(la.hi, lb.hi)};
+  V3 gap = orc::vmax(V3{0.0f, 0.0f, 0.0f}, orc::vmax(page.lo - span.hi, span.lo - page.hi));
+  out.near = orc::length(gap);
+  V3 au = ld(a + 4), av = ld(a + 8), bu = ld(b + 4), bv = ld(b + 8);
+  out.half_diagonal = 0.5f * orc::fmax_(orc::length(au + av), orc::length(bu + bv));
+  out.half_diagonal = orc::fmax_(out.half_diagonal, 0.5f * orc::fmax_(orc::length(au - av), orc::length(bu - bv)));
+  V3 pc = (page.lo + page.hi) * 0.5f;
+  float ph = 0.5f * orc::length(page.hi - page.lo);
+  V3 ca = light_center_v(a) - pc, cb = light_center_v(b) - pc;
+  out.far = orc::fmax_(orc::length(ca), orc::length(cb)) + ph + out.half_diagonal;
+  V3 na = orc::normalize(orc::cross(au, av)), nb = orc::normalize(orc::cross(bu, bv));
+  float cos_c = orc::fmin_(std::fabs(orc::dot(na, orc::normalize(ca))), std::fabs(orc::dot(nb, orc::normalize(cb))));
+  if (out.near > 0.0f)
+    out.cos_min = cos_c - orc::asin_c(orc::fmin_((ph + out.half_diagonal) / out.near, 1.0f));
+  else
+    out.cos_min = -1.0f;
+  return out;
+}
+
+// WGSL :4949-4971
+static float lattice_bound(const float* a, const float* b, const Box3& page, float kappa) {
+  float delta = light_shift(a, b, page);
+  if (delta <= 0.0f) return 0.0f;
+  if (delta >= 1.0f || kappa >= 1.0e30f) return 1.0f;
+  LatticeReach r = lattice_reach(a, b, page);
+  if (r.cos_min <= 0.0f || r.near <= 2.0f * r.half_diagonal) return 1.0f;
+  V3 au = ld(a + 4), av = ld(a + 8), bu = ld(b + 4), bv = ld(b + 8);
+  float area = orc::fmin_(orc::length(orc::cross(au, av)), orc::length(orc::cross(bu, bv)));
+  float omega = area * r.cos_min / (r.far * r.far);
+  float lu = orc::fmax_(orc::length(au), orc::length(bu));
+  float lv = orc::fmax_(orc::length(av), orc::length(bv));
+  float perimeter = 4.0f * (orc::asin_c(orc::fmin_(lu / (2.0f * r.near), 1.0f)) +
+                            orc::asin_c(orc::fmin_(lv / (2.0f * r.near), 1.0f)));
+  float rho = orc::fmin_(GUARD * orc::asin_c(orc::fmin_(r.half_diagonal / r.near, 1.0f)) * 1.0001f, 1.4f);
+  float t = orc::tan_c(rho);
+  float sec2 = 1.0f + t * t;
+  float moved = sec2 * delta;
+  float sw = sec2 * perimeter * moved + 3.14159265f * moved * moved;
+  return orc::fmin_(1.5f * kappa * sw / omega * 1.0001f, 1.0f);
+}
+
+// WGSL :4981-5000
+static bool lattice_grew(const float* a, const float* b, const Box3& page) {
+  LatticeReach r = lattice_reach(a, b, page);
+  if (r.cos_min <= 0.0f || r.near <= 2.0f * r.half_diagonal) return true;
+  V3 ca = light_center_v(a), cb = light_center_v(b);
+  float t = orc::length(cb - ca);
+  float rest = 0.0f;
+  for (uint32_t k = 0; k < 4; k++)
+    rest = orc::fmax_(rest, orc::length((rect_corner(b, k) - cb) - (rect_corner(a, k) - ca)));
+  float diameter = 2.0f * orc::asin_c(orc::fmin_(r.half_diagonal / r.near, 1.0f));
+  float rho = orc::fmin_(GUARD * 0.5f * diameter * 1.0001f, 1.4f);
+  float tr = orc::tan_c(rho);
+  float sec2 = 1.0f + tr * tr;
+  float delta = light_shift(a, b, page);
+  float spread = sec2 * (3.0f * (t / r.near) * (2.0f * r.half_diagonal / r.near) + rest / r.near) +
+                 3.0f * tr * sec2 * sec2 * diameter * delta;
+  V3 au = ld(a + 4), av = ld(a + 8), bu = ld(b + 4), bv = ld(b + 8);
+  float area = orc::fmin_(orc::length(orc::cross(au, av)), orc::length(orc::cross(bu, bv)));
+  float width = area * r.cos_min / (r.far * r.far) / (sec2 * diameter);
+  return 2.0f * spread * 1.0001f > LATTICE_MARGIN * width;
+}
+
+// WGSL :4643-4652
+static void ledger_reset(State& s, uint32_t slot, uint32_t j) {
+  uint32_t base = ledger_base(s, slot, j);
+  s.ledger[base] = s.frame;
+  s.ledger[base + 1] = 0x7f7fffffu;
+  s.ledger[base + 2] = 0;
+  s.ledger[base + 3] = s.frame;
+  s.ledger[base + 4] = 0;
+  for (uint32_t w = 5; w < LEDGER_WORDS; w++) s.ledger[base + w] = 0;
+  s.ledger_bound[(size_t)slot * s.n_moving + j] = 0.0f;
+}
+
+static void guard_of(const State& s, V3 apex, const float* r, bool trace, bool& any, float& alpha) {
+  float p[3] = {apex.x, apex.y, apex.z};
+  any = oracle_guard_trace(s.tris, s.ranges, s.n_objects, s.blas, s.blas_items, s.blas_cov, s.nodes, s.items,
+                           s.xforms, s.opts, p, r, trace ? 1 : 0, &alpha) != 0;
+}
+
+// WGSL :4547-4638: texel k of slot `slot`'s record for moving light j
+// (light index `l`, pose `r` now, `then` at the page's version).
+static void ledger_texel(State& s, bool on, uint32_t slot, uint32_t j, uint32_t l, uint32_t k, V3 apex, V3 n,
+                         const float* r, float open, const float* then, bool lattice, bool overflow) {
+  uint32_t base = ledger_base(s, slot, j);
+  uint32_t version = s.ledger[base];
+  uint32_t offset = std::min(s.frame - version, MAX_OFFSET);
+  uint32_t state = 0;
+  float guard = 3.4e38f;
+  float reach = 0.0f, margin = 3.4e38f, bin = 0.0f, kappa = 3.0e38f;
+  float ap[3] = {apex.x, apex.y, apex.z}, np[3] = {n.x, n.y, n.z};
+  if (lattice && on && open > 0.0f) {
+    float f[13];
+    float kap = oracle_lattice_frame(ap, then, np, f);
+    if (f[12] != 0.0f) {
+      reach = orc::fmin_(GUARD * f[11], 1.4f);
+      bin = std::sqrt(f[9] * f[9] + f[10] * f[10]);
+      margin = f[11] + bin;
+      kappa = kap;
+    }
+  }
+  if (on && open > 0.0f && offset != 0) {
+    state = 3;
+    if (lattice) guard = reach - margin;
+  } else if (on && open > 0.0f) {
+    bool any;
+    float alpha;
+    guard_of(s, apex, r, true, any, alpha);
+    state = any ? 3 : 2;
+    if (-oracle_below_horizon(ap, np, r) <= orc::fmax_(2.0f * orc::fmin_(GUARD * alpha, 1.4f), 0.1f)) state = 3;
+    guard = orc::fmin_(GUARD * alpha, 1.4f) - alpha;
+    if (lattice) guard = orc::fmin_(orc::fmin_(GUARD * alpha, 1.4f), reach) - orc::fmax_(margin, alpha + bin);
+  } else if (on) {
+    float below = oracle_below_horizon(ap, np, r);
+    bool any;
+    float alpha;
+    guard_of(s, apex, r, false, any, alpha);
+    if (below >= 2.0f * alpha)
+      guard = below;
+    else
+      state = 3;
+  }
+  if ((state != 3 || lattice) && guard < 3.0e38f) {
+    float slack = guard;
+    if (offset != 0) {
+      Box3 point{apex, apex};
+      slack = guard - light_shift(pose_at(s, version, l), r, point);
+    }
+    s.ledger[base + 1] = std::min(s.ledger[base + 1], u_of(orc::fmax_(slack, 0.0f)));
+  }
+  uint32_t code = 0;
+  if (lattice && state == 3) {
+    if (on && open > 0.0f && kappa <= LATTICE_KAPPA_MAX)
+      s.ledger[base + 2] = std::max(s.ledger[base + 2], u_of(kappa));
+    else
+      code = 4;
+  }
+  uint32_t shift = 16 * (k % 2), word = base + 5 + k / 2;
+  s.ledger[word] &= ~(0xffffu << shift);
+  s.ledger[word] |= ((offset << 6) | code | state) << shift;
+  uint32_t flags = (state == 3 ? 1u : 0u) | (offset != 0 ? 2u : 0u) | (overflow ? 8u : 0u);
+  if (flags) s.ledger[base + 4] |= flags;
+}
+
+static float share_of(uint32_t slot, uint32_t frame, uint32_t salt) {
+  uint32_t k[3] = {slot, frame, salt};
+  return 0.5f + 0.5f * (float)(hash3(k) & 0xffffu) / 65535.0f;
+}
+
+// WGSL :5007-5164: one slot's check of every moving light's record.
+static void vc_ledger(State& s, uint32_t slot, bool lit) {
+  const uint32_t* info = &s.slot_info[12 * slot];
+  uint32_t set_aside = s.slot_stale[slot];
+  if (info[3] == 0 || set_aside == 1) return;
+  bool visible = s.slot_used[slot] + 1 >= s.frame;
+  if (set_aside == 2 && !visible) return;
+  if (set_aside == 0 && !lit) return;
+  if (visible) s.stats[11]++;
+  V3 lo{f_of(info[4]), f_of(info[5]), f_of(info[6])}, hi{f_of(info[8]), f_of(info[9]), f_of(info[10])};
+  Box3 page = world_box(s, info[0], lo, hi);
+  const uint32_t key[3] = {info[0], info[1], info[2]};
+  const uint32_t limit = std::min(append_jobs_cap(s), 0x7fffffffu);
+  uint32_t entry = NONE;
+  bool counted = false;
+  for (uint32_t l = 0; l < s.n_lights; l++) {
+    uint32_t j = s.moving_index[l];
+    if (j == NONE) continue;
+    uint32_t base = ledger_base(s, slot, j);
+    uint32_t version = s.ledger[base];
+    if (version == 0) continue;
+    const float* now = &s.lights[16 * l];
+    float slack = f_of(s.ledger[base + 1]);
+    uint32_t page_flags = s.ledger[base + 4];
+    bool mixed_page = (page_flags & 1u) != 0;
+    uint32_t age = s.frame - version;
+    bool whole = age + 1 >= std::min(POSE_RING, MAX_OFFSET);
+    bool lattice = !sphere_light(now);
+    if (lattice && !whole && (page_flags & 8u)) {
+      whole = true;
+      if (visible) s.stats[22]++;
+    }
+    uint32_t stale[3] = {0, 0, 0}, count = 0;
+    float bound = 0.0f;
+    bool held = false;
+    if (!whole) {
+      float d0 = light_shift(pose_at(s, version, l), now, page);
+      if (d0 > slack * share_of(slot, version, 0x5bd1e995u)) {
+        whole = true;
+      } else if (!mixed_page) {
+        if (visible) s.stats[13]++;
+        s.ledger_bound[(size_t)slot * s.n_moving + j] = 0.0f;
+        continue;
+      } else {
+        if (visible) s.stats[12]++;
+        if (lattice) {
+          uint32_t since = s.ledger[base + 3];
+          float kappa = f_of(s.ledger[base + 2]);
+          if (s.frame - since < POSE_RING) {
+            float b = lattice_bound(pose_at(s, since, l), now, page, kappa);
+            held = b <= s.epsilon * share_of(slot, since, 0x27d4eb2du);
+            if (held && lattice_grew(pose_at(s, version, l), now, page)) {
+              held = false;
+              if (visible) s.stats[22]++;
+            }
+            if (held) bound = b;
+          }
+        }
+        uint32_t want = held ? 7u : 3u;
+        for (uint32_t k = 0; k < TEXELS; k++) {
+          uint32_t half = (s.ledger[base + 5 + k / 2] >> (16 * (k % 2))) & 0xffffu;
+          if ((half & want) == want) {
+            stale[k / 32] |= 1u << (k % 32);
+            count++;
+          }
+        }
+        if (held && visible) {
+          s.stats[21]++;
+          s.stats[23] += count;
+        }
+        whole = count == TEXELS && !held;
+      }
+    }
+    if (whole) count = TEXELS;
+    s.ledger_bound[(size_t)slot * s.n_moving + j] = bound;
+    if (count == 0) continue;
+    if (!visible) {
+      if (s.slot_stale[slot] == 0) s.stats[10]++;
+      s.slot_stale[slot] = 2;
+      return;
+    }
+    if (entry == NONE) {
+      entry = append_page(s, slot, key, lo, hi);
+      if (entry == NONE) {
+        s.slot_stale[slot] = 2;
+        s.stats[10]++;
+        return;
+      }
+    }
+    uint32_t first = NONE;
+    uint32_t before = s.stats[14];
+    s.stats[14] += count;
+    if (before + count <= limit) first = append_jobs(s, count);
+    if (first == NONE) {
+      s.stats[10]++;
+      s.bake_bounds[8 * entry + 3] = 1.0f;
+      s.slot_stale[slot] = 2;
+      return;
+    }
+    if (whole) {
+      ledger_reset(s, slot, j);
+      for (uint32_t k = 0; k < TEXELS; k++)
+        s.bake_jobs[2 * (first + k)] = entry, s.bake_jobs[2 * (first + k) + 1] = k | (l << 8);
+    } else {
+      uint32_t at = 0;
+      for (uint32_t k = 0; k < TEXELS; k++)
+        if (stale[k / 32] & (1u << (k % 32))) {
+          s.bake_jobs[2 * (first + at)] = entry, s.bake_jobs[2 * (first + at) + 1] = k | (l << 8);
+          at++;
+        }
+      if (!held) s.ledger[base + 3] = s.frame;
+    }
+    s.stats[9] += count;
+    if (!counted) {
+      counted = true;
+      s.stats[8]++;
+    }
+  }
+  if (set_aside == 2 && entry == NONE) s.slot_stale[slot] = 0;
+}
+
 // WGSL :4233-4379 (rigid moves, no sky)
 static void vc_stale(State& s, uint32_t slot) {
   const uint32_t* info = &s.slot_info[12 * slot];
@@ -644,8 +1002,12 @@ static void vc_stale_texel(State& s, uint32_t j) {
   }
 }
 
-// WGSL :3432-3554, static lights: one job.
-static void vc_bake_job(State& s, uint32_t j, uint32_t total) {
+// WGSL :3432-3554: one job. A moving light's texel (WGSL :3448-3530) bakes
+// on its page's lattice when the light is a rect one, snaps ratios within ε
+// of 0 or 1 when not, and records its ledger state (`record`); `pool` is
+// the pool written (the cache's, or a check's copy).
+static void vc_bake_job(State& s, uint32_t j, uint32_t total, bool record = true, float* pool = nullptr) {
+  if (!pool) pool = s.pool.data();
   uint32_t jx = s.bake_jobs[2 * std::min(j, total - 1)], jy = s.bake_jobs[2 * std::min(j, total - 1) + 1];
   bool listed = j < total && jx != NONE;
   if (!listed) return;  // an unlisted invocation writes nothing
@@ -662,23 +1024,51 @@ static void vc_bake_job(State& s, uint32_t j, uint32_t total) {
   const float* light = &s.lights[16 * l];
   float apex[3] = {sf.apex.x, sf.apex.y, sf.apex.z}, nrm[3] = {sf.normal.x, sf.normal.y, sf.normal.z};
   float open = oracle_unshadowed_rect(apex, nrm, light);
+  const uint32_t mj = s.n_moving ? s.moving_index[l] : NONE;
+  const bool lattice = mj != NONE && !sphere_light(light);
+  const float* then = light;
+  if (lattice) {
+    uint32_t version = s.ledger[ledger_base(s, e[3], mj)];
+    if (version != 0 && s.frame - version < POSE_RING) then = pose_at(s, version, l);
+  }
+  bool on_lattice = lattice;
+  if (lattice && on && open > 0.0f) on_lattice = oracle_lattice_frame(apex, then, nrm, nullptr) <= LATTICE_KAPPA_MAX;
+  int lat_flags[2] = {1, 0};
+  float lat = -1.0f;
+  if (lattice)
+    lat = oracle_lattice_ratio(s.tris, s.ranges, s.n_objects, s.blas, s.blas_items, s.blas_cov, s.nodes, s.items,
+                               s.xforms, s.opts, on && on_lattice && open > 0.0f, apex, nrm, light, then, lat_flags,
+                               nullptr);
+  const bool overflow = on && on_lattice && open > 0.0f && lat < 0.0f;
   float cc[3];
   oracle_light_center(light, cc);
   V3 toward = orc::normalize(V3{cc[0], cc[1], cc[2]} - sf.apex);
   float tw[3] = {toward.x, toward.y, toward.z};
-  bool back = on && open <= 0.0f && oracle_faces_light(light, apex, tw);
+  bool back = on && open <= 0.0f && mj == NONE && oracle_faces_light(light, apex, tw);
   float seen = back ? oracle_unshadowed_rect(apex, tw, light) : open;
   float value = 0.0f;
-  uint64_t masks[2] = {0, ~0ull};
-  if (on)
+  uint64_t masks[2] = {0, 0};
+  if (on && (!on_lattice || overflow))
     oracle_query_skip(s.tris, s.ranges, s.n_objects, s.blas, s.blas_items, s.blas_cov, s.nodes, s.items, s.xforms,
                       s.opts, apex, back ? tw : nrm, light, back ? m : NONE, back ? sf.tri : NONE, &value, masks);
   bool last_empty = masks[1] == 0, last_full = masks[0] == ~0ull;
   float ratio = !on ? 0.0f : (seen > 0.0f ? orc::clampf(value / seen, 0.0f, 1.0f) : 1.0f);
   if (on && seen > 0.0f && last_empty) ratio = 1.0f;
   if (on && seen > 0.0f && last_full) ratio = 0.0f;
+  if (mj != NONE) {
+    if (ratio < s.epsilon) ratio = 0.0f;
+    if (ratio > 1.0f - s.epsilon) ratio = 1.0f;
+  }
   float light_value = back ? 0.0f : value;
-  s.pool[((size_t)e[3] * TEXELS + k) * s.channels + l] = s.light_ratio ? ratio : (on ? light_value : 0.0f);
+  if (on_lattice && !overflow) {
+    ratio = !on ? 0.0f : (open > 0.0f ? lat : 1.0f);
+    if (on && open > 0.0f && lat_flags[0]) ratio = 1.0f;
+    if (on && open > 0.0f && lat_flags[1]) ratio = 0.0f;
+    light_value = ratio * open;
+  }
+  pool[((size_t)e[3] * TEXELS + k) * s.channels + l] = s.light_ratio ? ratio : (on ? light_value : 0.0f);
+  if (mj != NONE && record)
+    ledger_texel(s, on, e[3], mj, l, k, sf.apex, sf.normal, light, open, then, lattice, overflow);
 }
 
 // WGSL :3693-3836 without sky or bounce light: one page.
@@ -992,6 +1382,13 @@ void oracle_vc_frame(void* h, uint64_t* out_stats) {
   for (uint32_t slot : s.used_marks) s.slot_used[slot] = s.frame;
   for (uint32_t slot : s.cleared) std::fill(&s.slot_info[12 * slot], &s.slot_info[12 * slot + 12], 0u);
   s.cleared.clear();
+  if (s.n_moving) {
+    // This frame's light poses (vcache.rs:2261-2266), then fresh ledger
+    // records for the planned pages (WGSL :4655-4662 `vc_ledger_reset`).
+    std::copy(s.lights.begin(), s.lights.end(), s.poses.begin() + (size_t)(s.frame % POSE_RING) * s.n_lights * 16);
+    for (uint32_t p = 0; p < plan_pages; p++)
+      for (uint32_t j = 0; j < s.n_moving; j++) ledger_reset(s, s.bake_list[4 * p + 3], j);
+  }
   bool moving = !s.moved.empty() && s.n_movers > 0;
   if (!moving) s.n_movers = 0;
   if (moving) {
@@ -999,6 +1396,9 @@ void oracle_vc_frame(void* h, uint64_t* out_stats) {
     uint64_t threads = (uint64_t)s.candidate_cap * TEXELS;
     for (uint64_t j = 0; j < threads; j++) vc_stale_texel(s, (uint32_t)j);
   }
+  // The ledger runs every frame with moving lights (instanced.rs:176-185).
+  if (s.n_moving)
+    for (uint32_t slot = 0; slot < s.slot_extent; slot++) vc_ledger(s, slot, s.lights_moved);
   uint32_t total = plan_jobs + std::min(s.stats[5], app_jobs);
   for (uint32_t j = 0; j < total; j++) vc_bake_job(s, j, total);
   uint32_t pages = plan_pages + std::min(s.stats[0], s.append_pages);
@@ -1023,10 +1423,113 @@ void oracle_vc_frame(void* h, uint64_t* out_stats) {
   uint64_t st[10] = {plan_pages, plan_jobs, std::min(s.stats[0], s.append_pages), std::min(s.stats[5], app_jobs),
                      s.stats[4], s.stats[1], s.stats[6], s.stats[3], s.stats[2], evicted};
   if (out_stats) std::memcpy(out_stats, st, sizeof(st));
+  const uint32_t ledger_words[10] = {8, 9, 10, 11, 12, 13, 14, 21, 22, 23};
+  for (int i = 0; i < 10; i++) s.last_ledger[i] = s.stats[ledger_words[i]];
   s.plan_list.clear(), s.plan_bounds.clear(), s.plan_jobs.clear(), s.used_marks.clear();
   s.moved.clear();
   s.n_movers = 0;
+  s.lights_moved = false;
   s.frame++;
+}
+
+// WGSL :4666-4687 `light_shift` from pose `a` to pose `b` seen from the
+// box (lo3, hi3).
+float oracle_vc_light_shift(const float* a, const float* b, const float* box) {
+  return light_shift(a, b, Box3{V3{box[0], box[1], box[2]}, V3{box[3], box[4], box[5]}});
+}
+
+// Moving lights (vcache.rs:1399-1430): lights[i] of n move through the
+// ledger with threshold `epsilon`; before the first frame, with no movers.
+int oracle_vc_set_moving(void* h, const uint32_t* lights, uint32_t n, float epsilon) {
+  State& s = *static_cast<State*>(h);
+  if (s.frame != 1 || n == 0) return -1;
+  s.moving_index.assign(s.n_lights, NONE);
+  for (uint32_t j = 0; j < n; j++) {
+    if (lights[j] >= s.n_lights || s.moving_index[lights[j]] != NONE) return -1;
+    s.moving_index[lights[j]] = j;
+  }
+  s.n_moving = n;
+  s.epsilon = epsilon;
+  s.ledger.assign((size_t)s.pages * n * LEDGER_WORDS, 0);
+  s.ledger_bound.assign((size_t)s.pages * n, 0.0f);
+  s.poses.assign((size_t)POSE_RING * s.n_lights * 16, 0.0f);
+  return 0;
+}
+
+// vcache.rs:2375-2379 `set_light_pose`: light `index` is `light` from the next frame.
+int oracle_vc_move_light(void* h, uint32_t index, const float* light) {
+  State& s = *static_cast<State*>(h);
+  if (index >= s.n_lights || s.n_moving == 0 || s.moving_index[index] == NONE) return -1;
+  std::copy(light, light + 16, s.lights.begin() + 16 * (size_t)index);
+  s.lights_moved = true;
+  return 0;
+}
+
+// The last frame's ledger counters: pages re-baked, texels re-baked, pages
+// set aside, pages checked, pages re-baking mixed texels, pages held whole,
+// jobs asked, pages held on their lattice, lattices that may have grown,
+// unbounded texels of held pages (WGSL mover_stats 8-14, 21-23).
+void oracle_vc_ledger_stats(void* h, uint64_t* out) {
+  State& s = *static_cast<State*>(h);
+  for (int i = 0; i < 10; i++) out[i] = s.last_ledger[i];
+}
+
+void oracle_vc_ledger_state(void* h, uint32_t* ledger, float* bound) {
+  State& s = *static_cast<State*>(h);
+  if (ledger) std::memcpy(ledger, s.ledger.data(), s.ledger.size() * 4);
+  if (bound) std::memcpy(bound, s.ledger_bound.data(), s.ledger_bound.size() * 4);
+}
+
+// instanced.rs:3741-3850 `verify_ledger`: every readable page's texels of
+// moving light `light` baked afresh (its ledger untouched) against the
+// cache: out = pages, pages past their certified bound (+1e-5), pages past
+// ε, pages changed at all, the largest change, the largest bound.
+void oracle_vc_verify_ledger(void* h, uint32_t light, float* out) {
+  State& s = *static_cast<State*>(h);
+  for (int i = 0; i < 6; i++) out[i] = 0.0f;
+  if (light >= s.n_lights || s.n_moving == 0 || s.moving_index[light] == NONE) return;
+  const uint32_t j = s.moving_index[light];
+  std::vector<float> fresh(s.pool);
+  std::vector<uint32_t> list, jobs;
+  std::vector<uint32_t> slots;
+  for (uint32_t slot = 0; slot < s.slot_key.size(); slot++) {
+    if (s.slot_key[slot].empty() || s.slot_stale[slot] != 0) continue;
+    const uint32_t* k = s.slot_key[slot].data();
+    uint32_t index = (uint32_t)(list.size() / 4);
+    list.insert(list.end(), {k[0], k[1], k[2], slot});
+    for (uint32_t t = 0; t < TEXELS; t++) jobs.insert(jobs.end(), {index, t | (light << 8)});
+    slots.push_back(slot);
+  }
+  std::swap(s.bake_list, list);
+  std::swap(s.bake_jobs, jobs);
+  const uint32_t total = (uint32_t)(s.bake_jobs.size() / 2);
+  for (uint32_t q = 0; q < total; q++) vc_bake_job(s, q, total, false, fresh.data());
+  std::swap(s.bake_list, list);
+  std::swap(s.bake_jobs, jobs);
+  uint32_t pages = 0, violations = 0, over = 0, changed = 0;
+  float largest = 0.0f, largest_bound = 0.0f;
+  for (uint32_t p = 0; p < slots.size(); p++) {
+    const uint32_t slot = slots[p];
+    const uint32_t* k = s.slot_key[slot].data();
+    uint32_t c = k[1] & CHART_MASK, mip = k[1] >> MIP_SHIFT, g[2];
+    grid_of(s, c, k[0], mip, g);
+    uint32_t px = (k[2] & 0xffffu) * PAGE, py = (k[2] >> 16) * PAGE;
+    float change = 0.0f;
+    for (uint32_t t = 0; t < TEXELS; t++) {
+      if (px + t % STRIDE >= g[0] || py + t / STRIDE >= g[1]) continue;  // border texels copy inside ones
+      size_t at = ((size_t)slot * TEXELS + t) * s.channels + light;
+      change = orc::fmax_(change, std::fabs(s.pool[at] - fresh[at]));
+    }
+    float bound = s.ledger_bound[(size_t)slot * s.n_moving + j];
+    pages++;
+    if (change > bound + 1e-5f) violations++;
+    if (change > s.epsilon) over++;
+    if (change > 0.0f) changed++;
+    largest = orc::fmax_(largest, change);
+    largest_bound = orc::fmax_(largest_bound, bound);
+  }
+  out[0] = (float)pages, out[1] = (float)violations, out[2] = (float)over, out[3] = (float)changed;
+  out[4] = largest, out[5] = largest_bound;
 }
 
 // WGSL :3134 vc_find, then the page's texels.

@@ -659,8 +659,55 @@ def view_cache_form(device, boulders: int = 512, moves: int = 5) -> dict:
     differ, compared = vc.verify()
     out["light_edit"] = {"pages": st["pages"], "cones": st["cones"], "ms": dt * 1e3,
                          "verify": {"values_compared": compared, "values_differ": differ}}
+    out["moving_lights"] = moving_lights_leg(scene, surf, vlights, opts, keys)
     out["view"] = view_from_cache(device, scene, surf, vlights, opts, eye)
     return out
+
+
+def moving_lights_leg(scene, surf, vlights, opts, keys, frames: int = 6) -> dict:
+    """The same pages in caches whose sun (a distant square) and nearest
+    rect light move through the error ledger, both on their lattices, at two
+    speeds: the sun's corner turning 1.2e-6 rad a frame about the vertical
+    axis (a real sun at 60 frames/s) with the rect light moving 1 mm a
+    frame, and 1e-4 rad with 2 cm. After the first bake, each frame the
+    ledger re-bakes only what its bounds say may be off by more than
+    epsilon (mixed texels, or whole pages past their slack, whose fresh
+    texels trace guard cones). Then the reference's `verify_ledger` check
+    for both lights."""
+    import numpy as np
+
+    from continuum_amd.kernel import ViewCache
+
+    def run(turn: float, step: float) -> dict:
+        vc = ViewCache(scene, surf, vlights, opts, pages=32768, append_pages=32768, candidates=16, base_texel=0.05)
+        vc.set_moving_lights([0, 1], 0.01)
+        vc.request(keys)
+        vc.frame()
+        sun0 = np.array(vc.lights[0], np.float32, copy=True)
+        rect0 = np.array(vc.lights[1], np.float32, copy=True)
+        rows = []
+        for k in range(1, frames + 1):
+            rect = rect0.copy()
+            rect[0] += np.float32(step * k)
+            c, s_ = math.cos(turn * k), math.sin(turn * k)
+            sun = sun0.copy()
+            sun[0] = np.float32(c * float(sun0[0]) - s_ * float(sun0[1]))
+            sun[1] = np.float32(s_ * float(sun0[0]) + c * float(sun0[1]))
+            vc.move_light(1, rect)
+            vc.move_light(0, sun)
+            vc.request(keys)
+            t0 = time.perf_counter()
+            vc.frame()
+            dt = time.perf_counter() - t0
+            rows.append(dict(ms=dt * 1e3, **vc.ledger_stats()))
+        checks = {str(light): vc.verify_ledger(light) for light in (0, 1)}
+        del vc
+        return {"sun_turn_rad_per_frame": turn, "rect_step_m_per_frame": step, "frames": rows,
+                "ms_mean": float(np.mean([r["ms"] for r in rows])),
+                "texels_rebaked_mean": float(np.mean([r["texels"] for r in rows])), "verify_ledger": checks}
+
+    return {"workload": "the same pages; sun panel and nearest rect light moving (epsilon 0.01)",
+            "slow": run(1.2e-6, 0.001), "fast": run(1e-4, 0.02)}
 
 
 def sky_form(scene, flat, sc, recv, h_recv, check: bool = True, stride: int = 64) -> dict:

@@ -30,6 +30,8 @@ class RendererOptions:
     """The parts of `RendererOptions` (renderer.rs:40-190) this library has."""
     cache: bool = True                    # shade through the view cache (else a cone per pixel and light)
     movable: Sequence[int] = ()           # instances that may move
+    moving_lights: Sequence[int] = ()     # lights that move through the cache's error ledger
+    ledger_epsilon: float = 0.01          # the ledger's threshold (`CacheOptions::ledger_epsilon`)
     bake_budget: int = 2_000_000          # cones of page bakes a frame (`CacheOptions::bake_budget`)
     pages: int = 1 << 16                  # cache slots
     base_texel: float = 0.05              # metres per texel at mip 0
@@ -76,6 +78,8 @@ class Renderer:
             self.cache = ViewCache(self.scene, surface, self.lights, self.opts, pages=self.options.pages,
                                    append_pages=1024, candidates=8192, base_texel=self.options.base_texel,
                                    light_ratio=1, visible_frames=2)
+            if self.options.moving_lights:
+                self.cache.set_moving_lights(list(self.options.moving_lights), self.options.ledger_epsilon)
             self._out = torch.zeros((len(self.lights), px), dtype=torch.float32, device=self.dev)
             self._page = torch.zeros(px, dtype=torch.int32, device=self.dev)
         else:
@@ -95,14 +99,24 @@ class Renderer:
 
     def set_light(self, index: int, light) -> None:
         """renderer.rs:421: light `index` moved, resized or re-aimed (and
-        re-coloured); the cache re-bakes its channel of every resident page."""
+        re-coloured). A moving light (`moving_lights`) goes through the
+        cache's error ledger from the next frame; another has its channel of
+        every resident page re-baked now."""
         import torch
 
         row = np.ascontiguousarray(light, np.float32).reshape(16)
         self.lights[index] = row
         self._light_rows[index] = torch.from_numpy(row.copy()).to(self.dev)
         if self.cache is not None:
-            self.cache.set_lights(row[None], index)
+            if index in self.options.moving_lights:
+                self.cache.move_light(index, row)
+            else:
+                self.cache.set_lights(row[None], index)
+
+    def verify_ledger(self, light: int) -> dict:
+        """renderer.rs:640 `verify_ledger`: moving light `light`'s pages shown,
+        re-baked afresh, against their certified bounds."""
+        return self.cache.verify_ledger(light)
 
     def set_light_radiance(self, index: int, radiance) -> None:
         """renderer.rs:548: re-colour light `index`; nothing is baked."""

@@ -77,3 +77,25 @@ def test_cached_view_settles_verifies_and_follows_edits(device):
     r.render(cam)
     assert r.verify_cache()[0] == 0
     assert np.isfinite(r.read_colors()).all()
+
+
+def test_a_moving_light_goes_through_the_ledger(device):
+    """RendererOptions::moving_lights: set_light on a moving light goes
+    through the cache's error ledger, and the reference's verify_ledger
+    finds every page shown within its bound and epsilon."""
+    tris9, counts, pl, surf, lights, movable, cam = _world()
+    opts = RendererOptions(cache=True, pages=4096, base_texel=0.1, moving_lights=[0])
+    r = Renderer(device, (W, H), tris9, counts, pl, surf, lights, opts)
+    for frame in range(60):
+        r.render(cam)
+        if r.is_settled(2):
+            break
+    assert r.is_settled(2)
+    for k in range(1, 6):
+        moved = lights[0].copy()
+        moved[0:3] += np.float32([0.01 * k, -0.005 * k, 0.0])
+        r.set_light(0, moved)
+        r.render(cam)
+    chk = r.verify_ledger(0)
+    assert chk["pages"] > 0 and chk["violations"] == 0 and chk["over_epsilon"] == 0, chk
+    assert np.isfinite(r.read_colors()).all()

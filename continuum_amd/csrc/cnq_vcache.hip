@@ -1146,14 +1146,19 @@ __global__ __launch_bounds__(64, WALK_WAVES) void vc_bake_kernel(VcArgs a, const
   // and fewer as the bake nears its end (down to 4 once fewer jobs remain
   // than 4 for each wave the GPU holds). A packet's walk visits the union
   // of its lanes' nodes, and a narrower packet's union is shorter, so the
-  // last waves, which the frame waits for, finish sooner.
+  // last waves, which the frame waits for, finish sooner. A bake smaller
+  // than one round of 64-job packets (a view's few new pages) shrinks its
+  // packets as if four times as many jobs remained: waves then take several
+  // narrow packets each, and its few long walks no longer set its length;
+  // larger bakes keep their wide packets.
+  const uint32_t spread = (uint64_t)total < (uint64_t)a.bake_group * a.bake_resident ? 4u : 1u;
   for (;;) {
     uint32_t start = 0, group = 0;
     if (lane == 0) {
       const uint32_t done = __atomic_load_n(&a.stats[7], __ATOMIC_RELAXED);
       const uint32_t left = done < total ? total - done : 0u;
       group = a.bake_group;
-      while (group > 4u && (uint64_t)left < (uint64_t)group * a.bake_resident) group >>= 1;
+      while (group > 4u && (uint64_t)left < (uint64_t)group * a.bake_resident * spread) group >>= 1;
       start = atomicAdd(&a.stats[7], group);
     }
     start = uni(start);

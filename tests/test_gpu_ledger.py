@@ -116,3 +116,27 @@ def test_moving_light_arguments_are_checked(device):
         vc.move_light(0, sphere_light([0.0, 0.0, 5.0], 0.5))  # a rect light stays a rect
     with pytest.raises(RuntimeError):
         vc.set_lights(_sun(0.002, 5)[None, :], first=0)  # a moving light moves by move_light
+
+
+def test_pages_past_the_append_capacity_are_set_aside_and_the_rest_stay_sound(device):
+    """With room for only a few re-baked pages a frame, which pages the
+    GPU's atomics append first is not the oracle's order, so the invariant
+    is checked instead (as for the movers' stale passes): pages that did not
+    fit are set aside (kept, missing), and every page still shown is within
+    its certified bound and epsilon of a fresh bake."""
+    vc, ov, pl, kw = _pair(device, 0.05, append_pages=4)
+    keys = _keys(vc, pl, mips=(0,))
+    vc.request(keys)
+    vc.frame()
+    for fr in range(1, 4):
+        vc.move_light(0, _sun(0.05, fr))
+        vc.move_light(1, sphere_light([6.0 + 0.3 * fr, -4.0, 5.0], 0.5))
+        vc.request(keys)
+        vc.frame()
+        st = vc.ledger_stats()
+        assert st["pages"] <= 4 and st["set_aside"] > 0, st
+        for light in (0, 1):
+            chk = vc.verify_ledger(light)
+            assert chk["violations"] == 0 and chk["over_epsilon"] == 0, (fr, light, chk)
+    stale = vc.state()["slot_stale"]
+    assert (stale == 2).sum() > 0 and (stale == 1).sum() == 0

@@ -734,7 +734,9 @@ __device__ __forceinline__ LatticeOut lattice_ratio(const Ctx& x, const QueryPar
 }
 
 // The guard cone's walk (WGSL :4576-4583) for the lanes with `trace`:
-// whether any bin is touched.
+// whether any bin is touched. The reference walks the whole guard cone and
+// then looks for a touched bin; a lane here stops at the first mesh or
+// splat that lands coverage in a bin, which settles the same answer.
 template <bool COUNT>
 __device__ __forceinline__ bool guard_touched(const Ctx& x, const QueryParams& p, Lane& L, const Cone& c,
                                               bool trace, float* cov, uint64_t* work) {
@@ -749,6 +751,7 @@ __device__ __forceinline__ bool guard_touched(const Ctx& x, const QueryParams& p
   w.sky_max = false;
   w.overflow = false;
   w.truncated = false;
+  w.stop_on_touch = true;
 #pragma unroll
   for (int k = 0; k < 6; k++) w.work[k] = 0;
   if (uni(ballot(trace) != 0ull ? 1u : 0u)) scene_coverage<COUNT>(x, c, trace, w, L);

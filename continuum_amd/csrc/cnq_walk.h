@@ -360,6 +360,9 @@ struct Walk {
   bool sky_max;
   bool overflow;   // deeper than the reference's 48 entries
   bool truncated;  // deeper than this stack's 64 (the walk stopped)
+  // A walk that only asks whether anything meets the cone (a guard cone)
+  // stops once a mesh's coverage lands in a bin: the answer is settled.
+  bool stop_on_touch = false;
   uint32_t work[6];  // object nodes tested, blas nodes tested, triangles tested, triangles drawn,
                      // meshes entered, blas nodes splatted
 };
@@ -629,6 +632,7 @@ __device__ __forceinline__ bool cover_triangle(const Cone& c, Lane& L, f3 v0, f3
 template <bool SKY = false>
 __device__ __forceinline__ void combine(Walk& w, Lane& L, bool snap) {
   float running = 0.0f;
+  bool landed = false;
   const uint32_t end = L.span_hi < BINS ? L.span_hi : BINS;
   for (uint32_t b = L.span_lo; b < end; b++) {
     {
@@ -646,9 +650,11 @@ __device__ __forceinline__ void combine(Walk& w, Lane& L, bool snap) {
         else
           w.cov[b] = mx(before, mn(covered, 1.0f));
         if (before < 1.0f && w.cov[b] >= 1.0f) w.full++;
+        landed = true;
       }
     }
   }
+  if (w.stop_on_touch && landed) w.full = BINS;
 }
 
 __device__ __forceinline__ void clear_acc(Lane& L) {

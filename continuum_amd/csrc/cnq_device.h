@@ -438,43 +438,67 @@ __device__ __forceinline__ uint32_t clip_plane3(Lane& L, uint32_t src, uint32_t 
   return m;
 }
 
-// WGSL :280-294, in the projected plane.
+// WGSL :280-294, in the projected plane. The source polygon's vertices are
+// read in one batch of loads before the clip (slots past `n` re-read vertex
+// 0 and go unused), so a pass waits on memory once, not once per vertex;
+// the arithmetic and the order of the outputs are the loop's own.
 __device__ __forceinline__ uint32_t clip_line2(Lane& L, uint32_t src, uint32_t n, f2 normal, float offset,
                                                uint32_t dst) {
   uint32_t m = 0;
   if (n == 0) return 0;
-  const f2 first = L.p2(src, 0);
+  f2 v[MAX_POLY];
+#pragma unroll
+  for (uint32_t i = 0; i < MAX_POLY; i++) v[i] = L.p2(src, i < n ? i : 0u);
+  const f2 first = v[0];
   f2 a = first;
   float da = offset - dot2(normal, a);
-  for (uint32_t i = 0; i < n; i++) {
-    const f2 b = i + 1 < n ? L.p2(src, i + 1) : first;
-    const float db = offset - dot2(normal, b);
-    if (da >= 0.0f && m < MAX_POLY) L.put2(dst, m++, a);
-    if (da >= 0.0f && db < 0.0f && m < MAX_POLY) L.put2(dst, m++, a + (b - a) * (da / (da - db)));
-    if (da < 0.0f && db >= 0.0f && m < MAX_POLY) L.put2(dst, m++, b + (a - b) * (db / (db - da)));
-    a = b;
-    da = db;
+#pragma unroll
+  for (uint32_t i = 0; i < MAX_POLY; i++) {
+    if (i < n) {
+      const f2 b = i + 1 < n ? v[i + 1 < MAX_POLY ? i + 1 : 0] : first;
+      const float db = offset - dot2(normal, b);
+      if (da >= 0.0f && m < MAX_POLY) L.put2(dst, m++, a);
+      if (da >= 0.0f && db < 0.0f && m < MAX_POLY) L.put2(dst, m++, a + (b - a) * (da / (da - db)));
+      if (da < 0.0f && db >= 0.0f && m < MAX_POLY) L.put2(dst, m++, b + (a - b) * (db / (db - da)));
+      a = b;
+      da = db;
+    }
   }
   return m;
 }
 
-// The four clips to [0, 8]^2 (WGSL :432-437), ping-ponging between the two
-// 2D slots; `at` is where the polygon is, and is where it ends up.
+// The four clips to [0, 8]^2 (WGSL :432-437: x >= 0, x <= 8, y >= 0, y <= 8,
+// in that order), ping-ponging between the two 2D slots; `at` is where the
+// polygon is, and is where it ends up.
 __device__ __forceinline__ uint32_t clip_to_grid(Lane& L, uint32_t& at, uint32_t n) {
   const float g = (float)GRID;
   const uint32_t other = at == Q2A ? Q2B : Q2A;
-  n = clip_line2(L, at, n, f2{-1.0f, 0.0f}, 0.0f, other);
-  n = clip_line2(L, other, n, f2{1.0f, 0.0f}, g, at);
-  n = clip_line2(L, at, n, f2{0.0f, -1.0f}, 0.0f, other);
-  n = clip_line2(L, other, n, f2{0.0f, 1.0f}, g, at);
+#pragma unroll 1
+  for (uint32_t k = 0; k < 4; k++) {
+    const f2 normal{k == 0 ? -1.0f : (k == 1 ? 1.0f : 0.0f), k == 2 ? -1.0f : (k == 3 ? 1.0f : 0.0f)};
+    const bool odd = (k & 1u) != 0;
+    n = clip_line2(L, odd ? other : at, n, normal, odd ? g : 0.0f, odd ? at : other);
+  }
   return n;
 }
 
+// Vertex `k` (uniform) of a polygon held in registers.
+__device__ __forceinline__ f2 pick2(const f2 (&v)[MAX_POLY], uint32_t k) {
+  f2 r = v[0];
+#pragma unroll
+  for (uint32_t j = 1; j < MAX_POLY; j++) r = k == j ? v[j] : r;
+  return r;
+}
+
+// Draws the polygon's edges in order (its vertices read in one batch).
 __device__ __forceinline__ void draw_poly(Lane& L, uint32_t at, uint32_t n, float weight) {
-  const f2 first = clamp_grid(L.p2(at, 0));
+  f2 v[MAX_POLY];
+#pragma unroll
+  for (uint32_t i = 0; i < MAX_POLY; i++) v[i] = L.p2(at, i < n ? i : 0u);
+  const f2 first = clamp_grid(v[0]);
   f2 a = first;
   for (uint32_t i = 0; i < n; i++) {
-    const f2 b = i + 1 < n ? clamp_grid(L.p2(at, i + 1)) : first;
+    const f2 b = i + 1 < n ? clamp_grid(pick2(v, i + 1)) : first;
     draw_edge(L, a, b, weight);
     a = b;
   }
@@ -493,78 +517,63 @@ __device__ __forceinline__ bool in_grid(f2 q) {
   return !(q.x < 0.0f || q.y < 0.0f || q.x > g || q.y > g);
 }
 
-__device__ __forceinline__ bool rasterize_general(const Cone& c, Lane& L, f3 v0, f3 v1, f3 v2, float weight,
-                                                  bool far_clip, float su, float sv);
-
-// WGSL :350-451. A triangle wholly in front of the near plane, before the far
-// plane and inside the window projects to a triangle inside the grid, drawn
-// straight from registers; any other case builds its polygons in LDS.
-__device__ __forceinline__ bool rasterize(const Cone& c, Lane& L, f3 v0, f3 v1, f3 v2, float weight) {
-  const float f0 = c.far_offset - dot3(c.far_normal, v0);
-  const float f1 = c.far_offset - dot3(c.far_normal, v1);
-  const float f2_ = c.far_offset - dot3(c.far_normal, v2);
-  const bool far_clip = f0 < 0.0f || f1 < 0.0f || f2_ < 0.0f;
-  const float su = (float)GRID / (c.w2 - c.w0);
-  const float sv = (float)GRID / (c.w3 - c.w1);
+// The general path of `rasterize` (WGSL :358-431) up to the grid clip: the
+// triangle as a polygon, clipped by the far plane (`far_clip`) and the near
+// plane where it crosses them, then either projected (wholly in the window)
+// or the window's square clipped by the planes through the apex and each of
+// its edges. Leaves the 2D polygon at `at` and returns its vertex count (0:
+// nothing to draw); `grid_clip`: a vertex lies outside the grid. Without a
+// far clip the caller passes the triangle's local corners (l0, l1, l2); where
+// all three lie in front of the near plane (the common case) the edge planes
+// come from them in registers, and the 3D polygon never goes to memory.
+__device__ __forceinline__ uint32_t general_polygon(const Cone& c, Lane& L, f3 v0, f3 v1, f3 v2, bool far_clip,
+                                                    f3 l0, f3 l1, f3 l2, float su, float sv, uint32_t& at,
+                                                    bool& grid_clip) {
   const float g = (float)GRID;
-  if (!far_clip) {
-    const f3 l0 = to_local(c, v0), l1 = to_local(c, v1), l2 = to_local(c, v2);
-    if (l0.z >= NEAR && l1.z >= NEAR && l2.z >= NEAR && in_window(c, l0) && in_window(c, l1) && in_window(c, l2)) {
-      const f2 q0 = project(c, l0, su, sv), q1 = project(c, l1, su, sv), q2 = project(c, l2, su, sv);
-      if (in_grid(q0) && in_grid(q1) && in_grid(q2)) {
-        draw_edge(L, q0, q1, weight);
-        draw_edge(L, q1, q2, weight);
-        draw_edge(L, q2, q0, weight);
-        return true;
-      }
-      // Rounding put a corner a hair outside the grid: clip as the general path does.
-      L.put2(Q2A, 0, q0);
-      L.put2(Q2A, 1, q1);
-      L.put2(Q2A, 2, q2);
-      uint32_t at = Q2A;
-      const uint32_t n = clip_to_grid(L, at, 3);
-      if (n < 3) return false;
-      draw_poly(L, at, n, weight);
-      return true;
-    }
-  }
-  return rasterize_general(c, L, v0, v1, v2, weight, far_clip, su, sv);
-}
-
-// The general path of `rasterize`: the triangle clipped as a polygon.
-__device__ __forceinline__ bool rasterize_general(const Cone& c, Lane& L, f3 v0, f3 v1, f3 v2, float weight,
-                                                  bool far_clip, float su, float sv) {
-  const float g = (float)GRID;
-  // General path: the triangle as a polygon at P3A.
-  L.put3(P3A, 0, v0);
-  L.put3(P3A, 1, v1);
-  L.put3(P3A, 2, v2);
+  grid_clip = false;
+  at = Q2A;
   uint32_t n = 3;
-  if (far_clip) {
+  bool all_near_ok = true, all_in_window = true, in_regs = false;
+  if (!far_clip) {
+    all_near_ok = l0.z >= NEAR && l1.z >= NEAR && l2.z >= NEAR;
+    all_in_window = in_window(c, l0) && in_window(c, l1) && in_window(c, l2);
+    in_regs = all_near_ok;
+    if (!all_near_ok) {
+      L.put3(P3A, 0, l0);
+      L.put3(P3A, 1, l1);
+      L.put3(P3A, 2, l2);
+    }
+  } else {
+    L.put3(P3A, 0, v0);
+    L.put3(P3A, 1, v1);
+    L.put3(P3A, 2, v2);
     n = clip_plane3(L, P3A, n, c.far_normal, c.far_offset, P3B);
-    for (uint32_t i = 0; i < n; i++) L.put3(P3A, i, L.p3(P3B, i));
-  }
-  bool all_near_ok = true, all_in_window = true;
-  for (uint32_t i = 0; i < n; i++) {
-    const f3 v = to_local(c, L.p3(P3A, i));
-    L.put3(P3A, i, v);
-    if (v.z < NEAR) all_near_ok = false;
-    if (!in_window(c, v)) all_in_window = false;
+    for (uint32_t i = 0; i < n; i++) {
+      const f3 v = to_local(c, L.p3(P3B, i));
+      L.put3(P3A, i, v);
+      if (v.z < NEAR) all_near_ok = false;
+      if (!in_window(c, v)) all_in_window = false;
+    }
   }
   if (!all_near_ok) {
     n = clip_plane3(L, P3A, n, f3{0.0f, 0.0f, -1.0f}, -NEAR, P3B);
     for (uint32_t i = 0; i < n; i++) L.put3(P3A, i, L.p3(P3B, i));
   }
-  if (n < 3) return false;
-  uint32_t at = Q2A, qn;
+  if (n < 3) return 0;
+  uint32_t qn;
   if (all_near_ok && all_in_window) {
     qn = n;
-    for (uint32_t i = 0; i < n; i++) L.put2(Q2A, i, project(c, L.p3(P3A, i), su, sv));
+    for (uint32_t i = 0; i < n; i++) {
+      const f3 p = in_regs ? (i == 0 ? l0 : (i == 1 ? l1 : l2)) : L.p3(P3A, i);
+      L.put2(Q2A, i, project(c, p, su, sv));
+    }
   } else {
     // The window's square clipped by the planes through the apex and each
     // edge, with the whole triangle's winding (WGSL :376-425).
-    const float d = dot3(to_local(c, v0), cross3(to_local(c, v1), to_local(c, v2)));
-    if (d == 0.0f) return false;
+    const f3 t0 = far_clip ? to_local(c, v0) : l0, t1 = far_clip ? to_local(c, v1) : l1,
+             t2 = far_clip ? to_local(c, v2) : l2;
+    const float d = dot3(t0, cross3(t1, t2));
+    if (d == 0.0f) return 0;
     const float side = d > 0.0f ? 1.0f : -1.0f;
     qn = 4;
     L.put2(Q2A, 0, f2{0.0f, 0.0f});
@@ -577,7 +586,15 @@ __device__ __forceinline__ bool rasterize_general(const Cone& c, Lane& L, f3 v0,
       L.put2(Q2A, 3, f2{g, 0.0f});
     }
     for (uint32_t i = 0; i < n; i++) {
-      const f3 e = cross3(L.p3(P3A, i), L.p3(P3A, i + 1 == n ? 0 : i + 1));
+      f3 p, q;
+      if (in_regs) {
+        p = i == 0 ? l0 : (i == 1 ? l1 : l2);
+        q = i == 0 ? l1 : (i == 1 ? l2 : l0);
+      } else {
+        p = L.p3(P3A, i);
+        q = L.p3(P3A, i + 1 == n ? 0 : i + 1);
+      }
+      const f3 e = cross3(p, q);
       const float len = len3(e);
       if (len == 0.0f) continue;
       const f3 k = e * (side / len);
@@ -586,10 +603,50 @@ __device__ __forceinline__ bool rasterize_general(const Cone& c, Lane& L, f3 v0,
       at = to;
     }
   }
-  bool inside = true;
-  for (uint32_t i = 0; i < qn; i++)
-    if (!in_grid(L.p2(at, i))) inside = false;
-  if (!inside) qn = clip_to_grid(L, at, qn);
+  f2 v[MAX_POLY];
+#pragma unroll
+  for (uint32_t i = 0; i < MAX_POLY; i++) v[i] = L.p2(at, i < qn ? i : 0u);
+#pragma unroll
+  for (uint32_t i = 0; i < MAX_POLY; i++)
+    if (i < qn && !in_grid(v[i])) grid_clip = true;
+  return qn;
+}
+
+// WGSL :350-451. A triangle wholly in front of the near plane, before the far
+// plane and inside the window projects to a triangle inside the grid, drawn
+// straight from registers; any other case builds its polygon in the lane's
+// workspace, and every such polygon goes through one grid clip and one draw.
+__device__ __forceinline__ bool rasterize(const Cone& c, Lane& L, f3 v0, f3 v1, f3 v2, float weight) {
+  const float f0 = c.far_offset - dot3(c.far_normal, v0);
+  const float f1 = c.far_offset - dot3(c.far_normal, v1);
+  const float f2_ = c.far_offset - dot3(c.far_normal, v2);
+  const bool far_clip = f0 < 0.0f || f1 < 0.0f || f2_ < 0.0f;
+  const float su = (float)GRID / (c.w2 - c.w0);
+  const float sv = (float)GRID / (c.w3 - c.w1);
+  f3 l0{0.0f, 0.0f, 0.0f}, l1 = l0, l2 = l0;
+  uint32_t at = Q2A, qn = 0;
+  bool grid_clip = false, built = false;
+  if (!far_clip) {
+    l0 = to_local(c, v0), l1 = to_local(c, v1), l2 = to_local(c, v2);
+    if (l0.z >= NEAR && l1.z >= NEAR && l2.z >= NEAR && in_window(c, l0) && in_window(c, l1) && in_window(c, l2)) {
+      const f2 q0 = project(c, l0, su, sv), q1 = project(c, l1, su, sv), q2 = project(c, l2, su, sv);
+      if (in_grid(q0) && in_grid(q1) && in_grid(q2)) {
+        draw_edge(L, q0, q1, weight);
+        draw_edge(L, q1, q2, weight);
+        draw_edge(L, q2, q0, weight);
+        return true;
+      }
+      // Rounding put a corner a hair outside the grid: clip as the general path does.
+      L.put2(Q2A, 0, q0);
+      L.put2(Q2A, 1, q1);
+      L.put2(Q2A, 2, q2);
+      qn = 3;
+      grid_clip = true;
+      built = true;
+    }
+  }
+  if (!built) qn = general_polygon(c, L, v0, v1, v2, far_clip, l0, l1, l2, su, sv, at, grid_clip);
+  if (grid_clip) qn = clip_to_grid(L, at, qn);
   if (qn < 3) return false;
   draw_poly(L, at, qn, weight);
   return true;

@@ -423,17 +423,24 @@ __device__ __forceinline__ uint32_t clip_plane3(Lane& L, uint32_t src, uint32_t 
                                                 uint32_t dst) {
   uint32_t m = 0;
   if (n == 0) return 0;
-  const f3 first = L.p3(src, 0);
+  // The source's vertices in one batch of loads (slots past n re-read vertex 0).
+  f3 v[MAX_POLY];
+#pragma unroll
+  for (uint32_t i = 0; i < MAX_POLY; i++) v[i] = L.p3(src, i < n ? i : 0u);
+  const f3 first = v[0];
   f3 a = first;
   float da = offset - dot3(normal, a);
-  for (uint32_t i = 0; i < n; i++) {
-    const f3 b = i + 1 < n ? L.p3(src, i + 1) : first;
-    const float db = offset - dot3(normal, b);
-    if (da >= 0.0f && m < MAX_POLY) L.put3(dst, m++, a);
-    if (da >= 0.0f && db < 0.0f && m < MAX_POLY) L.put3(dst, m++, a + (b - a) * (da / (da - db)));
-    if (da < 0.0f && db >= 0.0f && m < MAX_POLY) L.put3(dst, m++, b + (a - b) * (db / (db - da)));
-    a = b;
-    da = db;
+#pragma unroll
+  for (uint32_t i = 0; i < MAX_POLY; i++) {
+    if (i < n) {
+      const f3 b = i + 1 < n ? v[i + 1 < MAX_POLY ? i + 1 : 0] : first;
+      const float db = offset - dot3(normal, b);
+      if (da >= 0.0f && m < MAX_POLY) L.put3(dst, m++, a);
+      if (da >= 0.0f && db < 0.0f && m < MAX_POLY) L.put3(dst, m++, a + (b - a) * (da / (da - db)));
+      if (da < 0.0f && db >= 0.0f && m < MAX_POLY) L.put3(dst, m++, b + (a - b) * (db / (db - da)));
+      a = b;
+      da = db;
+    }
   }
   return m;
 }

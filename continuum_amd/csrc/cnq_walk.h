@@ -444,45 +444,54 @@ __device__ __forceinline__ float node_front_area(const float* __restrict__ cov27
   return mixf(mixf(f[0], f[1], fs), mixf(f[2], f[3], fs), ft);
 }
 
-// WGSL :490-512: clockwise hull of the 8 points at P3A (2D, slots 0-15),
-// written at Q2A; returns its vertex count.
-__device__ __forceinline__ uint32_t hull8(Lane& L) {
+// Point `k` (per lane) of the 8 in registers.
+__device__ __forceinline__ f2 pick8(const f2 (&p)[8], uint32_t k) {
+  f2 r = p[0];
+#pragma unroll
+  for (uint32_t j = 1; j < 8; j++) r = k == j ? p[j] : r;
+  return r;
+}
+
+// WGSL :490-512: clockwise hull of the 8 points (2D, in registers), written
+// at Q2A; returns its vertex count.
+__device__ __forceinline__ uint32_t hull8(Lane& L, const f2 (&p)[8]) {
   uint32_t start = 0;
-  f2 ps = L.p2(P3A, 0);
+  f2 ps = p[0];
+#pragma unroll
   for (uint32_t k = 1; k < 8; k++) {
-    const f2 pk = L.p2(P3A, k);
-    if (pk.x < ps.x || (pk.x == ps.x && pk.y < ps.y)) {
+    if (p[k].x < ps.x || (p[k].x == ps.x && p[k].y < ps.y)) {
       start = k;
-      ps = pk;
+      ps = p[k];
     }
   }
   uint32_t n = 0;
   uint32_t cur = start;
+  f2 pc = ps;  // p[cur]
   for (;;) {
-    const f2 pc = L.p2(P3A, cur);
     L.put2(Q2A, n++, pc);
     uint32_t next = (cur + 1) & 7u;
-    f2 pn = L.p2(P3A, next);
+    f2 pn = pick8(p, next);
+#pragma unroll
     for (uint32_t k = 0; k < 8; k++) {
-      const f2 pk = L.p2(P3A, k);
       const f2 e = pn - pc;
-      const f2 f = pk - pc;
+      const f2 f = p[k] - pc;
       const float cr = e.x * f.y - e.y * f.x;
       if (cr > 0.0f || (cr == 0.0f && dot2(f, f) > dot2(e, e))) {
         next = k;
-        pn = pk;
+        pn = p[k];
       }
     }
     cur = next;
+    pc = pn;
     if (cur == start || n >= 8) break;
   }
   return n;
 }
 
-// The splat of a node's 8 projected corners (at P3A) with weight `alpha`:
-// their hull clipped to the grid, drawn into the accumulator.
-__device__ __forceinline__ bool draw_hull(Lane& L, float alpha) {
-  uint32_t n = hull8(L);
+// The splat of a node's 8 projected corners with weight `alpha`: their hull
+// clipped to the grid, drawn into the accumulator.
+__device__ __forceinline__ bool draw_hull(Lane& L, const f2 (&p)[8], float alpha) {
+  uint32_t n = hull8(L, p);
   uint32_t at = Q2A;
   n = clip_to_grid(L, at, n);
   if (n < 3) return false;
@@ -563,10 +572,11 @@ __device__ __forceinline__ uint32_t node_cone_test(const Cone& c, Lane& L, float
   }
   if (mx(mxv.x - mnv.x, mxv.y - mnv.y) > prefilter_bins) return 3;
   if (mxv.x <= 0.0f || mxv.y <= 0.0f || mnv.x >= g || mnv.y >= g) return 1;
+  f2 pts[8];
 #pragma unroll
   for (uint32_t k = 0; k < 8; k++) {
     const f3 p = to_local(c, box_corner(lo, hi, k));
-    L.put2(P3A, k, f2{(p.x / p.z - c.w0) / (c.w2 - c.w0) * g, (p.y / p.z - c.w1) / (c.w3 - c.w1) * g});
+    pts[k] = f2{(p.x / p.z - c.w0) / (c.w2 - c.w0) * g, (p.y / p.z - c.w1) / (c.w3 - c.w1) * g};
   }
   f3 e = hi - lo;
   f3 v = norm3(c.apex - (lo + hi) * 0.5f);
@@ -574,7 +584,7 @@ __device__ __forceinline__ uint32_t node_cone_test(const Cone& c, Lane& L, float
   if (silhouette <= 1e-30f) return 1;
   float alpha = node_front_area(cov27, v) / silhouette;
   if (alpha <= 0.0f) return 1;
-  return draw_hull(L, alpha) ? 2 : 1;
+  return draw_hull(L, pts, alpha) ? 2 : 1;
 }
 
 // How a mesh's triangles cover (WGSL :578-586, set per mesh :730-742): the
@@ -763,10 +773,12 @@ __device__ __forceinline__ bool splat_object_node(const Ctx& x, const Cone& c, W
   }
   if (mx(mxv.x - mnv.x, mxv.y - mnv.y) > w.prefilter_bins) return false;
   if (mxv.x <= 0.0f || mxv.y <= 0.0f || mnv.x >= g || mnv.y >= g) return true;
-  // A splat: the hull's points (as computed above) into the workspace.
+  // A splat: the hull's points (as computed above).
+  f2 pts[8];
+#pragma unroll
   for (uint32_t k = 0; k < 8; k++) {
     const f3 p = to_local(c, box_corner(lo, hi, k));
-    L.put2(P3A, k, f2{(p.x / p.z - c.w0) / (c.w2 - c.w0) * g, (p.y / p.z - c.w1) / (c.w3 - c.w1) * g});
+    pts[k] = f2{(p.x / p.z - c.w0) / (c.w2 - c.w0) * g, (p.y / p.z - c.w1) / (c.w3 - c.w1) * g};
   }
   f3 e = hi - lo;
   f3 v = norm3(c.apex - (lo + hi) * 0.5f);
@@ -775,7 +787,7 @@ __device__ __forceinline__ bool splat_object_node(const Ctx& x, const Cone& c, W
   float alpha = 1.0f - exp_c(-node_front_area(x.blas_cov + 27 * (size_t)(x.object_cov + n), v) / silhouette);
   if (alpha <= 0.0f) return true;
   clear_acc(L);
-  if (draw_hull(L, alpha)) combine<SKY>(w, L, false);
+  if (draw_hull(L, pts, alpha)) combine<SKY>(w, L, false);
   return true;
 }
 

@@ -335,7 +335,15 @@ struct Lane {
   float* poly;      // this lane's polygon slot 0
   uint32_t pstride;  // polygon slot i at poly[i * pstride]
   uint32_t span_lo, span_hi;
-  __device__ __forceinline__ void add(uint32_t cell, float v) { acc[cell * 64u] += v; }
+  // The accumulator is in LDS: an LDS atomic add (ds_add_f32) needs no read
+  // round trip, and rounds as v_add_f32 does (round to nearest even,
+  // denormals kept: checked bit for bit on the device). A lane's adds to its
+  // own cells take effect in program order, so each cell sums its terms in
+  // the same order as the plain read-add-write.
+  __device__ __forceinline__ void add(uint32_t cell, float v) {
+    __hip_atomic_fetch_add((__attribute__((address_space(3))) float*)(acc + cell * 64u), v, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_WORKGROUP);
+  }
   __device__ __forceinline__ float get(uint32_t cell) const { return acc[cell * 64u]; }
   __device__ __forceinline__ void set(uint32_t cell, float v) { acc[cell * 64u] = v; }
   __device__ __forceinline__ f3 p3(uint32_t base, uint32_t i) const {

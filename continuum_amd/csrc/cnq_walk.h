@@ -611,10 +611,11 @@ __device__ __forceinline__ float pow5(float x) {
   return (x2 * x2) * x;
 }
 
-// WGSL :594-616
+// WGSL :594-607: whether the cone draws the triangle (facing it, not wholly
+// behind the window, touching the cone).
 template <bool SKY = false>
-__device__ __forceinline__ bool cover_triangle(const Cone& c, Lane& L, f3 v0, f3 v1, f3 v2, const Cover& k,
-                                               uint32_t stage, uint32_t& drawn_count) {
+__device__ __forceinline__ bool triangle_drawn(const Cone& c, f3 v0, f3 v1, f3 v2, uint32_t stage,
+                                               uint32_t& drawn_count) {
   if (stage == 4) return false;
   f3 n = cross3(v1 - v0, v2 - v0);
   if (dot3(n, c.apex - v0) <= 0.0f) return false;
@@ -623,17 +624,31 @@ __device__ __forceinline__ bool cover_triangle(const Cone& c, Lane& L, f3 v0, f3
     return false;
   if (!may_touch<SKY>(c, v0, v1, v2)) return false;
   drawn_count++;
+  return true;
+}
+
+// WGSL :608-614: the weight a drawn triangle covers with. Thin glass passes
+// what Fresnel's reflectance (Schlick's, F0 0.04, at the cone's centre
+// direction) leaves, times the factor.
+__device__ __forceinline__ float triangle_weight(const Cone& c, f3 v0, f3 v1, f3 v2, const Cover& k) {
   float weight = k.weight;
   if (k.transmit > 0.0f) {
-    // WGSL :608-614: thin glass passes what Fresnel's reflectance (Schlick's,
-    // F0 0.04, at the cone's centre direction) leaves, times the factor.
+    const f3 n = cross3(v1 - v0, v2 - v0);
     const float cos_t = fabsf(dot3(norm3(n), norm3(direction(c, 0.5f, 0.5f))));
     const float f = 0.04f + 0.96f * pow5(1.0f - cos_t);
     float passes = k.transmit * (1.0f - f);
     if (k.volume) passes *= 1.0f - f;
     weight = 1.0f - passes;
   }
-  return rasterize(c, L, v0, v1, v2, weight);
+  return weight;
+}
+
+// WGSL :594-616
+template <bool SKY = false>
+__device__ __forceinline__ bool cover_triangle(const Cone& c, Lane& L, f3 v0, f3 v1, f3 v2, const Cover& k,
+                                               uint32_t stage, uint32_t& drawn_count) {
+  if (!triangle_drawn<SKY>(c, v0, v1, v2, stage, drawn_count)) return false;
+  return rasterize(c, L, v0, v1, v2, triangle_weight(c, v0, v1, v2, k));
 }
 
 // Adds one mesh's accumulation buffer to the coverage (WGSL :809-828): by
@@ -720,14 +735,35 @@ __device__ __forceinline__ void add_mesh_coverage(const Ctx& x, const Cone& wc, 
         const uint64_t dm = ballot(descend);
         if (!dm) continue;
         if (node.b > 0) {
-          const uint32_t k0 = node.a, k1 = node.a + node.b;
+          // The leaf's triangles are tested one at a time (uniform loads);
+          // then each lane draws the ones it passed, in order, a triangle a
+          // round, so the wave takes as many rounds as its busiest lane
+          // draws triangles rather than one for each triangle any lane
+          // draws. A lane's accumulator sums its triangles' edges in the
+          // same order either way.
+          // (Leaves of more than 32 triangles, which only a caller's own
+          // hierarchy can hold, go in runs of 32.)
+          const uint32_t k1 = node.a + node.b;
           if (descend) w.cone_work += node.b;
-          for (uint32_t k = k0; k < k1; k++) {
-            const float* t = x.tri_leaf + 9 * (size_t)k;
-            const f3 v0{t[0], t[1], t[2]}, v1{t[3], t[4], t[5]}, v2{t[6], t[7], t[8]};
-            if (descend && (!SKIP || x.blas_items[k] != cover_skip)) {
-              if (COUNT) w.work[2]++;
-              if (cover_triangle<SKY>(c, L, v0, v1, v2, weight, x.profile_stage, w.work[3])) touched = true;
+          for (uint32_t k0 = node.a; k0 < k1; k0 += 32u) {
+            const uint32_t kn = k1 - k0 < 32u ? k1 : k0 + 32u;
+            uint32_t drawn = 0;  // bit k - k0: this lane draws triangle k
+            for (uint32_t k = k0; k < kn; k++) {
+              const float* t = x.tri_leaf + 9 * (size_t)k;
+              const f3 v0{t[0], t[1], t[2]}, v1{t[3], t[4], t[5]}, v2{t[6], t[7], t[8]};
+              if (descend && (!SKIP || x.blas_items[k] != cover_skip)) {
+                if (COUNT) w.work[2]++;
+                if (triangle_drawn<SKY>(c, v0, v1, v2, x.profile_stage, w.work[3])) drawn |= 1u << (k - k0);
+              }
+            }
+            while (ballot(drawn != 0)) {
+              if (drawn) {
+                const uint32_t k = k0 + (uint32_t)__builtin_ctz(drawn);
+                drawn &= drawn - 1u;
+                const float* t = x.tri_leaf + 9 * (size_t)k;
+                const f3 v0{t[0], t[1], t[2]}, v1{t[3], t[4], t[5]}, v2{t[6], t[7], t[8]};
+                if (rasterize(c, L, v0, v1, v2, triangle_weight(c, v0, v1, v2, weight))) touched = true;
+              }
             }
           }
         } else {

@@ -489,11 +489,16 @@ __device__ __forceinline__ uint32_t hull8(Lane& L, const f2 (&p)[8]) {
 }
 
 // The splat of a node's 8 projected corners with weight `alpha`: their hull
-// clipped to the grid, drawn into the accumulator.
+// clipped to the grid, drawn into the accumulator. When every corner lies in
+// the grid, so does the hull, and the four clips would return it unchanged
+// (each keeps every vertex and adds none), so they are skipped.
 __device__ __forceinline__ bool draw_hull(Lane& L, const f2 (&p)[8], float alpha) {
+  bool inside = true;
+#pragma unroll
+  for (uint32_t k = 0; k < 8; k++) inside = inside && in_grid(p[k]);
   uint32_t n = hull8(L, p);
   uint32_t at = Q2A;
-  n = clip_to_grid(L, at, n);
+  if (!inside) n = clip_to_grid(L, at, n);
   if (n < 3) return false;
   draw_poly(L, at, n, alpha);
   return true;

@@ -381,13 +381,20 @@ __device__ __forceinline__ void draw_edge(Lane& acc, f2 q0, f2 q1, float weight)
   float x = p0.x;
   uint32_t yc = (uint32_t)ceilf(p1.y);
   uint32_t y_end = yc < GRID ? yc : GRID;
-  for (uint32_t y = (uint32_t)p0.y; y < y_end; y++) {
+  const uint32_t y_begin = (uint32_t)p0.y;
+  // The rows' span, once per edge (the rows run upward), and the signed
+  // weight: dir is +-1, so (dy dir) w and dy (dir w) are the same float.
+  if (y_begin < y_end) {
+    acc.span_lo = acc.span_lo < y_begin * GRID ? acc.span_lo : y_begin * GRID;
+    const uint32_t hi = (y_end - 1u) * GRID + GRID + 1u;
+    acc.span_hi = acc.span_hi > hi ? acc.span_hi : hi;
+  }
+  const float dir_weight = dir * weight;
+  for (uint32_t y = y_begin; y < y_end; y++) {
     uint32_t row = y * GRID;
-    acc.span_lo = acc.span_lo < row ? acc.span_lo : row;
-    acc.span_hi = acc.span_hi > row + GRID + 1u ? acc.span_hi : row + GRID + 1u;
     float dy = mn((float)(y + 1u), p1.y) - mx((float)y, p0.y);
     float x_next = clampf(x + dxdy * dy, 0.0f, (float)GRID);
-    float d = dy * dir * weight;
+    float d = dy * dir_weight;
     float x0 = mn(x, x_next);
     float x1 = mx(x, x_next);
     float x0_floor = floorf(x0);

@@ -553,6 +553,10 @@ int cnq_scene_create(cnq_device* dev, const cnq_scene_desc* d, cnq_scene** out) 
   s->n_objects = d->n_objects;
   s->n_blas = d->n_blas;
   s->n_blas_items = d->n_blas_items;
+  for (uint32_t n = 0; n < d->n_blas; n++) {
+    const uint32_t b = d->blas[12 * (size_t)n + 9];
+    if (b > s->max_leaf) s->max_leaf = b;
+  }
   s->n_nodes = d->n_nodes;
   s->n_items = d->n_items;
   s->n_xforms = d->n_xforms;

@@ -93,6 +93,7 @@ struct cnq_scene {
   uint32_t n_cov_rows = 0;  // rows of 27 coverage samples (>= n_blas)
   uint32_t bound_chunks = 1;  // parts of an object's vertices in the mesh-bounds pass
   uint64_t n_blas_items;
+  uint32_t max_leaf = 0;  // the most triangles a triangle-hierarchy leaf held at upload
   bool count_work = false;
   uint8_t* query_flags = nullptr;  // cnq_scene_set_query_flags: caller-owned device bytes
   uint64_t n_query_flags = 0;

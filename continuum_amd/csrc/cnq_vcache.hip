@@ -1126,8 +1126,12 @@ __device__ __forceinline__ void ledger_texel(const VcArgs& a, const Ctx& x, cons
 // `work` (as the cone kernel's) and capped walks to counters[0]. With
 // MOVING (a cache with moving lights), a moving light's texel bakes as WGSL
 // :3448-3530 and :3551-3553 say: a rect light on its page's lattice, ratios
-// within ε of 0 or 1 snapped, and its ledger record kept.
-template <bool COUNT, bool MOVING>
+// within ε of 0 or 1 snapped, and its ledger record kept. With QUEUE (leaves
+// of at most 32 triangles), each lane queues the triangles it draws across
+// a mesh's leaves and draws them in shared rounds (the walk's draw queue):
+// a bake packet's texels, spread over a page, mostly draw different
+// triangles.
+template <bool COUNT, bool MOVING, bool QUEUE = false>
 __global__ __launch_bounds__(64, WALK_WAVES) void vc_bake_kernel(VcArgs a, const NodeC* __restrict__ blas,
                                                      const NodeC* __restrict__ nodes,
                                                      const float* __restrict__ tri_leaf,
@@ -1218,7 +1222,7 @@ __global__ __launch_bounds__(64, WALK_WAVES) void vc_bake_kernel(VcArgs a, const
                             : rect_open(L, s.apex, toward, r);
     }
     const Skip skip{back ? t.m : NONE, back ? s.tri : NONE};
-    const QueryOut o = evaluate_at<COUNT, true>(x, p, L, listed, s.apex, back ? toward : s.normal,
+    const QueryOut o = evaluate_at<COUNT, true, QUEUE>(x, p, L, listed, s.apex, back ? toward : s.normal,
                                                 on && (!on_lattice || overflow), r, nullptr, COUNT ? work : nullptr,
                                                 skip);
     if (COUNT) {
@@ -1828,6 +1832,13 @@ hipError_t launch_bake(cnq_vcache* c, const cnq::VcArgs& a, const cnq::QueryPara
     else
       hipLaunchKernelGGL((cnq::vc_bake_kernel<true, false>), dim3(grid), dim3(64), 0, st, a, s->blas.p, s->nodes.p,
                          s->tri_leaf.p, s->items.p, s->blas_items.p, s->blas_cov.p, p, s->work.p, s->counters.p);
+  } else if (s->max_leaf <= 32) {
+    if (moving)
+      hipLaunchKernelGGL((cnq::vc_bake_kernel<false, true, true>), dim3(grid), dim3(64), 0, st, a, s->blas.p,
+                         s->nodes.p, s->tri_leaf.p, s->items.p, s->blas_items.p, s->blas_cov.p, p, nullptr, nullptr);
+    else
+      hipLaunchKernelGGL((cnq::vc_bake_kernel<false, false, true>), dim3(grid), dim3(64), 0, st, a, s->blas.p,
+                         s->nodes.p, s->tri_leaf.p, s->items.p, s->blas_items.p, s->blas_cov.p, p, nullptr, nullptr);
   } else {
     if (moving)
       hipLaunchKernelGGL((cnq::vc_bake_kernel<false, true>), dim3(grid), dim3(64), 0, st, a, s->blas.p, s->nodes.p,

@@ -505,9 +505,11 @@ __device__ __forceinline__ bool draw_hull(Lane& L, const f2 (&p)[8], float alpha
 }
 
 // WGSL :519-574: 0 culled, 1 splatted with nothing drawn, 2 splatted and drawn, 3 descend.
-template <bool SKY = false>
+// `before_draw` runs (for this lane) just before a splat draws into the
+// accumulator, so whatever the lane still has queued goes in first.
+template <bool SKY = false, typename F>
 __device__ __forceinline__ uint32_t node_cone_test(const Cone& c, Lane& L, float prefilter_bins, f3 lo, f3 hi,
-                                                   const float* __restrict__ cov27) {
+                                                   const float* __restrict__ cov27, F&& before_draw) {
   const float g = (float)GRID;
   f2 mnv{3.4e38f, 3.4e38f}, mxv{-3.4e38f, -3.4e38f};
   bool far_out = true, near_out = true, out_u0 = true, out_u1 = true, out_v0 = true, out_v1 = true;
@@ -589,6 +591,7 @@ __device__ __forceinline__ uint32_t node_cone_test(const Cone& c, Lane& L, float
   if (silhouette <= 1e-30f) return 1;
   float alpha = node_front_area(cov27, v) / silhouette;
   if (alpha <= 0.0f) return 1;
+  before_draw();
   return draw_hull(L, pts, alpha) ? 2 : 1;
 }
 
@@ -694,8 +697,55 @@ __device__ __forceinline__ void clear_acc(Lane& L) {
   L.span_hi = 0;
 }
 
+// A lane's triangles waiting to be drawn into its accumulator (walks with
+// QUEUE): up to DRAW_QUEUE leaves (the leaf's first triangle and a bit per
+// triangle), first in, first out. A mesh's walk tests each leaf it opens
+// with the whole packet and queues what each lane draws; a round draws one
+// triangle per lane, its oldest, whenever a lane's queue is full and at the
+// mesh's end, so rounds serve lanes that draw in different leaves at once.
+// A lane draws its triangles in its walk's order, and a splat waits for the
+// lane's queued triangles, so its accumulator sums the same terms in the
+// same order. (The leaves must hold at most 32 triangles.)
+constexpr uint32_t DRAW_QUEUE = 4;
+struct DrawQueue {
+  uint32_t base[DRAW_QUEUE], bits[DRAW_QUEUE], n;
+  __device__ __forceinline__ void push(uint32_t k0, uint32_t drawn) {
+#pragma unroll
+    for (uint32_t i = 0; i < DRAW_QUEUE; i++) {
+      base[i] = n == i ? k0 : base[i];
+      bits[i] = n == i ? drawn : bits[i];
+    }
+    n++;
+  }
+  // The oldest queued triangle, taken off the queue (n > 0).
+  __device__ __forceinline__ uint32_t pop() {
+    const uint32_t k = base[0] + (uint32_t)__builtin_ctz(bits[0]);
+    bits[0] &= bits[0] - 1u;
+    if (bits[0] == 0) {
+#pragma unroll
+      for (uint32_t i = 0; i + 1 < DRAW_QUEUE; i++) {
+        base[i] = base[i + 1];
+        bits[i] = bits[i + 1];
+      }
+      n--;
+    }
+    return k;
+  }
+};
+
+// One round: each lane with a queued triangle draws its oldest.
+__device__ __forceinline__ void draw_round(const Ctx& x, const Cone& c, Lane& L, DrawQueue& q, const Cover& weight,
+                                           bool& touched) {
+  if (q.n) {
+    const uint32_t k = q.pop();
+    const float* t = x.tri_leaf + 9 * (size_t)k;
+    const f3 v0{t[0], t[1], t[2]}, v1{t[3], t[4], t[5]}, v2{t[6], t[7], t[8]};
+    if (rasterize(c, L, v0, v1, v2, triangle_weight(c, v0, v1, v2, weight))) touched = true;
+  }
+}
+
 // WGSL :719-829 for mesh `m` (uniform), for the lanes with `on`.
-template <bool COUNT, bool SKIP = false, bool SKY = false>
+template <bool COUNT, bool SKIP = false, bool SKY = false, bool QUEUE = false>
 __device__ __forceinline__ void add_mesh_coverage(const Ctx& x, const Cone& wc, uint32_t m, bool on, Walk& w,
                                                   Stack& st, uint32_t base, Lane& L, Skip skip = Skip{NONE, NONE}) {
   if (x.profile_stage == 3) return;
@@ -713,6 +763,10 @@ __device__ __forceinline__ void add_mesh_coverage(const Ctx& x, const Cone& wc, 
   clear_acc(L);
   bool touched = false;
   if (x.hierarchy == 2) {
+    DrawQueue q;
+    q.n = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < DRAW_QUEUE; i++) q.base[i] = 0, q.bits[i] = 0;
     if (root != NONE) {
       if (base + 1 > STACK_LIMIT) {
         w.truncated = true;
@@ -720,7 +774,17 @@ __device__ __forceinline__ void add_mesh_coverage(const Ctx& x, const Cone& wc, 
       }
       st.put(base, root, ballot(on));
       uint32_t sp = base + 1;
-      while (sp > base) {
+      for (;;) {
+        if (QUEUE) {
+          // Draw rounds: while a lane's queue is full (so each has room for
+          // the one leaf a step adds), and at the walk's end until every
+          // queue is empty (one place, so the drawing code is emitted once).
+          const bool end = sp <= base;
+          while (ballot(end ? q.n != 0 : q.n == DRAW_QUEUE)) draw_round(x, c, L, q, weight, touched);
+          if (end) break;
+        } else if (sp <= base) {
+          break;
+        }
         sp--;
         const uint32_t n = st.node_at(sp);
         const uint64_t need = st.mask_at(sp);
@@ -731,8 +795,12 @@ __device__ __forceinline__ void add_mesh_coverage(const Ctx& x, const Cone& wc, 
         bool descend = false;
         if (mine) {
           if (COUNT) w.work[1]++;
-          const uint32_t test = node_cone_test<SKY>(c, L, w.prefilter_bins, f3{node.lo[0], node.lo[1], node.lo[2]},
-                                               f3{node.hi[0], node.hi[1], node.hi[2]}, x.blas_cov + 27 * (size_t)n);
+          const uint32_t test = node_cone_test<SKY>(
+              c, L, w.prefilter_bins, f3{node.lo[0], node.lo[1], node.lo[2]}, f3{node.hi[0], node.hi[1], node.hi[2]},
+              x.blas_cov + 27 * (size_t)n, [&]() {
+                if (QUEUE)
+                  while (q.n) draw_round(x, c, L, q, weight, touched);
+              });
           if (test == 2) touched = true;
           if (COUNT && (test == 1 || test == 2)) w.work[5]++;
           descend = test == 3;
@@ -760,6 +828,10 @@ __device__ __forceinline__ void add_mesh_coverage(const Ctx& x, const Cone& wc, 
                 if (COUNT) w.work[2]++;
                 if (triangle_drawn<SKY>(c, v0, v1, v2, x.profile_stage, w.work[3])) drawn |= 1u << (k - k0);
               }
+            }
+            if (QUEUE) {
+              if (drawn) q.push(k0, drawn);
+              continue;
             }
             while (ballot(drawn != 0)) {
               if (drawn) {
@@ -835,7 +907,7 @@ __device__ __forceinline__ bool splat_object_node(const Ctx& x, const Cone& c, W
 // WGSL :1160-1227 for the lanes with `traced`. Sky walks (SKY, unless
 // `sky_max`) keep the children's fixed order, as their products would round
 // differently in another.
-template <bool COUNT, bool SKIP = false, bool SKY = false>
+template <bool COUNT, bool SKIP = false, bool SKY = false, bool QUEUE = false>
 __device__ __forceinline__ void scene_coverage(const Ctx& x, const Cone& c, bool traced, Walk& w, Lane& L,
                                                Skip skip = Skip{NONE, NONE}) {
   w.cone_work = 0;
@@ -865,7 +937,7 @@ __device__ __forceinline__ void scene_coverage(const Ctx& x, const Cone& c, bool
       const uint32_t count = node.b & ~NODE_NO_SPLAT;
       if (count > 0) {
         for (uint32_t k = node.a; k < node.a + count; k++)
-          add_mesh_coverage<COUNT, SKIP, SKY>(x, c, x.items[k], descend, w, st, sp, L, skip);
+          add_mesh_coverage<COUNT, SKIP, SKY, QUEUE>(x, c, x.items[k], descend, w, st, sp, L, skip);
       } else {
         uint32_t nearc = node.a, farc = node.a + 1;
         if (!SKY || uni(w.sky_max ? 1u : 0u)) {
@@ -898,7 +970,7 @@ __device__ __forceinline__ void scene_coverage(const Ctx& x, const Cone& c, bool
     }
   } else {
     for (uint32_t m = 0; m < x.n_objects; m++)
-      add_mesh_coverage<COUNT, SKIP, SKY>(x, c, m, traced && w.full < BINS, w, st, 0, L, skip);
+      add_mesh_coverage<COUNT, SKIP, SKY, QUEUE>(x, c, m, traced && w.full < BINS, w, st, 0, L, skip);
   }
 }
 
@@ -912,7 +984,7 @@ struct QueryOut {
 
 // `active`: the lane traces (a valid query at a point that is not a sky or
 // light pixel); `skip`: the triangle this cone does not see (with SKIP).
-template <bool COUNT, bool SKIP = false>
+template <bool COUNT, bool SKIP = false, bool QUEUE = false>
 __device__ __forceinline__ QueryOut evaluate_at(const Ctx& x, const QueryParams& p, Lane& L, bool valid, f3 point,
                                                 f3 normal, bool active, const LightRect& r,
                                                 float* __restrict__ cov_out, uint64_t* __restrict__ work_out,
@@ -942,7 +1014,7 @@ __device__ __forceinline__ QueryOut evaluate_at(const Ctx& x, const QueryParams&
     lc = light_cone_rect(point, r, p.wide_extent);
     const bool traced = on && lc.ok;
     w.cone_cap = lc.wide_window ? p.wide_cone_cap : p.light_cone_cap;
-    if (p.profile_stage != 2) scene_coverage<COUNT, SKIP>(x, lc.c, traced, w, L, skip);
+    if (p.profile_stage != 2) scene_coverage<COUNT, SKIP, false, QUEUE>(x, lc.c, traced, w, L, skip);
     capped = traced && w.cone_work >= w.cone_cap;
     if (w.cone_work >= w.cone_cap && lc.wide_window && !lc.wide_clipped) {
 #pragma unroll

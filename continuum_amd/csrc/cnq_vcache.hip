@@ -672,7 +672,7 @@ struct LatticeOut {
 
 // WGSL :4875-4899 `lattice_ratio` with :4853-4869 `lattice_share`, for the
 // lanes with `on` (every lane takes part in the walk).
-template <bool COUNT>
+template <bool COUNT, bool QUEUE = false>
 __device__ __forceinline__ LatticeOut lattice_ratio(const Ctx& x, const QueryParams& p, Lane& L, bool on, f3 apex,
                                                     f3 normal, const float* now, const float* then, float* cov,
                                                     uint64_t* work) {
@@ -697,7 +697,7 @@ __device__ __forceinline__ LatticeOut lattice_ratio(const Ctx& x, const QueryPar
   w.truncated = false;
 #pragma unroll
   for (int k = 0; k < 6; k++) w.work[k] = 0;
-  if (uni(ballot(traced) != 0ull ? 1u : 0u)) scene_coverage<COUNT>(x, lc.c, traced, w, L);
+  if (uni(ballot(traced) != 0ull ? 1u : 0u)) scene_coverage<COUNT, false, false, QUEUE>(x, lc.c, traced, w, L);
   if (COUNT && work) {
 #pragma unroll
     for (int k = 0; k < 6; k++) {
@@ -737,7 +737,7 @@ __device__ __forceinline__ LatticeOut lattice_ratio(const Ctx& x, const QueryPar
 // whether any bin is touched. The reference walks the whole guard cone and
 // then looks for a touched bin; a lane here stops at the first mesh or
 // splat that lands coverage in a bin, which settles the same answer.
-template <bool COUNT>
+template <bool COUNT, bool QUEUE = false>
 __device__ __forceinline__ bool guard_touched(const Ctx& x, const QueryParams& p, Lane& L, const Cone& c,
                                               bool trace, float* cov, uint64_t* work) {
   Walk w;
@@ -754,7 +754,7 @@ __device__ __forceinline__ bool guard_touched(const Ctx& x, const QueryParams& p
   w.stop_on_touch = true;
 #pragma unroll
   for (int k = 0; k < 6; k++) w.work[k] = 0;
-  if (uni(ballot(trace) != 0ull ? 1u : 0u)) scene_coverage<COUNT>(x, c, trace, w, L);
+  if (uni(ballot(trace) != 0ull ? 1u : 0u)) scene_coverage<COUNT, false, false, QUEUE>(x, c, trace, w, L);
   if (COUNT && work) {
 #pragma unroll
     for (int k = 0; k < 6; k++) {
@@ -1060,7 +1060,7 @@ namespace {
 // fixed or off), the page's slack (the smallest guard, less the light's
 // shift since the version), κ on a lattice, and the page's flags. The guard
 // cone's walk runs as a packet for the lanes baked fresh at the version.
-template <bool COUNT>
+template <bool COUNT, bool QUEUE = false>
 __device__ __forceinline__ void ledger_texel(const VcArgs& a, const Ctx& x, const QueryParams& p, Lane& L,
                                              bool rec_on, bool on, uint32_t slot, uint32_t j, uint32_t l, uint32_t k,
                                              f3 apex, f3 n, const LightRect& r, float open, const float* then,
@@ -1083,7 +1083,7 @@ __device__ __forceinline__ void ledger_texel(const VcArgs& a, const Ctx& x, cons
   }
   const GuardCone g = guard_cone(apex, r);
   const bool fresh = rec_on && on && open > 0.0f && offset == 0u;
-  const bool any = guard_touched<COUNT>(x, p, L, g.c, fresh, cov, work);
+  const bool any = guard_touched<COUNT, QUEUE>(x, p, L, g.c, fresh, cov, work);
   if (!rec_on) return;
   if (on && open > 0.0f && offset != 0u) {
     state = 3;
@@ -1207,7 +1207,7 @@ __global__ __launch_bounds__(64, WALK_WAVES) void vc_bake_kernel(VcArgs a, const
       on_lattice = lattice;
       if (lattice && on && open > 0.0f)
         on_lattice = lattice_kappa(lattice_frame(s.apex, then), s.normal) <= LATTICE_KAPPA_MAX;
-      lat = lattice_ratio<COUNT>(x, p, L, on && on_lattice && open > 0.0f, s.apex, s.normal, lf, then, wcov,
+      lat = lattice_ratio<COUNT, QUEUE>(x, p, L, on && on_lattice && open > 0.0f, s.apex, s.normal, lf, then, wcov,
                                  COUNT ? work : nullptr);
       overflow = on && on_lattice && open > 0.0f && lat.ratio < 0.0f;
     }
@@ -1249,7 +1249,7 @@ __global__ __launch_bounds__(64, WALK_WAVES) void vc_bake_kernel(VcArgs a, const
     if (listed && t.inside)
       a.pool[((size_t)t.e[3] * VC_TEXELS + k) * a.channels + l] = a.light_ratio ? ratio : (on ? light_value : 0.0f);
     if (MOVING)
-      ledger_texel<COUNT>(a, x, p, L, mj != NONE && listed && t.inside && a.ledger_record != 0u, on, t.e[3], mj, l,
+      ledger_texel<COUNT, QUEUE>(a, x, p, L, mj != NONE && listed && t.inside && a.ledger_record != 0u, on, t.e[3], mj, l,
                           k, s.apex, s.normal, r, open, then, lattice, overflow, wcov, COUNT ? work : nullptr);
   }
 }

@@ -308,12 +308,12 @@ __device__ __forceinline__ bool may_touch(const Cone& c, f3 v0, f3 v1, f3 v2) {
   return true;
 }
 
-// ---- Per-query working memory in LDS, lane-interleaved (slot s of lane l
-// at s * 64 + l), so every access is bank-conflict free whatever slot each
-// lane touches: the 68-cell accumulation buffer, and 70 slots for clip
+// ---- Per-query working memory: the 68-cell accumulation buffer in LDS,
+// lane-interleaved (cell s of lane l at s * 64 + l), so every access is
+// bank-conflict free whatever cell each lane touches, and 70 slots for clip
 // polygons (a 3D polygon of up to 10 vertices at slots 0-29, its clip
-// target at 30-59; 2D polygons at 30-49 and 50-69 once the 3D clip is done),
-// the hull's 8 points, and the horizon-clipped quads of the weights.
+// target at 30-59; 2D polygons at 30-49 and 50-69 once the 3D clip is done;
+// a splat's hull at 30-45) and the horizon-clipped quads of the weights.
 constexpr uint32_t ACC_CELLS = 68;
 constexpr uint32_t POLY_SLOTS = 70;
 // The clipper's workspace lives in the lane's private memory (scratch, cached

@@ -39,8 +39,8 @@ PROFILE_DIR = ROOT / "profiles"
 
 # The plain launch's tile: 16 x 4 shading points a wave on the rock field
 # (a sweep over 4, 8, 16, 32 and 64 puts it first there), 8 x 8 on the
-# courtyard (79.1 ms a frame against 82.7 ms at 16 x 4, 102.5 at 4 x 16 and
-# 108.2 at 32 x 2).
+# courtyard (62.2 ms a frame against 66.8 ms at 16 x 4, 72.2 at 4 x 16 and
+# 86.5 at 32 x 2).
 PLAIN_TILE_W = 16
 COURTYARD_TILE_W = 8
 

@@ -62,3 +62,28 @@ def test_leaves_of_more_than_32_triangles_match_the_oracle(device, prefilter):
     assert np.array_equal(full[ok], rf[ok]) and np.array_equal(touched[ok], rt[ok]), "masks differ"
     assert np.array_equal(vis[ok].view(np.uint32), rv[ok].view(np.uint32)), "irradiance differs"
     assert int((touched != 0).sum()) > 0
+
+
+def test_a_bake_over_leaves_of_more_than_32_triangles_matches_the_oracle(device):
+    """The view cache's bake queues each lane's draws across a mesh's leaves
+    only where the scene's leaves hold at most 32 triangles; with larger ones
+    it draws leaf by leaf. The boulder world's hierarchies collapsed as
+    above bake the oracle's pages bit for bit."""
+    from continuum_amd.kernel import ViewCache
+    from continuum_amd.scenes import vcache_world
+    from tests.test_gpu_vcache import _keys, _same_state
+
+    tris9, counts, pl, surf, lights, _ = vcache_world()
+    flat, made = _collapsed(Flat.build_instanced(tris9, counts, pl, object_prefilter=True))
+    assert made > 0 and int(flat.blas[:, 9].max()) > 32
+    scene = Scene(device, flat)
+    kw = dict(pages=1024, append_pages=1024, candidates=1024, base_texel=0.25, light_ratio=1, visible_frames=1)
+    vc = ViewCache(scene, surf, lights, _lib.default_options(hierarchy=2, prefilter=1.0, object_cov=flat.object_cov),
+                   **kw)
+    ov = O.OracleViewCache(flat, surf, lights, O.opts_array(hierarchy=2, prefilter=1.0, object_cov=flat.object_cov),
+                           **kw)
+    keys = _keys(vc, pl)
+    sa, sb = vc.request(keys), ov.request(keys)
+    assert np.array_equal(sa, sb)
+    assert vc.frame() == ov.frame()
+    _same_state(vc, ov, "bake over large leaves")
